@@ -1,0 +1,813 @@
+// lsg_oracle.cpp -- CPU restatement of LoAdSG's matrix-free prewavelet matvec,
+// diagonal preconditioner and CG on sparse grids with zero Dirichlet boundaries.
+//
+// TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/, the smoke()
+// entry point and bench.py's cpu_baseline leg may load it.  It is never on the
+// product path.  It follows the reference algorithm step by step, in plain C++
+// with hash maps instead of the reference's prime/second tables, in the
+// reference's own slot order (the order of VectorSparseG's data table).
+//
+// Each function cites the reference file:line it restates (paths relative to
+// library/source/).  Parity status of each part (how it is pinned) is listed at
+// the bottom of this header:
+//   * index algebra (Idx1 helpers)       : exhaustive closed-form tests + golden
+//   * regular grid slot order            : golden index tables from the compiled reference
+//   * calcNodal / caseFunction / Convert : golden matvecs (Poisson, HelmHoltz, variable
+//                                          coefficient) from the compiled reference, plus
+//                                          dense brute force Psi^T K Psi for constant
+//                                          coefficients (exact by prewavelet semi-orthogonality)
+//   * precond3 diagonal                  : golden diagonal + brute force psi^T A_T psi
+//   * solveHomogen                       : golden iteration counts and solutions
+//
+// Build: see oracle/Makefile (g++ -O2 -fopenmp -shared).
+#include <cstdint>
+#include <cstring>
+#include <cmath>
+#include <vector>
+#include <array>
+#include <unordered_map>
+#include <algorithm>
+#include <list>
+#include <memory>
+#include <stdexcept>
+
+namespace orc {
+
+constexpr int MAXD = 8;
+
+// ---------------------------------------------------------------------------
+// 1D index algebra (indices/index.h:657-876).  Index 0 = left boundary (x=0),
+// 1 = right boundary (x=1), 2 = 0.5, sons 2i / 2i+1.
+// ---------------------------------------------------------------------------
+static inline int depth1(int32_t i) {            // index.h:812-817
+    if (i == 0) return 0;
+    return 31 - __builtin_clz((uint32_t)i);
+}
+static inline int internDepth1(int32_t i) {      // index.h:825-834
+    if (i == 0) return -1;
+    return 31 - __builtin_clz((uint32_t)i);
+}
+static inline double coord1(int32_t i) {         // index.h:718-730
+    if (i == 0) return 0.0;
+    int t = depth1(i);
+    double h = 1.0 / double(1u << t);
+    int32_t n = i & ~(1 << t);
+    return double(2 * n + 1) * h;
+}
+static inline int32_t poleLeft(int32_t i) {      // index.h:749-755
+    return i >> (__builtin_ctz((uint32_t)i) + 1);
+}
+static inline int32_t poleRight(int32_t i) {     // index.h:782-789
+    return i >> (__builtin_ctz(~(uint32_t)i) + 1);
+}
+static inline int32_t leftAtLevel(int32_t i, int level) {   // index.h:839-851
+    int my = depth1(i);
+    int l = level - my;
+    if (l == 0) return poleLeft(i);
+    i = i << 1;
+    for (int k = 1; k < l; ++k) i = (i << 1) + 1;
+    return i;
+}
+static inline int32_t rightAtLevel(int32_t i, int level) {  // index.h:859-871
+    int my = internDepth1(i);
+    int l = level - my;
+    if (l == 0) return poleRight(i);
+    i = (i << 1) + 1;
+    for (int k = 1; k < l; ++k) i = i << 1;
+    return i;
+}
+static inline bool isBoundary1(int32_t i) { return i == 0 || i == 1; }
+
+// 1D prewavelet of a point at depth t (its own level), as nodal values at
+// level t: nextFiveP (indices/index.cc:252-327) restricted to one dimension.
+// The multi-dimensional 5^d stencil of nextFiveP is the tensor product of
+// these lists (the compass skip logic only drops entries whose step would hit
+// the boundary, see index.cc:264-310).
+struct Entry1 { int32_t idx; double c; };
+static int prewavelet1D(int32_t i, int t, Entry1 out[5]) {
+    int n = 0;
+    // Mitte: 0.9 if depth > 1 and next to the boundary (index.cc:312-316)
+    double cm = 1.0;
+    if (depth1(i) > 1) {
+        if (poleLeft(i) == 0 || poleRight(i) == 1) cm = 0.9;
+    }
+    out[n++] = {i, cm};
+    int32_t l = poleLeft(i);
+    if (!isBoundary1(l)) {                       // Links (index.cc:264-276)
+        out[n++] = {l, -0.6};
+        out[n++] = {leftAtLevel(l, t), 0.1};     // LinksLinks (index.cc:278-288)
+    }
+    int32_t r = poleRight(i);
+    if (!isBoundary1(r)) {                       // Rechts / RechtsRechts (index.cc:289-310)
+        out[n++] = {r, -0.6};
+        out[n++] = {rightAtLevel(r, t), 0.1};
+    }
+    return n;
+}
+
+// ---------------------------------------------------------------------------
+// Point keys and depths
+// ---------------------------------------------------------------------------
+struct Key {
+    std::array<int32_t, MAXD> v{};
+    bool operator==(const Key& o) const { return v == o.v; }
+};
+struct KeyHash {
+    size_t operator()(const Key& k) const {
+        uint64_t h = 1469598103934665603ull;
+        for (int i = 0; i < MAXD; ++i) { h ^= (uint32_t)k.v[i]; h *= 1099511628211ull; }
+        return (size_t)h;
+    }
+};
+using Depth = std::array<int, MAXD>;
+struct DepthHash {
+    size_t operator()(const Depth& k) const {
+        uint64_t h = 1469598103934665603ull;
+        for (int i = 0; i < MAXD; ++i) { h ^= (uint32_t)k[i]; h *= 1099511628211ull; }
+        return (size_t)h;
+    }
+};
+
+// Coefficient families shared with the product library (same formulas, evaluated
+// here directly, not through the product's separable tables).
+enum CoefKind { COEF_CONST = 0, COEF_CFG2 = 1, COEF_CFG3 = 2, COEF_EXPSIN = 3 };
+struct Coef {
+    int kind = COEF_CONST;
+    double a0 = 1.0, c0 = 0.0;   // COEF_CONST: a = a0, c = c0
+    std::vector<double> xi;      // COEF_EXPSIN: a = exp(sum_k xi_k sin(k pi x_{k mod d}) / k)
+    double a(const double* x, int d) const {
+        switch (kind) {
+            case COEF_CONST: return a0;
+            case COEF_CFG2: { double p = 1.0; for (int i = 0; i < d; ++i) p *= std::cos(M_PI * x[i]); return 1.0 + 0.5 * p; }
+            case COEF_CFG3: { double s = 1.0; for (int i = 0; i < d; ++i) s += x[i] * x[i]; return s; }
+            case COEF_EXPSIN: {
+                double s = 0.0;
+                for (size_t k = 1; k <= xi.size(); ++k)
+                    s += xi[k - 1] * std::sin(double(k) * M_PI * x[k % d]) / double(k);
+                return std::exp(s);
+            }
+        }
+        return 0.0;
+    }
+    double c(const double* x, int d) const {
+        (void)d;
+        switch (kind) {
+            case COEF_CONST: return c0;
+            case COEF_CFG2: return 1.0 + x[0];
+            default: return 0.0;
+        }
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Grid (sgrid/sparseGrid.h:86-387): points in slot order + active flags.
+// ---------------------------------------------------------------------------
+struct Grid {
+    int d = 0;
+    std::vector<Key> pts;             // slot -> point
+    std::vector<uint8_t> active;      // slot -> active (false = hanging node)
+    std::unordered_map<Key, int64_t, KeyHash> slot;
+    std::vector<Depth> depthOf;       // slot -> depth
+    // DepthList (iterator/depthIterator.h:16-48): depths in first-appearance order
+    std::vector<Depth> depths;
+    std::unordered_map<Depth, int, DepthHash> depthIdx;
+    std::vector<std::vector<int64_t>> slotsOfDepth;   // MultiDepthHashGrid::_mapPosToGridPos
+    Depth maxDepth{};
+
+    bool occupied(const Key& k, int64_t& s) const {
+        auto it = slot.find(k);
+        if (it == slot.end()) return false;
+        s = it->second;
+        return true;
+    }
+    void build() {
+        slot.clear(); depthOf.resize(pts.size());
+        for (int64_t s = 0; s < (int64_t)pts.size(); ++s) {
+            slot[pts[s]] = s;
+            Depth T{};
+            for (int i = 0; i < d; ++i) T[i] = depth1(pts[s].v[i]);
+            depthOf[s] = T;
+            auto it = depthIdx.find(T);
+            if (it == depthIdx.end()) {
+                depthIdx[T] = (int)depths.size();
+                depths.push_back(T);
+                slotsOfDepth.emplace_back();
+            }
+            slotsOfDepth[depthIdx[T]].push_back(s);
+        }
+        maxDepth = Depth{};
+        for (auto& T : depths) for (int i = 0; i < d; ++i) maxDepth[i] = std::max(maxDepth[i], T[i]);
+    }
+    bool included(const Depth& T) const { return depthIdx.count(T) != 0; }
+};
+
+// AddRecursiveSonsOfPoint(center, level) (sgrid/sparseGrid.cc:557-566): DFS,
+// slot = first-visit order.  Every path to a point has the same length, so a
+// revisit explores an identical subtree and can be skipped without changing the
+// first-visit order.
+static void addRecursiveSons(Grid& g, const Key& I, int level,
+                             std::unordered_map<Key, char, KeyHash>& seen) {
+    if (seen.count(I)) return;
+    seen[I] = 1;
+    g.pts.push_back(I);
+    g.active.push_back(1);
+    if (level > 0) {
+        for (int dd = 0; dd < g.d; ++dd) {
+            Key L = I; L.v[dd] = I.v[dd] << 1;
+            addRecursiveSons(g, L, level - 1, seen);
+            Key R = I; R.v[dd] = (I.v[dd] << 1) + 1;
+            addRecursiveSons(g, R, level - 1, seen);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Multilevel index (sgrid/multilevelSparseGrid.cpp:9-34): for each depth T of
+// the list, G_T = all grid points p with depth(p) <= T componentwise.
+// ---------------------------------------------------------------------------
+struct MultiLevel {
+    std::vector<std::vector<int64_t>> slotsG;                       // per depth: slots of G_T
+    std::vector<std::unordered_map<int64_t, int64_t>> posInG;       // per depth: slot -> position
+    void build(const Grid& g) {
+        size_t nd = g.depths.size();
+        slotsG.assign(nd, {}); posInG.assign(nd, {});
+        for (size_t t = 0; t < nd; ++t) {
+            const Depth& T = g.depths[t];
+            // GetNextSmallerDepthsIterator order does not matter for values.
+            for (int64_t s = 0; s < (int64_t)g.pts.size(); ++s) {
+                bool le = true;
+                for (int i = 0; i < g.d; ++i) if (g.depthOf[s][i] > T[i]) { le = false; break; }
+                if (le) { posInG[t][s] = (int64_t)slotsG[t].size(); slotsG[t].push_back(s); }
+            }
+        }
+    }
+};
+using MLVec = std::vector<std::vector<double>>;   // per depth: values on G_T
+
+// ---------------------------------------------------------------------------
+// Operator: Q1 element matrices on a cell (stencils/Stencil.h)
+// ---------------------------------------------------------------------------
+struct Op {
+    Coef coef;
+    // integration(cell, dirP, dirQ): Poisson (Stencil.h:1707-1752) times a(center)
+    // plus HelmHoltz (Stencil.h:900-919) times c(center).
+    double element(int d, const double* h, const double* center, unsigned p, unsigned q) const {
+        double a = coef.a(center, d), c = coef.c(center, d);
+        double stiff = 0.0;
+        if (a != 0.0) {
+            for (int i = 0; i < d; ++i) {
+                double prod = (((p >> i) & 1u) == ((q >> i) & 1u)) ? 1.0 / h[i] : -1.0 / h[i];
+                for (int j = 0; j < d; ++j) {
+                    if (j == i) continue;
+                    prod *= (((p >> j) & 1u) == ((q >> j) & 1u)) ? h[j] / 3.0 : h[j] / 6.0;
+                }
+                stiff += prod;
+            }
+        }
+        double mass = 0.0;
+        if (c != 0.0) {
+            mass = 1.0;
+            for (int j = 0; j < d; ++j)
+                mass *= (((p >> j) & 1u) == ((q >> j) & 1u)) ? h[j] / 3.0 : h[j] / 6.0;
+        }
+        return a * stiff + c * mass;
+    }
+};
+
+struct Cell { Key lo; };   // lower-left corner index at level T (may be boundary index 0)
+
+// Cells of depth T+1 that are in the grid: all non-boundary corners occupied
+// (cells/CellStructure.h:23-61).  Enumerated per level-T full grid.
+static void cellsOfDepth(const Grid& g, const Depth& T, std::vector<std::array<int32_t, MAXD>>& cornersLoHi,
+                         std::vector<std::array<double, MAXD>>& centers) {
+    int d = g.d;
+    // 1D: level-T grid nodes at positions 0..2^t (0 and 2^t = boundary)
+    std::vector<std::vector<int32_t>> nodes(d);
+    for (int i = 0; i < d; ++i) {
+        int t = T[i];
+        int n = 1 << t;
+        nodes[i].resize(n + 1);
+        nodes[i][0] = 0; nodes[i][n] = 1;
+        for (int k = 1; k < n; ++k) {
+            // position k at level t -> index: depth s = t - ctz(k), n' = k >> (ctz(k)+1)
+            int tz = __builtin_ctz((unsigned)k);
+            int s = t - tz;
+            int32_t local = k >> (tz + 1);
+            nodes[i][k] = (1 << s) + local;
+        }
+    }
+    std::vector<int> cnt(d, 0);
+    while (true) {
+        // cell with lower corner position cnt[i]
+        bool ok = true;
+        for (unsigned c = 0; c < (1u << d) && ok; ++c) {
+            Key K{};
+            bool interior = true;
+            for (int i = 0; i < d; ++i) {
+                int pos = cnt[i] + ((c >> i) & 1u);
+                K.v[i] = nodes[i][pos];
+                if (isBoundary1(K.v[i])) interior = false;
+            }
+            int64_t s;
+            if (interior && !g.occupied(K, s)) ok = false;
+        }
+        if (ok) {
+            std::array<int32_t, MAXD> lo{};
+            std::array<double, MAXD> ctr{};
+            for (int i = 0; i < d; ++i) {
+                lo[i] = cnt[i];
+                ctr[i] = (double(cnt[i]) + 0.5) / double(1 << T[i]);
+            }
+            cornersLoHi.push_back(lo);
+            centers.push_back(ctr);
+        }
+        int i = 0;
+        for (; i < d; ++i) {
+            if (++cnt[i] < (1 << T[i])) break;
+            cnt[i] = 0;
+        }
+        if (i == d) break;
+    }
+}
+
+struct Oracle {
+    Grid g;
+    MultiLevel ml;
+    Op op;
+    // cached cells per depth index (cells of depth T+1)
+    std::vector<std::vector<std::array<int32_t, MAXD>>> cellLo;
+    std::vector<std::vector<std::array<double, MAXD>>> cellCtr;
+    std::vector<std::vector<std::vector<int32_t>>> levelNodes;   // per depth, per dim, position -> index
+
+    void init() {
+        g.build();
+        ml.build(g);
+        size_t nd = g.depths.size();
+        cellLo.assign(nd, {}); cellCtr.assign(nd, {}); levelNodes.assign(nd, {});
+        for (size_t t = 0; t < nd; ++t) {
+            cellsOfDepth(g, g.depths[t], cellLo[t], cellCtr[t]);
+            levelNodes[t].resize(g.d);
+            for (int i = 0; i < g.d; ++i) {
+                int L = g.depths[t][i];
+                int n = 1 << L;
+                auto& nodes = levelNodes[t][i];
+                nodes.resize(n + 1);
+                nodes[0] = 0; nodes[n] = 1;
+                for (int k = 1; k < n; ++k) {
+                    int tz = __builtin_ctz((unsigned)k);
+                    nodes[k] = (1 << (L - tz)) + (k >> (tz + 1));
+                }
+            }
+        }
+    }
+    int64_t N() const { return (int64_t)g.pts.size(); }
+
+    // tensor-product prewavelet stencil of point s at its own depth
+    void stencil5(int64_t s, std::vector<std::pair<Key, double>>& out) const {
+        int d = g.d;
+        Entry1 e[MAXD][5];
+        int ne[MAXD];
+        for (int i = 0; i < d; ++i) ne[i] = prewavelet1D(g.pts[s].v[i], g.depthOf[s][i], e[i]);
+        out.clear();
+        std::vector<int> c(d, 0);
+        while (true) {
+            Key J{};
+            double v = 1.0;
+            for (int i = 0; i < d; ++i) { J.v[i] = e[i][c[i]].idx; v *= e[i][c[i]].c; }
+            out.push_back({J, v});
+            int i = 0;
+            for (; i < d; ++i) { if (++c[i] < ne[i]) break; c[i] = 0; }
+            if (i == d) break;
+        }
+    }
+
+    // calcNodal(nodal_u, prew, T) (MatrixVectorMultiplication/MatrixVectorHomogen.cpp:43-74)
+    // followed by nodal.setMultiLevelValuesOMP (extemp/multilevelvector.h:200-223).
+    void calcNodal(const std::vector<double>& x, MLVec& nodal) const {
+        std::vector<std::pair<Key, double>> st;
+        for (size_t t = 0; t < g.depths.size(); ++t) {
+            std::vector<double> un(N(), 0.0);
+            for (int64_t s : g.slotsOfDepth[t]) {
+                if (!g.active[s]) continue;
+                double coeff = x[s];
+                stencil5(s, st);
+                for (auto& [J, c] : st) {
+                    int64_t k;
+                    if (g.occupied(J, k)) un[k] += coeff * c;
+                }
+            }
+            nodal[t].assign(ml.slotsG[t].size(), 0.0);
+            for (size_t p = 0; p < ml.slotsG[t].size(); ++p) nodal[t][p] = un[ml.slotsG[t][p]];
+        }
+    }
+
+    // DepthList::sortDepths / recursiveDepth (iterator/depthIterator.cpp:7-41,88-93)
+    void recursiveDepth(const bool* R, bool checkProl, int dd, Depth& T, std::vector<int>& out) const {
+        int D = g.d;
+        if (checkProl && dd > 0) {
+            if (!R[dd]) {
+                for (int t = 1; t <= g.maxDepth[dd]; ++t) { T[dd] = t; recursiveDepth(R, checkProl, dd - 1, T, out); }
+            } else recursiveDepth(R, checkProl, dd - 1, T, out);
+        }
+        if (checkProl && dd == 0) {
+            if (!R[dd]) {
+                for (int t = 1; t <= g.maxDepth[dd]; ++t) { T[dd] = t; recursiveDepth(R, false, D - 1, T, out); }
+            } else recursiveDepth(R, false, D - 1, T, out);
+        }
+        if (!checkProl && dd >= 0) {
+            if (R[dd]) {
+                for (int t = g.maxDepth[dd]; t > 1; --t) { T[dd] = t; recursiveDepth(R, checkProl, dd - 1, T, out); }
+            } else recursiveDepth(R, checkProl, dd - 1, T, out);
+        }
+        if (dd < 0) {
+            auto it = g.depthIdx.find(T);
+            if (it != g.depthIdx.end()) out.push_back(it->second);
+        }
+    }
+    std::vector<int> sortDepths(const bool* R) const {
+        std::vector<int> out;
+        Depth T{};
+        recursiveDepth(R, true, g.d - 1, T, out);
+        return out;
+    }
+
+    // slot-vector <-> multilevel helpers (extemp/vector.cc:600-675, multilevelvector.h:146-298)
+    void setFromML(std::vector<double>& v, const MLVec& a, int t) const {     // setMultiLevelValues2
+        for (size_t p = 0; p < ml.slotsG[t].size(); ++p) v[ml.slotsG[t][p]] = a[t][p];
+    }
+    void addFromML(std::vector<double>& v, const MLVec& a, int t) const {     // addMultiLevelValues2
+        for (size_t p = 0; p < ml.slotsG[t].size(); ++p) v[ml.slotsG[t][p]] += a[t][p];
+    }
+    void mlAdd(MLVec& a, const std::vector<double>& v, int t) const {         // MultiLevelVector::addMultiLevelValues
+        for (size_t p = 0; p < ml.slotsG[t].size(); ++p) a[t][p] += v[ml.slotsG[t][p]];
+    }
+    void mlSet(MLVec& a, const std::vector<double>& v, int t) const {         // MultiLevelVector::setMultiLevelValues2
+        for (size_t p = 0; p < ml.slotsG[t].size(); ++p) a[t][p] = v[ml.slotsG[t][p]];
+    }
+    double getVal(const std::vector<double>& v, const Key& K) const {          // VectorSparseG::getValue(I) (vector.h:331-336)
+        int64_t k;
+        if (g.occupied(K, k)) return v[k];
+        return 0.0;
+    }
+
+    // prolongation1D_inplace (MatrixVectorHomogen.h:444-474)
+    void prolongation1D(std::vector<double>& v, int t, int dd) const {
+        for (size_t ti = 0; ti < g.depths.size(); ++ti) {
+            if (g.depths[ti][dd] != t) continue;
+            for (int64_t s : g.slotsOfDepth[ti]) {
+                const Key& I = g.pts[s];
+                double value = v[s];
+                if (!isBoundary1(I.v[dd])) {
+                    Key R = I; R.v[dd] = poleRight(I.v[dd]);
+                    Key L = I; L.v[dd] = poleLeft(I.v[dd]);
+                    value = value + 0.5 * (getVal(v, R) + getVal(v, L));
+                }
+                v[s] = value;
+            }
+        }
+    }
+
+    // restriction1D_inverted_inplace (MatrixVectorHomogen.h:485-530)
+    void restriction1D(std::vector<double>& v, int t, int dd) const {
+        for (size_t ti = 0; ti < g.depths.size(); ++ti) {
+            const Depth& Tl = g.depths[ti];
+            if (Tl[dd] != t + 1) continue;
+            bool allPos = true;
+            for (int i = 0; i < g.d; ++i) if (Tl[i] <= 0) allPos = false;
+            if (!allPos) continue;
+            for (int64_t s : g.slotsOfDepth[ti]) {
+                const Key& I = g.pts[s];
+                double value = 0.5 * v[s];
+                if (t >= 0) {
+                    Key R = I; R.v[dd] = poleRight(I.v[dd]);
+                    if (R.v[dd] != 1) { int64_t k; if (g.occupied(R, k)) v[k] += value; }
+                    Key L = I; L.v[dd] = poleLeft(I.v[dd]);
+                    if (L.v[dd] != 1) { int64_t k; if (g.occupied(L, k)) v[k] += value; }
+                }
+            }
+        }
+    }
+
+    // applyStencilGlobal_symmetry + StencilTemplate::applyStencilOnCell
+    // (MatrixVectorHomogen.h:1157-1189, stencils/Stencil.h:187-246)
+    void applyStencil(const std::vector<double>& u, std::vector<double>& z, int ti) const {
+        const Depth& T = g.depths[ti];
+        int d = g.d;
+        double h[MAXD];
+        for (int i = 0; i < d; ++i) h[i] = 1.0 / double(1 << T[i]);
+        unsigned nc = 1u << d;
+        std::vector<int64_t> ks(nc);
+        std::vector<uint8_t> occ(nc);
+        for (size_t c = 0; c < cellLo[ti].size(); ++c) {
+            const auto& lo = cellLo[ti][c];
+            for (unsigned p = 0; p < nc; ++p) {
+                Key K{};
+                bool interior = true;
+                for (int i = 0; i < d; ++i) {
+                    K.v[i] = levelNodes[ti][i][lo[i] + ((p >> i) & 1u)];
+                    if (isBoundary1(K.v[i])) interior = false;
+                }
+                int64_t k = -1;
+                occ[p] = interior && g.occupied(K, k);
+                ks[p] = k;
+            }
+            for (unsigned p = 0; p < nc; ++p) {
+                if (!occ[p]) continue;
+                for (unsigned q = p; q < nc; ++q) {
+                    if (!occ[q]) continue;
+                    double val = op.element(d, h, cellCtr[ti][c].data(), p, q);
+                    if (p == q) z[ks[p]] += val * u[ks[p]];
+                    else { z[ks[q]] += val * u[ks[p]]; z[ks[p]] += val * u[ks[q]]; }
+                }
+            }
+        }
+    }
+
+    // ConvertToPrewavelet2 (MatrixVectorHomogen.h:540-562)
+    void convertToPrewavelet(const std::vector<double>& z, std::vector<double>& Ax, int ti) const {
+        std::vector<std::pair<Key, double>> st;
+        for (int64_t s : g.slotsOfDepth[ti]) {
+            if (!g.active[s]) continue;
+            stencil5(s, st);
+            double coeff = 0.0;
+            for (auto& [J, c] : st) coeff = coeff + getVal(z, J) * c;
+            Ax[s] = coeff;
+        }
+    }
+
+    // caseFunction (MatrixVectorHomogen.h:965-1090)
+    void caseFunction(const MLVec& nodal, const bool* R, std::vector<double>& AxNeu) const {
+        int d = g.d;
+        size_t nd = g.depths.size();
+        MLVec P = nodal;                         // P += nodal assigns (multilevelvector.h:112-116)
+        MLVec gM(nd);
+        for (size_t t = 0; t < nd; ++t) gM[t].assign(ml.slotsG[t].size(), 0.0);
+        std::vector<int> sorted = sortDepths(R);
+        if (sorted.empty()) return;
+        std::vector<double> unew(N()), u(N()), z(N()), gv(N());
+        for (int dd = 0; dd < d; ++dd) {
+            if (R[dd]) continue;
+            for (int ti : sorted) {
+                Depth Tf = g.depths[ti];
+                int t = Tf[dd] + 1;
+                Tf[dd] = t;
+                auto it = g.depthIdx.find(Tf);
+                if (it == g.depthIdx.end()) continue;
+                std::fill(unew.begin(), unew.end(), 0.0);
+                setFromML(unew, P, ti);
+                prolongation1D(unew, t, dd);
+                mlAdd(P, unew, it->second);
+            }
+        }
+        for (int ti : sorted) {
+            std::fill(u.begin(), u.end(), 0.0);
+            setFromML(u, P, ti);
+            std::fill(z.begin(), z.end(), 0.0);
+            applyStencil(u, z, ti);
+            addFromML(z, gM, ti);
+            Depth Tc = g.depths[ti];
+            for (int dd = 0; dd < d; ++dd) {
+                if (!R[dd]) continue;
+                Tc[dd] = g.depths[ti][dd] - 1;
+                auto it = g.depthIdx.find(Tc);
+                if (it != g.depthIdx.end()) {
+                    restriction1D(z, Tc[dd], dd);
+                    gv = z;
+                    addFromML(z, gM, it->second);
+                    mlSet(gM, gv, it->second);
+                }
+            }
+            auto it = g.depthIdx.find(Tc);
+            if (it != g.depthIdx.end()) convertToPrewavelet(z, AxNeu, it->second);
+        }
+    }
+
+    // multiplication (MatrixVectorHomogen.h:660-759), serial non-MPI branch; cases by
+    // CasesIterator (tests/old_versions/MatrixVectorMultiplicationPrewavelets.h:23-50)
+    void matvec(const std::vector<double>& x, std::vector<double>& Ax) const {
+        size_t nd = g.depths.size();
+        MLVec nodal(nd);
+        calcNodal(x, nodal);
+        Ax.assign(N(), 0.0);
+        std::vector<double> AxNeu(N());
+        bool R[MAXD];
+        for (int c = 0; c < (1 << g.d); ++c) {
+            for (int i = 0; i < g.d; ++i) R[i] = (c >> i) & 1;
+            std::fill(AxNeu.begin(), AxNeu.end(), 0.0);
+            caseFunction(nodal, R, AxNeu);
+            for (int64_t s = 0; s < N(); ++s) Ax[s] = Ax[s] + AxNeu[s];
+        }
+    }
+
+    // (A_T v)_J for a point J of G_T: sum over in-grid cells of depth T+1 containing J
+    // (Preconditioning::applyStencilLocal, applications/cg_method.h:691-709, and
+    // StencilTemplate::returnValue / cellInGrid, stencils/Stencil.h:667-719)
+    double applyLocal(int ti, const Key& J, const std::unordered_map<Key, double, KeyHash>& v) const {
+        const Depth& T = g.depths[ti];
+        int d = g.d;
+        double h[MAXD];
+        int pos[MAXD];
+        for (int i = 0; i < d; ++i) {
+            h[i] = 1.0 / double(1 << T[i]);
+            // position of J at level T
+            int32_t idx = J.v[i];
+            int s = depth1(idx);
+            int32_t n = idx & ~(1 << s);
+            pos[i] = (2 * n + 1) << (T[i] - s);
+        }
+        double val = 0.0;
+        unsigned nc = 1u << d;
+        for (unsigned cs = 0; cs < nc; ++cs) {   // cells around J: lower corner pos - 1 + bit
+            int lo[MAXD];
+            for (int i = 0; i < d; ++i) lo[i] = pos[i] - 1 + ((cs >> i) & 1u);
+            // J's corner id in this cell: bit i = 1 if J is the upper node in dim i
+            unsigned pj = 0;
+            for (int i = 0; i < d; ++i) if (((cs >> i) & 1u) == 0) pj |= (1u << i);
+            // in grid?
+            bool ok = true;
+            Key Ks[1 << 8];
+            bool interior[1 << 8];
+            for (unsigned q = 0; q < nc && ok; ++q) {
+                Key K{};
+                bool in = true;
+                for (int i = 0; i < d; ++i) {
+                    K.v[i] = levelNodes[ti][i][lo[i] + ((q >> i) & 1u)];
+                    if (isBoundary1(K.v[i])) in = false;
+                }
+                Ks[q] = K; interior[q] = in;
+                int64_t k;
+                if (in && !g.occupied(K, k)) ok = false;
+            }
+            if (!ok) continue;
+            double ctr[MAXD];
+            for (int i = 0; i < d; ++i) ctr[i] = (double(lo[i]) + 0.5) * h[i];
+            for (unsigned q = 0; q < nc; ++q) {
+                if (!interior[q]) continue;
+                auto it = v.find(Ks[q]);
+                if (it == v.end()) continue;
+                val += op.element(d, h, ctr, pj, q) * it->second;
+            }
+        }
+        return val;
+    }
+
+    // Preconditioning::precond3 (applications/cg_method.h:322-451): z_p = psi_p^T A_T psi_p
+    void precond(std::vector<double>& z) const {
+        z.assign(N(), 0.0);
+        std::vector<std::pair<Key, double>> st;
+        for (size_t ti = 0; ti < g.depths.size(); ++ti) {
+            for (int64_t s : g.slotsOfDepth[ti]) {
+                if (!g.active[s]) continue;
+                stencil5(s, st);
+                std::unordered_map<Key, double, KeyHash> v;
+                for (auto& [J, c] : st) { int64_t k; if (g.occupied(J, k)) v[J] = c; }
+                double coeff = 0.0;
+                for (auto& [J, c] : st) {
+                    int64_t k;
+                    if (!g.occupied(J, k)) continue;
+                    coeff += c * applyLocal((int)ti, J, v);
+                }
+                z[s] = coeff;
+            }
+        }
+    }
+
+    // CG::solveHomogen (applications/cg_method.h:759-886); absolute stop on r^T D^-1 r.
+    int cg(double eps, std::vector<double>& x, const std::vector<double>& f, int maxIt,
+           std::vector<double>* deltaHist) const {
+        int64_t n = N();
+        std::vector<double> Dg;
+        precond(Dg);
+        std::vector<double> r(n), h(n, 0.0), dvec(n), gvec(n);
+        matvec(x, r);
+        for (int64_t i = 0; i < n; ++i) r[i] = f[i] - r[i];
+        for (int64_t i = 0; i < n; ++i) if (g.active[i]) h[i] = r[i];
+        for (int64_t i = 0; i < n; ++i) if (g.active[i]) h[i] = h[i] / Dg[i];
+        for (int64_t i = 0; i < n; ++i) dvec[i] = -1.0 * h[i];
+        double delta = 0.0;
+        for (int64_t i = 0; i < n; ++i) delta = delta + r[i] * h[i];
+        if (deltaHist) deltaHist->push_back(delta);
+        int k = 0;
+        for (int it = 1; it <= maxIt && delta > eps; ++it) {
+            k = it;
+            matvec(dvec, gvec);
+            double dg = 0.0;
+            for (int64_t i = 0; i < n; ++i) dg = dg + dvec[i] * gvec[i];
+            double tau = delta / dg;
+            for (int64_t i = 0; i < n; ++i) r[i] = r[i] + tau * gvec[i];
+            for (int64_t i = 0; i < n; ++i) x[i] = x[i] - tau * dvec[i];
+            for (int64_t i = 0; i < n; ++i) h[i] = 1.0 * r[i];
+            for (int64_t i = 0; i < n; ++i) if (g.active[i]) h[i] = h[i] / Dg[i];
+            double dp = 0.0;
+            for (int64_t i = 0; i < n; ++i) dp = dp + r[i] * h[i];
+            if (deltaHist) deltaHist->push_back(dp);
+            if (dp < eps * eps) break;
+            double beta = dp / delta;
+            delta = dp;
+            for (int64_t i = 0; i < n; ++i) dvec[i] = -1.0 * h[i] + beta * dvec[i];
+        }
+        return k;
+    }
+};
+
+}  // namespace orc
+
+// ---------------------------------------------------------------------------
+// C ABI for ctypes (tests only)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int64_t orc_regular_grid(int d, int level, int32_t* idx_out, uint8_t* active_out, int64_t cap) {
+    using namespace orc;
+    if (d < 1 || d > MAXD) return -1;
+    Grid g; g.d = d;
+    std::unordered_map<Key, char, KeyHash> seen;
+    Key c{};
+    for (int i = 0; i < d; ++i) c.v[i] = 2;
+    addRecursiveSons(g, c, level, seen);
+    int64_t n = (int64_t)g.pts.size();
+    if (idx_out && cap >= n) {
+        for (int64_t s = 0; s < n; ++s) {
+            for (int i = 0; i < d; ++i) idx_out[s * d + i] = g.pts[s].v[i];
+            if (active_out) active_out[s] = g.active[s];
+        }
+    }
+    return n;
+}
+
+void* orc_create(int d, int64_t n, const int32_t* idx, const uint8_t* active) {
+    using namespace orc;
+    if (d < 1 || d > MAXD) return nullptr;
+    auto* o = new Oracle();
+    o->g.d = d;
+    o->g.pts.resize(n);
+    o->g.active.resize(n);
+    for (int64_t s = 0; s < n; ++s) {
+        for (int i = 0; i < d; ++i) o->g.pts[s].v[i] = idx[s * d + i];
+        o->g.active[s] = active ? active[s] : 1;
+    }
+    o->init();
+    return o;
+}
+
+void orc_destroy(void* h) { delete (orc::Oracle*)h; }
+
+int orc_set_coef(void* h, int kind, double a0, double c0, const double* xi, int nxi) {
+    auto* o = (orc::Oracle*)h;
+    o->op.coef.kind = kind;
+    o->op.coef.a0 = a0;
+    o->op.coef.c0 = c0;
+    o->op.coef.xi.assign(xi, xi + (xi ? nxi : 0));
+    return 0;
+}
+
+int orc_num_depths(void* h) { return (int)((orc::Oracle*)h)->g.depths.size(); }
+
+void orc_depths(void* h, int32_t* out) {
+    auto* o = (orc::Oracle*)h;
+    for (size_t t = 0; t < o->g.depths.size(); ++t)
+        for (int i = 0; i < o->g.d; ++i) out[t * o->g.d + i] = o->g.depths[t][i];
+}
+
+int64_t orc_multilevel_size(void* h) {
+    auto* o = (orc::Oracle*)h;
+    int64_t s = 0;
+    for (auto& v : o->ml.slotsG) s += (int64_t)v.size();
+    return s;
+}
+
+int orc_sorted_depths(void* h, int caseno, int32_t* out) {
+    auto* o = (orc::Oracle*)h;
+    bool R[orc::MAXD];
+    for (int i = 0; i < o->g.d; ++i) R[i] = (caseno >> i) & 1;
+    auto s = o->sortDepths(R);
+    if (out) for (size_t k = 0; k < s.size(); ++k) out[k] = s[k];
+    return (int)s.size();
+}
+
+int orc_matvec(void* h, const double* x, double* y) {
+    auto* o = (orc::Oracle*)h;
+    std::vector<double> xv(x, x + o->N()), yv;
+    o->matvec(xv, yv);
+    std::memcpy(y, yv.data(), sizeof(double) * o->N());
+    return 0;
+}
+
+int orc_precond(void* h, double* z) {
+    auto* o = (orc::Oracle*)h;
+    std::vector<double> zv;
+    o->precond(zv);
+    std::memcpy(z, zv.data(), sizeof(double) * o->N());
+    return 0;
+}
+
+int orc_cg(void* h, double eps, const double* f, double* x, int maxit, double* delta_hist, int hist_cap) {
+    auto* o = (orc::Oracle*)h;
+    std::vector<double> fv(f, f + o->N()), xv(x, x + o->N()), hist;
+    int it = o->cg(eps, xv, fv, maxit, &hist);
+    std::memcpy(x, xv.data(), sizeof(double) * o->N());
+    if (delta_hist) for (int k = 0; k < hist_cap && k < (int)hist.size(); ++k) delta_hist[k] = hist[k];
+    return it;
+}
+
+}  // extern "C"

@@ -1,0 +1,325 @@
+// ref_harness.cpp -- drives the compiled LoAdSG reference (built from the
+// unmodified sources under /root/reference by oracle/Makefile) to write golden
+// vectors as .npy files.  TEST INFRASTRUCTURE ONLY: its outputs are fixtures
+// for tests/ and a CPU timing sample for bench.py's cpu_baseline leg.
+//
+// It uses the reference's own public API exactly as its example drivers do
+// (examples_doxygen/poisson_regular.cpp:39-120):
+//   AdaptiveSparseGrid::AddRecursiveSonsOfPoint, completeDirichletGrid,
+//   MultiLevelAdaptiveSparseGrid, VectorSparseG, calcPrewByNodal,
+//   MatrixVectorHomogen(grid, mgrid, true, false)::multiplication,
+//   Preconditioning<S>, CG::solveHomogen.
+// The variable-coefficient operator is supplied through the reference's plug-in
+// hook StencilTemplate::integration / localValue (stencils/Stencil.h:735-748):
+// element matrix = a(cell centre) * Poisson + c(cell centre) * HelmHoltz.
+//
+// Usage:
+//   ref_harness regular  <level> <coef> <outdir> [what...]
+//   ref_harness adaptive <startlevel> <radius> <maxlevel> <coef> <outdir> [what...]
+//   ref_harness timemv   <level> <coef> <reps>            (timing sample, prints JSON)
+// coef: const:<a0>:<c0> | cfg2 | cfg3 | expsin:<xi1>,<xi2>,...
+// what: matvec precond cg   (default: all)
+#include "source/sparseEXPDE.h"
+
+#include <string>
+#include <vector>
+#include <cstdio>
+#include <cstring>
+#include <cmath>
+#include <chrono>
+#include <sstream>
+
+namespace {
+
+struct CoefSpec {
+    int kind = 0;  // 0 const, 1 cfg2, 2 cfg3, 3 expsin
+    double a0 = 1.0, c0 = 0.0;
+    std::vector<double> xi;
+    double a(const double* x) const {
+        const int d = DimensionSparseGrid;
+        switch (kind) {
+            case 0: return a0;
+            case 1: { double p = 1.0; for (int i = 0; i < d; ++i) p *= std::cos(M_PI * x[i]); return 1.0 + 0.5 * p; }
+            case 2: { double s = 1.0; for (int i = 0; i < d; ++i) s += x[i] * x[i]; return s; }
+            case 3: {
+                double s = 0.0;
+                for (size_t k = 1; k <= xi.size(); ++k)
+                    s += xi[k - 1] * std::sin(double(k) * M_PI * x[k % d]) / double(k);
+                return std::exp(s);
+            }
+        }
+        return 0.0;
+    }
+    double c(const double* x) const {
+        switch (kind) {
+            case 0: return c0;
+            case 1: return 1.0 + x[0];
+            default: return 0.0;
+        }
+    }
+};
+
+CoefSpec parseCoef(const std::string& s) {
+    CoefSpec c;
+    if (s.rfind("const:", 0) == 0) {
+        c.kind = 0;
+        std::sscanf(s.c_str() + 6, "%lf:%lf", &c.a0, &c.c0);
+    } else if (s == "cfg2") c.kind = 1;
+    else if (s == "cfg3") c.kind = 2;
+    else if (s.rfind("expsin:", 0) == 0) {
+        c.kind = 3;
+        std::stringstream ss(s.substr(7));
+        std::string tok;
+        while (std::getline(ss, tok, ',')) c.xi.push_back(std::stod(tok));
+    } else {
+        std::fprintf(stderr, "bad coef %s\n", s.c_str());
+        std::exit(2);
+    }
+    return c;
+}
+
+// Variable-coefficient plug-in: midpoint value of a and c on each cell times
+// the reference's own Poisson / HelmHoltz element integrals.
+class VarCoeffStencil : public StencilTemplate {
+public:
+    VarCoeffStencil(AdaptiveSparseGrid& g, const CoefSpec& cs) : StencilTemplate(g), P(g), H(g), spec(cs) {}
+    TypeMatrixVectorMultiplication getTypeMatrixVectorMultiplication() { return StencilOnTheFly; }
+    inline void initialize(Depth& T_) {
+        StencilTemplate::initialize(T_);
+        P.initialize(T_);
+        H.initialize(T_);
+    }
+    inline double integration(CellDimension& cell, CellIndexDirection& dirP, CellIndexDirection& dirQ) {
+        double x[DimensionSparseGrid];
+        for (int i = 0; i < DimensionSparseGrid; ++i) x[i] = cell.coordinate(i);
+        double a = spec.a(x), c = spec.c(x);
+        double v = 0.0;
+        if (a != 0.0) v += a * P.integration(cell, dirP, dirQ);
+        if (c != 0.0) v += c * H.integration(cell, dirP, dirQ);
+        return v;
+    }
+    inline double localValue(const IndexDimension& center, const IndexDimension& cell, const MultiDimCompass& mc) const {
+        double x[DimensionSparseGrid];
+        for (int i = 0; i < DimensionSparseGrid; ++i) x[i] = cell.coordinate(i) + 0.5 * meshwidth[i];
+        double a = spec.a(x), c = spec.c(x);
+        double v = 0.0;
+        if (a != 0.0) v += a * P.localValue(center, cell, mc);
+        if (c != 0.0) v += c * H.localValue(center, cell, mc);
+        return v;
+    }
+    Poisson P;
+    HelmHoltz H;
+    CoefSpec spec;
+};
+
+// --- minimal .npy writer ----------------------------------------------------
+void writeNpy(const std::string& path, const char* descr, const void* data, size_t elsize,
+              const std::vector<size_t>& shape) {
+    std::string shp = "(";
+    for (size_t i = 0; i < shape.size(); ++i) {
+        shp += std::to_string(shape[i]);
+        shp += (shape.size() == 1 || i + 1 < shape.size()) ? "," : "";
+        if (i + 1 < shape.size()) shp += " ";
+    }
+    shp += ")";
+    std::string hdr = std::string("{'descr': '") + descr + "', 'fortran_order': False, 'shape': " + shp + ", }";
+    size_t total = 10 + hdr.size() + 1;
+    size_t pad = (64 - total % 64) % 64;
+    hdr += std::string(pad, ' ');
+    hdr += "\n";
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) { std::perror(path.c_str()); std::exit(3); }
+    const char magic[] = "\x93NUMPY";
+    std::fwrite(magic, 1, 6, f);
+    unsigned char ver[2] = {1, 0};
+    std::fwrite(ver, 1, 2, f);
+    uint16_t hl = (uint16_t)hdr.size();
+    std::fwrite(&hl, 2, 1, f);
+    std::fwrite(hdr.data(), 1, hdr.size(), f);
+    size_t n = 1;
+    for (auto s : shape) n *= s;
+    std::fwrite(data, elsize, n, f);
+    std::fclose(f);
+}
+
+void writeVec(const std::string& dir, const std::string& name, VectorSparseG& v, size_t n) {
+    std::vector<double> buf(n);
+    for (size_t i = 0; i < n; ++i) buf[i] = v.getValue((unsigned long)i);
+    writeNpy(dir + "/" + name + ".npy", "<f8", buf.data(), 8, {n});
+}
+
+double sinFunc(IndexDimension I) {       // poisson_regular.cpp:5-11
+    double v = 1.0;
+    for (int d = 0; d < DimensionSparseGrid; d++) v *= std::sin(M_PI * I.coordinate(d));
+    return v;
+}
+
+bool want(const std::vector<std::string>& w, const char* s) {
+    if (w.empty()) return true;
+    for (auto& x : w) if (x == s) return true;
+    return false;
+}
+
+void runAll(AdaptiveSparseGrid& grid, const CoefSpec& cs, const std::string& out,
+            const std::vector<std::string>& what) {
+    MultiLevelAdaptiveSparseGrid mgrid(&grid);
+    size_t n = grid.getMaximalOccupiedSecondTable();
+    const int d = DimensionSparseGrid;
+    std::vector<int32_t> idx(n * d);
+    std::vector<uint8_t> act(n);
+    for (size_t i = 0; i < n; ++i) {
+        IndexDimension I = grid.getIndexOfTable(i);
+        for (int k = 0; k < d; ++k) idx[i * d + k] = I.getIndex(k);
+        act[i] = grid.getActiveTable()[i] ? 1 : 0;
+    }
+    writeNpy(out + "/idx.npy", "<i4", idx.data(), 4, {n, (size_t)d});
+    writeNpy(out + "/active.npy", "|u1", act.data(), 1, {n});
+
+    MatrixVectorHomogen m(grid, mgrid, true, false);
+    Poisson poisson(grid);
+    HelmHoltz helm(grid);
+    VarCoeffStencil var(grid, cs);
+
+    VectorSparseG x(grid), y(grid);
+    for (size_t i = 0; i < n; ++i) x.setValue((unsigned long)i, act[i] ? 1.0 / (1.0 + double(i % 7)) : 0.0);
+    writeVec(out, "x", x, n);
+
+    if (want(what, "matvec")) {
+        y = 0.0; m.multiplication(x, y, poisson); writeVec(out, "Ax_poisson", y, n);
+        y = 0.0; m.multiplication(x, y, helm);    writeVec(out, "Ax_helm", y, n);
+        y = 0.0; m.multiplication(x, y, var);     writeVec(out, "Ax_var", y, n);
+    }
+    if (want(what, "precond")) {
+        VectorSparseG z1(grid), z2(grid);
+        { Preconditioning<Poisson> P1(z1, poisson); }
+        { Preconditioning<VarCoeffStencil> P2(z2, var); }
+        writeVec(out, "diag_poisson", z1, n);
+        writeVec(out, "diag_var", z2, n);
+    }
+    if (want(what, "cg")) {
+        // RHS in the reference driver's convention f_h = M * prew(f_I)
+        // (poisson_regular.cpp:57-113) for f = d*pi^2*prod sin(pi x_i).
+        VectorSparseG fvec(grid), ftest(grid), prew(grid), u(grid);
+        fvec = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            if (!act[i]) continue;
+            IndexDimension I = grid.getIndexOfTable(i);
+            fvec.setValue((unsigned long)i, double(d) * M_PI * M_PI * sinFunc(I));
+        }
+        prew = 0.0;
+        calcPrewByNodal(prew, fvec);
+        writeVec(out, "f_prew", prew, n);
+        ftest = 0.0;
+        m.multiplication(prew, ftest, helm);
+        writeVec(out, "f_rhs", ftest, n);
+        int it = 0;
+        double tp = 0.0;
+        VectorSparseG sol(grid);
+        sol = 0.0;
+        CG::solveHomogen(1e-10, sol, ftest, &it, u, m, poisson, &tp);
+        writeVec(out, "cg_poisson_x", sol, n);
+        int32_t its = it;
+        writeNpy(out + "/cg_poisson_iters.npy", "<i4", &its, 4, {1});
+        VectorSparseG sol2(grid);
+        sol2 = 0.0;
+        it = 0;
+        CG::solveHomogen(1e-10, sol2, ftest, &it, u, m, var, &tp);
+        writeVec(out, "cg_var_x", sol2, n);
+        its = it;
+        writeNpy(out + "/cg_var_iters.npy", "<i4", &its, 4, {1});
+    }
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    if (argc < 2) { std::fprintf(stderr, "usage: see header\n"); return 2; }
+    std::string mode = argv[1];
+    if (mode == "regular" && argc >= 5) {
+        int level = std::atoi(argv[2]);
+        CoefSpec cs = parseCoef(argv[3]);
+        std::string out = argv[4];
+        std::vector<std::string> what(argv + 5, argv + argc);
+        AdaptiveSparseGrid grid;
+        IndexDimension center;
+        grid.AddRecursiveSonsOfPoint(center, level);
+        runAll(grid, cs, out, what);
+        return 0;
+    }
+    if (mode == "adaptive" && argc >= 7) {
+        // Config-3 style locally adaptive grid: a regular start grid, then repeated
+        // refinement (both sons in every dimension) of active points whose support
+        // box is closer than `radius` to the origin, while the son's sum of
+        // (depth_i - 1) <= maxlevel, then the reference's completeDirichletGrid().
+        int start = std::atoi(argv[2]);
+        double radius = std::atof(argv[3]);
+        int maxlevel = std::atoi(argv[4]);
+        CoefSpec cs = parseCoef(argv[5]);
+        std::string out = argv[6];
+        std::vector<std::string> what(argv + 7, argv + argc);
+        AdaptiveSparseGrid grid;
+        IndexDimension center;
+        grid.AddRecursiveSonsOfPoint(center, start);
+        for (bool changed = true; changed;) {
+            changed = false;
+            unsigned long end = grid.getMaximalOccupiedSecondTable();
+            for (unsigned long i = 0; i < end; ++i) {
+                if (!grid.getActiveTable()[i]) continue;
+                IndexDimension I = grid.getIndexOfTable(i);
+                // distance from the origin to the point's support box
+                // [x_k - 2^-depth_k, x_k + 2^-depth_k]
+                double r2 = 0.0;
+                for (int k = 0; k < DimensionSparseGrid; ++k) {
+                    double lo = I.coordinate(k) - 1.0 / double(1 << I.getDepth(k));
+                    if (lo > 0.0) r2 += lo * lo;
+                }
+                if (std::sqrt(r2) >= radius) continue;
+                for (int k = 0; k < DimensionSparseGrid; ++k) {
+                    IndexDimension S[2] = {I.leftSon(k), I.rightSon(k)};
+                    for (auto& J : S) {
+                        int lsum = 0;
+                        for (int q = 0; q < DimensionSparseGrid; ++q) lsum += J.getDepth(q) - 1;
+                        if (lsum > maxlevel) continue;
+                        unsigned long kk;
+                        if (!grid.occupied(kk, J)) { grid.AddPoint(J); changed = true; }
+                    }
+                }
+            }
+        }
+        grid.completeDirichletGrid();
+        runAll(grid, cs, out, what);
+        return 0;
+    }
+    if (mode == "timemv" && argc >= 5) {
+        int level = std::atoi(argv[2]);
+        CoefSpec cs = parseCoef(argv[3]);
+        int reps = std::atoi(argv[4]);
+        AdaptiveSparseGrid grid;
+        IndexDimension center;
+        auto t0 = std::chrono::steady_clock::now();
+        grid.AddRecursiveSonsOfPoint(center, level);
+        MultiLevelAdaptiveSparseGrid mgrid(&grid);
+        MatrixVectorHomogen m(grid, mgrid, true, false);
+        VarCoeffStencil var(grid, cs);
+        auto t1 = std::chrono::steady_clock::now();
+        VectorSparseG x(grid), y(grid);
+        size_t n = grid.getMaximalOccupiedSecondTable();
+        for (size_t i = 0; i < n; ++i) x.setValue((unsigned long)i, 1.0 / (1.0 + double(i % 7)));
+        std::vector<double> ts;
+        double chk = 0.0;
+        for (int r = 0; r < reps; ++r) {
+            auto a = std::chrono::steady_clock::now();
+            y = 0.0;
+            m.multiplication(x, y, var);
+            auto b = std::chrono::steady_clock::now();
+            ts.push_back(std::chrono::duration<double>(b - a).count());
+            chk += y.getValue((unsigned long)0);
+        }
+        std::sort(ts.begin(), ts.end());
+        std::printf("{\"n\": %zu, \"setup_s\": %.6f, \"matvec_s_median\": %.6f, \"reps\": %d, \"check\": %.17g}\n",
+                    n, std::chrono::duration<double>(t1 - t0).count(), ts[ts.size() / 2], reps, chk);
+        return 0;
+    }
+    std::fprintf(stderr, "bad arguments\n");
+    return 2;
+}

@@ -1,0 +1,58 @@
+"""Writes the golden fixtures under tests/golden/<case>/ by running the compiled
+LoAdSG reference (oracle/_ref/ref_harness_d<D>, built by `make -C oracle ref`
+from the unmodified sources in /root/reference).
+
+Each case directory holds .npy arrays written by the harness:
+  idx (n x d int32, reference slot order), active (n uint8), x (input vector),
+  Ax_poisson / Ax_helm / Ax_var (MatrixVectorHomogen::multiplication),
+  diag_poisson / diag_var (Preconditioning<S>::precond3),
+  f_prew / f_rhs (calcPrewByNodal and the driver's f_h = M prew(f_I)),
+  cg_poisson_x / cg_poisson_iters / cg_var_x / cg_var_iters (CG::solveHomogen, eps 1e-10).
+The reference preconditioner is run with OMP_NUM_THREADS=1 so that its stale
+hanging-node work-vector values (cg_method.h:356-433 resets v with an
+active-only `v = 0.0`) are deterministic.
+
+Run:  python tests/golden/make_golden.py
+"""
+import json
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+# name: (d, argv after the binary)
+CASES = {
+    "d2_l6_cfg2": (2, ["regular", "6", "cfg2"]),
+    "d3_l4_cfg3": (3, ["regular", "4", "cfg3"]),
+    "d4_l3_cfg2": (4, ["regular", "3", "cfg2"]),
+    "d5_l3_expsin": (5, ["regular", "3", "expsin:0.3,-0.5,0.2,0.4,-0.1,0.25,-0.35,0.15"]),
+    "d3_adaptive_cfg3": (3, ["adaptive", "3", "0.15", "6", "cfg3"]),
+}
+
+
+def main():
+    meta = {}
+    for name, (d, args) in CASES.items():
+        out = os.path.join(HERE, name)
+        os.makedirs(out, exist_ok=True)
+        exe = os.path.join(REF, "ref_harness_d%d" % d)
+        if not os.path.exists(exe):
+            sys.exit("missing %s: run `make -C oracle ref`" % exe)
+        env = dict(os.environ, OMP_NUM_THREADS="1")
+        cmd = [exe] + args[:-1 if args[0] == "regular" else None]
+        if args[0] == "regular":
+            cmd = [exe, "regular", args[1], args[2], out]
+        else:
+            cmd = [exe, "adaptive", args[1], args[2], args[3], args[4], out]
+        print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True, env=env)
+        meta[name] = {"d": d, "argv": args}
+    with open(os.path.join(HERE, "cases.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

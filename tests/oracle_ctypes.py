@@ -1,0 +1,111 @@
+"""ctypes binding of the CPU oracle (oracle/lsg_oracle.cpp).
+
+Test infrastructure only: the oracle is the checker, never the thing measured
+or shipped.  Build it with ``make -C oracle oracle``.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
+
+COEF_CONST, COEF_CFG2, COEF_CFG3, COEF_EXPSIN = 0, 1, 2, 3
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.orc_regular_grid.restype = ctypes.c_int64
+        L.orc_regular_grid.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64]
+        L.orc_create.restype = ctypes.c_void_p
+        L.orc_create.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_destroy.argtypes = [ctypes.c_void_p]
+        L.orc_set_coef.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                   ctypes.c_void_p, ctypes.c_int]
+        L.orc_num_depths.argtypes = [ctypes.c_void_p]
+        L.orc_depths.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_multilevel_size.restype = ctypes.c_int64
+        L.orc_multilevel_size.argtypes = [ctypes.c_void_p]
+        L.orc_sorted_depths.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p]
+        L.orc_matvec.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_precond.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_cg.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                             ctypes.c_void_p, ctypes.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def regular_grid(d, level):
+    """Reference slot order of AddRecursiveSonsOfPoint(center, level)."""
+    L = lib()
+    n = L.orc_regular_grid(d, level, None, None, 0)
+    idx = np.zeros((n, d), dtype=np.int32)
+    act = np.zeros(n, dtype=np.uint8)
+    L.orc_regular_grid(d, level, _p(idx), _p(act), n)
+    return idx, act
+
+
+class Oracle:
+    def __init__(self, idx, active=None):
+        idx = np.ascontiguousarray(idx, dtype=np.int32)
+        self.n, self.d = idx.shape
+        self.active = (np.ones(self.n, dtype=np.uint8) if active is None
+                       else np.ascontiguousarray(active, dtype=np.uint8))
+        self._idx = idx
+        self.h = lib().orc_create(self.d, self.n, _p(idx), _p(self.active))
+        if not self.h:
+            raise RuntimeError("orc_create failed")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def set_coef(self, kind, a0=1.0, c0=0.0, xi=None):
+        xi = np.zeros(0) if xi is None else np.ascontiguousarray(xi, dtype=np.float64)
+        lib().orc_set_coef(self.h, kind, a0, c0, _p(xi) if xi.size else None, int(xi.size))
+        return self
+
+    def depths(self):
+        nd = lib().orc_num_depths(self.h)
+        out = np.zeros((nd, self.d), dtype=np.int32)
+        lib().orc_depths(self.h, _p(out))
+        return out
+
+    def multilevel_size(self):
+        return lib().orc_multilevel_size(self.h)
+
+    def sorted_depths(self, case):
+        n = lib().orc_sorted_depths(self.h, case, None)
+        out = np.zeros(max(n, 1), dtype=np.int32)
+        lib().orc_sorted_depths(self.h, case, _p(out))
+        return out[:n]
+
+    def matvec(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n)
+        lib().orc_matvec(self.h, _p(x), _p(y))
+        return y
+
+    def precond(self):
+        z = np.zeros(self.n)
+        lib().orc_precond(self.h, _p(z))
+        return z
+
+    def cg(self, f, eps=1e-10, maxit=500, x0=None):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64)
+        hist = np.zeros(maxit + 2)
+        it = lib().orc_cg(self.h, eps, _p(f), _p(x), maxit, _p(hist), hist.size)
+        return x, it, hist[: it + 1]
