@@ -1,0 +1,133 @@
+/*
+ * loadsg_amd.h -- C ABI of the MI355X (gfx950) implementation of LoAdSG's
+ * matrix-free prewavelet operator application and CG solve on sparse grids
+ * with zero Dirichlet boundaries.
+ *
+ * Plain pointers and sizes only.  Every function returns an int status
+ * (LSG_OK = 0, negative on error; lsg_last_error() describes the last error of
+ * the calling thread).  Nothing throws across this boundary.
+ *
+ * Vectors cross the boundary in one of two layouts:
+ *   LSG_HOST_SLOT   host double arrays in the reference's slot order, i.e. the
+ *                   layout of VectorSparseG's data table
+ *                   (library/source/extemp/vector.h:247-441, vector.cc:12-23):
+ *                   entry i belongs to AdaptiveSparseGrid::getIndexOfTable(i).
+ *   LSG_DEVICE      device arrays in this library's depth-blocked order
+ *                   (lsg_grid_device_order gives the permutation), one block of
+ *                   lsg_grid_size() entries per batch instance, value type of
+ *                   the operator's precision (double for 64, float for 32).
+ */
+#ifndef LOADSG_AMD_H
+#define LOADSG_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LSG_OK 0
+#define LSG_ERR_ARG (-1)
+#define LSG_ERR_UNSUPPORTED (-2)
+#define LSG_ERR_CAPACITY (-3)
+#define LSG_ERR_DEVICE (-4)
+#define LSG_ERR_NAN (-5)
+#define LSG_ERR_NOCONV (-6)
+
+#define LSG_HOST_SLOT 0
+#define LSG_DEVICE 1
+
+/* coefficient families for -div(a grad u) + c u (evaluated at cell centres) */
+#define LSG_COEF_CONST 0   /* a = a0, c = c0 (Poisson: a0=1,c0=0; HelmHoltz: a0=0,c0=1) */
+#define LSG_COEF_CFG2 1    /* a = 1 + 0.5 prod cos(pi x_i), c = 1 + x_1 */
+#define LSG_COEF_CFG3 2    /* a = 1 + |x|^2, c = 0 */
+#define LSG_COEF_EXPSIN 3  /* a = exp(sum_k xi_k sin(k pi x_{1+(k mod d)}) / k), c = 0 */
+
+/* CG stopping rules */
+#define LSG_STOP_REF_ABS 0   /* reference rule: r^T D^-1 r < eps^2 (cg_method.h:845-871) */
+#define LSG_STOP_REL 1       /* ||f - A x||_2 <= eps ||f||_2 */
+
+typedef struct lsg_grid lsg_grid;
+typedef struct lsg_op lsg_op;
+
+typedef struct lsg_coef {
+    int kind;
+    double a0, c0;        /* LSG_COEF_CONST */
+    int nxi;              /* LSG_COEF_EXPSIN */
+    const double* xi;
+} lsg_coef;
+
+typedef struct lsg_solve_info {
+    int iterations;        /* CG iterations (reference counting, cg_method.h:853-883) */
+    int outer;             /* iterative-refinement sweeps (precision 32 only) */
+    double final_delta;    /* last r^T D^-1 r */
+    double rel_residual;   /* ||f - A x||_2 / ||f||_2 at exit (FP64) */
+    int converged;
+} lsg_solve_info;
+
+/* ------------------------------------------------------------------ grids */
+/* Grid from an index table in reference slot order: idx[i*d + k] is the 1D index
+ * (0/1 boundary, 2 = 0.5, sons 2j, 2j+1; indices/index.h:35-81) of point i in
+ * dimension k; active[i] = AdaptiveSparseGrid::getActiveTable()[i].
+ * Replaces the grid handed to MatrixVectorHomogen(AdaptiveSparseGrid&, ...)
+ * (MatrixVectorMultiplication/MatrixVectorHomogen.h:72-77). */
+int lsg_grid_create(int d, int64_t n, const int32_t* idx, const uint8_t* active, lsg_grid** out);
+/* Regular grid, identical slot order to AdaptiveSparseGrid::AddRecursiveSonsOfPoint(
+ * center, level) (sgrid/sparseGrid.cc:557-566). */
+int lsg_grid_regular(int d, int level, lsg_grid** out);
+int64_t lsg_grid_size(const lsg_grid* g);
+int lsg_grid_dim(const lsg_grid* g);
+/* Index table and active flags in reference slot order
+ * (AdaptiveSparseGrid_Base::getIndexOfTable / getActiveTable, sparseGrid.h:145-173). */
+int lsg_grid_slot_order(const lsg_grid* g, int32_t* idx_out, uint8_t* active_out);
+/* dev_of_slot[i] = position of slot i in the device (depth-blocked) order. */
+int lsg_grid_device_order(const lsg_grid* g, int64_t* dev_of_slot);
+/* Structural statistics: out[0]=#depths, [1]=multilevel size N_ML, [2]=#apply items,
+ * [3]=#wavefront levels, [4]=algorithmic values moved per matvec (streaming model),
+ * [5]=largest depth box, [6]=case-buffer size, [7]=W-buffer size. */
+int lsg_grid_stats(const lsg_grid* g, int64_t* out, int nout);
+int lsg_grid_destroy(lsg_grid* g);
+
+/* -------------------------------------------------------------- operators */
+/* Operator a(u,v) = int a grad u . grad v + c u v on the grid, matrix-free in the
+ * prewavelet basis: replaces MatrixVectorHomogen + a StencilTemplate plug-in
+ * (Poisson / HelmHoltz / a variable-coefficient integration(), stencils/Stencil.h:138-988,
+ * 1605-1793).  precision: 64 (FP64) or 32 (FP32 storage and arithmetic, FP64 reductions).
+ * batch: number of instances sharing the grid (coefficients may differ per instance). */
+int lsg_op_create(lsg_grid* g, const lsg_coef* coef, int precision, int batch, lsg_op** out);
+int lsg_op_set_coef(lsg_op* op, int instance, const lsg_coef* coef);
+int lsg_op_batch(const lsg_op* op);
+/* y = A x: MatrixVectorHomogen::multiplication(prew, Ax, stencil)
+ * (MatrixVectorHomogen.h:660-759).  LSG_HOST_SLOT: one instance (instance 0) in slot
+ * order.  LSG_DEVICE: all batch instances, device order. */
+int lsg_matvec(lsg_op* op, const void* x, void* y, int layout);
+/* Diagonal preconditioner z_p = a(psi_p, psi_p): Preconditioning<S>::precond3
+ * (applications/cg_method.h:322-451). */
+int lsg_precond_diag(lsg_op* op, void* z, int layout);
+/* CG solve A x = f: CG::solveHomogen(eps, x, f, iterations, usol, matrix, stencil, time)
+ * (applications/cg_method.h:759-886).  x holds the start vector on entry.
+ * info: array of lsg_op_batch() entries (may be NULL). */
+int lsg_cg_solve(lsg_op* op, const void* f, void* x, double eps, int stop_rule, int maxit,
+                 lsg_solve_info* info, int layout);
+/* Mixed precision: FP32 operator inner PCG inside FP64 iterative refinement; the
+ * outer residual f - A x, all dot products/norms and the stop test are FP64 on the
+ * device.  op64 and op32 must share the grid and batch.  stop rule LSG_STOP_REL. */
+int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_tol,
+                 double inner_rel_tol, int maxit, lsg_solve_info* info, int layout);
+/* Load vector b_p = int f psi_p of f = sum_j eta_j prod_i sin(m_j pi x_i) (exact 1D
+ * quadrature of each factor), for every instance: eta[inst*nterms + j], m[j]. */
+int lsg_rhs_sines(lsg_op* op, int nterms, const double* eta, const int32_t* m, void* b, int layout);
+/* Device timing of the last lsg_matvec: out[0]=total ms, out[1]=apply-kernel ms,
+ * out[2]=number of apply launches. */
+int lsg_last_timing(const lsg_op* op, double* out, int nout);
+/* Tuning: threads per workgroup for the apply stage (64, 128, 256, 512); 0 = autotune. */
+int lsg_op_set_block(lsg_op* op, int stage, int threads);
+int lsg_op_destroy(lsg_op* op);
+
+const char* lsg_last_error(void);
+int lsg_device_available(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LOADSG_AMD_H */

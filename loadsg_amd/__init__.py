@@ -1,0 +1,303 @@
+"""loadsg_amd -- MI355X (gfx950) matrix-free prewavelet operator and CG solver
+for sparse grids with zero Dirichlet boundaries (LoAdSG's hot path).
+
+Thin ctypes layer over the C ABI in include/loadsg_amd.h.  The names mirror the
+reference's C++ interface:
+
+  AdaptiveSparseGrid            library/source/sgrid/sparseGrid.h:86-387
+  MatrixVectorHomogen           library/source/MatrixVectorMultiplication/MatrixVectorHomogen.h:32-92
+      .multiplication(x, Ax)    MatrixVectorHomogen.h:660-759
+  Poisson / HelmHoltz           library/source/stencils/Stencil.h:873-988, 1605-1793
+  VariableCoefficient           a StencilTemplate::integration plug-in (Stencil.h:740)
+  CG.solveHomogen               library/source/applications/cg_method.h:27-33, 759-886
+  Preconditioning.diagonal      library/source/applications/cg_method.h:82-451
+
+Host vectors are float64 numpy arrays in the reference's slot order (the layout of
+VectorSparseG's data table).  Device vectors are torch tensors on the GPU in this
+library's depth-blocked order (see AdaptiveSparseGrid.device_order).
+
+There is no CPU fallback: if the HIP library is not built, importing works but
+every operation raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libloadsg_amd.so")
+
+LSG_HOST_SLOT, LSG_DEVICE = 0, 1
+STOP_REF_ABS, STOP_REL = 0, 1
+COEF_CONST, COEF_CFG2, COEF_CFG3, COEF_EXPSIN = 0, 1, 2, 3
+
+_lib = None
+
+
+class LoadsgError(RuntimeError):
+    pass
+
+
+class _Coef(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("a0", ctypes.c_double), ("c0", ctypes.c_double),
+                ("nxi", ctypes.c_int), ("xi", ctypes.POINTER(ctypes.c_double))]
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", ctypes.c_int), ("outer", ctypes.c_int), ("final_delta", ctypes.c_double),
+                ("rel_residual", ctypes.c_double), ("converged", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def lib():
+    """Load the gfx950 library (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LoadsgError("libloadsg_amd.so not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+                              "or `python loadsg_amd/build.py`")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, i32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        sig = {
+            "lsg_grid_create": (i32, [i32, i64, vp, vp, ctypes.POINTER(vp)]),
+            "lsg_grid_regular": (i32, [i32, i32, ctypes.POINTER(vp)]),
+            "lsg_grid_size": (i64, [vp]),
+            "lsg_grid_dim": (i32, [vp]),
+            "lsg_grid_slot_order": (i32, [vp, vp, vp]),
+            "lsg_grid_device_order": (i32, [vp, vp]),
+            "lsg_grid_stats": (i32, [vp, vp, i32]),
+            "lsg_grid_destroy": (i32, [vp]),
+            "lsg_op_create": (i32, [vp, ctypes.POINTER(_Coef), i32, i32, ctypes.POINTER(vp)]),
+            "lsg_op_set_coef": (i32, [vp, i32, ctypes.POINTER(_Coef)]),
+            "lsg_op_batch": (i32, [vp]),
+            "lsg_matvec": (i32, [vp, vp, vp, i32]),
+            "lsg_precond_diag": (i32, [vp, vp, i32]),
+            "lsg_cg_solve": (i32, [vp, vp, vp, dbl, i32, i32, vp, i32]),
+            "lsg_ir_solve": (i32, [vp, vp, vp, vp, dbl, dbl, i32, vp, i32]),
+            "lsg_rhs_sines": (i32, [vp, i32, vp, vp, vp, i32]),
+            "lsg_last_timing": (i32, [vp, vp, i32]),
+            "lsg_op_set_block": (i32, [vp, i32, i32]),
+            "lsg_op_destroy": (i32, [vp]),
+            "lsg_last_error": (ctypes.c_char_p, []),
+            "lsg_device_available": (i32, []),
+        }
+        for name, (res, args) in sig.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise LoadsgError("loadsg_amd error %d: %s" % (rc, lib().lsg_last_error().decode()))
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    if hasattr(a, "data_ptr"):
+        return ctypes.c_void_p(a.data_ptr())
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _sync_torch(*ts):
+    for t in ts:
+        if hasattr(t, "is_cuda") and t.is_cuda:
+            import torch
+            torch.cuda.synchronize(t.device)
+            return
+
+
+class AdaptiveSparseGrid:
+    """Sparse grid in the reference's slot order (AdaptiveSparseGrid, sparseGrid.h:86-387)."""
+
+    def __init__(self, idx=None, active=None, _handle=None):
+        self._h = ctypes.c_void_p()
+        if _handle is not None:
+            self._h = _handle
+        else:
+            idx = np.ascontiguousarray(idx, dtype=np.int32)
+            n, d = idx.shape
+            act = None if active is None else np.ascontiguousarray(active, dtype=np.uint8)
+            _check(lib().lsg_grid_create(d, n, _ptr(idx), _ptr(act), ctypes.byref(self._h)))
+
+    @classmethod
+    def regular(cls, d, level):
+        """AddRecursiveSonsOfPoint(center, level) (sparseGrid.cc:557-566)."""
+        h = ctypes.c_void_p()
+        _check(lib().lsg_grid_regular(d, level, ctypes.byref(h)))
+        return cls(_handle=h)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.lsg_grid_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def dim(self):
+        return lib().lsg_grid_dim(self._h)
+
+    def getMaximalOccupiedSecondTable(self):
+        return lib().lsg_grid_size(self._h)
+
+    size = property(getMaximalOccupiedSecondTable)
+
+    def slot_order(self):
+        n, d = self.size, self.dim
+        idx = np.zeros((n, d), dtype=np.int32)
+        act = np.zeros(n, dtype=np.uint8)
+        _check(lib().lsg_grid_slot_order(self._h, _ptr(idx), _ptr(act)))
+        return idx, act
+
+    def device_order(self):
+        """dev_of_slot[i] = device position of reference slot i."""
+        out = np.zeros(self.size, dtype=np.int64)
+        _check(lib().lsg_grid_device_order(self._h, _ptr(out)))
+        return out
+
+    def stats(self):
+        out = np.zeros(10, dtype=np.int64)
+        _check(lib().lsg_grid_stats(self._h, _ptr(out), 10))
+        keys = ["depths", "multilevel_size", "apply_items", "wavefront_levels", "stream_values", "max_box",
+                "case_buffer", "w_buffer", "part_buffer", "moved_values"]
+        return dict(zip(keys, (int(v) for v in out)))
+
+
+def _coef(kind, a0=1.0, c0=0.0, xi=None):
+    c = _Coef()
+    c.kind, c.a0, c.c0 = kind, a0, c0
+    if xi is not None and len(xi):
+        arr = (ctypes.c_double * len(xi))(*[float(v) for v in xi])
+        c.nxi, c.xi = len(xi), ctypes.cast(arr, ctypes.POINTER(ctypes.c_double))
+        c._keep = arr
+    else:
+        c.nxi, c.xi = 0, None
+    return c
+
+
+class Stencil:
+    """Operator plug-in: -div(a grad u) + c u with the coefficient family `kind`."""
+
+    def __init__(self, kind, a0=1.0, c0=0.0, xi=None):
+        self.kind, self.a0, self.c0, self.xi = kind, a0, c0, (None if xi is None else list(xi))
+
+    def _c(self):
+        return _coef(self.kind, self.a0, self.c0, self.xi)
+
+
+def Poisson():
+    """Poisson (Stencil.h:1605-1793)."""
+    return Stencil(COEF_CONST, 1.0, 0.0)
+
+
+def HelmHoltz():
+    """HelmHoltz = Q1 mass (Stencil.h:873-988)."""
+    return Stencil(COEF_CONST, 0.0, 1.0)
+
+
+def VariableCoefficient(kind, xi=None):
+    return Stencil(kind, xi=xi)
+
+
+class MatrixVectorHomogen:
+    """Matrix-free prewavelet operator (MatrixVectorHomogen.h:32-92) on one grid.
+
+    precision 64 or 32; batch instances share the grid (coefficients per instance
+    via set_coefficient)."""
+
+    def __init__(self, grid, stencil, precision=64, batch=1):
+        self.grid, self.precision, self.batch = grid, precision, batch
+        self._h = ctypes.c_void_p()
+        c = stencil._c()
+        _check(lib().lsg_op_create(grid._h, ctypes.byref(c), precision, batch, ctypes.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and self._h.value and _lib is not None:
+            _lib.lsg_op_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def set_coefficient(self, instance, stencil):
+        c = stencil._c()
+        _check(lib().lsg_op_set_coef(self._h, instance, ctypes.byref(c)))
+
+    def set_block(self, threads):
+        _check(lib().lsg_op_set_block(self._h, 0, threads))
+
+    def multiplication(self, prew, Ax=None):
+        """Ax = A prew.  numpy float64 (batch x N or N, slot order) or torch device tensors."""
+        if hasattr(prew, "data_ptr"):
+            _sync_torch(prew, Ax)
+            _check(lib().lsg_matvec(self._h, _ptr(prew), _ptr(Ax), LSG_DEVICE))
+            return Ax
+        x = np.ascontiguousarray(prew, dtype=np.float64)
+        y = np.zeros_like(x) if Ax is None else Ax
+        _check(lib().lsg_matvec(self._h, _ptr(x), _ptr(y), LSG_HOST_SLOT))
+        return y
+
+    def diagonal(self, out=None):
+        """Preconditioning<S>::precond3 diagonal (cg_method.h:322-451)."""
+        if out is not None and hasattr(out, "data_ptr"):
+            _check(lib().lsg_precond_diag(self._h, _ptr(out), LSG_DEVICE))
+            return out
+        z = np.zeros((self.batch, self.grid.size)) if self.batch > 1 else np.zeros(self.grid.size)
+        _check(lib().lsg_precond_diag(self._h, _ptr(z), LSG_HOST_SLOT))
+        return z
+
+    def rhs_sines(self, eta, m, out=None):
+        """b_p = int f psi_p for f = sum_j eta_j prod_i sin(m_j pi x_i); eta: batch x nterms."""
+        eta = np.ascontiguousarray(np.atleast_2d(eta), dtype=np.float64)
+        m = np.ascontiguousarray(m, dtype=np.int32)
+        if out is not None and hasattr(out, "data_ptr"):
+            _check(lib().lsg_rhs_sines(self._h, m.size, _ptr(eta), _ptr(m), _ptr(out), LSG_DEVICE))
+            return out
+        b = np.zeros((self.batch, self.grid.size)) if self.batch > 1 else np.zeros(self.grid.size)
+        _check(lib().lsg_rhs_sines(self._h, m.size, _ptr(eta), _ptr(m), _ptr(b), LSG_HOST_SLOT))
+        return b
+
+    def last_timing(self):
+        out = np.zeros(3)
+        _check(lib().lsg_last_timing(self._h, _ptr(out), 3))
+        return {"total_ms": out[0], "apply_ms": out[1], "apply_launches": int(out[2])}
+
+
+class CG:
+    """CG::solveHomogen (cg_method.h:759-886) on the device."""
+
+    @staticmethod
+    def solveHomogen(eps, x, f, matrix, stop_rule=STOP_REF_ABS, maxit=500):
+        info = (SolveInfo * matrix.batch)()
+        if hasattr(x, "data_ptr"):
+            _sync_torch(x, f)
+            _check(lib().lsg_cg_solve(matrix._h, _ptr(f), _ptr(x), eps, stop_rule, maxit, info, LSG_DEVICE))
+        else:
+            xa = np.ascontiguousarray(x, dtype=np.float64)
+            fa = np.ascontiguousarray(f, dtype=np.float64)
+            _check(lib().lsg_cg_solve(matrix._h, _ptr(fa), _ptr(xa), eps, stop_rule, maxit, info, LSG_HOST_SLOT))
+            x[...] = xa
+        res = [info[b].as_dict() for b in range(matrix.batch)]
+        return res[0] if matrix.batch == 1 else res
+
+    @staticmethod
+    def solveMixed(rel_tol, x, f, op64, op32, inner_rel_tol=1e-4, maxit=30):
+        """FP32 inner PCG under FP64 iterative refinement (all reductions FP64)."""
+        info = (SolveInfo * op64.batch)()
+        if hasattr(x, "data_ptr"):
+            _sync_torch(x, f)
+            _check(lib().lsg_ir_solve(op64._h, op32._h, _ptr(f), _ptr(x), rel_tol, inner_rel_tol, maxit, info,
+                                      LSG_DEVICE))
+        else:
+            xa = np.ascontiguousarray(x, dtype=np.float64)
+            fa = np.ascontiguousarray(f, dtype=np.float64)
+            _check(lib().lsg_ir_solve(op64._h, op32._h, _ptr(fa), _ptr(xa), rel_tol, inner_rel_tol, maxit, info,
+                                      LSG_HOST_SLOT))
+            x[...] = xa
+        res = [info[b].as_dict() for b in range(op64.batch)]
+        return res[0] if op64.batch == 1 else res
+
+
+def device_available():
+    return bool(lib().lsg_device_available())

@@ -1,0 +1,679 @@
+// lsg_api.hip -- C ABI (include/loadsg_amd.h): device memory, the stage sequence
+// of one matvec, and the CG / iterative-refinement drivers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/loadsg_amd.h"
+#include "lsg_kernels.hpp"
+
+using namespace lsg;
+
+namespace {
+thread_local std::string g_err;
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+}  // namespace
+
+#define HIPCHK(x)                                                                                  \
+    do {                                                                                           \
+        hipError_t e_ = (x);                                                                       \
+        if (e_ != hipSuccess) return fail(LSG_ERR_DEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct lsg_grid {
+    Grid g;
+};
+
+namespace {
+
+struct DevMem {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DevMem() = default;
+    DevMem(const DevMem&) = delete;
+    DevMem& operator=(const DevMem&) = delete;
+    ~DevMem() { if (p) (void)hipFree(p); }
+    hipError_t alloc(size_t b) {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        bytes = std::max<size_t>(b, 16);
+        return hipMalloc(&p, bytes);
+    }
+    template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Separable representation of the coefficient families: a = sum_m prod_j g_mj(x_j),
+// c = sum_m prod_j h_mj(x_j), sampled at the 1D cell centres (2c+1)/2^(t+1) of every
+// level t.  Table of one (term, dim): level t occupies [2^t - 2, 2^(t+1) - 2).
+int coefTerms(const lsg_coef& c, int d, int& nA, int& nC) {
+    switch (c.kind) {
+        case LSG_COEF_CONST: nA = c.a0 != 0.0; nC = c.c0 != 0.0; break;
+        case LSG_COEF_CFG2: nA = 2; nC = 2; break;
+        case LSG_COEF_CFG3: nA = 1 + d; nC = 0; break;
+        case LSG_COEF_EXPSIN: nA = 1; nC = 0; if (c.nxi < 0 || (c.nxi > 0 && !c.xi)) return -1; break;
+        default: return -1;
+    }
+    return 0;
+}
+
+void coefTables(const lsg_coef& c, int d, int maxLevel, int64_t tabLen, std::vector<double>& tab) {
+    int nA, nC;
+    coefTerms(c, d, nA, nC);
+    const int nt = nA + nC;
+    tab.assign((size_t)nt * d * tabLen, 1.0);
+    auto fill = [&](int term, int dim, auto fn) {
+        double* t0 = &tab[((size_t)term * d + dim) * tabLen];
+        for (int t = 1; t <= maxLevel; ++t)
+            for (int cc = 0; cc < (1 << t); ++cc) {
+                double x = (2.0 * cc + 1.0) / double(1 << (t + 1));
+                t0[(1 << t) - 2 + cc] = fn(x);
+            }
+    };
+    switch (c.kind) {
+        case LSG_COEF_CONST: {
+            int term = 0;
+            if (c.a0 != 0.0) { fill(term, 0, [&](double) { return c.a0; }); ++term; }
+            if (c.c0 != 0.0) { fill(term, 0, [&](double) { return c.c0; }); ++term; }
+            break;
+        }
+        case LSG_COEF_CFG2:   // a = 1 + 0.5 prod cos(pi x_i); c = 1 + x_1
+            for (int i = 0; i < d; ++i)
+                fill(1, i, [&](double x) { return (i == 0 ? 0.5 : 1.0) * std::cos(M_PI * x); });
+            fill(3, 0, [&](double x) { return x; });
+            break;
+        case LSG_COEF_CFG3:   // a = 1 + sum x_i^2
+            for (int i = 0; i < d; ++i) fill(1 + i, i, [&](double x) { return x * x; });
+            break;
+        case LSG_COEF_EXPSIN:  // a = prod_j exp(sum_{k mod d = j} xi_k sin(k pi x_j) / k)
+            for (int j = 0; j < d; ++j)
+                fill(0, j, [&](double x) {
+                    double s = 0.0;
+                    for (int k = 1; k <= c.nxi; ++k)
+                        if (k % d == j) s += c.xi[k - 1] * std::sin(double(k) * M_PI * x) / double(k);
+                    return std::exp(s);
+                });
+            break;
+    }
+}
+
+}  // namespace
+
+struct lsg_op {
+    lsg_grid* grid = nullptr;
+    int prec = 64;
+    size_t vs = 8;
+    int batch = 1;
+    int kind = 0;
+    int nA = 0, nC = 0;
+    int64_t tabLen = 0;
+    bool general = false;
+    int64_t maxW = 0;
+    size_t applyLds = 0;
+    int applyThreads = 0;          // 0 = autotune on first use
+    DevMem dNodalItems, dSumItems, dSumList;
+    std::vector<std::unique_ptr<DevMem>> dProl, dApply;
+    DevMem coefTab, nodal, cbuf, wbuf, part, xs, ys;
+    // solver buffers
+    DevMem r, h, dv, g, D, f, x, tmp, partials, st, aux;
+    bool solverReady = false, diagValid = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[4] = {};
+    bool timed = false;
+    int64_t n() const { return grid->g.n; }
+};
+
+namespace {
+
+template <typename V>
+int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
+    const Grid& g = op->grid->g;
+    const Plan& P = g.plan;
+    const int d = g.d;
+    const int B = op->batch;
+    hipStream_t s = op->stream;
+    HIPCHK(hipEventRecord(op->ev[0], s));
+    launch_nodal<V>(op->dNodalItems.as<NodalItem>(), (int)P.nodal.size(), d, x, g.n, op->nodal.as<V>(), g.nml,
+                    P.maxBox, B, st, s);
+    for (int dd = 0; dd < d; ++dd)
+        launch_prol<V>(op->dProl[dd]->as<ProlItem>(), (int)P.prol[dd].size(), d, op->nodal.as<V>(), g.nml,
+                       op->cbuf.as<V>(), P.caseSize, P.maxFamily, B, st, s);
+    HIPCHK(hipEventRecord(op->ev[1], s));
+    ApplyArgs<V> a{};
+    a.d = d;
+    a.general = op->general ? 1 : 0;
+    a.nA = op->nA;
+    a.nC = op->nC;
+    a.coefTab = op->coefTab.as<V>();
+    a.coefStride = (int64_t)(op->nA + op->nC) * d * op->tabLen;
+    a.tabLen = op->tabLen;
+    a.nodal = op->nodal.as<V>();
+    a.nStride = g.nml;
+    a.cbuf = op->cbuf.as<V>();
+    a.cStride = P.caseSize;
+    a.wbuf = op->wbuf.as<V>();
+    a.wStride = P.wSize;
+    a.part = op->part.as<V>();
+    a.pStride = P.partSize;
+    a.maxBox = P.maxBox;
+    a.st = st;
+    for (size_t L = 0; L < P.apply.size(); ++L) {
+        a.items = op->dApply[L]->as<ApplyItem>();
+        if (launch_apply<V>(a, (int)P.apply[L].size(), B, op->applyThreads, op->applyLds, s) != 0)
+            return fail(LSG_ERR_UNSUPPORTED, "depth box too large for one workgroup");
+    }
+    HIPCHK(hipEventRecord(op->ev[2], s));
+    launch_sum<V>(op->dSumItems.as<SumItem>(), (int)P.sum.size(), op->dSumList.as<int64_t>(), d, op->part.as<V>(),
+                  P.partSize, y, g.n, B, st, s);
+    HIPCHK(hipEventRecord(op->ev[3], s));
+    HIPCHK(hipGetLastError());
+    op->timed = true;
+    return LSG_OK;
+}
+
+int matvecDev(lsg_op* op, const void* x, void* y, const CGState* st) {
+    if (op->prec == 64) return runMatvec<double>(op, (const double*)x, (double*)y, st);
+    return runMatvec<float>(op, (const float*)x, (float*)y, st);
+}
+
+int chooseThreads(lsg_op* op, int threads) {
+    const int64_t mb = op->grid->g.plan.maxBox;
+    int cands[4] = {64, 128, 256, 512};
+    if (threads) {
+        if ((mb + threads - 1) / threads > 32) return fail(LSG_ERR_UNSUPPORTED, "block too small for the largest box");
+        op->applyThreads = threads;
+        return LSG_OK;
+    }
+    // autotune: time the apply stage for each admissible block size (results are
+    // identical for every block size: no reduction crosses threads in this stage)
+    float best = 1e30f;
+    int bestT = 0;
+    for (int t : cands) {
+        if ((mb + t - 1) / t > 32) continue;
+        op->applyThreads = t;
+        float tot = 0.f;
+        for (int rep = 0; rep < 3; ++rep) {
+            int rc = matvecDev(op, op->xs.p, op->ys.p, nullptr);
+            if (rc) return rc;
+            HIPCHK(hipEventSynchronize(op->ev[3]));
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, op->ev[1], op->ev[2]));
+            if (rep > 0) tot += ms;
+        }
+        if (tot < best) { best = tot; bestT = t; }
+    }
+    if (!bestT) return fail(LSG_ERR_UNSUPPORTED, "largest depth box exceeds 16384 points");
+    op->applyThreads = bestT;
+    return LSG_OK;
+}
+
+int ensureThreads(lsg_op* op) {
+    if (op->applyThreads) return LSG_OK;
+    return chooseThreads(op, 0);
+}
+
+// host slot-order <-> device order (instance-major batch)
+template <typename V>
+int uploadSlot(lsg_op* op, const double* h, V* dptr) {
+    const Grid& g = op->grid->g;
+    std::vector<V> buf((size_t)g.n * op->batch);
+    for (int b = 0; b < op->batch; ++b)
+        for (int64_t s = 0; s < g.n; ++s) buf[(size_t)b * g.n + g.devOfSlot[s]] = V(h[(size_t)b * g.n + s]);
+    HIPCHK(hipMemcpyAsync(dptr, buf.data(), buf.size() * sizeof(V), hipMemcpyHostToDevice, op->stream));
+    HIPCHK(hipStreamSynchronize(op->stream));
+    return LSG_OK;
+}
+template <typename V>
+int downloadSlot(lsg_op* op, const V* dptr, double* h) {
+    const Grid& g = op->grid->g;
+    std::vector<V> buf((size_t)g.n * op->batch);
+    HIPCHK(hipMemcpyAsync(buf.data(), dptr, buf.size() * sizeof(V), hipMemcpyDeviceToHost, op->stream));
+    HIPCHK(hipStreamSynchronize(op->stream));
+    for (int b = 0; b < op->batch; ++b)
+        for (int64_t s = 0; s < g.n; ++s) h[(size_t)b * g.n + s] = double(buf[(size_t)b * g.n + g.devOfSlot[s]]);
+    return LSG_OK;
+}
+int upload(lsg_op* op, const double* h, void* d) {
+    return op->prec == 64 ? uploadSlot<double>(op, h, (double*)d) : uploadSlot<float>(op, h, (float*)d);
+}
+int download(lsg_op* op, const void* d, double* h) {
+    return op->prec == 64 ? downloadSlot<double>(op, (const double*)d, h) : downloadSlot<float>(op, (const float*)d, h);
+}
+
+int ensureSolver(lsg_op* op) {
+    if (op->solverReady) return LSG_OK;
+    const size_t nb = (size_t)op->n() * op->batch * op->vs;
+    HIPCHK(op->r.alloc(nb));
+    HIPCHK(op->h.alloc(nb));
+    HIPCHK(op->dv.alloc(nb));
+    HIPCHK(op->g.alloc(nb));
+    HIPCHK(op->D.alloc(nb));
+    HIPCHK(op->f.alloc((size_t)op->n() * op->batch * 8));
+    HIPCHK(op->x.alloc((size_t)op->n() * op->batch * 8));
+    HIPCHK(op->tmp.alloc((size_t)op->n() * op->batch * 8));
+    const int nbk = cg_partial_blocks(op->n());
+    HIPCHK(op->partials.alloc((size_t)op->batch * 3 * nbk * sizeof(double)));
+    HIPCHK(op->st.alloc((size_t)op->batch * sizeof(CGState)));
+    op->solverReady = true;
+    return LSG_OK;
+}
+
+int computeDiag(lsg_op* op) {
+    if (op->diagValid) return LSG_OK;
+    const Grid& g = op->grid->g;
+    int64_t cs = (int64_t)(op->nA + op->nC) * g.d * op->tabLen;
+    if (op->prec == 64)
+        launch_diag<double>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, op->nA, op->nC,
+                            op->coefTab.as<double>(), cs, op->tabLen, op->D.as<double>(), g.n, op->batch, op->stream);
+    else
+        launch_diag<float>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, op->nA, op->nC,
+                           op->coefTab.as<float>(), cs, op->tabLen, op->D.as<float>(), g.n, op->batch, op->stream);
+    HIPCHK(hipGetLastError());
+    op->diagValid = true;
+    return LSG_OK;
+}
+
+// PCG on device vectors: f (V), x (V, or double when x64).  Mirrors
+// CG::solveHomogen (applications/cg_method.h:759-886).
+template <typename V>
+int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, int maxit,
+             std::vector<CGState>& out) {
+    const int64_t n = op->n();
+    const int B = op->batch;
+    hipStream_t s = op->stream;
+    int rc = ensureSolver(op);
+    if (rc) return rc;
+    if ((rc = computeDiag(op))) return rc;
+    if ((rc = ensureThreads(op))) return rc;
+    V* r = op->r.as<V>();
+    V* h = op->h.as<V>();
+    V* dv = op->dv.as<V>();
+    V* g = op->g.as<V>();
+    V* D = op->D.as<V>();
+    const int nb = cg_partial_blocks(n);
+    double* part = op->partials.as<double>();
+    CGState* st = op->st.as<CGState>();
+    // r = A x (x in V for the matvec)
+    const V* xv;
+    if (x64 && sizeof(V) != 8) {
+        launch_cast_d2f((const double*)x, (float*)g, n * B, s);
+        xv = g;
+    } else xv = (const V*)x;
+    if ((rc = runMatvec<V>(op, xv, r, nullptr))) return rc;
+    launch_cg_init<V>(f, r, h, dv, D, n, B, part, nb, s);
+    launch_cg_start(st, part, nb, eps, rule, maxit, B, s);
+    HIPCHK(hipGetLastError());
+    out.resize(B);
+    const int chunk = 8;
+    for (int done = 0; done < maxit;) {
+        int k = std::min(chunk, maxit - done);
+        for (int q = 0; q < k; ++q) {
+            if ((rc = runMatvec<V>(op, dv, g, st))) return rc;
+            launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s);
+        }
+        done += k;
+        HIPCHK(hipMemcpyAsync(out.data(), st, sizeof(CGState) * B, hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        bool any = false;
+        for (auto& c : out) any |= c.active != 0;
+        if (!any) break;
+    }
+    HIPCHK(hipMemcpyAsync(out.data(), st, sizeof(CGState) * B, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return LSG_OK;
+}
+
+// true relative residual ||f - A x|| / ||f|| in FP64 (x, f double, device)
+int trueResidual(lsg_op* op64, const double* f, const double* x, double* tmpAx, double* r, std::vector<double>& rel) {
+    const int64_t n = op64->n();
+    const int B = op64->batch;
+    int rc = runMatvec<double>(op64, x, tmpAx, nullptr);
+    if (rc) return rc;
+    const int nb = cg_partial_blocks(n);
+    launch_residual(f, tmpAx, r, n, B, op64->partials.as<double>(), nb, op64->stream);
+    std::vector<double> hp((size_t)B * 2 * nb);
+    HIPCHK(hipMemcpyAsync(hp.data(), op64->partials.p, hp.size() * 8, hipMemcpyDeviceToHost, op64->stream));
+    HIPCHK(hipStreamSynchronize(op64->stream));
+    rel.resize(B);
+    for (int b = 0; b < B; ++b) {
+        double rr = 0, ff = 0;
+        for (int k = 0; k < nb; ++k) rr += hp[((size_t)b * 2 + 0) * nb + k];
+        for (int k = 0; k < nb; ++k) ff += hp[((size_t)b * 2 + 1) * nb + k];
+        rel[b] = ff > 0 ? std::sqrt(rr / ff) : std::sqrt(rr);
+    }
+    return LSG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* lsg_last_error(void) { return g_err.c_str(); }
+
+int lsg_device_available(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n > 0;
+}
+
+int lsg_grid_create(int d, int64_t n, const int32_t* idx, const uint8_t* active, lsg_grid** out) {
+    if (!out) return fail(LSG_ERR_ARG, "out is null");
+    auto* g = new lsg_grid();
+    std::string err;
+    int rc = buildGrid(d, n, idx, active, g->g, err);
+    if (rc) { delete g; return fail(rc, err); }
+    *out = g;
+    return LSG_OK;
+}
+
+int lsg_grid_regular(int d, int level, lsg_grid** out) {
+    if (d < 1 || d > MAXD || level < 0 || level > 14) return fail(LSG_ERR_ARG, "bad dimension or level");
+    std::vector<int32_t> idx;
+    regularIndexTable(d, level, idx);
+    return lsg_grid_create(d, (int64_t)(idx.size() / d), idx.data(), nullptr, out);
+}
+
+int64_t lsg_grid_size(const lsg_grid* g) { return g ? g->g.n : -1; }
+int lsg_grid_dim(const lsg_grid* g) { return g ? g->g.d : -1; }
+
+int lsg_grid_slot_order(const lsg_grid* g, int32_t* idx_out, uint8_t* active_out) {
+    if (!g) return fail(LSG_ERR_ARG, "null grid");
+    if (idx_out) std::memcpy(idx_out, g->g.idx.data(), g->g.idx.size() * sizeof(int32_t));
+    if (active_out) std::memcpy(active_out, g->g.active.data(), g->g.active.size());
+    return LSG_OK;
+}
+
+int lsg_grid_device_order(const lsg_grid* g, int64_t* dev_of_slot) {
+    if (!g || !dev_of_slot) return fail(LSG_ERR_ARG, "null argument");
+    std::memcpy(dev_of_slot, g->g.devOfSlot.data(), g->g.devOfSlot.size() * sizeof(int64_t));
+    return LSG_OK;
+}
+
+int lsg_grid_stats(const lsg_grid* g, int64_t* out, int nout) {
+    if (!g || !out) return fail(LSG_ERR_ARG, "null argument");
+    const Plan& P = g->g.plan;
+    int64_t v[10] = {(int64_t)g->g.depths.size(), g->g.nml, P.nApply, (int64_t)P.apply.size(),
+                     (int64_t)P.streamValues, P.maxBox, P.caseSize, P.wSize, P.partSize, (int64_t)P.movedValues};
+    for (int i = 0; i < nout && i < 10; ++i) out[i] = v[i];
+    return LSG_OK;
+}
+
+int lsg_grid_destroy(lsg_grid* g) {
+    delete g;
+    return LSG_OK;
+}
+
+int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch, lsg_op** out) {
+    if (!grid || !coef || !out || batch < 1) return fail(LSG_ERR_ARG, "bad arguments");
+    if (precision != 64 && precision != 32) return fail(LSG_ERR_ARG, "precision must be 64 or 32");
+    std::unique_ptr<lsg_op> op(new lsg_op());
+    op->grid = grid;
+    op->prec = precision;
+    op->vs = precision == 64 ? 8 : 4;
+    op->batch = batch;
+    op->kind = coef->kind;
+    const Grid& g = grid->g;
+    const Plan& P = g.plan;
+    if (coefTerms(*coef, g.d, op->nA, op->nC)) return fail(LSG_ERR_ARG, "bad coefficient");
+    if (op->nA + op->nC == 0) return fail(LSG_ERR_ARG, "zero operator");
+    op->general = !(op->nA == 1 && op->nC == 0);
+    op->tabLen = (int64_t(1) << (g.maxLevel + 1)) - 2;
+    int64_t maxW = 0;
+    for (auto& lev : P.apply)
+        for (auto& a : lev) {
+            int64_t w = 0;
+            for (int i = 0; i < g.d; ++i) w += int64_t(1) << a.T[i];
+            maxW = std::max(maxW, w);
+        }
+    op->maxW = maxW;
+    op->applyLds = ((op->general ? 4 : 2) * P.maxBox + maxW) * op->vs;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    int maxLds = 0;
+    HIPCHK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    if ((int64_t)op->applyLds > maxLds || (int64_t)(2 * P.maxBox * op->vs) > maxLds)
+        return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS (" + std::to_string(op->applyLds) +
+                                             " > " + std::to_string(maxLds) + " bytes)");
+    HIPCHK(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    for (auto& e : op->ev) HIPCHK(hipEventCreate(&e));
+    // items
+    HIPCHK(op->dNodalItems.alloc(P.nodal.size() * sizeof(NodalItem)));
+    HIPCHK(hipMemcpy(op->dNodalItems.p, P.nodal.data(), P.nodal.size() * sizeof(NodalItem), hipMemcpyHostToDevice));
+    for (auto& v : P.prol) {
+        op->dProl.emplace_back(new DevMem());
+        HIPCHK(op->dProl.back()->alloc(v.size() * sizeof(ProlItem)));
+        if (!v.empty()) HIPCHK(hipMemcpy(op->dProl.back()->p, v.data(), v.size() * sizeof(ProlItem), hipMemcpyHostToDevice));
+    }
+    for (auto& v : P.apply) {
+        op->dApply.emplace_back(new DevMem());
+        HIPCHK(op->dApply.back()->alloc(v.size() * sizeof(ApplyItem)));
+        HIPCHK(hipMemcpy(op->dApply.back()->p, v.data(), v.size() * sizeof(ApplyItem), hipMemcpyHostToDevice));
+    }
+    HIPCHK(op->dSumItems.alloc(P.sum.size() * sizeof(SumItem)));
+    HIPCHK(hipMemcpy(op->dSumItems.p, P.sum.data(), P.sum.size() * sizeof(SumItem), hipMemcpyHostToDevice));
+    HIPCHK(op->dSumList.alloc(P.sumList.size() * sizeof(int64_t)));
+    if (!P.sumList.empty())
+        HIPCHK(hipMemcpy(op->dSumList.p, P.sumList.data(), P.sumList.size() * sizeof(int64_t), hipMemcpyHostToDevice));
+    // buffers
+    const size_t vs = op->vs;
+    HIPCHK(op->nodal.alloc((size_t)g.nml * batch * vs));
+    HIPCHK(op->cbuf.alloc((size_t)P.caseSize * batch * vs));
+    HIPCHK(op->wbuf.alloc((size_t)P.wSize * batch * vs));
+    HIPCHK(op->part.alloc((size_t)P.partSize * batch * vs));
+    HIPCHK(op->xs.alloc((size_t)g.n * batch * vs));
+    HIPCHK(op->ys.alloc((size_t)g.n * batch * vs));
+    HIPCHK(hipMemset(op->xs.p, 0, op->xs.bytes));
+    HIPCHK(op->coefTab.alloc((size_t)(op->nA + op->nC) * g.d * op->tabLen * batch * vs));
+    *out = op.release();
+    for (int b = 0; b < batch; ++b) {
+        int rc = lsg_op_set_coef(*out, b, coef);
+        if (rc) { lsg_op_destroy(*out); *out = nullptr; return rc; }
+    }
+    return LSG_OK;
+}
+
+int lsg_op_set_coef(lsg_op* op, int instance, const lsg_coef* coef) {
+    if (!op || !coef || instance < 0 || instance >= op->batch) return fail(LSG_ERR_ARG, "bad arguments");
+    int nA, nC;
+    const Grid& g = op->grid->g;
+    if (coefTerms(*coef, g.d, nA, nC) || nA != op->nA || nC != op->nC)
+        return fail(LSG_ERR_ARG, "coefficient family differs from the operator's");
+    std::vector<double> tab;
+    coefTables(*coef, g.d, g.maxLevel, op->tabLen, tab);
+    const size_t per = tab.size();
+    if (op->prec == 64) {
+        HIPCHK(hipMemcpy(op->coefTab.as<double>() + per * instance, tab.data(), per * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tf(tab.begin(), tab.end());
+        HIPCHK(hipMemcpy(op->coefTab.as<float>() + per * instance, tf.data(), per * 4, hipMemcpyHostToDevice));
+    }
+    op->diagValid = false;
+    return LSG_OK;
+}
+
+int lsg_op_batch(const lsg_op* op) { return op ? op->batch : -1; }
+
+int lsg_op_set_block(lsg_op* op, int stage, int threads) {
+    if (!op || stage != 0) return fail(LSG_ERR_ARG, "only stage 0 (apply) is tunable");
+    if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512)
+        return fail(LSG_ERR_ARG, "threads must be 64, 128, 256 or 512");
+    return chooseThreads(op, threads);
+}
+
+int lsg_matvec(lsg_op* op, const void* x, void* y, int layout) {
+    if (!op || !x || !y) return fail(LSG_ERR_ARG, "null argument");
+    int rc = ensureThreads(op);
+    if (rc) return rc;
+    if (layout == LSG_DEVICE) {
+        rc = matvecDev(op, x, y, nullptr);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(op->stream));
+        return LSG_OK;
+    }
+    if ((rc = upload(op, (const double*)x, op->xs.p))) return rc;
+    if ((rc = matvecDev(op, op->xs.p, op->ys.p, nullptr))) return rc;
+    return download(op, op->ys.p, (double*)y);
+}
+
+int lsg_precond_diag(lsg_op* op, void* z, int layout) {
+    if (!op || !z) return fail(LSG_ERR_ARG, "null argument");
+    int rc = ensureSolver(op);
+    if (rc) return rc;
+    if ((rc = computeDiag(op))) return rc;
+    if (layout == LSG_DEVICE) {
+        HIPCHK(hipMemcpyAsync(z, op->D.p, (size_t)op->n() * op->batch * op->vs, hipMemcpyDeviceToDevice, op->stream));
+        HIPCHK(hipStreamSynchronize(op->stream));
+        return LSG_OK;
+    }
+    return download(op, op->D.p, (double*)z);
+}
+
+int lsg_cg_solve(lsg_op* op, const void* f, void* x, double eps, int stop_rule, int maxit, lsg_solve_info* info,
+                 int layout) {
+    if (!op || !f || !x || maxit < 0) return fail(LSG_ERR_ARG, "bad arguments");
+    int rc = ensureSolver(op);
+    if (rc) return rc;
+    const int64_t tot = op->n() * op->batch;
+    const void* fD = f;
+    void* xD = x;
+    if (layout == LSG_HOST_SLOT) {
+        // f in op->f (V), x in op->x (V)
+        if ((rc = upload(op, (const double*)f, op->f.p))) return rc;
+        if ((rc = upload(op, (const double*)x, op->x.p))) return rc;
+        fD = op->f.p;
+        xD = op->x.p;
+    }
+    std::vector<CGState> st;
+    if (op->prec == 64) rc = cgDevice<double>(op, (const double*)fD, xD, true, eps, stop_rule, maxit, st);
+    else rc = cgDevice<float>(op, (const float*)fD, xD, false, eps, stop_rule, maxit, st);
+    if (rc) return rc;
+    (void)tot;
+    if (layout == LSG_HOST_SLOT && (rc = download(op, op->x.p, (double*)x))) return rc;
+    bool nan = false;
+    for (int b = 0; b < op->batch; ++b) {
+        if (info) {
+            info[b].iterations = st[b].it;
+            info[b].outer = 0;
+            info[b].final_delta = st[b].delta;
+            info[b].rel_residual = st[b].ff > 0 ? std::sqrt(st[b].rr / st[b].ff) : std::sqrt(st[b].rr);
+            info[b].converged = st[b].converged;
+        }
+        nan |= st[b].converged < 0;
+    }
+    if (nan) return fail(LSG_ERR_NAN, "NaN in CG");
+    return LSG_OK;
+}
+
+int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_tol, double inner_rel_tol, int maxit,
+                 lsg_solve_info* info, int layout) {
+    if (!op64 || !op32 || !f || !x) return fail(LSG_ERR_ARG, "null argument");
+    if (op64->prec != 64 || op32->prec != 32 || op64->grid != op32->grid || op64->batch != op32->batch)
+        return fail(LSG_ERR_ARG, "need an FP64 and an FP32 operator on the same grid and batch");
+    int rc;
+    if ((rc = ensureSolver(op64)) || (rc = ensureSolver(op32)) || (rc = ensureThreads(op64)) ||
+        (rc = ensureThreads(op32)))
+        return rc;
+    const int64_t n = op64->n();
+    const int B = op64->batch;
+    const int64_t tot = n * B;
+    double* fD;
+    double* xD;
+    if (layout == LSG_HOST_SLOT) {
+        if ((rc = upload(op64, (const double*)f, op64->f.p)) || (rc = upload(op64, (const double*)x, op64->x.p)))
+            return rc;
+        fD = op64->f.as<double>();
+        xD = op64->x.as<double>();
+    } else {
+        fD = (double*)f;
+        xD = (double*)x;
+    }
+    hipStream_t s = op64->stream;
+    double* Ax = op64->tmp.as<double>();
+    double* r64 = op64->r.as<double>();
+    float* r32 = op32->f.as<float>();       // inner right-hand side
+    float* e32 = op32->x.as<float>();       // inner solution
+    std::vector<int> iters(B, 0), outer(B, 0), conv(B, 0);
+    std::vector<double> rel;
+    std::vector<CGState> st;
+    for (int sweep = 0; sweep <= maxit; ++sweep) {
+        if ((rc = trueResidual(op64, fD, xD, Ax, r64, rel))) return rc;
+        bool all = true;
+        for (int b = 0; b < B; ++b) {
+            if (rel[b] != rel[b]) return fail(LSG_ERR_NAN, "NaN residual");
+            if (rel[b] <= rel_tol) {
+                conv[b] = 1;
+                HIPCHK(hipMemsetAsync(r64 + (size_t)b * n, 0, n * 8, s));
+            } else all = false;
+        }
+        if (all || sweep == maxit) break;
+        launch_cast_d2f(r64, r32, tot, s);
+        HIPCHK(hipMemsetAsync(e32, 0, tot * 4, s));
+        HIPCHK(hipStreamSynchronize(s));
+        if ((rc = cgDevice<float>(op32, r32, e32, false, inner_rel_tol, LSG_STOP_REL, 1000, st))) return rc;
+        launch_axpy_correction(xD, e32, tot, op32->stream);
+        HIPCHK(hipStreamSynchronize(op32->stream));
+        for (int b = 0; b < B; ++b)
+            if (!conv[b]) { iters[b] += st[b].it; outer[b] += 1; }
+    }
+    if (layout == LSG_HOST_SLOT && (rc = download(op64, op64->x.p, (double*)x))) return rc;
+    for (int b = 0; b < B; ++b)
+        if (info) {
+            info[b].iterations = iters[b];
+            info[b].outer = outer[b];
+            info[b].final_delta = 0.0;
+            info[b].rel_residual = rel[b];
+            info[b].converged = conv[b];
+        }
+    return LSG_OK;
+}
+
+int lsg_rhs_sines(lsg_op* op, int nterms, const double* eta, const int32_t* m, void* b, int layout) {
+    if (!op || nterms < 1 || !eta || !m || !b) return fail(LSG_ERR_ARG, "bad arguments");
+    const Grid& g = op->grid->g;
+    DevMem dEta, dM;
+    HIPCHK(dEta.alloc((size_t)nterms * op->batch * 8));
+    HIPCHK(dM.alloc((size_t)nterms * 4));
+    HIPCHK(hipMemcpy(dEta.p, eta, (size_t)nterms * op->batch * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dM.p, m, (size_t)nterms * 4, hipMemcpyHostToDevice));
+    void* dst = layout == LSG_DEVICE ? b : op->ys.p;
+    if (op->prec == 64)
+        launch_rhs_sines<double>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, nterms, dEta.as<double>(),
+                                 dM.as<int32_t>(), (double*)dst, g.n, op->batch, op->stream);
+    else
+        launch_rhs_sines<float>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, nterms, dEta.as<double>(),
+                                dM.as<int32_t>(), (float*)dst, g.n, op->batch, op->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(op->stream));
+    if (layout == LSG_DEVICE) return LSG_OK;
+    return download(op, op->ys.p, (double*)b);
+}
+
+int lsg_last_timing(const lsg_op* op, double* out, int nout) {
+    if (!op || !out || !op->timed) return fail(LSG_ERR_ARG, "no timed matvec");
+    HIPCHK(hipEventSynchronize(op->ev[3]));
+    float tot = 0.f, ap = 0.f;
+    HIPCHK(hipEventElapsedTime(&tot, op->ev[0], op->ev[3]));
+    HIPCHK(hipEventElapsedTime(&ap, op->ev[1], op->ev[2]));
+    double v[3] = {tot, ap, (double)op->grid->g.plan.apply.size()};
+    for (int i = 0; i < nout && i < 3; ++i) out[i] = v[i];
+    return LSG_OK;
+}
+
+int lsg_op_destroy(lsg_op* op) {
+    if (!op) return LSG_OK;
+    if (op->stream) (void)hipStreamSynchronize(op->stream);
+    for (auto& e : op->ev)
+        if (e) (void)hipEventDestroy(e);
+    hipStream_t s = op->stream;
+    delete op;
+    if (s) (void)hipStreamDestroy(s);
+    return LSG_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,821 @@
+// lsg_kernels.hip -- gfx950 kernels of the matrix-free prewavelet matvec and CG.
+//
+// One workgroup owns one work item (a depth box, a prolongation family, or a
+// (case, depth) apply item) of one batch instance and keeps the whole box in LDS;
+// every stage is a sequence of 1D sweeps along one dimension of the box.  Boxes at
+// the target shapes are small (d=5 level 7: at most 1323 points), so the global
+// traffic of a stage is its algorithmic input and output read/written once,
+// coalesced, and the sweeps run from LDS.  Reductions are deterministic
+// (fixed-order wave shuffles + fixed-order partial sums), so results are
+// bit-reproducible for a given launch shape.
+#include <hip/hip_runtime.h>
+
+#include "lsg_kernels.hpp"
+
+namespace lsg {
+
+// ---------------------------------------------------------------- box shapes
+struct Shape {
+    int d;
+    int n[MAXD];    // extent per dim
+    int S[MAXD];    // stride per dim (dim 0 fastest)
+    int total;
+    __device__ void box(const int32_t* T, int d_) {
+        d = d_;
+        int s = 1;
+        for (int i = 0; i < d; ++i) { n[i] = (1 << T[i]) - 1; S[i] = s; s *= n[i]; }
+        total = s;
+    }
+    __device__ void restage() {
+        int s = 1;
+        for (int i = 0; i < d; ++i) { S[i] = s; s *= n[i]; }
+        total = s;
+    }
+};
+
+template <typename V>
+__device__ inline V cCenter(int t, int k, int n) {      // 0.9 next to the boundary (index.cc:312-316)
+    return (t >= 2 && (k == 1 || k == n)) ? V(0.9) : V(1.0);
+}
+
+// -------------------------------------------------------------- calcNodal
+// nodal box of depth T = tensor product of the 1D prewavelet->nodal maps applied
+// to the depth-T block (calcNodal, MatrixVectorHomogen.cpp:43-74, with the 5^d
+// stencil of nextFiveP, indices/index.cc:252-327, applied dimension by dimension).
+template <typename V>
+__global__ void __launch_bounds__(256) nodal_kernel(const NodalItem* __restrict__ items, int d,
+                                                    const V* __restrict__ x, int64_t xStride,
+                                                    V* __restrict__ nodal, int64_t nStride, int64_t maxBox,
+                                                    const CGState* __restrict__ st) {
+    extern __shared__ char smem_raw[];
+    V* A = reinterpret_cast<V*>(smem_raw);
+    V* B = A + maxBox;
+    const int inst = blockIdx.y;
+    if (st && !st[inst].active) return;
+    const NodalItem& it = items[blockIdx.x];
+    const V* xin = x + inst * xStride + it.in_off;
+    V* out = nodal + inst * nStride + it.out_off;
+    // layout: dims < i expanded (2^t - 1), dims >= i still block (2^(t-1))
+    int ext[MAXD];
+    int cnt = 1;
+    for (int i = 0; i < d; ++i) { ext[i] = 1 << (it.T[i] - 1); cnt *= ext[i]; }
+    for (int p = threadIdx.x; p < cnt; p += blockDim.x) A[p] = xin[p];
+    __syncthreads();
+    V* src = A;
+    V* dst = B;
+    for (int i = 0; i < d; ++i) {
+        const int t = it.T[i];
+        const int m = ext[i];
+        const int nb = (1 << t) - 1;
+        int Slo = 1;
+        for (int q = 0; q < i; ++q) Slo *= ext[q];
+        const int outCnt = cnt / m * nb;
+        for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
+            int lo = p % Slo;
+            int rest = p / Slo;
+            int kq = rest % nb;
+            int hi = rest / nb;
+            int k = kq + 1;
+            const V* s = src + lo + Slo * (m * hi);
+            V v;
+            if (k & 1) {
+                int j = kq >> 1;
+                v = cCenter<V>(t, k, nb) * s[Slo * j];
+                if (j >= 1) v += V(0.1) * s[Slo * (j - 1)];
+                if (j + 1 < m) v += V(0.1) * s[Slo * (j + 1)];
+            } else {
+                int j = (k >> 1) - 1;
+                v = V(-0.6) * (s[Slo * j] + s[Slo * (j + 1)]);
+            }
+            dst[p] = v;
+        }
+        ext[i] = nb;
+        cnt = outCnt;
+        __syncthreads();
+        V* tmp = src; src = dst; dst = tmp;
+    }
+    for (int p = threadIdx.x; p < cnt; p += blockDim.x) out[p] = src[p];
+}
+
+// ---------------------------------------------------------- prolongation
+// P[T+e_dd] += I_dd P[T] along one family (prolongation1D_inplace +
+// addMultiLevelValues, MatrixVectorHomogen.h:444-474, 997-1019).
+template <typename V>
+__global__ void __launch_bounds__(256) prol_kernel(const ProlItem* __restrict__ items, int d,
+                                                   const V* __restrict__ nodal, int64_t nStride,
+                                                   V* __restrict__ cbuf, int64_t cStride, int64_t maxBox,
+                                                   const CGState* __restrict__ st) {
+    extern __shared__ char smem_raw[];
+    V* prev = reinterpret_cast<V*>(smem_raw);
+    V* cur = prev + maxBox;
+    const int inst = blockIdx.y;
+    if (st && !st[inst].active) return;
+    const ProlItem& it = items[blockIdx.x];
+    const int dd = it.dd;
+    const V* inBase = it.from_nodal ? nodal + inst * nStride : cbuf + inst * cStride;
+    V* outBase = cbuf + inst * cStride;
+    int32_t T[MAXD];
+    for (int i = 0; i < d; ++i) T[i] = it.T[i];
+    int Slo = 1, Shi = 1;
+    for (int i = 0; i < d; ++i) {
+        if (i < dd) Slo *= (1 << T[i]) - 1;
+        else if (i > dd) Shi *= (1 << T[i]) - 1;
+    }
+    for (int q = 0; q < it.nt; ++q) {
+        const int t = T[dd] + q;
+        const int nf = (1 << t) - 1;
+        const int nc = (1 << (t - 1)) - 1;
+        const int cnt = Slo * nf * Shi;
+        const V* in = inBase + it.in_off[q];
+        V* out = outBase + it.out_off[q];
+        for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
+            V v = in[p];
+            if (q > 0) {
+                int lo = p % Slo;
+                int rest = p / Slo;
+                int kq = rest % nf;
+                int hi = rest / nf;
+                int k = kq + 1;                     // fine position
+                const V* c = prev + lo + Slo * (nc * hi);
+                if (k & 1) {
+                    int cl = (k - 1) >> 1, cr = (k + 1) >> 1;   // coarse positions 0..nc+1
+                    V a = (cl >= 1) ? c[Slo * (cl - 1)] : V(0);
+                    V b = (cr <= nc) ? c[Slo * (cr - 1)] : V(0);
+                    v += V(0.5) * (a + b);
+                } else {
+                    v += c[Slo * ((k >> 1) - 1)];
+                }
+            }
+            cur[p] = v;
+            out[p] = v;
+        }
+        __syncthreads();
+        V* tmp = prev; prev = cur; cur = tmp;
+    }
+}
+
+// ------------------------------------------------------------------ apply
+// Weighted 1D Q1 operators at level t on one line (positions 1..n, cell c between
+// positions c and c+1 carries weight w[c]):
+//   (K v)_k = 2^t  [ w_{k-1}(v_k - v_{k-1}) + w_k (v_k - v_{k+1}) ]
+//   (M v)_k = 2^-t/6 [ w_{k-1}(v_{k-1} + 2 v_k) + w_k (2 v_k + v_{k+1}) ]
+// For a separable coefficient a = prod_j g_j(x_j) the element matrices of
+// Poisson::integration (Stencil.h:1707-1789) times a(cell centre) sum to
+// sum_i K_i (x) M_{j != i}, applied here by the Horner scheme
+//   Z <- M_i Z + K_i Y,  Y <- M_i Y   for i = d-1 .. 0.
+template <typename V, int PPT>
+__device__ inline void horner_step(V* __restrict__ Y, V* __restrict__ Z, const Shape& sh, int i,
+                                   const V* __restrict__ w, int t, bool zeroZ, bool needY) {
+    const int S = sh.S[i], n = sh.n[i];
+    const V ih = V(1 << t);
+    const V h6 = V(1.0) / (V(6.0) * V(1 << t));
+    V ny[PPT], nz[PPT];
+#pragma unroll
+    for (int s = 0; s < PPT; ++s) {
+        int p = threadIdx.x + s * blockDim.x;
+        if (p < sh.total) {
+            int kq = (p / S) % n;
+            V yc = Y[p];
+            V yl = kq > 0 ? Y[p - S] : V(0);
+            V yr = kq < n - 1 ? Y[p + S] : V(0);
+            V wl = w[kq], wr = w[kq + 1];
+            V My = h6 * (wl * (yl + V(2) * yc) + wr * (V(2) * yc + yr));
+            V Ky = ih * (wl * (yc - yl) + wr * (yc - yr));
+            V Mz = V(0);
+            if (!zeroZ) {
+                V zc = Z[p];
+                V zl = kq > 0 ? Z[p - S] : V(0);
+                V zr = kq < n - 1 ? Z[p + S] : V(0);
+                Mz = h6 * (wl * (zl + V(2) * zc) + wr * (V(2) * zc + zr));
+            }
+            ny[s] = My;
+            nz[s] = Mz + Ky;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < PPT; ++s) {
+        int p = threadIdx.x + s * blockDim.x;
+        if (p < sh.total) {
+            if (needY) Y[p] = ny[s];
+            Z[p] = nz[s];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename V, int PPT>
+__device__ inline void mass_step(V* __restrict__ Y, const Shape& sh, int i, const V* __restrict__ w, int t) {
+    const int S = sh.S[i], n = sh.n[i];
+    const V h6 = V(1.0) / (V(6.0) * V(1 << t));
+    V ny[PPT];
+#pragma unroll
+    for (int s = 0; s < PPT; ++s) {
+        int p = threadIdx.x + s * blockDim.x;
+        if (p < sh.total) {
+            int kq = (p / S) % n;
+            V yc = Y[p];
+            V yl = kq > 0 ? Y[p - S] : V(0);
+            V yr = kq < n - 1 ? Y[p + S] : V(0);
+            ny[s] = h6 * (w[kq] * (yl + V(2) * yc) + w[kq + 1] * (V(2) * yc + yr));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < PPT; ++s) {
+        int p = threadIdx.x + s * blockDim.x;
+        if (p < sh.total) Y[p] = ny[s];
+    }
+    __syncthreads();
+}
+
+// weights of one separable factor for the item's depth: dim i, level T_i,
+// 2^T_i cell-centre values, copied from the instance's 1D tables into LDS.
+template <typename V>
+__device__ inline void load_weights(V* __restrict__ W, int* woff, const V* __restrict__ tab, int64_t tabLen,
+                                    const int32_t* T, int d) {
+    int o = 0;
+    for (int i = 0; i < d; ++i) {
+        woff[i] = o;
+        o += 1 << T[i];
+    }
+    for (int i = 0; i < d; ++i) {
+        const V* src = tab + i * tabLen + ((1 << T[i]) - 2);
+        for (int c = threadIdx.x; c < (1 << T[i]); c += blockDim.x) W[woff[i] + c] = src[c];
+    }
+    __syncthreads();
+}
+
+// restriction1D_inverted_inplace along r (MatrixVectorHomogen.h:485-530): coarse
+// position j gets z(2j) + 0.5 (z(2j-1) + z(2j+1)); src box -> dst box.
+template <typename V>
+__device__ inline void restrict_dim(const V* __restrict__ src, V* __restrict__ dst, Shape& sh, int r) {
+    const int nf = sh.n[r];
+    const int nc = (nf - 1) >> 1;
+    const int Slo = sh.S[r];
+    const int outCnt = sh.total / nf * nc;
+    for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
+        int lo = p % Slo;
+        int rest = p / Slo;
+        int jc = rest % nc;
+        int hi = rest / nc;
+        const V* s = src + lo + Slo * (nf * hi);
+        int f = 2 * jc + 1;   // 0-based fine index of the coarse point
+        dst[p] = s[Slo * f] + V(0.5) * (s[Slo * (f - 1)] + s[Slo * (f + 1)]);
+    }
+    sh.n[r] = nc;
+    sh.restage();
+    __syncthreads();
+}
+
+// ConvertToPrewavelet2 (MatrixVectorHomogen.h:540-562): Ax_p = sum_J c_J z_J over
+// the 5^d prewavelet stencil of p, applied as d contractions box -> block.
+template <typename V>
+__device__ inline V* convert_prew(V* src, V* dst, Shape& sh, const int32_t* T) {
+    int ext[MAXD];
+    int cnt = sh.total;
+    for (int i = 0; i < sh.d; ++i) ext[i] = sh.n[i];
+    for (int i = 0; i < sh.d; ++i) {
+        const int t = T[i];
+        const int nb = ext[i];
+        const int m = (nb + 1) >> 1;
+        int Slo = 1;
+        for (int q = 0; q < i; ++q) Slo *= ext[q];
+        const int outCnt = cnt / nb * m;
+        for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
+            int lo = p % Slo;
+            int rest = p / Slo;
+            int j = rest % m;
+            int hi = rest / m;
+            const V* s = src + lo + Slo * (nb * hi);
+            int k = 2 * j + 1;        // 1-based position; 0-based index k-1
+            V v = cCenter<V>(t, k, nb) * s[Slo * (k - 1)];
+            if (k - 1 >= 1) v += V(-0.6) * s[Slo * (k - 2)] + V(0.1) * s[Slo * (k - 3)];
+            if (k + 1 <= nb) v += V(-0.6) * s[Slo * k] + V(0.1) * s[Slo * (k + 1)];
+            dst[p] = v;
+        }
+        ext[i] = m;
+        cnt = outCnt;
+        __syncthreads();
+        V* tmp = src; src = dst; dst = tmp;
+    }
+    sh.total = cnt;
+    return src;
+}
+
+template <typename V, int PPT>
+__global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
+    extern __shared__ char smem_raw[];
+    V* Y = reinterpret_cast<V*>(smem_raw);
+    V* Z = Y + a.maxBox;
+    V* U = Z + a.maxBox;                      // general path only
+    V* Zt = U + a.maxBox;                     // general path only
+    V* Wt = a.general ? Zt + a.maxBox : U;    // weights
+    __shared__ int woff[MAXD];
+    const int inst = blockIdx.y;
+    if (a.st && !a.st[inst].active) return;
+    const ApplyItem& it = a.items[blockIdx.x];
+    const int d = a.d;
+    Shape sh;
+    sh.box(it.T, d);
+    const V* in = (it.in_nodal ? a.nodal + inst * a.nStride : a.cbuf + inst * a.cStride) + it.in_off;
+    const V* coef = a.coefTab + inst * a.coefStride;
+
+    if (!a.general) {
+        // single separable stiffness term: Y = u, Z = A_T u
+        for (int p = threadIdx.x; p < sh.total; p += blockDim.x) Y[p] = in[p];
+        load_weights<V>(Wt, woff, coef, a.tabLen, it.T, d);
+        for (int i = d - 1; i >= 0; --i)
+            horner_step<V, PPT>(Y, Z, sh, i, Wt + woff[i], it.T[i], i == d - 1, i > 0);
+    } else {
+        for (int p = threadIdx.x; p < sh.total; p += blockDim.x) { U[p] = in[p]; Z[p] = V(0); }
+        __syncthreads();
+        for (int m = 0; m < a.nA + a.nC; ++m) {
+            load_weights<V>(Wt, woff, coef + (int64_t)m * d * a.tabLen, a.tabLen, it.T, d);
+            for (int p = threadIdx.x; p < sh.total; p += blockDim.x) Y[p] = U[p];
+            __syncthreads();
+            if (m < a.nA) {
+                for (int i = d - 1; i >= 0; --i)
+                    horner_step<V, PPT>(Y, Zt, sh, i, Wt + woff[i], it.T[i], i == d - 1, i > 0);
+                for (int p = threadIdx.x; p < sh.total; p += blockDim.x) Z[p] += Zt[p];
+            } else {
+                for (int i = d - 1; i >= 0; --i) mass_step<V, PPT>(Y, sh, i, Wt + woff[i], it.T[i]);
+                for (int p = threadIdx.x; p < sh.total; p += blockDim.x) Z[p] += Y[p];
+            }
+            __syncthreads();
+        }
+    }
+
+    V* wbase = a.wbuf + inst * a.wStride;
+    V* cur = Z;
+    V* oth = Y;
+    if (it.rd_off[0] >= 0) {
+        const V* g = wbase + it.rd_off[0];
+        for (int p = threadIdx.x; p < sh.total; p += blockDim.x) cur[p] += g[p];
+        __syncthreads();
+    }
+    for (int j = 1; j <= it.k; ++j) {
+        restrict_dim<V>(cur, oth, sh, it.r[j - 1]);
+        V* tmp = cur; cur = oth; oth = tmp;
+        const bool w = it.w_off[j] >= 0, rd = it.rd_off[j] >= 0;
+        if (w || rd) {
+            V* wd = w ? wbase + it.w_off[j] : nullptr;
+            const V* g = rd ? wbase + it.rd_off[j] : nullptr;
+            for (int p = threadIdx.x; p < sh.total; p += blockDim.x) {
+                V v = cur[p];
+                if (w) wd[p] = v;
+                if (rd) cur[p] = v + g[p];
+            }
+            __syncthreads();
+        }
+    }
+    int32_t Tc[MAXD];
+    for (int i = 0; i < d; ++i) Tc[i] = it.T[i];
+    for (int j = 0; j < it.k; ++j) Tc[it.r[j]] -= 1;
+    V* res = convert_prew<V>(cur, oth, sh, Tc);
+    V* out = a.part + inst * a.pStride + it.out_off;
+    for (int p = threadIdx.x; p < sh.total; p += blockDim.x) out[p] = res[p];
+}
+
+// ------------------------------------------------------------------- sum
+// Ax = sum over cases of the case partials (MatrixVectorHomogen.h:730-749), in
+// ascending case order.
+template <typename V>
+__global__ void __launch_bounds__(256) sum_kernel(const SumItem* __restrict__ items, const int64_t* __restrict__ list,
+                                                  int d, const V* __restrict__ part, int64_t pStride,
+                                                  V* __restrict__ y, int64_t yStride,
+                                                  const CGState* __restrict__ st) {
+    const int inst = blockIdx.y;
+    if (st && !st[inst].active) return;
+    const SumItem& it = items[blockIdx.x];
+    int cnt = 1;
+    for (int i = 0; i < d; ++i) cnt *= 1 << (it.T[i] - 1);
+    const V* pb = part + inst * pStride;
+    V* out = y + inst * yStride + it.blk_off;
+    for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
+        V s = V(0);
+        for (int c = 0; c < it.ncase; ++c) s = s + pb[list[it.list_start + c] + p];
+        out[p] = s;
+    }
+}
+
+// ------------------------------------------------------- preconditioner
+// z_p = psi_p^T A_T psi_p for the separable operator (Preconditioning::precond3,
+// cg_method.h:322-451): per dimension the 1D quadratic forms of the prewavelet
+// against the weighted 1D stiffness and mass matrices.
+template <typename V>
+__global__ void __launch_bounds__(256) diag_kernel(const SumItem* __restrict__ items, int d, int nA, int nC,
+                                                   const V* __restrict__ coefTab, int64_t coefStride, int64_t tabLen,
+                                                   V* __restrict__ z, int64_t zStride) {
+    const int inst = blockIdx.y;
+    const SumItem& it = items[blockIdx.x];
+    int m[MAXD];
+    int cnt = 1;
+    for (int i = 0; i < d; ++i) { m[i] = 1 << (it.T[i] - 1); cnt *= m[i]; }
+    const V* coef = coefTab + inst * coefStride;
+    for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
+        double total = 0.0;
+        for (int term = 0; term < nA + nC; ++term) {
+            double kq[MAXD], mq[MAXD];
+            int rem = p;
+            for (int i = 0; i < d; ++i) {
+                int j = rem % m[i];
+                rem /= m[i];
+                const int t = it.T[i];
+                const int n = (1 << t) - 1;
+                const int k = 2 * j + 1;
+                double psi[5];     // positions k-2..k+2
+                psi[2] = (t >= 2 && (k == 1 || k == n)) ? 0.9 : 1.0;
+                psi[1] = psi[0] = psi[3] = psi[4] = 0.0;
+                if (k - 1 >= 1) { psi[1] = -0.6; psi[0] = 0.1; }
+                if (k + 1 <= n) { psi[3] = -0.6; psi[4] = 0.1; }
+                const V* w = coef + ((int64_t)term * d + i) * tabLen + ((1 << t) - 2);
+                double h = 1.0 / double(1 << t);
+                double sk = 0.0, sm = 0.0;
+                for (int c = k - 3; c <= k + 2; ++c) {      // cell c spans positions c, c+1
+                    if (c < 0 || c > n) continue;
+                    int a0 = c - (k - 2), a1 = a0 + 1;
+                    double va = (a0 >= 0 && a0 <= 4 && c >= 1) ? psi[a0] : 0.0;
+                    double vb = (a1 >= 0 && a1 <= 4 && c + 1 <= n) ? psi[a1] : 0.0;
+                    double wc = double(w[c]);
+                    sk += wc * (vb - va) * (vb - va) / h;
+                    sm += wc * h / 3.0 * (va * va + va * vb + vb * vb);
+                }
+                kq[i] = sk;
+                mq[i] = sm;
+            }
+            if (term < nA) {
+                for (int i = 0; i < d; ++i) {
+                    double pr = kq[i];
+                    for (int j = 0; j < d; ++j) if (j != i) pr *= mq[j];
+                    total += pr;
+                }
+            } else {
+                double pr = 1.0;
+                for (int j = 0; j < d; ++j) pr *= mq[j];
+                total += pr;
+            }
+        }
+        z[inst * zStride + it.blk_off + p] = V(total);
+    }
+}
+
+// ------------------------------------------------------------------ RHS
+// b_p = int f psi_p for f = sum_j eta_j prod_i sin(m_j pi x_i): per dimension
+// int sin(w x) hat_q(x) dx = sin(w x_q) 2 (1 - cos(w h)) / (w^2 h).
+template <typename V>
+__global__ void __launch_bounds__(256) rhs_sines_kernel(const SumItem* __restrict__ items, int d, int nterms,
+                                                        const double* __restrict__ eta, const int32_t* __restrict__ mfreq,
+                                                        V* __restrict__ b, int64_t bStride) {
+    const int inst = blockIdx.y;
+    const SumItem& it = items[blockIdx.x];
+    int m[MAXD];
+    int cnt = 1;
+    for (int i = 0; i < d; ++i) { m[i] = 1 << (it.T[i] - 1); cnt *= m[i]; }
+    for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
+        double total = 0.0;
+        for (int term = 0; term < nterms; ++term) {
+            const double w = double(mfreq[term]) * M_PI;
+            double prod = eta[inst * nterms + term];
+            int rem = p;
+            for (int i = 0; i < d; ++i) {
+                int j = rem % m[i];
+                rem /= m[i];
+                const int t = it.T[i];
+                const int n = (1 << t) - 1;
+                const int k = 2 * j + 1;
+                const double h = 1.0 / double(1 << t);
+                const double fac = 2.0 * (1.0 - cos(w * h)) / (w * w * h);
+                double s = ((t >= 2 && (k == 1 || k == n)) ? 0.9 : 1.0) * sin(w * k * h);
+                if (k - 1 >= 1) s += -0.6 * sin(w * (k - 1) * h) + 0.1 * sin(w * (k - 2) * h);
+                if (k + 1 <= n) s += -0.6 * sin(w * (k + 1) * h) + 0.1 * sin(w * (k + 2) * h);
+                prod *= s * fac;
+            }
+            total += prod;
+        }
+        b[inst * bStride + it.blk_off + p] = V(total);
+    }
+}
+
+// ------------------------------------------------------------- reductions
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+// fixed-order block reduction of up to NV values per thread
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double* out) {
+    __shared__ double red[NV][16];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double s = wave_sum(v[q]);
+        if (lane == 0) red[q][wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) {
+            double s = 0.0;
+            for (int w = 0; w < nw; ++w) s += red[q][w];
+            out[q] = s;
+        }
+    }
+}
+
+// partials[inst][q][block] of sum_i a_q(i) * b_q(i)
+template <typename V, typename W>
+__global__ void __launch_bounds__(256) dot_kernel(const V* __restrict__ a, const W* __restrict__ b, int64_t n,
+                                                  int64_t stride, double* __restrict__ partials,
+                                                  const CGState* __restrict__ st) {
+    const int inst = blockIdx.y;
+    if (st && !st[inst].active) return;
+    double v[1] = {0.0};
+    const V* x = a + inst * stride;
+    const W* y = b + inst * stride;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        v[0] += double(x[i]) * double(y[i]);
+    double out[1];
+    block_sum<1>(v, out);
+    if (threadIdx.x == 0) partials[(int64_t)inst * gridDim.x + blockIdx.x] = out[0];
+}
+
+__device__ inline double sum_partials(const double* p, int nb) {
+    // one thread, fixed order
+    double s = 0.0;
+    for (int b = 0; b < nb; ++b) s += p[b];
+    return s;
+}
+
+// r = f - r ; h = r / D ; d = -h ; partials: r.h, r.r, f.f
+template <typename V>
+__global__ void __launch_bounds__(256) cg_init_kernel(const V* __restrict__ f, V* __restrict__ r, V* __restrict__ h,
+                                                      V* __restrict__ dv, const V* __restrict__ D, int64_t n,
+                                                      int64_t stride, double* __restrict__ partials) {
+    const int inst = blockIdx.y;
+    double v[3] = {0.0, 0.0, 0.0};
+    const int64_t o = inst * stride;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        V ri = f[o + i] - r[o + i];
+        V hi = ri / D[o + i];
+        r[o + i] = ri;
+        h[o + i] = hi;
+        dv[o + i] = V(-1) * hi;
+        v[0] += double(ri) * double(hi);
+        v[1] += double(ri) * double(ri);
+        v[2] += double(f[o + i]) * double(f[o + i]);
+    }
+    double out[3];
+    block_sum<3>(v, out);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 3; ++q) partials[((int64_t)inst * 3 + q) * gridDim.x + blockIdx.x] = out[q];
+}
+
+__global__ void cg_start_scalar(CGState* st, const double* __restrict__ partials, int nb, double eps, int rule,
+                                int maxit) {
+    const int inst = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    CGState& s = st[inst];
+    s.delta = sum_partials(partials + ((int64_t)inst * 3 + 0) * nb, nb);
+    s.rr = sum_partials(partials + ((int64_t)inst * 3 + 1) * nb, nb);
+    s.ff = sum_partials(partials + ((int64_t)inst * 3 + 2) * nb, nb);
+    s.eps = eps;
+    s.rule = rule;
+    s.maxit = maxit;
+    s.it = 0;
+    s.do_dir = 0;
+    s.converged = 0;
+    if (s.delta != s.delta) { s.active = 0; s.converged = -1; return; }
+    bool go = (rule == 0) ? (s.delta > eps) : (s.rr > eps * eps * s.ff);
+    s.active = (go && maxit >= 1) ? 1 : 0;
+    if (!go) s.converged = 1;
+}
+
+__global__ void cg_tau_scalar(CGState* st, const double* __restrict__ partials, int nb) {
+    const int inst = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    CGState& s = st[inst];
+    s.do_dir = 0;
+    if (!s.active) return;
+    double dg = sum_partials(partials + (int64_t)inst * nb, nb);
+    s.tau = s.delta / dg;
+    s.it += 1;
+}
+
+// r += tau g ; x -= tau d ; h = r / D ; partials r.h, r.r
+template <typename V, typename X>
+__global__ void __launch_bounds__(256) cg_update_kernel(V* __restrict__ r, X* __restrict__ x, V* __restrict__ h,
+                                                        const V* __restrict__ dv, const V* __restrict__ g,
+                                                        const V* __restrict__ D, int64_t n, int64_t stride,
+                                                        double* __restrict__ partials, const CGState* __restrict__ st) {
+    const int inst = blockIdx.y;
+    if (!st[inst].active) return;
+    const V tau = V(st[inst].tau);
+    double v[2] = {0.0, 0.0};
+    const int64_t o = inst * stride;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        V ri = r[o + i] + tau * g[o + i];
+        x[o + i] = x[o + i] - X(tau) * X(dv[o + i]);
+        V hi = V(1) * ri / D[o + i];
+        r[o + i] = ri;
+        h[o + i] = hi;
+        v[0] += double(ri) * double(hi);
+        v[1] += double(ri) * double(ri);
+    }
+    double out[2];
+    block_sum<2>(v, out);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 2; ++q) partials[((int64_t)inst * 2 + q) * gridDim.x + blockIdx.x] = out[q];
+}
+
+__global__ void cg_beta_scalar(CGState* st, const double* __restrict__ partials, int nb) {
+    const int inst = blockIdx.x;
+    if (threadIdx.x != 0) return;
+    CGState& s = st[inst];
+    if (!s.active) return;
+    double dp = sum_partials(partials + ((int64_t)inst * 2 + 0) * nb, nb);
+    s.rr = sum_partials(partials + ((int64_t)inst * 2 + 1) * nb, nb);
+    if (dp != dp) { s.active = 0; s.converged = -1; s.delta = dp; return; }
+    bool stop = (s.rule == 0) ? (dp < s.eps * s.eps) : (s.rr <= s.eps * s.eps * s.ff);
+    if (stop) { s.active = 0; s.converged = 1; s.delta = dp; return; }
+    s.beta = dp / s.delta;
+    s.delta = dp;
+    s.do_dir = 1;
+    bool cont = (s.rule == 0) ? (s.delta > s.eps) : true;
+    if (!cont || s.it >= s.maxit) {
+        s.active = 0;
+        s.converged = cont ? 0 : 1;
+    }
+}
+
+template <typename V>
+__global__ void __launch_bounds__(256) cg_dir_kernel(V* __restrict__ dv, const V* __restrict__ h, int64_t n,
+                                                     int64_t stride, const CGState* __restrict__ st) {
+    const int inst = blockIdx.y;
+    if (!st[inst].do_dir) return;
+    const V beta = V(st[inst].beta);
+    const int64_t o = inst * stride;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dv[o + i] = V(-1) * h[o + i] + beta * dv[o + i];
+}
+
+// elementwise helpers for iterative refinement
+template <typename A, typename B>
+__global__ void __launch_bounds__(256) cast_kernel(const A* __restrict__ a, B* __restrict__ b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        b[i] = B(a[i]);
+}
+// r = f - Ax (FP64), partials of r.r and f.f per instance
+__global__ void __launch_bounds__(256) residual_kernel(const double* __restrict__ f, const double* __restrict__ Ax,
+                                                       double* __restrict__ r, int64_t n, int64_t stride,
+                                                       double* __restrict__ partials) {
+    const int inst = blockIdx.y;
+    double v[2] = {0.0, 0.0};
+    const int64_t o = inst * stride;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double ri = f[o + i] - Ax[o + i];
+        r[o + i] = ri;
+        v[0] += ri * ri;
+        v[1] += f[o + i] * f[o + i];
+    }
+    double out[2];
+    block_sum<2>(v, out);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 2; ++q) partials[((int64_t)inst * 2 + q) * gridDim.x + blockIdx.x] = out[q];
+}
+// x += e (FP32 correction into the FP64 iterate)
+__global__ void __launch_bounds__(256) axpy_correction_kernel(double* __restrict__ x, const float* __restrict__ e,
+                                                              int64_t total) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        x[i] += double(e[i]);
+}
+__global__ void __launch_bounds__(256) zero_kernel(void* __restrict__ p, int64_t bytes) {
+    uint32_t* q = reinterpret_cast<uint32_t*>(p);
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < bytes / 4; i += (int64_t)gridDim.x * blockDim.x)
+        q[i] = 0u;
+}
+
+// ------------------------------------------------------------ launchers
+static inline unsigned gridFor(int64_t n, int per = 256 * 4) {
+    int64_t g = (n + per - 1) / per;
+    if (g < 1) g = 1;
+    if (g > 1024) g = 1024;
+    return (unsigned)g;
+}
+
+template <typename V>
+void launch_nodal(const NodalItem* items, int nitems, int d, const V* x, int64_t xStride, V* nodal, int64_t nStride,
+                  int64_t maxBox, int batch, const CGState* st, hipStream_t s) {
+    if (nitems == 0) return;
+    size_t lds = 2 * maxBox * sizeof(V);
+    hipLaunchKernelGGL(nodal_kernel<V>, dim3(nitems, batch), dim3(256), lds, s, items, d, x, xStride, nodal, nStride,
+                       maxBox, st);
+}
+template <typename V>
+void launch_prol(const ProlItem* items, int nitems, int d, const V* nodal, int64_t nStride, V* cbuf, int64_t cStride,
+                 int64_t maxBox, int batch, const CGState* st, hipStream_t s) {
+    if (nitems == 0) return;
+    size_t lds = 2 * maxBox * sizeof(V);
+    hipLaunchKernelGGL(prol_kernel<V>, dim3(nitems, batch), dim3(256), lds, s, items, d, nodal, nStride, cbuf, cStride,
+                       maxBox, st);
+}
+template <typename V>
+int launch_apply(const ApplyArgs<V>& a, int nitems, int batch, int threads, size_t lds, hipStream_t s) {
+    if (nitems == 0) return 0;
+    const int64_t ppt = (a.maxBox + threads - 1) / threads;
+    dim3 grid(nitems, batch), blk(threads);
+#define LSG_APPLY_CASE(P)                                                                     \
+    if (ppt <= P) {                                                                           \
+        hipLaunchKernelGGL((apply_kernel<V, P>), grid, blk, lds, s, a);                       \
+        return 0;                                                                             \
+    }
+    LSG_APPLY_CASE(1)
+    LSG_APPLY_CASE(2)
+    LSG_APPLY_CASE(4)
+    LSG_APPLY_CASE(8)
+    LSG_APPLY_CASE(16)
+    LSG_APPLY_CASE(32)
+#undef LSG_APPLY_CASE
+    return -1;
+}
+template <typename V>
+void launch_sum(const SumItem* items, int nitems, const int64_t* list, int d, const V* part, int64_t pStride, V* y,
+                int64_t yStride, int batch, const CGState* st, hipStream_t s) {
+    hipLaunchKernelGGL(sum_kernel<V>, dim3(nitems, batch), dim3(128), 0, s, items, list, d, part, pStride, y, yStride,
+                       st);
+}
+template <typename V>
+void launch_diag(const SumItem* items, int nitems, int d, int nA, int nC, const V* coefTab, int64_t coefStride,
+                 int64_t tabLen, V* z, int64_t zStride, int batch, hipStream_t s) {
+    hipLaunchKernelGGL(diag_kernel<V>, dim3(nitems, batch), dim3(128), 0, s, items, d, nA, nC, coefTab, coefStride,
+                       tabLen, z, zStride);
+}
+template <typename V>
+void launch_rhs_sines(const SumItem* items, int nitems, int d, int nterms, const double* eta, const int32_t* m, V* b,
+                      int64_t bStride, int batch, hipStream_t s) {
+    hipLaunchKernelGGL(rhs_sines_kernel<V>, dim3(nitems, batch), dim3(128), 0, s, items, d, nterms, eta, m, b, bStride);
+}
+
+template <typename V>
+void launch_cg_init(const V* f, V* r, V* h, V* dv, const V* D, int64_t n, int batch, double* partials, int nb,
+                    hipStream_t s) {
+    hipLaunchKernelGGL(cg_init_kernel<V>, dim3(nb, batch), dim3(256), 0, s, f, r, h, dv, D, n, n, partials);
+}
+void launch_cg_start(CGState* st, const double* partials, int nb, double eps, int rule, int maxit, int batch,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(cg_start_scalar, dim3(batch), dim3(64), 0, s, st, partials, nb, eps, rule, maxit);
+}
+template <typename V>
+void launch_cg_iter_tail(V* r, void* x, bool x64, V* h, V* dv, const V* g, const V* D, int64_t n, int batch,
+                         double* partials, int nb, CGState* st, hipStream_t s) {
+    hipLaunchKernelGGL((dot_kernel<V, V>), dim3(nb, batch), dim3(256), 0, s, dv, g, n, n, partials, st);
+    hipLaunchKernelGGL(cg_tau_scalar, dim3(batch), dim3(64), 0, s, st, partials, nb);
+    if (x64)
+        hipLaunchKernelGGL((cg_update_kernel<V, double>), dim3(nb, batch), dim3(256), 0, s, r, (double*)x, h, dv, g, D,
+                           n, n, partials, st);
+    else
+        hipLaunchKernelGGL((cg_update_kernel<V, V>), dim3(nb, batch), dim3(256), 0, s, r, (V*)x, h, dv, g, D, n, n,
+                           partials, st);
+    hipLaunchKernelGGL(cg_beta_scalar, dim3(batch), dim3(64), 0, s, st, partials, nb);
+    hipLaunchKernelGGL(cg_dir_kernel<V>, dim3(nb, batch), dim3(256), 0, s, dv, h, n, n, st);
+}
+void launch_residual(const double* f, const double* Ax, double* r, int64_t n, int batch, double* partials, int nb,
+                     hipStream_t s) {
+    hipLaunchKernelGGL(residual_kernel, dim3(nb, batch), dim3(256), 0, s, f, Ax, r, n, n, partials);
+}
+void launch_cast_d2f(const double* a, float* b, int64_t total, hipStream_t s) {
+    hipLaunchKernelGGL((cast_kernel<double, float>), dim3(gridFor(total)), dim3(256), 0, s, a, b, total);
+}
+void launch_cast_f2d(const float* a, double* b, int64_t total, hipStream_t s) {
+    hipLaunchKernelGGL((cast_kernel<float, double>), dim3(gridFor(total)), dim3(256), 0, s, a, b, total);
+}
+void launch_axpy_correction(double* x, const float* e, int64_t total, hipStream_t s) {
+    hipLaunchKernelGGL(axpy_correction_kernel, dim3(gridFor(total)), dim3(256), 0, s, x, e, total);
+}
+void launch_zero(void* p, int64_t bytes, hipStream_t s) {
+    hipLaunchKernelGGL(zero_kernel, dim3(gridFor(bytes / 4)), dim3(256), 0, s, p, bytes);
+}
+int cg_partial_blocks(int64_t n) { return (int)gridFor(n); }
+
+#define LSG_INST(V)                                                                                                  \
+    template void launch_nodal<V>(const NodalItem*, int, int, const V*, int64_t, V*, int64_t, int64_t, int,         \
+                                  const CGState*, hipStream_t);                                                          \
+    template void launch_prol<V>(const ProlItem*, int, int, const V*, int64_t, V*, int64_t, int64_t, int, const CGState*, \
+                                 hipStream_t);                                                                       \
+    template int launch_apply<V>(const ApplyArgs<V>&, int, int, int, size_t, hipStream_t);                          \
+    template void launch_sum<V>(const SumItem*, int, const int64_t*, int, const V*, int64_t, V*, int64_t, int,       \
+                                const CGState*, hipStream_t);                                                            \
+    template void launch_diag<V>(const SumItem*, int, int, int, int, const V*, int64_t, int64_t, V*, int64_t, int,   \
+                                 hipStream_t);                                                                       \
+    template void launch_rhs_sines<V>(const SumItem*, int, int, int, const double*, const int32_t*, V*, int64_t,     \
+                                      int, hipStream_t);                                                             \
+    template void launch_cg_init<V>(const V*, V*, V*, V*, const V*, int64_t, int, double*, int, hipStream_t);         \
+    template void launch_cg_iter_tail<V>(V*, void*, bool, V*, V*, const V*, const V*, int64_t, int, double*, int,     \
+                                         CGState*, hipStream_t);
+LSG_INST(double)
+LSG_INST(float)
+#undef LSG_INST
+
+}  // namespace lsg

@@ -1,0 +1,395 @@
+// lsg_plan.cpp -- grid flattening and matvec schedule (host side).
+//
+// The schedule restates the data flow of MatrixVectorHomogen::caseFunction
+// (library/source/MatrixVectorMultiplication/MatrixVectorHomogen.h:965-1090):
+//   * cases = restriction subsets R of the dimensions (CasesIterator,
+//     tests/old_versions/MatrixVectorMultiplicationPrewavelets.h:23-50);
+//   * per case, the depths S_c of DepthList::sortDepths
+//     (iterator/depthIterator.cpp:7-41,88-93) in the reference order;
+//   * prolongation: P[T+e_d] += I_d P[T] for every prolongation dim d, in
+//     ascending d and sorted-depth order (MatrixVectorHomogen.h:997-1019);
+//   * for every T of S_c: z = A_T P[T] + gM[T], then for each restriction dim
+//     (ascending) restrict, g = z, z += gM[Tc], gM[Tc] = g, and finally
+//     ConvertToPrewavelet2(z, Ax, Tc) (MatrixVectorHomogen.h:1023-1085).
+// The gM exchanges are replayed symbolically in the reference order, so every
+// read of gM is bound to the exact earlier write it sees.  That turns the
+// reference's sequential loop into a dependency DAG whose items at equal depth in
+// the DAG (wavefront levels) run concurrently on the GPU with identical data flow.
+#include "lsg_plan.hpp"
+#include "../../include/loadsg_amd.h"
+
+#include <algorithm>
+#include <cstring>
+#include <unordered_map>
+#include <unordered_set>
+
+namespace lsg {
+
+namespace {
+
+inline int depth1(int32_t i) { return i <= 1 ? 0 : 31 - __builtin_clz((uint32_t)i); }
+
+struct Key2 {
+    uint64_t a = 0, b = 0;
+    bool operator==(const Key2& o) const { return a == o.a && b == o.b; }
+};
+struct Key2Hash {
+    size_t operator()(const Key2& k) const { return (size_t)(k.a * 0x9E3779B97F4A7C15ull ^ (k.b + 0x632BE59BD9B4E019ull)); }
+};
+inline Key2 packKey(const int32_t* v, int d) {
+    Key2 k;
+    for (int i = 0; i < d; ++i) {
+        uint64_t x = (uint64_t)(uint32_t)v[i] & 0xFFFFull;
+        if (i < 4) k.a |= x << (16 * i);
+        else k.b |= x << (16 * (i - 4));
+    }
+    return k;
+}
+
+void dfsSons(int d, std::array<int32_t, MAXD>& I, int level, std::unordered_set<Key2, Key2Hash>& seen,
+             std::vector<int32_t>& out) {
+    Key2 k = packKey(I.data(), d);
+    if (seen.count(k)) return;     // every path to a point has the same length
+    seen.insert(k);
+    for (int i = 0; i < d; ++i) out.push_back(I[i]);
+    if (level <= 0) return;
+    for (int dd = 0; dd < d; ++dd) {
+        int32_t save = I[dd];
+        I[dd] = save << 1;
+        dfsSons(d, I, level - 1, seen, out);
+        I[dd] = (save << 1) + 1;
+        dfsSons(d, I, level - 1, seen, out);
+        I[dd] = save;
+    }
+}
+
+// DepthList::recursiveDepth (iterator/depthIterator.cpp:7-41)
+void recursiveDepth(const Grid& g, const bool* R, bool checkProl, int dd, Depth& T, std::vector<int>& out) {
+    int D = g.d;
+    if (checkProl && dd > 0) {
+        if (!R[dd]) {
+            for (int t = 1; t <= g.maxDepth[dd]; ++t) { T[dd] = t; recursiveDepth(g, R, checkProl, dd - 1, T, out); }
+        } else recursiveDepth(g, R, checkProl, dd - 1, T, out);
+    }
+    if (checkProl && dd == 0) {
+        if (!R[dd]) {
+            for (int t = 1; t <= g.maxDepth[dd]; ++t) { T[dd] = t; recursiveDepth(g, R, false, D - 1, T, out); }
+        } else recursiveDepth(g, R, false, D - 1, T, out);
+    }
+    if (!checkProl && dd >= 0) {
+        if (R[dd]) {
+            for (int t = g.maxDepth[dd]; t > 1; --t) { T[dd] = t; recursiveDepth(g, R, checkProl, dd - 1, T, out); }
+        } else recursiveDepth(g, R, checkProl, dd - 1, T, out);
+    }
+    if (dd < 0) {
+        auto it = g.depthIdx.find(T);
+        if (it != g.depthIdx.end()) out.push_back(it->second);
+    }
+}
+
+int64_t boxOf(const Depth& T, int d) {
+    int64_t s = 1;
+    for (int i = 0; i < d; ++i) s *= (int64_t(1) << T[i]) - 1;
+    return s;
+}
+int64_t blockOf(const Depth& T, int d) {
+    int64_t s = 1;
+    for (int i = 0; i < d; ++i) s *= int64_t(1) << (T[i] - 1);
+    return s;
+}
+
+int buildPlan(Grid& g, std::string& err) {
+    Plan& P = g.plan;
+    const int d = g.d;
+    const int nd = (int)g.depths.size();
+    P = Plan();
+
+    // --- calcNodal items
+    for (int t = 0; t < nd; ++t) {
+        NodalItem it{};
+        for (int i = 0; i < d; ++i) it.T[i] = g.depths[t][i];
+        it.in_off = g.blkOff[t];
+        it.out_off = g.boxOff[t];
+        P.nodal.push_back(it);
+        P.maxBox = std::max(P.maxBox, g.boxSize[t]);
+        P.maxBlock = std::max(P.maxBlock, g.blkSize[t]);
+    }
+
+    struct Read { int item = -1; int step = -1; };
+    struct Tmp {
+        int c, t;
+        int k;
+        int r[MAXD];
+        Read reads[MAXD + 1];
+        int outDepth;
+        int level = 0;
+        int64_t caseOff = -1;
+        bool wNeeded[MAXD + 1] = {};
+        int64_t wOff[MAXD + 1];
+    };
+    std::vector<Tmp> items;
+    P.prol.assign(d, {});
+    std::vector<std::vector<int64_t>> partOf(nd, std::vector<int64_t>(size_t(1) << d, -1));
+    double stream = double(g.n) + double(g.nml) + double(g.n);   // calcNodal in/out + final write
+
+    const int ncase = 1 << d;
+    for (int c = 0; c < ncase; ++c) {
+        bool R[MAXD] = {};
+        for (int i = 0; i < d; ++i) R[i] = (c >> i) & 1;
+        std::vector<int> sorted;
+        Depth T0{};
+        recursiveDepth(g, R, true, d - 1, T0, sorted);
+        if (sorted.empty()) continue;
+        bool hasProl = false;
+        for (int i = 0; i < d; ++i) if (!R[i]) hasProl = true;
+
+        // case buffer boxes
+        std::unordered_map<int, int64_t> caseOff;
+        if (hasProl) {
+            for (int t : sorted) { caseOff[t] = P.caseSize; P.caseSize += g.boxSize[t]; }
+        }
+        std::unordered_set<int> inS(sorted.begin(), sorted.end());
+
+        // prolongation families (ascending d, families along d)
+        bool firstProl = true;
+        for (int dd = 0; dd < d; ++dd) {
+            if (R[dd]) continue;
+            std::map<Depth, std::vector<int>> fam;   // key: T with T[dd] = 0
+            for (int t : sorted) {
+                Depth K = g.depths[t];
+                K[dd] = 0;
+                fam[K].push_back(t);
+            }
+            for (auto& [K, ts] : fam) {
+                std::sort(ts.begin(), ts.end(), [&](int a, int b) { return g.depths[a][dd] < g.depths[b][dd]; });
+                // the reference links T -> T + e_dd only when the finer depth exists
+                for (size_t q = 1; q < ts.size(); ++q) {
+                    if (g.depths[ts[q]][dd] != g.depths[ts[q - 1]][dd] + 1) {
+                        err = "depth list not contiguous along a prolongation dimension";
+                        return LSG_ERR_UNSUPPORTED;
+                    }
+                }
+                if ((int)ts.size() > MAXLEV) { err = "family too long"; return LSG_ERR_UNSUPPORTED; }
+                ProlItem pi{};
+                for (int i = 0; i < d; ++i) pi.T[i] = g.depths[ts[0]][i];
+                pi.dd = dd;
+                pi.nt = (int)ts.size();
+                pi.from_nodal = firstProl ? 1 : 0;
+                for (size_t q = 0; q < ts.size(); ++q) {
+                    pi.in_off[q] = firstProl ? g.boxOff[ts[q]] : caseOff[ts[q]];
+                    pi.out_off[q] = caseOff[ts[q]];
+                    P.maxFamily = std::max(P.maxFamily, g.boxSize[ts[q]]);
+                    if (q > 0) stream += double(g.boxSize[ts[q - 1]]) + 2.0 * double(g.boxSize[ts[q]]);
+                }
+                P.prol[dd].push_back(pi);
+            }
+            firstProl = false;
+        }
+
+        // symbolic replay of the gM exchanges in the reference order
+        std::unordered_map<int, Read> state;   // depth -> latest write
+        std::unordered_map<int, int> itemOfDepth;
+        for (int t : sorted) {
+            Tmp X{};
+            X.c = c; X.t = t; X.k = 0;
+            for (int i = 0; i < d; ++i) if (R[i]) X.r[X.k++] = i;
+            for (int j = 0; j <= MAXD; ++j) { X.reads[j] = Read(); X.wOff[j] = -1; }
+            X.caseOff = hasProl ? caseOff[t] : -1;
+            int me = (int)items.size();
+            auto st = state.find(t);
+            if (st != state.end()) X.reads[0] = st->second;
+            Depth Tc = g.depths[t];
+            stream += 3.0 * double(g.boxSize[t]);
+            for (int j = 1; j <= X.k; ++j) {
+                int rr = X.r[j - 1];
+                Depth prev = Tc;
+                Tc[rr] -= 1;
+                auto it = g.depthIdx.find(Tc);
+                if (it == g.depthIdx.end()) {
+                    err = "coarse depth of a restriction missing from the depth list";
+                    return LSG_ERR_UNSUPPORTED;
+                }
+                int tc = it->second;
+                stream += double(boxOf(prev, d)) + 3.0 * double(g.boxSize[tc]);
+                auto s2 = state.find(tc);
+                if (s2 != state.end()) X.reads[j] = s2->second;
+                state[tc] = Read{me, j};
+            }
+            auto it = g.depthIdx.find(Tc);
+            if (it == g.depthIdx.end()) { err = "target depth missing"; return LSG_ERR_UNSUPPORTED; }
+            X.outDepth = it->second;
+            stream += double(g.boxSize[X.outDepth]) + 2.0 * double(g.blkSize[X.outDepth]);
+            if (partOf[X.outDepth][c] >= 0) { err = "two depths of one case convert to the same depth"; return LSG_ERR_UNSUPPORTED; }
+            partOf[X.outDepth][c] = P.partSize;
+            P.partSize += g.blkSize[X.outDepth];
+            // dependencies and levels
+            for (int j = 0; j <= X.k; ++j) {
+                if (X.reads[j].item >= 0) {
+                    Tmp& Y = items[X.reads[j].item];
+                    Y.wNeeded[X.reads[j].step] = true;
+                    X.level = std::max(X.level, Y.level + 1);
+                }
+            }
+            items.push_back(X);
+            itemOfDepth[t] = me;
+        }
+    }
+
+    // W buffer allocation
+    for (auto& X : items) {
+        Depth Tc = g.depths[X.t];
+        for (int j = 1; j <= X.k; ++j) {
+            Tc[X.r[j - 1]] -= 1;
+            if (X.wNeeded[j]) { X.wOff[j] = P.wSize; P.wSize += boxOf(Tc, d); }
+        }
+    }
+    int nlev = 0;
+    for (auto& X : items) nlev = std::max(nlev, X.level + 1);
+    P.apply.assign(nlev, {});
+    for (auto& X : items) {
+        ApplyItem a{};
+        for (int i = 0; i < d; ++i) a.T[i] = g.depths[X.t][i];
+        a.k = X.k;
+        for (int j = 0; j < X.k; ++j) a.r[j] = X.r[j];
+        a.in_nodal = X.caseOff < 0 ? 1 : 0;
+        a.in_off = X.caseOff < 0 ? g.boxOff[X.t] : X.caseOff;
+        for (int j = 0; j <= MAXD; ++j) { a.rd_off[j] = -1; a.w_off[j] = -1; }
+        for (int j = 0; j <= X.k; ++j)
+            if (X.reads[j].item >= 0) a.rd_off[j] = items[X.reads[j].item].wOff[X.reads[j].step];
+        for (int j = 1; j <= X.k; ++j) a.w_off[j] = X.wOff[j];
+        a.out_off = partOf[X.outDepth][X.c];
+        P.apply[X.level].push_back(a);
+    }
+    P.nApply = (int64_t)items.size();
+
+    // sum items
+    for (int t = 0; t < nd; ++t) {
+        SumItem s{};
+        for (int i = 0; i < d; ++i) s.T[i] = g.depths[t][i];
+        s.blk_off = g.blkOff[t];
+        s.list_start = (int)P.sumList.size();
+        for (int c = 0; c < ncase; ++c)
+            if (partOf[t][c] >= 0) P.sumList.push_back(partOf[t][c]);
+        s.ncase = (int)P.sumList.size() - s.list_start;
+        P.sum.push_back(s);
+    }
+    P.streamValues = stream;
+
+    // values this implementation moves per matvec (each kernel reads its inputs and
+    // writes its outputs once)
+    double mv = 0.0;
+    for (auto& it : P.nodal) mv += double(blockCount(it.T, d)) + double(boxCount(it.T, d));
+    for (auto& v : P.prol)
+        for (auto& it : v) {
+            int32_t T[MAXD];
+            std::memcpy(T, it.T, sizeof(T));
+            for (int q = 0; q < it.nt; ++q) { T[it.dd] = it.T[it.dd] + q; mv += 2.0 * double(boxCount(T, d)); }
+        }
+    for (auto& v : P.apply)
+        for (auto& a : v) {
+            int32_t T[MAXD];
+            std::memcpy(T, a.T, sizeof(T));
+            mv += double(boxCount(T, d));
+            if (a.rd_off[0] >= 0) mv += double(boxCount(T, d));
+            for (int j = 1; j <= a.k; ++j) {
+                T[a.r[j - 1]] -= 1;
+                if (a.w_off[j] >= 0) mv += double(boxCount(T, d));
+                if (a.rd_off[j] >= 0) mv += double(boxCount(T, d));
+            }
+            mv += double(blockCount(T, d));
+        }
+    for (auto& s : P.sum) mv += double(blockCount(s.T, d)) * double(s.ncase + 1);
+    P.movedValues = mv;
+    return 0;
+}
+
+}  // namespace
+
+void regularIndexTable(int d, int level, std::vector<int32_t>& idx) {
+    idx.clear();
+    std::unordered_set<Key2, Key2Hash> seen;
+    std::array<int32_t, MAXD> I{};
+    for (int i = 0; i < d; ++i) I[i] = 2;
+    dfsSons(d, I, level, seen, idx);
+}
+
+int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid& g, std::string& err) {
+    if (d < 1 || d > MAXD || n < 1 || !idx) { err = "bad arguments"; return LSG_ERR_ARG; }
+    g = Grid();
+    g.d = d;
+    g.n = n;
+    g.idx.assign(idx, idx + n * d);
+    g.active.assign(n, 1);
+    if (active) g.active.assign(active, active + n);
+    std::unordered_set<Key2, Key2Hash> seen;
+    std::vector<int> tOf(n);
+    for (int64_t s = 0; s < n; ++s) {
+        Depth T{};
+        for (int i = 0; i < d; ++i) {
+            int32_t v = idx[s * d + i];
+            if (v < 2) { err = "boundary points are not part of a zero-Dirichlet grid"; return LSG_ERR_ARG; }
+            int t = depth1(v);
+            if (t > MAXLEV) { err = "1D depth above 15"; return LSG_ERR_UNSUPPORTED; }
+            if (v - (1 << t) >= (1 << (t - 1))) { err = "invalid 1D index (outside the unit interval)"; return LSG_ERR_ARG; }
+            T[i] = t;
+        }
+        if (!g.active[s]) { err = "hanging nodes (adaptive grids) are not supported by this build"; return LSG_ERR_UNSUPPORTED; }
+        if (!seen.insert(packKey(&idx[s * d], d)).second) { err = "duplicate point"; return LSG_ERR_ARG; }
+        auto it = g.depthIdx.find(T);
+        if (it == g.depthIdx.end()) {
+            int id = (int)g.depths.size();
+            g.depthIdx[T] = id;
+            g.depths.push_back(T);
+            tOf[s] = id;
+        } else tOf[s] = it->second;
+    }
+    const int nd = (int)g.depths.size();
+    g.maxDepth = Depth{};
+    for (auto& T : g.depths)
+        for (int i = 0; i < d; ++i) {
+            g.maxDepth[i] = std::max(g.maxDepth[i], T[i]);
+            g.maxLevel = std::max(g.maxLevel, T[i]);
+        }
+    // block layout
+    std::vector<int64_t> cnt(nd, 0);
+    for (int64_t s = 0; s < n; ++s) cnt[tOf[s]]++;
+    g.blkOff.resize(nd); g.blkSize.resize(nd); g.boxOff.resize(nd); g.boxSize.resize(nd);
+    int64_t off = 0, boff = 0;
+    for (int t = 0; t < nd; ++t) {
+        g.blkSize[t] = blockOf(g.depths[t], d);
+        if (cnt[t] != g.blkSize[t]) {
+            err = "depth block not full (locally adaptive grids are not supported by this build)";
+            return LSG_ERR_UNSUPPORTED;
+        }
+        g.blkOff[t] = off; off += g.blkSize[t];
+        g.boxSize[t] = boxOf(g.depths[t], d);
+        g.boxOff[t] = boff; boff += g.boxSize[t];
+        // downward closed: every G_T must be a full box
+        for (int i = 0; i < d; ++i) {
+            if (g.depths[t][i] >= 2) {
+                Depth Q = g.depths[t];
+                Q[i] -= 1;
+                if (!g.depthIdx.count(Q)) { err = "depth list not downward closed"; return LSG_ERR_UNSUPPORTED; }
+            }
+        }
+    }
+    g.nml = boff;
+    g.devOfSlot.resize(n);
+    g.slotOfDev.resize(n);
+    for (int64_t s = 0; s < n; ++s) {
+        const Depth& T = g.depths[tOf[s]];
+        int64_t loc = 0, mul = 1;
+        for (int i = 0; i < d; ++i) {
+            int32_t v = idx[s * d + i];
+            int64_t j = v - (1 << T[i]);
+            loc += j * mul;
+            mul *= int64_t(1) << (T[i] - 1);
+        }
+        int64_t dev = g.blkOff[tOf[s]] + loc;
+        g.devOfSlot[s] = dev;
+        g.slotOfDev[dev] = s;
+    }
+    return buildPlan(g, err);
+}
+
+}  // namespace lsg

@@ -1,0 +1,120 @@
+// lsg_plan.hpp -- host-side flattening of a LoAdSG sparse grid into the
+// depth-blocked layout the gfx950 kernels work on, and the per-stage work-item
+// tables of the matrix-free prewavelet matvec.
+//
+// Data layout in HBM (one copy per batch instance, instance-major):
+//   vector      : one block per depth T (depth list order), n_T = prod 2^(T_i-1)
+//                 entries, lexicographic in the local odd positions, dim 0 fastest.
+//   nodal       : one dense box per depth T, |G_T| = prod (2^T_i - 1) entries: the
+//                 level-T full grid without boundary, dim 0 fastest (the reference's
+//                 MultiLevelVector restricted to G_T, multilevelvector.h:72-144).
+//   case buffer : per case c with prolongation dims, one box per depth of the case.
+//   W buffer    : per (apply item, restriction step) whose value a later item reads.
+//   part buffer : per (case, target depth) block of case-partial outputs.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+namespace lsg {
+
+constexpr int MAXD = 8;      // maximal dimension
+constexpr int MAXLEV = 15;   // maximal 1D depth (MaximumDepth in library/source/abbrevi.h:12)
+
+using Depth = std::array<int, MAXD>;
+
+// ---- work items shared with the device (plain structs) ----------------------
+// calcNodal: depth block -> nodal box.
+struct NodalItem {
+    int32_t T[MAXD];
+    int64_t in_off;    // vector block offset
+    int64_t out_off;   // nodal box offset
+};
+
+// Prolongation prefix sweep along one dimension for a family of depths that
+// differ only in that dimension (T[dd] = t0, t0+1, ...).
+struct ProlItem {
+    int32_t T[MAXD];   // depth of the first box
+    int32_t dd;        // sweep dimension
+    int32_t nt;        // number of boxes
+    int32_t from_nodal;
+    int32_t pad;
+    int64_t in_off[MAXLEV];
+    int64_t out_off[MAXLEV];
+};
+
+// Stencil + restriction chain + prewavelet conversion for one (case, depth).
+struct ApplyItem {
+    int32_t T[MAXD];
+    int32_t k;            // number of restriction dims
+    int32_t r[MAXD];      // restriction dims, ascending
+    int32_t in_nodal;     // 1: input box from nodal buffer, 0: from case buffer
+    int32_t pad;
+    int64_t in_off;
+    int64_t rd_off[MAXD + 1];   // gM values added after step j (0 = after the stencil); -1 none
+    int64_t w_off[MAXD + 1];    // W_j written after restriction j (1..k); -1 if never read
+    int64_t out_off;            // part buffer block of the target depth
+};
+
+// Sum of the case partials of one depth block (fixed case order).
+struct SumItem {
+    int32_t T[MAXD];
+    int32_t ncase;
+    int32_t list_start;
+    int64_t blk_off;
+};
+
+struct Plan {
+    std::vector<NodalItem> nodal;
+    std::vector<std::vector<ProlItem>> prol;        // per sweep dimension
+    std::vector<std::vector<ApplyItem>> apply;      // per wavefront level
+    std::vector<SumItem> sum;
+    std::vector<int64_t> sumList;                   // part offsets for SumItem
+    int64_t caseSize = 0;
+    int64_t wSize = 0;
+    int64_t partSize = 0;
+    int64_t maxBox = 0;         // largest |G_T|
+    int64_t maxBlock = 0;       // largest n_T
+    int64_t maxFamily = 0;      // largest box in a prolongation family
+    int64_t nApply = 0;
+    double streamValues = 0;    // streaming-model values per matvec
+    double movedValues = 0;     // values this implementation reads+writes per matvec
+};
+
+struct Grid {
+    int d = 0;
+    int64_t n = 0;
+    std::vector<int32_t> idx;      // n x d, reference slot order
+    std::vector<uint8_t> active;
+    std::vector<Depth> depths;     // DepthList order (first appearance)
+    std::map<Depth, int> depthIdx;
+    std::vector<int64_t> blkOff, blkSize;   // vector blocks per depth
+    std::vector<int64_t> boxOff, boxSize;   // nodal boxes per depth
+    int64_t nml = 0;
+    std::vector<int64_t> devOfSlot;
+    std::vector<int64_t> slotOfDev;
+    Depth maxDepth{};
+    int maxLevel = 0;              // largest 1D depth
+    Plan plan;
+};
+
+// Build the grid from a reference slot-order index table.  Returns 0 or a
+// negative status with a message in err.
+int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid& g, std::string& err);
+// Reference slot order of AddRecursiveSonsOfPoint(center, level).
+void regularIndexTable(int d, int level, std::vector<int32_t>& idx);
+
+inline int64_t boxCount(const int32_t* T, int d) {
+    int64_t s = 1;
+    for (int i = 0; i < d; ++i) s *= (int64_t(1) << T[i]) - 1;
+    return s;
+}
+inline int64_t blockCount(const int32_t* T, int d) {
+    int64_t s = 1;
+    for (int i = 0; i < d; ++i) s *= int64_t(1) << (T[i] - 1);
+    return s;
+}
+
+}  // namespace lsg
