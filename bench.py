@@ -1,0 +1,214 @@
+"""Benchmark: solves/s to a 1e-8 relative residual on the d=5 level-7 sparse grid
+(N = 61,183) with the variable coefficient a(x) = exp(sum_k xi_k sin(k pi x_{1+(k mod d)})/k)
+(16 xi ~ N(0, 0.25)) and right-hand sides f = sum_j eta_j prod_i sin(m_j pi x_i)
+(8 eta ~ N(0,1), m_j in {1,2,3}); synthetic seeded inputs (PCG64 seed 20261203).
+
+One step = one batched solve of --batch instances from x0 = 0, FP32 inner PCG under
+FP64 iterative refinement, every instance to ||f - A x||_2 <= 1e-8 ||f||_2 (checked in
+FP64).  Inputs are resident in HBM before the timed region.  Also reported: the plain
+FP64 GPU CG (batch 1), the FP64 error of the mixed solutions, the apply-kernel
+roofline (FP64 batched matvec), and the reference CPU path timed on this host.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12       # MI355X HBM3E bytes/s
+SEED = 20261203
+D, LEVEL = 5, 7
+NXI, NETA = 16, 8
+
+
+def inputs(batch, rng):
+    xi = rng.normal(0.0, 0.5, (batch, NXI))          # variance 0.25
+    eta = rng.standard_normal((batch, NETA))
+    m = rng.integers(1, 4, NETA).astype(np.int32)
+    return xi, eta, m
+
+
+def apply_traffic_from_profile():
+    """HBM bytes per apply launch group measured with rocprofv3 --pmc (if recorded)."""
+    p = os.path.join(ROOT, "profiles", "apply_kernel_d5l7_pmc.md")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("traffic_bytes_per_matvec_per_instance:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def cpu_baseline(iters_fp64, budget_s):
+    """Reference CPU path (oracle/_ref/ref_harness_d5, built from the unmodified
+    LoAdSG sources) timed on this host: calcNodal plus a sample of the 32 cases of one
+    d=5 level-7 variable-coefficient matvec, ratio-scaled to the full matvec by the
+    per-case cell counts; solves/s = 1 / ((iters+1) * t_matvec), preconditioner set-up
+    excluded (favours the CPU)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_d5")
+    if not os.path.exists(exe):
+        return None
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close")
+    xi = ",".join("%.6f" % v for v in np.random.default_rng(SEED).normal(0.0, 0.5, NXI))
+    try:
+        r = subprocess.run([exe, "samplemv", str(LEVEL), "expsin:" + xi, str(budget_s)], env=env,
+                           capture_output=True, text=True, timeout=budget_s * 6 + 120)
+    except subprocess.TimeoutExpired:
+        return None
+    if r.returncode != 0:
+        return None
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    t_mv = res["matvec_s_estimate"]
+    return {"value": 1.0 / ((iters_fp64 + 1) * t_mv), "unit": "solves/s", "cores": cores, "kind": "reference",
+            "sample": "LoAdSG MatrixVectorHomogen (unmodified sources, %d OpenMP threads): calcNodal + cases %s of the "
+                      "32 of one d=5 L7 var-coeff matvec (%.1f s timed), scaled by per-case cell counts to %.1f s per "
+                      "matvec; x (FP64 GPU CG iterations + 1 = %d) matvecs per solve, preconditioner set-up "
+                      "excluded" % (cores, res["cases"], res["timed_s"], t_mv, iters_fp64 + 1),
+            "matvec_s": t_mv}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the path shards by instances; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import loadsg_amd as L
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(SEED)
+    B = args.batch
+    xi, eta, m = inputs(B, rng)
+    grid = L.AdaptiveSparseGrid.regular(D, LEVEL)
+    n = grid.size
+    stats = grid.stats()
+    st0 = L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[0])
+    op64 = L.MatrixVectorHomogen(grid, st0, precision=64, batch=B)
+    op32 = L.MatrixVectorHomogen(grid, st0, precision=32, batch=B)
+    for b in range(B):
+        s = L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[b])
+        op64.set_coefficient(b, s)
+        op32.set_coefficient(b, s)
+    f = torch.zeros(B, n, dtype=torch.float64, device=dev)
+    op64.rhs_sines(eta, m, out=f)
+    x = torch.zeros(B, n, dtype=torch.float64, device=dev)
+
+    # ---- plain FP64 GPU CG, one instance (the baseline the mixed/batched path is compared to)
+    op1 = L.MatrixVectorHomogen(grid, st0, precision=64, batch=1)
+    f1 = f[0:1].clone()
+    x1 = torch.zeros_like(f1)
+    L.CG.solveHomogen(1e-8, x1, f1, op1, stop_rule=L.STOP_REL)     # warm-up / autotune
+    x1.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info1 = L.CG.solveHomogen(1e-8, x1, f1, op1, stop_rule=L.STOP_REL)
+    torch.cuda.synchronize()
+    t_fp64 = time.perf_counter() - t0
+    fp64_solves = 1.0 / t_fp64
+
+    # ---- reference FP64 solutions (tight) for the error statement, a few instances
+    nchk = min(B, 4)
+    opc = L.MatrixVectorHomogen(grid, st0, precision=64, batch=nchk)
+    for b in range(nchk):
+        opc.set_coefficient(b, L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[b]))
+    xref = torch.zeros(nchk, n, dtype=torch.float64, device=dev)
+    L.CG.solveHomogen(1e-13, xref, f[:nchk].clone(), opc, stop_rule=L.STOP_REL, maxit=2000)
+
+    # ---- timed: batched mixed-precision solves
+    for _ in range(args.warmup):
+        x.zero_()
+        L.CG.solveMixed(1e-8, x, f, op64, op32, inner_rel_tol=args.inner_tol)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = None
+    for _ in range(args.steps):
+        x.zero_()
+        infos = L.CG.solveMixed(1e-8, x, f, op64, op32, inner_rel_tol=args.inner_tol)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    if B == 1:
+        infos = [infos]
+    assert all(i["converged"] == 1 and i["rel_residual"] <= 1e-8 for i in infos), infos
+    err = float(torch.max(torch.linalg.norm(x[:nchk] - xref, dim=1) / torch.linalg.norm(xref, dim=1)))
+
+    # ---- roofline of the dominant kernel (apply stage, FP64 batched matvec)
+    y = torch.zeros_like(x)
+    op64.multiplication(x, y)
+    reps = 5
+    apply_ms = total_ms = 0.0
+    for _ in range(reps):
+        op64.multiplication(x, y)
+        tm = op64.last_timing()
+        apply_ms += tm["apply_ms"]
+        total_ms += tm["total_ms"]
+    apply_ms /= reps
+    total_ms /= reps
+    apply_bytes = stats["apply_values"] * 8.0 * B
+    achieved = apply_bytes / (apply_ms * 1e-3)
+    stream_bw = stats["stream_values"] * 8.0 * B / (total_ms * 1e-3)
+    tr = apply_traffic_from_profile()
+
+    cpu = None if args.no_cpu_baseline else cpu_baseline(info1["iterations"], args.cpu_budget)
+
+    out = {
+        "metric": "solves/s to 1e-8 rel. residual (d=5 L7) + matvec HBM GB/s vs roofline",
+        "value": B / dt,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dt * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 inner / f64 refinement",
+        "data": "synthetic (seeded PCG64 %d)" % SEED,
+        "config": {"workload": "d=5 regular sparse grid level 7 (N=%d), var. coefficient exp-sine (16 xi), "
+                               "8-term sine RHS, batch %d, FP32 PCG under FP64 IR to 1e-8" % (n, B),
+                   "batch": B, "N": n, "inner_rel_tol": args.inner_tol},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK,
+                     "traffic": None if tr is None else tr * B,
+                     "kernel": "apply_kernel<double> (all %d wavefront launches of one batched matvec)"
+                               % stats["wavefront_levels"],
+                     "algorithmic_bytes": apply_bytes, "time_ms": apply_ms},
+        "cpu_baseline": cpu,
+        "extra": {
+            "fp64_plain_cg_solves_per_s": fp64_solves,
+            "fp64_plain_cg_iterations": info1["iterations"],
+            "speedup_vs_fp64_plain": (B / dt) / fp64_solves,
+            "mixed_inner_iterations_mean": float(np.mean([i["iterations"] for i in infos])),
+            "mixed_outer_sweeps_mean": float(np.mean([i["outer"] for i in infos])),
+            "max_rel_err_vs_fp64_solve": err,
+            "matvec_fp64_batched_ms": total_ms,
+            "matvec_stream_model_GBps": stream_bw / 1e9,
+            "matvec_stream_model_frac": stream_bw / HBM_PEAK,
+            "stream_values_per_matvec": stats["stream_values"],
+            "moved_values_per_matvec": stats["moved_values"],
+        },
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -84,7 +84,8 @@ int lsg_grid_slot_order(const lsg_grid* g, int32_t* idx_out, uint8_t* active_out
 int lsg_grid_device_order(const lsg_grid* g, int64_t* dev_of_slot);
 /* Structural statistics: out[0]=#depths, [1]=multilevel size N_ML, [2]=#apply items,
  * [3]=#wavefront levels, [4]=algorithmic values moved per matvec (streaming model),
- * [5]=largest depth box, [6]=case-buffer size, [7]=W-buffer size. */
+ * [5]=largest depth box, [6]=case-buffer size, [7]=W-buffer size, [8]=part-buffer size,
+ * [9]=values this implementation moves per matvec, [10]=of which the apply stage. */
 int lsg_grid_stats(const lsg_grid* g, int64_t* out, int nout);
 int lsg_grid_destroy(lsg_grid* g);
 

@@ -285,6 +285,7 @@ int buildPlan(Grid& g, std::string& err) {
             std::memcpy(T, it.T, sizeof(T));
             for (int q = 0; q < it.nt; ++q) { T[it.dd] = it.T[it.dd] + q; mv += 2.0 * double(boxCount(T, d)); }
         }
+    const double beforeApply = mv;
     for (auto& v : P.apply)
         for (auto& a : v) {
             int32_t T[MAXD];
@@ -298,6 +299,7 @@ int buildPlan(Grid& g, std::string& err) {
             }
             mv += double(blockCount(T, d));
         }
+    P.applyValues = mv - beforeApply;
     for (auto& s : P.sum) mv += double(blockCount(s.T, d)) * double(s.ncase + 1);
     P.movedValues = mv;
     return 0;

@@ -81,6 +81,7 @@ struct Plan {
     int64_t nApply = 0;
     double streamValues = 0;    // streaming-model values per matvec
     double movedValues = 0;     // values this implementation reads+writes per matvec
+    double applyValues = 0;     // of which the apply stage (input box, gM reads, W writes, output block)
 };
 
 struct Grid {

@@ -290,6 +290,88 @@ int main(int argc, char** argv) {
         runAll(grid, cs, out, what);
         return 0;
     }
+    if (mode == "samplemv" && argc >= 5) {
+        // CPU-baseline sample of one variable-coefficient matvec: the reference's own
+        // multiplication() steps (MatrixVectorHomogen.h:660-749) driven member by
+        // member (this file is compiled with -fno-access-control): calcNodal for all
+        // depths, then cases in a fixed permuted order until the time budget is used.
+        // The full matvec time is estimated by scaling the sampled case time with the
+        // per-case cell counts sum_{T in sorted(c)} prod 2^T_i.
+        int level = std::atoi(argv[2]);
+        CoefSpec cs = parseCoef(argv[3]);
+        double budget = std::atof(argv[4]);
+        const int d = DimensionSparseGrid;
+        auto t0 = std::chrono::steady_clock::now();
+        AdaptiveSparseGrid grid;
+        IndexDimension center;
+        grid.AddRecursiveSonsOfPoint(center, level);
+        MultiLevelAdaptiveSparseGrid mgrid(&grid);
+        MatrixVectorHomogen m(grid, mgrid, true, false);
+        VarCoeffStencil var(grid, cs);
+        auto t1 = std::chrono::steady_clock::now();
+        VectorSparseG prew(grid), Ax(grid);
+        size_t n = grid.getMaximalOccupiedSecondTable();
+        for (size_t i = 0; i < n; ++i) prew.setValue((unsigned long)i, 1.0 / (1.0 + double(i % 7)));
+        auto secs = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+            return std::chrono::duration<double>(b - a).count();
+        };
+        auto a0 = std::chrono::steady_clock::now();
+        Ax = 0.0;
+        m.nodal = 0.0;
+        grid.WorkOnHangingNodes = true;
+#pragma omp parallel
+        {
+            MultiDepthHashGrid mdg = *grid.getMultiDepthHashGrid();
+            VectorSparseG u_nodal(m.u.getSparseGrid());
+            int i = 0;
+            for (auto it = m.depthList.begin_all(); it != m.depthList.end_all(); ++it) {
+                if (i % omp_get_num_threads() == omp_get_thread_num()) {
+                    Depth T = *it;
+                    u_nodal = 0.0;
+                    m.calcNodal(u_nodal, prew, T);
+                    m.nodal.setMultiLevelValuesOMP(u_nodal, T, mdg);
+                }
+                i++;
+            }
+        }
+        grid.WorkOnHangingNodes = false;
+        double tNodal = secs(a0, std::chrono::steady_clock::now());
+        const int ncase = 1 << d;
+        std::vector<double> w(ncase, 0.0);
+        for (int c = 0; c < ncase; ++c) {
+            bool R[DimensionSparseGrid];
+            for (int i = 0; i < d; ++i) R[i] = (c >> i) & 1;
+            m.depthList.sortDepths(R);
+            for (Depth T : *m.depthList.getSortierteTiefen()) {
+                double cells = 1.0;
+                for (int i = 0; i < d; ++i) cells *= double(1 << T.at(i));
+                w[c] += cells;
+            }
+        }
+        double tCases = 0.0, wSampled = 0.0, wAll = 0.0;
+        for (double v : w) wAll += v;
+        std::string done;
+        for (int k = 0; k < ncase && tCases < budget; ++k) {
+            int c = (k * 11 + 0) % ncase;
+            bool R[DimensionSparseGrid];
+            for (int i = 0; i < d; ++i) R[i] = (c >> i) & 1;
+            auto b0 = std::chrono::steady_clock::now();
+            grid.WorkOnHangingNodes = true;
+            m.Ax_neu = 0.0;
+            m.gM = 0.0;
+            m.caseFunction(prew, R, var);
+            grid.WorkOnHangingNodes = false;
+            Ax = Ax + m.Ax_neu;
+            tCases += secs(b0, std::chrono::steady_clock::now());
+            wSampled += w[c];
+            done += (done.empty() ? "" : ",") + std::to_string(c);
+        }
+        double est = tNodal + tCases * (wAll / wSampled);
+        std::printf("{\"n\": %zu, \"setup_s\": %.3f, \"nodal_s\": %.4f, \"timed_s\": %.3f, \"cases\": \"%s\", "
+                    "\"case_weight_fraction\": %.4f, \"matvec_s_estimate\": %.4f}\n",
+                    n, secs(t0, t1), tNodal, tNodal + tCases, done.c_str(), wSampled / wAll, est);
+        return 0;
+    }
     if (mode == "timemv" && argc >= 5) {
         int level = std::atoi(argv[2]);
         CoefSpec cs = parseCoef(argv[3]);
