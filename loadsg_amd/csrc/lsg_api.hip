@@ -115,10 +115,21 @@ struct lsg_op {
     int64_t tabLen = 0;
     bool general = false;
     int64_t maxW = 0;
-    size_t applyLds = 0;
-    int applyThreads = 0;          // 0 = autotune on first use
-    DevMem dNodalItems, dSumItems, dSumList;
-    std::vector<std::unique_ptr<DevMem>> dProl, dApply;
+    // apply launches: per wavefront level, families bucketed by their largest box
+    struct ApplyLaunch {
+        int level, cls;
+        int count;
+        int64_t maxBox;     // largest box of the bucket (LDS box size)
+        int64_t maxW;       // largest weight table of the bucket
+        int G;              // batch instances per workgroup
+        std::unique_ptr<DevMem> fam;
+    };
+    std::vector<ApplyLaunch> launches;
+    int threadsOf[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // per size class; 0 = autotune on first use
+    int nClasses = 0;
+    int64_t maxLds = 0;
+    DevMem dNodalItems, dSumItems, dSumList, dApplyItems;
+    std::vector<std::unique_ptr<DevMem>> dProl;   // per sweep dim
     DevMem coefTab, nodal, cbuf, wbuf, part, xs, ys;
     // solver buffers
     DevMem r, h, dv, g, D, f, x, tmp, partials, st, aux;
@@ -161,12 +172,22 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
     a.wStride = P.wSize;
     a.part = op->part.as<V>();
     a.pStride = P.partSize;
-    a.maxBox = P.maxBox;
     a.st = st;
-    for (size_t L = 0; L < P.apply.size(); ++L) {
-        a.items = op->dApply[L]->as<ApplyItem>();
-        if (launch_apply<V>(a, (int)P.apply[L].size(), B, op->applyThreads, op->applyLds, s) != 0)
-            return fail(LSG_ERR_UNSUPPORTED, "depth box too large for one workgroup");
+    a.items = op->dApplyItems.as<ApplyItem>();
+    a.batch = B;
+    for (auto& L : op->launches) {
+        a.fam = L.fam->as<FamilyItem>();
+        a.maxBox = L.maxBox;
+        a.maxW = L.maxW;
+        a.G = L.G;
+        {
+            const int cb = std::min(10, 64 / g.d);      // a 1D depth t needs t bits (extent 2^t - 1)
+            a.cbits = g.maxLevel <= cb ? cb : 0;
+        }
+        const int thr = op->threadsOf[L.cls] ? op->threadsOf[L.cls] : 256;
+        const size_t lds = ((op->general ? 4 : 3) * L.maxBox + L.maxW) * sizeof(V);
+        if (launch_apply<V>(a, L.count, B, thr, lds, s) != 0)
+            return fail(LSG_ERR_UNSUPPORTED, "depth box too large for the chosen block size");
     }
     HIPCHK(hipEventRecord(op->ev[2], s));
     launch_sum<V>(op->dSumItems.as<SumItem>(), (int)P.sum.size(), op->dSumList.as<int64_t>(), d, op->part.as<V>(),
@@ -182,40 +203,68 @@ int matvecDev(lsg_op* op, const void* x, void* y, const CGState* st) {
     return runMatvec<float>(op, (const float*)x, (float*)y, st);
 }
 
+int64_t classMax(const lsg_op* op, int cls) {
+    int64_t m = 0;
+    for (auto& L : op->launches)
+        if (L.cls == cls) m = std::max(m, L.maxBox);
+    return m;
+}
+
+// LDS bytes of one apply workgroup (one family of one instance)
+int64_t applyLdsBytes(const lsg_op* op, int64_t maxBox, int64_t maxW, int threads) {
+    (void)threads;
+    return ((op->general ? 4 : 3) * maxBox + maxW) * (int64_t)op->vs;
+}
+
+bool threadsFit(const lsg_op* op, int cls, int threads) {
+    for (auto& L : op->launches)
+        if (L.cls == cls && (applyLdsBytes(op, L.maxBox, L.maxW, threads) > op->maxLds ||
+                             (L.maxBox + threads - 1) / threads > 16))
+            return false;
+    return true;
+}
+
 int chooseThreads(lsg_op* op, int threads) {
-    const int64_t mb = op->grid->g.plan.maxBox;
-    int cands[4] = {64, 128, 256, 512};
+    const int cands[4] = {64, 128, 256, 512};
     if (threads) {
-        if ((mb + threads - 1) / threads > 32) return fail(LSG_ERR_UNSUPPORTED, "block too small for the largest box");
-        op->applyThreads = threads;
+        for (int c = 0; c < op->nClasses; ++c) {
+            if (!threadsFit(op, c, threads)) return fail(LSG_ERR_UNSUPPORTED, "apply workgroup exceeds LDS");
+            op->threadsOf[c] = threads;
+        }
         return LSG_OK;
     }
-    // autotune: time the apply stage for each admissible block size (results are
-    // identical for every block size: no reduction crosses threads in this stage)
-    float best = 1e30f;
-    int bestT = 0;
-    for (int t : cands) {
-        if ((mb + t - 1) / t > 32) continue;
-        op->applyThreads = t;
-        float tot = 0.f;
-        for (int rep = 0; rep < 3; ++rep) {
-            int rc = matvecDev(op, op->xs.p, op->ys.p, nullptr);
-            if (rc) return rc;
-            HIPCHK(hipEventSynchronize(op->ev[3]));
-            float ms = 0.f;
-            HIPCHK(hipEventElapsedTime(&ms, op->ev[1], op->ev[2]));
-            if (rep > 0) tot += ms;
+    // autotune each size class: time the apply stage for every admissible number of
+    // waves per workgroup (one wave per batch instance; the other classes keep their
+    // current choice).  Results do not depend on this choice.
+    for (int c = 0; c < op->nClasses; ++c) op->threadsOf[c] = 512;
+    for (int c = 0; c < op->nClasses; ++c) {
+        float best = 1e30f;
+        int bestT = 0;
+        for (int t : cands) {
+            if (!threadsFit(op, c, t)) continue;
+            if (t > 64 && classMax(op, c) <= t / 2) continue;   // half the block covers the largest box
+            op->threadsOf[c] = t;
+            float tot = 0.f;
+            for (int rep = 0; rep < 3; ++rep) {
+                int rc = matvecDev(op, op->xs.p, op->ys.p, nullptr);
+                if (rc) return rc;
+                HIPCHK(hipEventSynchronize(op->ev[3]));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, op->ev[1], op->ev[2]));
+                if (rep > 0) tot += ms;
+            }
+            if (tot < best) { best = tot; bestT = t; }
         }
-        if (tot < best) { best = tot; bestT = t; }
+        if (!bestT) return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS");
+        op->threadsOf[c] = bestT;
     }
-    if (!bestT) return fail(LSG_ERR_UNSUPPORTED, "largest depth box exceeds 16384 points");
-    op->applyThreads = bestT;
     return LSG_OK;
 }
 
 int ensureThreads(lsg_op* op) {
-    if (op->applyThreads) return LSG_OK;
-    return chooseThreads(op, 0);
+    for (int c = 0; c < op->nClasses; ++c)
+        if (!op->threadsOf[c]) return chooseThreads(op, 0);
+    return LSG_OK;
 }
 
 // host slot-order <-> device order (instance-major batch)
@@ -425,22 +474,68 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     if (op->nA + op->nC == 0) return fail(LSG_ERR_ARG, "zero operator");
     op->general = !(op->nA == 1 && op->nC == 0);
     op->tabLen = (int64_t(1) << (g.maxLevel + 1)) - 2;
-    int64_t maxW = 0;
-    for (auto& lev : P.apply)
-        for (auto& a : lev) {
-            int64_t w = 0;
-            for (int i = 0; i < g.d; ++i) w += int64_t(1) << a.T[i];
-            maxW = std::max(maxW, w);
+    if (P.maxBox >= 65536) return fail(LSG_ERR_UNSUPPORTED, "depth boxes above 65535 points are not supported");
+    // size classes of the apply launches (largest box of a family)
+    const int64_t bounds[5] = {128, 512, 2048, 8192, 65536};
+    const int ncls = 5;
+    for (size_t L = 0; L < P.apply.size(); ++L) {
+        std::vector<std::vector<FamilyItem>> bucket(ncls);
+        std::vector<int64_t> bmax(ncls, 0), wmax(ncls, 0);
+        for (const FamilyItem& f : P.apply[L]) {
+            int64_t mb = 0, mw = 0;
+            for (int q = 0; q < f.nt; ++q) {
+                const ApplyItem& a = P.applyItems[f.first + q];
+                mb = std::max(mb, boxCount(a.T, g.d));
+                int64_t w = 0;
+                for (int i = 0; i < g.d; ++i) w += int64_t(1) << a.T[i];
+                mw = std::max(mw, w);
+            }
+            int c = 0;
+            while (c < ncls - 1 && mb > bounds[c]) ++c;
+            bucket[c].push_back(f);
+            bmax[c] = std::max(bmax[c], mb);
+            wmax[c] = std::max(wmax[c], mw);
         }
-    op->maxW = maxW;
-    op->applyLds = ((op->general ? 4 : 2) * P.maxBox + maxW) * op->vs;
+        for (int c = 0; c < ncls; ++c) {
+            if (bucket[c].empty()) continue;
+            lsg_op::ApplyLaunch la;
+            la.level = (int)L;
+            la.cls = c;
+            la.count = (int)bucket[c].size();
+            la.maxBox = bmax[c];
+            la.maxW = wmax[c];
+            // instances per workgroup: about 4096 points of work per sweep, at most
+            // 64 KB of LDS, exact fast division (G * box < 2^16)
+            {
+                const int64_t perInst = ((op->general ? 4 : 3) * la.maxBox + la.maxW) * (int64_t)op->vs;
+                int64_t G = std::max<int64_t>(1, 1024 / la.maxBox);
+                G = std::min<int64_t>(G, std::max<int64_t>(1, (48 * 1024) / perInst));
+                G = std::min<int64_t>(G, 65535 / la.maxBox);
+                G = std::min<int64_t>(G, MAXG);
+                G = std::min<int64_t>(G, batch);
+                la.G = (int)std::max<int64_t>(G, 1);
+            }
+            la.fam.reset(new DevMem());
+            op->launches.push_back(std::move(la));
+            op->nClasses = std::max(op->nClasses, c + 1);
+            auto& dm = *op->launches.back().fam;
+            HIPCHK(dm.alloc(bucket[c].size() * sizeof(FamilyItem)));
+            HIPCHK(hipMemcpy(dm.p, bucket[c].data(), bucket[c].size() * sizeof(FamilyItem), hipMemcpyHostToDevice));
+        }
+    }
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
     int maxLds = 0;
     HIPCHK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    if ((int64_t)op->applyLds > maxLds || (int64_t)(2 * P.maxBox * op->vs) > maxLds)
-        return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS (" + std::to_string(op->applyLds) +
-                                             " > " + std::to_string(maxLds) + " bytes)");
+    op->maxLds = maxLds;
+    for (auto& L : op->launches) {
+        const int64_t lds = applyLdsBytes(op.get(), L.maxBox, L.maxW, 512);
+        if (lds > maxLds)
+            return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS (" + std::to_string(lds) + " > " +
+                                                 std::to_string(maxLds) + " bytes)");
+    }
+    if ((int64_t)(2 * P.maxBox * op->vs) > maxLds)
+        return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS");
     HIPCHK(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
     for (auto& e : op->ev) HIPCHK(hipEventCreate(&e));
     // items
@@ -451,11 +546,10 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         HIPCHK(op->dProl.back()->alloc(v.size() * sizeof(ProlItem)));
         if (!v.empty()) HIPCHK(hipMemcpy(op->dProl.back()->p, v.data(), v.size() * sizeof(ProlItem), hipMemcpyHostToDevice));
     }
-    for (auto& v : P.apply) {
-        op->dApply.emplace_back(new DevMem());
-        HIPCHK(op->dApply.back()->alloc(v.size() * sizeof(ApplyItem)));
-        HIPCHK(hipMemcpy(op->dApply.back()->p, v.data(), v.size() * sizeof(ApplyItem), hipMemcpyHostToDevice));
-    }
+    HIPCHK(op->dApplyItems.alloc(P.applyItems.size() * sizeof(ApplyItem)));
+    HIPCHK(hipMemcpy(op->dApplyItems.p, P.applyItems.data(), P.applyItems.size() * sizeof(ApplyItem),
+                     hipMemcpyHostToDevice));
+
     HIPCHK(op->dSumItems.alloc(P.sum.size() * sizeof(SumItem)));
     HIPCHK(hipMemcpy(op->dSumItems.p, P.sum.data(), P.sum.size() * sizeof(SumItem), hipMemcpyHostToDevice));
     HIPCHK(op->dSumList.alloc(P.sumList.size() * sizeof(int64_t)));
@@ -661,7 +755,7 @@ int lsg_last_timing(const lsg_op* op, double* out, int nout) {
     float tot = 0.f, ap = 0.f;
     HIPCHK(hipEventElapsedTime(&tot, op->ev[0], op->ev[3]));
     HIPCHK(hipEventElapsedTime(&ap, op->ev[1], op->ev[2]));
-    double v[3] = {tot, ap, (double)op->grid->g.plan.apply.size()};
+    double v[3] = {tot, ap, (double)op->launches.size()};
     for (int i = 0; i < nout && i < 3; ++i) out[i] = v[i];
     return LSG_OK;
 }

@@ -15,33 +15,237 @@
 namespace lsg {
 
 // ---------------------------------------------------------------- box shapes
+// Box shape shared by the workgroup (in LDS; thread 0 updates it between barriers).
+// Integer division by an extent or stride uses the float reciprocal: exact for
+// dividends below 2^16 (boxes are limited to 65535 points by the host), since the
+// quotient's fractional part (r + 0.5)/n is at least 0.5/n away from an integer.
 struct Shape {
     int d;
-    int n[MAXD];    // extent per dim
-    int S[MAXD];    // stride per dim (dim 0 fastest)
     int total;
-    __device__ void box(const int32_t* T, int d_) {
-        d = d_;
-        int s = 1;
-        for (int i = 0; i < d; ++i) { n[i] = (1 << T[i]) - 1; S[i] = s; s *= n[i]; }
-        total = s;
-    }
-    __device__ void restage() {
-        int s = 1;
-        for (int i = 0; i < d; ++i) { S[i] = s; s *= n[i]; }
-        total = s;
-    }
+    int n[MAXD];      // extent per dim
+    int S[MAXD];      // stride per dim (dim 0 fastest)
+    float rn[MAXD];   // 1/n
+    float rS[MAXD];   // 1/S
 };
+
+__device__ inline int fdiv(int p, float r) { return __float2int_rz((float(p) + 0.5f) * r); }
+
+__device__ inline void shape_restage(Shape& sh) {
+    int s = 1;
+    for (int q = 0; q < sh.d; ++q) {
+        sh.S[q] = s;
+        sh.rS[q] = 1.0f / float(s);
+        sh.rn[q] = 1.0f / float(sh.n[q]);
+        s *= sh.n[q];
+    }
+    sh.total = s;
+}
+__device__ inline void shape_box(Shape& sh, const int32_t* T, int d) {
+    if (threadIdx.x == 0) {
+        sh.d = d;
+        for (int i = 0; i < d; ++i) sh.n[i] = (1 << T[i]) - 1;
+        shape_restage(sh);
+    }
+    __syncthreads();
+}
+__device__ inline void shape_set_extent(Shape& sh, int i, int n) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        sh.n[i] = n;
+        shape_restage(sh);
+    }
+    __syncthreads();
+}
+// coordinate of point p along dim i
+__device__ inline int coord(const Shape& sh, int p, int i) {
+    int q = fdiv(p, sh.rS[i]);
+    return q - fdiv(q, sh.rn[i]) * sh.n[i];
+}
 
 template <typename V>
 __device__ inline V cCenter(int t, int k, int n) {      // 0.9 next to the boundary (index.cc:312-316)
     return (t >= 2 && (k == 1 || k == n)) ? V(0.9) : V(1.0);
 }
 
+template <typename V>
+__device__ inline void copy_in(V* __restrict__ dst, const V* __restrict__ src, int cnt) {
+    for (int p = threadIdx.x; p < cnt; p += blockDim.x) dst[p] = src[p];
+}
+
+// -------------------------------------------------------- point sweeps
+// One output point per thread and step; all reads of a step are independent LDS
+// loads.  (lo, k, hi) decomposition of an output index along the swept dim uses the
+// exact reciprocal division above.
+struct Split {
+    int lo, k, hi;
+};
+__device__ inline Split split(int p, int Slo, float rSlo, int n, float rn) {
+    Split s;
+    int q = fdiv(p, rSlo);
+    s.lo = p - q * Slo;
+    s.hi = fdiv(q, rn);
+    s.k = q - s.hi * n;
+    return s;
+}
+
+// 1D prewavelet -> nodal along one dim (calcNodal, MatrixVectorHomogen.cpp:43-74, with
+// the 5^d stencil of nextFiveP, indices/index.cc:252-327): m = 2^(t-1) coefficients at
+// the odd positions -> n = 2^t - 1 nodal values.
+template <typename V>
+__device__ inline void expand_dim(const V* __restrict__ src, V* __restrict__ dst, int Slo, int m, int n, int outCnt,
+                                  int t) {
+    const float rSlo = 1.0f / float(Slo), rn = 1.0f / float(n);
+    for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
+        Split s = split(p, Slo, rSlo, n, rn);
+        const V* b = src + s.lo + s.hi * Slo * m;
+        const int k = s.k + 1;      // 1-based position
+        V v;
+        if (k & 1) {
+            const int j = s.k >> 1;
+            v = cCenter<V>(t, k, n) * b[j * Slo];
+            if (j >= 1) v += V(0.1) * b[(j - 1) * Slo];
+            if (j + 1 < m) v += V(0.1) * b[(j + 1) * Slo];
+        } else {
+            const int j = (k >> 1) - 1;
+            v = V(-0.6) * (b[j * Slo] + b[(j + 1) * Slo]);
+        }
+        dst[p] = v;
+    }
+}
+
+// ConvertToPrewavelet2 along one dim (MatrixVectorHomogen.h:540-562): n nodal-dual
+// values -> m = (n+1)/2 prewavelet values at the odd positions.
+template <typename V>
+__device__ inline void contract_dim(const V* __restrict__ src, V* __restrict__ dst, int Slo, int n, int outCnt,
+                                    int t) {
+    const int m = (n + 1) >> 1;
+    const float rSlo = 1.0f / float(Slo), rm = 1.0f / float(m);
+    for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
+        Split s = split(p, Slo, rSlo, m, rm);
+        const V* b = src + s.lo + s.hi * Slo * n;
+        const int k = 2 * s.k + 1;            // 1-based position, 0-based index k-1
+        V v = cCenter<V>(t, k, n) * b[(k - 1) * Slo];
+        if (k - 1 >= 1) v += V(-0.6) * b[(k - 2) * Slo] + V(0.1) * b[(k - 3) * Slo];
+        if (k + 1 <= n) v += V(-0.6) * b[k * Slo] + V(0.1) * b[(k + 1) * Slo];
+        dst[p] = v;
+    }
+}
+
+// restriction1D_inverted_inplace along r (MatrixVectorHomogen.h:485-530): coarse
+// position j gets z(2j) + 0.5 (z(2j-1) + z(2j+1)); src box -> dst box.
+template <typename V>
+__device__ inline void restrict_dim(const V* __restrict__ src, V* __restrict__ dst, const Shape& sh, int r) {
+    const int nf = sh.n[r], nc = (nf - 1) >> 1, S = sh.S[r];
+    const int outCnt = sh.total / nf * nc;
+    const float rS = sh.rS[r], rnc = 1.0f / float(nc);
+    for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
+        Split s = split(p, S, rS, nc, rnc);
+        const V* b = src + s.lo + s.hi * S * nf + (2 * s.k + 1) * S;
+        dst[p] = b[0] + V(0.5) * (b[-S] + b[S]);
+    }
+}
+
+// prolongation1D_inplace + addMultiLevelValues along fd (MatrixVectorHomogen.h:444-474,
+// 997-1019): cur (fine along fd) += I_fd(prev) (coarse along fd); sh = fine shape.
+template <typename V>
+__device__ inline void prolong_dim(V* __restrict__ cur, const V* __restrict__ prev, const Shape& sh, int fd) {
+    const int nf = sh.n[fd], nc = (nf - 1) >> 1, S = sh.S[fd];
+    const float rS = sh.rS[fd], rnf = sh.rn[fd];
+    for (int p = threadIdx.x; p < sh.total; p += blockDim.x) {
+        Split s = split(p, S, rS, nf, rnf);
+        const V* c = prev + s.lo + s.hi * S * nc;
+        const int k = s.k + 1;                // fine position
+        V add;
+        if (k & 1) {
+            const int cl = (k - 1) >> 1, cr = (k + 1) >> 1;   // coarse positions 0..nc+1
+            V a = (cl >= 1) ? c[(cl - 1) * S] : V(0);
+            V b = (cr <= nc) ? c[(cr - 1) * S] : V(0);
+            add = V(0.5) * (a + b);
+        } else {
+            add = c[((k >> 1) - 1) * S];
+        }
+        cur[p] += add;
+    }
+}
+
+// Weighted 1D Q1 operators at level t on one line (positions 1..n, cell c between
+// positions c and c+1 carries weight w[c]):
+//   (K v)_k = 2^t  [ w_{k-1}(v_k - v_{k-1}) + w_k (v_k - v_{k+1}) ]
+//   (M v)_k = 2^-t/6 [ w_{k-1}(v_{k-1} + 2 v_k) + w_k (2 v_k + v_{k+1}) ]
+// For a separable coefficient a = prod_j g_j(x_j) the element matrices of
+// Poisson::integration (Stencil.h:1707-1789) times a(cell centre) sum to
+// sum_i K_i (x) M_{j != i}, applied by the Horner scheme
+//   Z <- M_i Z + K_i Y,  Y <- M_i Y   for i = d-1 .. 0.
+template <typename V, int PPT>
+__device__ inline void horner_step(V* __restrict__ Y, V* __restrict__ Z, const Shape& sh, int i,
+                                   const V* __restrict__ w, int t, bool zeroZ, bool needY) {
+    const int S = sh.S[i], n = sh.n[i], total = sh.total;
+    const float rS = sh.rS[i], rn = sh.rn[i];
+    const V ih = V(1 << t);
+    const V h6 = V(1.0) / (V(6.0) * V(1 << t));
+    V ny[PPT], nz[PPT];
+#pragma unroll
+    for (int s = 0; s < PPT; ++s) {
+        const int p = threadIdx.x + s * blockDim.x;
+        if (p < total) {
+            const int q = fdiv(p, rS);
+            const int kq = q - fdiv(q, rn) * n;
+            const bool hasL = kq > 0, hasR = kq < n - 1;
+            const V yc = Y[p];
+            const V yl = hasL ? Y[p - S] : V(0);
+            const V yr = hasR ? Y[p + S] : V(0);
+            const V wl = w[kq], wr = w[kq + 1];
+            ny[s] = h6 * (wl * (yl + V(2) * yc) + wr * (V(2) * yc + yr));
+            V out = ih * (wl * (yc - yl) + wr * (yc - yr));
+            if (!zeroZ) {
+                const V zc = Z[p];
+                const V zl = hasL ? Z[p - S] : V(0);
+                const V zr = hasR ? Z[p + S] : V(0);
+                out += h6 * (wl * (zl + V(2) * zc) + wr * (V(2) * zc + zr));
+            }
+            nz[s] = out;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < PPT; ++s) {
+        const int p = threadIdx.x + s * blockDim.x;
+        if (p < total) {
+            if (needY) Y[p] = ny[s];
+            Z[p] = nz[s];
+        }
+    }
+    __syncthreads();
+}
+
+template <typename V, int PPT>
+__device__ inline void mass_step(V* __restrict__ Y, const Shape& sh, int i, const V* __restrict__ w, int t) {
+    const int S = sh.S[i], n = sh.n[i], total = sh.total;
+    const float rS = sh.rS[i], rn = sh.rn[i];
+    const V h6 = V(1.0) / (V(6.0) * V(1 << t));
+    V ny[PPT];
+#pragma unroll
+    for (int s = 0; s < PPT; ++s) {
+        const int p = threadIdx.x + s * blockDim.x;
+        if (p < total) {
+            const int q = fdiv(p, rS);
+            const int kq = q - fdiv(q, rn) * n;
+            const V yc = Y[p];
+            const V yl = kq > 0 ? Y[p - S] : V(0);
+            const V yr = kq < n - 1 ? Y[p + S] : V(0);
+            ny[s] = h6 * (w[kq] * (yl + V(2) * yc) + w[kq + 1] * (V(2) * yc + yr));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < PPT; ++s) {
+        const int p = threadIdx.x + s * blockDim.x;
+        if (p < total) Y[p] = ny[s];
+    }
+    __syncthreads();
+}
+
 // -------------------------------------------------------------- calcNodal
-// nodal box of depth T = tensor product of the 1D prewavelet->nodal maps applied
-// to the depth-T block (calcNodal, MatrixVectorHomogen.cpp:43-74, with the 5^d
-// stencil of nextFiveP, indices/index.cc:252-327, applied dimension by dimension).
 template <typename V>
 __global__ void __launch_bounds__(256) nodal_kernel(const NodalItem* __restrict__ items, int d,
                                                     const V* __restrict__ x, int64_t xStride,
@@ -55,42 +259,20 @@ __global__ void __launch_bounds__(256) nodal_kernel(const NodalItem* __restrict_
     const NodalItem& it = items[blockIdx.x];
     const V* xin = x + inst * xStride + it.in_off;
     V* out = nodal + inst * nStride + it.out_off;
-    // layout: dims < i expanded (2^t - 1), dims >= i still block (2^(t-1))
-    int ext[MAXD];
     int cnt = 1;
-    for (int i = 0; i < d; ++i) { ext[i] = 1 << (it.T[i] - 1); cnt *= ext[i]; }
-    for (int p = threadIdx.x; p < cnt; p += blockDim.x) A[p] = xin[p];
+    for (int i = 0; i < d; ++i) cnt *= 1 << (it.T[i] - 1);
+    copy_in(A, xin, cnt);
     __syncthreads();
     V* src = A;
     V* dst = B;
+    int Slo = 1;
     for (int i = 0; i < d; ++i) {
         const int t = it.T[i];
-        const int m = ext[i];
-        const int nb = (1 << t) - 1;
-        int Slo = 1;
-        for (int q = 0; q < i; ++q) Slo *= ext[q];
-        const int outCnt = cnt / m * nb;
-        for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
-            int lo = p % Slo;
-            int rest = p / Slo;
-            int kq = rest % nb;
-            int hi = rest / nb;
-            int k = kq + 1;
-            const V* s = src + lo + Slo * (m * hi);
-            V v;
-            if (k & 1) {
-                int j = kq >> 1;
-                v = cCenter<V>(t, k, nb) * s[Slo * j];
-                if (j >= 1) v += V(0.1) * s[Slo * (j - 1)];
-                if (j + 1 < m) v += V(0.1) * s[Slo * (j + 1)];
-            } else {
-                int j = (k >> 1) - 1;
-                v = V(-0.6) * (s[Slo * j] + s[Slo * (j + 1)]);
-            }
-            dst[p] = v;
-        }
-        ext[i] = nb;
+        const int m = 1 << (t - 1), n = (1 << t) - 1;
+        const int outCnt = cnt / m * n;
+        expand_dim<V>(src, dst, Slo, m, n, outCnt, t);
         cnt = outCnt;
+        Slo *= n;
         __syncthreads();
         V* tmp = src; src = dst; dst = tmp;
     }
@@ -108,6 +290,7 @@ __global__ void __launch_bounds__(256) prol_kernel(const ProlItem* __restrict__ 
     extern __shared__ char smem_raw[];
     V* prev = reinterpret_cast<V*>(smem_raw);
     V* cur = prev + maxBox;
+    __shared__ Shape sh;
     const int inst = blockIdx.y;
     if (st && !st[inst].active) return;
     const ProlItem& it = items[blockIdx.x];
@@ -115,266 +298,144 @@ __global__ void __launch_bounds__(256) prol_kernel(const ProlItem* __restrict__ 
     const V* inBase = it.from_nodal ? nodal + inst * nStride : cbuf + inst * cStride;
     V* outBase = cbuf + inst * cStride;
     int32_t T[MAXD];
-    for (int i = 0; i < d; ++i) T[i] = it.T[i];
-    int Slo = 1, Shi = 1;
-    for (int i = 0; i < d; ++i) {
-        if (i < dd) Slo *= (1 << T[i]) - 1;
-        else if (i > dd) Shi *= (1 << T[i]) - 1;
-    }
+    for (int i = 0; i < MAXD; ++i) T[i] = it.T[i];
     for (int q = 0; q < it.nt; ++q) {
-        const int t = T[dd] + q;
-        const int nf = (1 << t) - 1;
-        const int nc = (1 << (t - 1)) - 1;
-        const int cnt = Slo * nf * Shi;
-        const V* in = inBase + it.in_off[q];
-        V* out = outBase + it.out_off[q];
-        for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
-            V v = in[p];
-            if (q > 0) {
-                int lo = p % Slo;
-                int rest = p / Slo;
-                int kq = rest % nf;
-                int hi = rest / nf;
-                int k = kq + 1;                     // fine position
-                const V* c = prev + lo + Slo * (nc * hi);
-                if (k & 1) {
-                    int cl = (k - 1) >> 1, cr = (k + 1) >> 1;   // coarse positions 0..nc+1
-                    V a = (cl >= 1) ? c[Slo * (cl - 1)] : V(0);
-                    V b = (cr <= nc) ? c[Slo * (cr - 1)] : V(0);
-                    v += V(0.5) * (a + b);
-                } else {
-                    v += c[Slo * ((k >> 1) - 1)];
-                }
-            }
-            cur[p] = v;
-            out[p] = v;
+        T[dd] = it.T[dd] + q;
+        shape_box(sh, T, d);
+        const int cnt = sh.total;
+        copy_in(cur, inBase + it.in_off[q], cnt);
+        if (q > 0) {
+            __syncthreads();
+            prolong_dim<V>(cur, prev, sh, dd);
         }
         __syncthreads();
+        V* out = outBase + it.out_off[q];
+        for (int p = threadIdx.x; p < cnt; p += blockDim.x) out[p] = cur[p];
         V* tmp = prev; prev = cur; cur = tmp;
+        __syncthreads();
     }
 }
 
 // ------------------------------------------------------------------ apply
-// Weighted 1D Q1 operators at level t on one line (positions 1..n, cell c between
-// positions c and c+1 carries weight w[c]):
-//   (K v)_k = 2^t  [ w_{k-1}(v_k - v_{k-1}) + w_k (v_k - v_{k+1}) ]
-//   (M v)_k = 2^-t/6 [ w_{k-1}(v_{k-1} + 2 v_k) + w_k (2 v_k + v_{k+1}) ]
-// For a separable coefficient a = prod_j g_j(x_j) the element matrices of
-// Poisson::integration (Stencil.h:1707-1789) times a(cell centre) sum to
-// sum_i K_i (x) M_{j != i}, applied here by the Horner scheme
-//   Z <- M_i Z + K_i Y,  Y <- M_i Y   for i = d-1 .. 0.
-template <typename V, int PPT>
-__device__ inline void horner_step(V* __restrict__ Y, V* __restrict__ Z, const Shape& sh, int i,
-                                   const V* __restrict__ w, int t, bool zeroZ, bool needY) {
-    const int S = sh.S[i], n = sh.n[i];
-    const V ih = V(1 << t);
-    const V h6 = V(1.0) / (V(6.0) * V(1 << t));
-    V ny[PPT], nz[PPT];
-#pragma unroll
-    for (int s = 0; s < PPT; ++s) {
-        int p = threadIdx.x + s * blockDim.x;
-        if (p < sh.total) {
-            int kq = (p / S) % n;
-            V yc = Y[p];
-            V yl = kq > 0 ? Y[p - S] : V(0);
-            V yr = kq < n - 1 ? Y[p + S] : V(0);
-            V wl = w[kq], wr = w[kq + 1];
-            V My = h6 * (wl * (yl + V(2) * yc) + wr * (V(2) * yc + yr));
-            V Ky = ih * (wl * (yc - yl) + wr * (yc - yr));
-            V Mz = V(0);
-            if (!zeroZ) {
-                V zc = Z[p];
-                V zl = kq > 0 ? Z[p - S] : V(0);
-                V zr = kq < n - 1 ? Z[p + S] : V(0);
-                Mz = h6 * (wl * (zl + V(2) * zc) + wr * (V(2) * zc + zr));
-            }
-            ny[s] = My;
-            nz[s] = Mz + Ky;
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < PPT; ++s) {
-        int p = threadIdx.x + s * blockDim.x;
-        if (p < sh.total) {
-            if (needY) Y[p] = ny[s];
-            Z[p] = nz[s];
-        }
-    }
-    __syncthreads();
-}
-
-template <typename V, int PPT>
-__device__ inline void mass_step(V* __restrict__ Y, const Shape& sh, int i, const V* __restrict__ w, int t) {
-    const int S = sh.S[i], n = sh.n[i];
-    const V h6 = V(1.0) / (V(6.0) * V(1 << t));
-    V ny[PPT];
-#pragma unroll
-    for (int s = 0; s < PPT; ++s) {
-        int p = threadIdx.x + s * blockDim.x;
-        if (p < sh.total) {
-            int kq = (p / S) % n;
-            V yc = Y[p];
-            V yl = kq > 0 ? Y[p - S] : V(0);
-            V yr = kq < n - 1 ? Y[p + S] : V(0);
-            ny[s] = h6 * (w[kq] * (yl + V(2) * yc) + w[kq + 1] * (V(2) * yc + yr));
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < PPT; ++s) {
-        int p = threadIdx.x + s * blockDim.x;
-        if (p < sh.total) Y[p] = ny[s];
-    }
-    __syncthreads();
-}
-
-// weights of one separable factor for the item's depth: dim i, level T_i,
+// weights of one separable factor for the box's depth: dim i, level T_i,
 // 2^T_i cell-centre values, copied from the instance's 1D tables into LDS.
 template <typename V>
 __device__ inline void load_weights(V* __restrict__ W, int* woff, const V* __restrict__ tab, int64_t tabLen,
                                     const int32_t* T, int d) {
     int o = 0;
     for (int i = 0; i < d; ++i) {
-        woff[i] = o;
+        if (threadIdx.x == 0) woff[i] = o;
+        const V* src = tab + i * tabLen + ((1 << T[i]) - 2);
+        for (int c = threadIdx.x; c < (1 << T[i]); c += blockDim.x) W[o + c] = src[c];
         o += 1 << T[i];
     }
-    for (int i = 0; i < d; ++i) {
-        const V* src = tab + i * tabLen + ((1 << T[i]) - 2);
-        for (int c = threadIdx.x; c < (1 << T[i]); c += blockDim.x) W[woff[i] + c] = src[c];
-    }
     __syncthreads();
 }
 
-// restriction1D_inverted_inplace along r (MatrixVectorHomogen.h:485-530): coarse
-// position j gets z(2j) + 0.5 (z(2j-1) + z(2j+1)); src box -> dst box.
-template <typename V>
-__device__ inline void restrict_dim(const V* __restrict__ src, V* __restrict__ dst, Shape& sh, int r) {
-    const int nf = sh.n[r];
-    const int nc = (nf - 1) >> 1;
-    const int Slo = sh.S[r];
-    const int outCnt = sh.total / nf * nc;
-    for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
-        int lo = p % Slo;
-        int rest = p / Slo;
-        int jc = rest % nc;
-        int hi = rest / nc;
-        const V* s = src + lo + Slo * (nf * hi);
-        int f = 2 * jc + 1;   // 0-based fine index of the coarse point
-        dst[p] = s[Slo * f] + V(0.5) * (s[Slo * (f - 1)] + s[Slo * (f + 1)]);
-    }
-    sh.n[r] = nc;
-    sh.restage();
-    __syncthreads();
-}
-
-// ConvertToPrewavelet2 (MatrixVectorHomogen.h:540-562): Ax_p = sum_J c_J z_J over
-// the 5^d prewavelet stencil of p, applied as d contractions box -> block.
-template <typename V>
-__device__ inline V* convert_prew(V* src, V* dst, Shape& sh, const int32_t* T) {
-    int ext[MAXD];
-    int cnt = sh.total;
-    for (int i = 0; i < sh.d; ++i) ext[i] = sh.n[i];
-    for (int i = 0; i < sh.d; ++i) {
-        const int t = T[i];
-        const int nb = ext[i];
-        const int m = (nb + 1) >> 1;
-        int Slo = 1;
-        for (int q = 0; q < i; ++q) Slo *= ext[q];
-        const int outCnt = cnt / nb * m;
-        for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
-            int lo = p % Slo;
-            int rest = p / Slo;
-            int j = rest % m;
-            int hi = rest / m;
-            const V* s = src + lo + Slo * (nb * hi);
-            int k = 2 * j + 1;        // 1-based position; 0-based index k-1
-            V v = cCenter<V>(t, k, nb) * s[Slo * (k - 1)];
-            if (k - 1 >= 1) v += V(-0.6) * s[Slo * (k - 2)] + V(0.1) * s[Slo * (k - 3)];
-            if (k + 1 <= nb) v += V(-0.6) * s[Slo * k] + V(0.1) * s[Slo * (k + 1)];
-            dst[p] = v;
-        }
-        ext[i] = m;
-        cnt = outCnt;
-        __syncthreads();
-        V* tmp = src; src = dst; dst = tmp;
-    }
-    sh.total = cnt;
-    return src;
-}
-
+// One family (T, T+e_fd, ...) of one case: for each box, P = input (+ I_fd(previous
+// P)), then z = A_T P, the gM exchanges along the restriction dims and the
+// prewavelet conversion into the case-partial output (caseFunction,
+// MatrixVectorHomogen.h:965-1090).  LDS: U (P of the previous box), Y, Z
+// [, Zt for a sum of terms], weights; each of the box arrays holds a.maxBox values.
 template <typename V, int PPT>
 __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
     extern __shared__ char smem_raw[];
-    V* Y = reinterpret_cast<V*>(smem_raw);
+    V* U = reinterpret_cast<V*>(smem_raw);
+    V* Y = U + a.maxBox;
     V* Z = Y + a.maxBox;
-    V* U = Z + a.maxBox;                      // general path only
-    V* Zt = U + a.maxBox;                     // general path only
-    V* Wt = a.general ? Zt + a.maxBox : U;    // weights
+    V* Zt = Z + a.maxBox;                     // general path only
+    V* Wt = a.general ? Zt + a.maxBox : Zt;   // weights
     __shared__ int woff[MAXD];
+    __shared__ Shape sh;
     const int inst = blockIdx.y;
     if (a.st && !a.st[inst].active) return;
-    const ApplyItem& it = a.items[blockIdx.x];
+    const FamilyItem fam = a.fam[blockIdx.x];
     const int d = a.d;
-    Shape sh;
-    sh.box(it.T, d);
-    const V* in = (it.in_nodal ? a.nodal + inst * a.nStride : a.cbuf + inst * a.cStride) + it.in_off;
     const V* coef = a.coefTab + inst * a.coefStride;
-
-    if (!a.general) {
-        // single separable stiffness term: Y = u, Z = A_T u
-        for (int p = threadIdx.x; p < sh.total; p += blockDim.x) Y[p] = in[p];
-        load_weights<V>(Wt, woff, coef, a.tabLen, it.T, d);
-        for (int i = d - 1; i >= 0; --i)
-            horner_step<V, PPT>(Y, Z, sh, i, Wt + woff[i], it.T[i], i == d - 1, i > 0);
-    } else {
-        for (int p = threadIdx.x; p < sh.total; p += blockDim.x) { U[p] = in[p]; Z[p] = V(0); }
-        __syncthreads();
-        for (int m = 0; m < a.nA + a.nC; ++m) {
-            load_weights<V>(Wt, woff, coef + (int64_t)m * d * a.tabLen, a.tabLen, it.T, d);
-            for (int p = threadIdx.x; p < sh.total; p += blockDim.x) Y[p] = U[p];
-            __syncthreads();
-            if (m < a.nA) {
-                for (int i = d - 1; i >= 0; --i)
-                    horner_step<V, PPT>(Y, Zt, sh, i, Wt + woff[i], it.T[i], i == d - 1, i > 0);
-                for (int p = threadIdx.x; p < sh.total; p += blockDim.x) Z[p] += Zt[p];
-            } else {
-                for (int i = d - 1; i >= 0; --i) mass_step<V, PPT>(Y, sh, i, Wt + woff[i], it.T[i]);
-                for (int p = threadIdx.x; p < sh.total; p += blockDim.x) Z[p] += Y[p];
-            }
-            __syncthreads();
-        }
-    }
-
     V* wbase = a.wbuf + inst * a.wStride;
-    V* cur = Z;
-    V* oth = Y;
-    if (it.rd_off[0] >= 0) {
-        const V* g = wbase + it.rd_off[0];
-        for (int p = threadIdx.x; p < sh.total; p += blockDim.x) cur[p] += g[p];
+
+    for (int q = 0; q < fam.nt; ++q) {
+        const ApplyItem& it = a.items[fam.first + q];
+        shape_box(sh, it.T, d);
+        const V* in = (it.in_nodal ? a.nodal + inst * a.nStride : a.cbuf + inst * a.cStride) + it.in_off;
+        copy_in(Y, in, sh.total);
+        if (q > 0) {
+            __syncthreads();
+            prolong_dim<V>(Y, U, sh, fam.fd);
+        }
         __syncthreads();
-    }
-    for (int j = 1; j <= it.k; ++j) {
-        restrict_dim<V>(cur, oth, sh, it.r[j - 1]);
-        V* tmp = cur; cur = oth; oth = tmp;
-        const bool w = it.w_off[j] >= 0, rd = it.rd_off[j] >= 0;
-        if (w || rd) {
-            V* wd = w ? wbase + it.w_off[j] : nullptr;
-            const V* g = rd ? wbase + it.rd_off[j] : nullptr;
-            for (int p = threadIdx.x; p < sh.total; p += blockDim.x) {
-                V v = cur[p];
-                if (w) wd[p] = v;
-                if (rd) cur[p] = v + g[p];
-            }
+        if (q + 1 < fam.nt || a.general) {        // keep P for the next box / the next term
+            copy_in(U, Y, sh.total);
             __syncthreads();
         }
+        // stencil: Z = A_T P
+        if (!a.general) {
+            load_weights<V>(Wt, woff, coef, a.tabLen, it.T, d);
+            for (int i = d - 1; i >= 0; --i)
+                horner_step<V, PPT>(Y, Z, sh, i, Wt + woff[i], it.T[i], i == d - 1, i > 0);
+        } else {
+            for (int p = threadIdx.x; p < sh.total; p += blockDim.x) Z[p] = V(0);
+            for (int m = 0; m < a.nA + a.nC; ++m) {
+                load_weights<V>(Wt, woff, coef + (int64_t)m * d * a.tabLen, a.tabLen, it.T, d);
+                if (m > 0) {
+                    copy_in(Y, U, sh.total);
+                    __syncthreads();
+                }
+                if (m < a.nA) {
+                    for (int i = d - 1; i >= 0; --i)
+                        horner_step<V, PPT>(Y, Zt, sh, i, Wt + woff[i], it.T[i], i == d - 1, i > 0);
+                    for (int p = threadIdx.x; p < sh.total; p += blockDim.x) Z[p] += Zt[p];
+                } else {
+                    for (int i = d - 1; i >= 0; --i) mass_step<V, PPT>(Y, sh, i, Wt + woff[i], it.T[i]);
+                    for (int p = threadIdx.x; p < sh.total; p += blockDim.x) Z[p] += Y[p];
+                }
+                __syncthreads();
+            }
+        }
+        // gM exchanges (restriction dims ascending)
+        V* cur = Z;
+        V* oth = Y;
+        if (it.rd_off[0] >= 0) {
+            const V* g = wbase + it.rd_off[0];
+            for (int p = threadIdx.x; p < sh.total; p += blockDim.x) cur[p] += g[p];
+            __syncthreads();
+        }
+        for (int j = 1; j <= it.k; ++j) {
+            const int r = it.r[j - 1];
+            restrict_dim<V>(cur, oth, sh, r);
+            shape_set_extent(sh, r, (sh.n[r] - 1) >> 1);
+            V* tmp = cur; cur = oth; oth = tmp;
+            const bool w = it.w_off[j] >= 0, rd = it.rd_off[j] >= 0;
+            if (w || rd) {
+                V* wd = w ? wbase + it.w_off[j] : nullptr;
+                const V* g = rd ? wbase + it.rd_off[j] : nullptr;
+                for (int p = threadIdx.x; p < sh.total; p += blockDim.x) {
+                    V v = cur[p];
+                    if (w) wd[p] = v;
+                    if (rd) cur[p] = v + g[p];
+                }
+                __syncthreads();
+            }
+        }
+        // prewavelet conversion, d contractions box -> block
+        int cnt = sh.total;
+        int Slo = 1;
+        for (int i = 0; i < d; ++i) {
+            const int n = sh.n[i];
+            int t = it.T[i];
+            for (int j = 0; j < it.k; ++j) if (it.r[j] == i) t -= 1;
+            const int m = (n + 1) >> 1;
+            const int outCnt = cnt / n * m;
+            contract_dim<V>(cur, oth, Slo, n, outCnt, t);
+            cnt = outCnt;
+            Slo *= m;
+            __syncthreads();
+            V* tmp = cur; cur = oth; oth = tmp;
+        }
+        V* out = a.part + inst * a.pStride + it.out_off;
+        for (int p = threadIdx.x; p < cnt; p += blockDim.x) out[p] = cur[p];
+        __syncthreads();
     }
-    int32_t Tc[MAXD];
-    for (int i = 0; i < d; ++i) Tc[i] = it.T[i];
-    for (int j = 0; j < it.k; ++j) Tc[it.r[j]] -= 1;
-    V* res = convert_prew<V>(cur, oth, sh, Tc);
-    V* out = a.part + inst * a.pStride + it.out_off;
-    for (int p = threadIdx.x; p < sh.total; p += blockDim.x) out[p] = res[p];
 }
 
 // ------------------------------------------------------------------- sum
@@ -726,17 +787,16 @@ int launch_apply(const ApplyArgs<V>& a, int nitems, int batch, int threads, size
     if (nitems == 0) return 0;
     const int64_t ppt = (a.maxBox + threads - 1) / threads;
     dim3 grid(nitems, batch), blk(threads);
-#define LSG_APPLY_CASE(P)                                                                     \
-    if (ppt <= P) {                                                                           \
-        hipLaunchKernelGGL((apply_kernel<V, P>), grid, blk, lds, s, a);                       \
-        return 0;                                                                             \
+#define LSG_APPLY_CASE(P)                                               \
+    if (ppt <= P) {                                                     \
+        hipLaunchKernelGGL((apply_kernel<V, P>), grid, blk, lds, s, a); \
+        return 0;                                                       \
     }
     LSG_APPLY_CASE(1)
     LSG_APPLY_CASE(2)
     LSG_APPLY_CASE(4)
     LSG_APPLY_CASE(8)
     LSG_APPLY_CASE(16)
-    LSG_APPLY_CASE(32)
 #undef LSG_APPLY_CASE
     return -1;
 }

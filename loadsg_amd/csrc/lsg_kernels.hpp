@@ -6,6 +6,9 @@
 
 namespace lsg {
 
+constexpr int MAXG = 64;   // most batch instances one apply workgroup handles
+constexpr int WSHAPE_BYTES = 176;   // per-wave box-shape block in LDS (apply kernel)
+
 // Per-instance CG state, device resident (scalars of CG::solveHomogen,
 // applications/cg_method.h:763-883).
 struct CGState {
@@ -15,7 +18,8 @@ struct CGState {
 
 template <typename V>
 struct ApplyArgs {
-    const ApplyItem* items;
+    const FamilyItem* fam;    // families of one wavefront level
+    const ApplyItem* items;   // all apply items
     int d;
     int general;          // 0: one separable stiffness term (2 LDS boxes), 1: general sum of terms
     int nA, nC;           // stiffness / mass terms
@@ -30,7 +34,11 @@ struct ApplyArgs {
     int64_t wStride;
     V* part;
     int64_t pStride;
-    int64_t maxBox;
+    int64_t maxBox;       // LDS box size of this launch
+    int64_t maxW;         // LDS weight-table size of this launch
+    int G;                // batch instances per workgroup
+    int cbits;            // bits per packed point coordinate (0: recompute coordinates per sweep)
+    int batch;
     const CGState* st;    // may be null
 };
 

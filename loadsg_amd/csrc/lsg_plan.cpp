@@ -124,10 +124,13 @@ int buildPlan(Grid& g, std::string& err) {
         int outDepth;
         int level = 0;
         int64_t caseOff = -1;
+        int fam = -1;
         bool wNeeded[MAXD + 1] = {};
         int64_t wOff[MAXD + 1];
     };
     std::vector<Tmp> items;
+    struct Fam { int c; int fd; std::vector<int> ts; };
+    std::vector<Fam> famList;
     P.prol.assign(d, {});
     std::vector<std::vector<int64_t>> partOf(nd, std::vector<int64_t>(size_t(1) << d, -1));
     double stream = double(g.n) + double(g.nml) + double(g.n);   // calcNodal in/out + final write
@@ -140,36 +143,43 @@ int buildPlan(Grid& g, std::string& err) {
         Depth T0{};
         recursiveDepth(g, R, true, d - 1, T0, sorted);
         if (sorted.empty()) continue;
-        bool hasProl = false;
-        for (int i = 0; i < d; ++i) if (!R[i]) hasProl = true;
+        int lastP = -1, nP = 0;
+        for (int i = 0; i < d; ++i) if (!R[i]) { lastP = i; ++nP; }
 
-        // case buffer boxes
+        // case buffer boxes: prolongation dims other than the last are swept by the
+        // prolongation stage; the last one is fused into the apply stage
         std::unordered_map<int, int64_t> caseOff;
-        if (hasProl) {
+        if (nP >= 2) {
             for (int t : sorted) { caseOff[t] = P.caseSize; P.caseSize += g.boxSize[t]; }
         }
-        std::unordered_set<int> inS(sorted.begin(), sorted.end());
 
-        // prolongation families (ascending d, families along d)
-        bool firstProl = true;
-        for (int dd = 0; dd < d; ++dd) {
-            if (R[dd]) continue;
-            std::map<Depth, std::vector<int>> fam;   // key: T with T[dd] = 0
+        // prolongation families (ascending d, families along d); the reference links
+        // T -> T + e_dd only when the finer depth exists (MatrixVectorHomogen.h:1006)
+        auto families = [&](int dd, std::map<Depth, std::vector<int>>& fam) -> int {
+            fam.clear();
             for (int t : sorted) {
                 Depth K = g.depths[t];
                 K[dd] = 0;
                 fam[K].push_back(t);
             }
-            for (auto& [K, ts] : fam) {
+            for (auto& kv : fam) {
+                auto& ts = kv.second;
                 std::sort(ts.begin(), ts.end(), [&](int a, int b) { return g.depths[a][dd] < g.depths[b][dd]; });
-                // the reference links T -> T + e_dd only when the finer depth exists
-                for (size_t q = 1; q < ts.size(); ++q) {
+                for (size_t q = 1; q < ts.size(); ++q)
                     if (g.depths[ts[q]][dd] != g.depths[ts[q - 1]][dd] + 1) {
                         err = "depth list not contiguous along a prolongation dimension";
                         return LSG_ERR_UNSUPPORTED;
                     }
-                }
                 if ((int)ts.size() > MAXLEV) { err = "family too long"; return LSG_ERR_UNSUPPORTED; }
+            }
+            return 0;
+        };
+        bool firstProl = true;
+        for (int dd = 0; dd < d; ++dd) {
+            if (R[dd] || dd == lastP) continue;
+            std::map<Depth, std::vector<int>> fam;
+            if (int rc = families(dd, fam)) return rc;
+            for (auto& [K, ts] : fam) {
                 ProlItem pi{};
                 for (int i = 0; i < d; ++i) pi.T[i] = g.depths[ts[0]][i];
                 pi.dd = dd;
@@ -179,22 +189,44 @@ int buildPlan(Grid& g, std::string& err) {
                     pi.in_off[q] = firstProl ? g.boxOff[ts[q]] : caseOff[ts[q]];
                     pi.out_off[q] = caseOff[ts[q]];
                     P.maxFamily = std::max(P.maxFamily, g.boxSize[ts[q]]);
-                    if (q > 0) stream += double(g.boxSize[ts[q - 1]]) + 2.0 * double(g.boxSize[ts[q]]);
                 }
                 P.prol[dd].push_back(pi);
             }
             firstProl = false;
         }
+        // streaming-model prolongation traffic (every prolongation dim, reference order)
+        for (int dd = 0; dd < d; ++dd) {
+            if (R[dd]) continue;
+            std::map<Depth, std::vector<int>> fam;
+            if (int rc = families(dd, fam)) return rc;
+            for (auto& [K, ts] : fam)
+                for (size_t q = 1; q < ts.size(); ++q)
+                    stream += double(g.boxSize[ts[q - 1]]) + 2.0 * double(g.boxSize[ts[q]]);
+        }
+        // apply families along the last prolongation dim (single items if none)
+        std::vector<std::vector<int>> applyFam;
+        if (lastP >= 0) {
+            std::map<Depth, std::vector<int>> fam;
+            if (int rc = families(lastP, fam)) return rc;
+            for (auto& kv : fam) applyFam.push_back(kv.second);
+        } else {
+            for (int t : sorted) applyFam.push_back({t});
+        }
+        std::unordered_map<int, int> famOf;
+        for (size_t f = 0; f < applyFam.size(); ++f)
+            for (int t : applyFam[f]) famOf[t] = (int)f;
+        const int famBase = (int)famList.size();
+        for (auto& ts : applyFam) famList.push_back({c, lastP, ts});
 
         // symbolic replay of the gM exchanges in the reference order
         std::unordered_map<int, Read> state;   // depth -> latest write
-        std::unordered_map<int, int> itemOfDepth;
         for (int t : sorted) {
             Tmp X{};
             X.c = c; X.t = t; X.k = 0;
             for (int i = 0; i < d; ++i) if (R[i]) X.r[X.k++] = i;
             for (int j = 0; j <= MAXD; ++j) { X.reads[j] = Read(); X.wOff[j] = -1; }
-            X.caseOff = hasProl ? caseOff[t] : -1;
+            X.caseOff = nP >= 2 ? caseOff[t] : -1;
+            X.fam = famBase + famOf[t];
             int me = (int)items.size();
             auto st = state.find(t);
             if (st != state.end()) X.reads[0] = st->second;
@@ -231,7 +263,6 @@ int buildPlan(Grid& g, std::string& err) {
                 }
             }
             items.push_back(X);
-            itemOfDepth[t] = me;
         }
     }
 
@@ -243,10 +274,30 @@ int buildPlan(Grid& g, std::string& err) {
             if (X.wNeeded[j]) { X.wOff[j] = P.wSize; P.wSize += boxOf(Tc, d); }
         }
     }
+    // wavefront level of a family: one more than the deepest family any of its items
+    // reads a gM value from (fixpoint over the family DAG; a family never reads from
+    // itself because every read refers to a depth finer in a restriction dim)
+    std::vector<int> famLevel(famList.size(), 0);
+    for (bool changed = true; changed;) {
+        changed = false;
+        for (auto& X : items)
+            for (int j = 0; j <= X.k; ++j) {
+                if (X.reads[j].item < 0) continue;
+                int fy = items[X.reads[j].item].fam;
+                if (fy == X.fam) { err = "family reads its own gM values"; return LSG_ERR_UNSUPPORTED; }
+                if (famLevel[X.fam] < famLevel[fy] + 1) {
+                    famLevel[X.fam] = famLevel[fy] + 1;
+                    changed = true;
+                    if (famLevel[X.fam] > (int)famList.size()) { err = "cyclic family dependencies"; return LSG_ERR_UNSUPPORTED; }
+                }
+            }
+    }
     int nlev = 0;
-    for (auto& X : items) nlev = std::max(nlev, X.level + 1);
+    for (int v : famLevel) nlev = std::max(nlev, v + 1);
     P.apply.assign(nlev, {});
-    for (auto& X : items) {
+    std::map<std::pair<int, int>, int> itemOf;   // (case, depth) -> item
+    for (size_t q = 0; q < items.size(); ++q) itemOf[{items[q].c, items[q].t}] = (int)q;
+    auto emit = [&](const Tmp& X) {
         ApplyItem a{};
         for (int i = 0; i < d; ++i) a.T[i] = g.depths[X.t][i];
         a.k = X.k;
@@ -258,7 +309,16 @@ int buildPlan(Grid& g, std::string& err) {
             if (X.reads[j].item >= 0) a.rd_off[j] = items[X.reads[j].item].wOff[X.reads[j].step];
         for (int j = 1; j <= X.k; ++j) a.w_off[j] = X.wOff[j];
         a.out_off = partOf[X.outDepth][X.c];
-        P.apply[X.level].push_back(a);
+        P.applyItems.push_back(a);
+    };
+    for (size_t f = 0; f < famList.size(); ++f) {
+        const Fam& F = famList[f];
+        FamilyItem fi{};
+        fi.fd = F.fd;
+        fi.nt = (int)F.ts.size();
+        fi.first = (int)P.applyItems.size();
+        for (int t : F.ts) emit(items[itemOf[{F.c, t}]]);
+        P.apply[famLevel[f]].push_back(fi);
     }
     P.nApply = (int64_t)items.size();
 
@@ -286,8 +346,8 @@ int buildPlan(Grid& g, std::string& err) {
             for (int q = 0; q < it.nt; ++q) { T[it.dd] = it.T[it.dd] + q; mv += 2.0 * double(boxCount(T, d)); }
         }
     const double beforeApply = mv;
-    for (auto& v : P.apply)
-        for (auto& a : v) {
+    for (auto& a : P.applyItems) {
+        {
             int32_t T[MAXD];
             std::memcpy(T, a.T, sizeof(T));
             mv += double(boxCount(T, d));
@@ -299,6 +359,7 @@ int buildPlan(Grid& g, std::string& err) {
             }
             mv += double(blockCount(T, d));
         }
+    }
     P.applyValues = mv - beforeApply;
     for (auto& s : P.sum) mv += double(blockCount(s.T, d)) * double(s.ncase + 1);
     P.movedValues = mv;

@@ -58,6 +58,16 @@ struct ApplyItem {
     int64_t out_off;            // part buffer block of the target depth
 };
 
+// A family of apply items along the last prolongation dimension fd of a case
+// (depths T, T+e_fd, ...): one workgroup prolongs along fd box by box and applies
+// each box.  fd = -1: a single item (case without prolongation dims).
+struct FamilyItem {
+    int32_t fd;
+    int32_t nt;
+    int32_t first;      // index of the first ApplyItem
+    int32_t pad;
+};
+
 // Sum of the case partials of one depth block (fixed case order).
 struct SumItem {
     int32_t T[MAXD];
@@ -69,7 +79,8 @@ struct SumItem {
 struct Plan {
     std::vector<NodalItem> nodal;
     std::vector<std::vector<ProlItem>> prol;        // per sweep dimension
-    std::vector<std::vector<ApplyItem>> apply;      // per wavefront level
+    std::vector<ApplyItem> applyItems;              // all apply items, family by family
+    std::vector<std::vector<FamilyItem>> apply;     // per wavefront level
     std::vector<SumItem> sum;
     std::vector<int64_t> sumList;                   // part offsets for SumItem
     int64_t caseSize = 0;
