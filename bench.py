@@ -7,7 +7,7 @@ One step = one batched solve of --batch instances from x0 = 0, FP32 inner PCG un
 FP64 iterative refinement, every instance to ||f - A x||_2 <= 1e-8 ||f||_2 (checked in
 FP64).  Inputs are resident in HBM before the timed region.  Also reported: the plain
 FP64 GPU CG (batch 1), the FP64 error of the mixed solutions, the apply-kernel
-roofline (FP64 batched matvec), and the reference CPU path timed on this host.
+roofline (FP32 batched matvec, the inner-solver kernel), and the reference CPU path timed on this host.
 
 Prints one JSON line.
 """
@@ -29,6 +29,10 @@ D, LEVEL = 5, 7
 NXI, NETA = 16, 8
 
 
+def log(msg):
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def inputs(batch, rng):
     xi = rng.normal(0.0, 0.5, (batch, NXI))          # variance 0.25
     eta = rng.standard_normal((batch, NETA))
@@ -37,8 +41,9 @@ def inputs(batch, rng):
 
 
 def apply_traffic_from_profile():
-    """HBM bytes per apply launch group measured with rocprofv3 --pmc (if recorded)."""
-    p = os.path.join(ROOT, "profiles", "apply_kernel_d5l7_pmc.md")
+    """HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the apply stage of one
+    FP32 matvec per batch instance, as recorded from rocprofv3 --pmc runs."""
+    p = os.path.join(ROOT, "profiles", "apply_kernel_fp32_d5l7_b64_pmc.md")
     if not os.path.exists(p):
         return None
     for line in open(p):
@@ -113,6 +118,7 @@ def main():
     op64.rhs_sines(eta, m, out=f)
     x = torch.zeros(B, n, dtype=torch.float64, device=dev)
 
+    log("grid N=%d, %d instances set up" % (n, B))
     # ---- plain FP64 GPU CG, one instance (the baseline the mixed/batched path is compared to)
     op1 = L.MatrixVectorHomogen(grid, st0, precision=64, batch=1)
     f1 = f[0:1].clone()
@@ -126,6 +132,7 @@ def main():
     t_fp64 = time.perf_counter() - t0
     fp64_solves = 1.0 / t_fp64
 
+    log("plain FP64 CG: %d iterations, %.3f s" % (info1["iterations"], t_fp64))
     # ---- reference FP64 solutions (tight) for the error statement, a few instances
     nchk = min(B, 4)
     opc = L.MatrixVectorHomogen(grid, st0, precision=64, batch=nchk)
@@ -135,6 +142,7 @@ def main():
     L.CG.solveHomogen(1e-13, xref, f[:nchk].clone(), opc, stop_rule=L.STOP_REL, maxit=2000)
 
     # ---- timed: batched mixed-precision solves
+    log("tight FP64 reference solves done")
     for _ in range(args.warmup):
         x.zero_()
         L.CG.solveMixed(1e-8, x, f, op64, op32, inner_rel_tol=args.inner_tol)
@@ -146,28 +154,41 @@ def main():
         infos = L.CG.solveMixed(1e-8, x, f, op64, op32, inner_rel_tol=args.inner_tol)
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.steps
+    log("timed mixed-precision solves: %.3f s per step" % dt)
     if B == 1:
         infos = [infos]
     assert all(i["converged"] == 1 and i["rel_residual"] <= 1e-8 for i in infos), infos
     err = float(torch.max(torch.linalg.norm(x[:nchk] - xref, dim=1) / torch.linalg.norm(xref, dim=1)))
 
-    # ---- roofline of the dominant kernel (apply stage, FP64 batched matvec)
-    y = torch.zeros_like(x)
-    op64.multiplication(x, y)
-    reps = 5
-    apply_ms = total_ms = 0.0
-    for _ in range(reps):
-        op64.multiplication(x, y)
-        tm = op64.last_timing()
-        apply_ms += tm["apply_ms"]
-        total_ms += tm["total_ms"]
-    apply_ms /= reps
-    total_ms /= reps
-    apply_bytes = stats["apply_values"] * 8.0 * B
+    # ---- roofline of the dominant kernel: the apply stage of the FP32 batched matvec
+    # (the inner PCG iterations are FP32 matvecs); timed with HIP events on the
+    # operator's stream around the apply launches.  FP64 figures are reported beside it.
+    def apply_timing(op, xv, reps=5):
+        yv = torch.zeros_like(xv)
+        op.multiplication(xv, yv)
+        a_ms = t_ms = 0.0
+        nl = 0
+        for _ in range(reps):
+            op.multiplication(xv, yv)
+            tm = op.last_timing()
+            a_ms += tm["apply_ms"]
+            t_ms += tm["total_ms"]
+            nl = tm["apply_launches"]
+        return a_ms / reps, t_ms / reps, nl
+
+    apply_ms64, total_ms64, _ = apply_timing(op64, x)
+    apply_ms, total_ms, launches = apply_timing(op32, x.float())
+    # algorithmic bytes: the streaming-model values (each value each stage of the
+    # reference data flow reads or writes, counted once) of the stages the apply
+    # kernel performs, x 4 bytes x batch
+    apply_bytes = stats["stream_apply_values"] * 4.0 * B
     achieved = apply_bytes / (apply_ms * 1e-3)
-    stream_bw = stats["stream_values"] * 8.0 * B / (total_ms * 1e-3)
+    own_bw = stats["apply_values"] * 4.0 * B / (apply_ms * 1e-3)
+    stream_bw = stats["stream_values"] * 4.0 * B / (total_ms * 1e-3)
+    achieved64 = stats["stream_apply_values"] * 8.0 * B / (apply_ms64 * 1e-3)
     tr = apply_traffic_from_profile()
 
+    log("apply stage: FP32 %.3f ms, FP64 %.3f ms per batched matvec" % (apply_ms, apply_ms64))
     cpu = None if args.no_cpu_baseline else cpu_baseline(info1["iterations"], args.cpu_budget)
 
     out = {
@@ -189,9 +210,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
                      "traffic": None if tr is None else tr * B,
-                     "kernel": "apply_kernel<double> (all %d wavefront launches of one batched matvec)"
-                               % stats["wavefront_levels"],
-                     "algorithmic_bytes": apply_bytes, "time_ms": apply_ms},
+                     "kernel": "apply_kernel<float>: the %d apply launches of one FP32 matvec of batch %d "
+                               "(one 'launch' = one matvec's apply stage)" % (launches, B),
+                     "algorithmic_bytes": apply_bytes, "time_ms": apply_ms,
+                     "kernel_io_GBps": own_bw / 1e9,
+                     "traffic_GBps": None if tr is None else tr * B / (apply_ms * 1e-3) / 1e9},
         "cpu_baseline": cpu,
         "extra": {
             "fp64_plain_cg_solves_per_s": fp64_solves,
@@ -200,7 +223,11 @@ def main():
             "mixed_inner_iterations_mean": float(np.mean([i["iterations"] for i in infos])),
             "mixed_outer_sweeps_mean": float(np.mean([i["outer"] for i in infos])),
             "max_rel_err_vs_fp64_solve": err,
-            "matvec_fp64_batched_ms": total_ms,
+            "matvec_fp32_batched_ms": total_ms,
+            "matvec_fp64_batched_ms": total_ms64,
+            "apply_fp64_ms": apply_ms64,
+            "apply_fp64_GBps": achieved64 / 1e9,
+            "apply_fp64_frac": achieved64 / HBM_PEAK,
             "matvec_stream_model_GBps": stream_bw / 1e9,
             "matvec_stream_model_frac": stream_bw / HBM_PEAK,
             "stream_values_per_matvec": stats["stream_values"],

@@ -85,7 +85,9 @@ int lsg_grid_device_order(const lsg_grid* g, int64_t* dev_of_slot);
 /* Structural statistics: out[0]=#depths, [1]=multilevel size N_ML, [2]=#apply items,
  * [3]=#wavefront levels, [4]=algorithmic values moved per matvec (streaming model),
  * [5]=largest depth box, [6]=case-buffer size, [7]=W-buffer size, [8]=part-buffer size,
- * [9]=values this implementation moves per matvec, [10]=of which the apply stage. */
+ * [9]=values this implementation moves per matvec, [10]=of which the apply stage,
+ * [11]=streaming-model values of the stages the apply stage performs (prolongation along
+ * the last prolongation dimension, stencil, restrictions with the gM exchange, conversion). */
 int lsg_grid_stats(const lsg_grid* g, int64_t* out, int nout);
 int lsg_grid_destroy(lsg_grid* g);
 

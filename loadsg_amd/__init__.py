@@ -160,10 +160,10 @@ class AdaptiveSparseGrid:
         return out
 
     def stats(self):
-        out = np.zeros(11, dtype=np.int64)
-        _check(lib().lsg_grid_stats(self._h, _ptr(out), 11))
+        out = np.zeros(12, dtype=np.int64)
+        _check(lib().lsg_grid_stats(self._h, _ptr(out), 12))
         keys = ["depths", "multilevel_size", "apply_items", "wavefront_levels", "stream_values", "max_box",
-                "case_buffer", "w_buffer", "part_buffer", "moved_values", "apply_values"]
+                "case_buffer", "w_buffer", "part_buffer", "moved_values", "apply_values", "stream_apply_values"]
         return dict(zip(keys, (int(v) for v in out)))
 
 

@@ -447,10 +447,10 @@ int lsg_grid_device_order(const lsg_grid* g, int64_t* dev_of_slot) {
 int lsg_grid_stats(const lsg_grid* g, int64_t* out, int nout) {
     if (!g || !out) return fail(LSG_ERR_ARG, "null argument");
     const Plan& P = g->g.plan;
-    int64_t v[11] = {(int64_t)g->g.depths.size(), g->g.nml, P.nApply, (int64_t)P.apply.size(),
+    int64_t v[12] = {(int64_t)g->g.depths.size(), g->g.nml, P.nApply, (int64_t)P.apply.size(),
                      (int64_t)P.streamValues, P.maxBox, P.caseSize, P.wSize, P.partSize, (int64_t)P.movedValues,
-                     (int64_t)P.applyValues};
-    for (int i = 0; i < nout && i < 11; ++i) out[i] = v[i];
+                     (int64_t)P.applyValues, (int64_t)P.streamApplyValues};
+    for (int i = 0; i < nout && i < 12; ++i) out[i] = v[i];
     return LSG_OK;
 }
 

@@ -134,6 +134,7 @@ int buildPlan(Grid& g, std::string& err) {
     P.prol.assign(d, {});
     std::vector<std::vector<int64_t>> partOf(nd, std::vector<int64_t>(size_t(1) << d, -1));
     double stream = double(g.n) + double(g.nml) + double(g.n);   // calcNodal in/out + final write
+    double streamApply = 0.0;   // of which the stages the apply kernel performs
 
     const int ncase = 1 << d;
     for (int c = 0; c < ncase; ++c) {
@@ -200,8 +201,11 @@ int buildPlan(Grid& g, std::string& err) {
             std::map<Depth, std::vector<int>> fam;
             if (int rc = families(dd, fam)) return rc;
             for (auto& [K, ts] : fam)
-                for (size_t q = 1; q < ts.size(); ++q)
-                    stream += double(g.boxSize[ts[q - 1]]) + 2.0 * double(g.boxSize[ts[q]]);
+                for (size_t q = 1; q < ts.size(); ++q) {
+                    const double v = double(g.boxSize[ts[q - 1]]) + 2.0 * double(g.boxSize[ts[q]]);
+                    stream += v;
+                    if (dd == lastP) streamApply += v;
+                }
         }
         // apply families along the last prolongation dim (single items if none)
         std::vector<std::vector<int>> applyFam;
@@ -232,6 +236,7 @@ int buildPlan(Grid& g, std::string& err) {
             if (st != state.end()) X.reads[0] = st->second;
             Depth Tc = g.depths[t];
             stream += 3.0 * double(g.boxSize[t]);
+            streamApply += 3.0 * double(g.boxSize[t]);
             for (int j = 1; j <= X.k; ++j) {
                 int rr = X.r[j - 1];
                 Depth prev = Tc;
@@ -243,6 +248,7 @@ int buildPlan(Grid& g, std::string& err) {
                 }
                 int tc = it->second;
                 stream += double(boxOf(prev, d)) + 3.0 * double(g.boxSize[tc]);
+                streamApply += double(boxOf(prev, d)) + 3.0 * double(g.boxSize[tc]);
                 auto s2 = state.find(tc);
                 if (s2 != state.end()) X.reads[j] = s2->second;
                 state[tc] = Read{me, j};
@@ -251,6 +257,7 @@ int buildPlan(Grid& g, std::string& err) {
             if (it == g.depthIdx.end()) { err = "target depth missing"; return LSG_ERR_UNSUPPORTED; }
             X.outDepth = it->second;
             stream += double(g.boxSize[X.outDepth]) + 2.0 * double(g.blkSize[X.outDepth]);
+            streamApply += double(g.boxSize[X.outDepth]) + 2.0 * double(g.blkSize[X.outDepth]);
             if (partOf[X.outDepth][c] >= 0) { err = "two depths of one case convert to the same depth"; return LSG_ERR_UNSUPPORTED; }
             partOf[X.outDepth][c] = P.partSize;
             P.partSize += g.blkSize[X.outDepth];
@@ -334,6 +341,7 @@ int buildPlan(Grid& g, std::string& err) {
         P.sum.push_back(s);
     }
     P.streamValues = stream;
+    P.streamApplyValues = streamApply;
 
     // values this implementation moves per matvec (each kernel reads its inputs and
     // writes its outputs once)

@@ -91,6 +91,8 @@ struct Plan {
     int64_t maxFamily = 0;      // largest box in a prolongation family
     int64_t nApply = 0;
     double streamValues = 0;    // streaming-model values per matvec
+    double streamApplyValues = 0;   // of which the stages the apply kernel performs (last prolongation
+                                    // dim, stencil, restrictions with gM, prewavelet conversion)
     double movedValues = 0;     // values this implementation reads+writes per matvec
     double applyValues = 0;     // of which the apply stage (input box, gM reads, W writes, output block)
 };

@@ -1,0 +1,78 @@
+// shim_check.cpp -- checks the C++ drop-in (include/loadsg_amd_shim.hpp) against the
+// compiled LoAdSG reference inside one program.  TEST INFRASTRUCTURE ONLY.
+//
+// Both sides see the same reference objects: an AdaptiveSparseGrid built with
+// AddRecursiveSonsOfPoint(center, level) (sgrid/sparseGrid.cc:557-566) and
+// VectorSparseG vectors in slot order.  The reference side runs
+// ::MatrixVectorHomogen::multiplication (MatrixVectorHomogen.h:660-759) and
+// ::CG::solveHomogen (applications/cg_method.h:759-886) on the CPU; the drop-in
+// side runs lsg_shim::MatrixVectorHomogen / lsg_shim::CG with the same calls on the GPU.
+//
+// Usage: shim_check <level>      prints one JSON line; exit status 0 when
+//   matvecs agree to 1e-12 (relative, max norm), CG iteration counts are equal and
+//   the CG solutions agree to 1e-10.
+#include "source/sparseEXPDE.h"
+#include "loadsg_amd_shim.hpp"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+namespace {
+
+double relDiff(VectorSparseG& a, VectorSparseG& b, size_t n) {
+    double num = 0.0, den = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        num = std::fmax(num, std::fabs(a.getDatatableVector()[i] - b.getDatatableVector()[i]));
+        den = std::fmax(den, std::fabs(a.getDatatableVector()[i]));
+    }
+    return den > 0.0 ? num / den : num;
+}
+
+}  // namespace
+
+int main(int argc, char** argv) {
+    const int level = argc > 1 ? std::atoi(argv[1]) : 5;
+    if (!lsg_device_available()) {
+        std::printf("{\"error\": \"no GPU\"}\n");
+        return 2;
+    }
+    AdaptiveSparseGrid grid;
+    IndexDimension center;
+    grid.AddRecursiveSonsOfPoint(center, level);
+    MultiLevelAdaptiveSparseGrid mgrid(&grid);
+    const size_t n = grid.getMaximalOccupiedSecondTable();
+
+    ::MatrixVectorHomogen ref(grid, mgrid, true, false);
+    lsg_shim::MatrixVectorHomogen dev(grid, mgrid, true, false);
+    Poisson poisson(grid);
+    HelmHoltz helm(grid);
+
+    VectorSparseG x(grid), yRef(grid), yDev(grid);
+    for (size_t i = 0; i < n; ++i) x.setValue((unsigned long)i, 1.0 / (1.0 + double(i % 7)) - 0.3);
+
+    ref.multiplication(x, yRef, poisson);
+    dev.multiplication(x, yDev, poisson);
+    const double mvPoisson = relDiff(yRef, yDev, n);
+    ref.multiplication(x, yRef, helm);
+    dev.multiplication(x, yDev, helm);
+    const double mvHelm = relDiff(yRef, yDev, n);
+
+    // CG on f = M x (HelmHoltz product), reference stopping rule, eps 1e-8
+    VectorSparseG f(grid), solRef(grid), solDev(grid), usol(grid);
+    ref.multiplication(x, f, helm);
+    solRef = 0.0;
+    solDev = 0.0;
+    int itRef = 0, itDev = 0;
+    double tpRef = 0.0, tpDev = 0.0;
+    ::CG::solveHomogen(1e-8, solRef, f, &itRef, usol, ref, poisson, &tpRef);
+    lsg_shim::CG::solveHomogen(1e-8, solDev, f, &itDev, usol, dev, poisson, &tpDev);
+    const double cgRel = relDiff(solRef, solDev, n);
+
+    const bool ok = mvPoisson < 1e-12 && mvHelm < 1e-12 && itRef == itDev && cgRel < 1e-10;
+    std::printf("{\"d\": %d, \"level\": %d, \"n\": %zu, \"matvec_poisson_rel\": %.3e, \"matvec_helm_rel\": %.3e, "
+                "\"cg_iters_ref\": %d, \"cg_iters_dev\": %d, \"cg_x_rel\": %.3e, \"ok\": %s}\n",
+                DimensionSparseGrid, level, n, mvPoisson, mvHelm, itRef, itDev, cgRel, ok ? "true" : "false");
+    return ok ? 0 : 1;
+}
