@@ -130,6 +130,7 @@ struct lsg_op {
     int64_t maxLds = 0;
     DevMem dNodalItems, dSumItems, dSumList, dApplyItems;
     std::vector<std::unique_ptr<DevMem>> dProl;   // per sweep dim
+    std::vector<int64_t> prolMax;                 // largest box per sweep dim
     DevMem coefTab, nodal, cbuf, wbuf, part, xs, ys;
     // solver buffers
     DevMem r, h, dv, g, D, f, x, tmp, partials, st, aux;
@@ -154,7 +155,7 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
                     P.maxBox, B, st, s);
     for (int dd = 0; dd < d; ++dd)
         launch_prol<V>(op->dProl[dd]->as<ProlItem>(), (int)P.prol[dd].size(), d, op->nodal.as<V>(), g.nml,
-                       op->cbuf.as<V>(), P.caseSize, P.maxFamily, B, st, s);
+                       op->cbuf.as<V>(), P.caseSize, op->prolMax[dd], B, st, s);
     HIPCHK(hipEventRecord(op->ev[1], s));
     ApplyArgs<V> a{};
     a.d = d;
@@ -490,6 +491,16 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
                 for (int i = 0; i < g.d; ++i) w += int64_t(1) << a.T[i];
                 mw = std::max(mw, w);
             }
+            if (!op->general) {
+                // the separable apply kernel keeps the weights of the whole family:
+                // along fd the levels T_fd .. T_fd + nt - 1 back to back
+                const ApplyItem& a0 = P.applyItems[f.first];
+                int64_t w = 0;
+                for (int i = 0; i < g.d; ++i)
+                    w += (i == f.fd) ? (int64_t(1) << (a0.T[i] + f.nt)) - (int64_t(1) << a0.T[i])
+                                     : int64_t(1) << a0.T[i];
+                mw = std::max(mw, w);
+            }
             int c = 0;
             while (c < ncls - 1 && mb > bounds[c]) ++c;
             bucket[c].push_back(f);
@@ -498,6 +509,15 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         }
         for (int c = 0; c < ncls; ++c) {
             if (bucket[c].empty()) continue;
+            // longest families first (work = points of all boxes of the family)
+            std::stable_sort(bucket[c].begin(), bucket[c].end(), [&](const FamilyItem& x, const FamilyItem& y) {
+                auto work = [&](const FamilyItem& f) {
+                    int64_t w = 0;
+                    for (int q = 0; q < f.nt; ++q) w += boxCount(P.applyItems[f.first + q].T, g.d);
+                    return w;
+                };
+                return work(x) > work(y);
+            });
             lsg_op::ApplyLaunch la;
             la.level = (int)L;
             la.cls = c;
@@ -541,7 +561,26 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     // items
     HIPCHK(op->dNodalItems.alloc(P.nodal.size() * sizeof(NodalItem)));
     HIPCHK(hipMemcpy(op->dNodalItems.p, P.nodal.data(), P.nodal.size() * sizeof(NodalItem), hipMemcpyHostToDevice));
-    for (auto& v : P.prol) {
+    for (const auto& v0 : P.prol) {
+        // longest chains first; the largest box of the sweep sizes the LDS
+        std::vector<ProlItem> v(v0);
+        auto boxes = [&](const ProlItem& it, bool largest) {
+            int64_t w = 0, m = 0;
+            int32_t T[MAXD];
+            std::memcpy(T, it.T, sizeof(T));
+            for (int q = 0; q < it.nt; ++q) {
+                T[it.dd] = it.T[it.dd] + q;
+                const int64_t c = boxCount(T, g.d);
+                w += c;
+                m = std::max(m, c);
+            }
+            return largest ? m : w;
+        };
+        std::stable_sort(v.begin(), v.end(),
+                         [&](const ProlItem& x, const ProlItem& y) { return boxes(x, false) > boxes(y, false); });
+        int64_t mb = 1;
+        for (const ProlItem& it : v) mb = std::max(mb, boxes(it, true));
+        op->prolMax.push_back(mb);
         op->dProl.emplace_back(new DevMem());
         HIPCHK(op->dProl.back()->alloc(v.size() * sizeof(ProlItem)));
         if (!v.empty()) HIPCHK(hipMemcpy(op->dProl.back()->p, v.data(), v.size() * sizeof(ProlItem), hipMemcpyHostToDevice));

@@ -316,6 +316,97 @@ __global__ void __launch_bounds__(256) prol_kernel(const ProlItem* __restrict__ 
     }
 }
 
+// Prolongation chain along dd with the input of the next box prefetched into
+// registers: box q = input(q) + I_dd(box q-1) (prolongation1D_inplace +
+// addMultiLevelValues, MatrixVectorHomogen.h:444-474,997-1019), written to the case
+// buffer and kept in LDS for the next box; one barrier per box.  Grid: item-major,
+// instance fastest.
+template <typename V, int PPT>
+__global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict__ items, int d,
+                                                      const V* __restrict__ nodal, int64_t nStride,
+                                                      V* __restrict__ cbuf, int64_t cStride, int64_t maxBox,
+                                                      int batch, const CGState* __restrict__ st) {
+    extern __shared__ char smem_raw[];
+    V* prev = reinterpret_cast<V*>(smem_raw);
+    V* cur = prev + maxBox;
+    const int inst = blockIdx.x % batch;
+    if (st && !st[inst].active) return;
+    const ProlItem& it = items[blockIdx.x / batch];
+    const int dd = it.dd;
+    const V* inBase = it.from_nodal ? nodal + inst * nStride : cbuf + inst * cStride;
+    V* outBase = cbuf + inst * cStride;
+    int total = 1, S = 1, nf = 1;
+    for (int i = 0; i < d; ++i) {
+        const int n = (1 << it.T[i]) - 1;
+        if (i < dd) S *= n;
+        if (i == dd) nf = n;
+        total *= n;
+    }
+    V nin[PPT];
+    {
+        const V* in = inBase + it.in_off[0];
+#pragma unroll
+        for (int s = 0; s < PPT; ++s) {
+            const int p = threadIdx.x + s * blockDim.x;
+            if (p < total) nin[s] = in[p];
+        }
+    }
+    const float rS = 1.0f / float(S);
+    for (int q = 0; q < it.nt; ++q) {
+        const bool keep = q + 1 < it.nt;
+        V v[PPT];
+        if (q == 0) {
+#pragma unroll
+            for (int s = 0; s < PPT; ++s) v[s] = nin[s];
+        } else {
+            const int nc = (nf - 1) >> 1;
+            const float rnf = 1.0f / float(nf);
+#pragma unroll
+            for (int s = 0; s < PPT; ++s) {
+                const int p = threadIdx.x + s * blockDim.x;
+                if (p < total) {
+                    Split sp = split(p, S, rS, nf, rnf);
+                    const V* c = prev + sp.lo + sp.hi * S * nc;
+                    const int k = sp.k + 1;
+                    V add;
+                    if (k & 1) {
+                        const int cl = (k - 1) >> 1, cr = (k + 1) >> 1;
+                        V va = (cl >= 1) ? c[(cl - 1) * S] : V(0);
+                        V vb = (cr <= nc) ? c[(cr - 1) * S] : V(0);
+                        add = V(0.5) * (va + vb);
+                    } else {
+                        add = c[((k >> 1) - 1) * S];
+                    }
+                    v[s] = nin[s] + add;
+                }
+            }
+        }
+        const int totalN = keep ? total / nf * (2 * nf + 1) : 0;
+        if (keep) {
+            const V* in = inBase + it.in_off[q + 1];
+#pragma unroll
+            for (int s = 0; s < PPT; ++s) {
+                const int p = threadIdx.x + s * blockDim.x;
+                if (p < totalN) nin[s] = in[p];
+            }
+        }
+        V* out = outBase + it.out_off[q];
+#pragma unroll
+        for (int s = 0; s < PPT; ++s) {
+            const int p = threadIdx.x + s * blockDim.x;
+            if (p < total) {
+                out[p] = v[s];
+                if (keep) cur[p] = v[s];
+            }
+        }
+        if (!keep) break;
+        __syncthreads();
+        V* tmp = prev; prev = cur; cur = tmp;
+        total = totalN;
+        nf = 2 * nf + 1;
+    }
+}
+
 // ------------------------------------------------------------------ apply
 // weights of one separable factor for the box's depth: dim i, level T_i,
 // 2^T_i cell-centre values, copied from the instance's 1D tables into LDS.
@@ -347,9 +438,11 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
     V* Wt = a.general ? Zt + a.maxBox : Zt;   // weights
     __shared__ int woff[MAXD];
     __shared__ Shape sh;
-    const int inst = blockIdx.y;
+    // family-major order: all instances of a family are adjacent, and the host lists
+    // the families of a launch by decreasing work, so the longest start first
+    const int inst = blockIdx.x % a.batch;
     if (a.st && !a.st[inst].active) return;
-    const FamilyItem fam = a.fam[blockIdx.x];
+    const FamilyItem fam = a.fam[blockIdx.x / a.batch];
     const int d = a.d;
     const V* coef = a.coefTab + inst * a.coefStride;
     V* wbase = a.wbuf + inst * a.wStride;
@@ -435,6 +528,236 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
         V* out = a.part + inst * a.pStride + it.out_off;
         for (int p = threadIdx.x; p < cnt; p += blockDim.x) out[p] = cur[p];
         __syncthreads();
+    }
+}
+
+// ------------------------------------------------ apply, one separable term
+// Horner sweep of the separable apply kernel.  The first sweep (i = d-1) reads P
+// from Y, starts Z from zero and, in its write phase, keeps P in `keep` for the
+// prolongation of the next box; the last sweep (i = 0) does not update Y and adds
+// the gM value g0 read for this box (held in registers) to Z.
+template <typename V, int PPT>
+__device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const Shape& sh, int i,
+                                 const V* __restrict__ w, int t, bool first, bool needY, V* __restrict__ keep,
+                                 bool addG, const V (&g0)[PPT]) {
+    const int S = sh.S[i], n = sh.n[i], total = sh.total;
+    const float rS = sh.rS[i], rn = sh.rn[i];
+    const V ih = V(1 << t);
+    const V h6 = V(1.0) / (V(6.0) * V(1 << t));
+    V ny[PPT], nz[PPT], yk[PPT];
+#pragma unroll
+    for (int s = 0; s < PPT; ++s) {
+        const int p = threadIdx.x + s * blockDim.x;
+        if (p < total) {
+            const int q = fdiv(p, rS);
+            const int kq = q - fdiv(q, rn) * n;
+            const bool hasL = kq > 0, hasR = kq < n - 1;
+            const V yc = Y[p];
+            const V yl = hasL ? Y[p - S] : V(0);
+            const V yr = hasR ? Y[p + S] : V(0);
+            const V wl = w[kq], wr = w[kq + 1];
+            ny[s] = h6 * (wl * (yl + V(2) * yc) + wr * (V(2) * yc + yr));
+            V out = ih * (wl * (yc - yl) + wr * (yc - yr));
+            if (!first) {
+                const V zc = Z[p];
+                const V zl = hasL ? Z[p - S] : V(0);
+                const V zr = hasR ? Z[p + S] : V(0);
+                out += h6 * (wl * (zl + V(2) * zc) + wr * (V(2) * zc + zr));
+            }
+            nz[s] = out;
+            yk[s] = yc;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < PPT; ++s) {
+        const int p = threadIdx.x + s * blockDim.x;
+        if (p < total) {
+            if (needY) Y[p] = ny[s];
+            Z[p] = addG ? nz[s] + g0[s] : nz[s];
+            if (keep) keep[p] = yk[s];
+        }
+    }
+    __syncthreads();
+}
+
+// Apply stage for an operator with one separable stiffness term (a = prod_j g_j(x_j),
+// c = 0): same data flow as apply_kernel, with
+//  * the weights of the whole family loaded once (they differ between the boxes of a
+//    family only along fd, whose levels T_fd..T_fd+nt-1 are contiguous in the table),
+//  * the next box's input and this box's gM value prefetched into registers, so their
+//    global latency overlaps the sweeps,
+//  * the copy of P, the gM add, the gM write/read of each restriction and the output
+//    write fused into the neighbouring passes (one barrier per restriction and
+//    contraction, two per Horner sweep, one at the start and one at the end of a box).
+template <typename V, int PPT>
+__global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
+    extern __shared__ char smem_raw[];
+    V* U = reinterpret_cast<V*>(smem_raw);
+    V* Y = U + a.maxBox;
+    V* Z = Y + a.maxBox;
+    V* W = Z + a.maxBox;
+    __shared__ int woff[MAXD];
+    __shared__ Shape shs[MAXD + 1];
+    // family-major order: all instances of a family are adjacent, and the host lists
+    // the families of a launch by decreasing work, so the longest start first
+    const int inst = blockIdx.x % a.batch;
+    if (a.st && !a.st[inst].active) return;
+    const FamilyItem fam = a.fam[blockIdx.x / a.batch];
+    const int d = a.d;
+    const int fd = fam.fd;
+    const V* coef = a.coefTab + inst * a.coefStride;
+    const V* nodal = a.nodal + inst * a.nStride;
+    const V* cbuf = a.cbuf + inst * a.cStride;
+    V* wb = a.wbuf + inst * a.wStride;
+    const ApplyItem& it0 = a.items[fam.first];
+    const int T0fd = fd >= 0 ? it0.T[fd] : 0;
+
+    // weights of the whole family: dim i at woff[i]; along fd, level T0fd + q at
+    // woff[fd] + 2^(T0fd+q) - 2^T0fd
+    {
+        int o = 0;
+        for (int i = 0; i < d; ++i) {
+            const int ti = it0.T[i];
+            const int len = (i == fd) ? (1 << (ti + fam.nt)) - (1 << ti) : (1 << ti);
+            const V* src = coef + i * a.tabLen + ((1 << ti) - 2);
+            for (int c = threadIdx.x; c < len; c += blockDim.x) W[o + c] = src[c];
+            if (threadIdx.x == 0) woff[i] = o;
+            o += len;
+        }
+    }
+    // input of the first box
+    V nin[PPT];
+    {
+        int total = 1;
+        for (int i = 0; i < d; ++i) total *= (1 << it0.T[i]) - 1;
+        const V* in = (it0.in_nodal ? nodal : cbuf) + it0.in_off;
+#pragma unroll
+        for (int s = 0; s < PPT; ++s) {
+            const int p = threadIdx.x + s * blockDim.x;
+            if (p < total) nin[s] = in[p];
+        }
+    }
+
+    for (int q = 0; q < fam.nt; ++q) {
+        const ApplyItem& it = a.items[fam.first + q];
+        int total = 1, Sfd = 1, nfd = 1;
+        for (int i = 0; i < d; ++i) {
+            const int n = (1 << it.T[i]) - 1;
+            if (i < fd) Sfd *= n;
+            if (i == fd) nfd = n;
+            total *= n;
+        }
+        // P = input (+ I_fd(P of the previous box), prolongation1D_inplace +
+        // addMultiLevelValues, MatrixVectorHomogen.h:444-474,997-1019)
+        if (q == 0) {
+#pragma unroll
+            for (int s = 0; s < PPT; ++s) {
+                const int p = threadIdx.x + s * blockDim.x;
+                if (p < total) Y[p] = nin[s];
+            }
+        } else {
+            const int nc = (nfd - 1) >> 1;
+            const float rS = 1.0f / float(Sfd), rnf = 1.0f / float(nfd);
+#pragma unroll
+            for (int s = 0; s < PPT; ++s) {
+                const int p = threadIdx.x + s * blockDim.x;
+                if (p < total) {
+                    Split sp = split(p, Sfd, rS, nfd, rnf);
+                    const V* c = U + sp.lo + sp.hi * Sfd * nc;
+                    const int k = sp.k + 1;
+                    V add;
+                    if (k & 1) {
+                        const int cl = (k - 1) >> 1, cr = (k + 1) >> 1;
+                        V va = (cl >= 1) ? c[(cl - 1) * Sfd] : V(0);
+                        V vb = (cr <= nc) ? c[(cr - 1) * Sfd] : V(0);
+                        add = V(0.5) * (va + vb);
+                    } else {
+                        add = c[((k >> 1) - 1) * Sfd];
+                    }
+                    Y[p] = nin[s] + add;
+                }
+            }
+        }
+        // prefetch: gM value of this box and the input of the next box
+        V g0[PPT];
+        const bool rd0 = it.rd_off[0] >= 0;
+        if (rd0) {
+            const V* g = wb + it.rd_off[0];
+#pragma unroll
+            for (int s = 0; s < PPT; ++s) {
+                const int p = threadIdx.x + s * blockDim.x;
+                if (p < total) g0[s] = g[p];
+            }
+        }
+        if (q + 1 < fam.nt) {
+            const ApplyItem& itn = a.items[fam.first + q + 1];
+            const int totalN = total / nfd * ((nfd << 1) + 1);
+            const V* in = (itn.in_nodal ? nodal : cbuf) + itn.in_off;
+#pragma unroll
+            for (int s = 0; s < PPT; ++s) {
+                const int p = threadIdx.x + s * blockDim.x;
+                if (p < totalN) nin[s] = in[p];
+            }
+        }
+        // shapes of the box and after each restriction
+        if (threadIdx.x == 0) {
+            shs[0].d = d;
+            for (int i = 0; i < d; ++i) shs[0].n[i] = (1 << it.T[i]) - 1;
+            shape_restage(shs[0]);
+            for (int j = 1; j <= it.k; ++j) {
+                shs[j] = shs[j - 1];
+                const int r = it.r[j - 1];
+                shs[j].n[r] = (shs[j].n[r] - 1) >> 1;
+                shape_restage(shs[j]);
+            }
+        }
+        __syncthreads();
+        // z = A_T P (Horner sum factorisation), + gM read
+        const Shape& sh = shs[0];
+        const int wq = (1 << (T0fd + q)) - (1 << T0fd);
+        for (int i = d - 1; i >= 0; --i)
+            sep_sweep<V, PPT>(Y, Z, sh, i, W + woff[i] + (i == fd ? wq : 0), it.T[i], i == d - 1, i > 0,
+                              (i == d - 1 && q + 1 < fam.nt) ? U : nullptr, i == 0 && rd0, g0);
+        // restrictions along the restriction dims (ascending) with the gM exchange
+        V* cur = Z;
+        V* oth = Y;
+        for (int j = 1; j <= it.k; ++j) {
+            const int r = it.r[j - 1];
+            const Shape& s0 = shs[j - 1];
+            const int nf = s0.n[r], nc = (nf - 1) >> 1, S = s0.S[r];
+            const int outCnt = shs[j].total;
+            const float rS = s0.rS[r], rnc = 1.0f / float(nc);
+            const bool w = it.w_off[j] >= 0, rd = it.rd_off[j] >= 0;
+            V* wd = wb + it.w_off[j];
+            const V* g = wb + it.rd_off[j];
+            for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
+                Split sp = split(p, S, rS, nc, rnc);
+                const V* b = cur + sp.lo + sp.hi * S * nf + (2 * sp.k + 1) * S;
+                const V v = b[0] + V(0.5) * (b[-S] + b[S]);
+                if (w) wd[p] = v;
+                oth[p] = rd ? v + g[p] : v;
+            }
+            __syncthreads();
+            V* tmp = cur; cur = oth; oth = tmp;
+        }
+        // prewavelet conversion, d contractions box -> block; the last one writes the part
+        const Shape& sf = shs[it.k];
+        int cnt = sf.total;
+        int Slo = 1;
+        V* out = a.part + inst * a.pStride + it.out_off;
+        for (int i = 0; i < d; ++i) {
+            const int n = sf.n[i];
+            int t = it.T[i];
+            for (int j = 0; j < it.k; ++j) if (it.r[j] == i) t -= 1;
+            const int m = (n + 1) >> 1;
+            const int outCnt = cnt / n * m;
+            contract_dim<V>(cur, i + 1 < d ? oth : out, Slo, n, outCnt, t);
+            cnt = outCnt;
+            Slo *= m;
+            __syncthreads();
+            V* tmp = cur; cur = oth; oth = tmp;
+        }
     }
 }
 
@@ -778,7 +1101,22 @@ template <typename V>
 void launch_prol(const ProlItem* items, int nitems, int d, const V* nodal, int64_t nStride, V* cbuf, int64_t cStride,
                  int64_t maxBox, int batch, const CGState* st, hipStream_t s) {
     if (nitems == 0) return;
-    size_t lds = 2 * maxBox * sizeof(V);
+    const size_t lds = 2 * maxBox * sizeof(V);
+    const int threads = maxBox <= 16 * 256 ? 256 : 512;
+    const int64_t ppt = (maxBox + threads - 1) / threads;
+    const dim3 grid((unsigned)nitems * (unsigned)batch), blk(threads);
+#define LSG_PROL_CASE(P)                                                                                   \
+    if (ppt <= P) {                                                                                        \
+        hipLaunchKernelGGL((prol_pf_kernel<V, P>), grid, blk, lds, s, items, d, nodal, nStride, cbuf, cStride, \
+                           maxBox, batch, st);                                                             \
+        return;                                                                                            \
+    }
+    LSG_PROL_CASE(1)
+    LSG_PROL_CASE(2)
+    LSG_PROL_CASE(4)
+    LSG_PROL_CASE(8)
+    LSG_PROL_CASE(16)
+#undef LSG_PROL_CASE
     hipLaunchKernelGGL(prol_kernel<V>, dim3(nitems, batch), dim3(256), lds, s, items, d, nodal, nStride, cbuf, cStride,
                        maxBox, st);
 }
@@ -786,11 +1124,14 @@ template <typename V>
 int launch_apply(const ApplyArgs<V>& a, int nitems, int batch, int threads, size_t lds, hipStream_t s) {
     if (nitems == 0) return 0;
     const int64_t ppt = (a.maxBox + threads - 1) / threads;
-    dim3 grid(nitems, batch), blk(threads);
-#define LSG_APPLY_CASE(P)                                               \
-    if (ppt <= P) {                                                     \
-        hipLaunchKernelGGL((apply_kernel<V, P>), grid, blk, lds, s, a); \
-        return 0;                                                       \
+    dim3 grid((unsigned)nitems * (unsigned)batch), blk(threads);
+#define LSG_APPLY_CASE(P)                                                       \
+    if (ppt <= P) {                                                             \
+        if (a.general)                                                          \
+            hipLaunchKernelGGL((apply_kernel<V, P>), grid, blk, lds, s, a);     \
+        else                                                                    \
+            hipLaunchKernelGGL((apply_sep_kernel<V, P>), grid, blk, lds, s, a); \
+        return 0;                                                               \
     }
     LSG_APPLY_CASE(1)
     LSG_APPLY_CASE(2)
