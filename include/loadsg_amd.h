@@ -127,6 +127,26 @@ int lsg_last_timing(const lsg_op* op, double* out, int nout);
 int lsg_op_set_block(lsg_op* op, int stage, int threads);
 int lsg_op_destroy(lsg_op* op);
 
+/* ------------------------------------------------------ warm-start network */
+/* MLP n_in -> width (x hidden layers, GELU) -> n_out that maps the parameters of a
+ * problem instance to an initial guess of its solution (device order).  No
+ * counterpart in the reference.  BF16 operands on the matrix cores, FP32
+ * accumulation, FP32 master weights and Adam moments.  All x / y / target arrays
+ * are device pointers to FP32, row-major batch x n_in / batch x n_out.  Parameters
+ * are flattened layer by layer as W (in x out, row-major) followed by b (out). */
+typedef struct lsg_mlp lsg_mlp;
+int lsg_mlp_create(int n_in, int width, int hidden_layers, int64_t n_out, uint64_t seed, lsg_mlp** out);
+int lsg_mlp_forward(lsg_mlp* m, const float* x, float* y, int batch);
+/* mean squared error over batch x n_out and its gradients (kept inside; lsg_mlp_get_grads) */
+int lsg_mlp_gradients(lsg_mlp* m, const float* x, const float* target, int batch, double* loss);
+/* gradients + one Adam step (beta 0.9 / 0.999, eps 1e-8) with learning rate lr */
+int lsg_mlp_train_step(lsg_mlp* m, const float* x, const float* target, int batch, float lr, double* loss);
+int lsg_mlp_num_params(const lsg_mlp* m, int64_t* n);
+int lsg_mlp_get_params(const lsg_mlp* m, float* host);
+int lsg_mlp_set_params(lsg_mlp* m, const float* host);
+int lsg_mlp_get_grads(const lsg_mlp* m, float* host);
+int lsg_mlp_destroy(lsg_mlp* m);
+
 const char* lsg_last_error(void);
 int lsg_device_available(void);
 

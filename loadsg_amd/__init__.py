@@ -80,6 +80,15 @@ def lib():
             "lsg_last_timing": (i32, [vp, vp, i32]),
             "lsg_op_set_block": (i32, [vp, i32, i32]),
             "lsg_op_destroy": (i32, [vp]),
+            "lsg_mlp_create": (i32, [i32, i32, i32, i64, ctypes.c_uint64, ctypes.POINTER(vp)]),
+            "lsg_mlp_forward": (i32, [vp, vp, vp, i32]),
+            "lsg_mlp_gradients": (i32, [vp, vp, vp, i32, ctypes.POINTER(dbl)]),
+            "lsg_mlp_train_step": (i32, [vp, vp, vp, i32, ctypes.c_float, ctypes.POINTER(dbl)]),
+            "lsg_mlp_num_params": (i32, [vp, ctypes.POINTER(i64)]),
+            "lsg_mlp_get_params": (i32, [vp, vp]),
+            "lsg_mlp_set_params": (i32, [vp, vp]),
+            "lsg_mlp_get_grads": (i32, [vp, vp]),
+            "lsg_mlp_destroy": (i32, [vp]),
             "lsg_last_error": (ctypes.c_char_p, []),
             "lsg_device_available": (i32, []),
         }
@@ -301,3 +310,77 @@ class CG:
 
 def device_available():
     return bool(lib().lsg_device_available())
+
+
+class MLP:
+    """Warm-start network n_in -> width (x hidden, GELU) -> n_out on the GPU (BF16
+    matrix-core GEMMs, FP32 accumulation and master weights, Adam).  x / target /
+    output are float32 torch tensors on the GPU, shape (batch, n_in) / (batch, n_out)."""
+
+    def __init__(self, n_in, width, hidden, n_out, seed=0):
+        self._h = ctypes.c_void_p()
+        self.n_in, self.width, self.hidden, self.n_out = n_in, width, hidden, n_out
+        _check(lib().lsg_mlp_create(n_in, width, hidden, n_out, seed, ctypes.byref(self._h)))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.lsg_mlp_destroy(self._h)
+            self._h = None
+
+    @staticmethod
+    def _need(t, rows, cols):
+        import torch
+        if not (t.is_cuda and t.dtype == torch.float32 and t.is_contiguous() and t.dim() == 2 and
+                t.shape[1] == cols and (rows is None or t.shape[0] == rows)):
+            raise LoadsgError("expected a contiguous float32 GPU tensor of shape (batch, %d)" % cols)
+
+    def forward(self, x, out=None):
+        import torch
+        self._need(x, None, self.n_in)
+        y = torch.empty(x.shape[0], self.n_out, dtype=torch.float32, device=x.device) if out is None else out
+        self._need(y, x.shape[0], self.n_out)
+        _sync_torch(x)
+        _check(lib().lsg_mlp_forward(self._h, _ptr(x), _ptr(y), x.shape[0]))
+        return y
+
+    def gradients(self, x, target):
+        self._need(x, None, self.n_in)
+        self._need(target, x.shape[0], self.n_out)
+        _sync_torch(x)
+        loss = ctypes.c_double()
+        _check(lib().lsg_mlp_gradients(self._h, _ptr(x), _ptr(target), x.shape[0], ctypes.byref(loss)))
+        return loss.value
+
+    def train_step(self, x, target, lr):
+        self._need(x, None, self.n_in)
+        self._need(target, x.shape[0], self.n_out)
+        _sync_torch(x)
+        loss = ctypes.c_double()
+        _check(lib().lsg_mlp_train_step(self._h, _ptr(x), _ptr(target), x.shape[0], lr, ctypes.byref(loss)))
+        return loss.value
+
+    def num_params(self):
+        n = ctypes.c_int64()
+        _check(lib().lsg_mlp_num_params(self._h, ctypes.byref(n)))
+        return n.value
+
+    def params(self):
+        """Flat float32 copy: per layer W (in x out, row-major) then b."""
+        a = np.zeros(self.num_params(), dtype=np.float32)
+        _check(lib().lsg_mlp_get_params(self._h, _ptr(a)))
+        return a
+
+    def set_params(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        if a.size != self.num_params():
+            raise LoadsgError("parameter count mismatch")
+        _check(lib().lsg_mlp_set_params(self._h, _ptr(a)))
+
+    def grads(self):
+        a = np.zeros(self.num_params(), dtype=np.float32)
+        _check(lib().lsg_mlp_get_grads(self._h, _ptr(a)))
+        return a
+
+    def layer_shapes(self):
+        dims = [self.n_in] + [self.width] * self.hidden + [self.n_out]
+        return [(dims[i], dims[i + 1]) for i in range(len(dims) - 1)]

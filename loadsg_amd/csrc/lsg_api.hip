@@ -22,6 +22,11 @@ int fail(int code, const std::string& msg) {
 }
 }  // namespace
 
+namespace lsg {
+// error reporting shared with the other translation units of the C ABI
+int api_fail(int code, const std::string& msg) { return fail(code, msg); }
+}  // namespace lsg
+
 #define HIPCHK(x)                                                                                  \
     do {                                                                                           \
         hipError_t e_ = (x);                                                                       \
@@ -333,7 +338,7 @@ int computeDiag(lsg_op* op) {
 // CG::solveHomogen (applications/cg_method.h:759-886).
 template <typename V>
 int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, int maxit,
-             std::vector<CGState>& out) {
+             std::vector<CGState>& out, bool xZero = false, const double* epsInst = nullptr) {
     const int64_t n = op->n();
     const int B = op->batch;
     hipStream_t s = op->stream;
@@ -349,15 +354,19 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
     const int nb = cg_partial_blocks(n);
     double* part = op->partials.as<double>();
     CGState* st = op->st.as<CGState>();
-    // r = A x (x in V for the matvec)
-    const V* xv;
-    if (x64 && sizeof(V) != 8) {
-        launch_cast_d2f((const double*)x, (float*)g, n * B, s);
-        xv = g;
-    } else xv = (const V*)x;
-    if ((rc = runMatvec<V>(op, xv, r, nullptr))) return rc;
+    // r = A x (x in V for the matvec); a start vector known to be zero needs no matvec
+    if (xZero) {
+        launch_zero(r, n * B * (int64_t)sizeof(V), s);
+    } else {
+        const V* xv;
+        if (x64 && sizeof(V) != 8) {
+            launch_cast_d2f((const double*)x, (float*)g, n * B, s);
+            xv = g;
+        } else xv = (const V*)x;
+        if ((rc = runMatvec<V>(op, xv, r, nullptr))) return rc;
+    }
     launch_cg_init<V>(f, r, h, dv, D, n, B, part, nb, s);
-    launch_cg_start(st, part, nb, eps, rule, maxit, B, s);
+    launch_cg_start(st, part, nb, eps, epsInst, rule, maxit, B, s);
     HIPCHK(hipGetLastError());
     out.resize(B);
     const int chunk = 8;
@@ -733,10 +742,22 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
     float* r32 = op32->f.as<float>();       // inner right-hand side
     float* e32 = op32->x.as<float>();       // inner solution
     std::vector<int> iters(B, 0), outer(B, 0), conv(B, 0);
-    std::vector<double> rel;
+    std::vector<double> rel, epsH(B);
     std::vector<CGState> st;
+    DevMem epsD;
+    HIPCHK(epsD.alloc((size_t)B * 8));
     for (int sweep = 0; sweep <= maxit; ++sweep) {
         if ((rc = trueResidual(op64, fD, xD, Ax, r64, rel))) return rc;
+        if (sweep == 0) {
+            // a start vector worse than zero (||f - A x0|| > ||f||) is replaced by zero
+            bool any = false;
+            for (int b = 0; b < B; ++b)
+                if (rel[b] > 1.0) {
+                    HIPCHK(hipMemsetAsync(xD + (size_t)b * n, 0, n * 8, s));
+                    any = true;
+                }
+            if (any && (rc = trueResidual(op64, fD, xD, Ax, r64, rel))) return rc;
+        }
         bool all = true;
         for (int b = 0; b < B; ++b) {
             if (rel[b] != rel[b]) return fail(LSG_ERR_NAN, "NaN residual");
@@ -744,12 +765,17 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
                 conv[b] = 1;
                 HIPCHK(hipMemsetAsync(r64 + (size_t)b * n, 0, n * 8, s));
             } else all = false;
+            // inner tolerance: inner_rel_tol, or less when the remaining reduction is smaller
+            epsH[b] = std::min(0.5, std::max(inner_rel_tol, 0.3 * rel_tol / std::max(rel[b], 1e-300)));
         }
         if (all || sweep == maxit) break;
+        HIPCHK(hipMemcpyAsync(epsD.p, epsH.data(), (size_t)B * 8, hipMemcpyHostToDevice, s));
         launch_cast_d2f(r64, r32, tot, s);
         HIPCHK(hipMemsetAsync(e32, 0, tot * 4, s));
         HIPCHK(hipStreamSynchronize(s));
-        if ((rc = cgDevice<float>(op32, r32, e32, false, inner_rel_tol, LSG_STOP_REL, 1000, st))) return rc;
+        if ((rc = cgDevice<float>(op32, r32, e32, false, inner_rel_tol, LSG_STOP_REL, 1000, st, true,
+                                  epsD.as<double>())))
+            return rc;
         launch_axpy_correction(xD, e32, tot, op32->stream);
         HIPCHK(hipStreamSynchronize(op32->stream));
         for (int b = 0; b < B; ++b)

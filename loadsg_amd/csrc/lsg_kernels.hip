@@ -957,14 +957,15 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const V* __restrict__ f, V
         for (int q = 0; q < 3; ++q) partials[((int64_t)inst * 3 + q) * gridDim.x + blockIdx.x] = out[q];
 }
 
-__global__ void cg_start_scalar(CGState* st, const double* __restrict__ partials, int nb, double eps, int rule,
-                                int maxit) {
+__global__ void cg_start_scalar(CGState* st, const double* __restrict__ partials, int nb, double eps,
+                                const double* __restrict__ epsInst, int rule, int maxit) {
     const int inst = blockIdx.x;
     if (threadIdx.x != 0) return;
     CGState& s = st[inst];
     s.delta = sum_partials(partials + ((int64_t)inst * 3 + 0) * nb, nb);
     s.rr = sum_partials(partials + ((int64_t)inst * 3 + 1) * nb, nb);
     s.ff = sum_partials(partials + ((int64_t)inst * 3 + 2) * nb, nb);
+    if (epsInst) eps = epsInst[inst];
     s.eps = eps;
     s.rule = rule;
     s.maxit = maxit;
@@ -1164,9 +1165,9 @@ void launch_cg_init(const V* f, V* r, V* h, V* dv, const V* D, int64_t n, int ba
                     hipStream_t s) {
     hipLaunchKernelGGL(cg_init_kernel<V>, dim3(nb, batch), dim3(256), 0, s, f, r, h, dv, D, n, n, partials);
 }
-void launch_cg_start(CGState* st, const double* partials, int nb, double eps, int rule, int maxit, int batch,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(cg_start_scalar, dim3(batch), dim3(64), 0, s, st, partials, nb, eps, rule, maxit);
+void launch_cg_start(CGState* st, const double* partials, int nb, double eps, const double* epsInst, int rule,
+                     int maxit, int batch, hipStream_t s) {
+    hipLaunchKernelGGL(cg_start_scalar, dim3(batch), dim3(64), 0, s, st, partials, nb, eps, epsInst, rule, maxit);
 }
 template <typename V>
 void launch_cg_iter_tail(V* r, void* x, bool x64, V* h, V* dv, const V* g, const V* D, int64_t n, int batch,

@@ -62,8 +62,9 @@ void launch_rhs_sines(const SumItem* items, int nitems, int d, int nterms, const
 template <typename V>
 void launch_cg_init(const V* f, V* r, V* h, V* dv, const V* D, int64_t n, int batch, double* partials, int nb,
                     hipStream_t s);
-void launch_cg_start(CGState* st, const double* partials, int nb, double eps, int rule, int maxit, int batch,
-                     hipStream_t s);
+// epsInst: per-instance tolerances (device array) or null for eps
+void launch_cg_start(CGState* st, const double* partials, int nb, double eps, const double* epsInst, int rule,
+                     int maxit, int batch, hipStream_t s);
 template <typename V>
 void launch_cg_iter_tail(V* r, void* x, bool x64, V* h, V* dv, const V* g, const V* D, int64_t n, int batch,
                          double* partials, int nb, CGState* st, hipStream_t s);
