@@ -532,14 +532,15 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
 }
 
 // ------------------------------------------------ apply, one separable term
-// Horner sweep of the separable apply kernel.  The first sweep (i = d-1) reads P
-// from Y, starts Z from zero and, in its write phase, keeps P in `keep` for the
-// prolongation of the next box; the last sweep (i = 0) does not update Y and adds
-// the gM value g0 read for this box (held in registers) to Z.
+// Horner sweep of the separable apply kernel.  The first sweep reads P from Y, starts
+// Z from zero and, in its write phase, keeps P in `keep` for the prolongation of the
+// next box; the last sweep does not store Y, folds in the dimensions of extent 1
+// (Z = cM Z + cK Y, see apply_sep_kernel) and adds the gM value g0 read for this box
+// (held in registers).
 template <typename V, int PPT>
 __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const Shape& sh, int i,
                                  const V* __restrict__ w, int t, bool first, bool needY, V* __restrict__ keep,
-                                 bool addG, const V (&g0)[PPT]) {
+                                 bool last, bool fold, V cM, V cK, bool addG, const V (&g0)[PPT]) {
     const int S = sh.S[i], n = sh.n[i], total = sh.total;
     const float rS = sh.rS[i], rn = sh.rn[i];
     const V ih = V(1 << t);
@@ -573,8 +574,13 @@ __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const Sha
     for (int s = 0; s < PPT; ++s) {
         const int p = threadIdx.x + s * blockDim.x;
         if (p < total) {
-            if (needY) Y[p] = ny[s];
-            Z[p] = addG ? nz[s] + g0[s] : nz[s];
+            V z = nz[s];
+            if (last && fold) {
+                const V kz = cK * ny[s];
+                z = fma(cM, z, kz);     // explicit, so every instantiation rounds alike
+            }
+            if (needY && !last) Y[p] = ny[s];
+            Z[p] = (last && addG) ? z + g0[s] : z;
             if (keep) keep[p] = yk[s];
         }
     }
@@ -713,12 +719,51 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
             }
         }
         __syncthreads();
-        // z = A_T P (Horner sum factorisation), + gM read
+        // z = A_T P (Horner sum factorisation), + gM read.  Along a dimension of extent 1
+        // (T_i = 1) the 1D operators are the scalars m_i = (w0 + w1)/6 and
+        // k_i = 2 (w0 + w1); Horner steps commute, so those dimensions are applied last
+        // in closed form: Z = cM A' P + cK M' P with A', M' the stiffness and mass
+        // operators of the other dimensions and (cM, cK) from the scalar recurrence.
         const Shape& sh = shs[0];
         const int wq = (1 << (T0fd + q)) - (1 << T0fd);
-        for (int i = d - 1; i >= 0; --i)
-            sep_sweep<V, PPT>(Y, Z, sh, i, W + woff[i] + (i == fd ? wq : 0), it.T[i], i == d - 1, i > 0,
-                              (i == d - 1 && q + 1 < fam.nt) ? U : nullptr, i == 0 && rd0, g0);
+        int hi = -1, lo = -1;
+        bool fold = false;
+        V cM = V(1), cK = V(0);
+        for (int i = d - 1; i >= 0; --i) {
+            if (it.T[i] > 1) {
+                if (hi < 0) hi = i;
+                lo = i;
+            } else {
+                fold = true;
+                const V* w = W + woff[i] + (i == fd ? wq : 0);
+                const V s01 = w[0] + w[1];
+                const V m = s01 / V(6), k = V(2) * s01;
+                const V kM = k * cM;
+                cK = fma(m, cK, kM);
+                cM = m * cM;
+            }
+        }
+        const bool keepP = q + 1 < fam.nt;
+        if (hi < 0) {
+            // every extent is 1 (a single point): Z = cK P
+#pragma unroll
+            for (int s = 0; s < PPT; ++s) {
+                const int p = threadIdx.x + s * blockDim.x;
+                if (p < total) {
+                    const V yc = Y[p];
+                    const V z = cK * yc;
+                    Z[p] = rd0 ? z + g0[s] : z;
+                    if (keepP) U[p] = yc;
+                }
+            }
+            __syncthreads();
+        } else {
+            for (int i = hi; i >= lo; --i) {
+                if (it.T[i] == 1) continue;
+                sep_sweep<V, PPT>(Y, Z, sh, i, W + woff[i] + (i == fd ? wq : 0), it.T[i], i == hi, true,
+                                  (i == hi && keepP) ? U : nullptr, i == lo, fold, cM, cK, i == lo && rd0, g0);
+            }
+        }
         // restrictions along the restriction dims (ascending) with the gM exchange
         V* cur = Z;
         V* oth = Y;
@@ -742,17 +787,26 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
             V* tmp = cur; cur = oth; oth = tmp;
         }
         // prewavelet conversion, d contractions box -> block; the last one writes the part
+        // (extent 1 along a dimension means t = 1 there, where the conversion is the identity)
         const Shape& sf = shs[it.k];
         int cnt = sf.total;
         int Slo = 1;
         V* out = a.part + inst * a.pStride + it.out_off;
-        for (int i = 0; i < d; ++i) {
+        int lastC = -1;
+        for (int i = 0; i < d; ++i)
+            if (sf.n[i] > 1) lastC = i;
+        if (lastC < 0) {
+            if (threadIdx.x == 0) out[0] = cur[0];
+            __syncthreads();
+        }
+        for (int i = 0; i <= lastC; ++i) {
             const int n = sf.n[i];
+            if (n == 1) continue;
             int t = it.T[i];
             for (int j = 0; j < it.k; ++j) if (it.r[j] == i) t -= 1;
             const int m = (n + 1) >> 1;
             const int outCnt = cnt / n * m;
-            contract_dim<V>(cur, i + 1 < d ? oth : out, Slo, n, outCnt, t);
+            contract_dim<V>(cur, i < lastC ? oth : out, Slo, n, outCnt, t);
             cnt = outCnt;
             Slo *= m;
             __syncthreads();
