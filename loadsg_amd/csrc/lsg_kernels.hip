@@ -29,13 +29,20 @@ struct Shape {
 };
 
 __device__ inline int fdiv(int p, float r) { return __float2int_rz((float(p) + 0.5f) * r); }
+// Reciprocal of a positive integer for fdiv: the hardware reciprocal (1 ulp) keeps the
+// quotient exact, since its relative error times (p + 0.5)/n stays below 2^-6/n,
+// under the 0.5/n distance of (p + 0.5)/n from an integer.
+__device__ inline float rcpf(int n) { return __builtin_amdgcn_rcpf(float(n)); }
+// 1 / (6 * 2^t), exactly as the correctly rounded quotient (scaling by 2^-t is exact)
+template <typename V>
+__device__ inline V inv6pow2(int t) { return ldexp(V(1.0) / V(6.0), -t); }
 
 __device__ inline void shape_restage(Shape& sh) {
     int s = 1;
     for (int q = 0; q < sh.d; ++q) {
         sh.S[q] = s;
-        sh.rS[q] = 1.0f / float(s);
-        sh.rn[q] = 1.0f / float(sh.n[q]);
+        sh.rS[q] = rcpf(s);
+        sh.rn[q] = rcpf(sh.n[q]);
         s *= sh.n[q];
     }
     sh.total = s;
@@ -94,7 +101,7 @@ __device__ inline Split split(int p, int Slo, float rSlo, int n, float rn) {
 template <typename V>
 __device__ inline void expand_dim(const V* __restrict__ src, V* __restrict__ dst, int Slo, int m, int n, int outCnt,
                                   int t) {
-    const float rSlo = 1.0f / float(Slo), rn = 1.0f / float(n);
+    const float rSlo = rcpf(Slo), rn = rcpf(n);
     for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
         Split s = split(p, Slo, rSlo, n, rn);
         const V* b = src + s.lo + s.hi * Slo * m;
@@ -119,7 +126,7 @@ template <typename V>
 __device__ inline void contract_dim(const V* __restrict__ src, V* __restrict__ dst, int Slo, int n, int outCnt,
                                     int t) {
     const int m = (n + 1) >> 1;
-    const float rSlo = 1.0f / float(Slo), rm = 1.0f / float(m);
+    const float rSlo = rcpf(Slo), rm = rcpf(m);
     for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
         Split s = split(p, Slo, rSlo, m, rm);
         const V* b = src + s.lo + s.hi * Slo * n;
@@ -137,7 +144,7 @@ template <typename V>
 __device__ inline void restrict_dim(const V* __restrict__ src, V* __restrict__ dst, const Shape& sh, int r) {
     const int nf = sh.n[r], nc = (nf - 1) >> 1, S = sh.S[r];
     const int outCnt = sh.total / nf * nc;
-    const float rS = sh.rS[r], rnc = 1.0f / float(nc);
+    const float rS = sh.rS[r], rnc = rcpf(nc);
     for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
         Split s = split(p, S, rS, nc, rnc);
         const V* b = src + s.lo + s.hi * S * nf + (2 * s.k + 1) * S;
@@ -182,7 +189,7 @@ __device__ inline void horner_step(V* __restrict__ Y, V* __restrict__ Z, const S
     const int S = sh.S[i], n = sh.n[i], total = sh.total;
     const float rS = sh.rS[i], rn = sh.rn[i];
     const V ih = V(1 << t);
-    const V h6 = V(1.0) / (V(6.0) * V(1 << t));
+    const V h6 = inv6pow2<V>(t);
     V ny[PPT], nz[PPT];
 #pragma unroll
     for (int s = 0; s < PPT; ++s) {
@@ -222,7 +229,7 @@ template <typename V, int PPT>
 __device__ inline void mass_step(V* __restrict__ Y, const Shape& sh, int i, const V* __restrict__ w, int t) {
     const int S = sh.S[i], n = sh.n[i], total = sh.total;
     const float rS = sh.rS[i], rn = sh.rn[i];
-    const V h6 = V(1.0) / (V(6.0) * V(1 << t));
+    const V h6 = inv6pow2<V>(t);
     V ny[PPT];
 #pragma unroll
     for (int s = 0; s < PPT; ++s) {
@@ -351,7 +358,7 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
             if (p < total) nin[s] = in[p];
         }
     }
-    const float rS = 1.0f / float(S);
+    const float rS = rcpf(S);
     for (int q = 0; q < it.nt; ++q) {
         const bool keep = q + 1 < it.nt;
         V v[PPT];
@@ -360,7 +367,7 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
             for (int s = 0; s < PPT; ++s) v[s] = nin[s];
         } else {
             const int nc = (nf - 1) >> 1;
-            const float rnf = 1.0f / float(nf);
+            const float rnf = rcpf(nf);
 #pragma unroll
             for (int s = 0; s < PPT; ++s) {
                 const int p = threadIdx.x + s * blockDim.x;
@@ -544,7 +551,7 @@ __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const Sha
     const int S = sh.S[i], n = sh.n[i], total = sh.total;
     const float rS = sh.rS[i], rn = sh.rn[i];
     const V ih = V(1 << t);
-    const V h6 = V(1.0) / (V(6.0) * V(1 << t));
+    const V h6 = inv6pow2<V>(t);
     V ny[PPT], nz[PPT], yk[PPT];
 #pragma unroll
     for (int s = 0; s < PPT; ++s) {
@@ -664,7 +671,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
             }
         } else {
             const int nc = (nfd - 1) >> 1;
-            const float rS = 1.0f / float(Sfd), rnf = 1.0f / float(nfd);
+            const float rS = rcpf(Sfd), rnf = rcpf(nfd);
 #pragma unroll
             for (int s = 0; s < PPT; ++s) {
                 const int p = threadIdx.x + s * blockDim.x;
@@ -772,7 +779,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
             const Shape& s0 = shs[j - 1];
             const int nf = s0.n[r], nc = (nf - 1) >> 1, S = s0.S[r];
             const int outCnt = shs[j].total;
-            const float rS = s0.rS[r], rnc = 1.0f / float(nc);
+            const float rS = s0.rS[r], rnc = rcpf(nc);
             const bool w = it.w_off[j] >= 0, rd = it.rd_off[j] >= 0;
             V* wd = wb + it.w_off[j];
             const V* g = wb + it.rd_off[j];
