@@ -25,7 +25,7 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_lib", "libloadsg_amd.so")
+LIB_PATH = os.environ.get("LOADSG_AMD_LIB") or os.path.join(_HERE, "_lib", "libloadsg_amd.so")
 
 LSG_HOST_SLOT, LSG_DEVICE = 0, 1
 STOP_REF_ABS, STOP_REL = 0, 1

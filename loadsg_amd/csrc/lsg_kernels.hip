@@ -28,6 +28,13 @@ struct Shape {
     float rS[MAXD];   // 1/S
 };
 
+// The per-launch item tables are read-only for the kernels that use them: read them
+// through the constant address space so that the uniform loads become scalar loads
+// (their waits do not block the vector-memory loads in flight).
+#define LSG_CONST __attribute__((address_space(4)))
+template <typename T>
+__device__ inline const LSG_CONST T* cptr(const T* p) { return (const LSG_CONST T*)p; }
+
 __device__ inline int fdiv(int p, float r) { return __float2int_rz((float(p) + 0.5f) * r); }
 // Reciprocal of a positive integer for fdiv: the hardware reciprocal (1 ulp) keeps the
 // quotient exact, since its relative error times (p + 0.5)/n stays below 2^-6/n,
@@ -47,7 +54,8 @@ __device__ inline void shape_restage(Shape& sh) {
     }
     sh.total = s;
 }
-__device__ inline void shape_box(Shape& sh, const int32_t* T, int d) {
+template <typename TP>
+__device__ inline void shape_box(Shape& sh, TP T, int d) {
     if (threadIdx.x == 0) {
         sh.d = d;
         for (int i = 0; i < d; ++i) sh.n[i] = (1 << T[i]) - 1;
@@ -263,7 +271,7 @@ __global__ void __launch_bounds__(256) nodal_kernel(const NodalItem* __restrict_
     V* B = A + maxBox;
     const int inst = blockIdx.y;
     if (st && !st[inst].active) return;
-    const NodalItem& it = items[blockIdx.x];
+    const LSG_CONST NodalItem& it = cptr(items)[blockIdx.x];
     const V* xin = x + inst * xStride + it.in_off;
     V* out = nodal + inst * nStride + it.out_off;
     int cnt = 1;
@@ -300,7 +308,7 @@ __global__ void __launch_bounds__(256) prol_kernel(const ProlItem* __restrict__ 
     __shared__ Shape sh;
     const int inst = blockIdx.y;
     if (st && !st[inst].active) return;
-    const ProlItem& it = items[blockIdx.x];
+    const LSG_CONST ProlItem& it = cptr(items)[blockIdx.x];
     const int dd = it.dd;
     const V* inBase = it.from_nodal ? nodal + inst * nStride : cbuf + inst * cStride;
     V* outBase = cbuf + inst * cStride;
@@ -338,7 +346,7 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
     V* cur = prev + maxBox;
     const int inst = blockIdx.x % batch;
     if (st && !st[inst].active) return;
-    const ProlItem& it = items[blockIdx.x / batch];
+    const LSG_CONST ProlItem& it = cptr(items)[blockIdx.x / batch];
     const int dd = it.dd;
     const V* inBase = it.from_nodal ? nodal + inst * nStride : cbuf + inst * cStride;
     V* outBase = cbuf + inst * cStride;
@@ -417,9 +425,9 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
 // ------------------------------------------------------------------ apply
 // weights of one separable factor for the box's depth: dim i, level T_i,
 // 2^T_i cell-centre values, copied from the instance's 1D tables into LDS.
-template <typename V>
+template <typename V, typename TP>
 __device__ inline void load_weights(V* __restrict__ W, int* woff, const V* __restrict__ tab, int64_t tabLen,
-                                    const int32_t* T, int d) {
+                                    TP T, int d) {
     int o = 0;
     for (int i = 0; i < d; ++i) {
         if (threadIdx.x == 0) woff[i] = o;
@@ -449,13 +457,13 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
     // the families of a launch by decreasing work, so the longest start first
     const int inst = blockIdx.x % a.batch;
     if (a.st && !a.st[inst].active) return;
-    const FamilyItem fam = a.fam[blockIdx.x / a.batch];
+    const LSG_CONST FamilyItem& fam = cptr(a.fam)[blockIdx.x / a.batch];
     const int d = a.d;
     const V* coef = a.coefTab + inst * a.coefStride;
     V* wbase = a.wbuf + inst * a.wStride;
 
     for (int q = 0; q < fam.nt; ++q) {
-        const ApplyItem& it = a.items[fam.first + q];
+        const LSG_CONST ApplyItem& it = cptr(a.items)[fam.first + q];
         shape_box(sh, it.T, d);
         const V* in = (it.in_nodal ? a.nodal + inst * a.nStride : a.cbuf + inst * a.cStride) + it.in_off;
         copy_in(Y, in, sh.total);
@@ -616,14 +624,14 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
     // the families of a launch by decreasing work, so the longest start first
     const int inst = blockIdx.x % a.batch;
     if (a.st && !a.st[inst].active) return;
-    const FamilyItem fam = a.fam[blockIdx.x / a.batch];
+    const LSG_CONST FamilyItem& fam = cptr(a.fam)[blockIdx.x / a.batch];
     const int d = a.d;
     const int fd = fam.fd;
     const V* coef = a.coefTab + inst * a.coefStride;
     const V* nodal = a.nodal + inst * a.nStride;
     const V* cbuf = a.cbuf + inst * a.cStride;
     V* wb = a.wbuf + inst * a.wStride;
-    const ApplyItem& it0 = a.items[fam.first];
+    const LSG_CONST ApplyItem& it0 = cptr(a.items)[fam.first];
     const int T0fd = fd >= 0 ? it0.T[fd] : 0;
 
     // weights of the whole family: dim i at woff[i]; along fd, level T0fd + q at
@@ -653,7 +661,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
     }
 
     for (int q = 0; q < fam.nt; ++q) {
-        const ApplyItem& it = a.items[fam.first + q];
+        const LSG_CONST ApplyItem& it = cptr(a.items)[fam.first + q];
         int total = 1, Sfd = 1, nfd = 1;
         for (int i = 0; i < d; ++i) {
             const int n = (1 << it.T[i]) - 1;
@@ -704,7 +712,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
             }
         }
         if (q + 1 < fam.nt) {
-            const ApplyItem& itn = a.items[fam.first + q + 1];
+            const LSG_CONST ApplyItem& itn = cptr(a.items)[fam.first + q + 1];
             const int totalN = total / nfd * ((nfd << 1) + 1);
             const V* in = (itn.in_nodal ? nodal : cbuf) + itn.in_off;
 #pragma unroll
@@ -832,14 +840,14 @@ __global__ void __launch_bounds__(256) sum_kernel(const SumItem* __restrict__ it
                                                   const CGState* __restrict__ st) {
     const int inst = blockIdx.y;
     if (st && !st[inst].active) return;
-    const SumItem& it = items[blockIdx.x];
+    const LSG_CONST SumItem& it = cptr(items)[blockIdx.x];
     int cnt = 1;
     for (int i = 0; i < d; ++i) cnt *= 1 << (it.T[i] - 1);
     const V* pb = part + inst * pStride;
     V* out = y + inst * yStride + it.blk_off;
     for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
         V s = V(0);
-        for (int c = 0; c < it.ncase; ++c) s = s + pb[list[it.list_start + c] + p];
+        for (int c = 0; c < it.ncase; ++c) s = s + pb[cptr(list)[it.list_start + c] + p];
         out[p] = s;
     }
 }
@@ -853,7 +861,7 @@ __global__ void __launch_bounds__(256) diag_kernel(const SumItem* __restrict__ i
                                                    const V* __restrict__ coefTab, int64_t coefStride, int64_t tabLen,
                                                    V* __restrict__ z, int64_t zStride) {
     const int inst = blockIdx.y;
-    const SumItem& it = items[blockIdx.x];
+    const LSG_CONST SumItem& it = cptr(items)[blockIdx.x];
     int m[MAXD];
     int cnt = 1;
     for (int i = 0; i < d; ++i) { m[i] = 1 << (it.T[i] - 1); cnt *= m[i]; }
@@ -913,7 +921,7 @@ __global__ void __launch_bounds__(256) rhs_sines_kernel(const SumItem* __restric
                                                         const double* __restrict__ eta, const int32_t* __restrict__ mfreq,
                                                         V* __restrict__ b, int64_t bStride) {
     const int inst = blockIdx.y;
-    const SumItem& it = items[blockIdx.x];
+    const LSG_CONST SumItem& it = cptr(items)[blockIdx.x];
     int m[MAXD];
     int cnt = 1;
     for (int i = 0; i < d; ++i) { m[i] = 1 << (it.T[i] - 1); cnt *= m[i]; }
