@@ -242,8 +242,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
                      "traffic": None if tr is None else tr * B,
-                     "kernel": "apply_kernel<float>: the %d apply launches of one FP32 matvec of batch %d "
-                               "(one 'launch' = one matvec's apply stage)" % (launches, B),
+                     "kernel": "apply_sep_kernel<float> (separable-coefficient apply stage): the %d apply "
+                               "launches of one FP32 matvec of batch %d (one 'launch' = one matvec's apply stage)"
+                               % (launches, B),
                      "algorithmic_bytes": apply_bytes, "time_ms": apply_ms,
                      "kernel_io_GBps": own_bw / 1e9,
                      "traffic_GBps": None if tr is None else tr * B / (apply_ms * 1e-3) / 1e9},
