@@ -123,7 +123,7 @@ int lsg_rhs_sines(lsg_op* op, int nterms, const double* eta, const int32_t* m, v
 /* Device timing of the last lsg_matvec: out[0]=total ms, out[1]=apply-kernel ms,
  * out[2]=number of apply launches. */
 int lsg_last_timing(const lsg_op* op, double* out, int nout);
-/* Tuning: threads per workgroup for the apply stage (64, 128, 256, 512); 0 = autotune. */
+/* Tuning: threads per workgroup for the apply stage (a multiple of 64 up to 512); 0 = autotune. */
 int lsg_op_set_block(lsg_op* op, int stage, int threads);
 int lsg_op_destroy(lsg_op* op);
 

@@ -231,7 +231,7 @@ bool threadsFit(const lsg_op* op, int cls, int threads) {
 }
 
 int chooseThreads(lsg_op* op, int threads) {
-    const int cands[4] = {64, 128, 256, 512};
+    const int cands[8] = {64, 128, 192, 256, 320, 384, 448, 512};
     if (threads) {
         for (int c = 0; c < op->nClasses; ++c) {
             if (!threadsFit(op, c, threads)) return fail(LSG_ERR_UNSUPPORTED, "apply workgroup exceeds LDS");
@@ -644,8 +644,8 @@ int lsg_op_batch(const lsg_op* op) { return op ? op->batch : -1; }
 
 int lsg_op_set_block(lsg_op* op, int stage, int threads) {
     if (!op || stage != 0) return fail(LSG_ERR_ARG, "only stage 0 (apply) is tunable");
-    if (threads != 0 && threads != 64 && threads != 128 && threads != 256 && threads != 512)
-        return fail(LSG_ERR_ARG, "threads must be 64, 128, 256 or 512");
+    if (threads < 0 || threads > 512 || threads % 64 != 0)
+        return fail(LSG_ERR_ARG, "threads must be a multiple of 64 up to 512");
     return chooseThreads(op, threads);
 }
 

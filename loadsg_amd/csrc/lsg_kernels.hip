@@ -1205,8 +1205,12 @@ int launch_apply(const ApplyArgs<V>& a, int nitems, int batch, int threads, size
     }
     LSG_APPLY_CASE(1)
     LSG_APPLY_CASE(2)
+    LSG_APPLY_CASE(3)
     LSG_APPLY_CASE(4)
+    LSG_APPLY_CASE(5)
+    LSG_APPLY_CASE(6)
     LSG_APPLY_CASE(8)
+    LSG_APPLY_CASE(12)
     LSG_APPLY_CASE(16)
 #undef LSG_APPLY_CASE
     return -1;
