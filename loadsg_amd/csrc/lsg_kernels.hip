@@ -1183,8 +1183,11 @@ void launch_prol(const ProlItem* items, int nitems, int d, const V* nodal, int64
     }
     LSG_PROL_CASE(1)
     LSG_PROL_CASE(2)
+    LSG_PROL_CASE(3)
     LSG_PROL_CASE(4)
+    LSG_PROL_CASE(6)
     LSG_PROL_CASE(8)
+    LSG_PROL_CASE(12)
     LSG_PROL_CASE(16)
 #undef LSG_PROL_CASE
     hipLaunchKernelGGL(prol_kernel<V>, dim3(nitems, batch), dim3(256), lds, s, items, d, nodal, nStride, cbuf, cStride,
