@@ -191,7 +191,7 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
             a.cbits = g.maxLevel <= cb ? cb : 0;
         }
         const int thr = op->threadsOf[L.cls] ? op->threadsOf[L.cls] : 256;
-        const size_t lds = ((op->general ? 4 : 3) * L.maxBox + L.maxW) * sizeof(V);
+        const size_t lds = ((op->general ? 4 : 2) * L.maxBox + L.maxW) * sizeof(V);
         if (launch_apply<V>(a, L.count, B, thr, lds, s) != 0)
             return fail(LSG_ERR_UNSUPPORTED, "depth box too large for the chosen block size");
     }
@@ -219,7 +219,8 @@ int64_t classMax(const lsg_op* op, int cls) {
 // LDS bytes of one apply workgroup (one family of one instance)
 int64_t applyLdsBytes(const lsg_op* op, int64_t maxBox, int64_t maxW, int threads) {
     (void)threads;
-    return ((op->general ? 4 : 3) * maxBox + maxW) * (int64_t)op->vs;
+    // general: U, Y, Z, Zt; separable: two boxes (P parked between them)
+    return ((op->general ? 4 : 2) * maxBox + maxW) * (int64_t)op->vs;
 }
 
 bool threadsFit(const lsg_op* op, int cls, int threads) {
@@ -536,7 +537,7 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
             // instances per workgroup: about 4096 points of work per sweep, at most
             // 64 KB of LDS, exact fast division (G * box < 2^16)
             {
-                const int64_t perInst = ((op->general ? 4 : 3) * la.maxBox + la.maxW) * (int64_t)op->vs;
+                const int64_t perInst = ((op->general ? 4 : 2) * la.maxBox + la.maxW) * (int64_t)op->vs;
                 int64_t G = std::max<int64_t>(1, 1024 / la.maxBox);
                 G = std::min<int64_t>(G, std::max<int64_t>(1, (48 * 1024) / perInst));
                 G = std::min<int64_t>(G, 65535 / la.maxBox);

@@ -614,10 +614,14 @@ __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const Sha
 template <typename V, int PPT>
 __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
     extern __shared__ char smem_raw[];
+    // two LDS boxes A, B and the weights.  At the start of a box, U (one of A, B) holds
+    // P of the previous box; P of this box is built in the other one (Y), the Horner
+    // sweeps run in place on Y and on U (Z, free once P is built), and at the end of
+    // the box P (kept in registers) is parked in the buffer the last pass left free.
     V* U = reinterpret_cast<V*>(smem_raw);
     V* Y = U + a.maxBox;
-    V* Z = Y + a.maxBox;
-    V* W = Z + a.maxBox;
+    V* Z = U;
+    V* W = Y + a.maxBox;
     __shared__ int woff[MAXD];
     __shared__ Shape shs[MAXD + 1];
     // family-major order: all instances of a family are adjacent, and the host lists
@@ -671,11 +675,15 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
         }
         // P = input (+ I_fd(P of the previous box), prolongation1D_inplace +
         // addMultiLevelValues, MatrixVectorHomogen.h:444-474,997-1019)
+        V pk[PPT];      // P of this box, parked for the next box at the end
         if (q == 0) {
 #pragma unroll
             for (int s = 0; s < PPT; ++s) {
                 const int p = threadIdx.x + s * blockDim.x;
-                if (p < total) Y[p] = nin[s];
+                if (p < total) {
+                    Y[p] = nin[s];
+                    pk[s] = nin[s];
+                }
             }
         } else {
             const int nc = (nfd - 1) >> 1;
@@ -696,10 +704,13 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
                     } else {
                         add = c[((k >> 1) - 1) * Sfd];
                     }
-                    Y[p] = nin[s] + add;
+                    const V v = nin[s] + add;
+                    Y[p] = v;
+                    pk[s] = v;
                 }
             }
         }
+        Z = U;
         // prefetch: gM value of this box and the input of the next box
         V g0[PPT];
         const bool rd0 = it.rd_off[0] >= 0;
@@ -768,7 +779,6 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
                     const V yc = Y[p];
                     const V z = cK * yc;
                     Z[p] = rd0 ? z + g0[s] : z;
-                    if (keepP) U[p] = yc;
                 }
             }
             __syncthreads();
@@ -776,7 +786,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
             for (int i = hi; i >= lo; --i) {
                 if (it.T[i] == 1) continue;
                 sep_sweep<V, PPT>(Y, Z, sh, i, W + woff[i] + (i == fd ? wq : 0), it.T[i], i == hi, true,
-                                  (i == hi && keepP) ? U : nullptr, i == lo, fold, cM, cK, i == lo && rd0, g0);
+                                  nullptr, i == lo, fold, cM, cK, i == lo && rd0, g0);
             }
         }
         // restrictions along the restriction dims (ascending) with the gM exchange
@@ -824,9 +834,22 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
             contract_dim<V>(cur, i < lastC ? oth : out, Slo, n, outCnt, t);
             cnt = outCnt;
             Slo *= m;
-            __syncthreads();
-            V* tmp = cur; cur = oth; oth = tmp;
+            if (i < lastC) {
+                __syncthreads();
+                V* tmp = cur; cur = oth; oth = tmp;
+            }
         }
+        // park P in the free buffer (the last pass read cur and wrote global memory)
+        if (keepP) {
+#pragma unroll
+            for (int s = 0; s < PPT; ++s) {
+                const int p = threadIdx.x + s * blockDim.x;
+                if (p < total) oth[p] = pk[s];
+            }
+        }
+        __syncthreads();
+        U = oth;
+        Y = cur;
     }
 }
 
