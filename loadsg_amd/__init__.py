@@ -308,6 +308,25 @@ class CG:
         return res[0] if op64.batch == 1 else res
 
 
+def solve_many(problems, rel_tol, inner_rel_tol=1e-4, maxit=30):
+    """Mixed-precision solves of instances on different grids at once.
+
+    problems: list of (x, f, op64, op32), each an operator pair with its own grid and
+    batch and device tensors x, f of shape (batch, N) (see CG.solveMixed).  Every
+    operator pair runs on its own stream, driven by its own host thread, so the
+    groups share the GPU.  Returns the per-group lists of solve infos."""
+    from concurrent.futures import ThreadPoolExecutor
+    import torch
+    for x, f, _, _ in problems:
+        _sync_torch(x, f)
+    torch.cuda.synchronize()
+    with ThreadPoolExecutor(max(1, len(problems))) as ex:
+        futs = [ex.submit(CG.solveMixed, rel_tol, x, f, o64, o32, inner_rel_tol, maxit)
+                for (x, f, o64, o32) in problems]
+        res = [fu.result() for fu in futs]
+    return [r if isinstance(r, list) else [r] for r in res]
+
+
 def device_available():
     return bool(lib().lsg_device_available())
 
