@@ -45,7 +45,7 @@ def inputs(batch, rng):
 def apply_traffic_from_profile():
     """HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the apply stage of one
     FP32 matvec per batch instance, as recorded from rocprofv3 --pmc runs."""
-    p = os.path.join(ROOT, "profiles", "apply_kernel_fp32_d5l7_b64_pmc.md")
+    p = os.path.join(ROOT, "profiles", "apply_sep_kernel_fp32_d5l7_b64_pmc.md")
     if not os.path.exists(p):
         return None
     for line in open(p):
