@@ -137,6 +137,12 @@ struct lsg_op {
     std::vector<std::unique_ptr<DevMem>> dProl;   // per sweep dim
     std::vector<int64_t> prolMax;                 // largest box per sweep dim
     DevMem coefTab, nodal, cbuf, wbuf, part, xs, ys;
+    // sparse path: gather tables, per-level apply families, scratch
+    bool sparse = false;
+    std::vector<std::unique_ptr<DevMem>> spTables;
+    SparseDev sd{};
+    std::vector<std::pair<int, std::unique_ptr<DevMem>>> spLevels;   // (count, families)
+    DevMem zbuf, activeDev;
     // solver buffers
     DevMem r, h, dv, g, D, f, x, tmp, partials, st, aux;
     bool solverReady = false, diagValid = false;
@@ -149,7 +155,54 @@ struct lsg_op {
 namespace {
 
 template <typename V>
+int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st) {
+    const Grid& g = op->grid->g;
+    const Plan& P = g.plan;
+    const int d = g.d;
+    const int B = op->batch;
+    hipStream_t s = op->stream;
+    HIPCHK(hipEventRecord(op->ev[0], s));
+    launch_sp_nodal<V>(op->sd, x, g.n, op->nodal.as<V>(), g.nml, g.nml, B, st, s);
+    for (int dd = 0; dd < d; ++dd)
+        launch_sp_prol<V>(op->dProl[dd]->as<ProlItem>(), (int)P.prol[dd].size(), op->sd, op->nodal.as<V>(), g.nml,
+                          op->cbuf.as<V>(), P.caseSize, B, st, s);
+    HIPCHK(hipEventRecord(op->ev[1], s));
+    SparseApplyArgs<V> a{};
+    a.sp = op->sd;
+    a.items = op->dApplyItems.as<ApplyItem>();
+    a.nA = op->nA;
+    a.nC = op->nC;
+    a.coefTab = op->coefTab.as<V>();
+    a.coefStride = (int64_t)(op->nA + op->nC) * d * op->tabLen;
+    a.tabLen = op->tabLen;
+    a.nodal = op->nodal.as<V>();
+    a.nStride = g.nml;
+    a.cbuf = op->cbuf.as<V>();
+    a.cStride = P.caseSize;
+    a.wbuf = op->wbuf.as<V>();
+    a.wStride = P.wSize;
+    a.part = op->part.as<V>();
+    a.pStride = P.partSize;
+    a.zbuf = op->zbuf.as<V>();
+    a.zStride = P.zSize;
+    a.batch = B;
+    a.st = st;
+    for (auto& L : op->spLevels) {
+        a.fam = L.second->as<FamilyItem>();
+        launch_sp_apply<V>(a, L.first, s);
+    }
+    HIPCHK(hipEventRecord(op->ev[2], s));
+    launch_sum<V>(op->dSumItems.as<SumItem>(), (int)P.sum.size(), op->dSumList.as<int64_t>(), d, op->part.as<V>(),
+                  P.partSize, y, g.n, B, st, s);
+    HIPCHK(hipEventRecord(op->ev[3], s));
+    HIPCHK(hipGetLastError());
+    op->timed = true;
+    return LSG_OK;
+}
+
+template <typename V>
 int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
+    if (op->sparse) return runMatvecSparse<V>(op, x, y, st);
     const Grid& g = op->grid->g;
     const Plan& P = g.plan;
     const int d = g.d;
@@ -324,6 +377,19 @@ int computeDiag(lsg_op* op) {
     if (op->diagValid) return LSG_OK;
     const Grid& g = op->grid->g;
     int64_t cs = (int64_t)(op->nA + op->nC) * g.d * op->tabLen;
+    if (op->sparse) {
+        if (op->prec == 64)
+            launch_sp_diag<double>(op->sd, op->activeDev.as<uint8_t>(), (int)g.n, op->nA, op->nC,
+                                   op->coefTab.as<double>(), cs, op->tabLen, op->D.as<double>(), g.n, op->batch,
+                                   op->stream);
+        else
+            launch_sp_diag<float>(op->sd, op->activeDev.as<uint8_t>(), (int)g.n, op->nA, op->nC,
+                                  op->coefTab.as<float>(), cs, op->tabLen, op->D.as<float>(), g.n, op->batch,
+                                  op->stream);
+        HIPCHK(hipGetLastError());
+        op->diagValid = true;
+        return LSG_OK;
+    }
     if (op->prec == 64)
         launch_diag<double>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, op->nA, op->nC,
                             op->coefTab.as<double>(), cs, op->tabLen, op->D.as<double>(), g.n, op->batch, op->stream);
@@ -485,11 +551,13 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     if (op->nA + op->nC == 0) return fail(LSG_ERR_ARG, "zero operator");
     op->general = !(op->nA == 1 && op->nC == 0);
     op->tabLen = (int64_t(1) << (g.maxLevel + 1)) - 2;
-    if (P.maxBox >= 65536) return fail(LSG_ERR_UNSUPPORTED, "depth boxes above 65535 points are not supported");
+    op->sparse = g.sparse;
+    if (!op->sparse && P.maxBox >= 65536)
+        return fail(LSG_ERR_UNSUPPORTED, "depth boxes above 65535 points are not supported");
     // size classes of the apply launches (largest box of a family)
     const int64_t bounds[5] = {128, 512, 2048, 8192, 65536};
     const int ncls = 5;
-    for (size_t L = 0; L < P.apply.size(); ++L) {
+    for (size_t L = 0; !op->sparse && L < P.apply.size(); ++L) {
         std::vector<std::vector<FamilyItem>> bucket(ncls);
         std::vector<int64_t> bmax(ncls, 0), wmax(ncls, 0);
         for (const FamilyItem& f : P.apply[L]) {
@@ -564,7 +632,7 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
             return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS (" + std::to_string(lds) + " > " +
                                                  std::to_string(maxLds) + " bytes)");
     }
-    if ((int64_t)(2 * P.maxBox * op->vs) > maxLds)
+    if (!op->sparse && (int64_t)(2 * P.maxBox * op->vs) > maxLds)
         return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS");
     HIPCHK(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
     for (auto& e : op->ev) HIPCHK(hipEventCreate(&e));
@@ -614,6 +682,42 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     HIPCHK(op->ys.alloc((size_t)g.n * batch * vs));
     HIPCHK(hipMemset(op->xs.p, 0, op->xs.bytes));
     HIPCHK(op->coefTab.alloc((size_t)(op->nA + op->nC) * g.d * op->tabLen * batch * vs));
+    if (op->sparse) {
+        // gather tables of the sparse path, single-item families per wavefront level, scratch
+        const SparseTables& T = g.sp;
+        auto up = [&](const void* data, size_t bytes) -> const void* {
+            op->spTables.emplace_back(new DevMem());
+            DevMem& m = *op->spTables.back();
+            if (m.alloc(bytes) != hipSuccess) return nullptr;
+            if (bytes && hipMemcpy(m.p, data, bytes, hipMemcpyHostToDevice) != hipSuccess) return nullptr;
+            return m.p;
+        };
+#define LSG_UP(field, vec)                                                                    \
+    if (!(op->sd.field = (decltype(op->sd.field))up(vec.data(), vec.size() * sizeof(vec[0])))) \
+        return fail(LSG_ERR_DEVICE, "upload of the sparse tables failed");
+        LSG_UP(nodPtr, T.nodPtr) LSG_UP(nodCol, T.nodCol) LSG_UP(nodVal, T.nodVal)
+        LSG_UP(prolOff, T.prolOff) LSG_UP(prolAB, T.prolAB)
+        LSG_UP(cellOff, T.cellOff) LSG_UP(cellCorner, T.cellCorner) LSG_UP(cellLo, T.cellLo)
+        LSG_UP(rcPtr, T.rcPtr) LSG_UP(rcCell, T.rcCell) LSG_UP(rcCorner, T.rcCorner)
+        LSG_UP(restOff, T.restOff) LSG_UP(restSLR, T.restSLR)
+        LSG_UP(convPtr, T.convPtr) LSG_UP(convCol, T.convCol) LSG_UP(convVal, T.convVal)
+        LSG_UP(pos, T.pos) LSG_UP(tOfDev, T.tOfDev) LSG_UP(depthT, T.depthT)
+        LSG_UP(boxOff, g.boxOff) LSG_UP(boxSize, g.boxSize) LSG_UP(blkOff, g.blkOff) LSG_UP(blkSize, g.blkSize)
+#undef LSG_UP
+        op->sd.d = g.d;
+        op->sd.nd = (int)g.depths.size();
+        for (const auto& lev : P.apply) {
+            std::unique_ptr<DevMem> m(new DevMem());
+            HIPCHK(m->alloc(lev.size() * sizeof(FamilyItem)));
+            HIPCHK(hipMemcpy(m->p, lev.data(), lev.size() * sizeof(FamilyItem), hipMemcpyHostToDevice));
+            op->spLevels.emplace_back((int)lev.size(), std::move(m));
+        }
+        HIPCHK(op->zbuf.alloc((size_t)P.zSize * batch * vs));
+        std::vector<uint8_t> act(g.n);
+        for (int64_t s0 = 0; s0 < g.n; ++s0) act[g.devOfSlot[s0]] = g.active[s0];
+        HIPCHK(op->activeDev.alloc((size_t)g.n));
+        HIPCHK(hipMemcpy(op->activeDev.p, act.data(), (size_t)g.n, hipMemcpyHostToDevice));
+    }
     *out = op.release();
     for (int b = 0; b < batch; ++b) {
         int rc = lsg_op_set_coef(*out, b, coef);
@@ -803,12 +907,14 @@ int lsg_rhs_sines(lsg_op* op, int nterms, const double* eta, const int32_t* m, v
     HIPCHK(hipMemcpy(dEta.p, eta, (size_t)nterms * op->batch * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dM.p, m, (size_t)nterms * 4, hipMemcpyHostToDevice));
     void* dst = layout == LSG_DEVICE ? b : op->ys.p;
+    const int32_t* pos = op->sparse ? op->sd.pos : nullptr;
+    const uint8_t* act = op->sparse ? op->activeDev.as<uint8_t>() : nullptr;
     if (op->prec == 64)
         launch_rhs_sines<double>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, nterms, dEta.as<double>(),
-                                 dM.as<int32_t>(), (double*)dst, g.n, op->batch, op->stream);
+                                 dM.as<int32_t>(), (double*)dst, g.n, op->batch, op->stream, pos, act);
     else
         launch_rhs_sines<float>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, nterms, dEta.as<double>(),
-                                dM.as<int32_t>(), (float*)dst, g.n, op->batch, op->stream);
+                                dM.as<int32_t>(), (float*)dst, g.n, op->batch, op->stream, pos, act);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(op->stream));
     if (layout == LSG_DEVICE) return LSG_OK;

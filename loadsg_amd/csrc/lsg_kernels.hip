@@ -864,8 +864,7 @@ __global__ void __launch_bounds__(256) sum_kernel(const SumItem* __restrict__ it
     const int inst = blockIdx.y;
     if (st && !st[inst].active) return;
     const LSG_CONST SumItem& it = cptr(items)[blockIdx.x];
-    int cnt = 1;
-    for (int i = 0; i < d; ++i) cnt *= 1 << (it.T[i] - 1);
+    const int cnt = (int)it.count;
     const V* pb = part + inst * pStride;
     V* out = y + inst * yStride + it.blk_off;
     for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
@@ -942,12 +941,14 @@ __global__ void __launch_bounds__(256) diag_kernel(const SumItem* __restrict__ i
 template <typename V>
 __global__ void __launch_bounds__(256) rhs_sines_kernel(const SumItem* __restrict__ items, int d, int nterms,
                                                         const double* __restrict__ eta, const int32_t* __restrict__ mfreq,
-                                                        V* __restrict__ b, int64_t bStride) {
+                                                        V* __restrict__ b, int64_t bStride,
+                                                        const int32_t* __restrict__ pos,
+                                                        const uint8_t* __restrict__ active) {
     const int inst = blockIdx.y;
     const LSG_CONST SumItem& it = cptr(items)[blockIdx.x];
     int m[MAXD];
-    int cnt = 1;
-    for (int i = 0; i < d; ++i) { m[i] = 1 << (it.T[i] - 1); cnt *= m[i]; }
+    for (int i = 0; i < d; ++i) m[i] = 1 << (it.T[i] - 1);
+    const int cnt = (int)it.count;
     for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
         double total = 0.0;
         for (int term = 0; term < nterms; ++term) {
@@ -955,11 +956,13 @@ __global__ void __launch_bounds__(256) rhs_sines_kernel(const SumItem* __restric
             double prod = eta[inst * nterms + term];
             int rem = p;
             for (int i = 0; i < d; ++i) {
+                // position of the point at its own level: from the dense block order, or
+                // from the position table of a sparse grid
                 int j = rem % m[i];
                 rem /= m[i];
                 const int t = it.T[i];
                 const int n = (1 << t) - 1;
-                const int k = 2 * j + 1;
+                const int k = pos ? pos[(it.blk_off + p) * d + i] : 2 * j + 1;
                 const double h = 1.0 / double(1 << t);
                 const double fac = 2.0 * (1.0 - cos(w * h)) / (w * w * h);
                 double s = ((t >= 2 && (k == 1 || k == n)) ? 0.9 : 1.0) * sin(w * k * h);
@@ -969,6 +972,9 @@ __global__ void __launch_bounds__(256) rhs_sines_kernel(const SumItem* __restric
             }
             total += prod;
         }
+        // hanging nodes are no unknowns: their entries stay zero (active-only writes of
+        // VectorSparseG in the reference)
+        if (active && !active[it.blk_off + p]) total = 0.0;
         b[inst * bStride + it.blk_off + p] = V(total);
     }
 }
@@ -1255,8 +1261,9 @@ void launch_diag(const SumItem* items, int nitems, int d, int nA, int nC, const 
 }
 template <typename V>
 void launch_rhs_sines(const SumItem* items, int nitems, int d, int nterms, const double* eta, const int32_t* m, V* b,
-                      int64_t bStride, int batch, hipStream_t s) {
-    hipLaunchKernelGGL(rhs_sines_kernel<V>, dim3(nitems, batch), dim3(128), 0, s, items, d, nterms, eta, m, b, bStride);
+                      int64_t bStride, int batch, hipStream_t s, const int32_t* pos, const uint8_t* active) {
+    hipLaunchKernelGGL(rhs_sines_kernel<V>, dim3(nitems, batch), dim3(128), 0, s, items, d, nterms, eta, m, b, bStride,
+                       pos, active);
 }
 
 template <typename V>
@@ -1311,7 +1318,7 @@ int cg_partial_blocks(int64_t n) { return (int)gridFor(n); }
     template void launch_diag<V>(const SumItem*, int, int, int, int, const V*, int64_t, int64_t, V*, int64_t, int,   \
                                  hipStream_t);                                                                       \
     template void launch_rhs_sines<V>(const SumItem*, int, int, int, const double*, const int32_t*, V*, int64_t,     \
-                                      int, hipStream_t);                                                             \
+                                      int, hipStream_t, const int32_t*, const uint8_t*);                             \
     template void launch_cg_init<V>(const V*, V*, V*, V*, const V*, int64_t, int, double*, int, hipStream_t);         \
     template void launch_cg_iter_tail<V>(V*, void*, bool, V*, V*, const V*, const V*, int64_t, int, double*, int,     \
                                          CGState*, hipStream_t);

@@ -42,6 +42,68 @@ struct ApplyArgs {
     const CGState* st;    // may be null
 };
 
+// Device copies of the sparse-path gather tables (SparseTables, lsg_plan.hpp).
+struct SparseDev {
+    int d, nd;
+    const int64_t* nodPtr;
+    const int64_t* nodCol;
+    const double* nodVal;
+    const int64_t* prolOff;
+    const int32_t* prolAB;
+    const int64_t* cellOff;
+    const int32_t* cellCorner;
+    const int32_t* cellLo;
+    const int64_t* rcPtr;
+    const int32_t* rcCell;
+    const int32_t* rcCorner;
+    const int64_t* restOff;
+    const int32_t* restSLR;
+    const int64_t* convPtr;
+    const int32_t* convCol;
+    const double* convVal;
+    const int32_t* pos;
+    const int32_t* tOfDev;
+    const int32_t* depthT;
+    const int64_t* boxOff;
+    const int64_t* boxSize;
+    const int64_t* blkOff;
+    const int64_t* blkSize;
+};
+
+template <typename V>
+struct SparseApplyArgs {
+    SparseDev sp;
+    const FamilyItem* fam;    // single-item families of one wavefront level
+    const ApplyItem* items;
+    int nA, nC;
+    const V* coefTab;
+    int64_t coefStride, tabLen;
+    const V* nodal;
+    int64_t nStride;
+    const V* cbuf;
+    int64_t cStride;
+    V* wbuf;
+    int64_t wStride;
+    V* part;
+    int64_t pStride;
+    V* zbuf;
+    int64_t zStride;
+    int batch;
+    const CGState* st;
+};
+
+template <typename V>
+void launch_sp_nodal(const SparseDev& s, const V* x, int64_t xStride, V* nodal, int64_t nStride, int64_t nml,
+                     int batch, const CGState* st, hipStream_t str);
+template <typename V>
+void launch_sp_prol(const ProlItem* items, int nitems, const SparseDev& s, const V* nodal, int64_t nStride, V* cbuf,
+                    int64_t cStride, int batch, const CGState* st, hipStream_t str);
+template <typename V>
+void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str);
+template <typename V>
+void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, int nA, int nC, const V* coefTab,
+                    int64_t coefStride, int64_t tabLen, V* z, int64_t zStride, int batch, hipStream_t str);
+
 template <typename V>
 void launch_nodal(const NodalItem* items, int nitems, int d, const V* x, int64_t xStride, V* nodal, int64_t nStride,
                   int64_t maxBox, int batch, const CGState* st, hipStream_t s);
@@ -58,7 +120,8 @@ void launch_diag(const SumItem* items, int nitems, int d, int nA, int nC, const 
                  int64_t tabLen, V* z, int64_t zStride, int batch, hipStream_t s);
 template <typename V>
 void launch_rhs_sines(const SumItem* items, int nitems, int d, int nterms, const double* eta, const int32_t* m, V* b,
-                      int64_t bStride, int batch, hipStream_t s);
+                      int64_t bStride, int batch, hipStream_t s, const int32_t* pos = nullptr,
+                      const uint8_t* active = nullptr);
 template <typename V>
 void launch_cg_init(const V* f, V* r, V* h, V* dv, const V* D, int64_t n, int batch, double* partials, int nb,
                     hipStream_t s);

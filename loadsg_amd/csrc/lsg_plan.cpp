@@ -98,11 +98,261 @@ int64_t blockOf(const Depth& T, int d) {
     return s;
 }
 
+// ---- sparse path ------------------------------------------------------------
+// 1D index <-> position on the level-t grid (positions 1 .. 2^t - 1 are interior;
+// indices/index.h:657-876: index 2^s + n sits at position (2n+1) 2^(t-s)).
+inline int32_t posAt(int32_t v, int t) {
+    const int s = depth1(v);
+    return (2 * (v - (1 << s)) + 1) << (t - s);
+}
+inline int32_t idxAt(int32_t k, int t) {
+    const int tz = __builtin_ctz((uint32_t)k);
+    return (1 << (t - tz)) + (k >> (tz + 1));
+}
+// 1D prewavelet of the point at odd position k, level t: (position, coefficient)
+// pairs (IndexDimension::nextFive, indices/index.cc:180-327): centre 1 (0.9 next to
+// the boundary for t >= 2), neighbours -0.6, second neighbours 0.1, cut at the boundary.
+int prewavelet1D(int32_t k, int t, int32_t pos[5], double c[5]) {
+    const int32_t n = (1 << t);
+    int m = 0;
+    pos[m] = k;
+    c[m++] = (t >= 2 && (k == 1 || k == n - 1)) ? 0.9 : 1.0;
+    if (k - 1 >= 1) { pos[m] = k - 1; c[m++] = -0.6; pos[m] = k - 2; c[m++] = 0.1; }
+    if (k + 1 <= n - 1) { pos[m] = k + 1; c[m++] = -0.6; pos[m] = k + 2; c[m++] = 0.1; }
+    return m;
+}
+
+int buildSparse(Grid& g, const std::vector<int>& tOf, std::string& err) {
+    const int d = g.d;
+    const int nd = (int)g.depths.size();
+    const int64_t n = g.n;
+    SparseTables& S = g.sp;
+    S = SparseTables();
+    std::unordered_map<Key2, int64_t, Key2Hash> slotOf;
+    slotOf.reserve((size_t)n * 2);
+    for (int64_t s = 0; s < n; ++s) slotOf[packKey(&g.idx[s * d], d)] = s;
+    auto find = [&](const int32_t* v) -> int64_t {
+        auto it = slotOf.find(packKey(v, d));
+        return it == slotOf.end() ? -1 : it->second;
+    };
+    // G_t in slot order and the local index of every slot in it
+    std::vector<std::vector<int64_t>> G(nd);
+    std::vector<std::unordered_map<int64_t, int32_t>> loc(nd);
+    int64_t off = 0;
+    for (int t = 0; t < nd; ++t) {
+        const Depth& T = g.depths[t];
+        for (int64_t s = 0; s < n; ++s) {
+            const Depth& Ts = g.depths[tOf[s]];
+            bool le = true;
+            for (int i = 0; i < d; ++i) if (Ts[i] > T[i]) { le = false; break; }
+            if (le) { loc[t][s] = (int32_t)G[t].size(); G[t].push_back(s); }
+        }
+        if (G[t].size() >= (size_t)INT32_MAX) { err = "multilevel set too large"; return LSG_ERR_CAPACITY; }
+        g.boxSize[t] = (int64_t)G[t].size();
+        g.boxOff[t] = off;
+        off += g.boxSize[t];
+    }
+    g.nml = off;
+    auto locOf = [&](int t, const int32_t* v) -> int32_t {
+        const int64_t s = find(v);
+        if (s < 0) return -1;
+        auto it = loc[t].find(s);
+        return it == loc[t].end() ? -1 : it->second;
+    };
+
+    // calcNodal rows: every active point of depth T scatters its prewavelet to the stored
+    // points of its stencil (MatrixVectorHomogen.cpp:43-74), gathered here by target row
+    {
+        std::vector<std::vector<std::pair<int64_t, double>>> rows(g.nml);
+        for (int64_t s = 0; s < n; ++s) {
+            if (!g.active[s]) continue;
+            const int t = tOf[s];
+            const Depth& T = g.depths[t];
+            int32_t p1[MAXD][5];
+            double c1[MAXD][5];
+            int m1[MAXD];
+            for (int i = 0; i < d; ++i) m1[i] = prewavelet1D(posAt(g.idx[s * d + i], T[i]), T[i], p1[i], c1[i]);
+            int cnt[MAXD] = {};
+            while (true) {
+                int32_t J[MAXD];
+                double c = 1.0;
+                for (int i = 0; i < d; ++i) { J[i] = idxAt(p1[i][cnt[i]], T[i]); c *= c1[i][cnt[i]]; }
+                const int32_t l = locOf(t, J);
+                if (l >= 0) rows[g.boxOff[t] + l].push_back({g.devOfSlot[s], c});
+                int i = 0;
+                for (; i < d; ++i) { if (++cnt[i] < m1[i]) break; cnt[i] = 0; }
+                if (i == d) break;
+            }
+        }
+        S.nodPtr.assign(g.nml + 1, 0);
+        for (int64_t r = 0; r < g.nml; ++r) S.nodPtr[r + 1] = S.nodPtr[r] + (int64_t)rows[r].size();
+        for (auto& row : rows)
+            for (auto& e : row) { S.nodCol.push_back(e.first); S.nodVal.push_back(e.second); }
+    }
+    // prolongation gathers (prolongation1D_inplace, MatrixVectorHomogen.h:444-474)
+    S.prolOff.assign((size_t)nd * d, -1);
+    for (int t = 0; t < nd; ++t)
+        for (int dd = 0; dd < d; ++dd) {
+            Depth Tc = g.depths[t];
+            if (Tc[dd] < 2) continue;
+            Tc[dd] -= 1;
+            auto it = g.depthIdx.find(Tc);
+            if (it == g.depthIdx.end()) continue;
+            const int tc = it->second;
+            S.prolOff[(size_t)t * d + dd] = (int64_t)S.prolAB.size() / 2;
+            for (int64_t s : G[t]) {
+                const int32_t* v = &g.idx[s * d];
+                auto lc = loc[tc].find(s);
+                if (lc != loc[tc].end()) { S.prolAB.push_back(lc->second); S.prolAB.push_back(-2); continue; }
+                // depth_dd(s) = T_dd: the poles along dd (positions k -+ 1 at level T_dd)
+                const int tdd = g.depths[t][dd];
+                const int32_t k = posAt(v[dd], tdd);
+                int32_t w[MAXD];
+                for (int i = 0; i < d; ++i) w[i] = v[i];
+                int32_t a = -1, b = -1;
+                if (k - 1 >= 1) { w[dd] = idxAt(k - 1, tdd); a = locOf(tc, w); }
+                if (k + 1 <= (1 << tdd) - 1) { w[dd] = idxAt(k + 1, tdd); b = locOf(tc, w); }
+                S.prolAB.push_back(a);
+                S.prolAB.push_back(b);
+            }
+        }
+    // cells of depth T+1 in the grid: cells of the level-T full grid whose interior corners
+    // are all stored (cells/CellStructure.h:23-61), and for every row the cells around it
+    S.cellOff.assign(nd + 1, 0);
+    S.rcPtr.assign(g.nml + 1, 0);
+    {
+        const unsigned nc = 1u << d;
+        std::vector<std::vector<std::pair<int32_t, int32_t>>> rc(g.nml);
+        for (int t = 0; t < nd; ++t) {
+            const Depth& T = g.depths[t];
+            S.cellOff[t] = (int64_t)S.cellLo.size() / d;
+            std::unordered_map<Key2, int32_t, Key2Hash> seenCell;
+            for (int64_t s : G[t]) {
+                int32_t k[MAXD];
+                for (int i = 0; i < d; ++i) k[i] = posAt(g.idx[s * d + i], T[i]);
+                for (unsigned cs = 0; cs < nc; ++cs) {
+                    int32_t lo[MAXD];
+                    bool inside = true;
+                    for (int i = 0; i < d; ++i) {
+                        lo[i] = k[i] - 1 + (int32_t)((cs >> i) & 1u);
+                        if (lo[i] < 0 || lo[i] >= (1 << T[i])) inside = false;
+                    }
+                    if (!inside) continue;
+                    const Key2 ck = packKey(lo, d);
+                    int32_t cid;
+                    auto sc = seenCell.find(ck);
+                    if (sc != seenCell.end()) {
+                        cid = sc->second;
+                    } else {
+                        int32_t corner[1 << MAXD];
+                        bool ok = true;
+                        for (unsigned q = 0; q < nc && ok; ++q) {
+                            int32_t J[MAXD];
+                            bool interior = true;
+                            for (int i = 0; i < d; ++i) {
+                                const int32_t p = lo[i] + (int32_t)((q >> i) & 1u);
+                                if (p == 0 || p == (1 << T[i])) { interior = false; break; }
+                                J[i] = idxAt(p, T[i]);
+                            }
+                            if (!interior) { corner[q] = -1; continue; }
+                            corner[q] = locOf(t, J);
+                            if (corner[q] < 0) ok = false;
+                        }
+                        if (!ok) cid = -1;
+                        else {
+                            cid = (int32_t)((int64_t)S.cellLo.size() / d - S.cellOff[t]);
+                            for (unsigned q = 0; q < nc; ++q) S.cellCorner.push_back(corner[q]);
+                            for (int i = 0; i < d; ++i) S.cellLo.push_back(lo[i]);
+                        }
+                        seenCell[ck] = cid;
+                    }
+                    if (cid < 0) continue;
+                    // corner of s in this cell: bit i set if s is the upper node in dim i
+                    int32_t cj = 0;
+                    for (int i = 0; i < d; ++i) if (((cs >> i) & 1u) == 0) cj |= 1 << i;
+                    rc[g.boxOff[t] + loc[t][s]].push_back({cid, cj});
+                }
+            }
+        }
+        S.cellOff[nd] = (int64_t)S.cellLo.size() / d;
+        for (int64_t r = 0; r < g.nml; ++r) S.rcPtr[r + 1] = S.rcPtr[r] + (int64_t)rc[r].size();
+        for (auto& row : rc)
+            for (auto& e : row) { S.rcCell.push_back(e.first); S.rcCorner.push_back(e.second); }
+    }
+    // restriction gathers (restriction1D_inverted_inplace, MatrixVectorHomogen.h:485-530):
+    // z'(i) = z(i) + 0.5 (z(left child) + z(right child)) for i in G_{T - e_dd}
+    S.restOff.assign((size_t)nd * d, -1);
+    for (int t = 0; t < nd; ++t)
+        for (int dd = 0; dd < d; ++dd) {
+            Depth Tc = g.depths[t];
+            if (Tc[dd] < 2) continue;
+            Tc[dd] -= 1;
+            auto it = g.depthIdx.find(Tc);
+            if (it == g.depthIdx.end()) continue;
+            const int tc = it->second;
+            S.restOff[(size_t)t * d + dd] = (int64_t)S.restSLR.size() / 3;
+            const int tdd = g.depths[t][dd];
+            for (int64_t s : G[tc]) {
+                const int32_t* v = &g.idx[s * d];
+                int32_t w[MAXD];
+                for (int i = 0; i < d; ++i) w[i] = v[i];
+                const int32_t k = posAt(v[dd], tdd);     // even position at level T_dd
+                S.restSLR.push_back(loc[t].at(s));
+                int32_t a = -1, b = -1;
+                if (k - 1 >= 1) { w[dd] = idxAt(k - 1, tdd); a = locOf(t, w); }
+                if (k + 1 <= (1 << tdd) - 1) { w[dd] = idxAt(k + 1, tdd); b = locOf(t, w); }
+                S.restSLR.push_back(a);
+                S.restSLR.push_back(b);
+            }
+        }
+    // prewavelet conversion rows (ConvertToPrewavelet2, MatrixVectorHomogen.h:540-562)
+    S.convPtr.assign(n + 1, 0);
+    S.pos.assign((size_t)n * d, 0);
+    S.tOfDev.assign(n, 0);
+    S.depthT.assign((size_t)nd * d, 0);
+    for (int t = 0; t < nd; ++t)
+        for (int i = 0; i < d; ++i) S.depthT[(size_t)t * d + i] = g.depths[t][i];
+    for (int64_t s = 0; s < n; ++s) S.tOfDev[g.devOfSlot[s]] = tOf[s];
+    {
+        std::vector<std::vector<std::pair<int32_t, double>>> rows(n);
+        for (int64_t s = 0; s < n; ++s) {
+            const int t = tOf[s];
+            const Depth& T = g.depths[t];
+            const int64_t dev = g.devOfSlot[s];
+            for (int i = 0; i < d; ++i) S.pos[dev * d + i] = posAt(g.idx[s * d + i], T[i]);
+            if (!g.active[s]) continue;
+            int32_t p1[MAXD][5];
+            double c1[MAXD][5];
+            int m1[MAXD];
+            for (int i = 0; i < d; ++i) m1[i] = prewavelet1D(posAt(g.idx[s * d + i], T[i]), T[i], p1[i], c1[i]);
+            int cnt[MAXD] = {};
+            while (true) {
+                int32_t J[MAXD];
+                double c = 1.0;
+                for (int i = 0; i < d; ++i) { J[i] = idxAt(p1[i][cnt[i]], T[i]); c *= c1[i][cnt[i]]; }
+                const int32_t l = locOf(t, J);
+                if (l >= 0) rows[dev].push_back({l, c});
+                int i = 0;
+                for (; i < d; ++i) { if (++cnt[i] < m1[i]) break; cnt[i] = 0; }
+                if (i == d) break;
+            }
+        }
+        for (int64_t r = 0; r < n; ++r) S.convPtr[r + 1] = S.convPtr[r] + (int64_t)rows[r].size();
+        for (auto& row : rows)
+            for (auto& e : row) { S.convCol.push_back(e.first); S.convVal.push_back(e.second); }
+    }
+    return 0;
+}
+
 int buildPlan(Grid& g, std::string& err) {
     Plan& P = g.plan;
     const int d = g.d;
     const int nd = (int)g.depths.size();
     P = Plan();
+    // multilevel set / block sizes by depth (boxes and blocks of the dense layout, point
+    // lists of the sparse one)
+    auto boxSz = [&](const Depth& T) -> int64_t { return g.boxSize[g.depthIdx.at(T)]; };
+    auto toDepth = [&](const int32_t* T) { Depth D{}; for (int i = 0; i < d; ++i) D[i] = T[i]; return D; };
 
     // --- calcNodal items
     for (int t = 0; t < nd; ++t) {
@@ -150,34 +400,40 @@ int buildPlan(Grid& g, std::string& err) {
         // case buffer boxes: prolongation dims other than the last are swept by the
         // prolongation stage; the last one is fused into the apply stage
         std::unordered_map<int, int64_t> caseOff;
-        if (nP >= 2) {
+        // (sparse grids: every prolongation dim, the apply stage takes single items)
+        if (nP >= (g.sparse ? 1 : 2)) {
             for (int t : sorted) { caseOff[t] = P.caseSize; P.caseSize += g.boxSize[t]; }
         }
 
         // prolongation families (ascending d, families along d); the reference links
         // T -> T + e_dd only when the finer depth exists (MatrixVectorHomogen.h:1006)
+        // (a chain breaks where T + e_dd is missing: the reference then adds nothing)
         auto families = [&](int dd, std::map<Depth, std::vector<int>>& fam) -> int {
-            fam.clear();
+            std::map<Depth, std::vector<int>> lines;
             for (int t : sorted) {
                 Depth K = g.depths[t];
                 K[dd] = 0;
-                fam[K].push_back(t);
+                lines[K].push_back(t);
             }
-            for (auto& kv : fam) {
-                auto& ts = kv.second;
+            fam.clear();
+            for (auto& kv : lines) {
+                auto ts = kv.second;
                 std::sort(ts.begin(), ts.end(), [&](int a, int b) { return g.depths[a][dd] < g.depths[b][dd]; });
-                for (size_t q = 1; q < ts.size(); ++q)
-                    if (g.depths[ts[q]][dd] != g.depths[ts[q - 1]][dd] + 1) {
-                        err = "depth list not contiguous along a prolongation dimension";
-                        return LSG_ERR_UNSUPPORTED;
+                size_t q0 = 0;
+                for (size_t q = 1; q <= ts.size(); ++q)
+                    if (q == ts.size() || g.depths[ts[q]][dd] != g.depths[ts[q - 1]][dd] + 1) {
+                        Depth K = kv.first;
+                        K[dd] = g.depths[ts[q0]][dd];      // keyed by the first depth of the chain
+                        fam[K].assign(ts.begin() + q0, ts.begin() + q);
+                        if (q - q0 > (size_t)MAXLEV) { err = "family too long"; return LSG_ERR_UNSUPPORTED; }
+                        q0 = q;
                     }
-                if ((int)ts.size() > MAXLEV) { err = "family too long"; return LSG_ERR_UNSUPPORTED; }
             }
             return 0;
         };
         bool firstProl = true;
         for (int dd = 0; dd < d; ++dd) {
-            if (R[dd] || dd == lastP) continue;
+            if (R[dd] || (!g.sparse && dd == lastP)) continue;
             std::map<Depth, std::vector<int>> fam;
             if (int rc = families(dd, fam)) return rc;
             for (auto& [K, ts] : fam) {
@@ -189,6 +445,7 @@ int buildPlan(Grid& g, std::string& err) {
                 for (size_t q = 0; q < ts.size(); ++q) {
                     pi.in_off[q] = firstProl ? g.boxOff[ts[q]] : caseOff[ts[q]];
                     pi.out_off[q] = caseOff[ts[q]];
+                    pi.tidx[q] = ts[q];
                     P.maxFamily = std::max(P.maxFamily, g.boxSize[ts[q]]);
                 }
                 P.prol[dd].push_back(pi);
@@ -209,7 +466,7 @@ int buildPlan(Grid& g, std::string& err) {
         }
         // apply families along the last prolongation dim (single items if none)
         std::vector<std::vector<int>> applyFam;
-        if (lastP >= 0) {
+        if (lastP >= 0 && !g.sparse) {
             std::map<Depth, std::vector<int>> fam;
             if (int rc = families(lastP, fam)) return rc;
             for (auto& kv : fam) applyFam.push_back(kv.second);
@@ -229,7 +486,7 @@ int buildPlan(Grid& g, std::string& err) {
             X.c = c; X.t = t; X.k = 0;
             for (int i = 0; i < d; ++i) if (R[i]) X.r[X.k++] = i;
             for (int j = 0; j <= MAXD; ++j) { X.reads[j] = Read(); X.wOff[j] = -1; }
-            X.caseOff = nP >= 2 ? caseOff[t] : -1;
+            X.caseOff = nP >= (g.sparse ? 1 : 2) ? caseOff[t] : -1;
             X.fam = famBase + famOf[t];
             int me = (int)items.size();
             auto st = state.find(t);
@@ -247,8 +504,8 @@ int buildPlan(Grid& g, std::string& err) {
                     return LSG_ERR_UNSUPPORTED;
                 }
                 int tc = it->second;
-                stream += double(boxOf(prev, d)) + 3.0 * double(g.boxSize[tc]);
-                streamApply += double(boxOf(prev, d)) + 3.0 * double(g.boxSize[tc]);
+                stream += double(boxSz(prev)) + 3.0 * double(g.boxSize[tc]);
+                streamApply += double(boxSz(prev)) + 3.0 * double(g.boxSize[tc]);
                 auto s2 = state.find(tc);
                 if (s2 != state.end()) X.reads[j] = s2->second;
                 state[tc] = Read{me, j};
@@ -278,7 +535,7 @@ int buildPlan(Grid& g, std::string& err) {
         Depth Tc = g.depths[X.t];
         for (int j = 1; j <= X.k; ++j) {
             Tc[X.r[j - 1]] -= 1;
-            if (X.wNeeded[j]) { X.wOff[j] = P.wSize; P.wSize += boxOf(Tc, d); }
+            if (X.wNeeded[j]) { X.wOff[j] = P.wSize; P.wSize += boxSz(Tc); }
         }
     }
     // wavefront level of a family: one more than the deepest family any of its items
@@ -310,7 +567,21 @@ int buildPlan(Grid& g, std::string& err) {
         a.k = X.k;
         for (int j = 0; j < X.k; ++j) a.r[j] = X.r[j];
         a.in_nodal = X.caseOff < 0 ? 1 : 0;
+        a.tidx = X.t;
         a.in_off = X.caseOff < 0 ? g.boxOff[X.t] : X.caseOff;
+        {
+            Depth Tc = g.depths[X.t];
+            a.ctidx[0] = X.t;
+            for (int j = 1; j <= X.k; ++j) {
+                Tc[X.r[j - 1]] -= 1;
+                a.ctidx[j] = g.depthIdx.at(Tc);
+            }
+        }
+        a.z_off = -1;
+        if (g.sparse) {      // stencil result and restriction chain, two rows of |G_T|
+            a.z_off = P.zSize;
+            P.zSize += 2 * g.boxSize[X.t];
+        }
         for (int j = 0; j <= MAXD; ++j) { a.rd_off[j] = -1; a.w_off[j] = -1; }
         for (int j = 0; j <= X.k; ++j)
             if (X.reads[j].item >= 0) a.rd_off[j] = items[X.reads[j].item].wOff[X.reads[j].step];
@@ -334,6 +605,7 @@ int buildPlan(Grid& g, std::string& err) {
         SumItem s{};
         for (int i = 0; i < d; ++i) s.T[i] = g.depths[t][i];
         s.blk_off = g.blkOff[t];
+        s.count = g.blkSize[t];
         s.list_start = (int)P.sumList.size();
         for (int c = 0; c < ncase; ++c)
             if (partOf[t][c] >= 0) P.sumList.push_back(partOf[t][c]);
@@ -346,30 +618,24 @@ int buildPlan(Grid& g, std::string& err) {
     // values this implementation moves per matvec (each kernel reads its inputs and
     // writes its outputs once)
     double mv = 0.0;
-    for (auto& it : P.nodal) mv += double(blockCount(it.T, d)) + double(boxCount(it.T, d));
+    for (int t = 0; t < nd; ++t) mv += double(g.blkSize[t]) + double(g.boxSize[t]);
     for (auto& v : P.prol)
-        for (auto& it : v) {
-            int32_t T[MAXD];
-            std::memcpy(T, it.T, sizeof(T));
-            for (int q = 0; q < it.nt; ++q) { T[it.dd] = it.T[it.dd] + q; mv += 2.0 * double(boxCount(T, d)); }
-        }
+        for (auto& it : v)
+            for (int q = 0; q < it.nt; ++q) mv += 2.0 * double(g.boxSize[it.tidx[q]]);
     const double beforeApply = mv;
     for (auto& a : P.applyItems) {
-        {
-            int32_t T[MAXD];
-            std::memcpy(T, a.T, sizeof(T));
-            mv += double(boxCount(T, d));
-            if (a.rd_off[0] >= 0) mv += double(boxCount(T, d));
-            for (int j = 1; j <= a.k; ++j) {
-                T[a.r[j - 1]] -= 1;
-                if (a.w_off[j] >= 0) mv += double(boxCount(T, d));
-                if (a.rd_off[j] >= 0) mv += double(boxCount(T, d));
-            }
-            mv += double(blockCount(T, d));
+        Depth T = toDepth(a.T);
+        mv += double(boxSz(T));
+        if (a.rd_off[0] >= 0) mv += double(boxSz(T));
+        for (int j = 1; j <= a.k; ++j) {
+            T[a.r[j - 1]] -= 1;
+            if (a.w_off[j] >= 0) mv += double(boxSz(T));
+            if (a.rd_off[j] >= 0) mv += double(boxSz(T));
         }
+        mv += double(g.blkSize[g.depthIdx.at(T)]);
     }
     P.applyValues = mv - beforeApply;
-    for (auto& s : P.sum) mv += double(blockCount(s.T, d)) * double(s.ncase + 1);
+    for (auto& s : P.sum) mv += double(s.count) * double(s.ncase + 1);
     P.movedValues = mv;
     return 0;
 }
@@ -404,7 +670,6 @@ int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid&
             if (v - (1 << t) >= (1 << (t - 1))) { err = "invalid 1D index (outside the unit interval)"; return LSG_ERR_ARG; }
             T[i] = t;
         }
-        if (!g.active[s]) { err = "hanging nodes (adaptive grids) are not supported by this build"; return LSG_ERR_UNSUPPORTED; }
         if (!seen.insert(packKey(&idx[s * d], d)).second) { err = "duplicate point"; return LSG_ERR_ARG; }
         auto it = g.depthIdx.find(T);
         if (it == g.depthIdx.end()) {
@@ -421,21 +686,21 @@ int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid&
             g.maxDepth[i] = std::max(g.maxDepth[i], T[i]);
             g.maxLevel = std::max(g.maxLevel, T[i]);
         }
-    // block layout
+    // block layout; grids with hanging nodes or partly filled depth blocks take the
+    // sparse path (multilevel point lists instead of dense boxes)
     std::vector<int64_t> cnt(nd, 0);
     for (int64_t s = 0; s < n; ++s) cnt[tOf[s]]++;
     g.blkOff.resize(nd); g.blkSize.resize(nd); g.boxOff.resize(nd); g.boxSize.resize(nd);
+    g.sparse = false;
+    for (int64_t s = 0; s < n; ++s) if (!g.active[s]) g.sparse = true;
+    for (int t = 0; t < nd; ++t) if (cnt[t] != blockOf(g.depths[t], d)) g.sparse = true;
     int64_t off = 0, boff = 0;
     for (int t = 0; t < nd; ++t) {
-        g.blkSize[t] = blockOf(g.depths[t], d);
-        if (cnt[t] != g.blkSize[t]) {
-            err = "depth block not full (locally adaptive grids are not supported by this build)";
-            return LSG_ERR_UNSUPPORTED;
-        }
+        g.blkSize[t] = cnt[t];
         g.blkOff[t] = off; off += g.blkSize[t];
         g.boxSize[t] = boxOf(g.depths[t], d);
         g.boxOff[t] = boff; boff += g.boxSize[t];
-        // downward closed: every G_T must be a full box
+        // downward closed (every restriction target exists)
         for (int i = 0; i < d; ++i) {
             if (g.depths[t][i] >= 2) {
                 Depth Q = g.depths[t];
@@ -447,6 +712,17 @@ int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid&
     g.nml = boff;
     g.devOfSlot.resize(n);
     g.slotOfDev.resize(n);
+    if (g.sparse) {
+        // device order: depth blocks, slot order inside a block
+        std::vector<int64_t> fill(nd, 0);
+        for (int64_t s = 0; s < n; ++s) {
+            const int64_t dev = g.blkOff[tOf[s]] + fill[tOf[s]]++;
+            g.devOfSlot[s] = dev;
+            g.slotOfDev[dev] = s;
+        }
+        if (int rc = buildSparse(g, tOf, err)) return rc;
+        return buildPlan(g, err);
+    }
     for (int64_t s = 0; s < n; ++s) {
         const Depth& T = g.depths[tOf[s]];
         int64_t loc = 0, mul = 1;

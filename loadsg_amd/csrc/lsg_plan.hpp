@@ -43,6 +43,7 @@ struct ProlItem {
     int32_t pad;
     int64_t in_off[MAXLEV];
     int64_t out_off[MAXLEV];
+    int32_t tidx[MAXLEV];   // depth index of each box
 };
 
 // Stencil + restriction chain + prewavelet conversion for one (case, depth).
@@ -51,11 +52,14 @@ struct ApplyItem {
     int32_t k;            // number of restriction dims
     int32_t r[MAXD];      // restriction dims, ascending
     int32_t in_nodal;     // 1: input box from nodal buffer, 0: from case buffer
-    int32_t pad;
+    int32_t tidx;         // depth index of T
     int64_t in_off;
     int64_t rd_off[MAXD + 1];   // gM values added after step j (0 = after the stencil); -1 none
     int64_t w_off[MAXD + 1];    // W_j written after restriction j (1..k); -1 if never read
     int64_t out_off;            // part buffer block of the target depth
+    int64_t z_off;              // scratch of the sparse apply stage (sparse grids only)
+    int32_t ctidx[MAXD + 1];    // depth index after j restrictions (j = 0..k)
+    int32_t pad2;
 };
 
 // A family of apply items along the last prolongation dimension fd of a case
@@ -74,6 +78,7 @@ struct SumItem {
     int32_t ncase;
     int32_t list_start;
     int64_t blk_off;
+    int64_t count;      // points in the block
 };
 
 struct Plan {
@@ -86,6 +91,7 @@ struct Plan {
     int64_t caseSize = 0;
     int64_t wSize = 0;
     int64_t partSize = 0;
+    int64_t zSize = 0;          // scratch of the sparse apply stage
     int64_t maxBox = 0;         // largest |G_T|
     int64_t maxBlock = 0;       // largest n_T
     int64_t maxFamily = 0;      // largest box in a prolongation family
@@ -97,7 +103,46 @@ struct Plan {
     double applyValues = 0;     // of which the apply stage (input box, gM reads, W writes, output block)
 };
 
+// Gather tables of the sparse path (grids whose depth blocks are not full, or with
+// hanging nodes): G_T is the set of stored points of depth <= T (the reference's
+// MultiLevelAdaptiveSparseGrid, multilevelSparseGrid.h:13-143), kept as a list in slot
+// order; every stage becomes a gather over such lists.
+struct SparseTables {
+    // calcNodal: CSR over multilevel rows (boxOff[t] + i): sources (device index, coefficient)
+    std::vector<int64_t> nodPtr;
+    std::vector<int64_t> nodCol;
+    std::vector<double> nodVal;
+    // prolongation into depth t along dd from t - e_dd: per target row (a, b) local
+    // indices in G_{t-e_dd}; b == -2: copy a; else 0.5 (src[a] + src[b]) (-1 = absent)
+    std::vector<int64_t> prolOff;          // [t * d + dd], -1 if t - e_dd is not a depth
+    std::vector<int32_t> prolAB;           // pairs
+    // cells of depth T+1 in the grid (cells/CellStructure.h:23-61), per depth: corner local
+    // indices (2^d per cell, -1 = boundary corner) and the cell's lower position per dim
+    std::vector<int64_t> cellOff;          // [t], first cell of depth t
+    std::vector<int32_t> cellCorner;
+    std::vector<int32_t> cellLo;
+    // cells around each multilevel row: CSR (cell id, corner id of the row's point)
+    std::vector<int64_t> rcPtr;
+    std::vector<int32_t> rcCell;
+    std::vector<int32_t> rcCorner;
+    // restriction from depth t along dd into t - e_dd: per target row (self, left, right)
+    // local indices in G_t (-1 = absent)
+    std::vector<int64_t> restOff;          // [t * d + dd]
+    std::vector<int32_t> restSLR;          // triples
+    // ConvertToPrewavelet2: CSR over device rows (points of depth t in block order):
+    // (local index in G_t, coefficient); hanging nodes have empty rows
+    std::vector<int64_t> convPtr;
+    std::vector<int32_t> convCol;
+    std::vector<double> convVal;
+    // level-T_own position of each point (device order), for the diagonal
+    std::vector<int32_t> pos;
+    std::vector<int32_t> tOfDev;     // depth index of each point (device order)
+    std::vector<int32_t> depthT;     // depths, nd x d
+};
+
 struct Grid {
+    bool sparse = false;   // sparse path (see SparseTables)
+    SparseTables sp;
     int d = 0;
     int64_t n = 0;
     std::vector<int32_t> idx;      // n x d, reference slot order

@@ -1,0 +1,296 @@
+// lsg_sparse.hip -- gfx950 kernels of the sparse path: grids with hanging nodes or
+// partly filled depth blocks (locally adaptive sparse grids).
+//
+// The dense kernels work on full boxes G_T; here G_T is the list of stored points of
+// depth <= T (MultiLevelAdaptiveSparseGrid, multilevelSparseGrid.h:13-143) and every
+// stage is a gather over host-built tables (lsg_plan.cpp, buildSparse):
+//   calcNodal            CSR rows per multilevel point        (MatrixVectorHomogen.cpp:43-74)
+//   prolongation         (a, b) pairs per target point          (MatrixVectorHomogen.h:444-474)
+//   stencil              cells of depth T+1 in the grid around each point, element
+//                        matrices from the cell-centre coefficient (Stencil.h:187-246)
+//   restriction          (self, left, right) per coarse point   (MatrixVectorHomogen.h:485-530)
+//   ConvertToPrewavelet2 CSR rows per point of the target depth (MatrixVectorHomogen.h:540-562)
+// The case/depth schedule, the gM binding and the wavefront levels are the dense
+// path's (buildPlan), with one workgroup per (apply item, instance).  Passes of a
+// workgroup communicate through global scratch, separated by a device-scope fence and
+// a barrier.  Every sum has a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "lsg_kernels.hpp"
+
+namespace lsg {
+
+namespace {
+
+inline unsigned gridOf(int64_t n, int per = 256 * 4) {
+    int64_t b = (n + per - 1) / per;
+    if (b < 1) b = 1;
+    if (b > 65535) b = 65535;
+    return (unsigned)b;
+}
+
+__device__ inline void pass_barrier() {
+    __threadfence();
+    __syncthreads();
+}
+
+// coefficient a (stiffness terms) and c (mass terms) at the centre of the level-T cell
+// with lower position lo (cell index lo[j] along dim j)
+template <typename V>
+__device__ inline void cell_coef(const V* __restrict__ coef, int64_t tabLen, int nA, int nC, int d,
+                                 const int32_t* __restrict__ T, const int32_t* __restrict__ lo, V& aC, V& cC) {
+    aC = V(0);
+    cC = V(0);
+    for (int m = 0; m < nA + nC; ++m) {
+        V prod = V(1);
+        for (int j = 0; j < d; ++j) prod *= coef[(int64_t)(m * d + j) * tabLen + ((1 << T[j]) - 2) + lo[j]];
+        if (m < nA) aC += prod;
+        else cC += prod;
+    }
+}
+
+// element matrix entry between corners p and q (bit j = upper node in dim j) of a
+// level-T cell: a * sum_i K1_i prod_{j!=i} M1_j + c * prod_j M1_j with
+// K1 = 2^T [1 -1; -1 1], M1 = 2^-T [1/3 1/6; 1/6 1/3] (Poisson / HelmHoltz integration,
+// Stencil.h:873-919, 1707-1752)
+template <typename V>
+__device__ inline V element(int d, const int32_t* __restrict__ T, unsigned p, unsigned q, V aC, V cC) {
+    V mass = V(1);
+    V stiff = V(0);
+    for (int i = 0; i < d; ++i) {
+        const bool same = ((p >> i) & 1u) == ((q >> i) & 1u);
+        const V h = ldexp(V(1), -T[i]);
+        const V m1 = (same ? V(1) / V(3) : V(1) / V(6)) * h;
+        const V k1 = (same ? V(1) : V(-1)) / h;
+        // stiff = sum_i k1_i prod_{j != i} m1_j, accumulated with the mass product so far
+        stiff = stiff * m1 + k1 * mass;
+        mass *= m1;
+    }
+    return aC * stiff + cC * mass;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ calcNodal
+template <typename V>
+__global__ void __launch_bounds__(256) sp_nodal_kernel(SparseDev s, const V* __restrict__ x, int64_t xStride,
+                                                       V* __restrict__ nodal, int64_t nStride, int64_t nml,
+                                                       const CGState* __restrict__ st) {
+    const int inst = blockIdx.y;
+    if (st && !st[inst].active) return;
+    const V* xi = x + inst * xStride;
+    V* out = nodal + inst * nStride;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nml; r += (int64_t)gridDim.x * blockDim.x) {
+        V v = V(0);
+        for (int64_t k = s.nodPtr[r]; k < s.nodPtr[r + 1]; ++k) v += V(s.nodVal[k]) * xi[s.nodCol[k]];
+        out[r] = v;
+    }
+}
+
+// ---------------------------------------------------------------- prolongation
+// One workgroup per (chain along dd, instance): box q = input(q) + I_dd(box q-1).
+template <typename V>
+__global__ void __launch_bounds__(256) sp_prol_kernel(const ProlItem* __restrict__ items, SparseDev s,
+                                                      const V* __restrict__ nodal, int64_t nStride,
+                                                      V* __restrict__ cbuf, int64_t cStride, int batch,
+                                                      const CGState* __restrict__ st) {
+    const int inst = blockIdx.x % batch;
+    if (st && !st[inst].active) return;
+    const ProlItem& it = items[blockIdx.x / batch];
+    const int dd = it.dd;
+    const V* inBase = it.from_nodal ? nodal + inst * nStride : cbuf + inst * cStride;
+    V* outBase = cbuf + inst * cStride;
+    for (int q = 0; q < it.nt; ++q) {
+        const int t = it.tidx[q];
+        const int64_t size = s.boxSize[t];
+        const V* in = inBase + it.in_off[q];
+        V* out = outBase + it.out_off[q];
+        if (q == 0) {
+            for (int64_t i = threadIdx.x; i < size; i += blockDim.x) out[i] = in[i];
+        } else {
+            const V* prev = outBase + it.out_off[q - 1];
+            const int32_t* ab = s.prolAB + 2 * s.prolOff[(int64_t)t * s.d + dd];
+            for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
+                const int32_t a = ab[2 * i], b = ab[2 * i + 1];
+                V add;
+                if (b == -2) add = prev[a];
+                else add = V(0.5) * ((a >= 0 ? prev[a] : V(0)) + (b >= 0 ? prev[b] : V(0)));
+                out[i] = in[i] + add;
+            }
+        }
+        pass_barrier();
+    }
+}
+
+// ---------------------------------------------------------------------- apply
+// One workgroup per (case, depth) item and instance: stencil on G_T (+ gM), the
+// restrictions with their gM exchanges, and the conversion into the part buffer.
+template <typename V>
+__global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
+    const int inst = blockIdx.x % a.batch;
+    if (a.st && !a.st[inst].active) return;
+    const ApplyItem& it = a.items[a.fam[blockIdx.x / a.batch].first];
+    const SparseDev& s = a.sp;
+    const int d = s.d;
+    const int t = it.tidx;
+    const int64_t size = s.boxSize[t];
+    const V* u = (it.in_nodal ? a.nodal + inst * a.nStride : a.cbuf + inst * a.cStride) + it.in_off;
+    V* A = a.zbuf + inst * a.zStride + it.z_off;
+    V* B = A + size;
+    V* wb = a.wbuf + inst * a.wStride;
+    const V* coef = a.coefTab + inst * a.coefStride;
+    const unsigned nc = 1u << d;
+    // stencil: z_i = sum over the cells around point i of the element rows
+    {
+        const int64_t r0 = s.boxOff[t];
+        const int64_t c0 = s.cellOff[t];
+        const V* g = it.rd_off[0] >= 0 ? wb + it.rd_off[0] : nullptr;
+        for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
+            V z = V(0);
+            for (int64_t k = s.rcPtr[r0 + i]; k < s.rcPtr[r0 + i + 1]; ++k) {
+                const int64_t cell = c0 + s.rcCell[k];
+                const unsigned cj = (unsigned)s.rcCorner[k];
+                V aC, cC;
+                cell_coef<V>(coef, a.tabLen, a.nA, a.nC, d, it.T, s.cellLo + cell * d, aC, cC);
+                const int32_t* corner = s.cellCorner + cell * nc;
+                for (unsigned q = 0; q < nc; ++q) {
+                    const int32_t j = corner[q];
+                    if (j < 0) continue;
+                    z += element<V>(d, it.T, cj, q, aC, cC) * u[j];
+                }
+            }
+            A[i] = g ? z + g[i] : z;
+        }
+        pass_barrier();
+    }
+    // restrictions along the restriction dims (ascending) with the gM exchange
+    int64_t cur = size;
+    for (int j = 1; j <= it.k; ++j) {
+        const int dd = it.r[j - 1];
+        const int tf = it.ctidx[j - 1], tc = it.ctidx[j];
+        const int64_t nco = s.boxSize[tc];
+        const int32_t* slr = s.restSLR + 3 * s.restOff[(int64_t)tf * d + dd];
+        const bool w = it.w_off[j] >= 0, rd = it.rd_off[j] >= 0;
+        V* wd = wb + it.w_off[j];
+        const V* g = wb + it.rd_off[j];
+        for (int64_t i = threadIdx.x; i < nco; i += blockDim.x) {
+            const int32_t sf = slr[3 * i], l = slr[3 * i + 1], r = slr[3 * i + 2];
+            const V v = A[sf] + V(0.5) * ((l >= 0 ? A[l] : V(0)) + (r >= 0 ? A[r] : V(0)));
+            if (w) wd[i] = v;
+            B[i] = rd ? v + g[i] : v;
+        }
+        pass_barrier();
+        V* tmp = A; A = B; B = tmp;
+        cur = nco;
+    }
+    (void)cur;
+    // prewavelet conversion of the points of the target depth
+    {
+        const int tc = it.ctidx[it.k];
+        const int64_t b0 = s.blkOff[tc], nb = s.blkSize[tc];
+        V* out = a.part + inst * a.pStride + it.out_off;
+        for (int64_t i = threadIdx.x; i < nb; i += blockDim.x) {
+            V v = V(0);
+            for (int64_t k = s.convPtr[b0 + i]; k < s.convPtr[b0 + i + 1]; ++k) v += V(s.convVal[k]) * A[s.convCol[k]];
+            out[i] = v;
+        }
+    }
+}
+
+// ------------------------------------------------------------------- diagonal
+// z_p = psi_p^T A_T psi_p (Preconditioning::precond3, applications/cg_method.h:322-451,
+// with applyStencilLocal over the cells of depth T+1 in the grid): for every stored
+// point J of p's prewavelet stencil, the cells around J, their corners q and the
+// stencil coefficient psi_p(q) from the position offset.  Hanging nodes get 1.
+template <typename V>
+__global__ void __launch_bounds__(256) sp_diag_kernel(SparseDev s, const uint8_t* __restrict__ activeDev, int n,
+                                                      int nA, int nC, const V* __restrict__ coefTab,
+                                                      int64_t coefStride, int64_t tabLen, V* __restrict__ z,
+                                                      int64_t zStride) {
+    const int inst = blockIdx.y;
+    const V* coef = coefTab + inst * coefStride;
+    const int d = s.d;
+    const unsigned nc = 1u << d;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
+        if (!activeDev[p]) {
+            z[inst * zStride + p] = V(1);
+            continue;
+        }
+        const int t = s.tOfDev[p];
+        const int32_t* T = s.depthT + (int64_t)t * d;
+        const int32_t* pp = s.pos + p * d;
+        const int64_t r0 = s.boxOff[t], c0 = s.cellOff[t];
+        V total = V(0);
+        for (int64_t k = s.convPtr[p]; k < s.convPtr[p + 1]; ++k) {
+            const int32_t J = s.convCol[k];
+            V row = V(0);
+            for (int64_t e = s.rcPtr[r0 + J]; e < s.rcPtr[r0 + J + 1]; ++e) {
+                const int64_t cell = c0 + s.rcCell[e];
+                const unsigned cj = (unsigned)s.rcCorner[e];
+                const int32_t* lo = s.cellLo + cell * d;
+                V aC, cC;
+                cell_coef<V>(coef, tabLen, nA, nC, d, T, lo, aC, cC);
+                const int32_t* corner = s.cellCorner + cell * nc;
+                for (unsigned q = 0; q < nc; ++q) {
+                    if (corner[q] < 0) continue;
+                    // psi_p at corner q: product of the 1D stencil weights by offset
+                    V psi = V(1);
+                    for (int i = 0; i < d; ++i) {
+                        const int off = lo[i] + (int)((q >> i) & 1u) - pp[i];
+                        const int n1 = (1 << T[i]) - 1;
+                        V c1;
+                        if (off == 0) c1 = (T[i] >= 2 && (pp[i] == 1 || pp[i] == n1)) ? V(0.9) : V(1);
+                        else if (off == 1 || off == -1) c1 = V(-0.6);
+                        else if (off == 2 || off == -2) c1 = V(0.1);
+                        else c1 = V(0);
+                        psi *= c1;
+                    }
+                    if (psi == V(0)) continue;
+                    row += element<V>(d, T, cj, q, aC, cC) * psi;
+                }
+            }
+            total += V(s.convVal[k]) * row;
+        }
+        z[inst * zStride + p] = total;
+    }
+}
+
+// -------------------------------------------------------------------- launchers
+template <typename V>
+void launch_sp_nodal(const SparseDev& s, const V* x, int64_t xStride, V* nodal, int64_t nStride, int64_t nml,
+                     int batch, const CGState* st, hipStream_t str) {
+    hipLaunchKernelGGL(sp_nodal_kernel<V>, dim3(gridOf(nml), batch), dim3(256), 0, str, s, x, xStride, nodal, nStride,
+                       nml, st);
+}
+template <typename V>
+void launch_sp_prol(const ProlItem* items, int nitems, const SparseDev& s, const V* nodal, int64_t nStride, V* cbuf,
+                    int64_t cStride, int batch, const CGState* st, hipStream_t str) {
+    if (nitems == 0) return;
+    hipLaunchKernelGGL(sp_prol_kernel<V>, dim3((unsigned)nitems * (unsigned)batch), dim3(256), 0, str, items, s, nodal,
+                       nStride, cbuf, cStride, batch, st);
+}
+template <typename V>
+void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str) {
+    if (nitems == 0) return;
+    hipLaunchKernelGGL(sp_apply_kernel<V>, dim3((unsigned)nitems * (unsigned)a.batch), dim3(256), 0, str, a);
+}
+template <typename V>
+void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, int nA, int nC, const V* coefTab,
+                    int64_t coefStride, int64_t tabLen, V* z, int64_t zStride, int batch, hipStream_t str) {
+    hipLaunchKernelGGL(sp_diag_kernel<V>, dim3(gridOf(n, 256), batch), dim3(256), 0, str, s, activeDev, n, nA, nC,
+                       coefTab, coefStride, tabLen, z, zStride);
+}
+
+#define LSG_SP_INST(V)                                                                                              \
+    template void launch_sp_nodal<V>(const SparseDev&, const V*, int64_t, V*, int64_t, int64_t, int, const CGState*, \
+                                     hipStream_t);                                                                  \
+    template void launch_sp_prol<V>(const ProlItem*, int, const SparseDev&, const V*, int64_t, V*, int64_t, int,    \
+                                    const CGState*, hipStream_t);                                                   \
+    template void launch_sp_apply<V>(const SparseApplyArgs<V>&, int, hipStream_t);                                  \
+    template void launch_sp_diag<V>(const SparseDev&, const uint8_t*, int, int, int, const V*, int64_t, int64_t, V*,  \
+                                    int64_t, int, hipStream_t);
+LSG_SP_INST(double)
+LSG_SP_INST(float)
+#undef LSG_SP_INST
+
+}  // namespace lsg

@@ -30,6 +30,8 @@ CASES = {
     "d4_l3_cfg2": (4, ["regular", "3", "cfg2"]),
     "d5_l3_expsin": (5, ["regular", "3", "expsin:0.3,-0.5,0.2,0.4,-0.1,0.25,-0.35,0.15"]),
     "d3_adaptive_cfg3": (3, ["adaptive", "3", "0.15", "6", "cfg3"]),
+    "d2_adaptive_cfg2": (2, ["adaptive", "3", "0.2", "8", "cfg2"]),
+    "d3_adaptive_expsin": (3, ["adaptive", "2", "0.3", "5", "expsin:0.4,-0.3,0.25,-0.2,0.35,0.1"]),
 }
 
 

@@ -70,12 +70,22 @@ def test_streaming_model_value_count(d, level, stream):
     assert s["max_box"] <= 16384
 
 
-def test_adaptive_grid_with_hanging_nodes_is_rejected():
+def test_adaptive_grid_takes_the_sparse_path():
+    """A locally adaptive grid with hanging nodes (the reference's completeDirichletGrid
+    output) is accepted: depth blocks are point lists, G_T the stored points of depth <= T
+    (multilevel size equal to the oracle's), the device order a permutation."""
+    from oracle_ctypes import Oracle
     gold = os.path.join(ROOT, "tests", "golden", "d3_adaptive_cfg3")
     idx = np.load(os.path.join(gold, "idx.npy"))
     act = np.load(os.path.join(gold, "active.npy"))
-    with pytest.raises(L.LoadsgError):
-        L.AdaptiveSparseGrid(idx, act)
+    g = L.AdaptiveSparseGrid(idx, act)
+    assert g.size == idx.shape[0]
+    st = g.stats()
+    assert st["multilevel_size"] == Oracle(idx, act).multilevel_size()
+    dev = g.device_order()
+    assert sorted(dev.tolist()) == list(range(g.size))
+    i2, a2 = g.slot_order()
+    assert np.array_equal(i2, idx) and np.array_equal(a2, act)
 
 
 def test_invalid_input_errors():

@@ -1,0 +1,123 @@
+"""GPU parity of the sparse path: locally adaptive grids with hanging nodes (the
+compiled reference's completeDirichletGrid output) through the C ABI, against the
+reference's golden vectors and the CPU oracle.
+
+Tolerances as in test_gpu_parity: FP64 1e-12 relative (max-norm), FP32 2e-5,
+iteration counts exact.  The diagonal is compared with the oracle (which equals
+(A e_p)_p, see test_oracle.py); the reference's own diagonal on adaptive grids
+carries stale hanging-node values in 1-4 % of its entries.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import loadsg_amd as L
+from oracle_ctypes import COEF_CONST, Oracle
+from test_gpu_parity import load, relmax, stencil_of as _stencil_of
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+CASES = json.load(open(os.path.join(GOLD, "cases.json")))
+ADAPTIVE = sorted(c for c in CASES if CASES[c]["argv"][0] == "adaptive")
+
+
+def stencil_of(argv):
+    return _stencil_of([None, None, argv[-1]])     # coefficient spec is the last harness argument
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    L.lib()
+    yield
+
+
+@pytest.mark.parametrize("case", ADAPTIVE)
+def test_sparse_matvec_golden(case):
+    print("case", case, flush=True)
+    gv = load(case)
+    grid = L.AdaptiveSparseGrid(gv["idx"], gv["active"])
+    var, _ = stencil_of(CASES[case]["argv"])
+    for st, key in ((L.Poisson(), "Ax_poisson"), (L.HelmHoltz(), "Ax_helm"), (var, "Ax_var")):
+        op = L.MatrixVectorHomogen(grid, st)
+        y = op.multiplication(gv["x"])
+        assert relmax(y, gv[key]) < 1e-12, key
+        assert np.all(y[gv["active"] == 0] == 0.0)        # hanging nodes get no value
+        op32 = L.MatrixVectorHomogen(grid, st, precision=32)
+        assert relmax(op32.multiplication(gv["x"]), gv[key]) < 2e-5, key
+
+
+@pytest.mark.parametrize("case", ADAPTIVE)
+def test_sparse_diagonal_vs_oracle(case):
+    print("case", case, flush=True)
+    gv = load(case)
+    grid = L.AdaptiveSparseGrid(gv["idx"], gv["active"])
+    var, ovar = stencil_of(CASES[case]["argv"])
+    o = Oracle(gv["idx"], gv["active"])
+    act = gv["active"] == 1
+    for st, okind in ((L.Poisson(), dict(kind=COEF_CONST, a0=1.0, c0=0.0)), (var, ovar)):
+        z = L.MatrixVectorHomogen(grid, st).diagonal()
+        zo = o.set_coef(**okind).precond()
+        assert relmax(z[act], zo[act]) < 1e-12
+        assert np.all(z[~act] == 1.0)
+
+
+@pytest.mark.parametrize("case", ADAPTIVE)
+def test_sparse_cg_vs_oracle(case):
+    """Reference stopping rule: iteration counts equal to the oracle's (same diagonal),
+    solutions within 1e-10; against the reference's golden solution within 1e-8 (its
+    diagonal differs in the stale entries)."""
+    print("case", case, flush=True)
+    gv = load(case)
+    grid = L.AdaptiveSparseGrid(gv["idx"], gv["active"])
+    var, ovar = stencil_of(CASES[case]["argv"])
+    o = Oracle(gv["idx"], gv["active"])
+    for st, okind, key in ((L.Poisson(), dict(kind=COEF_CONST, a0=1.0, c0=0.0), "cg_poisson"),
+                           (var, ovar, "cg_var")):
+        op = L.MatrixVectorHomogen(grid, st)
+        x = np.zeros(grid.size)
+        info = L.CG.solveHomogen(1e-10, x, gv["f_rhs"], op, stop_rule=L.STOP_REF_ABS, maxit=3000)
+        xo, ito, _ = o.set_coef(**okind).cg(gv["f_rhs"], eps=1e-10)
+        assert info["iterations"] == ito
+        assert relmax(x, xo) < 1e-10
+        assert relmax(x, gv[key + "_x"]) < 1e-8
+
+
+def test_sparse_batch_and_mixed_solve():
+    """Batch instances on an adaptive grid match single instances bit for bit; the
+    mixed-precision solve reaches 1e-10 and agrees with the FP64 solve."""
+    import torch
+    gv = load("d3_adaptive_expsin")
+    grid = L.AdaptiveSparseGrid(gv["idx"], gv["active"])
+    rng = np.random.default_rng(2)
+    B = 3
+    xis = rng.normal(0.0, 0.5, (B, 6))
+    op64 = L.MatrixVectorHomogen(grid, L.VariableCoefficient(L.COEF_EXPSIN, xi=xis[0]), precision=64, batch=B)
+    op32 = L.MatrixVectorHomogen(grid, L.VariableCoefficient(L.COEF_EXPSIN, xi=xis[0]), precision=32, batch=B)
+    for b in range(B):
+        c = L.VariableCoefficient(L.COEF_EXPSIN, xi=xis[b])
+        op64.set_coefficient(b, c)
+        op32.set_coefficient(b, c)
+    x = torch.tensor(rng.standard_normal((B, grid.size)), device="cuda")
+    y = torch.zeros_like(x)
+    op64.multiplication(x, y)
+    dev = grid.device_order()
+    for b in range(B):
+        single = L.MatrixVectorHomogen(grid, L.VariableCoefficient(L.COEF_EXPSIN, xi=xis[b]))
+        xs = np.zeros(grid.size)
+        xs[:] = x[b].cpu().numpy()[dev]
+        ys = single.multiplication(xs)
+        assert np.array_equal(ys, y[b].cpu().numpy()[dev])
+    f = torch.zeros(B, grid.size, dtype=torch.float64, device="cuda")
+    op64.rhs_sines(rng.standard_normal((B, 2)), np.array([1, 2], dtype=np.int32), out=f)
+    xm = torch.zeros_like(f)
+    info = L.CG.solveMixed(1e-10, xm, f, op64, op32)
+    assert all(i["converged"] == 1 and i["rel_residual"] <= 1e-10 for i in info)
+    xr = torch.zeros_like(f)
+    L.CG.solveHomogen(1e-13, xr, f, op64, stop_rule=L.STOP_REL, maxit=5000)
+    assert float(torch.max(torch.linalg.norm(xm - xr, dim=1) / torch.linalg.norm(xr, dim=1))) < 1e-8

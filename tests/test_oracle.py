@@ -87,9 +87,16 @@ def test_golden_precond(case):
         else:
             # Adaptive grids: the reference work vector keeps stale hanging-node
             # values between points (cg_method.h:356-433, `v = 0.0` is active-only),
-            # which perturbs a few entries.  All other entries agree to rounding.
-            assert np.mean(rel < 1e-11) > 0.98
+            # which perturbs 1-4 % of the entries.  All other entries agree to rounding,
+            # and the oracle's value is the definition psi_p^T A psi_p = (A e_p)_p with the
+            # (golden-pinned) matvec, checked on sampled points.
+            assert np.mean(rel < 1e-11) > 0.95
             assert np.median(rel) < 1e-13
+            pts = np.random.default_rng(0).choice(np.flatnonzero(act), 12, replace=False)
+            for p in pts:
+                e = np.zeros(len(z))
+                e[p] = 1.0
+                assert abs(o.matvec(e)[p] - z[p]) <= 1e-13 * abs(z[p])
 
 
 @pytest.mark.parametrize("case", sorted(CASES))
