@@ -142,7 +142,9 @@ struct lsg_op {
     std::vector<std::unique_ptr<DevMem>> spTables;
     SparseDev sd{};
     std::vector<std::pair<int, std::unique_ptr<DevMem>>> spLevels;   // (count, families)
-    DevMem zbuf, activeDev;
+    DevMem zbuf, activeDev, cellCoef;
+    int64_t nCells = 0;
+    bool cellCoefValid = false;
     // solver buffers
     DevMem r, h, dv, g, D, f, x, tmp, partials, st, aux;
     bool solverReady = false, diagValid = false;
@@ -161,6 +163,12 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st) {
     const int d = g.d;
     const int B = op->batch;
     hipStream_t s = op->stream;
+    if (!op->cellCoefValid) {
+        launch_sp_cellcoef<V>(op->sd, op->nCells, op->nA, op->nC, op->coefTab.as<V>(),
+                              (int64_t)(op->nA + op->nC) * d * op->tabLen, op->tabLen, op->cellCoef.as<V>(),
+                              2 * op->nCells, B, s);
+        op->cellCoefValid = true;
+    }
     HIPCHK(hipEventRecord(op->ev[0], s));
     launch_sp_nodal<V>(op->sd, x, g.n, op->nodal.as<V>(), g.nml, g.nml, B, st, s);
     for (int dd = 0; dd < d; ++dd)
@@ -187,6 +195,8 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st) {
     a.zStride = P.zSize;
     a.batch = B;
     a.st = st;
+    a.cellCoef = op->cellCoef.as<V>();
+    a.ccStride = 2 * op->nCells;
     for (auto& L : op->spLevels) {
         a.fam = L.second->as<FamilyItem>();
         launch_sp_apply<V>(a, L.first, s);
@@ -698,7 +708,7 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         LSG_UP(nodPtr, T.nodPtr) LSG_UP(nodCol, T.nodCol) LSG_UP(nodVal, T.nodVal)
         LSG_UP(prolOff, T.prolOff) LSG_UP(prolAB, T.prolAB)
         LSG_UP(cellOff, T.cellOff) LSG_UP(cellCorner, T.cellCorner) LSG_UP(cellLo, T.cellLo)
-        LSG_UP(rcPtr, T.rcPtr) LSG_UP(rcCell, T.rcCell) LSG_UP(rcCorner, T.rcCorner)
+        LSG_UP(rcPtr, T.rcPtr) LSG_UP(rcCell, T.rcCell) LSG_UP(rcCorner, T.rcCorner) LSG_UP(rcY, T.rcY)
         LSG_UP(restOff, T.restOff) LSG_UP(restSLR, T.restSLR)
         LSG_UP(convPtr, T.convPtr) LSG_UP(convCol, T.convCol) LSG_UP(convVal, T.convVal)
         LSG_UP(pos, T.pos) LSG_UP(tOfDev, T.tOfDev) LSG_UP(depthT, T.depthT)
@@ -713,6 +723,8 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
             op->spLevels.emplace_back((int)lev.size(), std::move(m));
         }
         HIPCHK(op->zbuf.alloc((size_t)P.zSize * batch * vs));
+        op->nCells = T.cellOff.back();
+        HIPCHK(op->cellCoef.alloc((size_t)std::max<int64_t>(1, 2 * op->nCells) * batch * vs));
         std::vector<uint8_t> act(g.n);
         for (int64_t s0 = 0; s0 < g.n; ++s0) act[g.devOfSlot[s0]] = g.active[s0];
         HIPCHK(op->activeDev.alloc((size_t)g.n));
@@ -742,6 +754,7 @@ int lsg_op_set_coef(lsg_op* op, int instance, const lsg_coef* coef) {
         HIPCHK(hipMemcpy(op->coefTab.as<float>() + per * instance, tf.data(), per * 4, hipMemcpyHostToDevice));
     }
     op->diagValid = false;
+    op->cellCoefValid = false;
     return LSG_OK;
 }
 

@@ -46,7 +46,7 @@ struct ApplyArgs {
 struct SparseDev {
     int d, nd;
     const int64_t* nodPtr;
-    const int64_t* nodCol;
+    const int32_t* nodCol;
     const double* nodVal;
     const int64_t* prolOff;
     const int32_t* prolAB;
@@ -56,6 +56,7 @@ struct SparseDev {
     const int64_t* rcPtr;
     const int32_t* rcCell;
     const int32_t* rcCorner;
+    const int32_t* rcY;
     const int64_t* restOff;
     const int32_t* restSLR;
     const int64_t* convPtr;
@@ -90,8 +91,13 @@ struct SparseApplyArgs {
     int64_t zStride;
     int batch;
     const CGState* st;
+    const V* cellCoef;        // per instance: (a, c) at the centre of every cell of depth T+1
+    int64_t ccStride;
 };
 
+template <typename V>
+void launch_sp_cellcoef(const SparseDev& s, int64_t nCells, int nA, int nC, const V* coefTab, int64_t coefStride,
+                        int64_t tabLen, V* cc, int64_t ccStride, int batch, hipStream_t str);
 template <typename V>
 void launch_sp_nodal(const SparseDev& s, const V* x, int64_t xStride, V* nodal, int64_t nStride, int64_t nml,
                      int batch, const CGState* st, hipStream_t str);

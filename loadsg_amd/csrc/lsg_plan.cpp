@@ -187,7 +187,7 @@ int buildSparse(Grid& g, const std::vector<int>& tOf, std::string& err) {
         S.nodPtr.assign(g.nml + 1, 0);
         for (int64_t r = 0; r < g.nml; ++r) S.nodPtr[r + 1] = S.nodPtr[r] + (int64_t)rows[r].size();
         for (auto& row : rows)
-            for (auto& e : row) { S.nodCol.push_back(e.first); S.nodVal.push_back(e.second); }
+            for (auto& e : row) { S.nodCol.push_back((int32_t)e.first); S.nodVal.push_back(e.second); }
     }
     // prolongation gathers (prolongation1D_inplace, MatrixVectorHomogen.h:444-474)
     S.prolOff.assign((size_t)nd * d, -1);
@@ -275,9 +275,18 @@ int buildSparse(Grid& g, const std::vector<int>& tOf, std::string& err) {
             }
         }
         S.cellOff[nd] = (int64_t)S.cellLo.size() / d;
+        for (int t = 0; t < nd; ++t)
+            if ((S.cellOff[t + 1] - S.cellOff[t]) * (int64_t)nc >= (int64_t)INT32_MAX) {
+                err = "too many cells in one depth";
+                return LSG_ERR_CAPACITY;
+            }
         for (int64_t r = 0; r < g.nml; ++r) S.rcPtr[r + 1] = S.rcPtr[r] + (int64_t)rc[r].size();
         for (auto& row : rc)
-            for (auto& e : row) { S.rcCell.push_back(e.first); S.rcCorner.push_back(e.second); }
+            for (auto& e : row) {
+                S.rcCell.push_back(e.first);
+                S.rcCorner.push_back(e.second);
+                S.rcY.push_back((int32_t)((int64_t)e.first * nc + e.second));
+            }
     }
     // restriction gathers (restriction1D_inverted_inplace, MatrixVectorHomogen.h:485-530):
     // z'(i) = z(i) + 0.5 (z(left child) + z(right child)) for i in G_{T - e_dd}
@@ -579,8 +588,9 @@ int buildPlan(Grid& g, std::string& err) {
         }
         a.z_off = -1;
         if (g.sparse) {      // stencil result and restriction chain, two rows of |G_T|
-            a.z_off = P.zSize;
+            a.z_off = P.zSize;    // (+ the element products of the cells of depth T+1)
             P.zSize += 2 * g.boxSize[X.t];
+            if (g.d <= SP_CELL_MAXD) P.zSize += (g.sp.cellOff[X.t + 1] - g.sp.cellOff[X.t]) << g.d;
         }
         for (int j = 0; j <= MAXD; ++j) { a.rd_off[j] = -1; a.w_off[j] = -1; }
         for (int j = 0; j <= X.k; ++j)

@@ -21,6 +21,7 @@
 namespace lsg {
 
 constexpr int MAXD = 8;      // maximal dimension
+constexpr int SP_CELL_MAXD = 4;   // sparse apply: cell-wise element products up to this dimension
 constexpr int MAXLEV = 15;   // maximal 1D depth (MaximumDepth in library/source/abbrevi.h:12)
 
 using Depth = std::array<int, MAXD>;
@@ -110,7 +111,7 @@ struct Plan {
 struct SparseTables {
     // calcNodal: CSR over multilevel rows (boxOff[t] + i): sources (device index, coefficient)
     std::vector<int64_t> nodPtr;
-    std::vector<int64_t> nodCol;
+    std::vector<int32_t> nodCol;
     std::vector<double> nodVal;
     // prolongation into depth t along dd from t - e_dd: per target row (a, b) local
     // indices in G_{t-e_dd}; b == -2: copy a; else 0.5 (src[a] + src[b]) (-1 = absent)
@@ -125,6 +126,7 @@ struct SparseTables {
     std::vector<int64_t> rcPtr;
     std::vector<int32_t> rcCell;
     std::vector<int32_t> rcCorner;
+    std::vector<int32_t> rcY;              // cell * 2^d + corner: the row's entry of the cell results
     // restriction from depth t along dd into t - e_dd: per target row (self, left, right)
     // local indices in G_t (-1 = absent)
     std::vector<int64_t> restOff;          // [t * d + dd]

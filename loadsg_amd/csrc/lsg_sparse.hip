@@ -12,8 +12,7 @@
 //   ConvertToPrewavelet2 CSR rows per point of the target depth (MatrixVectorHomogen.h:540-562)
 // The case/depth schedule, the gM binding and the wavefront levels are the dense
 // path's (buildPlan), with one workgroup per (apply item, instance).  Passes of a
-// workgroup communicate through global scratch, separated by a device-scope fence and
-// a barrier.  Every sum has a fixed order.
+// workgroup communicate through global scratch, separated by a workgroup barrier.  Every sum has a fixed order.
 #include <hip/hip_runtime.h>
 
 #include "lsg_kernels.hpp"
@@ -29,8 +28,11 @@ inline unsigned gridOf(int64_t n, int per = 256 * 4) {
     return (unsigned)b;
 }
 
+// Passes of one workgroup hand data over through global memory.  All waves of a
+// workgroup run on one CU, so the workgroup-scope release/acquire of the barrier
+// makes the stores visible; a device-scope fence would write back and invalidate the
+// whole L2 of the XCD at every pass.
 __device__ inline void pass_barrier() {
-    __threadfence();
     __syncthreads();
 }
 
@@ -61,7 +63,7 @@ __device__ inline V element(int d, const int32_t* __restrict__ T, unsigned p, un
         const bool same = ((p >> i) & 1u) == ((q >> i) & 1u);
         const V h = ldexp(V(1), -T[i]);
         const V m1 = (same ? V(1) / V(3) : V(1) / V(6)) * h;
-        const V k1 = (same ? V(1) : V(-1)) / h;
+        const V k1 = ldexp(same ? V(1) : V(-1), T[i]);       // = +-1 / h exactly
         // stiff = sum_i k1_i prod_{j != i} m1_j, accumulated with the mass product so far
         stiff = stiff * m1 + k1 * mass;
         mass *= m1;
@@ -70,6 +72,29 @@ __device__ inline V element(int d, const int32_t* __restrict__ T, unsigned p, un
 }
 
 }  // namespace
+
+// ---------------------------------------------------------- cell coefficients
+// (a, c) at the centre of every cell of depth T+1 of every depth, per instance; the
+// coefficient does not change between matvecs, so the apply stage only loads these.
+template <typename V>
+__global__ void __launch_bounds__(256) sp_cellcoef_kernel(SparseDev s, int64_t nCells, int nA, int nC,
+                                                          const V* __restrict__ coefTab, int64_t coefStride,
+                                                          int64_t tabLen, V* __restrict__ cc, int64_t ccStride) {
+    const int inst = blockIdx.y;
+    const V* coef = coefTab + inst * coefStride;
+    for (int64_t c = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; c < nCells; c += (int64_t)gridDim.x * blockDim.x) {
+        int lo = 0, hi = s.nd;                 // depth of the cell: cellOff[t] <= c < cellOff[t + 1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s.cellOff[mid] <= c) lo = mid;
+            else hi = mid;
+        }
+        V aC, cC;
+        cell_coef<V>(coef, tabLen, nA, nC, s.d, s.depthT + (int64_t)lo * s.d, s.cellLo + c * s.d, aC, cC);
+        cc[inst * ccStride + 2 * c] = aC;
+        cc[inst * ccStride + 2 * c + 1] = cC;
+    }
+}
 
 // ------------------------------------------------------------------ calcNodal
 template <typename V>
@@ -82,7 +107,8 @@ __global__ void __launch_bounds__(256) sp_nodal_kernel(SparseDev s, const V* __r
     V* out = nodal + inst * nStride;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nml; r += (int64_t)gridDim.x * blockDim.x) {
         V v = V(0);
-        for (int64_t k = s.nodPtr[r]; k < s.nodPtr[r + 1]; ++k) v += V(s.nodVal[k]) * xi[s.nodCol[k]];
+        const int64_t k1 = s.nodPtr[r + 1];
+        for (int64_t k = s.nodPtr[r]; k < k1; ++k) v = fma(V(s.nodVal[k]), xi[s.nodCol[k]], v);
         out[r] = v;
     }
 }
@@ -125,7 +151,11 @@ __global__ void __launch_bounds__(256) sp_prol_kernel(const ProlItem* __restrict
 // ---------------------------------------------------------------------- apply
 // One workgroup per (case, depth) item and instance: stencil on G_T (+ gM), the
 // restrictions with their gM exchanges, and the conversion into the part buffer.
-template <typename V>
+// D > 0 (d = D <= SP_CELL_MAXD): the stencil runs cell by cell -- one thread loads
+// the 2^D corner values of a cell once, forms the local product with the element
+// matrix held in registers and stores it; each point then sums its entries of the
+// cells around it.  D = 0: any d, each point gathers the corners of its cells.
+template <typename V, int D>
 __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
     const int inst = blockIdx.x % a.batch;
     if (a.st && !a.st[inst].active) return;
@@ -138,30 +168,99 @@ __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
     V* A = a.zbuf + inst * a.zStride + it.z_off;
     V* B = A + size;
     V* wb = a.wbuf + inst * a.wStride;
-    const V* coef = a.coefTab + inst * a.coefStride;
+    const V* cc = a.cellCoef + inst * a.ccStride;
     const unsigned nc = 1u << d;
-    // stencil: z_i = sum over the cells around point i of the element rows
+    const bool mass = a.nC > 0;
+    if constexpr (D > 0) {
+        // element matrix of a level-T cell, stiffness and mass parts: the entry between
+        // corners p and q depends only on the mask p ^ q
+        constexpr int NC = 1 << D;
+        V eS[NC], eM[NC];
+#pragma unroll
+        for (int m = 0; m < NC; ++m) {
+            eS[m] = element<V>(D, it.T, 0u, (unsigned)m, V(1), V(0));
+            eM[m] = element<V>(D, it.T, 0u, (unsigned)m, V(0), V(1));
+        }
+        const int64_t c0 = s.cellOff[t];
+        const int64_t ncell = s.cellOff[t + 1] - c0;
+        V* Y = A + 2 * size;
+        for (int64_t c = threadIdx.x; c < ncell; c += blockDim.x) {
+            const int32_t* corner = s.cellCorner + (c0 + c) * NC;
+            V uq[NC];
+#pragma unroll
+            for (int q = 0; q < NC; ++q) {
+                const int32_t j = corner[q];
+                uq[q] = j >= 0 ? u[j] : V(0);
+            }
+            const V aC = cc[2 * (c0 + c)];
+            V y[NC];
+#pragma unroll
+            for (int p = 0; p < NC; ++p) {
+                V acc = V(0);
+#pragma unroll
+                for (int q = 0; q < NC; ++q) acc = fma(eS[p ^ q], uq[q], acc);
+                y[p] = aC * acc;
+            }
+            if (mass) {
+                const V cC = cc[2 * (c0 + c) + 1];
+#pragma unroll
+                for (int p = 0; p < NC; ++p) {
+                    V acc = V(0);
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) acc = fma(eM[p ^ q], uq[q], acc);
+                    y[p] = fma(cC, acc, y[p]);
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < NC; ++p) Y[c * NC + p] = y[p];
+        }
+        pass_barrier();
+        // z_i = sum of the row's entries of the cells around point i (+ gM)
+        const int64_t r0 = s.boxOff[t];
+        const V* g = it.rd_off[0] >= 0 ? wb + it.rd_off[0] : nullptr;
+        for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
+            V z = V(0);
+            const int64_t k1 = s.rcPtr[r0 + i + 1];
+            for (int64_t k = s.rcPtr[r0 + i]; k < k1; ++k) z += Y[s.rcY[k]];
+            A[i] = g ? z + g[i] : z;
+        }
+        pass_barrier();
+    } else {
+    // element matrix of a level-T cell, split into its stiffness and mass parts: the
+    // entry between corners p and q depends only on the mask p ^ q
+    __shared__ V eS[1 << MAXD], eM[1 << MAXD];
+    for (unsigned m = threadIdx.x; m < nc; m += blockDim.x) {
+        eS[m] = element<V>(d, it.T, 0u, m, V(1), V(0));
+        eM[m] = element<V>(d, it.T, 0u, m, V(0), V(1));
+    }
+    __syncthreads();
+    // stencil: z_i = sum over the cells around point i of a_cell (K u)_i + c_cell (M u)_i
     {
         const int64_t r0 = s.boxOff[t];
         const int64_t c0 = s.cellOff[t];
         const V* g = it.rd_off[0] >= 0 ? wb + it.rd_off[0] : nullptr;
         for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
             V z = V(0);
-            for (int64_t k = s.rcPtr[r0 + i]; k < s.rcPtr[r0 + i + 1]; ++k) {
+            const int64_t k1 = s.rcPtr[r0 + i + 1];
+            for (int64_t k = s.rcPtr[r0 + i]; k < k1; ++k) {
                 const int64_t cell = c0 + s.rcCell[k];
                 const unsigned cj = (unsigned)s.rcCorner[k];
-                V aC, cC;
-                cell_coef<V>(coef, a.tabLen, a.nA, a.nC, d, it.T, s.cellLo + cell * d, aC, cC);
                 const int32_t* corner = s.cellCorner + cell * nc;
+                V zs = V(0), zm = V(0);
                 for (unsigned q = 0; q < nc; ++q) {
                     const int32_t j = corner[q];
                     if (j < 0) continue;
-                    z += element<V>(d, it.T, cj, q, aC, cC) * u[j];
+                    const V uj = u[j];
+                    zs = fma(eS[cj ^ q], uj, zs);
+                    if (mass) zm = fma(eM[cj ^ q], uj, zm);
                 }
+                z = fma(cc[2 * cell], zs, z);
+                if (mass) z = fma(cc[2 * cell + 1], zm, z);
             }
             A[i] = g ? z + g[i] : z;
         }
         pass_barrier();
+    }
     }
     // restrictions along the restriction dims (ascending) with the gM exchange
     int64_t cur = size;
@@ -257,6 +356,13 @@ __global__ void __launch_bounds__(256) sp_diag_kernel(SparseDev s, const uint8_t
 
 // -------------------------------------------------------------------- launchers
 template <typename V>
+void launch_sp_cellcoef(const SparseDev& s, int64_t nCells, int nA, int nC, const V* coefTab, int64_t coefStride,
+                        int64_t tabLen, V* cc, int64_t ccStride, int batch, hipStream_t str) {
+    if (nCells == 0) return;
+    hipLaunchKernelGGL(sp_cellcoef_kernel<V>, dim3(gridOf(nCells, 256), batch), dim3(256), 0, str, s, nCells, nA, nC,
+                       coefTab, coefStride, tabLen, cc, ccStride);
+}
+template <typename V>
 void launch_sp_nodal(const SparseDev& s, const V* x, int64_t xStride, V* nodal, int64_t nStride, int64_t nml,
                      int batch, const CGState* st, hipStream_t str) {
     hipLaunchKernelGGL(sp_nodal_kernel<V>, dim3(gridOf(nml), batch), dim3(256), 0, str, s, x, xStride, nodal, nStride,
@@ -272,7 +378,15 @@ void launch_sp_prol(const ProlItem* items, int nitems, const SparseDev& s, const
 template <typename V>
 void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str) {
     if (nitems == 0) return;
-    hipLaunchKernelGGL(sp_apply_kernel<V>, dim3((unsigned)nitems * (unsigned)a.batch), dim3(256), 0, str, a);
+    const dim3 grid((unsigned)nitems * (unsigned)a.batch);
+    static_assert(SP_CELL_MAXD == 4, "one instantiation per cell-wise dimension");
+    switch (a.sp.d) {
+        case 1: hipLaunchKernelGGL((sp_apply_kernel<V, 1>), grid, dim3(256), 0, str, a); break;
+        case 2: hipLaunchKernelGGL((sp_apply_kernel<V, 2>), grid, dim3(256), 0, str, a); break;
+        case 3: hipLaunchKernelGGL((sp_apply_kernel<V, 3>), grid, dim3(256), 0, str, a); break;
+        case 4: hipLaunchKernelGGL((sp_apply_kernel<V, 4>), grid, dim3(256), 0, str, a); break;
+        default: hipLaunchKernelGGL((sp_apply_kernel<V, 0>), grid, dim3(256), 0, str, a); break;
+    }
 }
 template <typename V>
 void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, int nA, int nC, const V* coefTab,
@@ -282,6 +396,8 @@ void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, int nA,
 }
 
 #define LSG_SP_INST(V)                                                                                              \
+    template void launch_sp_cellcoef<V>(const SparseDev&, int64_t, int, int, const V*, int64_t, int64_t, V*, int64_t, \
+                                        int, hipStream_t);                                                          \
     template void launch_sp_nodal<V>(const SparseDev&, const V*, int64_t, V*, int64_t, int64_t, int, const CGState*, \
                                      hipStream_t);                                                                  \
     template void launch_sp_prol<V>(const ProlItem*, int, const SparseDev&, const V*, int64_t, V*, int64_t, int,    \

@@ -1,0 +1,139 @@
+"""Throughput of the sparse path on a locally adaptive grid with hanging nodes.
+
+Workload: a d=3 grid refined towards the origin (a regular start grid whose active
+points with support box closer than 0.15 to the origin are refined in every dimension
+up to a sum level, then completeDirichletGrid), written by the reference's own grid
+code into an .npz of `idx` (n x d int32, slot order) and `active` (n uint8).
+Coefficient a(x) = 1 + |x|^2, c = 0.  Right-hand sides are the load vectors of
+f = sum_j eta_j prod_i sin(m_j pi x_i), eta ~ N(0, 1), m = (1, 2), one per instance.
+
+Measured, with the inputs resident on the device:
+  * FP64 and FP32 matvecs at batch 1 and at --batch (HIP events around the launches);
+  * the plain FP64 PCG to 1e-10 relative residual at batch 1;
+  * the mixed-precision solve (FP32 inner PCG, FP64 refinement) to 1e-10 at batch 1
+    and at --batch, with its error against the FP64 solve of the same systems.
+Prints one JSON line.  Run as
+    python3 loadsg_amd/measure_adaptive.py --grid tests/golden/grids/cfg3_start4_level8.npz
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+
+
+def _matvec_ms(op, x, y, reps):
+    import torch
+    op.multiplication(x, y)            # first call autotunes / warms up
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        op.multiplication(x, y)
+    ev1.record()
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / reps
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--grid", default=os.path.join(ROOT, "tests", "golden", "grids", "cfg3_start4_level8.npz"))
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--matvec-only", action="store_true", help="time the matvecs only (for profiling)")
+    args = ap.parse_args()
+
+    import torch
+    import loadsg_amd as L
+
+    z = np.load(args.grid)
+    grid = L.AdaptiveSparseGrid(z["idx"], z["active"])
+    st = grid.stats()
+    N = grid.size
+    dev = torch.device("cuda:0")
+    coef = L.VariableCoefficient(L.COEF_CFG3)
+    m = np.array([1, 2], dtype=np.int32)
+    rng = np.random.default_rng(20261203)
+    out = {"workload": "adaptive d=3 cfg3 " + os.path.basename(args.grid), "points": int(N),
+           "hanging": int((z["active"] == 0).sum()), "depths": st["depths"],
+           "multilevel_size": st["multilevel_size"], "apply_items": st["apply_items"],
+           "wavefront_levels": st["wavefront_levels"], "batch": args.batch}
+
+    # matvec timings
+    for B in (1, args.batch):
+        for prec, dt in ((64, torch.float64), (32, torch.float32)):
+            op = L.MatrixVectorHomogen(grid, coef, precision=prec, batch=B)
+            x = torch.randn(B, N, dtype=dt, device=dev)
+            y = torch.zeros_like(x)
+            ms = _matvec_ms(op, x, y, args.reps)
+            out["matvec_fp%d_b%d_ms" % (prec, B)] = ms
+            out["matvec_fp%d_b%d_per_s" % (prec, B)] = B * 1e3 / ms
+            del op
+    print("matvecs done", flush=True, file=sys.stderr)
+    if args.matvec_only:
+        print(json.dumps(out), flush=True)
+        return
+
+    def ops(B):
+        return (L.MatrixVectorHomogen(grid, coef, precision=64, batch=B),
+                L.MatrixVectorHomogen(grid, coef, precision=32, batch=B))
+
+    # plain FP64 PCG, batch 1
+    op64, op32 = ops(1)
+    f1 = torch.zeros(1, N, dtype=torch.float64, device=dev)
+    op64.rhs_sines(rng.standard_normal((1, 2)), m, out=f1)
+    xr = torch.zeros_like(f1)
+    L.CG.solveHomogen(args.tol, xr.clone(), f1, op64, stop_rule=L.STOP_REL, maxit=5000)   # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info = L.CG.solveHomogen(args.tol, xr, f1, op64, stop_rule=L.STOP_REL, maxit=5000)
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    out.update(fp64_cg_b1_s=t64, fp64_cg_b1_solves_per_s=1.0 / t64, fp64_cg_iterations=info["iterations"],
+               fp64_cg_rel_residual=info["rel_residual"])
+    # mixed precision, batch 1
+    xm = torch.zeros_like(f1)
+    L.CG.solveMixed(args.tol, xm.clone(), f1, op64, op32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    im = L.CG.solveMixed(args.tol, xm, f1, op64, op32)
+    torch.cuda.synchronize()
+    tm = time.perf_counter() - t0
+    out.update(mixed_b1_s=tm, mixed_b1_solves_per_s=1.0 / tm, mixed_b1_inner_iterations=im["iterations"],
+               mixed_b1_rel_residual=im["rel_residual"],
+               mixed_b1_err_vs_fp64=float(torch.linalg.norm(xm - xr) / torch.linalg.norm(xr)))
+    del op64, op32
+    print("batch-1 solves done", flush=True, file=sys.stderr)
+
+    # mixed precision, batched
+    B = args.batch
+    op64, op32 = ops(B)
+    f = torch.zeros(B, N, dtype=torch.float64, device=dev)
+    op64.rhs_sines(rng.standard_normal((B, 2)), m, out=f)
+    xb = torch.zeros_like(f)
+    L.CG.solveMixed(args.tol, xb.clone(), f, op64, op32)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ib = L.CG.solveMixed(args.tol, xb, f, op64, op32)
+    torch.cuda.synchronize()
+    tb = time.perf_counter() - t0
+    xf = torch.zeros_like(f)
+    L.CG.solveHomogen(args.tol * 1e-3, xf, f, op64, stop_rule=L.STOP_REL, maxit=5000)
+    err = torch.linalg.norm(xb - xf, dim=1) / torch.linalg.norm(xf, dim=1)
+    out.update(mixed_batch_s=tb, mixed_batch_solves_per_s=B / tb,
+               mixed_batch_mean_inner_iterations=float(np.mean([i["iterations"] for i in ib])),
+               mixed_batch_max_rel_residual=float(max(i["rel_residual"] for i in ib)),
+               mixed_batch_all_converged=bool(all(i["converged"] == 1 for i in ib)),
+               mixed_batch_max_err_vs_fp64=float(err.max()))
+    out["speedup_mixed_batch_vs_fp64_b1"] = out["mixed_batch_solves_per_s"] / out["fp64_cg_b1_solves_per_s"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -12,12 +12,22 @@ The reference preconditioner is run with OMP_NUM_THREADS=1 so that its stale
 hanging-node work-vector values (cg_method.h:356-433 resets v with an
 active-only `v = 0.0`) are deterministic.
 
-Run:  python tests/golden/make_golden.py
+It also writes tests/golden/grids/cfg3_start4_level<L>.npz (idx, active only): the
+harness's config-3 style refinement (start level 4, support boxes within 0.15 of the
+origin, sum level L) followed by completeDirichletGrid, for the throughput measurement
+and the larger sparse-path tests.  The generator's time grows about 3.5x per level
+(level 10: about 7 minutes on one core).
+
+Run:  python tests/golden/make_golden.py [--grids 7,8,9,10]
 """
+import argparse
 import json
 import os
 import subprocess
 import sys
+import tempfile
+
+import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -35,7 +45,28 @@ CASES = {
 }
 
 
+def make_grids(levels):
+    exe = os.path.join(REF, "ref_harness_d3")
+    if not os.path.exists(exe):
+        sys.exit("missing %s: run `make -C oracle ref`" % exe)
+    os.makedirs(os.path.join(HERE, "grids"), exist_ok=True)
+    for lev in levels:
+        with tempfile.TemporaryDirectory() as tmp:
+            cmd = [exe, "adaptive", "4", "0.15", str(lev), "cfg3", tmp, "none"]
+            print(" ".join(cmd), flush=True)
+            subprocess.run(cmd, check=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+            idx = np.load(os.path.join(tmp, "idx.npy")).astype(np.int32)
+            act = np.load(os.path.join(tmp, "active.npy")).astype(np.uint8)
+        np.savez_compressed(os.path.join(HERE, "grids", "cfg3_start4_level%d.npz" % lev), idx=idx, active=act)
+
+
 def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--grids", default="", help="comma-separated sum levels of the refined grids to write")
+    args = ap.parse_args()
+    if args.grids:
+        make_grids([int(v) for v in args.grids.split(",")])
+        return
     meta = {}
     for name, (d, args) in CASES.items():
         out = os.path.join(HERE, name)

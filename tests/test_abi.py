@@ -88,6 +88,18 @@ def test_adaptive_grid_takes_the_sparse_path():
     assert np.array_equal(i2, idx) and np.array_equal(a2, act)
 
 
+@pytest.mark.parametrize("level", [7, 8, 9, 10])
+def test_refined_cfg3_grids_build(level):
+    """The config-3 style grids refined towards the origin (the reference's own refinement
+    and completeDirichletGrid output, start level 4, sum level 7..10) build on the sparse
+    path with the oracle's multilevel size."""
+    from oracle_ctypes import Oracle
+    z = np.load(os.path.join(ROOT, "tests", "golden", "grids", "cfg3_start4_level%d.npz" % level))
+    g = L.AdaptiveSparseGrid(z["idx"], z["active"])
+    assert g.size == z["idx"].shape[0] and (z["active"] == 0).any()
+    assert g.stats()["multilevel_size"] == Oracle(z["idx"], z["active"]).multilevel_size()
+
+
 def test_invalid_input_errors():
     with pytest.raises(L.LoadsgError):
         L.AdaptiveSparseGrid(np.array([[0, 2]], dtype=np.int32))        # boundary point
