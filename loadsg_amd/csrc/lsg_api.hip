@@ -119,6 +119,7 @@ struct lsg_op {
     int nA = 0, nC = 0;
     int64_t tabLen = 0;
     bool general = false;
+    int packc = 0;          // coordinates of a box point fit one word (every depth: sum T_i <= 32)
     int64_t maxW = 0;
     // apply launches: per wavefront level, families bucketed by their largest box
     struct ApplyLaunch {
@@ -249,10 +250,7 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
         a.maxBox = L.maxBox;
         a.maxW = L.maxW;
         a.G = L.G;
-        {
-            const int cb = std::min(10, 64 / g.d);      // a 1D depth t needs t bits (extent 2^t - 1)
-            a.cbits = g.maxLevel <= cb ? cb : 0;
-        }
+        a.packc = op->packc;
         const int thr = op->threadsOf[L.cls] ? op->threadsOf[L.cls] : 256;
         const size_t lds = ((op->general ? 4 : 2) * L.maxBox + L.maxW) * sizeof(V);
         if (launch_apply<V>(a, L.count, B, thr, lds, s) != 0)
@@ -560,6 +558,12 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     if (coefTerms(*coef, g.d, op->nA, op->nC)) return fail(LSG_ERR_ARG, "bad coefficient");
     if (op->nA + op->nC == 0) return fail(LSG_ERR_ARG, "zero operator");
     op->general = !(op->nA == 1 && op->nC == 0);
+    op->packc = 1;
+    for (const auto& T : g.depths) {
+        int bits = 0;
+        for (int i = 0; i < g.d; ++i) bits += T[i];
+        if (bits > 32) op->packc = 0;
+    }
     op->tabLen = (int64_t(1) << (g.maxLevel + 1)) - 2;
     op->sparse = g.sparse;
     if (!op->sparse && P.maxBox >= 65536)

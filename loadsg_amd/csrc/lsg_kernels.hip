@@ -35,7 +35,11 @@ struct Shape {
 template <typename T>
 __device__ inline const LSG_CONST T* cptr(const T* p) { return (const LSG_CONST T*)p; }
 
-__device__ inline int fdiv(int p, float r) { return __float2int_rz((float(p) + 0.5f) * r); }
+// floor(p / n) for 0 <= p < 2^16 from r = rcp(n): (p + 0.5) r rounded once (fma with the
+// exact 0.5 r), truncated by the conversion
+__device__ inline int fdiv(int p, float r) { return (int)__builtin_fmaf(float(p), r, 0.5f * r); }
+// products of box indices and extents (all below 2^16): the full-rate 24-bit multiply
+__device__ inline int mul24(int a, int b) { return __umul24((unsigned)a, (unsigned)b); }
 // Reciprocal of a positive integer for fdiv: the hardware reciprocal (1 ulp) keeps the
 // quotient exact, since its relative error times (p + 0.5)/n stays below 2^-6/n,
 // under the 0.5/n distance of (p + 0.5)/n from an integer.
@@ -97,9 +101,9 @@ struct Split {
 __device__ inline Split split(int p, int Slo, float rSlo, int n, float rn) {
     Split s;
     int q = fdiv(p, rSlo);
-    s.lo = p - q * Slo;
+    s.lo = p - mul24(q, Slo);
     s.hi = fdiv(q, rn);
-    s.k = q - s.hi * n;
+    s.k = q - mul24(s.hi, n);
     return s;
 }
 
@@ -112,17 +116,19 @@ __device__ inline void expand_dim(const V* __restrict__ src, V* __restrict__ dst
     const float rSlo = rcpf(Slo), rn = rcpf(n);
     for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
         Split s = split(p, Slo, rSlo, n, rn);
-        const V* b = src + s.lo + s.hi * Slo * m;
+        const V* b = src + s.lo + mul24(s.hi, Slo * m);
         const int k = s.k + 1;      // 1-based position
         V v;
         if (k & 1) {
             const int j = s.k >> 1;
-            v = cCenter<V>(t, k, n) * b[j * Slo];
-            if (j >= 1) v += V(0.1) * b[(j - 1) * Slo];
-            if (j + 1 < m) v += V(0.1) * b[(j + 1) * Slo];
+            const V* bj = b + mul24(j, Slo);
+            v = cCenter<V>(t, k, n) * bj[0];
+            if (j >= 1) v += V(0.1) * bj[-Slo];
+            if (j + 1 < m) v += V(0.1) * bj[Slo];
         } else {
             const int j = (k >> 1) - 1;
-            v = V(-0.6) * (b[j * Slo] + b[(j + 1) * Slo]);
+            const V* bj = b + mul24(j, Slo);
+            v = V(-0.6) * (bj[0] + bj[Slo]);
         }
         dst[p] = v;
     }
@@ -137,11 +143,12 @@ __device__ inline void contract_dim(const V* __restrict__ src, V* __restrict__ d
     const float rSlo = rcpf(Slo), rm = rcpf(m);
     for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
         Split s = split(p, Slo, rSlo, m, rm);
-        const V* b = src + s.lo + s.hi * Slo * n;
+        const V* b = src + s.lo + mul24(s.hi, Slo * n);
         const int k = 2 * s.k + 1;            // 1-based position, 0-based index k-1
-        V v = cCenter<V>(t, k, n) * b[(k - 1) * Slo];
-        if (k - 1 >= 1) v += V(-0.6) * b[(k - 2) * Slo] + V(0.1) * b[(k - 3) * Slo];
-        if (k + 1 <= n) v += V(-0.6) * b[k * Slo] + V(0.1) * b[(k + 1) * Slo];
+        const V* bk = b + mul24(k - 1, Slo);
+        V v = cCenter<V>(t, k, n) * bk[0];
+        if (k - 1 >= 1) v += V(-0.6) * bk[-Slo] + V(0.1) * bk[-2 * Slo];
+        if (k + 1 <= n) v += V(-0.6) * bk[Slo] + V(0.1) * bk[2 * Slo];
         dst[p] = v;
     }
 }
@@ -155,7 +162,7 @@ __device__ inline void restrict_dim(const V* __restrict__ src, V* __restrict__ d
     const float rS = sh.rS[r], rnc = rcpf(nc);
     for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
         Split s = split(p, S, rS, nc, rnc);
-        const V* b = src + s.lo + s.hi * S * nf + (2 * s.k + 1) * S;
+        const V* b = src + s.lo + mul24(s.hi, S * nf) + mul24(2 * s.k + 1, S);
         dst[p] = b[0] + V(0.5) * (b[-S] + b[S]);
     }
 }
@@ -168,16 +175,17 @@ __device__ inline void prolong_dim(V* __restrict__ cur, const V* __restrict__ pr
     const float rS = sh.rS[fd], rnf = sh.rn[fd];
     for (int p = threadIdx.x; p < sh.total; p += blockDim.x) {
         Split s = split(p, S, rS, nf, rnf);
-        const V* c = prev + s.lo + s.hi * S * nc;
+        const V* c = prev + s.lo + mul24(s.hi, S * nc);
         const int k = s.k + 1;                // fine position
         V add;
+        const V* cb = c + mul24((k - 1) >> 1, S);     // coarse position (k-1)/2 + 1 (even k: k/2)
         if (k & 1) {
             const int cl = (k - 1) >> 1, cr = (k + 1) >> 1;   // coarse positions 0..nc+1
-            V a = (cl >= 1) ? c[(cl - 1) * S] : V(0);
-            V b = (cr <= nc) ? c[(cr - 1) * S] : V(0);
+            V a = (cl >= 1) ? cb[-S] : V(0);
+            V b = (cr <= nc) ? cb[0] : V(0);
             add = V(0.5) * (a + b);
         } else {
-            add = c[((k >> 1) - 1) * S];
+            add = cb[0];
         }
         cur[p] += add;
     }
@@ -204,7 +212,7 @@ __device__ inline void horner_step(V* __restrict__ Y, V* __restrict__ Z, const S
         const int p = threadIdx.x + s * blockDim.x;
         if (p < total) {
             const int q = fdiv(p, rS);
-            const int kq = q - fdiv(q, rn) * n;
+            const int kq = q - mul24(fdiv(q, rn), n);
             const bool hasL = kq > 0, hasR = kq < n - 1;
             const V yc = Y[p];
             const V yl = hasL ? Y[p - S] : V(0);
@@ -244,7 +252,7 @@ __device__ inline void mass_step(V* __restrict__ Y, const Shape& sh, int i, cons
         const int p = threadIdx.x + s * blockDim.x;
         if (p < total) {
             const int q = fdiv(p, rS);
-            const int kq = q - fdiv(q, rn) * n;
+            const int kq = q - mul24(fdiv(q, rn), n);
             const V yc = Y[p];
             const V yl = kq > 0 ? Y[p - S] : V(0);
             const V yr = kq < n - 1 ? Y[p + S] : V(0);
@@ -381,16 +389,17 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
                 const int p = threadIdx.x + s * blockDim.x;
                 if (p < total) {
                     Split sp = split(p, S, rS, nf, rnf);
-                    const V* c = prev + sp.lo + sp.hi * S * nc;
+                    const V* c = prev + sp.lo + mul24(sp.hi, S * nc);
                     const int k = sp.k + 1;
                     V add;
+                    const V* cb = c + mul24((k - 1) >> 1, S);
                     if (k & 1) {
                         const int cl = (k - 1) >> 1, cr = (k + 1) >> 1;
-                        V va = (cl >= 1) ? c[(cl - 1) * S] : V(0);
-                        V vb = (cr <= nc) ? c[(cr - 1) * S] : V(0);
+                        V va = (cl >= 1) ? cb[-S] : V(0);
+                        V vb = (cr <= nc) ? cb[0] : V(0);
                         add = V(0.5) * (va + vb);
                     } else {
-                        add = c[((k >> 1) - 1) * S];
+                        add = cb[0];
                     }
                     v[s] = nin[s] + add;
                 }
@@ -561,23 +570,27 @@ __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const Sha
     const V ih = V(1 << t);
     const V h6 = inv6pow2<V>(t);
     V ny[PPT], nz[PPT], yk[PPT];
+    const int B = blockDim.x;
 #pragma unroll
     for (int s = 0; s < PPT; ++s) {
-        const int p = threadIdx.x + s * blockDim.x;
+        const int p = threadIdx.x + s * B;
         if (p < total) {
             const int q = fdiv(p, rS);
-            const int kq = q - fdiv(q, rn) * n;
+            const int kq = q - mul24(fdiv(q, rn), n);
             const bool hasL = kq > 0, hasR = kq < n - 1;
+            // neighbours loaded from an in-box address, then selected
             const V yc = Y[p];
-            const V yl = hasL ? Y[p - S] : V(0);
-            const V yr = hasR ? Y[p + S] : V(0);
+            const V yl0 = Y[hasL ? p - S : p], yr0 = Y[hasR ? p + S : p];
+            const V yl = hasL ? yl0 : V(0);
+            const V yr = hasR ? yr0 : V(0);
             const V wl = w[kq], wr = w[kq + 1];
             ny[s] = h6 * (wl * (yl + V(2) * yc) + wr * (V(2) * yc + yr));
             V out = ih * (wl * (yc - yl) + wr * (yc - yr));
             if (!first) {
                 const V zc = Z[p];
-                const V zl = hasL ? Z[p - S] : V(0);
-                const V zr = hasR ? Z[p + S] : V(0);
+                const V zl0 = Z[hasL ? p - S : p], zr0 = Z[hasR ? p + S : p];
+                const V zl = hasL ? zl0 : V(0);
+                const V zr = hasR ? zr0 : V(0);
                 out += h6 * (wl * (zl + V(2) * zc) + wr * (V(2) * zc + zr));
             }
             nz[s] = out;
@@ -587,7 +600,7 @@ __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const Sha
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < PPT; ++s) {
-        const int p = threadIdx.x + s * blockDim.x;
+        const int p = threadIdx.x + s * B;
         if (p < total) {
             V z = nz[s];
             if (last && fold) {
@@ -693,16 +706,17 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
                 const int p = threadIdx.x + s * blockDim.x;
                 if (p < total) {
                     Split sp = split(p, Sfd, rS, nfd, rnf);
-                    const V* c = U + sp.lo + sp.hi * Sfd * nc;
+                    const V* c = U + sp.lo + mul24(sp.hi, Sfd * nc);
                     const int k = sp.k + 1;
                     V add;
+                    const V* cb = c + mul24((k - 1) >> 1, Sfd);
                     if (k & 1) {
                         const int cl = (k - 1) >> 1, cr = (k + 1) >> 1;
-                        V va = (cl >= 1) ? c[(cl - 1) * Sfd] : V(0);
-                        V vb = (cr <= nc) ? c[(cr - 1) * Sfd] : V(0);
+                        V va = (cl >= 1) ? cb[-Sfd] : V(0);
+                        V vb = (cr <= nc) ? cb[0] : V(0);
                         add = V(0.5) * (va + vb);
                     } else {
-                        add = c[((k >> 1) - 1) * Sfd];
+                        add = cb[0];
                     }
                     const V v = nin[s] + add;
                     Y[p] = v;
@@ -803,7 +817,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
             const V* g = wb + it.rd_off[j];
             for (int p = threadIdx.x; p < outCnt; p += blockDim.x) {
                 Split sp = split(p, S, rS, nc, rnc);
-                const V* b = cur + sp.lo + sp.hi * S * nf + (2 * sp.k + 1) * S;
+                const V* b = cur + sp.lo + mul24(sp.hi, S * nf) + mul24(2 * sp.k + 1, S);
                 const V v = b[0] + V(0.5) * (b[-S] + b[S]);
                 if (w) wd[p] = v;
                 oth[p] = rd ? v + g[p] : v;

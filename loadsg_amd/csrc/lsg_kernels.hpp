@@ -37,7 +37,8 @@ struct ApplyArgs {
     int64_t maxBox;       // LDS box size of this launch
     int64_t maxW;         // LDS weight-table size of this launch
     int G;                // batch instances per workgroup
-    int cbits;            // bits per packed point coordinate (0: recompute coordinates per sweep)
+    int packc;            // 1: every box has sum_i T_i <= 32, so a point's coordinates pack into
+                          // one word (k_i in T_i bits); 0: recompute coordinates per sweep
     int batch;
     const CGState* st;    // may be null
 };

@@ -21,6 +21,7 @@
 #include <algorithm>
 #include <cstring>
 #include <unordered_map>
+#include <set>
 #include <unordered_set>
 
 namespace lsg {
@@ -395,71 +396,102 @@ int buildPlan(Grid& g, std::string& err) {
     double stream = double(g.n) + double(g.nml) + double(g.n);   // calcNodal in/out + final write
     double streamApply = 0.0;   // of which the stages the apply kernel performs
 
+    // chains along dd over a depth list: maximal runs of depths T, T + e_dd, ... (the
+    // reference links T -> T + e_dd only when the finer depth exists,
+    // MatrixVectorHomogen.h:1006; where T + e_dd is missing it adds nothing)
+    auto chainsOf = [&](int dd, const std::vector<int>& list, std::map<Depth, std::vector<int>>& fam) -> int {
+        std::map<Depth, std::vector<int>> lines;
+        for (int t : list) {
+            Depth K = g.depths[t];
+            K[dd] = 0;
+            lines[K].push_back(t);
+        }
+        fam.clear();
+        for (auto& kv : lines) {
+            auto ts = kv.second;
+            std::sort(ts.begin(), ts.end(), [&](int a, int b) { return g.depths[a][dd] < g.depths[b][dd]; });
+            size_t q0 = 0;
+            for (size_t q = 1; q <= ts.size(); ++q)
+                if (q == ts.size() || g.depths[ts[q]][dd] != g.depths[ts[q - 1]][dd] + 1) {
+                    Depth K = kv.first;
+                    K[dd] = g.depths[ts[q0]][dd];      // keyed by the first depth of the chain
+                    fam[K].assign(ts.begin() + q0, ts.begin() + q);
+                    if (q - q0 > (size_t)MAXLEV) { err = "family too long"; return LSG_ERR_UNSUPPORTED; }
+                    q0 = q;
+                }
+        }
+        return 0;
+    };
+
+    // Prolongation stages shared between cases.  A case prolongates along its
+    // prolongation dims p1 < p2 < ... in ascending order, over the lines of its depth
+    // list S_c = {T : T_i >= 2 for the restriction dims i} (DepthList::sortDepths).  A
+    // line along a prolongation dim is either a whole line of the grid or absent from
+    // S_c, so the result after the first j prolongation dims depends only on the prefix
+    // Q = {p1..pj} and is the same arithmetic for every case with that prefix: each
+    // prefix is computed once, over the union of the depth lists of its cases.  Dense
+    // grids leave the last prolongation dim to the apply stage; sparse grids
+    // materialise every prolongation dim.
     const int ncase = 1 << d;
+    std::vector<std::vector<int>> sortedOf(ncase);
+    std::map<unsigned, std::set<int>> prefixDepths;       // prefix mask -> union of depth lists
     for (int c = 0; c < ncase; ++c) {
         bool R[MAXD] = {};
         for (int i = 0; i < d; ++i) R[i] = (c >> i) & 1;
-        std::vector<int> sorted;
         Depth T0{};
-        recursiveDepth(g, R, true, d - 1, T0, sorted);
+        recursiveDepth(g, R, true, d - 1, T0, sortedOf[c]);
+        if (sortedOf[c].empty()) continue;
+        std::vector<int> Pd;
+        for (int i = 0; i < d; ++i) if (!R[i]) Pd.push_back(i);
+        const int nStages = g.sparse ? (int)Pd.size() : (int)Pd.size() - 1;
+        unsigned Q = 0;
+        for (int j = 0; j < nStages; ++j) {
+            Q |= 1u << Pd[j];
+            prefixDepths[Q].insert(sortedOf[c].begin(), sortedOf[c].end());
+        }
+    }
+    std::map<unsigned, std::unordered_map<int, int64_t>> prefixOff;
+    for (auto& [Q, ts] : prefixDepths)
+        for (int t : ts) { prefixOff[Q][t] = P.caseSize; P.caseSize += g.boxSize[t]; }
+    for (auto& [Q, ts] : prefixDepths) {
+        const int dd = 31 - __builtin_clz(Q);
+        const unsigned parent = Q & ~(1u << dd);
+        std::vector<int> list(ts.begin(), ts.end());
+        std::map<Depth, std::vector<int>> fam;
+        if (int rc = chainsOf(dd, list, fam)) return rc;
+        for (auto& [K, tq] : fam) {
+            ProlItem pi{};
+            for (int i = 0; i < d; ++i) pi.T[i] = g.depths[tq[0]][i];
+            pi.dd = dd;
+            pi.nt = (int)tq.size();
+            pi.from_nodal = parent == 0 ? 1 : 0;
+            for (size_t q = 0; q < tq.size(); ++q) {
+                pi.in_off[q] = parent == 0 ? g.boxOff[tq[q]] : prefixOff.at(parent).at(tq[q]);
+                pi.out_off[q] = prefixOff[Q][tq[q]];
+                pi.tidx[q] = tq[q];
+                P.maxFamily = std::max(P.maxFamily, g.boxSize[tq[q]]);
+            }
+            P.prol[dd].push_back(pi);
+        }
+    }
+
+    for (int c = 0; c < ncase; ++c) {
+        bool R[MAXD] = {};
+        for (int i = 0; i < d; ++i) R[i] = (c >> i) & 1;
+        const std::vector<int>& sorted = sortedOf[c];
         if (sorted.empty()) continue;
         int lastP = -1, nP = 0;
         for (int i = 0; i < d; ++i) if (!R[i]) { lastP = i; ++nP; }
-
-        // case buffer boxes: prolongation dims other than the last are swept by the
-        // prolongation stage; the last one is fused into the apply stage
-        std::unordered_map<int, int64_t> caseOff;
-        // (sparse grids: every prolongation dim, the apply stage takes single items)
-        if (nP >= (g.sparse ? 1 : 2)) {
-            for (int t : sorted) { caseOff[t] = P.caseSize; P.caseSize += g.boxSize[t]; }
-        }
-
-        // prolongation families (ascending d, families along d); the reference links
-        // T -> T + e_dd only when the finer depth exists (MatrixVectorHomogen.h:1006)
-        // (a chain breaks where T + e_dd is missing: the reference then adds nothing)
         auto families = [&](int dd, std::map<Depth, std::vector<int>>& fam) -> int {
-            std::map<Depth, std::vector<int>> lines;
-            for (int t : sorted) {
-                Depth K = g.depths[t];
-                K[dd] = 0;
-                lines[K].push_back(t);
-            }
-            fam.clear();
-            for (auto& kv : lines) {
-                auto ts = kv.second;
-                std::sort(ts.begin(), ts.end(), [&](int a, int b) { return g.depths[a][dd] < g.depths[b][dd]; });
-                size_t q0 = 0;
-                for (size_t q = 1; q <= ts.size(); ++q)
-                    if (q == ts.size() || g.depths[ts[q]][dd] != g.depths[ts[q - 1]][dd] + 1) {
-                        Depth K = kv.first;
-                        K[dd] = g.depths[ts[q0]][dd];      // keyed by the first depth of the chain
-                        fam[K].assign(ts.begin() + q0, ts.begin() + q);
-                        if (q - q0 > (size_t)MAXLEV) { err = "family too long"; return LSG_ERR_UNSUPPORTED; }
-                        q0 = q;
-                    }
-            }
-            return 0;
+            return chainsOf(dd, sorted, fam);
         };
-        bool firstProl = true;
-        for (int dd = 0; dd < d; ++dd) {
-            if (R[dd] || (!g.sparse && dd == lastP)) continue;
-            std::map<Depth, std::vector<int>> fam;
-            if (int rc = families(dd, fam)) return rc;
-            for (auto& [K, ts] : fam) {
-                ProlItem pi{};
-                for (int i = 0; i < d; ++i) pi.T[i] = g.depths[ts[0]][i];
-                pi.dd = dd;
-                pi.nt = (int)ts.size();
-                pi.from_nodal = firstProl ? 1 : 0;
-                for (size_t q = 0; q < ts.size(); ++q) {
-                    pi.in_off[q] = firstProl ? g.boxOff[ts[q]] : caseOff[ts[q]];
-                    pi.out_off[q] = caseOff[ts[q]];
-                    pi.tidx[q] = ts[q];
-                    P.maxFamily = std::max(P.maxFamily, g.boxSize[ts[q]]);
-                }
-                P.prol[dd].push_back(pi);
-            }
-            firstProl = false;
+        // the apply stage reads the boxes prolongated along all prolongation dims but
+        // the last one (sparse grids: all of them)
+        std::unordered_map<int, int64_t> caseOff;
+        if (nP >= (g.sparse ? 1 : 2)) {
+            unsigned Q = 0;
+            for (int i = 0; i < d; ++i) if (!R[i] && (g.sparse || i != lastP)) Q |= 1u << i;
+            for (int t : sorted) caseOff[t] = prefixOff.at(Q).at(t);
         }
         // streaming-model prolongation traffic (every prolongation dim, reference order)
         for (int dd = 0; dd < d; ++dd) {
