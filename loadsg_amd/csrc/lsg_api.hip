@@ -119,7 +119,6 @@ struct lsg_op {
     int nA = 0, nC = 0;
     int64_t tabLen = 0;
     bool general = false;
-    int packc = 0;          // coordinates of a box point fit one word (every depth: sum T_i <= 32)
     int64_t maxW = 0;
     // apply launches: per wavefront level, families bucketed by their largest box
     struct ApplyLaunch {
@@ -144,6 +143,9 @@ struct lsg_op {
     SparseDev sd{};
     std::vector<std::pair<int, std::unique_ptr<DevMem>>> spLevels;   // (count, families)
     DevMem zbuf, activeDev, cellCoef;
+    // instances still iterating in a CG solve: the matvec grids cover only these slots
+    DevMem actMap;
+    int nMap = 0;             // > 0: launch nMap slots mapped through actMap
     int64_t nCells = 0;
     bool cellCoefValid = false;
     // solver buffers
@@ -162,19 +164,20 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st) {
     const Grid& g = op->grid->g;
     const Plan& P = g.plan;
     const int d = g.d;
-    const int B = op->batch;
+    const int* map = (st && op->nMap > 0) ? op->actMap.as<int>() : nullptr;
+    const int B = map ? op->nMap : op->batch;
     hipStream_t s = op->stream;
     if (!op->cellCoefValid) {
         launch_sp_cellcoef<V>(op->sd, op->nCells, op->nA, op->nC, op->coefTab.as<V>(),
                               (int64_t)(op->nA + op->nC) * d * op->tabLen, op->tabLen, op->cellCoef.as<V>(),
-                              2 * op->nCells, B, s);
+                              2 * op->nCells, op->batch, s);
         op->cellCoefValid = true;
     }
     HIPCHK(hipEventRecord(op->ev[0], s));
-    launch_sp_nodal<V>(op->sd, x, g.n, op->nodal.as<V>(), g.nml, g.nml, B, st, s);
+    launch_sp_nodal<V>(op->sd, x, g.n, op->nodal.as<V>(), g.nml, g.nml, B, st, s, map);
     for (int dd = 0; dd < d; ++dd)
         launch_sp_prol<V>(op->dProl[dd]->as<ProlItem>(), (int)P.prol[dd].size(), op->sd, op->nodal.as<V>(), g.nml,
-                          op->cbuf.as<V>(), P.caseSize, B, st, s);
+                          op->cbuf.as<V>(), P.caseSize, B, st, s, map);
     HIPCHK(hipEventRecord(op->ev[1], s));
     SparseApplyArgs<V> a{};
     a.sp = op->sd;
@@ -196,6 +199,7 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st) {
     a.zStride = P.zSize;
     a.batch = B;
     a.st = st;
+    a.map = map;
     a.cellCoef = op->cellCoef.as<V>();
     a.ccStride = 2 * op->nCells;
     for (auto& L : op->spLevels) {
@@ -204,7 +208,7 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st) {
     }
     HIPCHK(hipEventRecord(op->ev[2], s));
     launch_sum<V>(op->dSumItems.as<SumItem>(), (int)P.sum.size(), op->dSumList.as<int64_t>(), d, op->part.as<V>(),
-                  P.partSize, y, g.n, B, st, s);
+                  P.partSize, y, g.n, B, st, s, map);
     HIPCHK(hipEventRecord(op->ev[3], s));
     HIPCHK(hipGetLastError());
     op->timed = true;
@@ -217,14 +221,16 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
     const Grid& g = op->grid->g;
     const Plan& P = g.plan;
     const int d = g.d;
-    const int B = op->batch;
+    // inside a CG solve with a compacted instance list, launch only those slots
+    const int* map = (st && op->nMap > 0) ? op->actMap.as<int>() : nullptr;
+    const int B = map ? op->nMap : op->batch;
     hipStream_t s = op->stream;
     HIPCHK(hipEventRecord(op->ev[0], s));
     launch_nodal<V>(op->dNodalItems.as<NodalItem>(), (int)P.nodal.size(), d, x, g.n, op->nodal.as<V>(), g.nml,
-                    P.maxBox, B, st, s);
+                    P.maxBox, B, st, s, map);
     for (int dd = 0; dd < d; ++dd)
         launch_prol<V>(op->dProl[dd]->as<ProlItem>(), (int)P.prol[dd].size(), d, op->nodal.as<V>(), g.nml,
-                       op->cbuf.as<V>(), P.caseSize, op->prolMax[dd], B, st, s);
+                       op->cbuf.as<V>(), P.caseSize, op->prolMax[dd], B, st, s, map);
     HIPCHK(hipEventRecord(op->ev[1], s));
     ApplyArgs<V> a{};
     a.d = d;
@@ -245,12 +251,12 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
     a.st = st;
     a.items = op->dApplyItems.as<ApplyItem>();
     a.batch = B;
+    a.map = map;
     for (auto& L : op->launches) {
         a.fam = L.fam->as<FamilyItem>();
         a.maxBox = L.maxBox;
         a.maxW = L.maxW;
         a.G = L.G;
-        a.packc = op->packc;
         const int thr = op->threadsOf[L.cls] ? op->threadsOf[L.cls] : 256;
         const size_t lds = ((op->general ? 4 : 2) * L.maxBox + L.maxW) * sizeof(V);
         if (launch_apply<V>(a, L.count, B, thr, lds, s) != 0)
@@ -258,7 +264,7 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
     }
     HIPCHK(hipEventRecord(op->ev[2], s));
     launch_sum<V>(op->dSumItems.as<SumItem>(), (int)P.sum.size(), op->dSumList.as<int64_t>(), d, op->part.as<V>(),
-                  P.partSize, y, g.n, B, st, s);
+                  P.partSize, y, g.n, B, st, s, map);
     HIPCHK(hipEventRecord(op->ev[3], s));
     HIPCHK(hipGetLastError());
     op->timed = true;
@@ -377,6 +383,7 @@ int ensureSolver(lsg_op* op) {
     const int nbk = cg_partial_blocks(op->n());
     HIPCHK(op->partials.alloc((size_t)op->batch * 3 * nbk * sizeof(double)));
     HIPCHK(op->st.alloc((size_t)op->batch * sizeof(CGState)));
+    HIPCHK(op->actMap.alloc((size_t)op->batch * sizeof(int)));
     op->solverReady = true;
     return LSG_OK;
 }
@@ -445,6 +452,9 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
     HIPCHK(hipGetLastError());
     out.resize(B);
     const int chunk = 8;
+    op->nMap = 0;
+    std::vector<int> act;     // host copy of the compacted list; the stream syncs before each rewrite
+    act.reserve(B);
     for (int done = 0; done < maxit;) {
         int k = std::min(chunk, maxit - done);
         for (int q = 0; q < k; ++q) {
@@ -454,10 +464,18 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
         done += k;
         HIPCHK(hipMemcpyAsync(out.data(), st, sizeof(CGState) * B, hipMemcpyDeviceToHost, s));
         HIPCHK(hipStreamSynchronize(s));
-        bool any = false;
-        for (auto& c : out) any |= c.active != 0;
-        if (!any) break;
+        act.clear();
+        for (int b = 0; b < B; ++b)
+            if (out[b].active) act.push_back(b);
+        if (act.empty()) break;
+        // finished instances drop out of the matvec grids: their workgroups would only
+        // read their flag and exit, and they delay the dispatch of the ones still running
+        if ((int)act.size() < B) {
+            HIPCHK(hipMemcpyAsync(op->actMap.p, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            op->nMap = (int)act.size();
+        }
     }
+    op->nMap = 0;
     HIPCHK(hipMemcpyAsync(out.data(), st, sizeof(CGState) * B, hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     return LSG_OK;
@@ -558,12 +576,6 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     if (coefTerms(*coef, g.d, op->nA, op->nC)) return fail(LSG_ERR_ARG, "bad coefficient");
     if (op->nA + op->nC == 0) return fail(LSG_ERR_ARG, "zero operator");
     op->general = !(op->nA == 1 && op->nC == 0);
-    op->packc = 1;
-    for (const auto& T : g.depths) {
-        int bits = 0;
-        for (int i = 0; i < g.d; ++i) bits += T[i];
-        if (bits > 32) op->packc = 0;
-    }
     op->tabLen = (int64_t(1) << (g.maxLevel + 1)) - 2;
     op->sparse = g.sparse;
     if (!op->sparse && P.maxBox >= 65536)

@@ -273,11 +273,11 @@ template <typename V>
 __global__ void __launch_bounds__(256) nodal_kernel(const NodalItem* __restrict__ items, int d,
                                                     const V* __restrict__ x, int64_t xStride,
                                                     V* __restrict__ nodal, int64_t nStride, int64_t maxBox,
-                                                    const CGState* __restrict__ st) {
+                                                    const CGState* __restrict__ st, const int* __restrict__ map) {
     extern __shared__ char smem_raw[];
     V* A = reinterpret_cast<V*>(smem_raw);
     V* B = A + maxBox;
-    const int inst = blockIdx.y;
+    const int inst = map ? map[blockIdx.y] : (int)blockIdx.y;
     if (st && !st[inst].active) return;
     const LSG_CONST NodalItem& it = cptr(items)[blockIdx.x];
     const V* xin = x + inst * xStride + it.in_off;
@@ -348,11 +348,12 @@ template <typename V, int PPT>
 __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict__ items, int d,
                                                       const V* __restrict__ nodal, int64_t nStride,
                                                       V* __restrict__ cbuf, int64_t cStride, int64_t maxBox,
-                                                      int batch, const CGState* __restrict__ st) {
+                                                      int batch, const CGState* __restrict__ st,
+                                                      const int* __restrict__ map) {
     extern __shared__ char smem_raw[];
     V* prev = reinterpret_cast<V*>(smem_raw);
     V* cur = prev + maxBox;
-    const int inst = blockIdx.x % batch;
+    const int inst = map ? map[blockIdx.x % batch] : (int)(blockIdx.x % batch);
     if (st && !st[inst].active) return;
     const LSG_CONST ProlItem& it = cptr(items)[blockIdx.x / batch];
     const int dd = it.dd;
@@ -464,7 +465,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
     __shared__ Shape sh;
     // family-major order: all instances of a family are adjacent, and the host lists
     // the families of a launch by decreasing work, so the longest start first
-    const int inst = blockIdx.x % a.batch;
+    const int inst = a.map ? a.map[blockIdx.x % a.batch] : (int)(blockIdx.x % a.batch);
     if (a.st && !a.st[inst].active) return;
     const LSG_CONST FamilyItem& fam = cptr(a.fam)[blockIdx.x / a.batch];
     const int d = a.d;
@@ -639,7 +640,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
     __shared__ Shape shs[MAXD + 1];
     // family-major order: all instances of a family are adjacent, and the host lists
     // the families of a launch by decreasing work, so the longest start first
-    const int inst = blockIdx.x % a.batch;
+    const int inst = a.map ? a.map[blockIdx.x % a.batch] : (int)(blockIdx.x % a.batch);
     if (a.st && !a.st[inst].active) return;
     const LSG_CONST FamilyItem& fam = cptr(a.fam)[blockIdx.x / a.batch];
     const int d = a.d;
@@ -874,8 +875,8 @@ template <typename V>
 __global__ void __launch_bounds__(256) sum_kernel(const SumItem* __restrict__ items, const int64_t* __restrict__ list,
                                                   int d, const V* __restrict__ part, int64_t pStride,
                                                   V* __restrict__ y, int64_t yStride,
-                                                  const CGState* __restrict__ st) {
-    const int inst = blockIdx.y;
+                                                  const CGState* __restrict__ st, const int* __restrict__ map) {
+    const int inst = map ? map[blockIdx.y] : (int)blockIdx.y;
     if (st && !st[inst].active) return;
     const LSG_CONST SumItem& it = cptr(items)[blockIdx.x];
     const int cnt = (int)it.count;
@@ -1204,15 +1205,15 @@ static inline unsigned gridFor(int64_t n, int per = 256 * 4) {
 
 template <typename V>
 void launch_nodal(const NodalItem* items, int nitems, int d, const V* x, int64_t xStride, V* nodal, int64_t nStride,
-                  int64_t maxBox, int batch, const CGState* st, hipStream_t s) {
+                  int64_t maxBox, int batch, const CGState* st, hipStream_t s, const int* map) {
     if (nitems == 0) return;
     size_t lds = 2 * maxBox * sizeof(V);
     hipLaunchKernelGGL(nodal_kernel<V>, dim3(nitems, batch), dim3(256), lds, s, items, d, x, xStride, nodal, nStride,
-                       maxBox, st);
+                       maxBox, st, map);
 }
 template <typename V>
 void launch_prol(const ProlItem* items, int nitems, int d, const V* nodal, int64_t nStride, V* cbuf, int64_t cStride,
-                 int64_t maxBox, int batch, const CGState* st, hipStream_t s) {
+                 int64_t maxBox, int batch, const CGState* st, hipStream_t s, const int* map) {
     if (nitems == 0) return;
     const size_t lds = 2 * maxBox * sizeof(V);
     const int threads = maxBox <= 16 * 256 ? 256 : 512;
@@ -1221,7 +1222,7 @@ void launch_prol(const ProlItem* items, int nitems, int d, const V* nodal, int64
 #define LSG_PROL_CASE(P)                                                                                   \
     if (ppt <= P) {                                                                                        \
         hipLaunchKernelGGL((prol_pf_kernel<V, P>), grid, blk, lds, s, items, d, nodal, nStride, cbuf, cStride, \
-                           maxBox, batch, st);                                                             \
+                           maxBox, batch, st, map);                                                        \
         return;                                                                                            \
     }
     LSG_PROL_CASE(1)
@@ -1263,9 +1264,9 @@ int launch_apply(const ApplyArgs<V>& a, int nitems, int batch, int threads, size
 }
 template <typename V>
 void launch_sum(const SumItem* items, int nitems, const int64_t* list, int d, const V* part, int64_t pStride, V* y,
-                int64_t yStride, int batch, const CGState* st, hipStream_t s) {
+                int64_t yStride, int batch, const CGState* st, hipStream_t s, const int* map) {
     hipLaunchKernelGGL(sum_kernel<V>, dim3(nitems, batch), dim3(128), 0, s, items, list, d, part, pStride, y, yStride,
-                       st);
+                       st, map);
 }
 template <typename V>
 void launch_diag(const SumItem* items, int nitems, int d, int nA, int nC, const V* coefTab, int64_t coefStride,
@@ -1323,12 +1324,12 @@ int cg_partial_blocks(int64_t n) { return (int)gridFor(n); }
 
 #define LSG_INST(V)                                                                                                  \
     template void launch_nodal<V>(const NodalItem*, int, int, const V*, int64_t, V*, int64_t, int64_t, int,         \
-                                  const CGState*, hipStream_t);                                                          \
+                                  const CGState*, hipStream_t, const int*);                                              \
     template void launch_prol<V>(const ProlItem*, int, int, const V*, int64_t, V*, int64_t, int64_t, int, const CGState*, \
-                                 hipStream_t);                                                                       \
+                                 hipStream_t, const int*);                                                           \
     template int launch_apply<V>(const ApplyArgs<V>&, int, int, int, size_t, hipStream_t);                          \
     template void launch_sum<V>(const SumItem*, int, const int64_t*, int, const V*, int64_t, V*, int64_t, int,       \
-                                const CGState*, hipStream_t);                                                            \
+                                const CGState*, hipStream_t, const int*);                                                \
     template void launch_diag<V>(const SumItem*, int, int, int, int, const V*, int64_t, int64_t, V*, int64_t, int,   \
                                  hipStream_t);                                                                       \
     template void launch_rhs_sines<V>(const SumItem*, int, int, int, const double*, const int32_t*, V*, int64_t,     \

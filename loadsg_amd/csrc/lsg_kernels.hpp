@@ -37,10 +37,9 @@ struct ApplyArgs {
     int64_t maxBox;       // LDS box size of this launch
     int64_t maxW;         // LDS weight-table size of this launch
     int G;                // batch instances per workgroup
-    int packc;            // 1: every box has sum_i T_i <= 32, so a point's coordinates pack into
-                          // one word (k_i in T_i bits); 0: recompute coordinates per sweep
-    int batch;
+    int batch;            // instance slots in the grid (workgroup = family-major, slot fastest)
     const CGState* st;    // may be null
+    const int* map;       // slot -> instance (null: slot = instance)
 };
 
 // Device copies of the sparse-path gather tables (SparseTables, lsg_plan.hpp).
@@ -92,6 +91,7 @@ struct SparseApplyArgs {
     int64_t zStride;
     int batch;
     const CGState* st;
+    const int* map;           // slot -> instance (null: slot = instance)
     const V* cellCoef;        // per instance: (a, c) at the centre of every cell of depth T+1
     int64_t ccStride;
 };
@@ -101,10 +101,10 @@ void launch_sp_cellcoef(const SparseDev& s, int64_t nCells, int nA, int nC, cons
                         int64_t tabLen, V* cc, int64_t ccStride, int batch, hipStream_t str);
 template <typename V>
 void launch_sp_nodal(const SparseDev& s, const V* x, int64_t xStride, V* nodal, int64_t nStride, int64_t nml,
-                     int batch, const CGState* st, hipStream_t str);
+                     int batch, const CGState* st, hipStream_t str, const int* map = nullptr);
 template <typename V>
 void launch_sp_prol(const ProlItem* items, int nitems, const SparseDev& s, const V* nodal, int64_t nStride, V* cbuf,
-                    int64_t cStride, int batch, const CGState* st, hipStream_t str);
+                    int64_t cStride, int batch, const CGState* st, hipStream_t str, const int* map = nullptr);
 template <typename V>
 void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str);
 template <typename V>
@@ -113,15 +113,15 @@ void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, int nA,
 
 template <typename V>
 void launch_nodal(const NodalItem* items, int nitems, int d, const V* x, int64_t xStride, V* nodal, int64_t nStride,
-                  int64_t maxBox, int batch, const CGState* st, hipStream_t s);
+                  int64_t maxBox, int batch, const CGState* st, hipStream_t s, const int* map = nullptr);
 template <typename V>
 void launch_prol(const ProlItem* items, int nitems, int d, const V* nodal, int64_t nStride, V* cbuf, int64_t cStride,
-                 int64_t maxBox, int batch, const CGState* st, hipStream_t s);
+                 int64_t maxBox, int batch, const CGState* st, hipStream_t s, const int* map = nullptr);
 template <typename V>
 int launch_apply(const ApplyArgs<V>& a, int nitems, int batch, int threads, size_t lds, hipStream_t s);
 template <typename V>
 void launch_sum(const SumItem* items, int nitems, const int64_t* list, int d, const V* part, int64_t pStride, V* y,
-                int64_t yStride, int batch, const CGState* st, hipStream_t s);
+                int64_t yStride, int batch, const CGState* st, hipStream_t s, const int* map = nullptr);
 template <typename V>
 void launch_diag(const SumItem* items, int nitems, int d, int nA, int nC, const V* coefTab, int64_t coefStride,
                  int64_t tabLen, V* z, int64_t zStride, int batch, hipStream_t s);

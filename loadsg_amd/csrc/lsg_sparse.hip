@@ -100,8 +100,8 @@ __global__ void __launch_bounds__(256) sp_cellcoef_kernel(SparseDev s, int64_t n
 template <typename V>
 __global__ void __launch_bounds__(256) sp_nodal_kernel(SparseDev s, const V* __restrict__ x, int64_t xStride,
                                                        V* __restrict__ nodal, int64_t nStride, int64_t nml,
-                                                       const CGState* __restrict__ st) {
-    const int inst = blockIdx.y;
+                                                       const CGState* __restrict__ st, const int* __restrict__ map) {
+    const int inst = map ? map[blockIdx.y] : (int)blockIdx.y;
     if (st && !st[inst].active) return;
     const V* xi = x + inst * xStride;
     V* out = nodal + inst * nStride;
@@ -119,8 +119,8 @@ template <typename V>
 __global__ void __launch_bounds__(256) sp_prol_kernel(const ProlItem* __restrict__ items, SparseDev s,
                                                       const V* __restrict__ nodal, int64_t nStride,
                                                       V* __restrict__ cbuf, int64_t cStride, int batch,
-                                                      const CGState* __restrict__ st) {
-    const int inst = blockIdx.x % batch;
+                                                      const CGState* __restrict__ st, const int* __restrict__ map) {
+    const int inst = map ? map[blockIdx.x % batch] : (int)(blockIdx.x % batch);
     if (st && !st[inst].active) return;
     const ProlItem& it = items[blockIdx.x / batch];
     const int dd = it.dd;
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(256) sp_prol_kernel(const ProlItem* __restrict
 // cells around it.  D = 0: any d, each point gathers the corners of its cells.
 template <typename V, int D>
 __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
-    const int inst = blockIdx.x % a.batch;
+    const int inst = a.map ? a.map[blockIdx.x % a.batch] : (int)(blockIdx.x % a.batch);
     if (a.st && !a.st[inst].active) return;
     const ApplyItem& it = a.items[a.fam[blockIdx.x / a.batch].first];
     const SparseDev& s = a.sp;
@@ -364,16 +364,16 @@ void launch_sp_cellcoef(const SparseDev& s, int64_t nCells, int nA, int nC, cons
 }
 template <typename V>
 void launch_sp_nodal(const SparseDev& s, const V* x, int64_t xStride, V* nodal, int64_t nStride, int64_t nml,
-                     int batch, const CGState* st, hipStream_t str) {
+                     int batch, const CGState* st, hipStream_t str, const int* map) {
     hipLaunchKernelGGL(sp_nodal_kernel<V>, dim3(gridOf(nml), batch), dim3(256), 0, str, s, x, xStride, nodal, nStride,
-                       nml, st);
+                       nml, st, map);
 }
 template <typename V>
 void launch_sp_prol(const ProlItem* items, int nitems, const SparseDev& s, const V* nodal, int64_t nStride, V* cbuf,
-                    int64_t cStride, int batch, const CGState* st, hipStream_t str) {
+                    int64_t cStride, int batch, const CGState* st, hipStream_t str, const int* map) {
     if (nitems == 0) return;
     hipLaunchKernelGGL(sp_prol_kernel<V>, dim3((unsigned)nitems * (unsigned)batch), dim3(256), 0, str, items, s, nodal,
-                       nStride, cbuf, cStride, batch, st);
+                       nStride, cbuf, cStride, batch, st, map);
 }
 template <typename V>
 void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str) {
@@ -399,9 +399,9 @@ void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, int nA,
     template void launch_sp_cellcoef<V>(const SparseDev&, int64_t, int, int, const V*, int64_t, int64_t, V*, int64_t, \
                                         int, hipStream_t);                                                          \
     template void launch_sp_nodal<V>(const SparseDev&, const V*, int64_t, V*, int64_t, int64_t, int, const CGState*, \
-                                     hipStream_t);                                                                  \
+                                     hipStream_t, const int*);                                                      \
     template void launch_sp_prol<V>(const ProlItem*, int, const SparseDev&, const V*, int64_t, V*, int64_t, int,    \
-                                    const CGState*, hipStream_t);                                                   \
+                                    const CGState*, hipStream_t, const int*);                                       \
     template void launch_sp_apply<V>(const SparseApplyArgs<V>&, int, hipStream_t);                                  \
     template void launch_sp_diag<V>(const SparseDev&, const uint8_t*, int, int, int, const V*, int64_t, int64_t, V*,  \
                                     int64_t, int, hipStream_t);
