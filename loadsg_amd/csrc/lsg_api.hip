@@ -133,7 +133,7 @@ struct lsg_op {
     int threadsOf[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // per size class; 0 = autotune on first use
     int nClasses = 0;
     int64_t maxLds = 0;
-    DevMem dNodalItems, dSumItems, dSumList, dApplyItems;
+    DevMem dNodalItems, dSumItems, dSumList, dApplyItems, dShapes;
     std::vector<std::unique_ptr<DevMem>> dProl;   // per sweep dim
     std::vector<int64_t> prolMax;                 // largest box per sweep dim
     DevMem coefTab, nodal, cbuf, wbuf, part, xs, ys;
@@ -250,6 +250,7 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
     a.pStride = P.partSize;
     a.st = st;
     a.items = op->dApplyItems.as<ApplyItem>();
+    a.shapes = op->dShapes.as<BoxShape>();
     a.batch = B;
     a.map = map;
     for (auto& L : op->launches) {
@@ -689,6 +690,9 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         HIPCHK(op->dProl.back()->alloc(v.size() * sizeof(ProlItem)));
         if (!v.empty()) HIPCHK(hipMemcpy(op->dProl.back()->p, v.data(), v.size() * sizeof(ProlItem), hipMemcpyHostToDevice));
     }
+    HIPCHK(op->dShapes.alloc(std::max<size_t>(1, P.shapes.size()) * sizeof(BoxShape)));
+    if (!P.shapes.empty())
+        HIPCHK(hipMemcpy(op->dShapes.p, P.shapes.data(), P.shapes.size() * sizeof(BoxShape), hipMemcpyHostToDevice));
     HIPCHK(op->dApplyItems.alloc(P.applyItems.size() * sizeof(ApplyItem)));
     HIPCHK(hipMemcpy(op->dApplyItems.p, P.applyItems.data(), P.applyItems.size() * sizeof(ApplyItem),
                      hipMemcpyHostToDevice));

@@ -563,7 +563,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
 // (Z = cM Z + cK Y, see apply_sep_kernel) and adds the gM value g0 read for this box
 // (held in registers).
 template <typename V, int PPT>
-__device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const Shape& sh, int i,
+__device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const LSG_CONST BoxShape& sh, int i,
                                  const V* __restrict__ w, int t, bool first, bool needY, V* __restrict__ keep,
                                  bool last, bool fold, V cM, V cK, bool addG, const V (&g0)[PPT]) {
     const int S = sh.S[i], n = sh.n[i], total = sh.total;
@@ -637,7 +637,6 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
     V* Z = U;
     V* W = Y + a.maxBox;
     __shared__ int woff[MAXD];
-    __shared__ Shape shs[MAXD + 1];
     // family-major order: all instances of a family are adjacent, and the host lists
     // the families of a launch by decreasing work, so the longest start first
     const int inst = a.map ? a.map[blockIdx.x % a.batch] : (int)(blockIdx.x % a.batch);
@@ -747,25 +746,15 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
                 if (p < totalN) nin[s] = in[p];
             }
         }
-        // shapes of the box and after each restriction
-        if (threadIdx.x == 0) {
-            shs[0].d = d;
-            for (int i = 0; i < d; ++i) shs[0].n[i] = (1 << it.T[i]) - 1;
-            shape_restage(shs[0]);
-            for (int j = 1; j <= it.k; ++j) {
-                shs[j] = shs[j - 1];
-                const int r = it.r[j - 1];
-                shs[j].n[r] = (shs[j].n[r] - 1) >> 1;
-                shape_restage(shs[j]);
-            }
-        }
+        // shapes of the box and after each restriction (host-computed, scalar loads)
+        const LSG_CONST BoxShape* shs = cptr(a.shapes) + it.shp;
         __syncthreads();
         // z = A_T P (Horner sum factorisation), + gM read.  Along a dimension of extent 1
         // (T_i = 1) the 1D operators are the scalars m_i = (w0 + w1)/6 and
         // k_i = 2 (w0 + w1); Horner steps commute, so those dimensions are applied last
         // in closed form: Z = cM A' P + cK M' P with A', M' the stiffness and mass
         // operators of the other dimensions and (cM, cK) from the scalar recurrence.
-        const Shape& sh = shs[0];
+        const LSG_CONST BoxShape& sh = shs[0];
         const int wq = (1 << (T0fd + q)) - (1 << T0fd);
         int hi = -1, lo = -1;
         bool fold = false;
@@ -809,7 +798,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
         V* oth = Y;
         for (int j = 1; j <= it.k; ++j) {
             const int r = it.r[j - 1];
-            const Shape& s0 = shs[j - 1];
+            const LSG_CONST BoxShape& s0 = shs[j - 1];
             const int nf = s0.n[r], nc = (nf - 1) >> 1, S = s0.S[r];
             const int outCnt = shs[j].total;
             const float rS = s0.rS[r], rnc = rcpf(nc);
@@ -828,7 +817,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
         }
         // prewavelet conversion, d contractions box -> block; the last one writes the part
         // (extent 1 along a dimension means t = 1 there, where the conversion is the identity)
-        const Shape& sf = shs[it.k];
+        const LSG_CONST BoxShape& sf = shs[it.k];
         int cnt = sf.total;
         int Slo = 1;
         V* out = a.part + inst * a.pStride + it.out_off;

@@ -20,6 +20,7 @@ template <typename V>
 struct ApplyArgs {
     const FamilyItem* fam;    // families of one wavefront level
     const ApplyItem* items;   // all apply items
+    const BoxShape* shapes;   // box shapes of the apply items (ApplyItem::shp)
     int d;
     int general;          // 0: one separable stiffness term (2 LDS boxes), 1: general sum of terms
     int nA, nC;           // stiffness / mass terms

@@ -618,6 +618,25 @@ int buildPlan(Grid& g, std::string& err) {
                 a.ctidx[j] = g.depthIdx.at(Tc);
             }
         }
+        // box shapes after 0..k restrictions (the dense apply kernels read them as they are)
+        a.shp = (int32_t)P.shapes.size();
+        {
+            BoxShape sh{};
+            sh.d = d;
+            for (int i = 0; i < d; ++i) sh.n[i] = (1 << a.T[i]) - 1;
+            for (int j = 0; j <= X.k; ++j) {
+                if (j > 0) sh.n[X.r[j - 1]] = (sh.n[X.r[j - 1]] - 1) >> 1;
+                int st = 1;
+                for (int i = 0; i < d; ++i) {
+                    sh.S[i] = st;
+                    sh.rS[i] = 1.0f / float(st);
+                    sh.rn[i] = 1.0f / float(sh.n[i]);
+                    st *= sh.n[i];
+                }
+                sh.total = st;
+                P.shapes.push_back(sh);
+            }
+        }
         a.z_off = -1;
         if (g.sparse) {      // stencil result and restriction chain, two rows of |G_T|
             a.z_off = P.zSize;    // (+ the element products of the cells of depth T+1)

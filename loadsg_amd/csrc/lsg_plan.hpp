@@ -60,7 +60,18 @@ struct ApplyItem {
     int64_t out_off;            // part buffer block of the target depth
     int64_t z_off;              // scratch of the sparse apply stage (sparse grids only)
     int32_t ctidx[MAXD + 1];    // depth index after j restrictions (j = 0..k)
-    int32_t pad2;
+    int32_t shp;                // first of the k + 1 box shapes (Plan::shapes) of this item
+};
+
+// Shape of a dense box (dimension 0 fastest) with the reciprocals of its extents and
+// strides for the exact float division of point indices (correctly rounded 1/x).
+struct BoxShape {
+    int32_t d;
+    int32_t total;
+    int32_t n[MAXD];      // extent per dim
+    int32_t S[MAXD];      // stride per dim
+    float rn[MAXD];       // 1/n
+    float rS[MAXD];       // 1/S
 };
 
 // A family of apply items along the last prolongation dimension fd of a case
@@ -89,6 +100,7 @@ struct Plan {
     std::vector<std::vector<FamilyItem>> apply;     // per wavefront level
     std::vector<SumItem> sum;
     std::vector<int64_t> sumList;                   // part offsets for SumItem
+    std::vector<BoxShape> shapes;                   // per apply item: the box after 0..k restrictions
     int64_t caseSize = 0;
     int64_t wSize = 0;
     int64_t partSize = 0;
