@@ -565,7 +565,7 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
 template <typename V, int PPT>
 __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const LSG_CONST BoxShape& sh, int i,
                                  const V* __restrict__ w, int t, bool first, bool needY, V* __restrict__ keep,
-                                 bool last, bool fold, V cM, V cK, bool addG, const V (&g0)[PPT]) {
+                                 bool last, bool fold, V cM, V cK, const V* __restrict__ g0) {
     const int S = sh.S[i], n = sh.n[i], total = sh.total;
     const float rS = sh.rS[i], rn = sh.rn[i];
     const V ih = V(1 << t);
@@ -609,7 +609,7 @@ __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const LSG
                 z = fma(cM, z, kz);     // explicit, so every instantiation rounds alike
             }
             if (needY && !last) Y[p] = ny[s];
-            Z[p] = (last && addG) ? z + g0[s] : z;
+            Z[p] = (last && g0) ? z + g0[p] : z;
             if (keep) keep[p] = yk[s];
         }
     }
@@ -664,9 +664,12 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
             o += len;
         }
     }
-    // input of the first box
-    V nin[PPT];
-    {
+    // input of the first box.  FP32: the next box's input is prefetched into registers;
+    // FP64: read at the start of each box (the registers cost more occupancy than the
+    // prefetch hides, measured)
+    constexpr bool kPf = sizeof(V) == 4;
+    V nin[kPf ? PPT : 1];
+    if constexpr (kPf) {
         int total = 1;
         for (int i = 0; i < d; ++i) total *= (1 << it0.T[i]) - 1;
         const V* in = (it0.in_nodal ? nodal : cbuf) + it0.in_off;
@@ -689,13 +692,15 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
         // P = input (+ I_fd(P of the previous box), prolongation1D_inplace +
         // addMultiLevelValues, MatrixVectorHomogen.h:444-474,997-1019)
         V pk[PPT];      // P of this box, parked for the next box at the end
+        const V* in = (it.in_nodal ? nodal : cbuf) + it.in_off;
         if (q == 0) {
 #pragma unroll
             for (int s = 0; s < PPT; ++s) {
                 const int p = threadIdx.x + s * blockDim.x;
                 if (p < total) {
-                    Y[p] = nin[s];
-                    pk[s] = nin[s];
+                    const V v = kPf ? nin[kPf ? s : 0] : in[p];
+                    Y[p] = v;
+                    pk[s] = v;
                 }
             }
         } else {
@@ -718,32 +723,25 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
                     } else {
                         add = cb[0];
                     }
-                    const V v = nin[s] + add;
+                    const V v = (kPf ? nin[kPf ? s : 0] : in[p]) + add;
                     Y[p] = v;
                     pk[s] = v;
                 }
             }
         }
         Z = U;
-        // prefetch: gM value of this box and the input of the next box
-        V g0[PPT];
+        // gM value of this box (added in the last pass of the stencil); prefetch the
+        // input of the next box
         const bool rd0 = it.rd_off[0] >= 0;
-        if (rd0) {
-            const V* g = wb + it.rd_off[0];
-#pragma unroll
-            for (int s = 0; s < PPT; ++s) {
-                const int p = threadIdx.x + s * blockDim.x;
-                if (p < total) g0[s] = g[p];
-            }
-        }
-        if (q + 1 < fam.nt) {
+        const V* g0 = rd0 ? wb + it.rd_off[0] : nullptr;
+        if (kPf && q + 1 < fam.nt) {
             const LSG_CONST ApplyItem& itn = cptr(a.items)[fam.first + q + 1];
             const int totalN = total / nfd * ((nfd << 1) + 1);
-            const V* in = (itn.in_nodal ? nodal : cbuf) + itn.in_off;
+            const V* inN = (itn.in_nodal ? nodal : cbuf) + itn.in_off;
 #pragma unroll
             for (int s = 0; s < PPT; ++s) {
                 const int p = threadIdx.x + s * blockDim.x;
-                if (p < totalN) nin[s] = in[p];
+                if (p < totalN) nin[kPf ? s : 0] = inN[p];
             }
         }
         // shapes of the box and after each restriction (host-computed, scalar loads)
@@ -782,7 +780,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
                 if (p < total) {
                     const V yc = Y[p];
                     const V z = cK * yc;
-                    Z[p] = rd0 ? z + g0[s] : z;
+                    Z[p] = rd0 ? z + g0[p] : z;
                 }
             }
             __syncthreads();
@@ -790,7 +788,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
             for (int i = hi; i >= lo; --i) {
                 if (it.T[i] == 1) continue;
                 sep_sweep<V, PPT>(Y, Z, sh, i, W + woff[i] + (i == fd ? wq : 0), it.T[i], i == hi, true,
-                                  nullptr, i == lo, fold, cM, cK, i == lo && rd0, g0);
+                                  nullptr, i == lo, fold, cM, cK, i == lo ? g0 : nullptr);
             }
         }
         // restrictions along the restriction dims (ascending) with the gM exchange
