@@ -309,16 +309,17 @@ template <typename V>
 __global__ void __launch_bounds__(256) prol_kernel(const ProlItem* __restrict__ items, int d,
                                                    const V* __restrict__ nodal, int64_t nStride,
                                                    V* __restrict__ cbuf, int64_t cStride, int64_t maxBox,
-                                                   const CGState* __restrict__ st) {
+                                                   const CGState* __restrict__ st, const int* __restrict__ map) {
     extern __shared__ char smem_raw[];
     V* prev = reinterpret_cast<V*>(smem_raw);
     V* cur = prev + maxBox;
     __shared__ Shape sh;
-    const int inst = blockIdx.y;
+    const int inst = map ? map[blockIdx.y] : (int)blockIdx.y;
     if (st && !st[inst].active) return;
     const LSG_CONST ProlItem& it = cptr(items)[blockIdx.x];
     const int dd = it.dd;
-    const V* inBase = it.from_nodal ? nodal + inst * nStride : cbuf + inst * cStride;
+    const V* nodalI = nodal + inst * nStride;
+    const V* cbufI = cbuf + inst * cStride;
     V* outBase = cbuf + inst * cStride;
     int32_t T[MAXD];
     for (int i = 0; i < MAXD; ++i) T[i] = it.T[i];
@@ -326,14 +327,16 @@ __global__ void __launch_bounds__(256) prol_kernel(const ProlItem* __restrict__ 
         T[dd] = it.T[dd] + q;
         shape_box(sh, T, d);
         const int cnt = sh.total;
-        copy_in(cur, inBase + it.in_off[q], cnt);
+        copy_in(cur, (((it.from_nodal >> q) & 1) ? nodalI : cbufI) + it.in_off[q], cnt);
         if (q > 0) {
             __syncthreads();
             prolong_dim<V>(cur, prev, sh, dd);
         }
         __syncthreads();
-        V* out = outBase + it.out_off[q];
-        for (int p = threadIdx.x; p < cnt; p += blockDim.x) out[p] = cur[p];
+        if (q > 0) {
+            V* out = outBase + it.out_off[q];
+            for (int p = threadIdx.x; p < cnt; p += blockDim.x) out[p] = cur[p];
+        }
         V* tmp = prev; prev = cur; cur = tmp;
         __syncthreads();
     }
@@ -357,7 +360,10 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
     if (st && !st[inst].active) return;
     const LSG_CONST ProlItem& it = cptr(items)[blockIdx.x / batch];
     const int dd = it.dd;
-    const V* inBase = it.from_nodal ? nodal + inst * nStride : cbuf + inst * cStride;
+    // box q reads the nodal buffer if bit q of from_nodal is set, else the case buffer;
+    // box 0 is not written (its values stay where they are read from)
+    const V* nodalI = nodal + inst * nStride;
+    const V* cbufI = cbuf + inst * cStride;
     V* outBase = cbuf + inst * cStride;
     int total = 1, S = 1, nf = 1;
     for (int i = 0; i < d; ++i) {
@@ -368,7 +374,7 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
     }
     V nin[PPT];
     {
-        const V* in = inBase + it.in_off[0];
+        const V* in = ((it.from_nodal & 1) ? nodalI : cbufI) + it.in_off[0];
 #pragma unroll
         for (int s = 0; s < PPT; ++s) {
             const int p = threadIdx.x + s * blockDim.x;
@@ -408,7 +414,7 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
         }
         const int totalN = keep ? total / nf * (2 * nf + 1) : 0;
         if (keep) {
-            const V* in = inBase + it.in_off[q + 1];
+            const V* in = (((it.from_nodal >> (q + 1)) & 1) ? nodalI : cbufI) + it.in_off[q + 1];
 #pragma unroll
             for (int s = 0; s < PPT; ++s) {
                 const int p = threadIdx.x + s * blockDim.x;
@@ -420,7 +426,7 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
         for (int s = 0; s < PPT; ++s) {
             const int p = threadIdx.x + s * blockDim.x;
             if (p < total) {
-                out[p] = v[s];
+                if (q > 0) out[p] = v[s];
                 if (keep) cur[p] = v[s];
             }
         }
@@ -1222,7 +1228,7 @@ void launch_prol(const ProlItem* items, int nitems, int d, const V* nodal, int64
     LSG_PROL_CASE(16)
 #undef LSG_PROL_CASE
     hipLaunchKernelGGL(prol_kernel<V>, dim3(nitems, batch), dim3(256), lds, s, items, d, nodal, nStride, cbuf, cStride,
-                       maxBox, st);
+                       maxBox, st, map);
 }
 template <typename V>
 int launch_apply(const ApplyArgs<V>& a, int nitems, int batch, int threads, size_t lds, hipStream_t s) {

@@ -450,24 +450,39 @@ int buildPlan(Grid& g, std::string& err) {
             prefixDepths[Q].insert(sortedOf[c].begin(), sortedOf[c].end());
         }
     }
-    std::map<unsigned, std::unordered_map<int, int64_t>> prefixOff;
-    for (auto& [Q, ts] : prefixDepths)
-        for (int t : ts) { prefixOff[Q][t] = P.caseSize; P.caseSize += g.boxSize[t]; }
+    // Where P_Q[t] lives: the first box of a chain has nothing coarser along dd, so
+    // P_Q equals P_parent there and is not copied: it aliases the parent's values (the
+    // nodal box when the parent is empty).  Chains of one box need no work at all.
+    // (std::map visits a prefix after its parent, whose mask is smaller.)
+    struct Src { bool nodal; int64_t off; };
+    std::map<unsigned, std::unordered_map<int, Src>> prefixSrc;
     for (auto& [Q, ts] : prefixDepths) {
         const int dd = 31 - __builtin_clz(Q);
         const unsigned parent = Q & ~(1u << dd);
+        auto srcOf = [&](unsigned Qp, int t) -> Src {
+            return Qp == 0 ? Src{true, g.boxOff[t]} : prefixSrc.at(Qp).at(t);
+        };
         std::vector<int> list(ts.begin(), ts.end());
         std::map<Depth, std::vector<int>> fam;
         if (int rc = chainsOf(dd, list, fam)) return rc;
+        auto& mine = prefixSrc[Q];
         for (auto& [K, tq] : fam) {
+            mine[tq[0]] = srcOf(parent, tq[0]);
+            for (size_t q = 1; q < tq.size(); ++q) {
+                mine[tq[q]] = Src{false, P.caseSize};
+                P.caseSize += g.boxSize[tq[q]];
+            }
+            if (tq.size() < 2) continue;
             ProlItem pi{};
             for (int i = 0; i < d; ++i) pi.T[i] = g.depths[tq[0]][i];
             pi.dd = dd;
             pi.nt = (int)tq.size();
-            pi.from_nodal = parent == 0 ? 1 : 0;
+            pi.from_nodal = 0;
             for (size_t q = 0; q < tq.size(); ++q) {
-                pi.in_off[q] = parent == 0 ? g.boxOff[tq[q]] : prefixOff.at(parent).at(tq[q]);
-                pi.out_off[q] = prefixOff[Q][tq[q]];
+                const Src in = srcOf(parent, tq[q]);
+                if (in.nodal) pi.from_nodal |= 1 << q;
+                pi.in_off[q] = in.off;
+                pi.out_off[q] = q == 0 ? -1 : mine[tq[q]].off;
                 pi.tidx[q] = tq[q];
                 P.maxFamily = std::max(P.maxFamily, g.boxSize[tq[q]]);
             }
@@ -487,11 +502,15 @@ int buildPlan(Grid& g, std::string& err) {
         };
         // the apply stage reads the boxes prolongated along all prolongation dims but
         // the last one (sparse grids: all of them)
+        // (-1: the nodal box itself, whose offset is boxOff[t])
         std::unordered_map<int, int64_t> caseOff;
         if (nP >= (g.sparse ? 1 : 2)) {
             unsigned Q = 0;
             for (int i = 0; i < d; ++i) if (!R[i] && (g.sparse || i != lastP)) Q |= 1u << i;
-            for (int t : sorted) caseOff[t] = prefixOff.at(Q).at(t);
+            for (int t : sorted) {
+                const Src& sr = prefixSrc.at(Q).at(t);
+                caseOff[t] = sr.nodal ? -1 : sr.off;
+            }
         }
         // streaming-model prolongation traffic (every prolongation dim, reference order)
         for (int dd = 0; dd < d; ++dd) {

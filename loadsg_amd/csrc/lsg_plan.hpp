@@ -40,7 +40,7 @@ struct ProlItem {
     int32_t T[MAXD];   // depth of the first box
     int32_t dd;        // sweep dimension
     int32_t nt;        // number of boxes
-    int32_t from_nodal;
+    int32_t from_nodal;  // bit q: box q's input is in the nodal buffer (else the case buffer)
     int32_t pad;
     int64_t in_off[MAXLEV];
     int64_t out_off[MAXLEV];

@@ -124,17 +124,19 @@ __global__ void __launch_bounds__(256) sp_prol_kernel(const ProlItem* __restrict
     if (st && !st[inst].active) return;
     const ProlItem& it = items[blockIdx.x / batch];
     const int dd = it.dd;
-    const V* inBase = it.from_nodal ? nodal + inst * nStride : cbuf + inst * cStride;
+    // box q reads the nodal buffer if bit q of from_nodal is set, else the case buffer;
+    // box 0 keeps its values where they are (nothing coarser along dd)
+    const V* nodalI = nodal + inst * nStride;
+    const V* cbufI = cbuf + inst * cStride;
     V* outBase = cbuf + inst * cStride;
-    for (int q = 0; q < it.nt; ++q) {
+    for (int q = 1; q < it.nt; ++q) {
         const int t = it.tidx[q];
         const int64_t size = s.boxSize[t];
-        const V* in = inBase + it.in_off[q];
+        const V* in = (((it.from_nodal >> q) & 1) ? nodalI : cbufI) + it.in_off[q];
         V* out = outBase + it.out_off[q];
-        if (q == 0) {
-            for (int64_t i = threadIdx.x; i < size; i += blockDim.x) out[i] = in[i];
-        } else {
-            const V* prev = outBase + it.out_off[q - 1];
+        {
+            const V* prev = q == 1 ? (((it.from_nodal & 1) ? nodalI : cbufI) + it.in_off[0])
+                                   : outBase + it.out_off[q - 1];
             const int32_t* ab = s.prolAB + 2 * s.prolOff[(int64_t)t * s.d + dd];
             for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
                 const int32_t a = ab[2 * i], b = ab[2 * i + 1];
