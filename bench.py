@@ -12,6 +12,18 @@ FP64 GPU CG (batch 1), the FP64 error of the mixed solutions, the apply-kernel
 roofline (FP32 batched matvec, the inner-solver kernel), and the reference CPU path timed on this host.
 
 Prints one JSON line.
+
+--workload selects one of the other configurations instead (one JSON line each, with
+its own metric; the default line above is the headline):
+  cfg2  d=4 regular level 8, a = 1 + 0.5 prod cos(pi x_i), c = 1 + x_1: FP64 matvec and
+        FP64 PCG to 1e-10 (one instance, and matvecs at batch 64), beside the reference
+        CPU matvec sampled on this host;
+  cfg3  the locally adaptive d=3 grid (start level 4, refined towards the origin to sum
+        level 11, tests/golden/grids), a = 1 + |x|^2: FP32 PCG under FP64 refinement to
+        1e-10 against the FP64 solve (loadsg_amd/measure_adaptive.py);
+  cfg4  64 instances with d uniform in {3..6} and level in {5..8} (seeded), exp-sine
+        coefficients (16 xi), 8-term sine right-hand sides, solved together to 1e-8
+        (loadsg_amd.solve_many: one stream per grid shape).
 """
 import argparse
 import json
@@ -84,6 +96,141 @@ def cpu_baseline(iters_fp64, budget_s):
             "matvec_s": t_mv}
 
 
+def ref_sample_matvec(d, level, coef, budget_s):
+    """Reference CPU matvec (oracle/_ref/ref_harness_d<d> samplemv, the unmodified
+    LoAdSG sources) timed on this host: seconds per matvec, or None."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_d%d" % d)
+    if not os.path.exists(exe):
+        return None, 0
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close")
+    try:
+        r = subprocess.run([exe, "samplemv", str(level), coef, str(budget_s)], env=env, capture_output=True,
+                           text=True, timeout=budget_s * 6 + 120)
+    except subprocess.TimeoutExpired:
+        return None, cores
+    if r.returncode != 0:
+        return None, cores
+    return json.loads(r.stdout.strip().splitlines()[-1]), cores
+
+
+def workload_cfg2(args):
+    import torch
+    import loadsg_amd as L
+    d, level = 4, 8
+    grid = L.AdaptiveSparseGrid.regular(d, level)
+    n = grid.size
+    coef = L.VariableCoefficient(L.COEF_CFG2)
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(SEED)
+    res = {}
+    for B in (1, 64):
+        op = L.MatrixVectorHomogen(grid, coef, precision=64, batch=B)
+        xv = torch.randn(B, n, dtype=torch.float64, device=dev)
+        yv = torch.zeros_like(xv)
+        op.multiplication(xv, yv)
+        torch.cuda.synchronize()
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record()
+        for _ in range(10):
+            op.multiplication(xv, yv)
+        ev1.record()
+        torch.cuda.synchronize()
+        res[B] = ev0.elapsed_time(ev1) / 10
+    op = L.MatrixVectorHomogen(grid, coef, precision=64, batch=1)
+    f = torch.zeros(1, n, dtype=torch.float64, device=dev)
+    op.rhs_sines(rng.standard_normal((1, NETA)), rng.integers(1, 4, NETA).astype(np.int32), out=f)
+    x = torch.zeros_like(f)
+    L.CG.solveHomogen(1e-10, x, f, op, stop_rule=L.STOP_REL, maxit=5000)
+    x.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    info = L.CG.solveHomogen(1e-10, x, f, op, stop_rule=L.STOP_REL, maxit=5000)
+    torch.cuda.synchronize()
+    t_cg = time.perf_counter() - t0
+    log("cfg2: matvec %.3f ms (b1), %.3f ms (b64); CG %d iterations %.3f s" % (res[1], res[64], info["iterations"],
+                                                                              t_cg))
+    cpu = None
+    if not args.no_cpu_baseline:
+        r, cores = ref_sample_matvec(d, level, "cfg2", args.cpu_budget)
+        if r is not None:
+            t_mv = r["matvec_s_estimate"]
+            cpu = {"value": 1.0 / t_mv, "unit": "matvecs/s", "cores": cores, "kind": "reference",
+                   "sample": "LoAdSG MatrixVectorHomogen, d=4 level 8, CFG2 coefficient (%d OpenMP threads): "
+                             "calcNodal + cases %s (%.0f%% of the cell work), scaled to the full matvec"
+                             % (cores, r["cases"], 100 * r["case_weight_fraction"]),
+                   "matvec_s": t_mv, "cg_s_estimate": (info["iterations"] + 1) * t_mv}
+    out = {"metric": "FP64 matvecs/s and PCG to 1e-10 (d=4 L8, variable coefficient)", "value": 1e3 / res[1],
+           "unit": "matvecs/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "d=4 regular sparse grid level 8 (N=%d), a=1+0.5 prod cos(pi x_i), c=1+x_1" % n},
+           "cpu_baseline": cpu,
+           "extra": {"matvec_b1_ms": res[1], "matvec_b64_ms": res[64], "matvecs_per_s_b64": 64e3 / res[64],
+                     "cg_iterations": info["iterations"], "cg_rel_residual": info["rel_residual"],
+                     "cg_s": t_cg, "cg_solves_per_s": 1.0 / t_cg,
+                     "speedup_matvec_vs_cpu": None if cpu is None else cpu["matvec_s"] * 1e3 / res[1],
+                     "speedup_cg_vs_cpu": None if cpu is None else cpu["cg_s_estimate"] / t_cg}}
+    print(json.dumps(out))
+
+
+def workload_cfg3(args):
+    from loadsg_amd.measure_adaptive import run
+    r = run(os.path.join(ROOT, "tests", "golden", "grids", "cfg3_start4_level11.npz"), batch=64, reps=10, tol=1e-10)
+    out = {"metric": "solves/s to 1e-10 rel. residual, locally adaptive d=3 grid (FP32 PCG under FP64 IR)",
+           "value": r["mixed_batch_solves_per_s"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
+           "dtype": "f32 inner / f64 refinement", "data": "reference-generated grid, synthetic right-hand sides",
+           "config": {"workload": r["workload"], "points": r["points"], "hanging": r["hanging"], "batch": 64},
+           "extra": r}
+    print(json.dumps(out))
+
+
+def workload_cfg4(args):
+    import torch
+    import loadsg_amd as L
+    rng = np.random.default_rng(SEED + 4)
+    B = 64
+    ds = rng.integers(3, 7, B)
+    ls = rng.integers(5, 9, B)
+    xi = rng.normal(0.0, 0.5, (B, NXI))
+    eta = rng.standard_normal((B, NETA))
+    m = rng.integers(1, 4, NETA).astype(np.int32)
+    groups = {}
+    for b in range(B):
+        groups.setdefault((int(ds[b]), int(ls[b])), []).append(b)
+    probs = []
+    for (d, level), idx in sorted(groups.items()):
+        grid = L.AdaptiveSparseGrid.regular(d, level)
+        nb = len(idx)
+        o64 = L.MatrixVectorHomogen(grid, L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[idx[0]]), precision=64, batch=nb)
+        o32 = L.MatrixVectorHomogen(grid, L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[idx[0]]), precision=32, batch=nb)
+        for k, b in enumerate(idx):
+            c = L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[b])
+            o64.set_coefficient(k, c)
+            o32.set_coefficient(k, c)
+        f = torch.zeros(nb, grid.size, dtype=torch.float64, device="cuda")
+        o64.rhs_sines(eta[idx], m, out=f)
+        probs.append((torch.zeros_like(f), f, o64, o32))
+    log("cfg4: %d groups set up" % len(probs))
+    L.solve_many(probs, 1e-8)                     # warm-up (autotuning)
+    for x, _, _, _ in probs:
+        x.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos = L.solve_many(probs, 1e-8)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    flat = [i for g in infos for i in g]
+    assert all(i["converged"] == 1 and i["rel_residual"] <= 1e-8 for i in flat)
+    out = {"metric": "solves/s to 1e-8 rel. residual, 64 uneven instances (d 3..6, level 5..8)", "value": B / dt,
+           "unit": "solves/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f32 inner / f64 refinement",
+           "data": "synthetic (seeded PCG64 %d)" % (SEED + 4),
+           "config": {"workload": "64 instances, %d (d, level) groups" % len(probs),
+                      "groups": {"%d,%d" % k: len(v) for k, v in sorted(groups.items())}},
+           "extra": {"s": dt, "inner_iterations_mean": float(np.mean([i["iterations"] for i in flat])),
+                     "inner_iterations_max": int(max(i["iterations"] for i in flat)),
+                     "points_total": int(sum(p[1].shape[0] * p[1].shape[1] for p in probs))}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -96,11 +243,15 @@ def main():
     ap.add_argument("--warm-xi", type=int, default=512,
                     help="coefficient samples for the warm-start training set (x 8 basis solves); 0 = zero start")
     ap.add_argument("--warm-steps", type=int, default=4000)
+    ap.add_argument("--workload", choices=("d5l7", "cfg2", "cfg3", "cfg4"), default="d5l7")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the path shards by instances; see DESIGN.md)",
+        print("bench.py: this build runs on one GPU only (batched solves shard by instances across GPUs)",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload != "d5l7":
+        {"cfg2": workload_cfg2, "cfg3": workload_cfg3, "cfg4": workload_cfg4}[args.workload](args)
+        return
 
     import torch
     import loadsg_amd as L

@@ -40,19 +40,12 @@ def _matvec_ms(op, x, y, reps):
     return ev0.elapsed_time(ev1) / reps
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--grid", default=os.path.join(ROOT, "tests", "golden", "grids", "cfg3_start4_level8.npz"))
-    ap.add_argument("--batch", type=int, default=64)
-    ap.add_argument("--reps", type=int, default=20)
-    ap.add_argument("--tol", type=float, default=1e-10)
-    ap.add_argument("--matvec-only", action="store_true", help="time the matvecs only (for profiling)")
-    args = ap.parse_args()
-
+def run(grid_path, batch=64, reps=20, tol=1e-10, matvec_only=False):
+    """Measure the sparse path on the grid in `grid_path`; returns a dict."""
     import torch
     import loadsg_amd as L
 
-    z = np.load(args.grid)
+    z = np.load(grid_path)
     grid = L.AdaptiveSparseGrid(z["idx"], z["active"])
     st = grid.stats()
     N = grid.size
@@ -60,25 +53,24 @@ def main():
     coef = L.VariableCoefficient(L.COEF_CFG3)
     m = np.array([1, 2], dtype=np.int32)
     rng = np.random.default_rng(20261203)
-    out = {"workload": "adaptive d=3 cfg3 " + os.path.basename(args.grid), "points": int(N),
+    out = {"workload": "adaptive d=3 cfg3 " + os.path.basename(grid_path), "points": int(N),
            "hanging": int((z["active"] == 0).sum()), "depths": st["depths"],
            "multilevel_size": st["multilevel_size"], "apply_items": st["apply_items"],
-           "wavefront_levels": st["wavefront_levels"], "batch": args.batch}
+           "wavefront_levels": st["wavefront_levels"], "batch": batch}
 
     # matvec timings
-    for B in (1, args.batch):
+    for B in (1, batch):
         for prec, dt in ((64, torch.float64), (32, torch.float32)):
             op = L.MatrixVectorHomogen(grid, coef, precision=prec, batch=B)
             x = torch.randn(B, N, dtype=dt, device=dev)
             y = torch.zeros_like(x)
-            ms = _matvec_ms(op, x, y, args.reps)
+            ms = _matvec_ms(op, x, y, reps)
             out["matvec_fp%d_b%d_ms" % (prec, B)] = ms
             out["matvec_fp%d_b%d_per_s" % (prec, B)] = B * 1e3 / ms
             del op
     print("matvecs done", flush=True, file=sys.stderr)
-    if args.matvec_only:
-        print(json.dumps(out), flush=True)
-        return
+    if matvec_only:
+        return out
 
     def ops(B):
         return (L.MatrixVectorHomogen(grid, coef, precision=64, batch=B),
@@ -89,20 +81,20 @@ def main():
     f1 = torch.zeros(1, N, dtype=torch.float64, device=dev)
     op64.rhs_sines(rng.standard_normal((1, 2)), m, out=f1)
     xr = torch.zeros_like(f1)
-    L.CG.solveHomogen(args.tol, xr.clone(), f1, op64, stop_rule=L.STOP_REL, maxit=5000)   # warm-up
+    L.CG.solveHomogen(tol, xr.clone(), f1, op64, stop_rule=L.STOP_REL, maxit=5000)   # warm-up
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    info = L.CG.solveHomogen(args.tol, xr, f1, op64, stop_rule=L.STOP_REL, maxit=5000)
+    info = L.CG.solveHomogen(tol, xr, f1, op64, stop_rule=L.STOP_REL, maxit=5000)
     torch.cuda.synchronize()
     t64 = time.perf_counter() - t0
     out.update(fp64_cg_b1_s=t64, fp64_cg_b1_solves_per_s=1.0 / t64, fp64_cg_iterations=info["iterations"],
                fp64_cg_rel_residual=info["rel_residual"])
     # mixed precision, batch 1
     xm = torch.zeros_like(f1)
-    L.CG.solveMixed(args.tol, xm.clone(), f1, op64, op32)
+    L.CG.solveMixed(tol, xm.clone(), f1, op64, op32)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    im = L.CG.solveMixed(args.tol, xm, f1, op64, op32)
+    im = L.CG.solveMixed(tol, xm, f1, op64, op32)
     torch.cuda.synchronize()
     tm = time.perf_counter() - t0
     out.update(mixed_b1_s=tm, mixed_b1_solves_per_s=1.0 / tm, mixed_b1_inner_iterations=im["iterations"],
@@ -112,19 +104,19 @@ def main():
     print("batch-1 solves done", flush=True, file=sys.stderr)
 
     # mixed precision, batched
-    B = args.batch
+    B = batch
     op64, op32 = ops(B)
     f = torch.zeros(B, N, dtype=torch.float64, device=dev)
     op64.rhs_sines(rng.standard_normal((B, 2)), m, out=f)
     xb = torch.zeros_like(f)
-    L.CG.solveMixed(args.tol, xb.clone(), f, op64, op32)
+    L.CG.solveMixed(tol, xb.clone(), f, op64, op32)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ib = L.CG.solveMixed(args.tol, xb, f, op64, op32)
+    ib = L.CG.solveMixed(tol, xb, f, op64, op32)
     torch.cuda.synchronize()
     tb = time.perf_counter() - t0
     xf = torch.zeros_like(f)
-    L.CG.solveHomogen(args.tol * 1e-3, xf, f, op64, stop_rule=L.STOP_REL, maxit=5000)
+    L.CG.solveHomogen(tol * 1e-3, xf, f, op64, stop_rule=L.STOP_REL, maxit=5000)
     err = torch.linalg.norm(xb - xf, dim=1) / torch.linalg.norm(xf, dim=1)
     out.update(mixed_batch_s=tb, mixed_batch_solves_per_s=B / tb,
                mixed_batch_mean_inner_iterations=float(np.mean([i["iterations"] for i in ib])),
@@ -132,7 +124,18 @@ def main():
                mixed_batch_all_converged=bool(all(i["converged"] == 1 for i in ib)),
                mixed_batch_max_err_vs_fp64=float(err.max()))
     out["speedup_mixed_batch_vs_fp64_b1"] = out["mixed_batch_solves_per_s"] / out["fp64_cg_b1_solves_per_s"]
-    print(json.dumps(out), flush=True)
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--grid", default=os.path.join(ROOT, "tests", "golden", "grids", "cfg3_start4_level8.npz"))
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--matvec-only", action="store_true", help="time the matvecs only (for profiling)")
+    args = ap.parse_args()
+    print(json.dumps(run(args.grid, args.batch, args.reps, args.tol, args.matvec_only)), flush=True)
 
 
 if __name__ == "__main__":
