@@ -16,6 +16,7 @@ from collections import defaultdict
 
 
 def _short(name):
+    name = name.replace("(anonymous namespace)::", "")
     return re.sub(r"\(.*", "", name)
 
 
