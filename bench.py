@@ -54,10 +54,10 @@ def inputs(batch, rng):
     return xi, eta, m
 
 
-def apply_traffic_from_profile():
+def apply_traffic_from_profile(batch):
     """HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the apply stage of one
-    FP32 matvec per batch instance, as recorded from rocprofv3 --pmc runs."""
-    p = os.path.join(ROOT, "profiles", "apply_sep_kernel_fp32_d5l7_b64_pmc.md")
+    FP32 matvec per batch instance, as recorded from rocprofv3 --pmc runs at this batch."""
+    p = os.path.join(ROOT, "profiles", "apply_sep_kernel_fp32_d5l7_b%d_pmc.md" % batch)
     if not os.path.exists(p):
         return None
     for line in open(p):
@@ -236,7 +236,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--batch", type=int, default=256)
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -368,7 +368,7 @@ def main():
     own_bw = stats["apply_values"] * 4.0 * B / (apply_ms * 1e-3)
     stream_bw = stats["stream_values"] * 4.0 * B / (total_ms * 1e-3)
     achieved64 = stats["stream_apply_values"] * 8.0 * B / (apply_ms64 * 1e-3)
-    tr = apply_traffic_from_profile()
+    tr = apply_traffic_from_profile(B)
 
     log("apply stage: FP32 %.3f ms, FP64 %.3f ms per batched matvec" % (apply_ms, apply_ms64))
     cpu = None if args.no_cpu_baseline else cpu_baseline(info1["iterations"], args.cpu_budget)
