@@ -130,7 +130,7 @@ struct lsg_op {
         std::unique_ptr<DevMem> fam;
     };
     std::vector<ApplyLaunch> launches;
-    int threadsOf[8] = {0, 0, 0, 0, 0, 0, 0, 0};   // per size class; 0 = autotune on first use
+    int threadsOf[16] = {};   // per size class; 0 = autotune on first use
     int nClasses = 0;
     int64_t maxLds = 0;
     DevMem dNodalItems, dSumItems, dSumList, dApplyItems, dShapes;
@@ -581,9 +581,11 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     op->sparse = g.sparse;
     if (!op->sparse && P.maxBox >= 65536)
         return fail(LSG_ERR_UNSUPPORTED, "depth boxes above 65535 points are not supported");
-    // size classes of the apply launches (largest box of a family)
-    const int64_t bounds[5] = {128, 512, 2048, 8192, 65536};
-    const int ncls = 5;
+    // size classes of the apply launches (largest box of a family): a launch sizes its
+    // LDS and points per thread for its largest box, so narrow classes keep the
+    // occupancy of the smaller families up
+    const int64_t bounds[12] = {128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 4096, 8192, 65536};
+    const int ncls = 12;
     for (size_t L = 0; !op->sparse && L < P.apply.size(); ++L) {
         std::vector<std::vector<FamilyItem>> bucket(ncls);
         std::vector<int64_t> bmax(ncls, 0), wmax(ncls, 0);
