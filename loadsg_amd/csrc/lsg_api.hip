@@ -583,9 +583,13 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         return fail(LSG_ERR_UNSUPPORTED, "depth boxes above 65535 points are not supported");
     // size classes of the apply launches (largest box of a family): a launch sizes its
     // LDS and points per thread for its largest box, so narrow classes keep the
-    // occupancy of the smaller families up
-    const int64_t bounds[12] = {128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 4096, 8192, 65536};
-    const int ncls = 12;
+    // occupancy of the smaller families up; small batches are launch-latency bound and
+    // take fewer, wider classes
+    const int64_t fine[12] = {128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 4096, 8192, 65536};
+    const int64_t coarse[5] = {128, 512, 2048, 8192, 65536};
+    const bool fineCls = batch >= 16;
+    const int64_t* bounds = fineCls ? fine : coarse;
+    const int ncls = fineCls ? 12 : 5;
     for (size_t L = 0; !op->sparse && L < P.apply.size(); ++L) {
         std::vector<std::vector<FamilyItem>> bucket(ncls);
         std::vector<int64_t> bmax(ncls, 0), wmax(ncls, 0);

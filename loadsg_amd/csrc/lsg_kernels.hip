@@ -155,8 +155,8 @@ __device__ inline void contract_dim(const V* __restrict__ src, V* __restrict__ d
 
 // restriction1D_inverted_inplace along r (MatrixVectorHomogen.h:485-530): coarse
 // position j gets z(2j) + 0.5 (z(2j-1) + z(2j+1)); src box -> dst box.
-template <typename V>
-__device__ inline void restrict_dim(const V* __restrict__ src, V* __restrict__ dst, const Shape& sh, int r) {
+template <typename V, typename SH>
+__device__ inline void restrict_dim(const V* __restrict__ src, V* __restrict__ dst, const SH& sh, int r) {
     const int nf = sh.n[r], nc = (nf - 1) >> 1, S = sh.S[r];
     const int outCnt = sh.total / nf * nc;
     const float rS = sh.rS[r], rnc = rcpf(nc);
@@ -169,8 +169,8 @@ __device__ inline void restrict_dim(const V* __restrict__ src, V* __restrict__ d
 
 // prolongation1D_inplace + addMultiLevelValues along fd (MatrixVectorHomogen.h:444-474,
 // 997-1019): cur (fine along fd) += I_fd(prev) (coarse along fd); sh = fine shape.
-template <typename V>
-__device__ inline void prolong_dim(V* __restrict__ cur, const V* __restrict__ prev, const Shape& sh, int fd) {
+template <typename V, typename SH>
+__device__ inline void prolong_dim(V* __restrict__ cur, const V* __restrict__ prev, const SH& sh, int fd) {
     const int nf = sh.n[fd], nc = (nf - 1) >> 1, S = sh.S[fd];
     const float rS = sh.rS[fd], rnf = sh.rn[fd];
     for (int p = threadIdx.x; p < sh.total; p += blockDim.x) {
@@ -199,8 +199,8 @@ __device__ inline void prolong_dim(V* __restrict__ cur, const V* __restrict__ pr
 // Poisson::integration (Stencil.h:1707-1789) times a(cell centre) sum to
 // sum_i K_i (x) M_{j != i}, applied by the Horner scheme
 //   Z <- M_i Z + K_i Y,  Y <- M_i Y   for i = d-1 .. 0.
-template <typename V, int PPT>
-__device__ inline void horner_step(V* __restrict__ Y, V* __restrict__ Z, const Shape& sh, int i,
+template <typename V, int PPT, typename SH>
+__device__ inline void horner_step(V* __restrict__ Y, V* __restrict__ Z, const SH& sh, int i,
                                    const V* __restrict__ w, int t, bool zeroZ, bool needY) {
     const int S = sh.S[i], n = sh.n[i], total = sh.total;
     const float rS = sh.rS[i], rn = sh.rn[i];
@@ -241,8 +241,8 @@ __device__ inline void horner_step(V* __restrict__ Y, V* __restrict__ Z, const S
     __syncthreads();
 }
 
-template <typename V, int PPT>
-__device__ inline void mass_step(V* __restrict__ Y, const Shape& sh, int i, const V* __restrict__ w, int t) {
+template <typename V, int PPT, typename SH>
+__device__ inline void mass_step(V* __restrict__ Y, const SH& sh, int i, const V* __restrict__ w, int t) {
     const int S = sh.S[i], n = sh.n[i], total = sh.total;
     const float rS = sh.rS[i], rn = sh.rn[i];
     const V h6 = inv6pow2<V>(t);
@@ -468,7 +468,6 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
     V* Zt = Z + a.maxBox;                     // general path only
     V* Wt = a.general ? Zt + a.maxBox : Zt;   // weights
     __shared__ int woff[MAXD];
-    __shared__ Shape sh;
     // family-major order: all instances of a family are adjacent, and the host lists
     // the families of a launch by decreasing work, so the longest start first
     const int inst = a.map ? a.map[blockIdx.x % a.batch] : (int)(blockIdx.x % a.batch);
@@ -480,7 +479,9 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
 
     for (int q = 0; q < fam.nt; ++q) {
         const LSG_CONST ApplyItem& it = cptr(a.items)[fam.first + q];
-        shape_box(sh, it.T, d);
+        // shapes of the box and after each restriction (host-computed, scalar loads)
+        const LSG_CONST BoxShape* shs = cptr(a.shapes) + it.shp;
+        const LSG_CONST BoxShape& sh = shs[0];
         const V* in = (it.in_nodal ? a.nodal + inst * a.nStride : a.cbuf + inst * a.cStride) + it.in_off;
         copy_in(Y, in, sh.total);
         if (q > 0) {
@@ -526,14 +527,14 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
         }
         for (int j = 1; j <= it.k; ++j) {
             const int r = it.r[j - 1];
-            restrict_dim<V>(cur, oth, sh, r);
-            shape_set_extent(sh, r, (sh.n[r] - 1) >> 1);
+            restrict_dim<V>(cur, oth, shs[j - 1], r);
+            __syncthreads();
             V* tmp = cur; cur = oth; oth = tmp;
             const bool w = it.w_off[j] >= 0, rd = it.rd_off[j] >= 0;
             if (w || rd) {
                 V* wd = w ? wbase + it.w_off[j] : nullptr;
                 const V* g = rd ? wbase + it.rd_off[j] : nullptr;
-                for (int p = threadIdx.x; p < sh.total; p += blockDim.x) {
+                for (int p = threadIdx.x; p < shs[j].total; p += blockDim.x) {
                     V v = cur[p];
                     if (w) wd[p] = v;
                     if (rd) cur[p] = v + g[p];
@@ -542,10 +543,11 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
             }
         }
         // prewavelet conversion, d contractions box -> block
-        int cnt = sh.total;
+        const LSG_CONST BoxShape& sf = shs[it.k];
+        int cnt = sf.total;
         int Slo = 1;
         for (int i = 0; i < d; ++i) {
-            const int n = sh.n[i];
+            const int n = sf.n[i];
             int t = it.T[i];
             for (int j = 0; j < it.k; ++j) if (it.r[j] == i) t -= 1;
             const int m = (n + 1) >> 1;
