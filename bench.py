@@ -367,6 +367,10 @@ def main():
     achieved = apply_bytes / (apply_ms * 1e-3)
     own_bw = stats["apply_values"] * 4.0 * B / (apply_ms * 1e-3)
     stream_bw = stats["stream_values"] * 4.0 * B / (total_ms * 1e-3)
+    stream_bw64 = stats["stream_values"] * 8.0 * B / (total_ms64 * 1e-3)
+    # the plain FP64 CG's own matvec (batch 1), streaming-model bytes over its time
+    _, total_ms1, _ = apply_timing(op1, x1)
+    stream_bw1 = stats["stream_values"] * 8.0 / (total_ms1 * 1e-3)
     achieved64 = stats["stream_apply_values"] * 8.0 * B / (apply_ms64 * 1e-3)
     tr = apply_traffic_from_profile(B)
 
@@ -414,6 +418,9 @@ def main():
             "max_rel_err_vs_fp64_solve": err,
             "matvec_fp32_batched_ms": total_ms,
             "matvec_fp64_batched_ms": total_ms64,
+            "matvec_fp64_batched_stream_model_frac": stream_bw64 / HBM_PEAK,
+            "matvec_fp64_b1_ms": total_ms1,
+            "matvec_fp64_b1_stream_model_frac": stream_bw1 / HBM_PEAK,
             "apply_fp64_ms": apply_ms64,
             "apply_fp64_GBps": achieved64 / 1e9,
             "apply_fp64_frac": achieved64 / HBM_PEAK,
