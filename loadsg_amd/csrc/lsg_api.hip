@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -154,6 +155,13 @@ struct lsg_op {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
     bool timed = false;
+    // CG iteration chunk captured as a hipGraph (while every instance is launched):
+    // valid for the x pointer / precision / chunk length it was captured with
+    hipGraphExec_t chunkExec = nullptr;
+    const void* chunkX = nullptr;
+    int chunkK = 0;
+    bool chunkX64 = false;
+    bool capturing = false;   // matvec launches record no timing events while captured
     int64_t n() const { return grid->g.n; }
 };
 
@@ -167,18 +175,18 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st) {
     const int* map = (st && op->nMap > 0) ? op->actMap.as<int>() : nullptr;
     const int B = map ? op->nMap : op->batch;
     hipStream_t s = op->stream;
-    if (!op->cellCoefValid) {
+    if (!op->cellCoefValid) {     // (computed before a CG chunk is captured)
         launch_sp_cellcoef<V>(op->sd, op->nCells, op->nA, op->nC, op->coefTab.as<V>(),
                               (int64_t)(op->nA + op->nC) * d * op->tabLen, op->tabLen, op->cellCoef.as<V>(),
                               2 * op->nCells, op->batch, s);
         op->cellCoefValid = true;
     }
-    HIPCHK(hipEventRecord(op->ev[0], s));
+    if (!op->capturing) HIPCHK(hipEventRecord(op->ev[0], s));
     launch_sp_nodal<V>(op->sd, x, g.n, op->nodal.as<V>(), g.nml, g.nml, B, st, s, map);
     for (int dd = 0; dd < d; ++dd)
         launch_sp_prol<V>(op->dProl[dd]->as<ProlItem>(), (int)P.prol[dd].size(), op->sd, op->nodal.as<V>(), g.nml,
                           op->cbuf.as<V>(), P.caseSize, B, st, s, map);
-    HIPCHK(hipEventRecord(op->ev[1], s));
+    if (!op->capturing) HIPCHK(hipEventRecord(op->ev[1], s));
     SparseApplyArgs<V> a{};
     a.sp = op->sd;
     a.items = op->dApplyItems.as<ApplyItem>();
@@ -206,10 +214,10 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st) {
         a.fam = L.second->as<FamilyItem>();
         launch_sp_apply<V>(a, L.first, s);
     }
-    HIPCHK(hipEventRecord(op->ev[2], s));
+    if (!op->capturing) HIPCHK(hipEventRecord(op->ev[2], s));
     launch_sum<V>(op->dSumItems.as<SumItem>(), (int)P.sum.size(), op->dSumList.as<int64_t>(), d, op->part.as<V>(),
                   P.partSize, y, g.n, B, st, s, map);
-    HIPCHK(hipEventRecord(op->ev[3], s));
+    if (!op->capturing) HIPCHK(hipEventRecord(op->ev[3], s));
     HIPCHK(hipGetLastError());
     op->timed = true;
     return LSG_OK;
@@ -225,13 +233,13 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
     const int* map = (st && op->nMap > 0) ? op->actMap.as<int>() : nullptr;
     const int B = map ? op->nMap : op->batch;
     hipStream_t s = op->stream;
-    HIPCHK(hipEventRecord(op->ev[0], s));
+    if (!op->capturing) HIPCHK(hipEventRecord(op->ev[0], s));
     launch_nodal<V>(op->dNodalItems.as<NodalItem>(), (int)P.nodal.size(), d, x, g.n, op->nodal.as<V>(), g.nml,
                     P.maxBox, B, st, s, map);
     for (int dd = 0; dd < d; ++dd)
         launch_prol<V>(op->dProl[dd]->as<ProlItem>(), (int)P.prol[dd].size(), d, op->nodal.as<V>(), g.nml,
                        op->cbuf.as<V>(), P.caseSize, op->prolMax[dd], B, st, s, map);
-    HIPCHK(hipEventRecord(op->ev[1], s));
+    if (!op->capturing) HIPCHK(hipEventRecord(op->ev[1], s));
     ApplyArgs<V> a{};
     a.d = d;
     a.general = op->general ? 1 : 0;
@@ -263,10 +271,10 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
         if (launch_apply<V>(a, L.count, B, thr, lds, s) != 0)
             return fail(LSG_ERR_UNSUPPORTED, "depth box too large for the chosen block size");
     }
-    HIPCHK(hipEventRecord(op->ev[2], s));
+    if (!op->capturing) HIPCHK(hipEventRecord(op->ev[2], s));
     launch_sum<V>(op->dSumItems.as<SumItem>(), (int)P.sum.size(), op->dSumList.as<int64_t>(), d, op->part.as<V>(),
                   P.partSize, y, g.n, B, st, s, map);
-    HIPCHK(hipEventRecord(op->ev[3], s));
+    if (!op->capturing) HIPCHK(hipEventRecord(op->ev[3], s));
     HIPCHK(hipGetLastError());
     op->timed = true;
     return LSG_OK;
@@ -301,6 +309,10 @@ bool threadsFit(const lsg_op* op, int cls, int threads) {
 
 int chooseThreads(lsg_op* op, int threads) {
     const int cands[8] = {64, 128, 192, 256, 320, 384, 448, 512};
+    if (op->chunkExec) {          // a captured CG chunk holds the old launch shapes
+        HIPCHK(hipGraphExecDestroy(op->chunkExec));
+        op->chunkExec = nullptr;
+    }
     if (threads) {
         for (int c = 0; c < op->nClasses; ++c) {
             if (!threadsFit(op, c, threads)) return fail(LSG_ERR_UNSUPPORTED, "apply workgroup exceeds LDS");
@@ -389,6 +401,10 @@ int ensureSolver(lsg_op* op) {
     return LSG_OK;
 }
 
+// CG chunks replay a captured hipGraph unless LSG_NO_GRAPH is set (direct launches;
+// read at every solve)
+bool useGraphs() { return std::getenv("LSG_NO_GRAPH") == nullptr; }
+
 int computeDiag(lsg_op* op) {
     if (op->diagValid) return LSG_OK;
     const Grid& g = op->grid->g;
@@ -458,9 +474,43 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
     act.reserve(B);
     for (int done = 0; done < maxit;) {
         int k = std::min(chunk, maxit - done);
-        for (int q = 0; q < k; ++q) {
-            if ((rc = runMatvec<V>(op, dv, g, st))) return rc;
-            launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s);
+        if (op->nMap == 0 && useGraphs()) {
+            // the chunk's launches depend only on the operator, x and k: capture once,
+            // then replay (the kernels read the convergence state on the device)
+            if (!op->chunkExec || op->chunkX != x || op->chunkK != k || op->chunkX64 != x64) {
+                if (op->sparse && !op->cellCoefValid) {    // keep this one-off launch out of the graph
+                    if ((rc = runMatvec<V>(op, dv, g, nullptr))) return rc;
+                    HIPCHK(hipStreamSynchronize(s));
+                }
+                if (op->chunkExec) HIPCHK(hipGraphExecDestroy(op->chunkExec));
+                op->chunkExec = nullptr;
+                hipGraph_t graph = nullptr;
+                HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                op->capturing = true;
+                for (int q = 0; q < k && !rc; ++q) {
+                    rc = runMatvec<V>(op, dv, g, st);
+                    if (!rc) launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s);
+                }
+                op->capturing = false;
+                const hipError_t ec = hipStreamEndCapture(s, &graph);
+                if (rc) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return rc;
+                }
+                HIPCHK(ec);
+                const hipError_t ei = hipGraphInstantiate(&op->chunkExec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIPCHK(ei);
+                op->chunkX = x;
+                op->chunkK = k;
+                op->chunkX64 = x64;
+            }
+            HIPCHK(hipGraphLaunch(op->chunkExec, s));
+        } else {
+            for (int q = 0; q < k; ++q) {
+                if ((rc = runMatvec<V>(op, dv, g, st))) return rc;
+                launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s);
+            }
         }
         done += k;
         HIPCHK(hipMemcpyAsync(out.data(), st, sizeof(CGState) * B, hipMemcpyDeviceToHost, s));
@@ -976,6 +1026,7 @@ int lsg_op_destroy(lsg_op* op) {
     if (op->stream) (void)hipStreamSynchronize(op->stream);
     for (auto& e : op->ev)
         if (e) (void)hipEventDestroy(e);
+    if (op->chunkExec) (void)hipGraphExecDestroy(op->chunkExec);
     hipStream_t s = op->stream;
     delete op;
     if (s) (void)hipStreamDestroy(s);
