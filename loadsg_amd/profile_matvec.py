@@ -24,7 +24,7 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=512)
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--copy-mib", type=int, default=1024)
