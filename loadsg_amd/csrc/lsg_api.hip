@@ -959,8 +959,13 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
                 conv[b] = 1;
                 HIPCHK(hipMemsetAsync(r64 + (size_t)b * n, 0, n * 8, s));
             } else all = false;
-            // inner tolerance: inner_rel_tol, or less when the remaining reduction is smaller
-            epsH[b] = std::min(0.5, std::max(inner_rel_tol, 0.3 * rel_tol / std::max(rel[b], 1e-300)));
+            // inner tolerance: inner_rel_tol, or less when the remaining reduction is smaller.
+            // The first sweep aims at inner_rel_tol * ||f|| rather than at inner_rel_tol times
+            // the start residual, so that a good start (rel < 1) shortens it; from a zero
+            // start (rel = 1) both are the same.
+            const double rr = std::max(rel[b], 1e-300);
+            const double target = sweep == 0 ? inner_rel_tol / std::min(rr, 1.0) : inner_rel_tol;
+            epsH[b] = std::min(0.5, std::max(target, 0.3 * rel_tol / rr));
         }
         if (all || sweep == maxit) break;
         HIPCHK(hipMemcpyAsync(epsD.p, epsH.data(), (size_t)B * 8, hipMemcpyHostToDevice, s));
