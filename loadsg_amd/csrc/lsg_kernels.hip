@@ -1211,7 +1211,9 @@ void launch_prol(const ProlItem* items, int nitems, int d, const V* nodal, int64
                  int64_t maxBox, int batch, const CGState* st, hipStream_t s, const int* map) {
     if (nitems == 0) return;
     const size_t lds = 2 * maxBox * sizeof(V);
-    const int threads = maxBox <= 16 * 256 ? 256 : 512;
+    // the narrowest workgroup whose 16 points per thread cover the largest box (a chain
+    // is serial, so more, smaller workgroups hide its latency better; measured)
+    const int threads = maxBox <= 16 * 128 ? 128 : (maxBox <= 16 * 256 ? 256 : 512);
     const int64_t ppt = (maxBox + threads - 1) / threads;
     const dim3 grid((unsigned)nitems * (unsigned)batch), blk(threads);
 #define LSG_PROL_CASE(P)                                                                                   \
