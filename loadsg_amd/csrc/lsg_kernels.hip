@@ -270,7 +270,7 @@ __device__ inline void mass_step(V* __restrict__ Y, const SH& sh, int i, const V
 
 // -------------------------------------------------------------- calcNodal
 template <typename V>
-__global__ void __launch_bounds__(256) nodal_kernel(const NodalItem* __restrict__ items, int d,
+__global__ void __launch_bounds__(128) nodal_kernel(const NodalItem* __restrict__ items, int d,
                                                     const V* __restrict__ x, int64_t xStride,
                                                     V* __restrict__ nodal, int64_t nStride, int64_t maxBox,
                                                     const CGState* __restrict__ st, const int* __restrict__ map) {
@@ -1203,7 +1203,7 @@ void launch_nodal(const NodalItem* items, int nitems, int d, const V* x, int64_t
                   int64_t maxBox, int batch, const CGState* st, hipStream_t s, const int* map) {
     if (nitems == 0) return;
     size_t lds = 2 * maxBox * sizeof(V);
-    hipLaunchKernelGGL(nodal_kernel<V>, dim3(nitems, batch), dim3(256), lds, s, items, d, x, xStride, nodal, nStride,
+    hipLaunchKernelGGL(nodal_kernel<V>, dim3(nitems, batch), dim3(128), lds, s, items, d, x, xStride, nodal, nStride,
                        maxBox, st, map);
 }
 template <typename V>
