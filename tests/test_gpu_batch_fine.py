@@ -1,0 +1,48 @@
+"""Batches of 16 or more take the fine apply size classes and, inside CG, grids compacted
+to the instances still iterating.  Neither may change results: a batch of 20 must match
+20 single-instance operators (coarse classes, no compaction) bit for bit, for the
+separable kernel (exp-sine coefficient) and the multi-term one (CFG2), in the matvec and
+in the mixed-precision solve (where instances finish after different iteration counts)."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("kind", ["expsin", "cfg2"])
+def test_fine_classes_and_compaction_match_single(kind):
+    import torch
+    import loadsg_amd as L
+    print("batch 20 d=4 L5", kind, flush=True)
+    rng = np.random.default_rng(21)
+    B = 20
+    grid = L.AdaptiveSparseGrid.regular(4, 5)
+    xis = rng.normal(0.0, 0.5, (B, 8))
+
+    def coef(b):
+        return L.VariableCoefficient(L.COEF_EXPSIN, xi=xis[b]) if kind == "expsin" else \
+            L.VariableCoefficient(L.COEF_CFG2)
+
+    op64 = L.MatrixVectorHomogen(grid, coef(0), precision=64, batch=B)
+    op32 = L.MatrixVectorHomogen(grid, coef(0), precision=32, batch=B)
+    for b in range(B):
+        op64.set_coefficient(b, coef(b))
+        op32.set_coefficient(b, coef(b))
+    x = torch.tensor(rng.standard_normal((B, grid.size)), device="cuda")
+    y = torch.zeros_like(x)
+    op64.multiplication(x, y)
+    # right-hand sides of very different size: instances converge after different counts
+    f = torch.tensor(rng.standard_normal((B, grid.size)) * np.logspace(-3, 3, B)[:, None], device="cuda")
+    xm = torch.zeros_like(f)
+    info = L.CG.solveMixed(1e-9, xm, f, op64, op32)
+    its = [i["iterations"] for i in info]
+    for b in range(B):
+        s64 = L.MatrixVectorHomogen(grid, coef(b), precision=64, batch=1)
+        s32 = L.MatrixVectorHomogen(grid, coef(b), precision=32, batch=1)
+        ys = torch.zeros_like(x[b:b + 1])
+        s64.multiplication(x[b:b + 1].contiguous(), ys)
+        assert torch.equal(ys[0], y[b])
+        xs = torch.zeros_like(f[b:b + 1])
+        i1 = L.CG.solveMixed(1e-9, xs, f[b:b + 1].contiguous(), s64, s32)
+        assert i1["iterations"] == its[b]
+        assert torch.equal(xs[0], xm[b])
