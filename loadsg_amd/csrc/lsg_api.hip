@@ -167,13 +167,16 @@ struct lsg_op {
 
 namespace {
 
+// nSub > 0: launch only the first nSub slots listed in actMap (a residual over the
+// instances still unconverged); otherwise, inside a CG solve, the compacted CG list.
 template <typename V>
-int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st) {
+int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st, int nSub) {
     const Grid& g = op->grid->g;
     const Plan& P = g.plan;
     const int d = g.d;
-    const int* map = (st && op->nMap > 0) ? op->actMap.as<int>() : nullptr;
-    const int B = map ? op->nMap : op->batch;
+    const int nm = nSub > 0 ? nSub : (st ? op->nMap : 0);
+    const int* map = nm > 0 ? op->actMap.as<int>() : nullptr;
+    const int B = map ? nm : op->batch;
     hipStream_t s = op->stream;
     if (!op->cellCoefValid) {     // (computed before a CG chunk is captured)
         launch_sp_cellcoef<V>(op->sd, op->nCells, op->nA, op->nC, op->coefTab.as<V>(),
@@ -224,14 +227,16 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st) {
 }
 
 template <typename V>
-int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st) {
-    if (op->sparse) return runMatvecSparse<V>(op, x, y, st);
+int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st, int nSub = 0) {
+    if (op->sparse) return runMatvecSparse<V>(op, x, y, st, nSub);
     const Grid& g = op->grid->g;
     const Plan& P = g.plan;
     const int d = g.d;
-    // inside a CG solve with a compacted instance list, launch only those slots
-    const int* map = (st && op->nMap > 0) ? op->actMap.as<int>() : nullptr;
-    const int B = map ? op->nMap : op->batch;
+    // inside a CG solve with a compacted instance list, or for a residual over a
+    // subset (nSub > 0), launch only the slots listed in actMap
+    const int nm = nSub > 0 ? nSub : (st ? op->nMap : 0);
+    const int* map = nm > 0 ? op->actMap.as<int>() : nullptr;
+    const int B = map ? nm : op->batch;
     hipStream_t s = op->stream;
     if (!op->capturing) HIPCHK(hipEventRecord(op->ev[0], s));
     launch_nodal<V>(op->dNodalItems.as<NodalItem>(), (int)P.nodal.size(), d, x, g.n, op->nodal.as<V>(), g.nml,
@@ -434,10 +439,13 @@ int computeDiag(lsg_op* op) {
 }
 
 // PCG on device vectors: f (V), x (V, or double when x64).  Mirrors
-// CG::solveHomogen (applications/cg_method.h:759-886).
+// CG::solveHomogen (applications/cg_method.h:759-886).  `open`, if given, lists the
+// instances that may need iterations (the others have a zero right-hand side): the
+// matvecs launch only those from the first chunk on.
 template <typename V>
 int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, int maxit,
-             std::vector<CGState>& out, bool xZero = false, const double* epsInst = nullptr) {
+             std::vector<CGState>& out, bool xZero = false, const double* epsInst = nullptr,
+             const std::vector<int>* open = nullptr) {
     const int64_t n = op->n();
     const int B = op->batch;
     hipStream_t s = op->stream;
@@ -472,6 +480,11 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
     op->nMap = 0;
     std::vector<int> act;     // host copy of the compacted list; the stream syncs before each rewrite
     act.reserve(B);
+    if (open && !open->empty() && (int)open->size() < B) {
+        act = *open;
+        HIPCHK(hipMemcpyAsync(op->actMap.p, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        op->nMap = (int)act.size();
+    }
     for (int done = 0; done < maxit;) {
         int k = std::min(chunk, maxit - done);
         if (op->nMap == 0 && useGraphs()) {
@@ -532,19 +545,32 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
     return LSG_OK;
 }
 
-// true relative residual ||f - A x|| / ||f|| in FP64 (x, f double, device)
-int trueResidual(lsg_op* op64, const double* f, const double* x, double* tmpAx, double* r, std::vector<double>& rel) {
+// true relative residual ||f - A x|| / ||f|| in FP64 (x, f double, device).  With a
+// non-null `sub` (a strict subset of the instances), only those instances' r and rel
+// are recomputed; the others keep theirs.
+int trueResidual(lsg_op* op64, const double* f, const double* x, double* tmpAx, double* r, std::vector<double>& rel,
+                 const std::vector<int>* sub = nullptr) {
     const int64_t n = op64->n();
     const int B = op64->batch;
-    int rc = runMatvec<double>(op64, x, tmpAx, nullptr);
+    hipStream_t s = op64->stream;
+    const int nSub = (sub && (int)sub->size() < B) ? (int)sub->size() : 0;
+    if (nSub > 0) HIPCHK(hipMemcpyAsync(op64->actMap.p, sub->data(), nSub * sizeof(int), hipMemcpyHostToDevice, s));
+    int rc = runMatvec<double>(op64, x, tmpAx, nullptr, nSub);
     if (rc) return rc;
     const int nb = cg_partial_blocks(n);
-    launch_residual(f, tmpAx, r, n, B, op64->partials.as<double>(), nb, op64->stream);
+    launch_residual(f, tmpAx, r, n, nSub > 0 ? nSub : B, op64->partials.as<double>(), nb, s,
+                    nSub > 0 ? op64->actMap.as<int>() : nullptr);
     std::vector<double> hp((size_t)B * 2 * nb);
-    HIPCHK(hipMemcpyAsync(hp.data(), op64->partials.p, hp.size() * 8, hipMemcpyDeviceToHost, op64->stream));
-    HIPCHK(hipStreamSynchronize(op64->stream));
+    HIPCHK(hipMemcpyAsync(hp.data(), op64->partials.p, hp.size() * 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
     rel.resize(B);
-    for (int b = 0; b < B; ++b) {
+    std::vector<int> all;
+    if (nSub == 0) {
+        all.resize(B);
+        for (int b = 0; b < B; ++b) all[b] = b;
+        sub = &all;
+    }
+    for (int b : *sub) {
         double rr = 0, ff = 0;
         for (int k = 0; k < nb; ++k) rr += hp[((size_t)b * 2 + 0) * nb + k];
         for (int k = 0; k < nb; ++k) ff += hp[((size_t)b * 2 + 1) * nb + k];
@@ -940,8 +966,15 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
     std::vector<CGState> st;
     DevMem epsD;
     HIPCHK(epsD.alloc((size_t)B * 8));
+    std::vector<int> open;    // instances not yet converged
     for (int sweep = 0; sweep <= maxit; ++sweep) {
-        if ((rc = trueResidual(op64, fD, xD, Ax, r64, rel))) return rc;
+        // a converged instance's correction is zero from then on (its inner right-hand
+        // side is zero), so its x, residual and rel stay as they are: recompute the
+        // others only
+        open.clear();
+        for (int b = 0; b < B; ++b)
+            if (!conv[b]) open.push_back(b);
+        if ((rc = trueResidual(op64, fD, xD, Ax, r64, rel, &open))) return rc;
         if (sweep == 0) {
             // a start vector worse than zero (||f - A x0|| > ||f||) is replaced by zero
             bool any = false;
@@ -956,8 +989,8 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
         for (int b = 0; b < B; ++b) {
             if (rel[b] != rel[b]) return fail(LSG_ERR_NAN, "NaN residual");
             if (rel[b] <= rel_tol) {
+                if (!conv[b]) HIPCHK(hipMemsetAsync(r64 + (size_t)b * n, 0, n * 8, s));
                 conv[b] = 1;
-                HIPCHK(hipMemsetAsync(r64 + (size_t)b * n, 0, n * 8, s));
             } else all = false;
             // inner tolerance: inner_rel_tol, or less when the remaining reduction is smaller.
             // The first sweep aims at inner_rel_tol * ||f|| rather than at inner_rel_tol times
@@ -972,8 +1005,11 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
         launch_cast_d2f(r64, r32, tot, s);
         HIPCHK(hipMemsetAsync(e32, 0, tot * 4, s));
         HIPCHK(hipStreamSynchronize(s));
+        open.clear();
+        for (int b = 0; b < B; ++b)
+            if (!conv[b]) open.push_back(b);
         if ((rc = cgDevice<float>(op32, r32, e32, false, inner_rel_tol, LSG_STOP_REL, 1000, st, true,
-                                  epsD.as<double>())))
+                                  epsD.as<double>(), &open)))
             return rc;
         launch_axpy_correction(xD, e32, tot, op32->stream);
         HIPCHK(hipStreamSynchronize(op32->stream));

@@ -1163,8 +1163,8 @@ __global__ void __launch_bounds__(256) cast_kernel(const A* __restrict__ a, B* _
 // r = f - Ax (FP64), partials of r.r and f.f per instance
 __global__ void __launch_bounds__(256) residual_kernel(const double* __restrict__ f, const double* __restrict__ Ax,
                                                        double* __restrict__ r, int64_t n, int64_t stride,
-                                                       double* __restrict__ partials) {
-    const int inst = blockIdx.y;
+                                                       double* __restrict__ partials, const int* __restrict__ map) {
+    const int inst = map ? map[blockIdx.y] : (int)blockIdx.y;
     double v[2] = {0.0, 0.0};
     const int64_t o = inst * stride;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1302,8 +1302,8 @@ void launch_cg_iter_tail(V* r, void* x, bool x64, V* h, V* dv, const V* g, const
     hipLaunchKernelGGL(cg_dir_kernel<V>, dim3(nb, batch), dim3(256), 0, s, dv, h, n, n, st);
 }
 void launch_residual(const double* f, const double* Ax, double* r, int64_t n, int batch, double* partials, int nb,
-                     hipStream_t s) {
-    hipLaunchKernelGGL(residual_kernel, dim3(nb, batch), dim3(256), 0, s, f, Ax, r, n, n, partials);
+                     hipStream_t s, const int* map) {
+    hipLaunchKernelGGL(residual_kernel, dim3(nb, batch), dim3(256), 0, s, f, Ax, r, n, n, partials, map);
 }
 void launch_cast_d2f(const double* a, float* b, int64_t total, hipStream_t s) {
     hipLaunchKernelGGL((cast_kernel<double, float>), dim3(gridFor(total)), dim3(256), 0, s, a, b, total);

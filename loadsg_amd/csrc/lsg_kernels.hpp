@@ -139,8 +139,9 @@ void launch_cg_start(CGState* st, const double* partials, int nb, double eps, co
 template <typename V>
 void launch_cg_iter_tail(V* r, void* x, bool x64, V* h, V* dv, const V* g, const V* D, int64_t n, int batch,
                          double* partials, int nb, CGState* st, hipStream_t s);
+// r = f - Ax and the r.r, f.f partials for `batch` instances (slots map[0..batch) if map)
 void launch_residual(const double* f, const double* Ax, double* r, int64_t n, int batch, double* partials, int nb,
-                     hipStream_t s);
+                     hipStream_t s, const int* map = nullptr);
 void launch_cast_d2f(const double* a, float* b, int64_t total, hipStream_t s);
 void launch_cast_f2d(const float* a, double* b, int64_t total, hipStream_t s);
 void launch_axpy_correction(double* x, const float* e, int64_t total, hipStream_t s);

@@ -46,3 +46,53 @@ def test_fine_classes_and_compaction_match_single(kind):
         i1 = L.CG.solveMixed(1e-9, xs, f[b:b + 1].contiguous(), s64, s32)
         assert i1["iterations"] == its[b]
         assert torch.equal(xs[0], xm[b])
+
+
+def test_refinement_subsets_match_single():
+    """Starts of different quality make instances leave the refinement after different
+    sweeps (an exact start after none, the others after one or more; a start worse than
+    zero is replaced by zero); from the second sweep on, the FP64 residuals and the inner
+    solves are then run over the unconverged instances only.  Each instance must match
+    its single-instance solve bit for bit: x, sweeps, inner iterations and the residual."""
+    import torch
+    import loadsg_amd as L
+    print("batch 20 d=4 L5 mixed starts", flush=True)
+    rng = np.random.default_rng(22)
+    B = 20
+    grid = L.AdaptiveSparseGrid.regular(4, 5)
+    xis = rng.normal(0.0, 0.5, (B, 8))
+
+    def coef(b):
+        return L.VariableCoefficient(L.COEF_EXPSIN, xi=xis[b])
+
+    op64 = L.MatrixVectorHomogen(grid, coef(0), precision=64, batch=B)
+    op32 = L.MatrixVectorHomogen(grid, coef(0), precision=32, batch=B)
+    for b in range(B):
+        op64.set_coefficient(b, coef(b))
+        op32.set_coefficient(b, coef(b))
+    f = torch.tensor(rng.standard_normal((B, grid.size)), device="cuda")
+    xr = torch.zeros_like(f)
+    L.CG.solveHomogen(1e-13, xr, f, op64, stop_rule=L.STOP_REL, maxit=5000)
+    x0 = torch.zeros_like(f)
+    kind = [b % 4 for b in range(B)]
+    for b in range(B):
+        if kind[b] == 0:
+            x0[b] = xr[b]
+        elif kind[b] == 1:
+            x0[b] = xr[b] * (1.0 + 1e-5 * torch.tensor(rng.standard_normal(grid.size), device="cuda"))
+        elif kind[b] == 3:
+            x0[b] = -3.0 * xr[b]
+    xm = x0.clone()
+    info = L.CG.solveMixed(1e-9, xm, f, op64, op32)
+    outer = [i["outer"] for i in info]
+    assert {outer[b] for b in range(B) if kind[b] == 0} == {0}
+    assert max(outer) >= 2, outer
+    for b in range(B):
+        s64 = L.MatrixVectorHomogen(grid, coef(b), precision=64, batch=1)
+        s32 = L.MatrixVectorHomogen(grid, coef(b), precision=32, batch=1)
+        xs = x0[b:b + 1].clone()
+        i1 = L.CG.solveMixed(1e-9, xs, f[b:b + 1].contiguous(), s64, s32)
+        assert (i1["outer"], i1["iterations"]) == (outer[b], info[b]["iterations"]), b
+        assert i1["rel_residual"] == info[b]["rel_residual"], b
+        assert info[b]["converged"] == 1 and info[b]["rel_residual"] <= 1e-9
+        assert torch.equal(xs[0], xm[b]), b
