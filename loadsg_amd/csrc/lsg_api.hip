@@ -137,6 +137,7 @@ struct lsg_op {
     DevMem dNodalItems, dSumItems, dSumList, dApplyItems, dShapes;
     std::vector<std::unique_ptr<DevMem>> dProl;   // per sweep dim
     std::vector<int64_t> prolMax;                 // largest box per sweep dim
+    std::vector<int64_t> prolChain;               // most values in one chain per sweep dim
     DevMem coefTab, nodal, cbuf, wbuf, part, xs, ys;
     // sparse path: gather tables, per-level apply families, scratch
     bool sparse = false;
@@ -243,7 +244,7 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st, int nSub = 0) {
                     P.maxBox, B, st, s, map);
     for (int dd = 0; dd < d; ++dd)
         launch_prol<V>(op->dProl[dd]->as<ProlItem>(), (int)P.prol[dd].size(), d, op->nodal.as<V>(), g.nml,
-                       op->cbuf.as<V>(), P.caseSize, op->prolMax[dd], B, st, s, map);
+                       op->cbuf.as<V>(), P.caseSize, op->prolMax[dd], op->prolChain[dd], B, st, s, map);
     if (!op->capturing) HIPCHK(hipEventRecord(op->ev[1], s));
     ApplyArgs<V> a{};
     a.d = d;
@@ -765,9 +766,13 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         };
         std::stable_sort(v.begin(), v.end(),
                          [&](const ProlItem& x, const ProlItem& y) { return boxes(x, false) > boxes(y, false); });
-        int64_t mb = 1;
-        for (const ProlItem& it : v) mb = std::max(mb, boxes(it, true));
+        int64_t mb = 1, mc = 1;
+        for (const ProlItem& it : v) {
+            mb = std::max(mb, boxes(it, true));
+            mc = std::max(mc, boxes(it, false));
+        }
         op->prolMax.push_back(mb);
+        op->prolChain.push_back(mc);
         op->dProl.emplace_back(new DevMem());
         HIPCHK(op->dProl.back()->alloc(v.size() * sizeof(ProlItem)));
         if (!v.empty()) HIPCHK(hipMemcpy(op->dProl.back()->p, v.data(), v.size() * sizeof(ProlItem), hipMemcpyHostToDevice));

@@ -342,6 +342,99 @@ __global__ void __launch_bounds__(256) prol_kernel(const ProlItem* __restrict__ 
     }
 }
 
+// Prolongation chain along dd with the inputs of all its boxes loaded up front: the
+// chain's values are dealt out to the threads in chain order (thread t holds chain
+// elements t, t + B, t + 2B, ...), so one global round trip covers the whole chain.
+// Then box q = input(q) + I_dd(box q-1) (prolongation1D_inplace + addMultiLevelValues,
+// MatrixVectorHomogen.h:444-474,997-1019) box by box through LDS, one barrier per box;
+// every box but box 0 (whose values stay where they are read from) is written to the
+// case buffer.  SC = chain elements per thread; the host guarantees SC * B >= the
+// longest chain of the launch.  Grid: item-major, instance fastest.
+template <typename V, int SC>
+__global__ void __launch_bounds__(512) prol_chain_kernel(const ProlItem* __restrict__ items, int d,
+                                                         const V* __restrict__ nodal, int64_t nStride,
+                                                         V* __restrict__ cbuf, int64_t cStride, int64_t maxBox,
+                                                         int batch, const CGState* __restrict__ st,
+                                                         const int* __restrict__ map) {
+    extern __shared__ char smem_raw[];
+    V* prev = reinterpret_cast<V*>(smem_raw);
+    V* cur = prev + maxBox;
+    const int inst = map ? map[blockIdx.x % batch] : (int)(blockIdx.x % batch);
+    if (st && !st[inst].active) return;
+    const LSG_CONST ProlItem& it = cptr(items)[blockIdx.x / batch];
+    const int dd = it.dd, nt = it.nt;
+    const V* nodalI = nodal + inst * nStride;
+    const V* cbufI = cbuf + inst * cStride;
+    V* outBase = cbuf + inst * cStride;
+    int total0 = 1, S = 1, nf0 = 1;
+    for (int i = 0; i < d; ++i) {
+        const int n = (1 << it.T[i]) - 1;
+        if (i < dd) S *= n;
+        if (i == dd) nf0 = n;
+        total0 *= n;
+    }
+    const int B = blockDim.x;
+    V nin[SC];
+    {
+        int off = 0, tq = total0, nq = nf0;
+        for (int q = 0; q < nt; ++q) {
+            const V* in = (((it.from_nodal >> q) & 1) ? nodalI : cbufI) + it.in_off[q];
+#pragma unroll
+            for (int s = 0; s < SC; ++s) {
+                const int p = threadIdx.x + s * B - off;
+                if (p >= 0 && p < tq) nin[s] = in[p];
+            }
+            off += tq;
+            tq = tq / nq * (2 * nq + 1);
+            nq = 2 * nq + 1;
+        }
+    }
+    const float rS = rcpf(S);
+    int off = 0, tq = total0, nq = nf0;
+    for (int q = 0; q < nt; ++q) {
+        const bool keep = q + 1 < nt;
+        if (q == 0) {
+#pragma unroll
+            for (int s = 0; s < SC; ++s) {
+                const int p = threadIdx.x + s * B;
+                if (p < tq) cur[p] = nin[s];
+            }
+        } else {
+            V* out = outBase + it.out_off[q];
+            const int nc = (nq - 1) >> 1;
+            const float rnf = rcpf(nq);
+#pragma unroll
+            for (int s = 0; s < SC; ++s) {
+                const int p = threadIdx.x + s * B - off;
+                if (p >= 0 && p < tq) {
+                    Split sp = split(p, S, rS, nq, rnf);
+                    const V* c = prev + sp.lo + mul24(sp.hi, S * nc);
+                    const int k = sp.k + 1;
+                    const V* cb = c + mul24((k - 1) >> 1, S);
+                    V add;
+                    if (k & 1) {
+                        const int cl = (k - 1) >> 1, cr = (k + 1) >> 1;
+                        V va = (cl >= 1) ? cb[-S] : V(0);
+                        V vb = (cr <= nc) ? cb[0] : V(0);
+                        add = V(0.5) * (va + vb);
+                    } else {
+                        add = cb[0];
+                    }
+                    const V v = nin[s] + add;
+                    out[p] = v;
+                    if (keep) cur[p] = v;
+                }
+            }
+        }
+        if (!keep) break;
+        __syncthreads();
+        V* tmp = prev; prev = cur; cur = tmp;
+        off += tq;
+        tq = tq / nq * (2 * nq + 1);
+        nq = 2 * nq + 1;
+    }
+}
+
 // Prolongation chain along dd with the input of the next box prefetched into
 // registers: box q = input(q) + I_dd(box q-1) (prolongation1D_inplace +
 // addMultiLevelValues, MatrixVectorHomogen.h:444-474,997-1019), written to the case
@@ -1208,9 +1301,34 @@ void launch_nodal(const NodalItem* items, int nitems, int d, const V* x, int64_t
 }
 template <typename V>
 void launch_prol(const ProlItem* items, int nitems, int d, const V* nodal, int64_t nStride, V* cbuf, int64_t cStride,
-                 int64_t maxBox, int batch, const CGState* st, hipStream_t s, const int* map) {
+                 int64_t maxBox, int64_t maxChain, int batch, const CGState* st, hipStream_t s, const int* map) {
     if (nitems == 0) return;
     const size_t lds = 2 * maxBox * sizeof(V);
+    if (sizeof(V) == 4) {
+        // FP32: whole chains in registers, in the narrowest workgroup that holds the
+        // longest chain with at most 24 values per thread (FP64 keeps the per-box
+        // prefetch below: there the registers cost more occupancy than the single
+        // round trip saves, measured)
+        int threads = 128;
+        while (threads < 512 && maxChain > 24 * (int64_t)threads) threads *= 2;
+        const int64_t sc = (maxChain + threads - 1) / threads;
+        const dim3 grid((unsigned)nitems * (unsigned)batch), blk(threads);
+#define LSG_CHAIN_CASE(P)                                                                                     \
+        if (sc <= P) {                                                                                        \
+            hipLaunchKernelGGL((prol_chain_kernel<V, P>), grid, blk, lds, s, items, d, nodal, nStride, cbuf,   \
+                               cStride, maxBox, batch, st, map);                                              \
+            return;                                                                                           \
+        }
+        LSG_CHAIN_CASE(2)
+        LSG_CHAIN_CASE(4)
+        LSG_CHAIN_CASE(6)
+        LSG_CHAIN_CASE(8)
+        LSG_CHAIN_CASE(12)
+        LSG_CHAIN_CASE(16)
+        LSG_CHAIN_CASE(20)
+        LSG_CHAIN_CASE(24)
+#undef LSG_CHAIN_CASE
+    }
     // the narrowest workgroup whose 16 points per thread cover the largest box (a chain
     // is serial, so more, smaller workgroups hide its latency better; measured)
     const int threads = maxBox <= 16 * 128 ? 128 : (maxBox <= 16 * 256 ? 256 : 512);
@@ -1262,7 +1380,9 @@ int launch_apply(const ApplyArgs<V>& a, int nitems, int batch, int threads, size
 template <typename V>
 void launch_sum(const SumItem* items, int nitems, const int64_t* list, int d, const V* part, int64_t pStride, V* y,
                 int64_t yStride, int batch, const CGState* st, hipStream_t s, const int* map) {
-    hipLaunchKernelGGL(sum_kernel<V>, dim3(nitems, batch), dim3(128), 0, s, items, list, d, part, pStride, y, yStride,
+    // one wave per depth block: the blocks are small (2^(|T|_1 - d) values, 64 at most
+    // for d=5 level 7), so wider workgroups would mostly idle (measured)
+    hipLaunchKernelGGL(sum_kernel<V>, dim3(nitems, batch), dim3(64), 0, s, items, list, d, part, pStride, y, yStride,
                        st, map);
 }
 template <typename V>
@@ -1322,7 +1442,8 @@ int cg_partial_blocks(int64_t n) { return (int)gridFor(n); }
 #define LSG_INST(V)                                                                                                  \
     template void launch_nodal<V>(const NodalItem*, int, int, const V*, int64_t, V*, int64_t, int64_t, int,         \
                                   const CGState*, hipStream_t, const int*);                                              \
-    template void launch_prol<V>(const ProlItem*, int, int, const V*, int64_t, V*, int64_t, int64_t, int, const CGState*, \
+    template void launch_prol<V>(const ProlItem*, int, int, const V*, int64_t, V*, int64_t, int64_t, int64_t, int,      \
+                                 const CGState*,                                                                     \
                                  hipStream_t, const int*);                                                           \
     template int launch_apply<V>(const ApplyArgs<V>&, int, int, int, size_t, hipStream_t);                          \
     template void launch_sum<V>(const SumItem*, int, const int64_t*, int, const V*, int64_t, V*, int64_t, int,       \

@@ -117,7 +117,8 @@ void launch_nodal(const NodalItem* items, int nitems, int d, const V* x, int64_t
                   int64_t maxBox, int batch, const CGState* st, hipStream_t s, const int* map = nullptr);
 template <typename V>
 void launch_prol(const ProlItem* items, int nitems, int d, const V* nodal, int64_t nStride, V* cbuf, int64_t cStride,
-                 int64_t maxBox, int batch, const CGState* st, hipStream_t s, const int* map = nullptr);
+                 int64_t maxBox, int64_t maxChain, int batch, const CGState* st, hipStream_t s,
+                 const int* map = nullptr);
 template <typename V>
 int launch_apply(const ApplyArgs<V>& a, int nitems, int batch, int threads, size_t lds, hipStream_t s);
 template <typename V>
