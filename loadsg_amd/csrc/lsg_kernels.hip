@@ -282,24 +282,35 @@ __global__ void __launch_bounds__(128) nodal_kernel(const NodalItem* __restrict_
     const LSG_CONST NodalItem& it = cptr(items)[blockIdx.x];
     const V* xin = x + inst * xStride + it.in_off;
     V* out = nodal + inst * nStride + it.out_off;
-    int cnt = 1;
-    for (int i = 0; i < d; ++i) cnt *= 1 << (it.T[i] - 1);
+    int cnt = 1, lastE = -1;
+    for (int i = 0; i < d; ++i) {
+        cnt *= 1 << (it.T[i] - 1);
+        if (it.T[i] > 1) lastE = i;
+    }
+    // along a dimension of level 1 the 1D map is the identity (one coefficient, centre
+    // weight 1, no neighbours), so only the other dimensions get a pass; the last pass
+    // writes the nodal box
+    if (lastE < 0) {
+        if (threadIdx.x == 0) out[0] = xin[0];
+        return;
+    }
     copy_in(A, xin, cnt);
     __syncthreads();
     V* src = A;
     V* dst = B;
     int Slo = 1;
-    for (int i = 0; i < d; ++i) {
+    for (int i = 0; i <= lastE; ++i) {
         const int t = it.T[i];
+        if (t == 1) continue;
         const int m = 1 << (t - 1), n = (1 << t) - 1;
         const int outCnt = cnt / m * n;
-        expand_dim<V>(src, dst, Slo, m, n, outCnt, t);
+        expand_dim<V>(src, i == lastE ? out : dst, Slo, m, n, outCnt, t);
+        if (i == lastE) break;
         cnt = outCnt;
         Slo *= n;
         __syncthreads();
         V* tmp = src; src = dst; dst = tmp;
     }
-    for (int p = threadIdx.x; p < cnt; p += blockDim.x) out[p] = src[p];
 }
 
 // ---------------------------------------------------------- prolongation
