@@ -969,8 +969,10 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
     std::vector<int> iters(B, 0), outer(B, 0), conv(B, 0);
     std::vector<double> rel, epsH(B);
     std::vector<CGState> st;
-    DevMem epsD;
+    DevMem epsD, rowsD;
     HIPCHK(epsD.alloc((size_t)B * 8));
+    HIPCHK(rowsD.alloc((size_t)B * sizeof(int)));
+    std::vector<int> newly;   // instances converged in this sweep: their r64 rows are zeroed
     std::vector<int> open;    // instances not yet converged
     for (int sweep = 0; sweep <= maxit; ++sweep) {
         // a converged instance's correction is zero from then on (its inner right-hand
@@ -991,10 +993,11 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
             if (any && (rc = trueResidual(op64, fD, xD, Ax, r64, rel))) return rc;
         }
         bool all = true;
+        newly.clear();
         for (int b = 0; b < B; ++b) {
             if (rel[b] != rel[b]) return fail(LSG_ERR_NAN, "NaN residual");
             if (rel[b] <= rel_tol) {
-                if (!conv[b]) HIPCHK(hipMemsetAsync(r64 + (size_t)b * n, 0, n * 8, s));
+                if (!conv[b]) newly.push_back(b);
                 conv[b] = 1;
             } else all = false;
             // inner tolerance: inner_rel_tol, or less when the remaining reduction is smaller.
@@ -1006,6 +1009,10 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
             epsH[b] = std::min(0.5, std::max(target, 0.3 * rel_tol / rr));
         }
         if (all || sweep == maxit) break;
+        if (!newly.empty()) {
+            HIPCHK(hipMemcpyAsync(rowsD.p, newly.data(), newly.size() * sizeof(int), hipMemcpyHostToDevice, s));
+            launch_zero_rows(r64, n, rowsD.as<int>(), (int)newly.size(), s);
+        }
         HIPCHK(hipMemcpyAsync(epsD.p, epsH.data(), (size_t)B * 8, hipMemcpyHostToDevice, s));
         launch_cast_d2f(r64, r32, tot, s);
         HIPCHK(hipMemsetAsync(e32, 0, tot * 4, s));

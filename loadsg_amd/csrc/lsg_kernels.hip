@@ -1294,6 +1294,14 @@ __global__ void __launch_bounds__(256) zero_kernel(void* __restrict__ p, int64_t
         q[i] = 0u;
 }
 
+// rows[j] of a (rows x n) FP64 array set to zero, for j < nrows (grid.y = nrows)
+__global__ void __launch_bounds__(256) zero_rows_kernel(double* __restrict__ a, int64_t n,
+                                                        const int* __restrict__ rows) {
+    double* r = a + rows[blockIdx.y] * n;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        r[i] = 0.0;
+}
+
 // ------------------------------------------------------------ launchers
 static inline unsigned gridFor(int64_t n, int per = 256 * 4) {
     int64_t g = (n + per - 1) / per;
@@ -1447,6 +1455,10 @@ void launch_axpy_correction(double* x, const float* e, int64_t total, hipStream_
 }
 void launch_zero(void* p, int64_t bytes, hipStream_t s) {
     hipLaunchKernelGGL(zero_kernel, dim3(gridFor(bytes / 4)), dim3(256), 0, s, p, bytes);
+}
+void launch_zero_rows(double* a, int64_t n, const int* rows, int nrows, hipStream_t s) {
+    if (nrows <= 0) return;
+    hipLaunchKernelGGL(zero_rows_kernel, dim3(std::min<unsigned>(gridFor(n), 64u), nrows), dim3(256), 0, s, a, n, rows);
 }
 int cg_partial_blocks(int64_t n) { return (int)gridFor(n); }
 
