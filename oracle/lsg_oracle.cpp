@@ -710,6 +710,204 @@ struct Oracle {
     }
 };
 
+
+// ---------------------------------------------------------------------------
+// Basis transformations (BasisTransformations/BasisTransformations.cpp) on grids
+// whose multilevel boxes are complete: for every depth T of the grid, every point of
+// the level-T full grid G_T is stored (regular grids).  Vectors are in slot order.
+// ---------------------------------------------------------------------------
+struct Trafo {
+    const Grid& g;
+    const MultiLevel& ml;
+    std::vector<int> order;    // depths by |T|_1 descending: ListOfDepthOrderedSubgrids from
+                               // gotoEnd() through previous() (sgrid/ListOfDepthOrderedGrids.cc:77-110)
+    Trafo(const Grid& g_, const MultiLevel& ml_) : g(g_), ml(ml_) {
+        order.resize(g.depths.size());
+        for (size_t t = 0; t < order.size(); ++t) order[t] = (int)t;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sum(g.depths[a]) > sum(g.depths[b]); });
+    }
+    int sum(const Depth& T) const { int s = 0; for (int i = 0; i < g.d; ++i) s += T[i]; return s; }
+    bool complete() const {
+        for (size_t t = 0; t < g.depths.size(); ++t) {
+            int64_t c = 1;
+            for (int i = 0; i < g.d; ++i) c *= (int64_t(1) << g.depths[t][i]) - 1;
+            if ((int64_t)ml.slotsG[t].size() != c) return false;
+        }
+        return true;
+    }
+    bool leq(const Depth& A, const Depth& B) const {
+        for (int i = 0; i < g.d; ++i) if (A[i] > B[i]) return false;
+        return true;
+    }
+    // ShiftOperator Links/Rechts (sgoperation/shift.*): the value at the neighbour of the
+    // point's own depth in dimension i, i.e. at its pole; 0 on the boundary.
+    double pole(const std::vector<double>& u, int64_t s, int i, bool left) const {
+        Key k = g.pts[s];
+        int32_t p = left ? poleLeft(k.v[i]) : poleRight(k.v[i]);
+        if (isBoundary1(p)) return 0.0;
+        k.v[i] = p;
+        int64_t q;
+        return g.occupied(k, q) ? u[q] : 0.0;
+    }
+    // CalcHierarchicalBasis (BasisTransformations.cpp:13-37, 390-409): per dimension,
+    // finest subgrids first, u -= (Links(u) + Rechts(u)) / 2; `upTo` restricts it to the
+    // subgrids of depth <= upTo.
+    void hierarchize(std::vector<double>& u, const Depth* upTo = nullptr) const {
+        for (int i = 0; i < g.d; ++i)
+            for (int t : order) {
+                if (upTo && !leq(g.depths[t], *upTo)) continue;
+                for (int64_t s : g.slotsOfDepth[t]) u[s] = u[s] - 0.5 * (pole(u, s, i, true) + pole(u, s, i, false));
+            }
+    }
+    // CalcNodalBasis (BasisTransformations.cpp:174-198): coarsest subgrids first.
+    void dehierarchize(std::vector<double>& u) const {
+        for (int i = 0; i < g.d; ++i)
+            for (auto it = order.rbegin(); it != order.rend(); ++it)
+                for (int64_t s : g.slotsOfDepth[*it]) u[s] = u[s] + 0.5 * (pole(u, s, i, true) + pole(u, s, i, false));
+    }
+    // PrewaveletMatrixHomogen::create (sgoperation/matrix.cpp:74-139) for a full line of
+    // level t (n = 2^t - 1 positions, 1-based): column j is the basis function at position
+    // j written in level-t nodal values -- the prewavelet (0.1, -0.6, 1, -0.6, 0.1; 0.9 next
+    // to the boundary) at the positions of depth t, the level-(t-1) hat (0.5, 1, 0.5) at
+    // the coarser ones.  Row-major n x n.
+    static std::vector<double> lineMatrix(int t) {
+        const int n = (1 << t) - 1;
+        std::vector<double> M((size_t)n * n, 0.0);
+        auto set = [&](int r, int c, double v) { if (r >= 1 && r <= n) M[(size_t)(r - 1) * n + (c - 1)] = v; };
+        for (int j = 1; j <= n; ++j) {
+            if (j & 1) {
+                if (t == 1) { set(j, j, 1.0); continue; }
+                if (j == 1) { set(j, j, 0.9); set(j + 1, j, -0.6); set(j + 2, j, 0.1); }
+                else if (j == n) { set(j, j, 0.9); set(j - 1, j, -0.6); set(j - 2, j, 0.1); }
+                else { set(j, j, 1.0); set(j - 1, j, -0.6); set(j - 2, j, 0.1); set(j + 1, j, -0.6); set(j + 2, j, 0.1); }
+            } else {
+                set(j, j, 1.0); set(j - 1, j, 0.5); set(j + 1, j, 0.5);
+            }
+        }
+        return M;
+    }
+    // Solve M x = b by Gaussian elimination with partial pivoting (the reference uses a
+    // banded LR without pivoting, PrewaveletMatrixHomogen::solve, matrix.cpp:141-155;
+    // the solution is the same up to rounding).
+    static void solveDense(std::vector<double> M, std::vector<double>& b) {
+        const int n = (int)b.size();
+        for (int c = 0; c < n; ++c) {
+            int p = c;
+            for (int r = c + 1; r < n; ++r) if (std::fabs(M[(size_t)r * n + c]) > std::fabs(M[(size_t)p * n + c])) p = r;
+            if (p != c) {
+                for (int k = 0; k < n; ++k) std::swap(M[(size_t)c * n + k], M[(size_t)p * n + k]);
+                std::swap(b[c], b[p]);
+            }
+            for (int r = c + 1; r < n; ++r) {
+                double f = M[(size_t)r * n + c] / M[(size_t)c * n + c];
+                if (f == 0.0) continue;
+                for (int k = c; k < n; ++k) M[(size_t)r * n + k] -= f * M[(size_t)c * n + k];
+                b[r] -= f * b[c];
+            }
+        }
+        for (int r = n - 1; r >= 0; --r) {
+            double s = b[r];
+            for (int k = r + 1; k < n; ++k) s -= M[(size_t)r * n + k] * b[k];
+            b[r] = s / M[(size_t)r * n + r];
+        }
+    }
+    // slot of the point at level-T box position k[] (1-based per dimension)
+    int64_t slotAt(const Depth& T, const int* k) const {
+        Key key{};
+        for (int i = 0; i < g.d; ++i) {
+            int tz = __builtin_ctz((unsigned)k[i]);
+            key.v[i] = (1 << (T[i] - tz)) + (k[i] >> (tz + 1));
+        }
+        int64_t s = -1;
+        g.occupied(key, s);
+        return s;
+    }
+    // SolvePrewavelet / GetPrewaveletCoefficients (BasisTransformations.cpp:100-172,
+    // 468-484): along every dimension, every line of G_T is replaced by the solution of
+    // M x = line; the values left at the points of depth T are their prewavelet
+    // coefficients.
+    void solvePrewavelet(const Depth& T, const std::vector<double>& u, std::vector<double>& prew) const {
+        const int d = g.d;
+        int n[MAXD];
+        int64_t tot = 1;
+        for (int i = 0; i < d; ++i) { n[i] = (1 << T[i]) - 1; tot *= n[i]; }
+        std::vector<int64_t> slot(tot);
+        std::vector<double> box(tot);
+        for (int64_t p = 0; p < tot; ++p) {
+            int k[MAXD];
+            int64_t r = p;
+            for (int i = 0; i < d; ++i) { k[i] = int(r % n[i]) + 1; r /= n[i]; }
+            slot[p] = slotAt(T, k);
+            box[p] = u[slot[p]];
+        }
+        for (int i = 0; i < d; ++i) {
+            if (n[i] == 1) continue;
+            auto M = lineMatrix(T[i]);
+            int64_t stride = 1;
+            for (int j = 0; j < i; ++j) stride *= n[j];
+            std::vector<double> line(n[i]);
+            for (int64_t p = 0; p < tot; ++p) {
+                if ((p / stride) % n[i] != 0) continue;   // line starts
+                for (int q = 0; q < n[i]; ++q) line[q] = box[p + q * stride];
+                solveDense(M, line);
+                for (int q = 0; q < n[i]; ++q) box[p + q * stride] = line[q];
+            }
+        }
+        for (int64_t p = 0; p < tot; ++p)
+            if (g.depthOf[slot[p]] == T) prew[slot[p]] = box[p];
+    }
+    // CalcHierbyPrew (BasisTransformations.cpp:321-353): the depth-T prewavelets of prew
+    // as nodal values (the 5^d stencil of nextFive) hierarchized on the subgrids <= T.
+    void hierByPrew(const Depth& T, int t, const std::vector<double>& prew, std::vector<double>& hier) const {
+        hier.assign(g.pts.size(), 0.0);
+        const int d = g.d;
+        for (int64_t s : g.slotsOfDepth[t]) {
+            Entry1 e[MAXD][5];
+            int ne[MAXD];
+            for (int i = 0; i < d; ++i) ne[i] = prewavelet1D(g.pts[s].v[i], T[i], e[i]);
+            int c[MAXD] = {};
+            while (true) {
+                Key k{};
+                double w = prew[s];
+                for (int i = 0; i < d; ++i) { k.v[i] = e[i][c[i]].idx; w *= e[i][c[i]].c; }
+                int64_t q;
+                if (g.occupied(k, q)) hier[q] += w;
+                int i = 0;
+                while (i < d && ++c[i] == ne[i]) c[i++] = 0;
+                if (i == d) break;
+            }
+        }
+        hierarchize(hier, &T);
+    }
+    // calcPrewByNodal (BasisTransformations.cpp:486-545)
+    void prewByNodal(const std::vector<double>& u0, std::vector<double>& prew) const {
+        std::vector<double> hier = u0, u = u0, hnew;
+        hierarchize(hier);
+        prew.assign(g.pts.size(), 0.0);
+        for (int t : order) {
+            const Depth& T = g.depths[t];
+            solvePrewavelet(T, u, prew);
+            hierByPrew(T, t, prew, hnew);
+            for (size_t s = 0; s < hier.size(); ++s)
+                if (leq(g.depthOf[s], T)) hier[s] = hier[s] - hnew[s];
+            u = hier;
+            dehierarchize(u);
+        }
+    }
+    // calcNodalByPrew (BasisTransformations.cpp:670-707)
+    void nodalByPrew(const std::vector<double>& prew, std::vector<double>& u) const {
+        std::vector<double> hier(g.pts.size(), 0.0), hnew;
+        for (int t : order) {
+            const Depth& T = g.depths[t];
+            hierByPrew(T, t, prew, hnew);
+            for (size_t s = 0; s < hier.size(); ++s)
+                if (leq(g.depthOf[s], T)) hier[s] = hier[s] + hnew[s];
+        }
+        u = hier;
+        dehierarchize(u);
+    }
+};
+
 }  // namespace orc
 
 // ---------------------------------------------------------------------------
@@ -808,6 +1006,25 @@ int orc_cg(void* h, double eps, const double* f, double* x, int maxit, double* d
     std::memcpy(x, xv.data(), sizeof(double) * o->N());
     if (delta_hist) for (int k = 0; k < hist_cap && k < (int)hist.size(); ++k) delta_hist[k] = hist[k];
     return it;
+}
+
+// Basis transformations (complete grids only; returns -1 otherwise).
+// what: 0 hierarchize, 1 dehierarchize, 2 prewavelets from nodal values,
+// 3 nodal values from prewavelets.
+int orc_transform(void* h, int what, const double* in, double* out) {
+    auto* o = (orc::Oracle*)h;
+    orc::Trafo tr(o->g, o->ml);
+    if (!tr.complete()) return -1;
+    std::vector<double> a(in, in + o->N()), b;
+    switch (what) {
+        case 0: tr.hierarchize(a); b = a; break;
+        case 1: tr.dehierarchize(a); b = a; break;
+        case 2: tr.prewByNodal(a, b); break;
+        case 3: tr.nodalByPrew(a, b); break;
+        default: return -2;
+    }
+    std::memcpy(out, b.data(), sizeof(double) * o->N());
+    return 0;
 }
 
 }  // extern "C"

@@ -18,7 +18,7 @@
 //   ref_harness adaptive <startlevel> <radius> <maxlevel> <coef> <outdir> [what...]
 //   ref_harness timemv   <level> <coef> <reps>            (timing sample, prints JSON)
 // coef: const:<a0>:<c0> | cfg2 | cfg3 | expsin:<xi1>,<xi2>,...
-// what: matvec precond cg   (default: all)
+// what: matvec precond cg transform   (default: all)
 #include "source/sparseEXPDE.h"
 
 #include <string>
@@ -227,6 +227,33 @@ void runAll(AdaptiveSparseGrid& grid, const CoefSpec& cs, const std::string& out
         writeVec(out, "cg_var_x", sol2, n);
         its = it;
         writeNpy(out + "/cg_var_iters.npy", "<i4", &its, 4, {1});
+    }
+    if (want(what, "transform")) {
+        // Basis transformations (BasisTransformations.cpp) and the norms built on them
+        // (applications/H1_Norm.cpp:8-78) for nodal values u = prod sin(pi x_i) + x_1
+        // at the active points.
+        VectorSparseG u(grid), pr(grid), nod(grid), hier(grid), xin(grid);
+        u = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            if (!act[i]) continue;
+            IndexDimension I = grid.getIndexOfTable(i);
+            u.setValue((unsigned long)i, sinFunc(I) + I.coordinate(0));
+        }
+        writeVec(out, "tr_u", u, n);
+        hier = u;
+        CalcHierarchicalBasis(hier);
+        writeVec(out, "tr_hier", hier, n);
+        pr = 0.0;
+        calcPrewByNodal(pr, u);
+        writeVec(out, "tr_prew", pr, n);
+        xin = x;
+        nod = 0.0;
+        calcNodalByPrew(xin, nod);
+        writeVec(out, "tr_nodal_of_x", nod, n);
+        H1 h1(grid, mgrid);
+        L2 l2(grid, mgrid);
+        double nrm[2] = {h1.getValue(u), l2.getValue(u)};
+        writeNpy(out + "/tr_norms.npy", "<f8", nrm, 8, {2});
     }
 }
 

@@ -8,6 +8,9 @@ Each case directory holds .npy arrays written by the harness:
   diag_poisson / diag_var (Preconditioning<S>::precond3),
   f_prew / f_rhs (calcPrewByNodal and the driver's f_h = M prew(f_I)),
   cg_poisson_x / cg_poisson_iters / cg_var_x / cg_var_iters (CG::solveHomogen, eps 1e-10).
+  tr_u (nodal values prod sin(pi x_i) + x_1), tr_hier (CalcHierarchicalBasis), tr_prew
+  (calcPrewByNodal), tr_nodal_of_x (calcNodalByPrew of x), tr_norms (H1::getValue,
+  L2::getValue of tr_u).
 The reference preconditioner is run with OMP_NUM_THREADS=1 so that its stale
 hanging-node work-vector values (cg_method.h:356-433 resets v with an
 active-only `v = 0.0`) are deterministic.

@@ -38,6 +38,7 @@ def lib():
         L.orc_precond.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         L.orc_cg.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                              ctypes.c_void_p, ctypes.c_int]
+        L.orc_transform.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -109,3 +110,15 @@ class Oracle:
         hist = np.zeros(maxit + 2)
         it = lib().orc_cg(self.h, eps, _p(f), _p(x), maxit, _p(hist), hist.size)
         return x, it, hist[: it + 1]
+
+    # Basis transformations (complete grids): 0 hierarchize, 1 dehierarchize,
+    # 2 prewavelets from nodal values, 3 nodal values from prewavelets.
+    HIER, DEHIER, PREW_BY_NODAL, NODAL_BY_PREW = 0, 1, 2, 3
+
+    def transform(self, what, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        out = np.zeros(self.n)
+        rc = lib().orc_transform(self.h, what, _p(v), _p(out))
+        if rc:
+            raise ValueError("oracle transform: grid boxes are not complete (rc %d)" % rc)
+        return out
