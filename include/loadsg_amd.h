@@ -120,6 +120,25 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
 /* Load vector b_p = int f psi_p of f = sum_j eta_j prod_i sin(m_j pi x_i) (exact 1D
  * quadrature of each factor), for every instance: eta[inst*nterms + j], m[j]. */
 int lsg_rhs_sines(lsg_op* op, int nterms, const double* eta, const int32_t* m, void* b, int layout);
+/* ------------------------------------------------- basis transformations */
+/* FP64 operators on grids with complete depth boxes (regular grids); LSG_ERR_UNSUPPORTED
+ * otherwise.  Vectors as in lsg_matvec; in and out must be different arrays.
+ * Hierarchical surpluses from nodal values: CalcHierarchicalBasis(u)
+ * (BasisTransformations/BasisTransformations.cpp:390-409). */
+int lsg_hierarchize(lsg_op* op, const void* u, void* hier, int layout);
+/* Nodal values from hierarchical surpluses: CalcNodalBasis(u) (BasisTransformations.cpp:174-198). */
+int lsg_dehierarchize(lsg_op* op, const void* hier, void* u, int layout);
+/* Prewavelet coefficients of the sparse-grid interpolant of nodal values u:
+ * calcPrewByNodal(prew, u) (BasisTransformations.cpp:486-545). */
+int lsg_prew_by_nodal(lsg_op* op, const void* u, void* prew, int layout);
+/* Nodal values of the prewavelet expansion prew: calcNodalByPrew(prew, u)
+ * (BasisTransformations.cpp:670-707). */
+int lsg_nodal_by_prew(lsg_op* op, const void* prew, void* u, int layout);
+/* out[b] = sqrt(v^T A v), v = prewavelets of the nodal values u of instance b, A the
+ * operator: H1::getValue with a Poisson operator, L2::getValue with a mass operator
+ * (applications/H1_Norm.cpp:8-78).  out holds lsg_op_batch() values. */
+int lsg_energy_norm(lsg_op* op, const void* u, double* out, int layout);
+
 /* Device timing of the last lsg_matvec: out[0]=total ms, out[1]=apply-kernel ms,
  * out[2]=number of apply launches. */
 int lsg_last_timing(const lsg_op* op, double* out, int nout);

@@ -78,6 +78,11 @@ def lib():
             "lsg_ir_solve": (i32, [vp, vp, vp, vp, dbl, dbl, i32, vp, i32]),
             "lsg_rhs_sines": (i32, [vp, i32, vp, vp, vp, i32]),
             "lsg_last_timing": (i32, [vp, vp, i32]),
+            "lsg_hierarchize": (i32, [vp, vp, vp, i32]),
+            "lsg_dehierarchize": (i32, [vp, vp, vp, i32]),
+            "lsg_prew_by_nodal": (i32, [vp, vp, vp, i32]),
+            "lsg_nodal_by_prew": (i32, [vp, vp, vp, i32]),
+            "lsg_energy_norm": (i32, [vp, vp, vp, i32]),
             "lsg_op_set_block": (i32, [vp, i32, i32]),
             "lsg_op_destroy": (i32, [vp]),
             "lsg_mlp_create": (i32, [i32, i32, i32, i64, ctypes.c_uint64, ctypes.POINTER(vp)]),
@@ -267,10 +272,91 @@ class MatrixVectorHomogen:
         _check(lib().lsg_rhs_sines(self._h, m.size, _ptr(eta), _ptr(m), _ptr(b), LSG_HOST_SLOT))
         return b
 
+    # ---- basis transformations (FP64 operators, grids with complete depth boxes)
+    def _transform(self, fn, v, out):
+        if hasattr(v, "data_ptr"):
+            _sync_torch(v, out)
+            _check(fn(self._h, _ptr(v), _ptr(out), LSG_DEVICE))
+            return out
+        x = np.ascontiguousarray(v, dtype=np.float64)
+        y = np.zeros_like(x) if out is None else out
+        _check(fn(self._h, _ptr(x), _ptr(y), LSG_HOST_SLOT))
+        return y
+
+    def hierarchize(self, u, out=None):
+        """CalcHierarchicalBasis (BasisTransformations.cpp:390-409)."""
+        return self._transform(lib().lsg_hierarchize, u, out)
+
+    def dehierarchize(self, hier, out=None):
+        """CalcNodalBasis (BasisTransformations.cpp:174-198)."""
+        return self._transform(lib().lsg_dehierarchize, hier, out)
+
+    def prew_by_nodal(self, u, out=None):
+        """calcPrewByNodal (BasisTransformations.cpp:486-545)."""
+        return self._transform(lib().lsg_prew_by_nodal, u, out)
+
+    def nodal_by_prew(self, prew, out=None):
+        """calcNodalByPrew (BasisTransformations.cpp:670-707)."""
+        return self._transform(lib().lsg_nodal_by_prew, prew, out)
+
+    def energy_norm(self, u):
+        """sqrt(v^T A v), v = prewavelets of nodal values u, per instance
+        (H1::getValue / L2::getValue, applications/H1_Norm.cpp:8-78)."""
+        out = np.zeros(self.batch)
+        if hasattr(u, "data_ptr"):
+            _sync_torch(u)
+            _check(lib().lsg_energy_norm(self._h, _ptr(u), _ptr(out), LSG_DEVICE))
+        else:
+            x = np.ascontiguousarray(u, dtype=np.float64)
+            _check(lib().lsg_energy_norm(self._h, _ptr(x), _ptr(out), LSG_HOST_SLOT))
+        return out if self.batch > 1 else float(out[0])
+
     def last_timing(self):
         out = np.zeros(3)
         _check(lib().lsg_last_timing(self._h, _ptr(out), 3))
         return {"total_ms": out[0], "apply_ms": out[1], "apply_launches": int(out[2])}
+
+
+def CalcHierarchicalBasis(u, op):
+    """Hierarchical surpluses of nodal values u (BasisTransformations.cpp:390-409); op is
+    an FP64 MatrixVectorHomogen on u's grid (it holds the device context)."""
+    return op.hierarchize(u)
+
+
+def CalcNodalBasis(hier, op):
+    """Nodal values from hierarchical surpluses (BasisTransformations.cpp:174-198)."""
+    return op.dehierarchize(hier)
+
+
+def calcPrewByNodal(u, op):
+    """Prewavelet coefficients of nodal values u (BasisTransformations.cpp:486-545)."""
+    return op.prew_by_nodal(u)
+
+
+def calcNodalByPrew(prew, op):
+    """Nodal values of a prewavelet expansion (BasisTransformations.cpp:670-707)."""
+    return op.nodal_by_prew(prew)
+
+
+class H1:
+    """H1 energy norm of nodal values: H1::getValue (applications/H1_Norm.cpp:8-41)."""
+
+    def __init__(self, grid, batch=1):
+        self.op = MatrixVectorHomogen(grid, Poisson(), 64, batch)
+
+    def getValue(self, u):
+        return self.op.energy_norm(u)
+
+
+class L2:
+    """L2 norm of the sparse-grid interpolant of nodal values: L2::getValue
+    (applications/H1_Norm.cpp:45-78), the mass-matrix energy norm."""
+
+    def __init__(self, grid, batch=1):
+        self.op = MatrixVectorHomogen(grid, HelmHoltz(), 64, batch)
+
+    def getValue(self, u):
+        return self.op.energy_norm(u)
 
 
 class CG:

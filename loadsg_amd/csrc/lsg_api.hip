@@ -111,6 +111,15 @@ void coefTables(const lsg_coef& c, int d, int maxLevel, int64_t tabLen, std::vec
 
 }  // namespace
 
+namespace {
+// Device copies of the basis-transformation tables and their work vectors.
+struct TrDev {
+    TrTables host;
+    DevMem items, gidx, list, rowDev, rowPtr, col, fac;
+    DevMem hier, work, work2, scratch, norms;
+};
+}  // namespace
+
 struct lsg_op {
     lsg_grid* grid = nullptr;
     int prec = 64;
@@ -163,6 +172,7 @@ struct lsg_op {
     int chunkK = 0;
     bool chunkX64 = false;
     bool capturing = false;   // matvec launches record no timing events while captured
+    std::unique_ptr<TrDev> tr;   // basis transformations (built on first use)
     int64_t n() const { return grid->g.n; }
 };
 
@@ -578,6 +588,102 @@ int trueResidual(lsg_op* op64, const double* f, const double* x, double* tmpAx, 
         rel[b] = ff > 0 ? std::sqrt(rr / ff) : std::sqrt(rr);
     }
     return LSG_OK;
+}
+
+// Tables and work vectors of the basis transformations (FP64 operators on the dense path).
+int ensureTransform(lsg_op* op) {
+    if (op->tr) return LSG_OK;
+    if (op->prec != 64) return fail(LSG_ERR_UNSUPPORTED, "basis transformations run on an FP64 operator");
+    const Grid& g = op->grid->g;
+    auto tr = std::make_unique<TrDev>();
+    std::string err;
+    if (int rc = buildTransform(g, tr->host, err)) return fail(rc, err);
+    const TrTables& h = tr->host;
+    if (h.maxBox * 8 > 65536) return fail(LSG_ERR_UNSUPPORTED, "depth box too large for the transform workgroup");
+    auto up = [&](DevMem& m, const void* src, size_t bytes) -> hipError_t {
+        hipError_t e = m.alloc(bytes);
+        if (e != hipSuccess) return e;
+        return bytes ? hipMemcpy(m.p, src, bytes, hipMemcpyHostToDevice) : hipSuccess;
+    };
+    HIPCHK(up(tr->items, h.items.data(), h.items.size() * sizeof(TrItem)));
+    HIPCHK(up(tr->gidx, h.gidx.data(), h.gidx.size() * 4));
+    HIPCHK(up(tr->list, h.levelList.data(), h.levelList.size() * 4));
+    HIPCHK(up(tr->rowDev, h.rowDev.data(), h.rowDev.size() * 4));
+    HIPCHK(up(tr->rowPtr, h.rowPtr.data(), h.rowPtr.size() * 8));
+    HIPCHK(up(tr->col, h.col.data(), h.col.size() * 4));
+    HIPCHK(up(tr->fac, h.fac.data(), h.fac.size() * 8));
+    const size_t nb = (size_t)g.n * op->batch * 8;
+    HIPCHK(tr->hier.alloc(nb));
+    HIPCHK(tr->work.alloc(nb));
+    HIPCHK(tr->work2.alloc(nb));
+    HIPCHK(tr->scratch.alloc((size_t)g.nml * op->batch * 8));
+    HIPCHK(tr->norms.alloc((size_t)op->batch * 8));
+    op->tr = std::move(tr);
+    return LSG_OK;
+}
+
+enum { TR_HIER = 0, TR_DEHIER = 1, TR_LEVEL = 2, TR_SYNTH = 3 };
+
+void trBox(lsg_op* op, int mode, const int32_t* list, int count, const double* in, double* out) {
+    const Grid& g = op->grid->g;
+    TrDev& t = *op->tr;
+    launch_tr_box(mode, t.items.as<TrItem>(), list, count, t.gidx.as<int32_t>(), t.fac.as<double>(), t.host.facTot,
+                  t.host.maxBox, in, out, t.scratch.as<double>(), g.n, g.nml, op->batch, op->stream);
+}
+
+void trGather(lsg_op* op, size_t group, double* v, int sub) {
+    const Grid& g = op->grid->g;
+    TrDev& t = *op->tr;
+    const auto& gr = t.host.groups[group];
+    launch_tr_gather(t.rowDev.as<int32_t>() + gr[0], t.rowPtr.as<int64_t>() + gr[0], t.col.as<int32_t>(), gr[1],
+                     t.scratch.as<double>(), v, g.n, g.nml, sub, op->batch, op->stream);
+}
+
+// Device vectors (all instances, device order): what 0 hierarchize, 1 dehierarchize,
+// 2 prewavelets from nodal values, 3 nodal values from prewavelets.
+int transformDev(lsg_op* op, int what, const double* in, double* out) {
+    int rc = ensureTransform(op);
+    if (rc) return rc;
+    TrDev& t = *op->tr;
+    const int nd = (int)t.host.items.size();
+    switch (what) {
+        case 0: trBox(op, TR_HIER, nullptr, nd, in, out); break;
+        case 1: trBox(op, TR_DEHIER, nullptr, nd, in, out); break;
+        case 2: {   // calcPrewByNodal (BasisTransformations.cpp:486-545)
+            double* hier = t.hier.as<double>();
+            trBox(op, TR_HIER, nullptr, nd, in, hier);
+            for (size_t l = 0; l < t.host.levels.size(); ++l) {
+                trBox(op, TR_LEVEL, t.list.as<int32_t>() + t.host.levels[l][0], t.host.levels[l][1], hier, out);
+                if (l + 1 < t.host.levels.size()) trGather(op, l, hier, 1);
+            }
+            break;
+        }
+        case 3: {   // calcNodalByPrew (BasisTransformations.cpp:670-707)
+            double* hier = t.hier.as<double>();
+            trBox(op, TR_SYNTH, nullptr, nd, in, nullptr);
+            trGather(op, t.host.groups.size() - 1, hier, 0);
+            trBox(op, TR_DEHIER, nullptr, nd, hier, out);
+            break;
+        }
+        default: return fail(LSG_ERR_ARG, "unknown transform");
+    }
+    HIPCHK(hipGetLastError());
+    return LSG_OK;
+}
+
+int transformAbi(lsg_op* op, int what, const void* in, void* out, int layout) {
+    if (!op || !in || !out) return fail(LSG_ERR_ARG, "null argument");
+    int rc = ensureTransform(op);
+    if (rc) return rc;
+    if (layout == LSG_DEVICE) {
+        if (in == out) return fail(LSG_ERR_ARG, "transform input and output must differ");
+        if ((rc = transformDev(op, what, (const double*)in, (double*)out))) return rc;
+        HIPCHK(hipStreamSynchronize(op->stream));
+        return LSG_OK;
+    }
+    if ((rc = upload(op, (const double*)in, op->tr->work.p))) return rc;
+    if ((rc = transformDev(op, what, op->tr->work.as<double>(), op->tr->work2.as<double>()))) return rc;
+    return download(op, op->tr->work2.p, (double*)out);
 }
 
 }  // namespace
@@ -1061,6 +1167,35 @@ int lsg_rhs_sines(lsg_op* op, int nterms, const double* eta, const int32_t* m, v
     HIPCHK(hipStreamSynchronize(op->stream));
     if (layout == LSG_DEVICE) return LSG_OK;
     return download(op, op->ys.p, (double*)b);
+}
+
+int lsg_hierarchize(lsg_op* op, const void* u, void* hier, int layout) { return transformAbi(op, 0, u, hier, layout); }
+
+int lsg_dehierarchize(lsg_op* op, const void* hier, void* u, int layout) { return transformAbi(op, 1, hier, u, layout); }
+
+int lsg_prew_by_nodal(lsg_op* op, const void* u, void* prew, int layout) { return transformAbi(op, 2, u, prew, layout); }
+
+int lsg_nodal_by_prew(lsg_op* op, const void* prew, void* u, int layout) { return transformAbi(op, 3, prew, u, layout); }
+
+int lsg_energy_norm(lsg_op* op, const void* u, double* out, int layout) {
+    if (!op || !u || !out) return fail(LSG_ERR_ARG, "null argument");
+    int rc = ensureTransform(op);
+    if (rc) return rc;
+    if ((rc = ensureThreads(op))) return rc;
+    TrDev& t = *op->tr;
+    const double* in = (const double*)u;
+    if (layout != LSG_DEVICE) {
+        if ((rc = upload(op, (const double*)u, t.work2.p))) return rc;
+        in = t.work2.as<double>();
+    }
+    // v = prew(u) (H1_Norm.cpp:24 calcPrewByNodal), Av (:27), sqrt(v . Av) over the points (:29-37)
+    if ((rc = transformDev(op, 2, in, t.work.as<double>()))) return rc;
+    if ((rc = matvecDev(op, t.work.p, op->ys.p, nullptr))) return rc;
+    launch_tr_energy(t.work.as<double>(), op->ys.as<double>(), op->n(), t.norms.as<double>(), op->batch, op->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, t.norms.p, (size_t)op->batch * 8, hipMemcpyDeviceToHost, op->stream));
+    HIPCHK(hipStreamSynchronize(op->stream));
+    return LSG_OK;
 }
 
 int lsg_last_timing(const lsg_op* op, double* out, int nout) {

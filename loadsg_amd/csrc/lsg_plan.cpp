@@ -819,4 +819,139 @@ int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid&
     return buildPlan(g, err);
 }
 
+
+// ---------------------------------------------------------------------------
+// Basis transformations (BasisTransformations/BasisTransformations.cpp)
+// ---------------------------------------------------------------------------
+namespace {
+
+// LU factors without pivoting of the level-t line matrix of PrewaveletMatrixHomogen::create
+// (sgoperation/matrix.cpp:74-139): column j (1-based) is the basis function at line
+// position j in level-t nodal values, the prewavelet (0.1, -0.6, 1, -0.6, 0.1; 0.9 next
+// to the boundary, 1 at level 1) at the odd positions and the level-(t-1) hat
+// (0.5, 1, 0.5) at the even ones.  The matrix is pentadiagonal and its pivots lie in
+// [0.9, 4/3], as in the reference's own unpivoted LR (matrix.cpp:141-182).
+void lineFactors(int t, double* l1, double* l2, double* u0, double* u1, double* u2) {
+    const int n = (1 << t) - 1;
+    std::vector<std::array<double, 5>> B(n + 2);
+    for (auto& r : B) r.fill(0.0);
+    auto at = [&](int r, int c) -> double& { return B[r][c - r + 2]; };
+    for (int j = 1; j <= n; ++j) {
+        auto set = [&](int r, double v) { if (r >= 1 && r <= n) at(r, j) = v; };
+        if (j & 1) {
+            if (t == 1) { set(j, 1.0); continue; }
+            set(j, (j == 1 || j == n) ? 0.9 : 1.0);
+            set(j - 1, -0.6); set(j + 1, -0.6); set(j - 2, 0.1); set(j + 2, 0.1);
+        } else {
+            set(j, 1.0); set(j - 1, 0.5); set(j + 1, 0.5);
+        }
+    }
+    for (int c = 1; c <= n; ++c) {
+        l1[c - 1] = 0.0;
+        l2[c - 1] = 0.0;
+    }
+    for (int c = 1; c <= n; ++c) {
+        const double p = at(c, c);
+        for (int r = c + 1; r <= std::min(n, c + 2); ++r) {
+            const double f = at(r, c) / p;
+            (r == c + 1 ? l1 : l2)[r - 1] = f;
+            for (int k = c; k <= std::min(n, c + 2); ++k) at(r, k) -= f * at(c, k);
+        }
+        u0[c - 1] = p;
+        u1[c - 1] = c + 1 <= n ? at(c, c + 1) : 0.0;
+        u2[c - 1] = c + 2 <= n ? at(c, c + 2) : 0.0;
+    }
+}
+
+}  // namespace
+
+int buildTransform(const Grid& g, TrTables& tr, std::string& err) {
+    if (g.sparse) { err = "basis transformations need complete depth boxes (dense path)"; return LSG_ERR_UNSUPPORTED; }
+    const int d = g.d;
+    const int nd = (int)g.depths.size();
+    tr.items.assign(nd, TrItem{});
+    tr.gidx.assign((size_t)g.nml, 0);
+    tr.maxBox = 0;
+    for (int t = 0; t < nd; ++t) {
+        TrItem& it = tr.items[t];
+        it.d = d;
+        int32_t S = 1;
+        for (int i = 0; i < d; ++i) {
+            it.T[i] = g.depths[t][i];
+            it.n[i] = (1 << it.T[i]) - 1;
+            it.S[i] = S;
+            S *= it.n[i];
+        }
+        it.total = S;
+        it.box_off = g.boxOff[t];
+        tr.maxBox = std::max<int64_t>(tr.maxBox, S);
+        for (int32_t p = 0; p < S; ++p) {
+            Depth Q{};
+            int64_t loc = 0, mul = 1;
+            int32_t r = p;
+            for (int i = 0; i < d; ++i) {
+                const int k = r % it.n[i] + 1;
+                r /= it.n[i];
+                const int tz = __builtin_ctz((unsigned)k);
+                Q[i] = it.T[i] - tz;
+                loc += int64_t(k >> (tz + 1)) * mul;
+                mul *= int64_t(1) << (Q[i] - 1);
+            }
+            auto f = g.depthIdx.find(Q);
+            if (f == g.depthIdx.end()) { err = "depth box not complete"; return LSG_ERR_UNSUPPORTED; }
+            tr.gidx[it.box_off + p] = (int32_t)(g.blkOff[f->second] + loc);
+        }
+    }
+    // depths grouped by |T|_1, finest first
+    std::map<int, std::vector<int32_t>, std::greater<int>> bySum;
+    for (int t = 0; t < nd; ++t) {
+        int s = 0;
+        for (int i = 0; i < d; ++i) s += g.depths[t][i];
+        bySum[s].push_back(t);
+    }
+    tr.levelList.clear();
+    tr.levels.clear();
+    tr.rowDev.clear();
+    tr.col.clear();
+    tr.rowPtr.assign(1, 0);
+    tr.groups.clear();
+    auto addGroup = [&](const std::vector<int32_t>& ts) {
+        std::vector<std::pair<int32_t, int32_t>> pairs;   // (device index, scratch position)
+        for (int32_t t : ts) {
+            const TrItem& it = tr.items[t];
+            for (int32_t p = 0; p < it.total; ++p)
+                pairs.push_back({tr.gidx[it.box_off + p], (int32_t)(it.box_off + p)});
+        }
+        std::stable_sort(pairs.begin(), pairs.end(),
+                         [](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) { return a.first < b.first; });
+        const int64_t first = (int64_t)tr.rowDev.size();
+        for (size_t k = 0; k < pairs.size(); ++k) {
+            if (k == 0 || pairs[k].first != pairs[k - 1].first) {
+                if (k) tr.rowPtr.push_back((int64_t)tr.col.size());
+                tr.rowDev.push_back(pairs[k].first);
+            }
+            tr.col.push_back(pairs[k].second);
+        }
+        if (!pairs.empty()) tr.rowPtr.push_back((int64_t)tr.col.size());
+        tr.groups.push_back({first, (int64_t)tr.rowDev.size() - first});
+    };
+    std::vector<int32_t> all;
+    for (auto& kv : bySum) {
+        tr.levels.push_back({(int32_t)tr.levelList.size(), (int32_t)kv.second.size()});
+        tr.levelList.insert(tr.levelList.end(), kv.second.begin(), kv.second.end());
+        addGroup(kv.second);
+        all.insert(all.end(), kv.second.begin(), kv.second.end());
+    }
+    std::sort(all.begin(), all.end());
+    addGroup(all);
+    // line-matrix factors
+    tr.facTot = facBase(g.maxLevel + 1);
+    tr.fac.assign((size_t)5 * tr.facTot, 0.0);
+    for (int t = 1; t <= g.maxLevel; ++t) {
+        double* b = tr.fac.data() + facBase(t);
+        lineFactors(t, b, b + tr.facTot, b + 2 * tr.facTot, b + 3 * tr.facTot, b + 4 * tr.facTot);
+    }
+    return LSG_OK;
+}
+
 }  // namespace lsg

@@ -173,6 +173,38 @@ struct Grid {
     Plan plan;
 };
 
+// Basis transformations on the dense path (every box G_T complete): one item per depth.
+struct TrItem {
+    int32_t T[MAXD];
+    int32_t n[MAXD];      // box extents 2^T_i - 1
+    int32_t S[MAXD];      // box strides (dimension 0 fastest)
+    int32_t d;
+    int32_t total;        // |G_T|
+    int64_t box_off;      // first entry of the box in the gather table and the scratch
+};
+
+struct TrTables {
+    std::vector<TrItem> items;                 // per depth, depth list order
+    std::vector<int32_t> gidx;                 // per box entry: device index of its point
+    std::vector<int32_t> levelList;            // depth ids, |T|_1 descending
+    std::vector<std::array<int32_t, 2>> levels;   // (first in levelList, count) per |T|_1, descending
+    // Scatter-add of box results onto points, as a gather per point: CSR whose rows are
+    // device indices and whose entries are scratch positions.  Row groups: one per
+    // |T|_1 (same order as levels), then one over all depths.
+    std::vector<int32_t> rowDev;
+    std::vector<int64_t> rowPtr;               // rows + 1
+    std::vector<int32_t> col;
+    std::vector<std::array<int64_t, 2>> groups;   // (first row, number of rows)
+    // LU factors (no pivoting) of the line matrices of levels 1..maxLevel:
+    // fac[a * facTot + facBase(t) + k], a = 0..4 for l1, l2, u0, u1, u2
+    std::vector<double> fac;
+    int64_t facTot = 0;
+    int64_t maxBox = 0;
+};
+inline int64_t facBase(int t) { return (int64_t(1) << t) - t - 1; }
+// Tables of the basis transformations for a dense-path grid.
+int buildTransform(const Grid& g, TrTables& tr, std::string& err);
+
 // Build the grid from a reference slot-order index table.  Returns 0 or a
 // negative status with a message in err.
 int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid& g, std::string& err);

@@ -1,0 +1,257 @@
+// lsg_transform.hip -- basis transformations between nodal values, hierarchical
+// surpluses and prewavelet coefficients on gfx950, and the energy norms built on them
+// (the reference's BasisTransformations/BasisTransformations.cpp and
+// applications/H1_Norm.cpp).
+//
+// Every step works on the dense box G_T of one depth T (the level-T full grid, all
+// points of depth <= T), staged in LDS by one workgroup per (depth, instance):
+//   * hierarchization / dehierarchization of a sparse-grid vector: the surplus of a
+//     point of depth T only involves points of G_T, so each depth box is
+//     (de)hierarchized on its own and its depth-T entries are written back;
+//   * prewavelets from nodal values (calcPrewByNodal, :486-545): depths are taken by
+//     |T|_1 descending.  For one |T|_1 every depth box is dehierarchized from the
+//     running hierarchical representation, solved line by line with the prewavelet
+//     line matrix (PrewaveletMatrixHomogen), and its depth-T entries are the prewavelet
+//     coefficients.  Their hierarchical representation on G_T (CalcHierbyPrew,
+//     :321-353) goes to a scratch box and is subtracted point by point in a second
+//     launch.  Depths with equal |T|_1 are incomparable, and the prewavelet part of
+//     depth T of a function in the span of depths <= T' vanishes unless T <= T', so
+//     the depths of one |T|_1 do not depend on each other;
+//   * nodal values from prewavelets (calcNodalByPrew, :670-707): every depth's
+//     hierarchical representation in one launch, a gather-sum per point, then
+//     dehierarchization.
+// Lines along one dimension are handled by one thread each (sequential recurrences);
+// the boxes are small (at most a few thousand values), so a transform is a few
+// launches per |T|_1 level and is bound by launch latency, not by HBM.
+#include <hip/hip_runtime.h>
+
+#include "lsg_kernels.hpp"
+
+namespace lsg {
+
+namespace {
+
+enum TrMode { TR_HIER = 0, TR_DEHIER = 1, TR_LEVEL = 2, TR_SYNTH = 3 };
+
+// 1D hierarchization of a line of level t (n = 2^t - 1 values at stride s, 0 outside):
+// finest level first, v_q -= (v_{q-h} + v_{q+h}) / 2 (CalcHierarchicalBasis,
+// BasisTransformations.cpp:390-409).
+__device__ inline void hierLine(double* v, int t, int s, bool inverse) {
+    const int n = (1 << t) - 1;
+    for (int kk = 0; kk < t - 1; ++kk) {
+        const int k = inverse ? kk + 2 : t - kk;        // level being (de)hierarchized
+        const int h = 1 << (t - k);
+        for (int q = h; q <= n; q += 2 * h) {
+            const double l = q - h >= 1 ? v[(q - h - 1) * s] : 0.0;
+            const double r = q + h <= n ? v[(q + h - 1) * s] : 0.0;
+            const double m = 0.5 * (l + r);
+            v[(q - 1) * s] = inverse ? v[(q - 1) * s] + m : v[(q - 1) * s] - m;
+        }
+    }
+}
+
+// first factor of level t in the factor tables (levels 1.. t-1 before it), as facBase
+__device__ inline int64_t facStart(int t) { return (int64_t(1) << t) - t - 1; }
+
+// Solve M x = b on a line with the LU factors of its level (no pivoting).
+__device__ inline void solveLine(double* v, int t, int s, const double* fac, int64_t facTot) {
+    const int n = (1 << t) - 1;
+    const double* l1 = fac + facStart(t);
+    const double* l2 = l1 + facTot;
+    const double* u0 = l1 + 2 * facTot;
+    const double* u1 = l1 + 3 * facTot;
+    const double* u2 = l1 + 4 * facTot;
+    double y1 = 0.0, y2 = 0.0;   // y_{k-1}, y_{k-2}
+    for (int k = 0; k < n; ++k) {
+        const double y = v[k * s] - l1[k] * y1 - l2[k] * y2;
+        v[k * s] = y;
+        y2 = y1;
+        y1 = y;
+    }
+    double x1 = 0.0, x2 = 0.0;   // x_{k+1}, x_{k+2}
+    for (int k = n - 1; k >= 0; --k) {
+        const double x = (v[k * s] - u1[k] * x1 - u2[k] * x2) / u0[k];
+        v[k * s] = x;
+        x2 = x1;
+        x1 = x;
+    }
+}
+
+// Coefficient of the basis function at line position j (1-based) in the nodal value at
+// position r of a level-t line: the columns of PrewaveletMatrixHomogen::create
+// (sgoperation/matrix.cpp:74-139).
+__device__ inline double lineCoef(int t, int n, int j, int r) {
+    const int o = r - j;
+    if (j & 1) {
+        if (o == 0) return (t > 1 && (j == 1 || j == n)) ? 0.9 : 1.0;
+        if (o == 1 || o == -1) return -0.6;
+        return 0.1;   // |o| == 2
+    }
+    if (o == 0) return 1.0;
+    if (o == 1 || o == -1) return 0.5;
+    return 0.0;
+}
+
+// v <- M v on a line (in place, old values kept in a 2-value window).
+__device__ inline void applyLine(double* v, int t, int s) {
+    const int n = (1 << t) - 1;
+    if (n == 1) return;   // level 1: the identity
+    double old2 = 0.0, old1 = 0.0;   // old v at r-2, r-1
+    for (int r = 1; r <= n; ++r) {
+        const double c0 = v[(r - 1) * s];
+        double acc = 0.0;
+        if (r - 2 >= 1) acc += lineCoef(t, n, r - 2, r) * old2;
+        if (r - 1 >= 1) acc += lineCoef(t, n, r - 1, r) * old1;
+        acc += lineCoef(t, n, r, r) * c0;
+        if (r + 1 <= n) acc += lineCoef(t, n, r + 1, r) * v[r * s];
+        if (r + 2 <= n) acc += lineCoef(t, n, r + 2, r) * v[(r + 1) * s];
+        v[(r - 1) * s] = acc;
+        old2 = old1;
+        old1 = c0;
+    }
+}
+
+enum LineOp { L_HIER, L_DEHIER, L_SOLVE, L_APPLY };
+
+// One pass of a line operation along every dimension of the box (one thread per line).
+__device__ void boxPass(double* box, const TrItem& it, LineOp op, const double* fac, int64_t facTot) {
+    for (int i = 0; i < it.d; ++i) {
+        const int n = it.n[i];
+        if (n > 1) {
+            const int S = it.S[i];
+            const int lines = it.total / n;
+            for (int l = threadIdx.x; l < lines; l += blockDim.x) {
+                const int base = (l % S) + (l / S) * S * n;
+                double* v = box + base;
+                switch (op) {
+                    case L_HIER: hierLine(v, it.T[i], S, false); break;
+                    case L_DEHIER: hierLine(v, it.T[i], S, true); break;
+                    case L_SOLVE: solveLine(v, it.T[i], S, fac, facTot); break;
+                    case L_APPLY: applyLine(v, it.T[i], S); break;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// box entry p is a point of depth exactly T (odd position in every dimension)
+__device__ inline bool ownDepth(const TrItem& it, int p) {
+    for (int i = 0; i < it.d; ++i) {
+        const int k = p % it.n[i];
+        p /= it.n[i];
+        if (k & 1) return false;   // 0-based even = 1-based odd
+    }
+    return true;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) tr_box_kernel(const TrItem* __restrict__ items, const int32_t* __restrict__ list,
+                                                      const int32_t* __restrict__ gidx, const double* __restrict__ fac,
+                                                      int64_t facTot, const double* __restrict__ in,
+                                                      double* __restrict__ out, double* __restrict__ scratch, int64_t n,
+                                                      int64_t nml) {
+    extern __shared__ double box[];
+    const TrItem it = items[list ? list[blockIdx.x] : blockIdx.x];
+    const int64_t b = blockIdx.y;
+    const double* x = in + b * n;
+    const int32_t* gi = gidx + it.box_off;
+    for (int p = threadIdx.x; p < it.total; p += blockDim.x) {
+        if (MODE == TR_SYNTH) box[p] = ownDepth(it, p) ? x[gi[p]] : 0.0;
+        else box[p] = x[gi[p]];
+    }
+    __syncthreads();
+    if (MODE == TR_HIER) boxPass(box, it, L_HIER, fac, facTot);
+    if (MODE == TR_DEHIER || MODE == TR_LEVEL) boxPass(box, it, L_DEHIER, fac, facTot);
+    if (MODE == TR_LEVEL) {
+        boxPass(box, it, L_SOLVE, fac, facTot);
+        double* y = out + b * n;
+        for (int p = threadIdx.x; p < it.total; p += blockDim.x) {
+            if (ownDepth(it, p)) y[gi[p]] = box[p];
+            else box[p] = 0.0;
+        }
+        __syncthreads();
+    }
+    if (MODE == TR_HIER || MODE == TR_DEHIER) {
+        double* y = out + b * n;
+        for (int p = threadIdx.x; p < it.total; p += blockDim.x)
+            if (ownDepth(it, p)) y[gi[p]] = box[p];
+        return;
+    }
+    // TR_LEVEL / TR_SYNTH: hierarchical representation of the depth-T prewavelets on G_T
+    boxPass(box, it, L_APPLY, fac, facTot);
+    boxPass(box, it, L_HIER, fac, facTot);
+    double* z = scratch + b * nml + it.box_off;
+    for (int p = threadIdx.x; p < it.total; p += blockDim.x) z[p] = box[p];
+}
+
+// v[rowDev[r]] = (sub ? v[rowDev[r]] : 0) -/+ sum of the row's scratch entries, in row order
+__global__ void tr_gather_kernel(const int32_t* __restrict__ rowDev, const int64_t* __restrict__ rowPtr,
+                                 const int32_t* __restrict__ col, int64_t nrows, const double* __restrict__ scratch,
+                                 double* __restrict__ v, int64_t n, int64_t nml, int sub) {
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nrows) return;
+    const int64_t b = blockIdx.y;
+    const double* z = scratch + b * nml;
+    double s = 0.0;
+    for (int64_t k = rowPtr[r]; k < rowPtr[r + 1]; ++k) s += z[col[k]];
+    double* y = v + b * n + rowDev[r];
+    *y = sub ? *y - s : s;
+}
+
+// out[b] = sqrt(sum_i x_i y_i) for one instance per workgroup (fixed summation order)
+__global__ void __launch_bounds__(256) tr_energy_kernel(const double* __restrict__ x, const double* __restrict__ y,
+                                                        int64_t n, double* __restrict__ out) {
+    __shared__ double red[256];
+    const int64_t b = blockIdx.x;
+    double s = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += x[b * n + i] * y[b * n + i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[b] = sqrt(red[0]);
+}
+
+int boxThreads(int64_t maxBox) { return maxBox <= 128 ? 64 : (maxBox <= 512 ? 128 : 256); }
+
+}  // namespace
+
+void launch_tr_box(int mode, const TrItem* items, const int32_t* list, int nitems, const int32_t* gidx,
+                   const double* fac, int64_t facTot, int64_t maxBox, const double* in, double* out, double* scratch,
+                   int64_t n, int64_t nml, int batch, hipStream_t s) {
+    if (nitems <= 0 || batch <= 0) return;
+    const dim3 grid(nitems, batch);
+    const int thr = boxThreads(maxBox);
+    const size_t lds = (size_t)maxBox * sizeof(double);
+    switch (mode) {
+        case TR_HIER:
+            tr_box_kernel<TR_HIER><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml);
+            break;
+        case TR_DEHIER:
+            tr_box_kernel<TR_DEHIER><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml);
+            break;
+        case TR_LEVEL:
+            tr_box_kernel<TR_LEVEL><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml);
+            break;
+        default:
+            tr_box_kernel<TR_SYNTH><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml);
+            break;
+    }
+}
+
+void launch_tr_gather(const int32_t* rowDev, const int64_t* rowPtr, const int32_t* col, int64_t nrows,
+                      const double* scratch, double* v, int64_t n, int64_t nml, int sub, int batch, hipStream_t s) {
+    if (nrows <= 0 || batch <= 0) return;
+    const dim3 grid((unsigned)((nrows + 255) / 256), batch);
+    tr_gather_kernel<<<grid, 256, 0, s>>>(rowDev, rowPtr, col, nrows, scratch, v, n, nml, sub);
+}
+
+void launch_tr_energy(const double* x, const double* y, int64_t n, double* out, int batch, hipStream_t s) {
+    if (batch <= 0) return;
+    tr_energy_kernel<<<batch, 256, 0, s>>>(x, y, n, out);
+}
+
+}  // namespace lsg
