@@ -23,7 +23,9 @@ its own metric; the default line above is the headline):
         1e-10 against the FP64 solve (loadsg_amd/measure_adaptive.py);
   cfg4  64 instances with d uniform in {3..6} and level in {5..8} (seeded), exp-sine
         coefficients (16 xi), 8-term sine right-hand sides, solved together to 1e-8
-        (loadsg_amd.solve_many: one stream per grid shape).
+        (loadsg_amd.solve_many: one stream per grid shape);
+  transform  nodal values -> prewavelets (calcPrewByNodal), the inverse and the H1 norm on
+        the d=5 level-7 grid for --batch instances, beside the reference's calcPrewByNodal.
 """
 import argparse
 import json
@@ -231,6 +233,89 @@ def workload_cfg4(args):
     print(json.dumps(out))
 
 
+def workload_transform(args):
+    """Basis transformations on the d=5 level-7 grid: calcPrewByNodal (nodal values ->
+    prewavelets), calcNodalByPrew and the H1 energy norm (H1::getValue) of --batch
+    instances, inputs resident in HBM; beside the reference's own calcPrewByNodal on this
+    host (the reference's cost grows about 6.7x per level, so it is timed at level 6 and
+    the level-7 figure is an extrapolation by the measured level-5 -> 6 growth)."""
+    import torch
+    import loadsg_amd as L
+    grid = L.AdaptiveSparseGrid.regular(D, LEVEL)
+    n = grid.size
+    nml = grid.stats()["multilevel_size"]
+    B = args.batch
+    dev = torch.device("cuda:0")
+    op = L.MatrixVectorHomogen(grid, L.Poisson(), 64, batch=B)
+    g = torch.Generator(device=dev)
+    g.manual_seed(SEED)
+    u = torch.randn(B, n, dtype=torch.float64, device=dev, generator=g)
+    p = torch.zeros_like(u)
+    w = torch.zeros_like(u)
+    op.prew_by_nodal(u, p)
+    op.nodal_by_prew(p, w)
+    op.energy_norm(u)          # first call autotunes the matvec launch shapes (untimed)
+    torch.cuda.synchronize()
+    rt_err = float((w - u).abs().max() / u.abs().max())
+
+    def timed(fn, reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    reps = max(args.steps, 3)
+    t_pn = timed(lambda: op.prew_by_nodal(u, p), reps)
+    t_np = timed(lambda: op.nodal_by_prew(p, w), reps)
+    t_h = timed(lambda: op.hierarchize(u, w), reps)
+    t_norm = timed(lambda: op.energy_norm(u), reps)
+    # streaming model of calcPrewByNodal per instance: box gathers of the hierarchization
+    # and of every level (2 N_ML), the scratch boxes written and read back (2 N_ML), the
+    # row updates of the running representation (N_ML), the surpluses and prewavelets
+    # written (2 N)
+    alg = 8.0 * (5 * nml + 2 * n) * B
+    log("transform: prew_by_nodal %.3f ms, nodal_by_prew %.3f ms, hierarchize %.3f ms, H1 norm %.3f ms (batch %d)"
+        % (t_pn * 1e3, t_np * 1e3, t_h * 1e3, t_norm * 1e3, B))
+    cpu = None
+    if not args.no_cpu_baseline:
+        exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_d5")
+        cores = min(len(os.sched_getaffinity(0)), 16)
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close")
+        res = {}
+        for lev in (5, 6):
+            if not os.path.exists(exe):
+                break
+            log("transform: reference calcPrewByNodal at level %d" % lev)
+            r = subprocess.run([exe, "timetr", str(lev)], env=env, capture_output=True, text=True, timeout=600)
+            if r.returncode != 0:
+                break
+            res[lev] = json.loads(r.stdout.strip().splitlines()[-1])
+        if 6 in res and 5 in res:
+            growth = res[6]["prew_by_nodal_s"] / res[5]["prew_by_nodal_s"]
+            t7 = res[6]["prew_by_nodal_s"] * growth
+            cpu = {"value": 1.0 / t7, "unit": "transforms/s", "cores": cores, "kind": "reference",
+                   "sample": "LoAdSG calcPrewByNodal (unmodified sources, %d OpenMP threads) timed at d=5 level 6 "
+                             "(%.1f s, N=%d) and level 5 (%.2f s); level 7 extrapolated by the %.2fx growth: %.0f s"
+                             % (cores, res[6]["prew_by_nodal_s"], res[6]["n"], res[5]["prew_by_nodal_s"], growth, t7),
+                   "level6_s": res[6]["prew_by_nodal_s"], "level7_s_extrapolated": t7}
+    out = {"metric": "prewavelet transforms/s (calcPrewByNodal, d=5 L7)", "value": B / t_pn, "unit": "transforms/s",
+           "n_gpus": 1, "steps": reps, "higher_is_better": True, "dtype": "f64", "data": "synthetic (seeded normal)",
+           "config": {"workload": "d=5 regular sparse grid level 7 (N=%d), nodal values -> prewavelet coefficients, "
+                                  "batch %d" % (n, B), "batch": B, "N": n},
+           "roofline": {"bound": "hbm", "achieved": alg / t_pn / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                        "frac": alg / t_pn / HBM_PEAK, "traffic": None,
+                        "kernel": "tr_box_kernel + tr_gather_kernel: the whole calcPrewByNodal launch sequence "
+                                  "(timed as a sequence; per-kernel split in profiles/)",
+                        "algorithmic_bytes": alg, "time_ms": t_pn * 1e3},
+           "cpu_baseline": cpu,
+           "extra": {"prew_by_nodal_ms": t_pn * 1e3, "nodal_by_prew_ms": t_np * 1e3, "hierarchize_ms": t_h * 1e3,
+                     "h1_norm_ms": t_norm * 1e3, "round_trip_rel_err": rt_err, "n_ml": nml,
+                     "speedup_vs_reference": None if cpu is None else cpu["level7_s_extrapolated"] * B / t_pn}}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -243,14 +328,15 @@ def main():
     ap.add_argument("--warm-xi", type=int, default=512,
                     help="coefficient samples for the warm-start training set (x 8 basis solves); 0 = zero start")
     ap.add_argument("--warm-steps", type=int, default=4000)
-    ap.add_argument("--workload", choices=("d5l7", "cfg2", "cfg3", "cfg4"), default="d5l7")
+    ap.add_argument("--workload", choices=("d5l7", "cfg2", "cfg3", "cfg4", "transform"), default="d5l7")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (batched solves shard by instances across GPUs)",
               file=sys.stderr)
         sys.exit(2)
     if args.workload != "d5l7":
-        {"cfg2": workload_cfg2, "cfg3": workload_cfg3, "cfg4": workload_cfg4}[args.workload](args)
+        {"cfg2": workload_cfg2, "cfg3": workload_cfg3, "cfg4": workload_cfg4,
+         "transform": workload_transform}[args.workload](args)
         return
 
     import torch

@@ -161,6 +161,42 @@ struct CG {
     }
 };
 
+/* Basis transformations on the device (BasisTransformations/BasisTransformations.cpp);
+ * the reference's free functions take only the vectors, the shim also takes the
+ * MatrixVectorHomogen that holds the device grid.  Grids with complete depth boxes only. */
+inline void calcPrewByNodal(MatrixVectorHomogen& matrix, VectorSparseG& prew, VectorSparseG& u) {
+    lsg_op* op = matrix.opFor(Coefficient(LSG_COEF_CONST, 1.0, 0.0));
+    check(lsg_prew_by_nodal(op, u.getDatatableVector(), prew.getDatatableVector(), LSG_HOST_SLOT));
+}
+inline void calcNodalByPrew(MatrixVectorHomogen& matrix, VectorSparseG& prew, VectorSparseG& u) {
+    lsg_op* op = matrix.opFor(Coefficient(LSG_COEF_CONST, 1.0, 0.0));
+    check(lsg_nodal_by_prew(op, prew.getDatatableVector(), u.getDatatableVector(), LSG_HOST_SLOT));
+}
+
+/* H1::getValue / L2::getValue (applications/H1_Norm.cpp:8-78): the Poisson / mass energy
+ * norm of the prewavelet coefficients of nodal values u. */
+class H1 {
+  public:
+    H1(AdaptiveSparseGrid& grid, MultiLevelAdaptiveSparseGrid& mgrid) : m_(grid, mgrid) {}
+    double getValue(VectorSparseG& u) { return norm(m_, u, Coefficient(LSG_COEF_CONST, 1.0, 0.0)); }
+    static double norm(MatrixVectorHomogen& m, VectorSparseG& u, const Coefficient& c) {
+        double out = 0.0;
+        check(lsg_energy_norm(m.opFor(c), u.getDatatableVector(), &out, LSG_HOST_SLOT));
+        return out;
+    }
+
+  private:
+    MatrixVectorHomogen m_;
+};
+class L2 {
+  public:
+    L2(AdaptiveSparseGrid& grid, MultiLevelAdaptiveSparseGrid& mgrid) : m_(grid, mgrid) {}
+    double getValue(VectorSparseG& u) { return H1::norm(m_, u, Coefficient(LSG_COEF_CONST, 0.0, 1.0)); }
+
+  private:
+    MatrixVectorHomogen m_;
+};
+
 }  // namespace lsg_shim
 
 #endif /* LOADSG_AMD_SHIM_HPP */

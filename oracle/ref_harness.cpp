@@ -17,6 +17,7 @@
 //   ref_harness regular  <level> <coef> <outdir> [what...]
 //   ref_harness adaptive <startlevel> <radius> <maxlevel> <coef> <outdir> [what...]
 //   ref_harness timemv   <level> <coef> <reps>            (timing sample, prints JSON)
+//   ref_harness timetr   <level>                          (calcPrewByNodal / calcNodalByPrew timing, JSON)
 // coef: const:<a0>:<c0> | cfg2 | cfg3 | expsin:<xi1>,<xi2>,...
 // what: matvec precond cg transform   (default: all)
 #include "source/sparseEXPDE.h"
@@ -397,6 +398,33 @@ int main(int argc, char** argv) {
         std::printf("{\"n\": %zu, \"setup_s\": %.3f, \"nodal_s\": %.4f, \"timed_s\": %.3f, \"cases\": \"%s\", "
                     "\"case_weight_fraction\": %.4f, \"matvec_s_estimate\": %.4f}\n",
                     n, secs(t0, t1), tNodal, tNodal + tCases, done.c_str(), wSampled / wAll, est);
+        return 0;
+    }
+    if (mode == "timetr" && argc >= 3) {
+        // calcPrewByNodal and calcNodalByPrew (BasisTransformations.cpp:486-545, 670-707)
+        // on the regular grid of the given level, timed once each; prints JSON.
+        int level = std::atoi(argv[2]);
+        AdaptiveSparseGrid grid;
+        IndexDimension center;
+        grid.AddRecursiveSonsOfPoint(center, level);
+        size_t n = grid.getMaximalOccupiedSecondTable();
+        VectorSparseG u(grid), pr(grid), nod(grid);
+        u = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            IndexDimension I = grid.getIndexOfTable(i);
+            u.setValue((unsigned long)i, sinFunc(I) + I.coordinate(0));
+        }
+        pr = 0.0;
+        auto t0 = std::chrono::steady_clock::now();
+        calcPrewByNodal(pr, u);
+        auto t1 = std::chrono::steady_clock::now();
+        nod = 0.0;
+        calcNodalByPrew(pr, nod);
+        auto t2 = std::chrono::steady_clock::now();
+        double e = 0.0;
+        for (size_t i = 0; i < n; ++i) e = std::max(e, std::fabs(nod.getValue((unsigned long)i) - u.getValue((unsigned long)i)));
+        std::printf("{\"n\": %zu, \"prew_by_nodal_s\": %.6f, \"nodal_by_prew_s\": %.6f, \"roundtrip_err\": %.3e}\n", n,
+                    std::chrono::duration<double>(t1 - t0).count(), std::chrono::duration<double>(t2 - t1).count(), e);
         return 0;
     }
     if (mode == "timemv" && argc >= 5) {

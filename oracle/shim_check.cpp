@@ -7,10 +7,12 @@
 // ::MatrixVectorHomogen::multiplication (MatrixVectorHomogen.h:660-759) and
 // ::CG::solveHomogen (applications/cg_method.h:759-886) on the CPU; the drop-in
 // side runs lsg_shim::MatrixVectorHomogen / lsg_shim::CG with the same calls on the GPU.
+// The same is done for calcPrewByNodal / calcNodalByPrew (BasisTransformations.cpp:486-707)
+// and H1 / L2 ::getValue (applications/H1_Norm.cpp:8-78).
 //
 // Usage: shim_check <level>      prints one JSON line; exit status 0 when
 //   matvecs agree to 1e-12 (relative, max norm), CG iteration counts are equal and
-//   the CG solutions agree to 1e-10.
+//   the CG solutions agree to 1e-10, transforms and norms to 1e-12.
 #include "source/sparseEXPDE.h"
 #include "loadsg_amd_shim.hpp"
 
@@ -70,9 +72,27 @@ int main(int argc, char** argv) {
     lsg_shim::CG::solveHomogen(1e-8, solDev, f, &itDev, usol, dev, poisson, &tpDev);
     const double cgRel = relDiff(solRef, solDev, n);
 
-    const bool ok = mvPoisson < 1e-12 && mvHelm < 1e-12 && itRef == itDev && cgRel < 1e-10;
+    // basis transformations and norms (BasisTransformations.cpp, H1_Norm.cpp)
+    VectorSparseG u(grid), pRef(grid), pDev(grid), nRef(grid), nDev(grid);
+    for (size_t i = 0; i < n; ++i) u.setValue((unsigned long)i, std::sin(0.37 * double(i)) + 0.1);
+    pRef = 0.0;
+    calcPrewByNodal(pRef, u);
+    lsg_shim::calcPrewByNodal(dev, pDev, u);
+    const double trPrew = relDiff(pRef, pDev, n);
+    nRef = 0.0;
+    calcNodalByPrew(x, nRef);
+    lsg_shim::calcNodalByPrew(dev, x, nDev);
+    const double trNodal = relDiff(nRef, nDev, n);
+    const double h1Ref = ::H1(grid, mgrid).getValue(u), h1Dev = lsg_shim::H1(grid, mgrid).getValue(u);
+    const double l2Ref = ::L2(grid, mgrid).getValue(u), l2Dev = lsg_shim::L2(grid, mgrid).getValue(u);
+    const double normRel = std::fmax(std::fabs(h1Ref - h1Dev) / h1Ref, std::fabs(l2Ref - l2Dev) / l2Ref);
+
+    const bool ok = mvPoisson < 1e-12 && mvHelm < 1e-12 && itRef == itDev && cgRel < 1e-10 && trPrew < 1e-12 &&
+                    trNodal < 1e-12 && normRel < 1e-12;
     std::printf("{\"d\": %d, \"level\": %d, \"n\": %zu, \"matvec_poisson_rel\": %.3e, \"matvec_helm_rel\": %.3e, "
-                "\"cg_iters_ref\": %d, \"cg_iters_dev\": %d, \"cg_x_rel\": %.3e, \"ok\": %s}\n",
-                DimensionSparseGrid, level, n, mvPoisson, mvHelm, itRef, itDev, cgRel, ok ? "true" : "false");
+                "\"cg_iters_ref\": %d, \"cg_iters_dev\": %d, \"cg_x_rel\": %.3e, \"prew_by_nodal_rel\": %.3e, "
+                "\"nodal_by_prew_rel\": %.3e, \"norms_rel\": %.3e, \"ok\": %s}\n",
+                DimensionSparseGrid, level, n, mvPoisson, mvHelm, itRef, itDev, cgRel, trPrew, trNodal, normRel,
+                ok ? "true" : "false");
     return ok ? 0 : 1;
 }
