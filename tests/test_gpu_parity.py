@@ -189,6 +189,37 @@ def test_full_size_d5_l7_properties():
     assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(f)
 
 
+def test_full_size_d6_l8_properties():
+    """d=6 level 8 (N = 471,041, the largest config-4 shape): FP64 symmetry, linearity,
+    diagonal = e_p^T A e_p; FP32 matvec within 2e-5 of FP64; the mixed-precision solve
+    reaches a true FP64 relative residual of 1e-8."""
+    print("d6 l8", flush=True)
+    grid = L.AdaptiveSparseGrid.regular(6, 8)
+    n = grid.size
+    assert n == 471041
+    rng = np.random.default_rng(68)
+    st = L.VariableCoefficient(L.COEF_EXPSIN, xi=rng.normal(0.0, 0.5, 16))
+    op = L.MatrixVectorHomogen(grid, st)
+    op32 = L.MatrixVectorHomogen(grid, st, precision=32)
+    x, y = rng.standard_normal(n), rng.standard_normal(n)
+    Ax, Ay = op.multiplication(x), op.multiplication(y)
+    assert abs(x @ Ay - y @ Ax) <= 1e-12 * abs(x @ Ay)
+    assert relmax(op.multiplication(x - 0.5 * y), Ax - 0.5 * Ay) < 1e-12
+    assert x @ Ax > 0
+    D = op.diagonal()
+    for p in rng.choice(n, 3, replace=False):
+        e = np.zeros(n)
+        e[p] = 1.0
+        assert abs(op.multiplication(e)[p] - D[p]) <= 1e-12 * abs(D[p])
+    assert relmax(op32.multiplication(x), Ax) < 2e-5
+    f = op.rhs_sines(rng.standard_normal(8), rng.integers(1, 4, 8))
+    xs = np.zeros(n)
+    info = L.CG.solveMixed(1e-8, xs, f, op, op32, inner_rel_tol=1e-4)
+    assert info["converged"] == 1
+    r = f - op.multiplication(xs)
+    assert np.linalg.norm(r) <= 1.01e-8 * np.linalg.norm(f)
+
+
 def test_mixed_precision_refinement():
     """FP32 inner PCG under FP64 iterative refinement reaches the FP64 tolerance and
     agrees with the FP64-only solve to the stated error."""

@@ -122,6 +122,16 @@ def test_full_size_round_trips_and_batch():
         assert norms[b] == op.energy_norm(U[b])
 
 
+def test_d6_l8_round_trip():
+    """Largest config-4 shape (N = 471,041; depth boxes up to 3,969 values)."""
+    print("transform d=6 L=8 round trip", flush=True)
+    grid = L.AdaptiveSparseGrid.regular(6, 8)
+    op = L.MatrixVectorHomogen(grid, L.Poisson(), 64)
+    u = np.random.default_rng(68).standard_normal(grid.size)
+    assert relmax(op.nodal_by_prew(op.prew_by_nodal(u)), u) < 1e-12
+    assert relmax(op.dehierarchize(op.hierarchize(u)), u) < 1e-13
+
+
 def test_unsupported_inputs_raise():
     print("transform errors", flush=True)
     gv = load("d3_adaptive_cfg3")
