@@ -624,11 +624,11 @@ int ensureTransform(lsg_op* op) {
 
 enum { TR_HIER = 0, TR_DEHIER = 1, TR_LEVEL = 2, TR_SYNTH = 3 };
 
-void trBox(lsg_op* op, int mode, const int32_t* list, int count, const double* in, double* out) {
+void trBox(lsg_op* op, int mode, const int32_t* list, int count, int64_t maxBox, const double* in, double* out) {
     const Grid& g = op->grid->g;
     TrDev& t = *op->tr;
     launch_tr_box(mode, t.items.as<TrItem>(), list, count, t.gidx.as<int32_t>(), t.fac.as<double>(), t.host.facTot,
-                  t.host.maxBox, in, out, t.scratch.as<double>(), g.n, g.nml, op->batch, op->stream);
+                  maxBox, in, out, t.scratch.as<double>(), g.n, g.nml, op->batch, op->stream);
 }
 
 void trGather(lsg_op* op, size_t group, double* v, int sub) {
@@ -646,23 +646,25 @@ int transformDev(lsg_op* op, int what, const double* in, double* out) {
     if (rc) return rc;
     TrDev& t = *op->tr;
     const int nd = (int)t.host.items.size();
+    const int64_t mb = t.host.maxBox;
     switch (what) {
-        case 0: trBox(op, TR_HIER, nullptr, nd, in, out); break;
-        case 1: trBox(op, TR_DEHIER, nullptr, nd, in, out); break;
+        case 0: trBox(op, TR_HIER, nullptr, nd, mb, in, out); break;
+        case 1: trBox(op, TR_DEHIER, nullptr, nd, mb, in, out); break;
         case 2: {   // calcPrewByNodal (BasisTransformations.cpp:486-545)
             double* hier = t.hier.as<double>();
-            trBox(op, TR_HIER, nullptr, nd, in, hier);
+            trBox(op, TR_HIER, nullptr, nd, mb, in, hier);
             for (size_t l = 0; l < t.host.levels.size(); ++l) {
-                trBox(op, TR_LEVEL, t.list.as<int32_t>() + t.host.levels[l][0], t.host.levels[l][1], hier, out);
+                trBox(op, TR_LEVEL, t.list.as<int32_t>() + t.host.levels[l][0], t.host.levels[l][1],
+                      t.host.levelMaxBox[l], hier, out);
                 if (l + 1 < t.host.levels.size()) trGather(op, l, hier, 1);
             }
             break;
         }
         case 3: {   // calcNodalByPrew (BasisTransformations.cpp:670-707)
             double* hier = t.hier.as<double>();
-            trBox(op, TR_SYNTH, nullptr, nd, in, nullptr);
+            trBox(op, TR_SYNTH, nullptr, nd, mb, in, nullptr);
             trGather(op, t.host.groups.size() - 1, hier, 0);
-            trBox(op, TR_DEHIER, nullptr, nd, hier, out);
+            trBox(op, TR_DEHIER, nullptr, nd, mb, hier, out);
             break;
         }
         default: return fail(LSG_ERR_ARG, "unknown transform");

@@ -911,6 +911,7 @@ int buildTransform(const Grid& g, TrTables& tr, std::string& err) {
     }
     tr.levelList.clear();
     tr.levels.clear();
+    tr.levelMaxBox.clear();
     tr.rowDev.clear();
     tr.col.clear();
     tr.rowPtr.assign(1, 0);
@@ -938,6 +939,9 @@ int buildTransform(const Grid& g, TrTables& tr, std::string& err) {
     std::vector<int32_t> all;
     for (auto& kv : bySum) {
         tr.levels.push_back({(int32_t)tr.levelList.size(), (int32_t)kv.second.size()});
+        int64_t mb = 0;
+        for (int32_t t : kv.second) mb = std::max<int64_t>(mb, tr.items[t].total);
+        tr.levelMaxBox.push_back(mb);
         tr.levelList.insert(tr.levelList.end(), kv.second.begin(), kv.second.end());
         addGroup(kv.second);
         all.insert(all.end(), kv.second.begin(), kv.second.end());

@@ -188,6 +188,7 @@ struct TrTables {
     std::vector<int32_t> gidx;                 // per box entry: device index of its point
     std::vector<int32_t> levelList;            // depth ids, |T|_1 descending
     std::vector<std::array<int32_t, 2>> levels;   // (first in levelList, count) per |T|_1, descending
+    std::vector<int64_t> levelMaxBox;          // largest |G_T| per level
     // Scatter-add of box results onto points, as a gather per point: CSR whose rows are
     // device indices and whose entries are scratch positions.  Row groups: one per
     // |T|_1 (same order as levels), then one over all depths.

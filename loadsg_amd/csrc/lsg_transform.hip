@@ -20,10 +20,13 @@
 //   * nodal values from prewavelets (calcNodalByPrew, :670-707): every depth's
 //     hierarchical representation in one launch, a gather-sum per point, then
 //     dehierarchization.
-// Lines along one dimension are handled by one thread each (sequential recurrences);
-// the boxes are small (at most a few thousand values), so a transform is a few
-// launches per |T|_1 level and is bound by launch latency, not by HBM.
+// Lines along one dimension are handled by one thread each (sequential recurrences).
+// The boxes are small (at most a few thousand values), so a workgroup takes the boxes
+// of one depth for several instances (up to 32 KB of LDS) to give its threads enough
+// lines; a transform is two launches per |T|_1 level.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "lsg_kernels.hpp"
 
@@ -111,23 +114,34 @@ __device__ inline void applyLine(double* v, int t, int s) {
     }
 }
 
-enum LineOp { L_HIER, L_DEHIER, L_SOLVE, L_APPLY };
+// Line operations.  Operators along different dimensions commute, so a dehierarchization
+// followed by the line solve (and the line-matrix product followed by the hierarchization)
+// is done dimension by dimension in one pass over each line.
+enum LineOp { L_HIER, L_DEHIER, L_DEHIER_SOLVE, L_APPLY_HIER };
 
-// One pass of a line operation along every dimension of the box (one thread per line).
-__device__ void boxPass(double* box, const TrItem& it, LineOp op, const double* fac, int64_t facTot) {
+// One pass of a line operation along every dimension of the g boxes of the workgroup
+// (one thread per line).
+__device__ void boxPass(double* box, const TrItem& it, int g, LineOp op, const double* fac, int64_t facTot) {
     for (int i = 0; i < it.d; ++i) {
         const int n = it.n[i];
         if (n > 1) {
             const int S = it.S[i];
             const int lines = it.total / n;
-            for (int l = threadIdx.x; l < lines; l += blockDim.x) {
-                const int base = (l % S) + (l / S) * S * n;
-                double* v = box + base;
+            for (int l = threadIdx.x; l < g * lines; l += blockDim.x) {
+                const int j = l / lines;
+                const int ll = l - j * lines;
+                double* v = box + j * it.total + (ll % S) + (ll / S) * S * n;
                 switch (op) {
                     case L_HIER: hierLine(v, it.T[i], S, false); break;
                     case L_DEHIER: hierLine(v, it.T[i], S, true); break;
-                    case L_SOLVE: solveLine(v, it.T[i], S, fac, facTot); break;
-                    case L_APPLY: applyLine(v, it.T[i], S); break;
+                    case L_DEHIER_SOLVE:
+                        hierLine(v, it.T[i], S, true);
+                        solveLine(v, it.T[i], S, fac, facTot);
+                        break;
+                    case L_APPLY_HIER:
+                        applyLine(v, it.T[i], S);
+                        hierLine(v, it.T[i], S, false);
+                        break;
                 }
             }
         }
@@ -145,44 +159,54 @@ __device__ inline bool ownDepth(const TrItem& it, int p) {
     return true;
 }
 
+// One workgroup per (depth, group of G instances): the g <= G boxes of the group sit
+// one after another in LDS.
 template <int MODE>
 __global__ void __launch_bounds__(256) tr_box_kernel(const TrItem* __restrict__ items, const int32_t* __restrict__ list,
                                                       const int32_t* __restrict__ gidx, const double* __restrict__ fac,
                                                       int64_t facTot, const double* __restrict__ in,
                                                       double* __restrict__ out, double* __restrict__ scratch, int64_t n,
-                                                      int64_t nml) {
+                                                      int64_t nml, int G, int batch) {
     extern __shared__ double box[];
     const TrItem it = items[list ? list[blockIdx.x] : blockIdx.x];
-    const int64_t b = blockIdx.y;
-    const double* x = in + b * n;
+    const int64_t b0 = (int64_t)blockIdx.y * G;
+    const int g = (int)min<int64_t>(G, batch - b0);
+    const int tot = it.total;
     const int32_t* gi = gidx + it.box_off;
-    for (int p = threadIdx.x; p < it.total; p += blockDim.x) {
-        if (MODE == TR_SYNTH) box[p] = ownDepth(it, p) ? x[gi[p]] : 0.0;
-        else box[p] = x[gi[p]];
+    for (int p = threadIdx.x; p < g * tot; p += blockDim.x) {
+        const int j = p / tot;
+        const int q = p - j * tot;
+        if (MODE == TR_SYNTH && !ownDepth(it, q)) box[p] = 0.0;
+        else box[p] = in[(b0 + j) * n + gi[q]];
     }
     __syncthreads();
-    if (MODE == TR_HIER) boxPass(box, it, L_HIER, fac, facTot);
-    if (MODE == TR_DEHIER || MODE == TR_LEVEL) boxPass(box, it, L_DEHIER, fac, facTot);
+    if (MODE == TR_HIER) boxPass(box, it, g, L_HIER, fac, facTot);
+    if (MODE == TR_DEHIER) boxPass(box, it, g, L_DEHIER, fac, facTot);
     if (MODE == TR_LEVEL) {
-        boxPass(box, it, L_SOLVE, fac, facTot);
-        double* y = out + b * n;
-        for (int p = threadIdx.x; p < it.total; p += blockDim.x) {
-            if (ownDepth(it, p)) y[gi[p]] = box[p];
+        boxPass(box, it, g, L_DEHIER_SOLVE, fac, facTot);
+        for (int p = threadIdx.x; p < g * tot; p += blockDim.x) {
+            const int j = p / tot;
+            const int q = p - j * tot;
+            if (ownDepth(it, q)) out[(b0 + j) * n + gi[q]] = box[p];
             else box[p] = 0.0;
         }
         __syncthreads();
     }
     if (MODE == TR_HIER || MODE == TR_DEHIER) {
-        double* y = out + b * n;
-        for (int p = threadIdx.x; p < it.total; p += blockDim.x)
-            if (ownDepth(it, p)) y[gi[p]] = box[p];
+        for (int p = threadIdx.x; p < g * tot; p += blockDim.x) {
+            const int j = p / tot;
+            const int q = p - j * tot;
+            if (ownDepth(it, q)) out[(b0 + j) * n + gi[q]] = box[p];
+        }
         return;
     }
     // TR_LEVEL / TR_SYNTH: hierarchical representation of the depth-T prewavelets on G_T
-    boxPass(box, it, L_APPLY, fac, facTot);
-    boxPass(box, it, L_HIER, fac, facTot);
-    double* z = scratch + b * nml + it.box_off;
-    for (int p = threadIdx.x; p < it.total; p += blockDim.x) z[p] = box[p];
+    boxPass(box, it, g, L_APPLY_HIER, fac, facTot);
+    for (int p = threadIdx.x; p < g * tot; p += blockDim.x) {
+        const int j = p / tot;
+        const int q = p - j * tot;
+        scratch[(b0 + j) * nml + it.box_off + q] = box[p];
+    }
 }
 
 // v[rowDev[r]] = (sub ? v[rowDev[r]] : 0) -/+ sum of the row's scratch entries, in row order
@@ -215,7 +239,7 @@ __global__ void __launch_bounds__(256) tr_energy_kernel(const double* __restrict
     if (threadIdx.x == 0) out[b] = sqrt(red[0]);
 }
 
-int boxThreads(int64_t maxBox) { return maxBox <= 128 ? 64 : (maxBox <= 512 ? 128 : 256); }
+int boxThreads(int64_t vals) { return vals <= 128 ? 64 : (vals <= 512 ? 128 : 256); }
 
 }  // namespace
 
@@ -223,21 +247,27 @@ void launch_tr_box(int mode, const TrItem* items, const int32_t* list, int nitem
                    const double* fac, int64_t facTot, int64_t maxBox, const double* in, double* out, double* scratch,
                    int64_t n, int64_t nml, int batch, hipStream_t s) {
     if (nitems <= 0 || batch <= 0) return;
-    const dim3 grid(nitems, batch);
-    const int thr = boxThreads(maxBox);
-    const size_t lds = (size_t)maxBox * sizeof(double);
+    // instances per workgroup: up to 32 KB of LDS (one box when a box needs more)
+    const int G = (int)std::max<int64_t>(1, std::min<int64_t>({16, batch, 4096 / std::max<int64_t>(maxBox, 1)}));
+    const dim3 grid(nitems, (unsigned)((batch + G - 1) / G));
+    const int thr = boxThreads((int64_t)G * maxBox);
+    const size_t lds = (size_t)G * maxBox * sizeof(double);
     switch (mode) {
         case TR_HIER:
-            tr_box_kernel<TR_HIER><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml);
+            tr_box_kernel<TR_HIER><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml, G,
+                                                           batch);
             break;
         case TR_DEHIER:
-            tr_box_kernel<TR_DEHIER><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml);
+            tr_box_kernel<TR_DEHIER><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml, G,
+                                                             batch);
             break;
         case TR_LEVEL:
-            tr_box_kernel<TR_LEVEL><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml);
+            tr_box_kernel<TR_LEVEL><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml, G,
+                                                            batch);
             break;
         default:
-            tr_box_kernel<TR_SYNTH><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml);
+            tr_box_kernel<TR_SYNTH><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml, G,
+                                                            batch);
             break;
     }
 }
