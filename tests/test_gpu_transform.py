@@ -132,6 +132,31 @@ def test_d6_l8_round_trip():
     assert relmax(op.dehierarchize(op.hierarchize(u)), u) < 1e-13
 
 
+def test_poisson_driver_known_answers():
+    """The reference driver examples_doxygen/poisson_regular.cpp (d=3, level 7) entirely on
+    the device: f_h = M prew(f_I), CG to eps 1e-10 with the reference stop rule,
+    u_h = calcNodalByPrew(x), and the errors of u - u_h in the reference's norms, against
+    the values that driver prints (six significant digits)."""
+    from test_oracle_transform import POISSON_D3_L7 as K
+    print("poisson driver d=3 L=7", flush=True)
+    d = 3
+    grid = L.AdaptiveSparseGrid.regular(d, 7)
+    idx, _ = grid.slot_order()
+    u = np.prod(np.sin(math.pi * coords(idx)), axis=1)
+    poisson = L.MatrixVectorHomogen(grid, L.Poisson(), 64)
+    mass = L.MatrixVectorHomogen(grid, L.HelmHoltz(), 64)
+    f = mass.multiplication(L.calcPrewByNodal(d * math.pi ** 2 * u, poisson))
+    x = np.zeros(grid.size)
+    info = L.CG.solveHomogen(1e-10, x, f, poisson)
+    e = u - L.calcNodalByPrew(x, poisson)
+    l2 = L.L2(grid).getValue(e)
+    h1 = L.H1(grid).getValue(e)
+    assert info["iterations"] == K["iterations"]
+    assert abs(np.abs(e).max() - K["linf"]) < 1e-5 * K["linf"]
+    assert abs(l2 - K["l2"]) < 1e-5 * K["l2"]
+    assert abs(l2 + h1 - K["h1_plus_l2"]) < 1e-5 * K["h1_plus_l2"]
+
+
 def test_unsupported_inputs_raise():
     print("transform errors", flush=True)
     gv = load("d3_adaptive_cfg3")

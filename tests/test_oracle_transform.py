@@ -11,6 +11,8 @@ built on them (applications/H1_Norm.cpp) before the device path is checked again
    must equal E_psi p, prewavelets from nodal values must solve E_psi p = u, and
    hierarchical surpluses must solve E_phi a = u.  A wrong boundary coefficient, a
    dropped depth or a transposed line solve fails these.
+3. Known answers of the reference's driver examples_doxygen/poisson_regular.cpp at d=3
+   level 7 (iterations and three error norms, as that driver prints them).
 """
 import json
 import math
@@ -106,3 +108,31 @@ def test_round_trip_larger():
     p = o.transform(o.PREW_BY_NODAL, u)
     assert relmax(o.transform(o.NODAL_BY_PREW, p), u) < 1e-12
     assert relmax(o.transform(o.DEHIER, o.transform(o.HIER, u)), u) < 1e-13
+
+
+# Known answers printed by the reference's own driver examples_doxygen/poisson_regular.cpp
+# (d=3, level 7): CG iterations, L_infty(u - u_h), L2::getValue(u - u_h) and
+# L2 + H1::getValue(u - u_h), six significant digits.
+POISSON_D3_L7 = dict(iterations=21, linf=2.98384e-4, l2=2.34495e-5, h1_plus_l2=5.17872e-3)
+
+
+def test_poisson_driver_known_answers():
+    """The whole driver pipeline (poisson_regular.cpp:57-147) on the oracle: f_h =
+    M prew(f_I), CG to eps 1e-10 with the reference stop rule, u_h = calcNodalByPrew(x),
+    errors of u - u_h in the reference's norms."""
+    d, L = 3, 7
+    idx, act = regular_grid(d, L)
+    u = np.prod(np.sin(math.pi * coords(idx)), axis=1)
+    o = Oracle(idx, act)
+    p = o.transform(o.PREW_BY_NODAL, d * math.pi ** 2 * u)
+    f = o.set_coef(COEF_CONST, a0=0.0, c0=1.0).matvec(p)
+    x, it, _ = o.set_coef(COEF_CONST, a0=1.0, c0=0.0).cg(f, eps=1e-10, maxit=500)
+    e = u - o.transform(o.NODAL_BY_PREW, x)
+    pe = o.transform(o.PREW_BY_NODAL, e)
+    l2 = math.sqrt(pe @ o.set_coef(COEF_CONST, a0=0.0, c0=1.0).matvec(pe))
+    h1 = math.sqrt(pe @ o.set_coef(COEF_CONST, a0=1.0, c0=0.0).matvec(pe))
+    k = POISSON_D3_L7
+    assert it == k["iterations"]
+    assert abs(np.abs(e).max() - k["linf"]) < 1e-5 * k["linf"]
+    assert abs(l2 - k["l2"]) < 1e-5 * k["l2"]
+    assert abs(l2 + h1 - k["h1_plus_l2"]) < 1e-5 * k["h1_plus_l2"]
