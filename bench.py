@@ -328,7 +328,7 @@ def main():
     ap.add_argument("--warm-xi", type=int, default=2500,
                     help="coefficient samples for the warm-start training set (x 8 basis solves = 20,000 "
                          "GPU-generated solves); 0 = zero start")
-    ap.add_argument("--warm-steps", type=int, default=16000)
+    ap.add_argument("--warm-steps", type=int, default=48000)
     ap.add_argument("--workload", choices=("d5l7", "cfg2", "cfg3", "cfg4", "transform"), default="d5l7")
     args = ap.parse_args()
     if args.gpus != 1:

@@ -164,6 +164,12 @@ int lsg_mlp_num_params(const lsg_mlp* m, int64_t* n);
 int lsg_mlp_get_params(const lsg_mlp* m, float* host);
 int lsg_mlp_set_params(lsg_mlp* m, const float* host);
 int lsg_mlp_get_grads(const lsg_mlp* m, float* host);
+/* Training targets of a family whose solution is linear in part of the input:
+ * out[b][i] = sum_j w[b*J + j] U[(rows[b]*J + j)*n + i] (device pointers; U FP32
+ * S x J x n, rows int64, w FP32 batch x J, out FP32 batch x n).  No counterpart in
+ * the reference. */
+int lsg_mlp_mix_rows(lsg_mlp* m, const float* U, int64_t n, int J, const int64_t* rows, const float* w, int batch,
+                     float* out);
 int lsg_mlp_destroy(lsg_mlp* m);
 
 const char* lsg_last_error(void);

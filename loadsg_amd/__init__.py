@@ -93,6 +93,7 @@ def lib():
             "lsg_mlp_get_params": (i32, [vp, vp]),
             "lsg_mlp_set_params": (i32, [vp, vp]),
             "lsg_mlp_get_grads": (i32, [vp, vp]),
+            "lsg_mlp_mix_rows": (i32, [vp, vp, i64, i32, vp, vp, i32, vp]),
             "lsg_mlp_destroy": (i32, [vp]),
             "lsg_last_error": (ctypes.c_char_p, []),
             "lsg_device_available": (i32, []),
@@ -463,6 +464,18 @@ class MLP:
         loss = ctypes.c_double()
         _check(lib().lsg_mlp_train_step(self._h, _ptr(x), _ptr(target), x.shape[0], lr, ctypes.byref(loss)))
         return loss.value
+
+    def mix_rows(self, U, rows, w, out=None):
+        """out[b] = sum_j w[b, j] U[rows[b], j] for U (S x J x n), rows (batch,) int64 and
+        w (batch x J), all FP32 / int64 device tensors."""
+        import torch
+        S, J, n = U.shape
+        assert U.dtype == torch.float32 and U.is_contiguous() and rows.dtype == torch.int64
+        assert w.dtype == torch.float32 and w.is_contiguous() and tuple(w.shape) == (rows.shape[0], J)
+        out = torch.empty(rows.shape[0], n, dtype=torch.float32, device=U.device) if out is None else out
+        _sync_torch(U)
+        _check(lib().lsg_mlp_mix_rows(self._h, _ptr(U), n, J, _ptr(rows), _ptr(w), rows.shape[0], _ptr(out)))
+        return out
 
     def num_params(self):
         n = ctypes.c_int64()

@@ -46,11 +46,14 @@ __device__ inline float gelu_grad(float z) {
 
 // ------------------------------------------------------------------ GEMM
 // C[m][n] = sum_k A(m,k) B(k,n), A(m,k) = A[m*sam + k*sak], B(k,n) = B[k*sbk + n*sbn].
-// Workgroup tile 64 x 64, four waves of 32 x 32 each, K step 32 (two 32x32x16
-// MFMAs).  Tiles are staged in LDS with K contiguous (B transposed) so that every
-// lane reads its 8 BF16 operands with one 16-byte LDS load.  blockIdx.z splits K:
-// split z covers [z*kChunk, (z+1)*kChunk) and writes its partial sum to C + z*cSplit.
-constexpr int BM = 64, BN = 64, BK = 32, LDK = BK + 8;
+// Four waves in a 2 x 2 arrangement; each wave computes WM x WN tiles of 32 x 32 with
+// 32x32x16 BF16 MFMAs (FP32 accumulation), so the workgroup tile is 64 WM x 64 WN.
+// K steps of 64.  Tiles are staged in LDS with K contiguous (B transposed) so that every
+// lane reads its 8 BF16 operands with one 16-byte LDS load; the global loads put
+// consecutive lanes on consecutive addresses of the contiguous dimension.  blockIdx.z
+// splits K: split z covers [z*kChunk, (z+1)*kChunk) and writes its partial sum to
+// C + z*cSplit.
+constexpr int BK = 64, LDK = BK + 8;
 
 struct GemmArgs {
     const bf16* A;
@@ -67,74 +70,132 @@ struct GemmArgs {
     int act;
 };
 
+// One operand tile (T rows of the tile dimension x BK of K) staged into LDS as
+// dst[row * LDK + k].  Each thread owns fixed rows / k offsets for the whole K loop;
+// its loads walk a pointer by a fixed increment, so the per-load work is one add and
+// one bounds test.
+template <int T>
+struct TileLoader {
+    static constexpr int NL = T * BK / 256;   // loads per thread per K step
+    const bf16* base;        // address of this thread's first element at k = kBeg
+    int64_t step;            // address increment per K step of BK
+    int64_t inc;             // address increment between this thread's loads
+    int dst0, dinc;          // LDS index of the first load and its increment
+    int kof0, kinc;          // k offset (inside the step) of the first load and its increment
+    int r0, rinc, t0, nt;    // tile row of the first load, its increment, tile origin, rows
+    __device__ TileLoader(const bf16* P, int64_t st, int64_t sk, int t0_, int nt_, int kBeg) : t0(t0_), nt(nt_) {
+        const int t = threadIdx.x;
+        if (sk == 1) {       // K contiguous: lane -> k = t % BK, rows t / BK + (256 / BK) i
+            const int kk = t % BK;
+            r0 = t / BK;
+            rinc = 256 / BK;
+            base = P + (int64_t)(t0 + r0) * st + kBeg + kk;
+            step = BK;
+            inc = (int64_t)rinc * st;
+            dst0 = r0 * LDK + kk;
+            dinc = rinc * LDK;
+            kof0 = kk;
+            kinc = 0;
+        } else {             // tile dimension contiguous: lane -> row t % T, k = t / T + (256 / T) i
+            const int k0 = t / T;
+            r0 = t % T;
+            rinc = 0;
+            base = P + (int64_t)(t0 + r0) * st + (int64_t)(kBeg + k0) * sk;
+            step = (int64_t)BK * sk;
+            inc = (int64_t)(256 / T) * sk;
+            dst0 = r0 * LDK + k0;
+            dinc = 256 / T;
+            kof0 = k0;
+            kinc = 256 / T;
+        }
+    }
+    // global -> registers for the K step starting at p (kRem values of K left)
+    __device__ void fetch(bf16 (&v)[NL], const bf16* p, int kRem) const {
+        const bf16 zero = bf16(0.0f);
+#pragma unroll
+        for (int i = 0; i < NL; ++i) {
+            const bool in = t0 + r0 + rinc * i < nt && kof0 + kinc * i < kRem;
+            v[i] = in ? p[inc * i] : zero;
+        }
+    }
+    __device__ void store(bf16* __restrict__ lds, const bf16 (&v)[NL]) const {
+#pragma unroll
+        for (int i = 0; i < NL; ++i) lds[dst0 + dinc * i] = v[i];
+    }
+};
+
+template <int WM, int WN>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) bf16 As[BM * LDK];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[BN * LDK];
-    const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+    constexpr int TM = 64 * WM, TN = 64 * WN;
+    __shared__ __attribute__((aligned(16))) bf16 As[TM * LDK];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[TN * LDK];
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int kBeg = blockIdx.z * g.kChunk;
     const int kEnd = min(g.K, kBeg + g.kChunk);
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w >> 1, wn = w & 1;
-    floatx16 acc;
+    floatx16 acc[WM][WN];
 #pragma unroll
-    for (int r = 0; r < 16; ++r) acc[r] = 0.0f;
-    const bf16 zero = bf16(0.0f);
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.0f;
+    const TileLoader<TM> la(g.A, g.sam, g.sak, m0, g.M, kBeg);
+    const TileLoader<TN> lb(g.B, g.sbn, g.sbk, n0, g.N, kBeg);
+    const bf16* pa = la.base;
+    const bf16* pb = lb.base;
+    // the next K step is fetched into registers while this one is multiplied
+    bf16 ra[TileLoader<TM>::NL], rb[TileLoader<TN>::NL];
+    if (kBeg < kEnd) {
+        la.fetch(ra, pa, kEnd - kBeg);
+        lb.fetch(rb, pb, kEnd - kBeg);
+    }
     for (int k0 = kBeg; k0 < kEnd; k0 += BK) {
-        if (g.sak == 1) {          // K contiguous: 8 consecutive k of one row per thread
-            const int r = t >> 2, kk = (t & 3) * 8, m = m0 + r;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int k = k0 + kk + j;
-                As[r * LDK + kk + j] = (m < g.M && k < kEnd) ? g.A[(int64_t)m * g.sam + k] : zero;
-            }
-        } else {                   // M contiguous: 8 consecutive m of one k per thread
-            const int kk = t >> 3, r0 = (t & 7) * 8, k = k0 + kk;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int m = m0 + r0 + j;
-                As[(r0 + j) * LDK + kk] =
-                    (m < g.M && k < kEnd) ? g.A[(int64_t)m * g.sam + (int64_t)k * g.sak] : zero;
-            }
-        }
-        if (g.sbk == 1) {          // K contiguous
-            const int c = t >> 2, kk = (t & 3) * 8, n = n0 + c;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int k = k0 + kk + j;
-                Bs[c * LDK + kk + j] = (n < g.N && k < kEnd) ? g.B[k + (int64_t)n * g.sbn] : zero;
-            }
-        } else {                   // N contiguous
-            const int kk = t >> 3, c0 = (t & 7) * 8, k = k0 + kk;
-#pragma unroll
-            for (int j = 0; j < 8; ++j) {
-                const int n = n0 + c0 + j;
-                Bs[(c0 + j) * LDK + kk] =
-                    (n < g.N && k < kEnd) ? g.B[(int64_t)k * g.sbk + (int64_t)n * g.sbn] : zero;
-            }
-        }
+        la.store(As, ra);
+        lb.store(Bs, rb);
         __syncthreads();
+        if (k0 + BK < kEnd) {
+            pa += la.step;
+            pb += lb.step;
+            la.fetch(ra, pa, kEnd - k0 - BK);
+            lb.fetch(rb, pb, kEnd - k0 - BK);
+        }
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 16) {
-            const bf16x8 a =
-                *reinterpret_cast<const bf16x8*>(&As[(wm * 32 + (lane & 31)) * LDK + ks + 8 * (lane >> 5)]);
-            const bf16x8 b =
-                *reinterpret_cast<const bf16x8*>(&Bs[(wn * 32 + (lane & 31)) * LDK + ks + 8 * (lane >> 5)]);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            bf16x8 a[WM], b[WN];
+#pragma unroll
+            for (int i = 0; i < WM; ++i)
+                a[i] = *reinterpret_cast<const bf16x8*>(
+                    &As[(wm * 32 * WM + i * 32 + (lane & 31)) * LDK + ks + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int j = 0; j < WN; ++j)
+                b[j] = *reinterpret_cast<const bf16x8*>(
+                    &Bs[(wn * 32 * WN + j * 32 + (lane & 31)) * LDK + ks + 8 * (lane >> 5)]);
+#pragma unroll
+            for (int i = 0; i < WM; ++i)
+#pragma unroll
+                for (int j = 0; j < WN; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
     // 32x32 result layout: lane holds column (lane & 31), rows 8*(r/4) + 4*(lane/32) + r%4
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int m = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        const int n = n0 + wn * 32 + (lane & 31);
-        if (m < g.M && n < g.N) {
-            float v = acc[r];
-            if (g.bias) v += g.bias[n];
-            if (g.C) g.C[(int64_t)blockIdx.z * g.cSplit + (int64_t)m * g.ldc + n] = v;
-            if (g.Cb) g.Cb[(int64_t)m * g.ldcb + n] = bf16(g.act ? gelu_f(v) : v);
-        }
-    }
+    for (int i = 0; i < WM; ++i)
+#pragma unroll
+        for (int j = 0; j < WN; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int m = m0 + wm * 32 * WM + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                const int n = n0 + wn * 32 * WN + j * 32 + (lane & 31);
+                if (m < g.M && n < g.N) {
+                    float v = acc[i][j][r];
+                    if (g.bias) v += g.bias[n];
+                    if (g.C) g.C[(int64_t)blockIdx.z * g.cSplit + (int64_t)m * g.ldc + n] = v;
+                    if (g.Cb) g.Cb[(int64_t)m * g.ldcb + n] = bf16(g.act ? gelu_f(v) : v);
+                }
+            }
 }
 
 // sum of the split-K partials in split order (+ bias, GELU, BF16 copy); C is M x N dense
@@ -190,13 +251,35 @@ __global__ void gelu_back_kernel(const float* __restrict__ da, const float* __re
     }
 }
 
-// out[n] = sum over rows m of X[m][n], rows in order
-__global__ void colsum_kernel(const float* __restrict__ X, int M, int N, float* __restrict__ out) {
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
+// out[n] = sum over rows m of X[m][n]: 64 columns per workgroup, 4 row slices of
+// consecutive rows per column summed in order, then the slices in order (fixed order,
+// deterministic)
+__global__ void __launch_bounds__(256) colsum_kernel(const float* __restrict__ X, int M, int N,
+                                                     float* __restrict__ out) {
+    __shared__ float part[4][64];
+    const int c = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    const int n = blockIdx.x * 64 + c;
+    const int rows = (M + 3) / 4, r0 = sl * rows, r1 = min(M, r0 + rows);
     float s = 0.0f;
-    for (int m = 0; m < M; ++m) s += X[(int64_t)m * N + n];
-    out[n] = s;
+    if (n < N)
+        for (int m = r0; m < r1; ++m) s += X[(int64_t)m * N + n];
+    part[sl][c] = s;
+    __syncthreads();
+    if (sl == 0 && n < N) out[n] = ((part[0][c] + part[1][c]) + part[2][c]) + part[3][c];
+}
+
+// out[b][i] = sum_j w[b*J + j] U[(rows[b]*J + j)*n + i], j in order
+__global__ void __launch_bounds__(256) mix_rows_kernel(const float* __restrict__ U, int64_t n, int J,
+                                                       const int64_t* __restrict__ rows, const float* __restrict__ w,
+                                                       float* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t b = blockIdx.y;
+    const float* u = U + rows[b] * J * n + i;
+    const float* wb = w + b * J;
+    float s = 0.0f;
+    for (int j = 0; j < J; ++j) s = fmaf(wb[j], u[(int64_t)j * n], s);
+    out[b * n + i] = s;
 }
 
 // Adam with bias correction; refreshes the BF16 copy of the weights
@@ -258,24 +341,37 @@ constexpr int SPLIT_K_TARGET = 4096;   // K per split for long reductions
 
 int gemm(lsg_mlp* m, const bf16* A, int64_t sam, int64_t sak, const bf16* B, int64_t sbk, int64_t sbn, int M, int N,
          int K, const float* bias, float* C, bf16* Cb, int act) {
-    const int splits = K > 2 * SPLIT_K_TARGET ? (K + SPLIT_K_TARGET - 1) / SPLIT_K_TARGET : 1;
+    // 64 x 64 workgroup tiles: measured faster here than 128 x 128 (2 x 2 MFMA tiles per
+    // wave) and than K steps of 128
+    constexpr int TM = 64, TN = 64;
+    const int64_t tiles = (int64_t)((M + TM - 1) / TM) * ((N + TN - 1) / TN);
+    int splits = 1;
+    if (K > 2 * SPLIT_K_TARGET) splits = (K + SPLIT_K_TARGET - 1) / SPLIT_K_TARGET;
+    if (tiles * splits < 256 && K >= 4 * BK) {
+        // few workgroups: split K further so that about 256 run, at least 2 K steps each
+        splits = (int)std::max<int64_t>(splits, std::min<int64_t>((256 + tiles - 1) / tiles, K / (2 * BK)));
+    }
     GemmArgs g{};
     g.A = A; g.sam = sam; g.sak = sak;
     g.B = B; g.sbk = sbk; g.sbn = sbn;
     g.M = M; g.N = N; g.K = K;
     g.act = act;
     const dim3 blk(256);
+    auto launch = [&](int zs) {
+        const dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, zs);
+        hipLaunchKernelGGL((gemm_bf16_kernel<1, 1>), grid, blk, 0, m->s, g);
+    };
     if (splits == 1) {
         g.kChunk = ((K + BK - 1) / BK) * BK;
         g.bias = bias; g.C = C; g.ldc = N; g.cSplit = 0; g.Cb = Cb; g.ldcb = N;
-        hipLaunchKernelGGL(gemm_bf16_kernel, dim3((N + BN - 1) / BN, (M + BM - 1) / BM, 1), blk, 0, m->s, g);
+        launch(1);
     } else {
-        const int64_t need = (int64_t)splits * M * N * 4;
-        if ((int64_t)m->part.bytes < need) MLPCHK(m->part.alloc(need));
         g.kChunk = ((K / splits + BK - 1) / BK + 1) * BK;
         const int used = (K + g.kChunk - 1) / g.kChunk;
+        const int64_t need = (int64_t)used * M * N * 4;
+        if ((int64_t)m->part.bytes < need) MLPCHK(m->part.alloc(need));
         g.bias = nullptr; g.C = m->part.as<float>(); g.ldc = N; g.cSplit = (int64_t)M * N; g.Cb = nullptr;
-        hipLaunchKernelGGL(gemm_bf16_kernel, dim3((N + BN - 1) / BN, (M + BM - 1) / BM, used), blk, 0, m->s, g);
+        launch(used);
         hipLaunchKernelGGL(split_reduce_kernel, dim3(gridFor((int64_t)M * N)), blk, 0, m->s, m->part.as<float>(), used,
                            (int64_t)M * N, M, N, bias, C, Cb, act);
     }
@@ -342,7 +438,7 @@ int gradients(lsg_mlp* m, const float* x, const float* tgt, int batch, double* l
         // dW = A_prev^T dZ  (in x out, K = batch)
         rc = gemm(m, aPrev, 1, L.in, dZb, L.out, 1, L.in, L.out, batch, nullptr, L.gW.as<float>(), nullptr, 0);
         if (rc) return rc;
-        hipLaunchKernelGGL(colsum_kernel, dim3((L.out + 255) / 256), dim3(256), 0, m->s, dZf, batch, L.out,
+        hipLaunchKernelGGL(colsum_kernel, dim3((L.out + 63) / 64), dim3(256), 0, m->s, dZf, batch, L.out,
                            L.gb.as<float>());
         if (l == 0) break;
         // dA_prev = dZ W^T  (batch x in, K = out)
@@ -510,6 +606,17 @@ int lsg_mlp_set_params(lsg_mlp* m, const float* host) {
 int lsg_mlp_get_grads(const lsg_mlp* m, float* host) {
     if (!m || !host) return lsg::api_fail(LSG_ERR_ARG, "null argument");
     return mlpCopy(m, host, true, true);
+}
+
+int lsg_mlp_mix_rows(lsg_mlp* m, const float* U, int64_t n, int J, const int64_t* rows, const float* w, int batch,
+                     float* out) {
+    if (!m || !U || !rows || !w || !out || n < 1 || J < 1 || batch < 1 || batch > 65535)
+        return lsg::api_fail(LSG_ERR_ARG, "bad mix_rows arguments");
+    hipLaunchKernelGGL(mix_rows_kernel, dim3((unsigned)((n + 255) / 256), batch), dim3(256), 0, m->s, U, n, J, rows, w,
+                       out);
+    MLPCHK(hipGetLastError());
+    MLPCHK(hipStreamSynchronize(m->s));
+    return LSG_OK;
 }
 
 int lsg_mlp_destroy(lsg_mlp* m) {

@@ -5,6 +5,9 @@
         per-kernel table (calls, total ms, average / max us, share); with --kernel the
         last N dispatches whose name matches RE are also summed and divided into
         groups of K dispatches (K = apply launches of one matvec -> time per matvec).
+    python3 -m loadsg_amd.rocpd_summary sequence DB --kernel RE --tail N
+        average duration per position of the dispatch sequences that start at each
+        dispatch matching RE (e.g. one MLP training step), over the last N sequences.
     python3 -m loadsg_amd.rocpd_summary pmc DB --kernel RE --tail N --group K --calib RE
         per-counter sum over the last N dispatches matching --kernel, per group of K,
         and the counter of the last dispatch matching --calib (the copy kernel).
@@ -44,6 +47,33 @@ def trace(db, kernel=None, tail=0, group=1, top=20):
                                                        sum(d) / 1e6 / ng, group, ng))
 
 
+def sequence(db, start, tail):
+    """Split the dispatch list at every dispatch whose name matches `start` and average the
+    duration of each position over the last `tail` sequences that have the most common
+    length (one training step = one sequence)."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, start, end from kernels order by start").fetchall()
+    seqs, cur = [], []
+    for name, s, e in rows:
+        if re.search(start, name) and cur:
+            seqs.append(cur)
+            cur = []
+        cur.append((_short(name), e - s))
+    seqs.append(cur)
+    lengths = defaultdict(int)
+    for q in seqs:
+        lengths[len(q)] += 1
+    L = max(lengths, key=lambda k: lengths[k])
+    sel = [q for q in seqs if len(q) == L][-tail:] if tail else [q for q in seqs if len(q) == L]
+    tot = 0.0
+    print("%d sequences of %d dispatches (starting at %r); average per position:" % (len(sel), L, start))
+    for i in range(L):
+        v = sum(q[i][1] for q in sel) / len(sel) / 1e3
+        tot += v
+        print("%3d %-58s %10.2f us" % (i, sel[0][i][0][:58], v))
+    print("sum %.2f us per sequence" % tot)
+
+
 def pmc(db, kernel, tail, group, calib):
     con = sqlite3.connect(db)
     rows = con.execute("select dispatch_id, kernel_name, counter_name, value from counters_collection "
@@ -71,7 +101,7 @@ def pmc(db, kernel, tail, group, calib):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=("trace", "pmc"))
+    ap.add_argument("mode", choices=("trace", "pmc", "sequence"))
     ap.add_argument("db")
     ap.add_argument("--kernel")
     ap.add_argument("--tail", type=int, default=0)
@@ -81,6 +111,8 @@ def main():
     a = ap.parse_args()
     if a.mode == "trace":
         trace(a.db, a.kernel, a.tail, a.group, a.top)
+    elif a.mode == "sequence":
+        sequence(a.db, a.kernel, a.tail)
     else:
         pmc(a.db, a.kernel, a.tail, a.group, a.calib)
 

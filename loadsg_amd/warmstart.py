@@ -90,19 +90,20 @@ class WarmStart:
         xis = np.asarray(xis, dtype=np.float64)
         S, J = U.shape[0], U.shape[1]
         # output scale: RMS of D u over random eta
-        Us = U * self.D
+        Us = (U * self.D).contiguous()
         probe = torch.randn(min(S, 256), 1, J, device="cuda") @ Us[:min(S, 256)]
         self.u_scale = float(torch.sqrt(torch.mean(probe.double() ** 2)))
         Us /= self.u_scale
         t0 = time.perf_counter()
         losses = []
+        tgt = torch.empty(batch, Us.shape[2], dtype=torch.float32, device="cuda")
         for it in range(steps):
             idx = rng.integers(0, S, batch)
             eta = rng.standard_normal((batch, J))
-            tgt = (torch.tensor(eta, dtype=torch.float32, device="cuda").unsqueeze(1) @ Us[idx]).squeeze(1)
+            self.net.mix_rows(Us, torch.tensor(idx, dtype=torch.int64, device="cuda"),
+                              torch.tensor(eta, dtype=torch.float32, device="cuda"), out=tgt)
             c = 0.5 * (1 + np.cos(np.pi * it / steps))
-            losses.append(self.net.train_step(self.features(xis[idx], eta), tgt.contiguous(),
-                                              lr_final + (lr - lr_final) * c))
+            losses.append(self.net.train_step(self.features(xis[idx], eta), tgt, lr_final + (lr - lr_final) * c))
         torch.cuda.synchronize()
         self.stats.update({"train_steps": steps, "train_batch": batch, "train_s": time.perf_counter() - t0,
                            "train_loss_first": losses[0], "train_loss_last": float(np.mean(losses[-50:]))})

@@ -63,7 +63,8 @@ def _rel(a, b):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("n_in,width,hidden,n_out,batch", [(24, 64, 2, 1000, 16), (24, 128, 3, 20000, 37),
-                                                           (7, 96, 1, 130, 70)])
+                                                           (7, 96, 1, 130, 70), (24, 512, 3, 61183, 256),
+                                                           (24, 192, 2, 333, 130)])
 def test_forward_and_gradients_vs_restatement(n_in, width, hidden, n_out, batch):
     import torch
     print("mlp", n_in, width, hidden, n_out, batch, flush=True)
@@ -127,3 +128,17 @@ def test_adam_training_reduces_loss_and_params_roundtrip():
     m2.set_params(p)
     assert np.array_equal(m2.params(), p)
     assert np.array_equal(m2.forward(xd).cpu().numpy(), m.forward(xd).cpu().numpy())
+
+
+@pytest.mark.gpu
+def test_mix_rows_matches_batched_product():
+    """Training targets sum_j w[b, j] U[rows[b], j] against a float64 product."""
+    import torch
+    rng = np.random.default_rng(21)
+    S, J, n, B = 9, 8, 1237, 33
+    U = torch.tensor(rng.standard_normal((S, J, n)), dtype=torch.float32, device="cuda")
+    rows = torch.tensor(rng.integers(0, S, B), dtype=torch.int64, device="cuda")
+    w = torch.tensor(rng.standard_normal((B, J)), dtype=torch.float32, device="cuda")
+    out = _mlp(4, 64, 1, 10, seed=1).mix_rows(U, rows, w)
+    ref = (w.double().unsqueeze(1) @ U.double()[rows]).squeeze(1)
+    assert float((out.double() - ref).abs().max() / ref.abs().max()) < 1e-6
