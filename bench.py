@@ -413,6 +413,17 @@ def main():
     dt0 = time.perf_counter() - t0
     log("zero-start mixed-precision solve: %.3f s" % dt0)
 
+    # ---- batched FP64 PCG from zero (batching alone, without mixed precision or warm start)
+    x.zero_()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    infos64 = L.CG.solveHomogen(1e-8, x, f, op64, stop_rule=L.STOP_REL)
+    torch.cuda.synchronize()
+    dt64 = time.perf_counter() - t0
+    if B == 1:
+        infos64 = [infos64]
+    log("batched FP64 PCG: %.3f s" % dt64)
+
     # ---- timed: batched mixed-precision solves (warm-started when trained)
     for _ in range(args.warmup):
         solve_step(ws is not None)
@@ -495,6 +506,8 @@ def main():
             "fp64_plain_cg_solves_per_s": fp64_solves,
             "fp64_plain_cg_iterations": info1["iterations"],
             "speedup_vs_fp64_plain": (B / dt) / fp64_solves,
+            "fp64_batched_cg_solves_per_s": B / dt64,
+            "fp64_batched_cg_iterations_mean": float(np.mean([i["iterations"] for i in infos64])),
             "mixed_inner_iterations_mean": float(np.mean([i["iterations"] for i in infos])),
             "mixed_outer_sweeps_mean": float(np.mean([i["outer"] for i in infos])),
             "zero_start_solves_per_s": B / dt0,
