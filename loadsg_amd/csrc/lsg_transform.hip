@@ -4,7 +4,7 @@
 // applications/H1_Norm.cpp).
 //
 // Every step works on the dense box G_T of one depth T (the level-T full grid, all
-// points of depth <= T), staged in LDS by one workgroup per (depth, instance):
+// points of depth <= T), staged in LDS by one workgroup per (depth, group of instances):
 //   * hierarchization / dehierarchization of a sparse-grid vector: the surplus of a
 //     point of depth T only involves points of G_T, so each depth box is
 //     (de)hierarchized on its own and its depth-T entries are written back;
