@@ -127,8 +127,10 @@ struct TileLoader {
 template <int WM, int WN>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     constexpr int TM = 64 * WM, TN = 64 * WN;
-    __shared__ __attribute__((aligned(16))) bf16 As[TM * LDK];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[TN * LDK];
+    // two LDS buffers per operand: step k is multiplied from one while step k + 1 is
+    // stored into the other, one barrier per K step
+    __shared__ __attribute__((aligned(16))) bf16 As2[2][TM * LDK];
+    __shared__ __attribute__((aligned(16))) bf16 Bs2[2][TN * LDK];
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int kBeg = blockIdx.z * g.kChunk;
     const int kEnd = min(g.K, kBeg + g.kChunk);
@@ -150,17 +152,21 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     if (kBeg < kEnd) {
         la.fetch(ra, pa, kEnd - kBeg);
         lb.fetch(rb, pb, kEnd - kBeg);
+        la.store(As2[0], ra);
+        lb.store(Bs2[0], rb);
     }
+    __syncthreads();
+    int cur = 0;
     for (int k0 = kBeg; k0 < kEnd; k0 += BK) {
-        la.store(As, ra);
-        lb.store(Bs, rb);
-        __syncthreads();
-        if (k0 + BK < kEnd) {
+        const bool more = k0 + BK < kEnd;
+        if (more) {
             pa += la.step;
             pb += lb.step;
             la.fetch(ra, pa, kEnd - k0 - BK);
             lb.fetch(rb, pb, kEnd - k0 - BK);
         }
+        const bf16* As = As2[cur];
+        const bf16* Bs = Bs2[cur];
 #pragma unroll
         for (int ks = 0; ks < BK; ks += 16) {
             bf16x8 a[WM], b[WN];
@@ -178,7 +184,12 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
                 for (int j = 0; j < WN; ++j)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
+        if (more) {
+            la.store(As2[cur ^ 1], ra);
+            lb.store(Bs2[cur ^ 1], rb);
+        }
         __syncthreads();
+        cur ^= 1;
     }
     // 32x32 result layout: lane holds column (lane & 31), rows 8*(r/4) + 4*(lane/32) + r%4
 #pragma unroll
