@@ -139,6 +139,11 @@ int lsg_nodal_by_prew(lsg_op* op, const void* prew, void* u, int layout);
  * (applications/H1_Norm.cpp:8-78).  out holds lsg_op_batch() values. */
 int lsg_energy_norm(lsg_op* op, const void* u, double* out, int layout);
 
+/* Vector norms per instance: out[2b] = max |x_i| over all stored points (L_infty,
+ * applications/norms.h:12-52), out[2b+1] = sqrt(sum x_i^2 / dofs) over the active points
+ * (L_2, norms.h:54-100).  Any grid and precision; out holds 2 lsg_op_batch() values. */
+int lsg_vector_norms(lsg_op* op, const void* x, double* out, int layout);
+
 /* Device timing of the last lsg_matvec: out[0]=total ms, out[1]=apply-kernel ms,
  * out[2]=number of apply launches. */
 int lsg_last_timing(const lsg_op* op, double* out, int nout);

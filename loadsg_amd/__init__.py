@@ -83,6 +83,7 @@ def lib():
             "lsg_prew_by_nodal": (i32, [vp, vp, vp, i32]),
             "lsg_nodal_by_prew": (i32, [vp, vp, vp, i32]),
             "lsg_energy_norm": (i32, [vp, vp, vp, i32]),
+            "lsg_vector_norms": (i32, [vp, vp, vp, i32]),
             "lsg_op_set_block": (i32, [vp, i32, i32]),
             "lsg_op_destroy": (i32, [vp]),
             "lsg_mlp_create": (i32, [i32, i32, i32, i64, ctypes.c_uint64, ctypes.POINTER(vp)]),
@@ -312,6 +313,18 @@ class MatrixVectorHomogen:
             _check(lib().lsg_energy_norm(self._h, _ptr(x), _ptr(out), LSG_HOST_SLOT))
         return out if self.batch > 1 else float(out[0])
 
+    def vector_norms(self, x):
+        """(L_infty, L_2) of x per instance (applications/norms.h:12-100): max |x| over all
+        points and sqrt(sum x^2 / dofs) over the active ones."""
+        out = np.zeros((self.batch, 2))
+        if hasattr(x, "data_ptr"):
+            _sync_torch(x)
+            _check(lib().lsg_vector_norms(self._h, _ptr(x), _ptr(out), LSG_DEVICE))
+        else:
+            v = np.ascontiguousarray(x, dtype=np.float64)
+            _check(lib().lsg_vector_norms(self._h, _ptr(v), _ptr(out), LSG_HOST_SLOT))
+        return out if self.batch > 1 else out[0]
+
     def last_timing(self):
         out = np.zeros(3)
         _check(lib().lsg_last_timing(self._h, _ptr(out), 3))
@@ -337,6 +350,16 @@ def calcPrewByNodal(u, op):
 def calcNodalByPrew(prew, op):
     """Nodal values of a prewavelet expansion (BasisTransformations.cpp:670-707)."""
     return op.nodal_by_prew(prew)
+
+
+def L_infty(x, op):
+    """max |x_i| over the stored points (applications/norms.h:12-52), on the device."""
+    return op.vector_norms(x)[..., 0]
+
+
+def L_2(x, op):
+    """sqrt(sum x_i^2 / dofs) over the active points (applications/norms.h:54-100)."""
+    return op.vector_norms(x)[..., 1]
 
 
 class H1:

@@ -1200,6 +1200,33 @@ int lsg_energy_norm(lsg_op* op, const void* u, double* out, int layout) {
     return LSG_OK;
 }
 
+int lsg_vector_norms(lsg_op* op, const void* x, double* out, int layout) {
+    if (!op || !x || !out) return fail(LSG_ERR_ARG, "null argument");
+    const Grid& g = op->grid->g;
+    const void* xv = x;
+    if (layout != LSG_DEVICE) {
+        int rc = upload(op, (const double*)x, op->xs.p);
+        if (rc) return rc;
+        xv = op->xs.p;
+    }
+    int64_t dofs = 0;
+    for (uint8_t a : g.active) dofs += a ? 1 : 0;
+    if (g.active.empty()) dofs = g.n;
+    const uint8_t* act = op->sparse ? op->activeDev.as<uint8_t>() : nullptr;
+    DevMem res;
+    HIPCHK(res.alloc((size_t)op->batch * 2 * sizeof(double)));
+    if (op->prec == 64)
+        launch_vec_norms<double>((const double*)xv, g.n, act, (double)std::max<int64_t>(dofs, 1), res.as<double>(),
+                                 op->batch, op->stream);
+    else
+        launch_vec_norms<float>((const float*)xv, g.n, act, (double)std::max<int64_t>(dofs, 1), res.as<double>(),
+                                op->batch, op->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(out, res.p, (size_t)op->batch * 2 * sizeof(double), hipMemcpyDeviceToHost, op->stream));
+    HIPCHK(hipStreamSynchronize(op->stream));
+    return LSG_OK;
+}
+
 int lsg_last_timing(const lsg_op* op, double* out, int nout) {
     if (!op || !out || !op->timed) return fail(LSG_ERR_ARG, "no timed matvec");
     HIPCHK(hipEventSynchronize(op->ev[3]));

@@ -239,6 +239,37 @@ __global__ void __launch_bounds__(256) tr_energy_kernel(const double* __restrict
     if (threadIdx.x == 0) out[b] = sqrt(red[0]);
 }
 
+// out[2b] = max_i |x_i| over all stored points (L_infty, applications/norms.h:12-52),
+// out[2b+1] = sqrt(sum x_i^2 / dofs) over the active points (L_2, norms.h:54-100); one
+// workgroup per instance, fixed-order reductions
+template <typename V>
+__global__ void __launch_bounds__(256) vec_norms_kernel(const V* __restrict__ x, int64_t n,
+                                                        const uint8_t* __restrict__ active, double dofs,
+                                                        double* __restrict__ out) {
+    __shared__ double mx[256], sq[256];
+    const int64_t b = blockIdx.x;
+    double m = 0.0, q = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const double v = (double)x[b * n + i];
+        m = fmax(m, fabs(v));
+        if (!active || active[i]) q += v * v;
+    }
+    mx[threadIdx.x] = m;
+    sq[threadIdx.x] = q;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            mx[threadIdx.x] = fmax(mx[threadIdx.x], mx[threadIdx.x + w]);
+            sq[threadIdx.x] += sq[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * b] = mx[0];
+        out[2 * b + 1] = sqrt(sq[0] / dofs);
+    }
+}
+
 int boxThreads(int64_t vals) { return vals <= 128 ? 64 : (vals <= 512 ? 128 : 256); }
 
 }  // namespace
@@ -278,6 +309,15 @@ void launch_tr_gather(const int32_t* rowDev, const int64_t* rowPtr, const int32_
     const dim3 grid((unsigned)((nrows + 255) / 256), batch);
     tr_gather_kernel<<<grid, 256, 0, s>>>(rowDev, rowPtr, col, nrows, scratch, v, n, nml, sub);
 }
+
+template <typename V>
+void launch_vec_norms(const V* x, int64_t n, const uint8_t* active, double dofs, double* out, int batch,
+                      hipStream_t s) {
+    if (batch <= 0) return;
+    vec_norms_kernel<V><<<batch, 256, 0, s>>>(x, n, active, dofs, out);
+}
+template void launch_vec_norms<double>(const double*, int64_t, const uint8_t*, double, double*, int, hipStream_t);
+template void launch_vec_norms<float>(const float*, int64_t, const uint8_t*, double, double*, int, hipStream_t);
 
 void launch_tr_energy(const double* x, const double* y, int64_t n, double* out, int batch, hipStream_t s) {
     if (batch <= 0) return;

@@ -157,6 +157,30 @@ def test_poisson_driver_known_answers():
     assert abs(l2 + h1 - K["h1_plus_l2"]) < 1e-5 * K["h1_plus_l2"]
 
 
+@pytest.mark.parametrize("where", ["regular", "adaptive"])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_vector_norms(where, prec):
+    """L_infty over all stored points, L_2 over the active ones (norms.h:12-100)."""
+    print("vector norms", where, prec, flush=True)
+    if where == "regular":
+        grid = L.AdaptiveSparseGrid.regular(4, 5)
+        act = np.ones(grid.size, dtype=bool)
+    else:
+        gv = load("d3_adaptive_cfg3")
+        grid = L.AdaptiveSparseGrid(gv["idx"], gv["active"])
+        act = gv["active"].astype(bool)
+        assert not act.all()
+    op = L.MatrixVectorHomogen(grid, L.Poisson(), prec)
+    x = np.random.default_rng(4).standard_normal(grid.size)
+    x[np.argmax(~act) if not act.all() else 0] = 7.5     # largest entry (a hanging node when there is one)
+    if prec == 32:
+        x = x.astype(np.float32).astype(np.float64)
+    linf, l2 = op.vector_norms(x)
+    assert linf == np.max(np.abs(x))
+    assert abs(l2 - np.sqrt(np.sum(x[act] ** 2) / act.sum())) < 1e-13 * l2
+    assert L.L_infty(x, op) == linf and L.L_2(x, op) == l2
+
+
 def test_unsupported_inputs_raise():
     print("transform errors", flush=True)
     gv = load("d3_adaptive_cfg3")
