@@ -115,7 +115,7 @@ namespace {
 // Device copies of the basis-transformation tables and their work vectors.
 struct TrDev {
     TrTables host;
-    DevMem items, gidx, list, rowDev, rowPtr, col, fac;
+    DevMem items, gidx, list, rowDev, rowPtr, col, fac, poles;
     DevMem hier, work, work2, scratch, norms;
 };
 }  // namespace
@@ -612,6 +612,7 @@ int ensureTransform(lsg_op* op) {
     HIPCHK(up(tr->rowPtr, h.rowPtr.data(), h.rowPtr.size() * 8));
     HIPCHK(up(tr->col, h.col.data(), h.col.size() * 4));
     HIPCHK(up(tr->fac, h.fac.data(), h.fac.size() * 8));
+    HIPCHK(up(tr->poles, h.poles.data(), h.poles.size() * 4));
     const size_t nb = (size_t)g.n * op->batch * 8;
     HIPCHK(tr->hier.alloc(nb));
     HIPCHK(tr->work.alloc(nb));
@@ -639,6 +640,20 @@ void trGather(lsg_op* op, size_t group, double* v, int sub) {
                      t.scratch.as<double>(), v, g.n, g.nml, sub, op->batch, op->stream);
 }
 
+// CalcHierarchicalBasis as d point-parallel passes (one per dimension, in dimension
+// order), ping-ponging between out and the scratch so that the last pass writes out
+void trHierPoles(lsg_op* op, const double* in, double* out) {
+    const Grid& g = op->grid->g;
+    TrDev& t = *op->tr;
+    const int d = g.d;
+    const double* src = in;
+    for (int i = 0; i < d; ++i) {
+        double* dst = ((d - 1 - i) % 2 == 0) ? out : t.scratch.as<double>();
+        launch_tr_pole(t.poles.as<int32_t>() + (size_t)i * g.n * 2, src, dst, g.n, op->batch, op->stream);
+        src = dst;
+    }
+}
+
 // Device vectors (all instances, device order): what 0 hierarchize, 1 dehierarchize,
 // 2 prewavelets from nodal values, 3 nodal values from prewavelets.
 int transformDev(lsg_op* op, int what, const double* in, double* out) {
@@ -648,11 +663,11 @@ int transformDev(lsg_op* op, int what, const double* in, double* out) {
     const int nd = (int)t.host.items.size();
     const int64_t mb = t.host.maxBox;
     switch (what) {
-        case 0: trBox(op, TR_HIER, nullptr, nd, mb, in, out); break;
+        case 0: trHierPoles(op, in, out); break;
         case 1: trBox(op, TR_DEHIER, nullptr, nd, mb, in, out); break;
         case 2: {   // calcPrewByNodal (BasisTransformations.cpp:486-545)
             double* hier = t.hier.as<double>();
-            trBox(op, TR_HIER, nullptr, nd, mb, in, hier);
+            trHierPoles(op, in, hier);
             for (size_t l = 0; l < t.host.levels.size(); ++l) {
                 trBox(op, TR_LEVEL, t.list.as<int32_t>() + t.host.levels[l][0], t.host.levels[l][1],
                       t.host.levelMaxBox[l], hier, out);

@@ -902,6 +902,27 @@ int buildTransform(const Grid& g, TrTables& tr, std::string& err) {
             tr.gidx[it.box_off + p] = (int32_t)(g.blkOff[f->second] + loc);
         }
     }
+    // poles: the neighbours of a point of depth T inside its own box G_T
+    tr.poles.assign((size_t)d * g.n * 2, -1);
+    for (int t = 0; t < nd; ++t) {
+        const TrItem& it = tr.items[t];
+        for (int32_t p = 0; p < it.total; ++p) {
+            int32_t r = p;
+            bool own = true;
+            int k[MAXD];
+            for (int i = 0; i < d; ++i) {
+                k[i] = r % it.n[i];
+                r /= it.n[i];
+                if (k[i] & 1) own = false;   // 0-based even = depth T_i
+            }
+            if (!own) continue;
+            const int32_t dev = tr.gidx[it.box_off + p];
+            for (int i = 0; i < d; ++i) {
+                if (k[i] >= 1) tr.poles[((size_t)i * g.n + dev) * 2] = tr.gidx[it.box_off + p - it.S[i]];
+                if (k[i] + 1 < it.n[i]) tr.poles[((size_t)i * g.n + dev) * 2 + 1] = tr.gidx[it.box_off + p + it.S[i]];
+            }
+        }
+    }
     // depths grouped by |T|_1, finest first
     std::map<int, std::vector<int32_t>, std::greater<int>> bySum;
     for (int t = 0; t < nd; ++t) {

@@ -189,6 +189,10 @@ struct TrTables {
     std::vector<int32_t> levelList;            // depth ids, |T|_1 descending
     std::vector<std::array<int32_t, 2>> levels;   // (first in levelList, count) per |T|_1, descending
     std::vector<int64_t> levelMaxBox;          // largest |G_T| per level
+    // hierarchical parents (poles) of every point along every dimension, device indices:
+    // poles[(i * n + p) * 2 + {0, 1}] = left / right pole of point p along dim i, -1 on
+    // the boundary
+    std::vector<int32_t> poles;
     // Scatter-add of box results onto points, as a gather per point: CSR whose rows are
     // device indices and whose entries are scratch positions.  Row groups: one per
     // |T|_1 (same order as levels), then one over all depths.

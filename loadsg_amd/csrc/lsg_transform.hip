@@ -209,6 +209,20 @@ __global__ void __launch_bounds__(256) tr_box_kernel(const TrItem* __restrict__ 
     }
 }
 
+// one dimension of a hierarchization, point by point: out = in - (in[left pole] +
+// in[right pole]) / 2 (0 on the boundary), the arithmetic of hierLine
+__global__ void __launch_bounds__(256) tr_pole_kernel(const int32_t* __restrict__ poles, const double* __restrict__ in,
+                                                      double* __restrict__ out, int64_t n) {
+    const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= n) return;
+    const int64_t b = blockIdx.y;
+    const double* x = in + b * n;
+    const int32_t lp = poles[2 * p], rp = poles[2 * p + 1];
+    const double l = lp >= 0 ? x[lp] : 0.0;
+    const double r = rp >= 0 ? x[rp] : 0.0;
+    out[b * n + p] = x[p] - 0.5 * (l + r);
+}
+
 // v[rowDev[r]] = (sub ? v[rowDev[r]] : 0) -/+ sum of the row's scratch entries, in row order
 __global__ void tr_gather_kernel(const int32_t* __restrict__ rowDev, const int64_t* __restrict__ rowPtr,
                                  const int32_t* __restrict__ col, int64_t nrows, const double* __restrict__ scratch,
@@ -301,6 +315,11 @@ void launch_tr_box(int mode, const TrItem* items, const int32_t* list, int nitem
                                                             batch);
             break;
     }
+}
+
+void launch_tr_pole(const int32_t* poles, const double* in, double* out, int64_t n, int batch, hipStream_t s) {
+    if (n <= 0 || batch <= 0) return;
+    tr_pole_kernel<<<dim3((unsigned)((n + 255) / 256), batch), 256, 0, s>>>(poles, in, out, n);
 }
 
 void launch_tr_gather(const int32_t* rowDev, const int64_t* rowPtr, const int32_t* col, int64_t nrows,
