@@ -115,7 +115,7 @@ namespace {
 // Device copies of the basis-transformation tables and their work vectors.
 struct TrDev {
     TrTables host;
-    DevMem items, gidx, list, rowDev, rowPtr, col, fac, poles;
+    DevMem items, gidx, list, rowDev, rowPtr, col, fac, poles, byLevel;
     DevMem hier, work, work2, scratch, norms;
 };
 }  // namespace
@@ -613,6 +613,7 @@ int ensureTransform(lsg_op* op) {
     HIPCHK(up(tr->col, h.col.data(), h.col.size() * 4));
     HIPCHK(up(tr->fac, h.fac.data(), h.fac.size() * 8));
     HIPCHK(up(tr->poles, h.poles.data(), h.poles.size() * 4));
+    HIPCHK(up(tr->byLevel, h.byLevel.data(), h.byLevel.size() * 4));
     const size_t nb = (size_t)g.n * op->batch * 8;
     HIPCHK(tr->hier.alloc(nb));
     HIPCHK(tr->work.alloc(nb));
@@ -654,6 +655,21 @@ void trHierPoles(lsg_op* op, const double* in, double* out) {
     }
 }
 
+// CalcNodalBasis in place on v: per dimension (in dimension order), one launch per depth
+// along that dimension, coarse to fine
+int trDehierPoles(lsg_op* op, double* v) {
+    const Grid& g = op->grid->g;
+    TrDev& t = *op->tr;
+    for (int i = 0; i < g.d; ++i) {
+        const int64_t* off = &t.host.lvlOff[(size_t)i * (MAXLEV + 2)];
+        for (int l = 2; l <= MAXLEV; ++l)
+            launch_tr_dehier(t.byLevel.as<int32_t>() + (size_t)i * g.n + off[l], off[l + 1] - off[l],
+                             t.poles.as<int32_t>() + (size_t)i * g.n * 2, v, g.n, op->batch, op->stream);
+    }
+    HIPCHK(hipGetLastError());
+    return LSG_OK;
+}
+
 // Device vectors (all instances, device order): what 0 hierarchize, 1 dehierarchize,
 // 2 prewavelets from nodal values, 3 nodal values from prewavelets.
 int transformDev(lsg_op* op, int what, const double* in, double* out) {
@@ -664,7 +680,10 @@ int transformDev(lsg_op* op, int what, const double* in, double* out) {
     const int64_t mb = t.host.maxBox;
     switch (what) {
         case 0: trHierPoles(op, in, out); break;
-        case 1: trBox(op, TR_DEHIER, nullptr, nd, mb, in, out); break;
+        case 1:
+            HIPCHK(hipMemcpyAsync(out, in, (size_t)op->n() * op->batch * 8, hipMemcpyDeviceToDevice, op->stream));
+            if (int rc2 = trDehierPoles(op, out)) return rc2;
+            break;
         case 2: {   // calcPrewByNodal (BasisTransformations.cpp:486-545)
             double* hier = t.hier.as<double>();
             trHierPoles(op, in, hier);
@@ -679,7 +698,8 @@ int transformDev(lsg_op* op, int what, const double* in, double* out) {
             double* hier = t.hier.as<double>();
             trBox(op, TR_SYNTH, nullptr, nd, mb, in, nullptr);
             trGather(op, t.host.groups.size() - 1, hier, 0);
-            trBox(op, TR_DEHIER, nullptr, nd, mb, hier, out);
+            HIPCHK(hipMemcpyAsync(out, hier, (size_t)op->n() * op->batch * 8, hipMemcpyDeviceToDevice, op->stream));
+            if (int rc2 = trDehierPoles(op, out)) return rc2;
             break;
         }
         default: return fail(LSG_ERR_ARG, "unknown transform");

@@ -158,6 +158,8 @@ void launch_tr_box(int mode, const TrItem* items, const int32_t* list, int nitem
 void launch_tr_gather(const int32_t* rowDev, const int64_t* rowPtr, const int32_t* col, int64_t nrows,
                       const double* scratch, double* v, int64_t n, int64_t nml, int sub, int batch, hipStream_t s);
 void launch_tr_pole(const int32_t* poles, const double* in, double* out, int64_t n, int batch, hipStream_t s);
+void launch_tr_dehier(const int32_t* list, int64_t cnt, const int32_t* poles, double* v, int64_t n, int batch,
+                      hipStream_t s);
 void launch_tr_energy(const double* x, const double* y, int64_t n, double* out, int batch, hipStream_t s);
 // per instance: out[2b] = max |x| over all points, out[2b+1] = sqrt(sum over active x^2 / dofs)
 template <typename V>

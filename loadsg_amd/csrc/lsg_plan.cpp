@@ -923,6 +923,20 @@ int buildTransform(const Grid& g, TrTables& tr, std::string& err) {
             }
         }
     }
+    // points grouped by their depth along each dimension (coarse to fine), for the
+    // level-by-level dehierarchization
+    tr.byLevel.assign((size_t)d * g.n, 0);
+    tr.lvlOff.assign((size_t)d * (MAXLEV + 2), 0);
+    for (int i = 0; i < d; ++i) {
+        int64_t* off = &tr.lvlOff[(size_t)i * (MAXLEV + 2)];
+        std::vector<int64_t> cnt(MAXLEV + 2, 0);
+        for (int t = 0; t < nd; ++t) cnt[g.depths[t][i]] += g.blkSize[t];
+        for (int t = 1; t <= MAXLEV + 1; ++t) off[t] = off[t - 1] + cnt[t - 1];
+        std::vector<int64_t> fill(off, off + MAXLEV + 2);
+        for (int t = 0; t < nd; ++t)
+            for (int64_t q = 0; q < g.blkSize[t]; ++q)
+                tr.byLevel[(size_t)i * g.n + fill[g.depths[t][i]]++] = (int32_t)(g.blkOff[t] + q);
+    }
     // depths grouped by |T|_1, finest first
     std::map<int, std::vector<int32_t>, std::greater<int>> bySum;
     for (int t = 0; t < nd; ++t) {

@@ -193,6 +193,10 @@ struct TrTables {
     // poles[(i * n + p) * 2 + {0, 1}] = left / right pole of point p along dim i, -1 on
     // the boundary
     std::vector<int32_t> poles;
+    // points by their depth along each dimension: byLevel[i * n + k] for
+    // k in [lvlOff[i * (MAXLEV + 2) + t], lvlOff[i * (MAXLEV + 2) + t + 1]) have depth t along dim i
+    std::vector<int32_t> byLevel;
+    std::vector<int64_t> lvlOff;
     // Scatter-add of box results onto points, as a gather per point: CSR whose rows are
     // device indices and whose entries are scratch positions.  Row groups: one per
     // |T|_1 (same order as levels), then one over all depths.

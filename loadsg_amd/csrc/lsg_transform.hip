@@ -223,6 +223,22 @@ __global__ void __launch_bounds__(256) tr_pole_kernel(const int32_t* __restrict_
     out[b * n + p] = x[p] - 0.5 * (l + r);
 }
 
+// one level of one dimension of a dehierarchization, in place: v[p] += (v[left pole] +
+// v[right pole]) / 2 for the listed points (their poles are coarser along the dimension and
+// final already), the arithmetic of hierLine(inverse)
+__global__ void __launch_bounds__(256) tr_dehier_kernel(const int32_t* __restrict__ list, int64_t cnt,
+                                                        const int32_t* __restrict__ poles, double* __restrict__ v,
+                                                        int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cnt) return;
+    double* x = v + (int64_t)blockIdx.y * n;
+    const int32_t p = list[k];
+    const int32_t lp = poles[2 * (int64_t)p], rp = poles[2 * (int64_t)p + 1];
+    const double l = lp >= 0 ? x[lp] : 0.0;
+    const double r = rp >= 0 ? x[rp] : 0.0;
+    x[p] = x[p] + 0.5 * (l + r);
+}
+
 // v[rowDev[r]] = (sub ? v[rowDev[r]] : 0) -/+ sum of the row's scratch entries, in row order
 __global__ void tr_gather_kernel(const int32_t* __restrict__ rowDev, const int64_t* __restrict__ rowPtr,
                                  const int32_t* __restrict__ col, int64_t nrows, const double* __restrict__ scratch,
@@ -320,6 +336,12 @@ void launch_tr_box(int mode, const TrItem* items, const int32_t* list, int nitem
 void launch_tr_pole(const int32_t* poles, const double* in, double* out, int64_t n, int batch, hipStream_t s) {
     if (n <= 0 || batch <= 0) return;
     tr_pole_kernel<<<dim3((unsigned)((n + 255) / 256), batch), 256, 0, s>>>(poles, in, out, n);
+}
+
+void launch_tr_dehier(const int32_t* list, int64_t cnt, const int32_t* poles, double* v, int64_t n, int batch,
+                      hipStream_t s) {
+    if (cnt <= 0 || batch <= 0) return;
+    tr_dehier_kernel<<<dim3((unsigned)((cnt + 255) / 256), batch), 256, 0, s>>>(list, cnt, poles, v, n);
 }
 
 void launch_tr_gather(const int32_t* rowDev, const int64_t* rowPtr, const int32_t* col, int64_t nrows,
