@@ -624,7 +624,7 @@ int ensureTransform(lsg_op* op) {
     return LSG_OK;
 }
 
-enum { TR_HIER = 0, TR_DEHIER = 1, TR_LEVEL = 2, TR_SYNTH = 3 };
+enum { TR_LEVEL = 2, TR_SYNTH = 3 };   // tr_box_kernel modes
 
 void trBox(lsg_op* op, int mode, const int32_t* list, int count, int64_t maxBox, const double* in, double* out) {
     const Grid& g = op->grid->g;

@@ -150,8 +150,8 @@ void launch_zero(void* p, int64_t bytes, hipStream_t s);
 // rows rows[0..nrows) (device list) of a row-major (., n) FP64 array set to zero
 void launch_zero_rows(double* a, int64_t n, const int* rows, int nrows, hipStream_t s);
 int cg_partial_blocks(int64_t n);
-// basis transformations (lsg_transform.hip); mode 0 hierarchize, 1 dehierarchize,
-// 2 prewavelets of one |T|_1 level, 3 hierarchical representation of prewavelets
+// basis transformations (lsg_transform.hip); mode 2 prewavelets of one |T|_1 level,
+// 3 hierarchical representation of prewavelets
 void launch_tr_box(int mode, const TrItem* items, const int32_t* list, int nitems, const int32_t* gidx,
                    const double* fac, int64_t facTot, int64_t maxBox, const double* in, double* out, double* scratch,
                    int64_t n, int64_t nml, int batch, hipStream_t s);

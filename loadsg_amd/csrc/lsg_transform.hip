@@ -3,11 +3,14 @@
 // (the reference's BasisTransformations/BasisTransformations.cpp and
 // applications/H1_Norm.cpp).
 //
-// Every step works on the dense box G_T of one depth T (the level-T full grid, all
-// points of depth <= T), staged in LDS by one workgroup per (depth, group of instances):
-//   * hierarchization / dehierarchization of a sparse-grid vector: the surplus of a
-//     point of depth T only involves points of G_T, so each depth box is
-//     (de)hierarchized on its own and its depth-T entries are written back;
+//   * hierarchization / dehierarchization of a sparse-grid vector: point-parallel passes
+//     over a host-built table of every point's poles along every dimension (the 1D
+//     sweeps of CalcHierarchicalBasis / CalcNodalBasis, BasisTransformations.cpp:174-198,
+//     390-409), one launch per dimension, and for the dehierarchization one per
+//     (dimension, depth along it), coarse to fine, in place;
+// The prewavelet steps work on the dense box G_T of one depth T (the level-T full grid,
+// all points of depth <= T), staged in LDS by one workgroup per (depth, group of
+// instances):
 //   * prewavelets from nodal values (calcPrewByNodal, :486-545): depths are taken by
 //     |T|_1 descending.  For one |T|_1 every depth box is dehierarchized from the
 //     running hierarchical representation, solved line by line with the prewavelet
@@ -34,7 +37,7 @@ namespace lsg {
 
 namespace {
 
-enum TrMode { TR_HIER = 0, TR_DEHIER = 1, TR_LEVEL = 2, TR_SYNTH = 3 };
+enum TrMode { TR_LEVEL = 2, TR_SYNTH = 3 };
 
 // 1D hierarchization of a line of level t (n = 2^t - 1 values at stride s, 0 outside):
 // finest level first, v_q -= (v_{q-h} + v_{q+h}) / 2 (CalcHierarchicalBasis,
@@ -117,7 +120,7 @@ __device__ inline void applyLine(double* v, int t, int s) {
 // Line operations.  Operators along different dimensions commute, so a dehierarchization
 // followed by the line solve (and the line-matrix product followed by the hierarchization)
 // is done dimension by dimension in one pass over each line.
-enum LineOp { L_HIER, L_DEHIER, L_DEHIER_SOLVE, L_APPLY_HIER };
+enum LineOp { L_DEHIER_SOLVE, L_APPLY_HIER };
 
 // One pass of a line operation along every dimension of the g boxes of the workgroup
 // (one thread per line).
@@ -132,8 +135,6 @@ __device__ void boxPass(double* box, const TrItem& it, int g, LineOp op, const d
                 const int ll = l - j * lines;
                 double* v = box + j * it.total + (ll % S) + (ll / S) * S * n;
                 switch (op) {
-                    case L_HIER: hierLine(v, it.T[i], S, false); break;
-                    case L_DEHIER: hierLine(v, it.T[i], S, true); break;
                     case L_DEHIER_SOLVE:
                         hierLine(v, it.T[i], S, true);
                         solveLine(v, it.T[i], S, fac, facTot);
@@ -180,8 +181,6 @@ __global__ void __launch_bounds__(256) tr_box_kernel(const TrItem* __restrict__ 
         else box[p] = in[(b0 + j) * n + gi[q]];
     }
     __syncthreads();
-    if (MODE == TR_HIER) boxPass(box, it, g, L_HIER, fac, facTot);
-    if (MODE == TR_DEHIER) boxPass(box, it, g, L_DEHIER, fac, facTot);
     if (MODE == TR_LEVEL) {
         boxPass(box, it, g, L_DEHIER_SOLVE, fac, facTot);
         for (int p = threadIdx.x; p < g * tot; p += blockDim.x) {
@@ -191,14 +190,6 @@ __global__ void __launch_bounds__(256) tr_box_kernel(const TrItem* __restrict__ 
             else box[p] = 0.0;
         }
         __syncthreads();
-    }
-    if (MODE == TR_HIER || MODE == TR_DEHIER) {
-        for (int p = threadIdx.x; p < g * tot; p += blockDim.x) {
-            const int j = p / tot;
-            const int q = p - j * tot;
-            if (ownDepth(it, q)) out[(b0 + j) * n + gi[q]] = box[p];
-        }
-        return;
     }
     // TR_LEVEL / TR_SYNTH: hierarchical representation of the depth-T prewavelets on G_T
     boxPass(box, it, g, L_APPLY_HIER, fac, facTot);
@@ -314,14 +305,6 @@ void launch_tr_box(int mode, const TrItem* items, const int32_t* list, int nitem
     const int thr = boxThreads((int64_t)G * maxBox);
     const size_t lds = (size_t)G * maxBox * sizeof(double);
     switch (mode) {
-        case TR_HIER:
-            tr_box_kernel<TR_HIER><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml, G,
-                                                           batch);
-            break;
-        case TR_DEHIER:
-            tr_box_kernel<TR_DEHIER><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml, G,
-                                                             batch);
-            break;
         case TR_LEVEL:
             tr_box_kernel<TR_LEVEL><<<grid, thr, lds, s>>>(items, list, gidx, fac, facTot, in, out, scratch, n, nml, G,
                                                             batch);
