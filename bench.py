@@ -328,7 +328,7 @@ def main():
     ap.add_argument("--warm-xi", type=int, default=2500,
                     help="coefficient samples for the warm-start training set (x 8 basis solves = 20,000 "
                          "GPU-generated solves); 0 = zero start")
-    ap.add_argument("--warm-steps", type=int, default=48000)
+    ap.add_argument("--warm-steps", type=int, default=96000)
     ap.add_argument("--workload", choices=("d5l7", "cfg2", "cfg3", "cfg4", "transform"), default="d5l7")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -392,7 +392,7 @@ def main():
         from loadsg_amd.warmstart import WarmStart
         ws = WarmStart(grid, m, n_xi=NXI)
         xi_train = np.random.default_rng(SEED + 1).normal(0.0, 0.5, (args.warm_xi, NXI))
-        U = ws.basis_solutions(xi_train)
+        U = ws.basis_solutions(xi_train, batch=512)
         ws.fit(xi_train, U, steps=args.warm_steps)
         del U
         log("warm start trained: %s" % json.dumps(ws.stats))
