@@ -11,7 +11,9 @@
  *   LSG_HOST_SLOT   host double arrays in the reference's slot order, i.e. the
  *                   layout of VectorSparseG's data table
  *                   (library/source/extemp/vector.h:247-441, vector.cc:12-23):
- *                   entry i belongs to AdaptiveSparseGrid::getIndexOfTable(i).
+ *                   entry i belongs to AdaptiveSparseGrid::getIndexOfTable(i); one
+ *                   block of lsg_grid_size() entries per batch instance, instance-major
+ *                   (batch x n doubles; a batch-1 operator reads and writes n).
  *   LSG_DEVICE      device arrays in this library's depth-blocked order
  *                   (lsg_grid_device_order gives the permutation), one block of
  *                   lsg_grid_size() entries per batch instance, value type of
@@ -42,6 +44,13 @@ extern "C" {
 #define LSG_COEF_CFG2 1    /* a = 1 + 0.5 prod cos(pi x_i), c = 1 + x_1 */
 #define LSG_COEF_CFG3 2    /* a = 1 + |x|^2, c = 0 */
 #define LSG_COEF_EXPSIN 3  /* a = exp(sum_k xi_k sin(k pi x_{1+(k mod d)}) / k), c = 0 */
+/* General coefficients, sampled on the host at the centre of every cell of depth T+1 of
+ * every depth T (the cells StencilTemplate::applyStencilOnCell visits, Stencil.h:187-246;
+ * the value a plug-in's integration(cell, dirP, dirQ) would evaluate, Stencil.h:740):
+ * element matrix = a(centre) * Poisson + c(centre) * HelmHoltz on that cell.  Operators
+ * with these coefficients run on the cell-wise (sparse) path on every grid. */
+#define LSG_COEF_CELL 4    /* cell_a[k], cell_c[k] for the cells in lsg_grid_cells() order */
+#define LSG_COEF_FUNC 5    /* host functions a_fn(x, ctx), c_fn(x, ctx) called at every cell centre */
 
 /* CG stopping rules */
 #define LSG_STOP_REF_ABS 0   /* reference rule: r^T D^-1 r < eps^2 (cg_method.h:845-871) */
@@ -50,11 +59,19 @@ extern "C" {
 typedef struct lsg_grid lsg_grid;
 typedef struct lsg_op lsg_op;
 
+/* Host coefficient function: value at the point x (d coordinates in (0,1)). */
+typedef double (*lsg_coef_fn)(const double* x, void* ctx);
+
 typedef struct lsg_coef {
     int kind;
     double a0, c0;        /* LSG_COEF_CONST */
     int nxi;              /* LSG_COEF_EXPSIN */
     const double* xi;
+    const double* cell_a; /* LSG_COEF_CELL: one value per cell (NULL: no stiffness term) */
+    const double* cell_c; /* LSG_COEF_CELL: one value per cell (NULL: no mass term) */
+    lsg_coef_fn a_fn;     /* LSG_COEF_FUNC (NULL: no stiffness term); only called during the */
+    lsg_coef_fn c_fn;     /*   lsg_op_create / lsg_op_set_coef call that receives it */
+    void* ctx;
 } lsg_coef;
 
 typedef struct lsg_solve_info {
@@ -89,6 +106,13 @@ int lsg_grid_device_order(const lsg_grid* g, int64_t* dev_of_slot);
  * [11]=streaming-model values of the stages the apply stage performs (prolongation along
  * the last prolongation dimension, stencil, restrictions with the gM exchange, conversion). */
 int lsg_grid_stats(const lsg_grid* g, int64_t* out, int nout);
+/* Cells of depth T+1 of every depth T in the grid (CellData / CellStructure,
+ * cells/CellStructure.h:23-61: level-T cells whose interior corners are all stored), in
+ * the order LSG_COEF_CELL values are read: depth by depth in DepthList order
+ * (iterator/depthIterator.h:14-50).  *ncells receives the count; centres (ncells x d,
+ * may be NULL) the cell centres; depth_index (ncells, may be NULL) the position of the
+ * cell's depth in the depth list. */
+int lsg_grid_cells(lsg_grid* g, int64_t* ncells, double* centres, int32_t* depth_index);
 int lsg_grid_destroy(lsg_grid* g);
 
 /* -------------------------------------------------------------- operators */
@@ -101,20 +125,25 @@ int lsg_op_create(lsg_grid* g, const lsg_coef* coef, int precision, int batch, l
 int lsg_op_set_coef(lsg_op* op, int instance, const lsg_coef* coef);
 int lsg_op_batch(const lsg_op* op);
 /* y = A x: MatrixVectorHomogen::multiplication(prew, Ax, stencil)
- * (MatrixVectorHomogen.h:660-759).  LSG_HOST_SLOT: one instance (instance 0) in slot
- * order.  LSG_DEVICE: all batch instances, device order. */
+ * (MatrixVectorHomogen.h:660-759), for every batch instance (x and y hold batch x n
+ * values in either layout). */
 int lsg_matvec(lsg_op* op, const void* x, void* y, int layout);
 /* Diagonal preconditioner z_p = a(psi_p, psi_p): Preconditioning<S>::precond3
  * (applications/cg_method.h:322-451). */
 int lsg_precond_diag(lsg_op* op, void* z, int layout);
 /* CG solve A x = f: CG::solveHomogen(eps, x, f, iterations, usol, matrix, stencil, time)
  * (applications/cg_method.h:759-886).  x holds the start vector on entry.
- * info: array of lsg_op_batch() entries (may be NULL). */
+ * info: array of lsg_op_batch() entries (may be NULL).  Returns LSG_ERR_NOCONV (x and
+ * info still filled) when an instance stops at maxit before the stopping rule holds
+ * (the reference stops at CG::maxIteration = 500, cg_method.h:35,852), LSG_ERR_NAN when
+ * a CG scalar turns NaN. */
 int lsg_cg_solve(lsg_op* op, const void* f, void* x, double eps, int stop_rule, int maxit,
                  lsg_solve_info* info, int layout);
 /* Mixed precision: FP32 operator inner PCG inside FP64 iterative refinement; the
  * outer residual f - A x, all dot products/norms and the stop test are FP64 on the
- * device.  op64 and op32 must share the grid and batch.  stop rule LSG_STOP_REL. */
+ * device.  op64 and op32 must share the grid and batch.  stop rule LSG_STOP_REL.
+ * Returns LSG_ERR_NOCONV (x and info filled) when an instance misses rel_tol after
+ * maxit refinement sweeps, LSG_ERR_NAN on a NaN residual. */
 int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_tol,
                  double inner_rel_tol, int maxit, lsg_solve_info* info, int layout);
 /* Load vector b_p = int f psi_p of f = sum_j eta_j prod_i sin(m_j pi x_i) (exact 1D

@@ -18,7 +18,8 @@
  * built, as for VectorSparseG (vector.cc:12-23).
  *
  * Stencil plug-ins: Poisson and HelmHoltz map to LSG_COEF_CONST; a variable
- * coefficient is passed as lsg_shim::Coefficient (an lsg_coef).  Errors throw
+ * coefficient is passed as lsg_shim::Coefficient (an lsg_coef, or host functions a(x),
+ * c(x) sampled at the cell centres).  Errors throw
  * std::runtime_error carrying lsg_last_error() (the reference prints and exits).
  */
 #ifndef LOADSG_AMD_SHIM_HPP
@@ -27,6 +28,7 @@
 #include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -39,35 +41,64 @@ inline void check(int rc) {
     if (rc != LSG_OK) throw std::runtime_error(std::string("loadsg_amd: ") + lsg_last_error());
 }
 
-/* A variable-coefficient plug-in: -div(a grad u) + c u with one of the LSG_COEF_* families. */
+/* A variable-coefficient plug-in: -div(a grad u) + c u with one of the LSG_COEF_* families,
+ * or with general host functions a(x), c(x) (Coefficient::functions).  The latter is the
+ * counterpart of a StencilTemplate subclass whose integration(cell, dirP, dirQ)
+ * (stencils/Stencil.h:740) returns a(centre) * Poisson + c(centre) * HelmHoltz: the
+ * library calls the functions at every cell centre when the operator is built. */
 struct Coefficient {
     lsg_coef c;
     std::vector<double> xi;
+    std::function<double(const double*)> fa, fc;   // LSG_COEF_FUNC
+    std::string tag;    // names a function coefficient in the operator cache of MatrixVectorHomogen
     explicit Coefficient(int kind, double a0 = 1.0, double c0 = 0.0, std::vector<double> xi_ = {})
         : xi(std::move(xi_)) {
+        std::memset(&c, 0, sizeof(c));
         c.kind = kind;
         c.a0 = a0;
         c.c0 = c0;
         bind();
     }
-    Coefficient(const Coefficient& o) : c(o.c), xi(o.xi) { bind(); }
+    /* General coefficient from host functions of the point x (DimensionSparseGrid
+     * coordinates); pass an empty std::function for an absent term.  Operators are cached
+     * by tag: a different function needs a different tag. */
+    static Coefficient functions(std::string tag, std::function<double(const double*)> a,
+                                 std::function<double(const double*)> c) {
+        Coefficient k(LSG_COEF_FUNC);
+        k.fa = std::move(a);
+        k.fc = std::move(c);
+        k.tag = std::move(tag);
+        k.bind();
+        return k;
+    }
+    Coefficient(const Coefficient& o) : c(o.c), xi(o.xi), fa(o.fa), fc(o.fc), tag(o.tag) { bind(); }
     Coefficient& operator=(const Coefficient& o) {
         c = o.c;
         xi = o.xi;
+        fa = o.fa;
+        fc = o.fc;
+        tag = o.tag;
         bind();
         return *this;
     }
 
   private:
-    void bind() {   // c.xi always points at this object's own copy
+    static double callA(const double* x, void* ctx) { return static_cast<Coefficient*>(ctx)->fa(x); }
+    static double callC(const double* x, void* ctx) { return static_cast<Coefficient*>(ctx)->fc(x); }
+    void bind() {   // c.xi and c.ctx always point at this object's own members
         c.nxi = (int)xi.size();
         c.xi = xi.empty() ? nullptr : xi.data();
+        if (c.kind == LSG_COEF_FUNC) {
+            c.a_fn = fa ? &Coefficient::callA : nullptr;
+            c.c_fn = fc ? &Coefficient::callC : nullptr;
+            c.ctx = this;
+        }
     }
 };
 
 inline Coefficient coefficientOf(const Poisson&) { return Coefficient(LSG_COEF_CONST, 1.0, 0.0); }
 inline Coefficient coefficientOf(const HelmHoltz&) { return Coefficient(LSG_COEF_CONST, 0.0, 1.0); }
-inline Coefficient coefficientOf(const Coefficient& c) { return Coefficient(c.c.kind, c.c.a0, c.c.c0, c.xi); }
+inline Coefficient coefficientOf(const Coefficient& c) { return c; }
 
 /* Device copy of the grid: the slot table of AdaptiveSparseGrid
  * (getIndexOfTable / getActiveTable, sgrid/sparseGrid.h:145-173). */
@@ -132,7 +163,9 @@ class MatrixVectorHomogen {
         lsg_op* op;
     };
     static bool same(const Coefficient& a, const Coefficient& b) {
-        return a.c.kind == b.c.kind && a.c.a0 == b.c.a0 && a.c.c0 == b.c.c0 && a.xi == b.xi;
+        if (a.c.kind != b.c.kind) return false;
+        if (a.c.kind == LSG_COEF_FUNC) return a.tag == b.tag;
+        return a.c.a0 == b.c.a0 && a.c.c0 == b.c.c0 && a.xi == b.xi;
     }
     DeviceGrid grid_;
     std::vector<Entry> ops_;
@@ -143,7 +176,8 @@ class MatrixVectorHomogen {
 struct CG {
     template <class S>
     static bool solveHomogen(double eps, VectorSparseG& x, VectorSparseG& f, int* iterations, VectorSparseG& /*usol*/,
-                             MatrixVectorHomogen& matrix, S stencil, double* time_precon, int maxit = 100000) {
+                             MatrixVectorHomogen& matrix, S stencil, double* time_precon,
+                             int maxit = 500 /* CG::maxIteration, cg_method.h:35 */) {
         lsg_op* op = matrix.opFor(coefficientOf(stencil));
         if (time_precon) {
             std::vector<double> diag((size_t)matrix.deviceGrid().size());

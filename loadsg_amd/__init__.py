@@ -28,8 +28,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("LOADSG_AMD_LIB") or os.path.join(_HERE, "_lib", "libloadsg_amd.so")
 
 LSG_HOST_SLOT, LSG_DEVICE = 0, 1
+LSG_ERR_NAN, LSG_ERR_NOCONV = -5, -6
 STOP_REF_ABS, STOP_REL = 0, 1
 COEF_CONST, COEF_CFG2, COEF_CFG3, COEF_EXPSIN = 0, 1, 2, 3
+COEF_CELL, COEF_FUNC = 4, 5
 
 _lib = None
 
@@ -38,9 +40,14 @@ class LoadsgError(RuntimeError):
     pass
 
 
+_COEF_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+
+
 class _Coef(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("a0", ctypes.c_double), ("c0", ctypes.c_double),
-                ("nxi", ctypes.c_int), ("xi", ctypes.POINTER(ctypes.c_double))]
+                ("nxi", ctypes.c_int), ("xi", ctypes.POINTER(ctypes.c_double)),
+                ("cell_a", ctypes.POINTER(ctypes.c_double)), ("cell_c", ctypes.POINTER(ctypes.c_double)),
+                ("a_fn", _COEF_FN), ("c_fn", _COEF_FN), ("ctx", ctypes.c_void_p)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -68,6 +75,7 @@ def lib():
             "lsg_grid_slot_order": (i32, [vp, vp, vp]),
             "lsg_grid_device_order": (i32, [vp, vp]),
             "lsg_grid_stats": (i32, [vp, vp, i32]),
+            "lsg_grid_cells": (i32, [vp, ctypes.POINTER(i64), vp, vp]),
             "lsg_grid_destroy": (i32, [vp]),
             "lsg_op_create": (i32, [vp, ctypes.POINTER(_Coef), i32, i32, ctypes.POINTER(vp)]),
             "lsg_op_set_coef": (i32, [vp, i32, ctypes.POINTER(_Coef)]),
@@ -175,6 +183,16 @@ class AdaptiveSparseGrid:
         _check(lib().lsg_grid_device_order(self._h, _ptr(out)))
         return out
 
+    def cells(self):
+        """Cells of depth T+1 of every depth T (CellData, cells/CellStructure.h:23-61), in the
+        order a CellCoefficient is read: (centres (ncells x d), depth-list index of each cell)."""
+        n = ctypes.c_int64()
+        _check(lib().lsg_grid_cells(self._h, ctypes.byref(n), None, None))
+        x = np.zeros((n.value, self.dim))
+        t = np.zeros(n.value, dtype=np.int32)
+        _check(lib().lsg_grid_cells(self._h, ctypes.byref(n), _ptr(x), _ptr(t)))
+        return x, t
+
     def stats(self):
         out = np.zeros(12, dtype=np.int64)
         _check(lib().lsg_grid_stats(self._h, _ptr(out), 12))
@@ -201,8 +219,77 @@ class Stencil:
     def __init__(self, kind, a0=1.0, c0=0.0, xi=None):
         self.kind, self.a0, self.c0, self.xi = kind, a0, c0, (None if xi is None else list(xi))
 
-    def _c(self):
+    def _c(self, grid):
         return _coef(self.kind, self.a0, self.c0, self.xi)
+
+
+def _dbl_array(v, n, name):
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    if a.shape != (n,):
+        raise LoadsgError("%s: expected %d cell values, got shape %s" % (name, n, a.shape))
+    return a
+
+
+class CellCoefficient(Stencil):
+    """General coefficient given per cell: a[k], c[k] at the centre of cell k of
+    AdaptiveSparseGrid.cells() (None: no stiffness / no mass term).  The element matrix of
+    cell k is a[k] * Poisson + c[k] * HelmHoltz (a StencilTemplate::integration plug-in,
+    Stencil.h:740)."""
+
+    def __init__(self, a=None, c=None):
+        if a is None and c is None:
+            raise LoadsgError("CellCoefficient needs a or c")
+        self.kind, self.a, self.c = COEF_CELL, a, c
+
+    def _c(self, grid):
+        n = grid.cells()[0].shape[0]
+        cc = _Coef()
+        cc.kind = COEF_CELL
+        keep = []
+        for field, v in (("cell_a", self.a), ("cell_c", self.c)):
+            if v is not None:
+                arr = _dbl_array(v, n, field)
+                keep.append(arr)
+                setattr(cc, field, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+        cc._keep = keep
+        return cc
+
+
+class CellFunction(Stencil):
+    """General coefficient from vectorised functions of the cell centres: a(x), c(x) take
+    an (ncells x d) array and return ncells values; sampled through grid.cells()."""
+
+    def __init__(self, a=None, c=None):
+        if a is None and c is None:
+            raise LoadsgError("CellFunction needs a or c")
+        self.kind, self.a, self.c = COEF_CELL, a, c
+
+    def _c(self, grid):
+        x, _ = grid.cells()
+        return CellCoefficient(None if self.a is None else self.a(x), None if self.c is None else self.c(x))._c(grid)
+
+
+class HostFunction(Stencil):
+    """General coefficient from scalar host functions a(x), c(x) of one point (a length-d
+    array), called by the library at every cell centre (LSG_COEF_FUNC)."""
+
+    def __init__(self, a=None, c=None):
+        if a is None and c is None:
+            raise LoadsgError("HostFunction needs a or c")
+        self.kind, self.a, self.c = COEF_FUNC, a, c
+
+    def _c(self, grid):
+        d = grid.dim
+        cc = _Coef()
+        cc.kind = COEF_FUNC
+        keep = []
+        for field, fn in (("a_fn", self.a), ("c_fn", self.c)):
+            if fn is not None:
+                cb = _COEF_FN(lambda x, ctx, fn=fn: float(fn(np.ctypeslib.as_array(x, (d,)))))
+                keep.append(cb)
+                setattr(cc, field, cb)
+        cc._keep = keep
+        return cc
 
 
 def Poisson():
@@ -228,7 +315,7 @@ class MatrixVectorHomogen:
     def __init__(self, grid, stencil, precision=64, batch=1):
         self.grid, self.precision, self.batch = grid, precision, batch
         self._h = ctypes.c_void_p()
-        c = stencil._c()
+        c = stencil._c(grid)
         _check(lib().lsg_op_create(grid._h, ctypes.byref(c), precision, batch, ctypes.byref(self._h)))
 
     def __del__(self):
@@ -237,29 +324,63 @@ class MatrixVectorHomogen:
             self._h = ctypes.c_void_p()
 
     def set_coefficient(self, instance, stencil):
-        c = stencil._c()
+        c = stencil._c(self.grid)
         _check(lib().lsg_op_set_coef(self._h, instance, ctypes.byref(c)))
 
     def set_block(self, threads):
         _check(lib().lsg_op_set_block(self._h, 0, threads))
 
+    # ---- argument checks: the C ABI takes raw pointers to batch x N values
+    def _host(self, v, name="vector", out=False):
+        """float64 C-contiguous numpy array of batch x N values in slot order (shape (N,)
+        allowed for batch 1); out=True also requires a writable array of that dtype."""
+        n = self.grid.size
+        if out:
+            if not (isinstance(v, np.ndarray) and v.dtype == np.float64 and v.flags.c_contiguous and v.flags.writeable):
+                raise LoadsgError("%s: expected a writable C-contiguous float64 numpy array" % name)
+            a = v
+        else:
+            a = np.ascontiguousarray(v, dtype=np.float64)
+        if a.size != self.batch * n or (self.batch > 1 and a.shape != (self.batch, n)) or a.ndim > 2:
+            raise LoadsgError("%s: expected shape (%d, %d)%s, got %s" % (
+                name, self.batch, n, " or (%d,)" % n if self.batch == 1 else "", a.shape))
+        return a
+
+    def _dev(self, t, name="tensor", fp64=None):
+        """Contiguous GPU tensor of batch x N values in device order, of the operator's
+        precision (fp64=True/False overrides it)."""
+        import torch
+        want = torch.float64 if (self.precision == 64 if fp64 is None else fp64) else torch.float32
+        if not (t.is_cuda and t.dtype == want and t.is_contiguous() and t.numel() == self.batch * self.grid.size):
+            raise LoadsgError("%s: expected a contiguous %s GPU tensor of %d x %d values, got %s %s %s" % (
+                name, want, self.batch, self.grid.size, t.dtype, tuple(t.shape), t.device))
+        return t
+
+    def _new_host(self):
+        return np.zeros((self.batch, self.grid.size)) if self.batch > 1 else np.zeros(self.grid.size)
+
     def multiplication(self, prew, Ax=None):
-        """Ax = A prew.  numpy float64 (batch x N or N, slot order) or torch device tensors."""
+        """Ax = A prew.  numpy float64 (batch x N, or N for batch 1; slot order) or torch
+        device tensors (device order)."""
         if hasattr(prew, "data_ptr"):
+            self._dev(prew, "prew")
+            self._dev(Ax, "Ax")
             _sync_torch(prew, Ax)
             _check(lib().lsg_matvec(self._h, _ptr(prew), _ptr(Ax), LSG_DEVICE))
             return Ax
-        x = np.ascontiguousarray(prew, dtype=np.float64)
-        y = np.zeros_like(x) if Ax is None else Ax
+        x = self._host(prew, "prew")
+        y = np.zeros_like(x) if Ax is None else self._host(Ax, "Ax", out=True)
         _check(lib().lsg_matvec(self._h, _ptr(x), _ptr(y), LSG_HOST_SLOT))
         return y
 
     def diagonal(self, out=None):
         """Preconditioning<S>::precond3 diagonal (cg_method.h:322-451)."""
         if out is not None and hasattr(out, "data_ptr"):
+            self._dev(out, "out")
+            _sync_torch(out)
             _check(lib().lsg_precond_diag(self._h, _ptr(out), LSG_DEVICE))
             return out
-        z = np.zeros((self.batch, self.grid.size)) if self.batch > 1 else np.zeros(self.grid.size)
+        z = self._new_host()
         _check(lib().lsg_precond_diag(self._h, _ptr(z), LSG_HOST_SLOT))
         return z
 
@@ -267,21 +388,27 @@ class MatrixVectorHomogen:
         """b_p = int f psi_p for f = sum_j eta_j prod_i sin(m_j pi x_i); eta: batch x nterms."""
         eta = np.ascontiguousarray(np.atleast_2d(eta), dtype=np.float64)
         m = np.ascontiguousarray(m, dtype=np.int32)
+        if eta.shape != (self.batch, m.size):
+            raise LoadsgError("eta: expected shape (%d, %d), got %s" % (self.batch, m.size, eta.shape))
         if out is not None and hasattr(out, "data_ptr"):
+            self._dev(out, "out")
+            _sync_torch(out)
             _check(lib().lsg_rhs_sines(self._h, m.size, _ptr(eta), _ptr(m), _ptr(out), LSG_DEVICE))
             return out
-        b = np.zeros((self.batch, self.grid.size)) if self.batch > 1 else np.zeros(self.grid.size)
+        b = self._new_host()
         _check(lib().lsg_rhs_sines(self._h, m.size, _ptr(eta), _ptr(m), _ptr(b), LSG_HOST_SLOT))
         return b
 
     # ---- basis transformations (FP64 operators, grids with complete depth boxes)
     def _transform(self, fn, v, out):
         if hasattr(v, "data_ptr"):
+            self._dev(v, "input")
+            self._dev(out, "out")
             _sync_torch(v, out)
             _check(fn(self._h, _ptr(v), _ptr(out), LSG_DEVICE))
             return out
-        x = np.ascontiguousarray(v, dtype=np.float64)
-        y = np.zeros_like(x) if out is None else out
+        x = self._host(v, "input")
+        y = np.zeros_like(x) if out is None else self._host(out, "out", out=True)
         _check(fn(self._h, _ptr(x), _ptr(y), LSG_HOST_SLOT))
         return y
 
@@ -306,10 +433,11 @@ class MatrixVectorHomogen:
         (H1::getValue / L2::getValue, applications/H1_Norm.cpp:8-78)."""
         out = np.zeros(self.batch)
         if hasattr(u, "data_ptr"):
+            self._dev(u, "u")
             _sync_torch(u)
             _check(lib().lsg_energy_norm(self._h, _ptr(u), _ptr(out), LSG_DEVICE))
         else:
-            x = np.ascontiguousarray(u, dtype=np.float64)
+            x = self._host(u, "u")
             _check(lib().lsg_energy_norm(self._h, _ptr(x), _ptr(out), LSG_HOST_SLOT))
         return out if self.batch > 1 else float(out[0])
 
@@ -318,10 +446,11 @@ class MatrixVectorHomogen:
         points and sqrt(sum x^2 / dofs) over the active ones."""
         out = np.zeros((self.batch, 2))
         if hasattr(x, "data_ptr"):
+            self._dev(x, "x")
             _sync_torch(x)
             _check(lib().lsg_vector_norms(self._h, _ptr(x), _ptr(out), LSG_DEVICE))
         else:
-            v = np.ascontiguousarray(x, dtype=np.float64)
+            v = self._host(x, "x")
             _check(lib().lsg_vector_norms(self._h, _ptr(v), _ptr(out), LSG_HOST_SLOT))
         return out if self.batch > 1 else out[0]
 
@@ -388,15 +517,23 @@ class CG:
 
     @staticmethod
     def solveHomogen(eps, x, f, matrix, stop_rule=STOP_REF_ABS, maxit=500):
+        """Returns the solve info (a list for batches); an instance that stops at maxit has
+        converged == 0 (the library's status LSG_ERR_NOCONV is not raised, as the reference
+        returns normally after CG::maxIteration, cg_method.h:852)."""
         info = (SolveInfo * matrix.batch)()
         if hasattr(x, "data_ptr"):
+            # x is FP64 on FP64 operators and of the operator's precision otherwise
+            matrix._dev(x, "x")
+            matrix._dev(f, "f")
             _sync_torch(x, f)
-            _check(lib().lsg_cg_solve(matrix._h, _ptr(f), _ptr(x), eps, stop_rule, maxit, info, LSG_DEVICE))
+            rc = lib().lsg_cg_solve(matrix._h, _ptr(f), _ptr(x), eps, stop_rule, maxit, info, LSG_DEVICE)
         else:
-            xa = np.ascontiguousarray(x, dtype=np.float64)
-            fa = np.ascontiguousarray(f, dtype=np.float64)
-            _check(lib().lsg_cg_solve(matrix._h, _ptr(fa), _ptr(xa), eps, stop_rule, maxit, info, LSG_HOST_SLOT))
-            x[...] = xa
+            xa = matrix._host(x, "x")
+            fa = matrix._host(f, "f")
+            rc = lib().lsg_cg_solve(matrix._h, _ptr(fa), _ptr(xa), eps, stop_rule, maxit, info, LSG_HOST_SLOT)
+            x[...] = xa.reshape(np.shape(x))
+        if rc != LSG_ERR_NOCONV:
+            _check(rc)
         res = [info[b].as_dict() for b in range(matrix.batch)]
         return res[0] if matrix.batch == 1 else res
 
@@ -405,15 +542,19 @@ class CG:
         """FP32 inner PCG under FP64 iterative refinement (all reductions FP64)."""
         info = (SolveInfo * op64.batch)()
         if hasattr(x, "data_ptr"):
+            op64._dev(x, "x")
+            op64._dev(f, "f")
             _sync_torch(x, f)
-            _check(lib().lsg_ir_solve(op64._h, op32._h, _ptr(f), _ptr(x), rel_tol, inner_rel_tol, maxit, info,
-                                      LSG_DEVICE))
+            rc = lib().lsg_ir_solve(op64._h, op32._h, _ptr(f), _ptr(x), rel_tol, inner_rel_tol, maxit, info,
+                                    LSG_DEVICE)
         else:
-            xa = np.ascontiguousarray(x, dtype=np.float64)
-            fa = np.ascontiguousarray(f, dtype=np.float64)
-            _check(lib().lsg_ir_solve(op64._h, op32._h, _ptr(fa), _ptr(xa), rel_tol, inner_rel_tol, maxit, info,
-                                      LSG_HOST_SLOT))
-            x[...] = xa
+            xa = op64._host(x, "x")
+            fa = op64._host(f, "f")
+            rc = lib().lsg_ir_solve(op64._h, op32._h, _ptr(fa), _ptr(xa), rel_tol, inner_rel_tol, maxit, info,
+                                    LSG_HOST_SLOT)
+            x[...] = xa.reshape(np.shape(x))
+        if rc != LSG_ERR_NOCONV:
+            _check(rc)
         res = [info[b].as_dict() for b in range(op64.batch)]
         return res[0] if op64.batch == 1 else res
 

@@ -36,6 +36,10 @@ int api_fail(int code, const std::string& msg) { return fail(code, msg); }
 
 struct lsg_grid {
     Grid g;
+    // the same grid on the cell-wise (sparse) path, built on first use by an operator with
+    // a general coefficient (LSG_COEF_CELL / LSG_COEF_FUNC) or by lsg_grid_cells; it keeps
+    // g's device order
+    std::unique_ptr<Grid> cellG;
 };
 
 namespace {
@@ -58,8 +62,12 @@ struct DevMem {
 // Separable representation of the coefficient families: a = sum_m prod_j g_mj(x_j),
 // c = sum_m prod_j h_mj(x_j), sampled at the 1D cell centres (2c+1)/2^(t+1) of every
 // level t.  Table of one (term, dim): level t occupies [2^t - 2, 2^(t+1) - 2).
+bool cellCoefKind(int kind) { return kind == LSG_COEF_CELL || kind == LSG_COEF_FUNC; }
+
 int coefTerms(const lsg_coef& c, int d, int& nA, int& nC) {
     switch (c.kind) {
+        case LSG_COEF_CELL: nA = c.cell_a != nullptr; nC = c.cell_c != nullptr; break;
+        case LSG_COEF_FUNC: nA = c.a_fn != nullptr; nC = c.c_fn != nullptr; break;
         case LSG_COEF_CONST: nA = c.a0 != 0.0; nC = c.c0 != 0.0; break;
         case LSG_COEF_CFG2: nA = 2; nC = 2; break;
         case LSG_COEF_CFG3: nA = 1 + d; nC = 0; break;
@@ -74,6 +82,7 @@ void coefTables(const lsg_coef& c, int d, int maxLevel, int64_t tabLen, std::vec
     coefTerms(c, d, nA, nC);
     const int nt = nA + nC;
     tab.assign((size_t)nt * d * tabLen, 1.0);
+    if (cellCoefKind(c.kind)) return;   // sampled per cell instead (cellValues)
     auto fill = [&](int term, int dim, auto fn) {
         double* t0 = &tab[((size_t)term * d + dim) * tabLen];
         for (int t = 1; t <= maxLevel; ++t)
@@ -109,6 +118,63 @@ void coefTables(const lsg_coef& c, int d, int maxLevel, int64_t tabLen, std::vec
     }
 }
 
+// Grid of the cell-wise path: the grid itself when it is sparse, else its cellG copy.
+int cellGrid(lsg_grid* grid, const Grid*& out) {
+    if (grid->g.sparse) {
+        out = &grid->g;
+        return LSG_OK;
+    }
+    if (!grid->cellG) {
+        std::unique_ptr<Grid> cg(new Grid());
+        std::string err;
+        const Grid& g = grid->g;
+        if (int rc = buildGrid(g.d, g.n, g.idx.data(), g.active.data(), *cg, err, true)) return fail(rc, err);
+        grid->cellG = std::move(cg);
+    }
+    out = grid->cellG.get();
+    return LSG_OK;
+}
+
+// Centre of every cell of depth T+1 of every depth (cell k of depth t: lower position
+// cellLo, level T_i: x_i = (lo_i + 1/2) / 2^T_i), cells in table order.
+void cellCentres(const Grid& g, std::vector<double>& x, std::vector<int32_t>* depthOf) {
+    const SparseTables& S = g.sp;
+    const int d = g.d;
+    const int nd = (int)g.depths.size();
+    const int64_t nCells = S.cellOff[nd];
+    x.resize((size_t)nCells * d);
+    if (depthOf) depthOf->resize((size_t)nCells);
+    for (int t = 0; t < nd; ++t)
+        for (int64_t c = S.cellOff[t]; c < S.cellOff[t + 1]; ++c) {
+            for (int i = 0; i < d; ++i)
+                x[(size_t)c * d + i] = (double(S.cellLo[(size_t)c * d + i]) + 0.5) / double(1 << g.depths[t][i]);
+            if (depthOf) (*depthOf)[(size_t)c] = t;
+        }
+}
+
+// (a, c) per cell, interleaved, for a general coefficient
+int cellValues(const Grid& g, const lsg_coef& c, std::vector<double>& ac) {
+    const int d = g.d;
+    std::vector<double> x;
+    cellCentres(g, x, nullptr);
+    const int64_t nCells = (int64_t)(x.size() / d);
+    ac.assign((size_t)2 * nCells, 0.0);
+    for (int64_t k = 0; k < nCells; ++k) {
+        double a = 0.0, cc = 0.0;
+        if (c.kind == LSG_COEF_CELL) {
+            if (c.cell_a) a = c.cell_a[k];
+            if (c.cell_c) cc = c.cell_c[k];
+        } else {
+            if (c.a_fn) a = c.a_fn(&x[(size_t)k * d], c.ctx);
+            if (c.c_fn) cc = c.c_fn(&x[(size_t)k * d], c.ctx);
+        }
+        if (!std::isfinite(a) || !std::isfinite(cc)) return fail(LSG_ERR_NAN, "non-finite coefficient value");
+        ac[(size_t)2 * k] = a;
+        ac[(size_t)2 * k + 1] = cc;
+    }
+    return LSG_OK;
+}
+
 }  // namespace
 
 namespace {
@@ -122,6 +188,8 @@ struct TrDev {
 
 struct lsg_op {
     lsg_grid* grid = nullptr;
+    const Grid* G = nullptr;   // the grid the kernels run on: grid->g, or its cell-wise copy
+    bool cellKind = false;     // general coefficient given per cell (cellCoef filled on the host)
     int prec = 64;
     size_t vs = 8;
     int batch = 1;
@@ -161,6 +229,9 @@ struct lsg_op {
     bool cellCoefValid = false;
     // solver buffers
     DevMem r, h, dv, g, D, f, x, tmp, partials, st, aux;
+    DevMem epsInst, rows;        // iterative refinement: per-instance inner tolerances, rows to clear
+    DevMem norms2;               // lsg_vector_norms results (2 per instance)
+    DevMem rhsEta, rhsM;         // lsg_rhs_sines parameters (grown on demand)
     bool solverReady = false, diagValid = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {};
@@ -173,28 +244,41 @@ struct lsg_op {
     bool chunkX64 = false;
     bool capturing = false;   // matvec launches record no timing events while captured
     std::unique_ptr<TrDev> tr;   // basis transformations (built on first use)
-    int64_t n() const { return grid->g.n; }
+    int64_t n() const { return G->n; }
 };
 
 namespace {
+
+// Sparse path: (a, c) at the centre of every cell, per instance, from the coefficient
+// tables; the stencil and the diagonal read these.  Operators with a general coefficient
+// receive them from the host (lsg_op_set_coef) and never run this.
+int ensureCellCoef(lsg_op* op) {
+    if (!op->sparse || op->cellCoefValid) return LSG_OK;
+    if (op->capturing) return fail(LSG_ERR_ARG, "cell coefficients must be built before graph capture");
+    const int64_t cs = (int64_t)(op->nA + op->nC) * op->G->d * op->tabLen;
+    if (op->prec == 64)
+        launch_sp_cellcoef<double>(op->sd, op->nCells, op->nA, op->nC, op->coefTab.as<double>(), cs, op->tabLen,
+                                   op->cellCoef.as<double>(), 2 * op->nCells, op->batch, op->stream);
+    else
+        launch_sp_cellcoef<float>(op->sd, op->nCells, op->nA, op->nC, op->coefTab.as<float>(), cs, op->tabLen,
+                                  op->cellCoef.as<float>(), 2 * op->nCells, op->batch, op->stream);
+    HIPCHK(hipGetLastError());
+    op->cellCoefValid = true;
+    return LSG_OK;
+}
 
 // nSub > 0: launch only the first nSub slots listed in actMap (a residual over the
 // instances still unconverged); otherwise, inside a CG solve, the compacted CG list.
 template <typename V>
 int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st, int nSub) {
-    const Grid& g = op->grid->g;
+    const Grid& g = *op->G;
     const Plan& P = g.plan;
     const int d = g.d;
     const int nm = nSub > 0 ? nSub : (st ? op->nMap : 0);
     const int* map = nm > 0 ? op->actMap.as<int>() : nullptr;
     const int B = map ? nm : op->batch;
     hipStream_t s = op->stream;
-    if (!op->cellCoefValid) {     // (computed before a CG chunk is captured)
-        launch_sp_cellcoef<V>(op->sd, op->nCells, op->nA, op->nC, op->coefTab.as<V>(),
-                              (int64_t)(op->nA + op->nC) * d * op->tabLen, op->tabLen, op->cellCoef.as<V>(),
-                              2 * op->nCells, op->batch, s);
-        op->cellCoefValid = true;
-    }
+    if (int rc = ensureCellCoef(op)) return rc;   // (done before a CG chunk is captured)
     if (!op->capturing) HIPCHK(hipEventRecord(op->ev[0], s));
     launch_sp_nodal<V>(op->sd, x, g.n, op->nodal.as<V>(), g.nml, g.nml, B, st, s, map);
     for (int dd = 0; dd < d; ++dd)
@@ -240,7 +324,7 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st, int nSub) {
 template <typename V>
 int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st, int nSub = 0) {
     if (op->sparse) return runMatvecSparse<V>(op, x, y, st, nSub);
-    const Grid& g = op->grid->g;
+    const Grid& g = *op->G;
     const Plan& P = g.plan;
     const int d = g.d;
     // inside a CG solve with a compacted instance list, or for a residual over a
@@ -373,7 +457,7 @@ int ensureThreads(lsg_op* op) {
 // host slot-order <-> device order (instance-major batch)
 template <typename V>
 int uploadSlot(lsg_op* op, const double* h, V* dptr) {
-    const Grid& g = op->grid->g;
+    const Grid& g = *op->G;
     std::vector<V> buf((size_t)g.n * op->batch);
     for (int b = 0; b < op->batch; ++b)
         for (int64_t s = 0; s < g.n; ++s) buf[(size_t)b * g.n + g.devOfSlot[s]] = V(h[(size_t)b * g.n + s]);
@@ -383,7 +467,7 @@ int uploadSlot(lsg_op* op, const double* h, V* dptr) {
 }
 template <typename V>
 int downloadSlot(lsg_op* op, const V* dptr, double* h) {
-    const Grid& g = op->grid->g;
+    const Grid& g = *op->G;
     std::vector<V> buf((size_t)g.n * op->batch);
     HIPCHK(hipMemcpyAsync(buf.data(), dptr, buf.size() * sizeof(V), hipMemcpyDeviceToHost, op->stream));
     HIPCHK(hipStreamSynchronize(op->stream));
@@ -413,6 +497,8 @@ int ensureSolver(lsg_op* op) {
     HIPCHK(op->partials.alloc((size_t)op->batch * 3 * nbk * sizeof(double)));
     HIPCHK(op->st.alloc((size_t)op->batch * sizeof(CGState)));
     HIPCHK(op->actMap.alloc((size_t)op->batch * sizeof(int)));
+    HIPCHK(op->epsInst.alloc((size_t)op->batch * sizeof(double)));
+    HIPCHK(op->rows.alloc((size_t)op->batch * sizeof(int)));
     op->solverReady = true;
     return LSG_OK;
 }
@@ -423,17 +509,16 @@ bool useGraphs() { return std::getenv("LSG_NO_GRAPH") == nullptr; }
 
 int computeDiag(lsg_op* op) {
     if (op->diagValid) return LSG_OK;
-    const Grid& g = op->grid->g;
+    const Grid& g = *op->G;
     int64_t cs = (int64_t)(op->nA + op->nC) * g.d * op->tabLen;
     if (op->sparse) {
+        if (int rc = ensureCellCoef(op)) return rc;
         if (op->prec == 64)
-            launch_sp_diag<double>(op->sd, op->activeDev.as<uint8_t>(), (int)g.n, op->nA, op->nC,
-                                   op->coefTab.as<double>(), cs, op->tabLen, op->D.as<double>(), g.n, op->batch,
-                                   op->stream);
+            launch_sp_diag<double>(op->sd, op->activeDev.as<uint8_t>(), (int)g.n, op->cellCoef.as<double>(),
+                                   2 * op->nCells, op->D.as<double>(), g.n, op->batch, op->stream);
         else
-            launch_sp_diag<float>(op->sd, op->activeDev.as<uint8_t>(), (int)g.n, op->nA, op->nC,
-                                  op->coefTab.as<float>(), cs, op->tabLen, op->D.as<float>(), g.n, op->batch,
-                                  op->stream);
+            launch_sp_diag<float>(op->sd, op->activeDev.as<uint8_t>(), (int)g.n, op->cellCoef.as<float>(),
+                                  2 * op->nCells, op->D.as<float>(), g.n, op->batch, op->stream);
         HIPCHK(hipGetLastError());
         op->diagValid = true;
         return LSG_OK;
@@ -462,6 +547,9 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
     hipStream_t s = op->stream;
     int rc = ensureSolver(op);
     if (rc) return rc;
+    // the cell coefficients are refreshed eagerly here, outside any captured chunk: a chunk
+    // replayed after lsg_op_set_coef then reads the current ones
+    if ((rc = ensureCellCoef(op))) return rc;
     if ((rc = computeDiag(op))) return rc;
     if ((rc = ensureThreads(op))) return rc;
     V* r = op->r.as<V>();
@@ -502,10 +590,6 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
             // the chunk's launches depend only on the operator, x and k: capture once,
             // then replay (the kernels read the convergence state on the device)
             if (!op->chunkExec || op->chunkX != x || op->chunkK != k || op->chunkX64 != x64) {
-                if (op->sparse && !op->cellCoefValid) {    // keep this one-off launch out of the graph
-                    if ((rc = runMatvec<V>(op, dv, g, nullptr))) return rc;
-                    HIPCHK(hipStreamSynchronize(s));
-                }
                 if (op->chunkExec) HIPCHK(hipGraphExecDestroy(op->chunkExec));
                 op->chunkExec = nullptr;
                 hipGraph_t graph = nullptr;
@@ -778,6 +862,19 @@ int lsg_grid_stats(const lsg_grid* g, int64_t* out, int nout) {
     return LSG_OK;
 }
 
+int lsg_grid_cells(lsg_grid* g, int64_t* ncells, double* centres, int32_t* depth_index) {
+    if (!g || !ncells) return fail(LSG_ERR_ARG, "null argument");
+    const Grid* cg = nullptr;
+    if (int rc = cellGrid(g, cg)) return rc;
+    std::vector<double> x;
+    std::vector<int32_t> t;
+    cellCentres(*cg, x, &t);
+    *ncells = (int64_t)t.size();
+    if (centres) std::memcpy(centres, x.data(), x.size() * sizeof(double));
+    if (depth_index) std::memcpy(depth_index, t.data(), t.size() * sizeof(int32_t));
+    return LSG_OK;
+}
+
 int lsg_grid_destroy(lsg_grid* g) {
     delete g;
     return LSG_OK;
@@ -792,7 +889,13 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     op->vs = precision == 64 ? 8 : 4;
     op->batch = batch;
     op->kind = coef->kind;
-    const Grid& g = grid->g;
+    op->cellKind = cellCoefKind(coef->kind);
+    op->G = &grid->g;
+    if (op->cellKind) {
+        int rc = cellGrid(grid, op->G);
+        if (rc) return rc;
+    }
+    const Grid& g = *op->G;
     const Plan& P = g.plan;
     if (coefTerms(*coef, g.d, op->nA, op->nC)) return fail(LSG_ERR_ARG, "bad coefficient");
     if (op->nA + op->nC == 0) return fail(LSG_ERR_ARG, "zero operator");
@@ -991,9 +1094,25 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
 int lsg_op_set_coef(lsg_op* op, int instance, const lsg_coef* coef) {
     if (!op || !coef || instance < 0 || instance >= op->batch) return fail(LSG_ERR_ARG, "bad arguments");
     int nA, nC;
-    const Grid& g = op->grid->g;
-    if (coefTerms(*coef, g.d, nA, nC) || nA != op->nA || nC != op->nC)
+    const Grid& g = *op->G;
+    if (coefTerms(*coef, g.d, nA, nC) || nA != op->nA || nC != op->nC || cellCoefKind(coef->kind) != op->cellKind)
         return fail(LSG_ERR_ARG, "coefficient family differs from the operator's");
+    if (op->cellKind) {
+        // general coefficient: (a, c) per cell sampled on the host, straight into cellCoef
+        std::vector<double> ac;
+        if (int rc = cellValues(g, *coef, ac)) return rc;
+        const size_t per = (size_t)2 * op->nCells;
+        if (ac.size() != per) return fail(LSG_ERR_ARG, "cell count mismatch");
+        if (op->prec == 64) {
+            HIPCHK(hipMemcpy(op->cellCoef.as<double>() + per * instance, ac.data(), per * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> af(ac.begin(), ac.end());
+            HIPCHK(hipMemcpy(op->cellCoef.as<float>() + per * instance, af.data(), per * 4, hipMemcpyHostToDevice));
+        }
+        op->diagValid = false;
+        op->cellCoefValid = true;
+        return LSG_OK;
+    }
     std::vector<double> tab;
     coefTables(*coef, g.d, g.maxLevel, op->tabLen, tab);
     const size_t per = tab.size();
@@ -1066,7 +1185,7 @@ int lsg_cg_solve(lsg_op* op, const void* f, void* x, double eps, int stop_rule, 
     if (rc) return rc;
     (void)tot;
     if (layout == LSG_HOST_SLOT && (rc = download(op, op->x.p, (double*)x))) return rc;
-    bool nan = false;
+    bool nan = false, noconv = false;
     for (int b = 0; b < op->batch; ++b) {
         if (info) {
             info[b].iterations = st[b].it;
@@ -1076,15 +1195,17 @@ int lsg_cg_solve(lsg_op* op, const void* f, void* x, double eps, int stop_rule, 
             info[b].converged = st[b].converged;
         }
         nan |= st[b].converged < 0;
+        noconv |= st[b].converged == 0;
     }
     if (nan) return fail(LSG_ERR_NAN, "NaN in CG");
+    if (noconv) return fail(LSG_ERR_NOCONV, "CG stopped at maxit before the stopping rule was met");
     return LSG_OK;
 }
 
 int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_tol, double inner_rel_tol, int maxit,
                  lsg_solve_info* info, int layout) {
     if (!op64 || !op32 || !f || !x) return fail(LSG_ERR_ARG, "null argument");
-    if (op64->prec != 64 || op32->prec != 32 || op64->grid != op32->grid || op64->batch != op32->batch)
+    if (op64->prec != 64 || op32->prec != 32 || op64->G != op32->G || op64->batch != op32->batch)
         return fail(LSG_ERR_ARG, "need an FP64 and an FP32 operator on the same grid and batch");
     int rc;
     if ((rc = ensureSolver(op64)) || (rc = ensureSolver(op32)) || (rc = ensureThreads(op64)) ||
@@ -1112,9 +1233,8 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
     std::vector<int> iters(B, 0), outer(B, 0), conv(B, 0);
     std::vector<double> rel, epsH(B);
     std::vector<CGState> st;
-    DevMem epsD, rowsD;
-    HIPCHK(epsD.alloc((size_t)B * 8));
-    HIPCHK(rowsD.alloc((size_t)B * sizeof(int)));
+    DevMem& epsD = op64->epsInst;     // (allocated with the solver buffers, not per call)
+    DevMem& rowsD = op64->rows;
     std::vector<int> newly;   // instances converged in this sweep: their r64 rows are zeroed
     std::vector<int> open;    // instances not yet converged
     for (int sweep = 0; sweep <= maxit; ++sweep) {
@@ -1172,7 +1292,8 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
             if (!conv[b]) { iters[b] += st[b].it; outer[b] += 1; }
     }
     if (layout == LSG_HOST_SLOT && (rc = download(op64, op64->x.p, (double*)x))) return rc;
-    for (int b = 0; b < B; ++b)
+    bool noconv = false;
+    for (int b = 0; b < B; ++b) {
         if (info) {
             info[b].iterations = iters[b];
             info[b].outer = outer[b];
@@ -1180,15 +1301,20 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
             info[b].rel_residual = rel[b];
             info[b].converged = conv[b];
         }
+        noconv |= !conv[b];
+    }
+    if (noconv) return fail(LSG_ERR_NOCONV, "iterative refinement stopped at maxit sweeps before rel_tol was met");
     return LSG_OK;
 }
 
 int lsg_rhs_sines(lsg_op* op, int nterms, const double* eta, const int32_t* m, void* b, int layout) {
     if (!op || nterms < 1 || !eta || !m || !b) return fail(LSG_ERR_ARG, "bad arguments");
-    const Grid& g = op->grid->g;
-    DevMem dEta, dM;
-    HIPCHK(dEta.alloc((size_t)nterms * op->batch * 8));
-    HIPCHK(dM.alloc((size_t)nterms * 4));
+    const Grid& g = *op->G;
+    DevMem& dEta = op->rhsEta;
+    DevMem& dM = op->rhsM;
+    if (dEta.bytes < (size_t)nterms * op->batch * 8) HIPCHK(dEta.alloc((size_t)nterms * op->batch * 8));
+    if (dM.bytes < (size_t)nterms * 4) HIPCHK(dM.alloc((size_t)nterms * 4));
+    HIPCHK(hipStreamSynchronize(op->stream));   // an earlier call's kernels may still read them
     HIPCHK(hipMemcpy(dEta.p, eta, (size_t)nterms * op->batch * 8, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(dM.p, m, (size_t)nterms * 4, hipMemcpyHostToDevice));
     void* dst = layout == LSG_DEVICE ? b : op->ys.p;
@@ -1237,7 +1363,7 @@ int lsg_energy_norm(lsg_op* op, const void* u, double* out, int layout) {
 
 int lsg_vector_norms(lsg_op* op, const void* x, double* out, int layout) {
     if (!op || !x || !out) return fail(LSG_ERR_ARG, "null argument");
-    const Grid& g = op->grid->g;
+    const Grid& g = *op->G;
     const void* xv = x;
     if (layout != LSG_DEVICE) {
         int rc = upload(op, (const double*)x, op->xs.p);
@@ -1248,8 +1374,8 @@ int lsg_vector_norms(lsg_op* op, const void* x, double* out, int layout) {
     for (uint8_t a : g.active) dofs += a ? 1 : 0;
     if (g.active.empty()) dofs = g.n;
     const uint8_t* act = op->sparse ? op->activeDev.as<uint8_t>() : nullptr;
-    DevMem res;
-    HIPCHK(res.alloc((size_t)op->batch * 2 * sizeof(double)));
+    DevMem& res = op->norms2;
+    if (!res.p) HIPCHK(res.alloc((size_t)op->batch * 2 * sizeof(double)));
     if (op->prec == 64)
         launch_vec_norms<double>((const double*)xv, g.n, act, (double)std::max<int64_t>(dofs, 1), res.as<double>(),
                                  op->batch, op->stream);

@@ -109,8 +109,8 @@ void launch_sp_prol(const ProlItem* items, int nitems, const SparseDev& s, const
 template <typename V>
 void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str);
 template <typename V>
-void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, int nA, int nC, const V* coefTab,
-                    int64_t coefStride, int64_t tabLen, V* z, int64_t zStride, int batch, hipStream_t str);
+void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, const V* cellCoef, int64_t ccStride, V* z,
+                    int64_t zStride, int batch, hipStream_t str);
 
 template <typename V>
 void launch_nodal(const NodalItem* items, int nitems, int d, const V* x, int64_t xStride, V* nodal, int64_t nStride,

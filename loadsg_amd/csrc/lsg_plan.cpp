@@ -730,7 +730,8 @@ void regularIndexTable(int d, int level, std::vector<int32_t>& idx) {
     dfsSons(d, I, level, seen, idx);
 }
 
-int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid& g, std::string& err) {
+int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid& g, std::string& err,
+              bool forceSparse) {
     if (d < 1 || d > MAXD || n < 1 || !idx) { err = "bad arguments"; return LSG_ERR_ARG; }
     g = Grid();
     g.d = d;
@@ -774,6 +775,9 @@ int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid&
     g.sparse = false;
     for (int64_t s = 0; s < n; ++s) if (!g.active[s]) g.sparse = true;
     for (int t = 0; t < nd; ++t) if (cnt[t] != blockOf(g.depths[t], d)) g.sparse = true;
+    // a grid with complete depth blocks keeps the dense device order even when it is
+    // forced onto the sparse path (cell-wise coefficients), so vectors of both paths match
+    const bool denseOrder = !g.sparse;
     int64_t off = 0, boff = 0;
     for (int t = 0; t < nd; ++t) {
         g.blkSize[t] = cnt[t];
@@ -792,7 +796,7 @@ int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid&
     g.nml = boff;
     g.devOfSlot.resize(n);
     g.slotOfDev.resize(n);
-    if (g.sparse) {
+    if (!denseOrder) {
         // device order: depth blocks, slot order inside a block
         std::vector<int64_t> fill(nd, 0);
         for (int64_t s = 0; s < n; ++s) {
@@ -815,6 +819,10 @@ int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid&
         int64_t dev = g.blkOff[tOf[s]] + loc;
         g.devOfSlot[s] = dev;
         g.slotOfDev[dev] = s;
+    }
+    if (forceSparse) {
+        g.sparse = true;
+        if (int rc = buildSparse(g, tOf, err)) return rc;
     }
     return buildPlan(g, err);
 }

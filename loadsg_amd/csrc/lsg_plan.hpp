@@ -215,8 +215,11 @@ inline int64_t facBase(int t) { return (int64_t(1) << t) - t - 1; }
 int buildTransform(const Grid& g, TrTables& tr, std::string& err);
 
 // Build the grid from a reference slot-order index table.  Returns 0 or a
-// negative status with a message in err.
-int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid& g, std::string& err);
+// negative status with a message in err.  forceSparse: build the gather tables of the
+// sparse path even for a grid with complete depth blocks (cell-wise coefficients); such a
+// grid keeps the dense device order.
+int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid& g, std::string& err,
+              bool forceSparse = false);
 // Reference slot order of AddRecursiveSonsOfPoint(center, level).
 void regularIndexTable(int d, int level, std::vector<int32_t>& idx);
 

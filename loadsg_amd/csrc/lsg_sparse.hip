@@ -305,11 +305,10 @@ __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
 // stencil coefficient psi_p(q) from the position offset.  Hanging nodes get 1.
 template <typename V>
 __global__ void __launch_bounds__(256) sp_diag_kernel(SparseDev s, const uint8_t* __restrict__ activeDev, int n,
-                                                      int nA, int nC, const V* __restrict__ coefTab,
-                                                      int64_t coefStride, int64_t tabLen, V* __restrict__ z,
-                                                      int64_t zStride) {
+                                                      const V* __restrict__ cellCoef, int64_t ccStride,
+                                                      V* __restrict__ z, int64_t zStride) {
     const int inst = blockIdx.y;
-    const V* coef = coefTab + inst * coefStride;
+    const V* cc = cellCoef + inst * ccStride;
     const int d = s.d;
     const unsigned nc = 1u << d;
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
@@ -329,8 +328,7 @@ __global__ void __launch_bounds__(256) sp_diag_kernel(SparseDev s, const uint8_t
                 const int64_t cell = c0 + s.rcCell[e];
                 const unsigned cj = (unsigned)s.rcCorner[e];
                 const int32_t* lo = s.cellLo + cell * d;
-                V aC, cC;
-                cell_coef<V>(coef, tabLen, nA, nC, d, T, lo, aC, cC);
+                const V aC = cc[2 * cell], cC = cc[2 * cell + 1];
                 const int32_t* corner = s.cellCorner + cell * nc;
                 for (unsigned q = 0; q < nc; ++q) {
                     if (corner[q] < 0) continue;
@@ -391,10 +389,10 @@ void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str) {
     }
 }
 template <typename V>
-void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, int nA, int nC, const V* coefTab,
-                    int64_t coefStride, int64_t tabLen, V* z, int64_t zStride, int batch, hipStream_t str) {
-    hipLaunchKernelGGL(sp_diag_kernel<V>, dim3(gridOf(n, 256), batch), dim3(256), 0, str, s, activeDev, n, nA, nC,
-                       coefTab, coefStride, tabLen, z, zStride);
+void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, const V* cellCoef, int64_t ccStride, V* z,
+                    int64_t zStride, int batch, hipStream_t str) {
+    hipLaunchKernelGGL(sp_diag_kernel<V>, dim3(gridOf(n, 256), batch), dim3(256), 0, str, s, activeDev, n, cellCoef,
+                       ccStride, z, zStride);
 }
 
 #define LSG_SP_INST(V)                                                                                              \
@@ -405,8 +403,8 @@ void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, int nA,
     template void launch_sp_prol<V>(const ProlItem*, int, const SparseDev&, const V*, int64_t, V*, int64_t, int,    \
                                     const CGState*, hipStream_t, const int*);                                       \
     template void launch_sp_apply<V>(const SparseApplyArgs<V>&, int, hipStream_t);                                  \
-    template void launch_sp_diag<V>(const SparseDev&, const uint8_t*, int, int, int, const V*, int64_t, int64_t, V*,  \
-                                    int64_t, int, hipStream_t);
+    template void launch_sp_diag<V>(const SparseDev&, const uint8_t*, int, const V*, int64_t, V*, int64_t, int,      \
+                                    hipStream_t);
 LSG_SP_INST(double)
 LSG_SP_INST(float)
 #undef LSG_SP_INST

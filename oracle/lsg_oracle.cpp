@@ -130,7 +130,9 @@ struct DepthHash {
 
 // Coefficient families shared with the product library (same formulas, evaluated
 // here directly, not through the product's separable tables).
-enum CoefKind { COEF_CONST = 0, COEF_CFG2 = 1, COEF_CFG3 = 2, COEF_EXPSIN = 3 };
+enum CoefKind { COEF_CONST = 0, COEF_CFG2 = 1, COEF_CFG3 = 2, COEF_EXPSIN = 3, COEF_GEN = 10 };
+// COEF_GEN: a non-separable pair, a = 1 + x1 x2 + sin(pi (x1 + x2)), c = 1 + sin(pi x1 x2) / 2
+// (x2 = x1 when d = 1); the product library receives it sampled per cell.
 struct Coef {
     int kind = COEF_CONST;
     double a0 = 1.0, c0 = 0.0;   // COEF_CONST: a = a0, c = c0
@@ -146,14 +148,15 @@ struct Coef {
                     s += xi[k - 1] * std::sin(double(k) * M_PI * x[k % d]) / double(k);
                 return std::exp(s);
             }
+            case COEF_GEN: return 1.0 + x[0] * x[1 % d] + std::sin(M_PI * (x[0] + x[1 % d]));
         }
         return 0.0;
     }
     double c(const double* x, int d) const {
-        (void)d;
         switch (kind) {
             case COEF_CONST: return c0;
             case COEF_CFG2: return 1.0 + x[0];
+            case COEF_GEN: return 1.0 + 0.5 * std::sin(M_PI * x[0] * x[1 % d]);
             default: return 0.0;
         }
     }

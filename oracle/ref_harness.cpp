@@ -18,9 +18,11 @@
 //   ref_harness adaptive <startlevel> <radius> <maxlevel> <coef> <outdir> [what...]
 //   ref_harness timemv   <level> <coef> <reps>            (timing sample, prints JSON)
 //   ref_harness timetr   <level>                          (calcPrewByNodal / calcNodalByPrew timing, JSON)
-// coef: const:<a0>:<c0> | cfg2 | cfg3 | expsin:<xi1>,<xi2>,...
+// coef: const:<a0>:<c0> | cfg2 | cfg3 | expsin:<xi1>,<xi2>,... | gen
+//       (gen: a = 1 + x1 x2 + sin(pi (x1 + x2)), c = 1 + sin(pi x1 x2) / 2, not separable)
 // what: matvec precond cg transform   (default: all)
 #include "source/sparseEXPDE.h"
+#include "ref_plugin.hpp"
 
 #include <string>
 #include <vector>
@@ -31,87 +33,6 @@
 #include <sstream>
 
 namespace {
-
-struct CoefSpec {
-    int kind = 0;  // 0 const, 1 cfg2, 2 cfg3, 3 expsin
-    double a0 = 1.0, c0 = 0.0;
-    std::vector<double> xi;
-    double a(const double* x) const {
-        const int d = DimensionSparseGrid;
-        switch (kind) {
-            case 0: return a0;
-            case 1: { double p = 1.0; for (int i = 0; i < d; ++i) p *= std::cos(M_PI * x[i]); return 1.0 + 0.5 * p; }
-            case 2: { double s = 1.0; for (int i = 0; i < d; ++i) s += x[i] * x[i]; return s; }
-            case 3: {
-                double s = 0.0;
-                for (size_t k = 1; k <= xi.size(); ++k)
-                    s += xi[k - 1] * std::sin(double(k) * M_PI * x[k % d]) / double(k);
-                return std::exp(s);
-            }
-        }
-        return 0.0;
-    }
-    double c(const double* x) const {
-        switch (kind) {
-            case 0: return c0;
-            case 1: return 1.0 + x[0];
-            default: return 0.0;
-        }
-    }
-};
-
-CoefSpec parseCoef(const std::string& s) {
-    CoefSpec c;
-    if (s.rfind("const:", 0) == 0) {
-        c.kind = 0;
-        std::sscanf(s.c_str() + 6, "%lf:%lf", &c.a0, &c.c0);
-    } else if (s == "cfg2") c.kind = 1;
-    else if (s == "cfg3") c.kind = 2;
-    else if (s.rfind("expsin:", 0) == 0) {
-        c.kind = 3;
-        std::stringstream ss(s.substr(7));
-        std::string tok;
-        while (std::getline(ss, tok, ',')) c.xi.push_back(std::stod(tok));
-    } else {
-        std::fprintf(stderr, "bad coef %s\n", s.c_str());
-        std::exit(2);
-    }
-    return c;
-}
-
-// Variable-coefficient plug-in: midpoint value of a and c on each cell times
-// the reference's own Poisson / HelmHoltz element integrals.
-class VarCoeffStencil : public StencilTemplate {
-public:
-    VarCoeffStencil(AdaptiveSparseGrid& g, const CoefSpec& cs) : StencilTemplate(g), P(g), H(g), spec(cs) {}
-    TypeMatrixVectorMultiplication getTypeMatrixVectorMultiplication() { return StencilOnTheFly; }
-    inline void initialize(Depth& T_) {
-        StencilTemplate::initialize(T_);
-        P.initialize(T_);
-        H.initialize(T_);
-    }
-    inline double integration(CellDimension& cell, CellIndexDirection& dirP, CellIndexDirection& dirQ) {
-        double x[DimensionSparseGrid];
-        for (int i = 0; i < DimensionSparseGrid; ++i) x[i] = cell.coordinate(i);
-        double a = spec.a(x), c = spec.c(x);
-        double v = 0.0;
-        if (a != 0.0) v += a * P.integration(cell, dirP, dirQ);
-        if (c != 0.0) v += c * H.integration(cell, dirP, dirQ);
-        return v;
-    }
-    inline double localValue(const IndexDimension& center, const IndexDimension& cell, const MultiDimCompass& mc) const {
-        double x[DimensionSparseGrid];
-        for (int i = 0; i < DimensionSparseGrid; ++i) x[i] = cell.coordinate(i) + 0.5 * meshwidth[i];
-        double a = spec.a(x), c = spec.c(x);
-        double v = 0.0;
-        if (a != 0.0) v += a * P.localValue(center, cell, mc);
-        if (c != 0.0) v += c * H.localValue(center, cell, mc);
-        return v;
-    }
-    Poisson P;
-    HelmHoltz H;
-    CoefSpec spec;
-};
 
 // --- minimal .npy writer ----------------------------------------------------
 void writeNpy(const std::string& path, const char* descr, const void* data, size_t elsize,

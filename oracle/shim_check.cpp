@@ -8,13 +8,17 @@
 // ::CG::solveHomogen (applications/cg_method.h:759-886) on the CPU; the drop-in
 // side runs lsg_shim::MatrixVectorHomogen / lsg_shim::CG with the same calls on the GPU.
 // The same is done for calcPrewByNodal / calcNodalByPrew (BasisTransformations.cpp:486-707)
-// and H1 / L2 ::getValue (applications/H1_Norm.cpp:8-78).
+// and H1 / L2 ::getValue (applications/H1_Norm.cpp:8-78), and for a general coefficient:
+// the reference with the plug-in of ref_plugin.hpp (a(centre) Poisson + c(centre)
+// HelmHoltz per cell, non-separable a and c), the drop-in with
+// lsg_shim::Coefficient::functions of the same a and c.
 //
 // Usage: shim_check <level>      prints one JSON line; exit status 0 when
 //   matvecs agree to 1e-12 (relative, max norm), CG iteration counts are equal and
 //   the CG solutions agree to 1e-10, transforms and norms to 1e-12.
 #include "source/sparseEXPDE.h"
 #include "loadsg_amd_shim.hpp"
+#include "ref_plugin.hpp"
 
 #include <cmath>
 #include <cstdio>
@@ -87,12 +91,30 @@ int main(int argc, char** argv) {
     const double l2Ref = ::L2(grid, mgrid).getValue(u), l2Dev = lsg_shim::L2(grid, mgrid).getValue(u);
     const double normRel = std::fmax(std::fabs(h1Ref - h1Dev) / h1Ref, std::fabs(l2Ref - l2Dev) / l2Ref);
 
+    // general (non-separable) coefficient: reference plug-in vs host functions
+    CoefSpec gen;
+    gen.kind = 10;
+    VarCoeffStencil var(grid, gen);
+    const lsg_shim::Coefficient fun = lsg_shim::Coefficient::functions(
+        "gen", [&](const double* p) { return gen.a(p); }, [&](const double* p) { return gen.c(p); });
+    ref.multiplication(x, yRef, var);
+    dev.multiplication(x, yDev, fun);
+    const double mvGen = relDiff(yRef, yDev, n);
+    VectorSparseG solRefG(grid), solDevG(grid);
+    solRefG = 0.0;
+    solDevG = 0.0;
+    int itRefG = 0, itDevG = 0;
+    ::CG::solveHomogen(1e-8, solRefG, f, &itRefG, usol, ref, var, &tpRef);
+    lsg_shim::CG::solveHomogen(1e-8, solDevG, f, &itDevG, usol, dev, fun, &tpDev);
+    const double cgRelG = relDiff(solRefG, solDevG, n);
+
     const bool ok = mvPoisson < 1e-12 && mvHelm < 1e-12 && itRef == itDev && cgRel < 1e-10 && trPrew < 1e-12 &&
-                    trNodal < 1e-12 && normRel < 1e-12;
+                    trNodal < 1e-12 && normRel < 1e-12 && mvGen < 1e-12 && itRefG == itDevG && cgRelG < 1e-10;
     std::printf("{\"d\": %d, \"level\": %d, \"n\": %zu, \"matvec_poisson_rel\": %.3e, \"matvec_helm_rel\": %.3e, "
                 "\"cg_iters_ref\": %d, \"cg_iters_dev\": %d, \"cg_x_rel\": %.3e, \"prew_by_nodal_rel\": %.3e, "
-                "\"nodal_by_prew_rel\": %.3e, \"norms_rel\": %.3e, \"ok\": %s}\n",
-                DimensionSparseGrid, level, n, mvPoisson, mvHelm, itRef, itDev, cgRel, trPrew, trNodal, normRel,
-                ok ? "true" : "false");
+                "\"nodal_by_prew_rel\": %.3e, \"norms_rel\": %.3e, \"matvec_gen_rel\": %.3e, "
+                "\"cg_gen_iters_ref\": %d, \"cg_gen_iters_dev\": %d, \"cg_gen_x_rel\": %.3e, \"ok\": %s}\n",
+                DimensionSparseGrid, level, n, mvPoisson, mvHelm, itRef, itDev, cgRel, trPrew, trNodal, normRel, mvGen,
+                itRefG, itDevG, cgRelG, ok ? "true" : "false");
     return ok ? 0 : 1;
 }

@@ -45,6 +45,10 @@ CASES = {
     "d3_adaptive_cfg3": (3, ["adaptive", "3", "0.15", "6", "cfg3"]),
     "d2_adaptive_cfg2": (2, ["adaptive", "3", "0.2", "8", "cfg2"]),
     "d3_adaptive_expsin": (3, ["adaptive", "2", "0.3", "5", "expsin:0.4,-0.3,0.25,-0.2,0.35,0.1"]),
+    # non-separable a and c (the harness's "gen" plug-in), for the per-cell coefficient path
+    "d3_l4_gen": (3, ["regular", "4", "gen"]),
+    "d5_l3_gen": (5, ["regular", "3", "gen"]),
+    "d3_adaptive_gen": (3, ["adaptive", "3", "0.15", "6", "gen"]),
 }
 
 
@@ -66,12 +70,17 @@ def make_grids(levels):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--grids", default="", help="comma-separated sum levels of the refined grids to write")
+    ap.add_argument("--only", default="", help="comma-separated case names to (re)write; default all")
     args = ap.parse_args()
     if args.grids:
         make_grids([int(v) for v in args.grids.split(",")])
         return
+    only = set(filter(None, args.only.split(",")))
     meta = {}
     for name, (d, args) in CASES.items():
+        meta[name] = {"d": d, "argv": args}
+        if only and name not in only:
+            continue
         out = os.path.join(HERE, name)
         os.makedirs(out, exist_ok=True)
         exe = os.path.join(REF, "ref_harness_d%d" % d)
@@ -85,7 +94,6 @@ def main():
             cmd = [exe, "adaptive", args[1], args[2], args[3], args[4], out]
         print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True, env=env)
-        meta[name] = {"d": d, "argv": args}
     with open(os.path.join(HERE, "cases.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
 

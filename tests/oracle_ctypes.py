@@ -12,6 +12,15 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "_build", "liboracle.so")
 
 COEF_CONST, COEF_CFG2, COEF_CFG3, COEF_EXPSIN = 0, 1, 2, 3
+# non-separable pair a = 1 + x1 x2 + sin(pi (x1 + x2)), c = 1 + sin(pi x1 x2) / 2 (x2 = x1 at d = 1)
+COEF_GEN = 10
+
+
+def gen_coef(x):
+    """(a, c) of COEF_GEN at the points x (rows of d coordinates)."""
+    x = np.atleast_2d(x)
+    x1, x2 = x[:, 0], x[:, 1 % x.shape[1]]
+    return 1.0 + x1 * x2 + np.sin(np.pi * (x1 + x2)), 1.0 + 0.5 * np.sin(np.pi * x1 * x2)
 
 _lib = None
 

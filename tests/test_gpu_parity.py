@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import loadsg_amd as L
-from oracle_ctypes import COEF_CFG2, COEF_CFG3, COEF_CONST, COEF_EXPSIN, Oracle, regular_grid
+from oracle_ctypes import COEF_CFG2, COEF_CFG3, COEF_CONST, COEF_EXPSIN, COEF_GEN, Oracle, gen_coef, regular_grid
 
 pytestmark = pytest.mark.gpu
 
@@ -42,6 +42,8 @@ def stencil_of(argv):
         return L.VariableCoefficient(L.COEF_CFG2), dict(kind=COEF_CFG2)
     if spec == "cfg3":
         return L.VariableCoefficient(L.COEF_CFG3), dict(kind=COEF_CFG3)
+    if spec == "gen":     # non-separable a, c: sampled per cell on the host (cell-wise path)
+        return (L.CellFunction(a=lambda x: gen_coef(x)[0], c=lambda x: gen_coef(x)[1]), dict(kind=COEF_GEN))
     xi = [float(t) for t in spec[7:].split(",")]
     return L.VariableCoefficient(L.COEF_EXPSIN, xi=xi), dict(kind=COEF_EXPSIN, xi=np.array(xi))
 

@@ -17,7 +17,7 @@ import os
 import numpy as np
 import pytest
 
-from oracle_ctypes import (COEF_CFG2, COEF_CFG3, COEF_CONST, COEF_EXPSIN, Oracle,
+from oracle_ctypes import (COEF_CFG2, COEF_CFG3, COEF_CONST, COEF_EXPSIN, COEF_GEN, Oracle,
                            regular_grid)
 
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
@@ -30,6 +30,8 @@ def coef_of(argv):
         return dict(kind=COEF_CFG2)
     if spec == "cfg3":
         return dict(kind=COEF_CFG3)
+    if spec == "gen":
+        return dict(kind=COEF_GEN)
     if spec.startswith("expsin:"):
         return dict(kind=COEF_EXPSIN, xi=np.array([float(t) for t in spec[7:].split(",")]))
     raise ValueError(spec)
