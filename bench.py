@@ -56,6 +56,36 @@ def inputs(batch, rng):
     return xi, eta, m
 
 
+def physical_cores():
+    """Physical cores available to this process (lscpu: cores per socket x sockets, capped
+    by the affinity mask)."""
+    avail = len(os.sched_getaffinity(0))
+    try:
+        out = subprocess.run(["lscpu", "-p=CORE,SOCKET"], capture_output=True, text=True, timeout=30).stdout
+        cores = {tuple(l.split(",")) for l in out.splitlines() if l and not l.startswith("#")}
+        if cores:
+            return max(1, min(avail, len(cores)))
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return avail
+
+
+def baseline_cores():
+    """Threads for the reference CPU path: the physical cores this process may use, at most
+    16 (the CPU share of one GPU on the measurement host)."""
+    return min(physical_cores(), 16)
+
+
+def cpu_model():
+    try:
+        for line in subprocess.run(["lscpu"], capture_output=True, text=True, timeout=30).stdout.splitlines():
+            if line.startswith("Model name:"):
+                return line.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return None
+
+
 def apply_traffic_from_profile(batch):
     """HBM bytes (FETCH_SIZE + WRITE_SIZE, copy-calibrated) of the apply stage of one
     FP32 matvec per batch instance, as recorded from rocprofv3 --pmc runs at this batch."""
@@ -77,8 +107,7 @@ def cpu_baseline(iters_fp64, budget_s):
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_d5")
     if not os.path.exists(exe):
         return None
-    cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
+    cores = baseline_cores()
     env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close")
     xi = ",".join("%.6f" % v for v in np.random.default_rng(SEED).normal(0.0, 0.5, NXI))
     try:
@@ -91,6 +120,7 @@ def cpu_baseline(iters_fp64, budget_s):
     res = json.loads(r.stdout.strip().splitlines()[-1])
     t_mv = res["matvec_s_estimate"]
     return {"value": 1.0 / ((iters_fp64 + 1) * t_mv), "unit": "solves/s", "cores": cores, "kind": "reference",
+            "cpu_model": cpu_model(),
             "sample": "LoAdSG MatrixVectorHomogen (unmodified sources, %d OpenMP threads): calcNodal + cases %s of the "
                       "32 of one d=5 L7 var-coeff matvec (%.1f s timed), scaled by per-case cell counts to %.1f s per "
                       "matvec; x (FP64 GPU CG iterations + 1 = %d) matvecs per solve, preconditioner set-up "
@@ -104,7 +134,7 @@ def ref_sample_matvec(d, level, coef, budget_s):
     exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_d%d" % d)
     if not os.path.exists(exe):
         return None, 0
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = baseline_cores()
     env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close")
     try:
         r = subprocess.run([exe, "samplemv", str(level), coef, str(budget_s)], env=env, capture_output=True,
@@ -114,6 +144,73 @@ def ref_sample_matvec(d, level, coef, budget_s):
     if r.returncode != 0:
         return None, cores
     return json.loads(r.stdout.strip().splitlines()[-1]), cores
+
+
+def workload_cfg1(args):
+    """Config 1: 2D Poisson on the regular level-6 grid (N = 769), u = prod sin(pi x_i),
+    unpreconditioned CG from x0 = 0 to ||f - A x|| <= 1e-10 ||f|| on b_p = int f psi_p;
+    matvecs/s, solves/s and the discretisation errors, beside the reference's whole PCG
+    solve (oracle/_ref/ref_harness_d2 timesolve) timed on this host."""
+    import torch
+    import loadsg_amd as L
+    from loadsg_amd import manufactured as MS
+    d, level = 2, 6
+    grid = L.AdaptiveSparseGrid.regular(d, level)
+    n = grid.size
+    prob = MS.cfg1(d)
+    op = L.MatrixVectorHomogen(grid, L.Poisson())
+    b = prob.load(op)
+    perm = grid.device_order()
+    dev = torch.device("cuda:0")
+    xs = torch.tensor((1.0 / (1.0 + (np.arange(n) % 7)))[np.argsort(perm)][None, :], device=dev)
+    ys = torch.zeros_like(xs)
+    op.multiplication(xs, ys)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(100):
+        op.multiplication(xs, ys)
+    torch.cuda.synchronize()
+    t_mv = (time.perf_counter() - t0) / 100
+    fd = np.zeros(n)
+    fd[perm] = b
+    f = torch.tensor(fd[None, :], device=dev)
+    x = torch.zeros_like(f)
+    rule = L.STOP_REL | L.CG_NO_PRECOND
+    L.CG.solveHomogen(1e-10, x, f, op, stop_rule=rule, maxit=5000)
+    reps = 20
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        x.zero_()
+        info = L.CG.solveHomogen(1e-10, x, f, op, stop_rule=rule, maxit=5000)
+    torch.cuda.synchronize()
+    t_cg = (time.perf_counter() - t0) / reps
+    errs = prob.errors(grid, x[0].cpu().numpy()[perm], b, op)
+    log("cfg1: matvec %.1f us; plain CG %d iterations %.2f ms; L2 error %.3e, energy error %.3e"
+        % (t_mv * 1e6, info["iterations"], t_cg * 1e3, errs["l2"], errs["energy"]))
+    cpu = None
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_d2")
+    if not args.no_cpu_baseline and os.path.exists(exe):
+        cores = baseline_cores()
+        env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close")
+        r = subprocess.run([exe, "timesolve", str(level), "const:1:0", "1e-10"], env=env, capture_output=True,
+                           text=True, timeout=600)
+        if r.returncode == 0:
+            rr = json.loads(r.stdout.strip().splitlines()[-1])
+            cpu = {"value": 1.0 / rr["cg_s"], "unit": "solves/s", "cores": cores, "kind": "reference",
+                   "cpu_model": cpu_model(),
+                   "sample": "LoAdSG CG::solveHomogen (diagonal PCG built inside, reference stop rule eps 1e-10, "
+                             "%d iterations) on the same grid, whole solve timed; matvec %.2f ms"
+                             % (rr["iterations"], 1e3 * rr["matvec_s"]),
+                   "matvec_s": rr["matvec_s"], "cg_s": rr["cg_s"], "setup_s": rr["setup_s"]}
+    out = {"metric": "unpreconditioned CG solves/s to 1e-10 rel. residual (d=2 L6 Poisson)", "value": 1.0 / t_cg,
+           "unit": "solves/s", "n_gpus": 1, "higher_is_better": True, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "d=2 regular sparse grid level 6 (N=%d), Poisson, u = prod sin(pi x_i)" % n},
+           "cpu_baseline": cpu,
+           "extra": {"matvec_us": t_mv * 1e6, "cg_iterations": info["iterations"],
+                     "cg_rel_residual": info["rel_residual"], "cg_ms": t_cg * 1e3, "l2_error": errs["l2"],
+                     "energy_error": errs["energy"], "nodal_max_error": errs["nodal_max"]}}
+    print(json.dumps(out))
 
 
 def workload_cfg2(args):
@@ -139,9 +236,15 @@ def workload_cfg2(args):
         ev1.record()
         torch.cuda.synchronize()
         res[B] = ev0.elapsed_time(ev1) / 10
+    from loadsg_amd import manufactured as MS
+    prob = MS.cfg2(d)
     op = L.MatrixVectorHomogen(grid, coef, precision=64, batch=1)
-    f = torch.zeros(1, n, dtype=torch.float64, device=dev)
-    op.rhs_sines(rng.standard_normal((1, NETA)), rng.integers(1, 4, NETA).astype(np.int32), out=f)
+    # load vector of the manufactured u = prod x_i (1 - x_i) exp(x_1): b_p = int f psi_p
+    b_host = prob.load(op)
+    perm = grid.device_order()
+    fd = np.zeros(n)
+    fd[perm] = b_host
+    f = torch.tensor(fd[None, :], device=dev)
     x = torch.zeros_like(f)
     L.CG.solveHomogen(1e-10, x, f, op, stop_rule=L.STOP_REL, maxit=5000)
     x.zero_()
@@ -150,14 +253,16 @@ def workload_cfg2(args):
     info = L.CG.solveHomogen(1e-10, x, f, op, stop_rule=L.STOP_REL, maxit=5000)
     torch.cuda.synchronize()
     t_cg = time.perf_counter() - t0
-    log("cfg2: matvec %.3f ms (b1), %.3f ms (b64); CG %d iterations %.3f s" % (res[1], res[64], info["iterations"],
-                                                                              t_cg))
+    errs = prob.errors(grid, x[0].cpu().numpy()[perm], b_host, op)
+    log("cfg2: matvec %.3f ms (b1), %.3f ms (b64); CG %d iterations %.3f s; L2 error %.3e, nodal max %.3e"
+        % (res[1], res[64], info["iterations"], t_cg, errs["l2"], errs["nodal_max"]))
     cpu = None
     if not args.no_cpu_baseline:
         r, cores = ref_sample_matvec(d, level, "cfg2", args.cpu_budget)
         if r is not None:
             t_mv = r["matvec_s_estimate"]
             cpu = {"value": 1.0 / t_mv, "unit": "matvecs/s", "cores": cores, "kind": "reference",
+                   "cpu_model": cpu_model(),
                    "sample": "LoAdSG MatrixVectorHomogen, d=4 level 8, CFG2 coefficient (%d OpenMP threads): "
                              "calcNodal + cases %s (%.0f%% of the cell work), scaled to the full matvec"
                              % (cores, r["cases"], 100 * r["case_weight_fraction"]),
@@ -167,6 +272,8 @@ def workload_cfg2(args):
            "config": {"workload": "d=4 regular sparse grid level 8 (N=%d), a=1+0.5 prod cos(pi x_i), c=1+x_1" % n},
            "cpu_baseline": cpu,
            "extra": {"matvec_b1_ms": res[1], "matvec_b64_ms": res[64], "matvecs_per_s_b64": 64e3 / res[64],
+                     "rhs": "b_p = int f psi_p, f = -div(a grad u) + c u for u = prod x_i (1 - x_i) exp(x_1)",
+                     "l2_error": errs["l2"], "nodal_max_error": errs["nodal_max"],
                      "cg_iterations": info["iterations"], "cg_rel_residual": info["rel_residual"],
                      "cg_s": t_cg, "cg_solves_per_s": 1.0 / t_cg,
                      "speedup_matvec_vs_cpu": None if cpu is None else cpu["matvec_s"] * 1e3 / res[1],
@@ -281,7 +388,7 @@ def workload_transform(args):
     cpu = None
     if not args.no_cpu_baseline:
         exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness_d5")
-        cores = min(len(os.sched_getaffinity(0)), 16)
+        cores = baseline_cores()
         env = dict(os.environ, OMP_NUM_THREADS=str(cores), OMP_PROC_BIND="close")
         res = {}
         for lev in (5, 6):
@@ -296,6 +403,7 @@ def workload_transform(args):
             growth = res[6]["prew_by_nodal_s"] / res[5]["prew_by_nodal_s"]
             t7 = res[6]["prew_by_nodal_s"] * growth
             cpu = {"value": 1.0 / t7, "unit": "transforms/s", "cores": cores, "kind": "reference",
+                   "cpu_model": cpu_model(),
                    "sample": "LoAdSG calcPrewByNodal (unmodified sources, %d OpenMP threads) timed at d=5 level 6 "
                              "(%.1f s, N=%d) and level 5 (%.2f s); level 7 extrapolated by the %.2fx growth: %.0f s"
                              % (cores, res[6]["prew_by_nodal_s"], res[6]["n"], res[5]["prew_by_nodal_s"], growth, t7),
@@ -329,14 +437,14 @@ def main():
                     help="coefficient samples for the warm-start training set (x 8 basis solves = 20,000 "
                          "GPU-generated solves); 0 = zero start")
     ap.add_argument("--warm-steps", type=int, default=96000)
-    ap.add_argument("--workload", choices=("d5l7", "cfg2", "cfg3", "cfg4", "transform"), default="d5l7")
+    ap.add_argument("--workload", choices=("d5l7", "cfg1", "cfg2", "cfg3", "cfg4", "transform"), default="d5l7")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (batched solves shard by instances across GPUs)",
               file=sys.stderr)
         sys.exit(2)
     if args.workload != "d5l7":
-        {"cfg2": workload_cfg2, "cfg3": workload_cfg3, "cfg4": workload_cfg4,
+        {"cfg1": workload_cfg1, "cfg2": workload_cfg2, "cfg3": workload_cfg3, "cfg4": workload_cfg4,
          "transform": workload_transform}[args.workload](args)
         return
 

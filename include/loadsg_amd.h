@@ -55,6 +55,9 @@ extern "C" {
 /* CG stopping rules */
 #define LSG_STOP_REF_ABS 0   /* reference rule: r^T D^-1 r < eps^2 (cg_method.h:845-871) */
 #define LSG_STOP_REL 1       /* ||f - A x||_2 <= eps ||f||_2 */
+/* or-ed into stop_rule of lsg_cg_solve: plain CG, no diagonal preconditioner (D = 1; with
+ * LSG_STOP_REF_ABS the rule reads r^T r < eps^2) */
+#define LSG_CG_NO_PRECOND 0x100
 
 typedef struct lsg_grid lsg_grid;
 typedef struct lsg_op lsg_op;
@@ -149,6 +152,15 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
 /* Load vector b_p = int f psi_p of f = sum_j eta_j prod_i sin(m_j pi x_i) (exact 1D
  * quadrature of each factor), for every instance: eta[inst*nterms + j], m[j]. */
 int lsg_rhs_sines(lsg_op* op, int nterms, const double* eta, const int32_t* m, void* b, int layout);
+/* Load vector of a sum of separable functions f = sum_j w_j prod_i g_ji(x_i):
+ * b_p = int f psi_p = sum_j w[inst*nterms + j] prod_i int g_ji psi^(i)_p, the 1D moments
+ * integrated with a 4-point Gauss rule on every cell of the grid's finest 1D level (exact
+ * for polynomial factors of degree <= 6).  vals[(j*d + i)*count + q] is g_ji at node q of
+ * lsg_rhs_nodes.  Replaces the load-vector assembly of a driver (the reference forms
+ * f_h = M prew(f_I) instead, examples_doxygen/poisson_regular.cpp:57-113); no counterpart
+ * in the reference library. */
+int lsg_rhs_nodes(const lsg_op* op, int64_t* count, double* nodes);
+int lsg_rhs_separable(lsg_op* op, int nterms, const double* vals, const double* w, void* b, int layout);
 /* ------------------------------------------------- basis transformations */
 /* FP64 operators on grids with complete depth boxes (regular grids); LSG_ERR_UNSUPPORTED
  * otherwise.  Vectors as in lsg_matvec; in and out must be different arrays.

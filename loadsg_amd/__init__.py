@@ -30,6 +30,7 @@ LIB_PATH = os.environ.get("LOADSG_AMD_LIB") or os.path.join(_HERE, "_lib", "libl
 LSG_HOST_SLOT, LSG_DEVICE = 0, 1
 LSG_ERR_NAN, LSG_ERR_NOCONV = -5, -6
 STOP_REF_ABS, STOP_REL = 0, 1
+CG_NO_PRECOND = 0x100     # or-ed into stop_rule: plain CG without the diagonal preconditioner
 COEF_CONST, COEF_CFG2, COEF_CFG3, COEF_EXPSIN = 0, 1, 2, 3
 COEF_CELL, COEF_FUNC = 4, 5
 
@@ -85,6 +86,8 @@ def lib():
             "lsg_cg_solve": (i32, [vp, vp, vp, dbl, i32, i32, vp, i32]),
             "lsg_ir_solve": (i32, [vp, vp, vp, vp, dbl, dbl, i32, vp, i32]),
             "lsg_rhs_sines": (i32, [vp, i32, vp, vp, vp, i32]),
+            "lsg_rhs_nodes": (i32, [vp, ctypes.POINTER(i64), vp]),
+            "lsg_rhs_separable": (i32, [vp, i32, vp, vp, vp, i32]),
             "lsg_last_timing": (i32, [vp, vp, i32]),
             "lsg_hierarchize": (i32, [vp, vp, vp, i32]),
             "lsg_dehierarchize": (i32, [vp, vp, vp, i32]),
@@ -397,6 +400,41 @@ class MatrixVectorHomogen:
             return out
         b = self._new_host()
         _check(lib().lsg_rhs_sines(self._h, m.size, _ptr(eta), _ptr(m), _ptr(b), LSG_HOST_SLOT))
+        return b
+
+    def rhs_nodes(self):
+        """1D quadrature nodes at which rhs_separable reads its factors."""
+        n = ctypes.c_int64()
+        _check(lib().lsg_rhs_nodes(self._h, ctypes.byref(n), None))
+        x = np.zeros(n.value)
+        _check(lib().lsg_rhs_nodes(self._h, ctypes.byref(n), _ptr(x)))
+        return x
+
+    def rhs_separable(self, factors, weights=None, out=None):
+        """Load vector b_p = int f psi_p of f = sum_j w_j prod_i g_ji(x_i).  factors[j][i] is a
+        vectorised function of one coordinate (an array of points -> values); weights has
+        shape (batch, nterms) (default: all ones)."""
+        d = self.grid.dim
+        x = self.rhs_nodes()
+        nt = len(factors)
+        vals = np.zeros((nt, d, x.size))
+        for j, fac in enumerate(factors):
+            if len(fac) != d:
+                raise LoadsgError("term %d: expected %d factors" % (j, d))
+            for i, g in enumerate(fac):
+                vals[j, i] = np.broadcast_to(np.asarray(g(x), dtype=np.float64), x.shape)
+        w = np.ones((self.batch, nt)) if weights is None else np.ascontiguousarray(np.atleast_2d(weights),
+                                                                                   dtype=np.float64)
+        if w.shape != (self.batch, nt):
+            raise LoadsgError("weights: expected shape (%d, %d), got %s" % (self.batch, nt, w.shape))
+        vals = np.ascontiguousarray(vals)
+        if out is not None and hasattr(out, "data_ptr"):
+            self._dev(out, "out")
+            _sync_torch(out)
+            _check(lib().lsg_rhs_separable(self._h, nt, _ptr(vals), _ptr(w), _ptr(out), LSG_DEVICE))
+            return out
+        b = self._new_host()
+        _check(lib().lsg_rhs_separable(self._h, nt, _ptr(vals), _ptr(w), _ptr(b), LSG_HOST_SLOT))
         return b
 
     # ---- basis transformations (FP64 operators, grids with complete depth boxes)

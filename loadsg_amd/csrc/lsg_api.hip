@@ -175,6 +175,53 @@ int cellValues(const Grid& g, const lsg_coef& c, std::vector<double>& ac) {
     return LSG_OK;
 }
 
+// ---- separable load vectors: 1D moments on the host
+// 4-point Gauss-Legendre rule on [-1, 1] (abscissae and weights of the Legendre
+// polynomial P_4), applied on every cell of the finest 1D level of the grid: every
+// level-t hat and prewavelet is linear there.
+constexpr double GL_X[4] = {-0.861136311594052575223946488893, -0.339981043584856264802665759103,
+                            0.339981043584856264802665759103, 0.861136311594052575223946488893};
+constexpr double GL_W[4] = {0.347854845137453857373063949222, 0.652145154862546142626936050778,
+                            0.652145154862546142626936050778, 0.347854845137453857373063949222};
+
+void rhsNodes(int maxLevel, std::vector<double>& x) {
+    const int64_t cells = int64_t(1) << maxLevel;
+    x.resize((size_t)cells * 4);
+    for (int64_t c = 0; c < cells; ++c)
+        for (int q = 0; q < 4; ++q) x[(size_t)c * 4 + q] = (double(c) + 0.5 * (1.0 + GL_X[q])) / double(cells);
+}
+
+// moments m(t, k) = int g psi_{t,k} of one factor given at the rhsNodes, for every level
+// t <= maxLevel and odd k: first the level-t hat moments, then the prewavelet stencil;
+// entry 2^(t-1) - 1 + (k - 1) / 2 of tab (length 2^maxLevel - 1)
+void factorMoments(int maxLevel, const double* g, double* tab) {
+    const int64_t cells = int64_t(1) << maxLevel;
+    const double hf = 1.0 / double(cells);
+    std::vector<double> H;
+    for (int t = 1; t <= maxLevel; ++t) {
+        const int64_t nJ = (int64_t(1) << t) - 1, span = int64_t(1) << (maxLevel - t);
+        H.assign((size_t)nJ + 2, 0.0);
+        for (int64_t J = 1; J <= nJ; ++J) {
+            double acc = 0.0;
+            for (int64_t c = (J - 1) * span; c < (J + 1) * span; ++c)
+                for (int q = 0; q < 4; ++q) {
+                    const double x = (double(c) + 0.5 * (1.0 + GL_X[q])) * hf;
+                    const double hat = 1.0 - std::fabs(x * double(int64_t(1) << t) - double(J));
+                    acc += 0.5 * hf * GL_W[q] * g[(size_t)c * 4 + q] * hat;
+                }
+            H[(size_t)J] = acc;
+        }
+        for (int64_t k = 1; k <= nJ; k += 2) {
+            int32_t pos[5];
+            double co[5];
+            const int m = prewaveletStencil1D((int32_t)k, t, pos, co);
+            double v = 0.0;
+            for (int e = 0; e < m; ++e) v += co[e] * H[(size_t)pos[e]];
+            tab[(int64_t(1) << (t - 1)) - 1 + (k - 1) / 2] = v;
+        }
+    }
+}
+
 }  // namespace
 
 namespace {
@@ -231,6 +278,7 @@ struct lsg_op {
     DevMem r, h, dv, g, D, f, x, tmp, partials, st, aux;
     DevMem epsInst, rows;        // iterative refinement: per-instance inner tolerances, rows to clear
     DevMem norms2;               // lsg_vector_norms results (2 per instance)
+    DevMem ones;                 // D = 1 of the unpreconditioned CG
     DevMem rhsEta, rhsM;         // lsg_rhs_sines parameters (grown on demand)
     bool solverReady = false, diagValid = false;
     hipStream_t stream = nullptr;
@@ -240,6 +288,7 @@ struct lsg_op {
     // valid for the x pointer / precision / chunk length it was captured with
     hipGraphExec_t chunkExec = nullptr;
     const void* chunkX = nullptr;
+    const void* chunkD = nullptr;   // diagonal the chunk divides by (D, or ones when unpreconditioned)
     int chunkK = 0;
     bool chunkX64 = false;
     bool capturing = false;   // matvec launches record no timing events while captured
@@ -541,7 +590,7 @@ int computeDiag(lsg_op* op) {
 template <typename V>
 int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, int maxit,
              std::vector<CGState>& out, bool xZero = false, const double* epsInst = nullptr,
-             const std::vector<int>* open = nullptr) {
+             const std::vector<int>* open = nullptr, bool precond = true) {
     const int64_t n = op->n();
     const int B = op->batch;
     hipStream_t s = op->stream;
@@ -550,13 +599,24 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
     // the cell coefficients are refreshed eagerly here, outside any captured chunk: a chunk
     // replayed after lsg_op_set_coef then reads the current ones
     if ((rc = ensureCellCoef(op))) return rc;
-    if ((rc = computeDiag(op))) return rc;
+    if (precond) {
+        if ((rc = computeDiag(op))) return rc;
+    } else if (!op->ones.p) {     // unpreconditioned CG: D = 1
+        HIPCHK(op->ones.alloc((size_t)n * B * op->vs));
+        if (op->prec == 64) {
+            std::vector<double> one((size_t)n * B, 1.0);
+            HIPCHK(hipMemcpy(op->ones.p, one.data(), one.size() * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> one((size_t)n * B, 1.0f);
+            HIPCHK(hipMemcpy(op->ones.p, one.data(), one.size() * 4, hipMemcpyHostToDevice));
+        }
+    }
     if ((rc = ensureThreads(op))) return rc;
     V* r = op->r.as<V>();
     V* h = op->h.as<V>();
     V* dv = op->dv.as<V>();
     V* g = op->g.as<V>();
-    V* D = op->D.as<V>();
+    V* D = precond ? op->D.as<V>() : op->ones.as<V>();
     const int nb = cg_partial_blocks(n);
     double* part = op->partials.as<double>();
     CGState* st = op->st.as<CGState>();
@@ -589,7 +649,7 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
         if (op->nMap == 0 && useGraphs()) {
             // the chunk's launches depend only on the operator, x and k: capture once,
             // then replay (the kernels read the convergence state on the device)
-            if (!op->chunkExec || op->chunkX != x || op->chunkK != k || op->chunkX64 != x64) {
+            if (!op->chunkExec || op->chunkX != x || op->chunkK != k || op->chunkX64 != x64 || op->chunkD != D) {
                 if (op->chunkExec) HIPCHK(hipGraphExecDestroy(op->chunkExec));
                 op->chunkExec = nullptr;
                 hipGraph_t graph = nullptr;
@@ -612,6 +672,7 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
                 op->chunkX = x;
                 op->chunkK = k;
                 op->chunkX64 = x64;
+                op->chunkD = D;
             }
             HIPCHK(hipGraphLaunch(op->chunkExec, s));
         } else {
@@ -1180,8 +1241,13 @@ int lsg_cg_solve(lsg_op* op, const void* f, void* x, double eps, int stop_rule, 
         xD = op->x.p;
     }
     std::vector<CGState> st;
-    if (op->prec == 64) rc = cgDevice<double>(op, (const double*)fD, xD, true, eps, stop_rule, maxit, st);
-    else rc = cgDevice<float>(op, (const float*)fD, xD, false, eps, stop_rule, maxit, st);
+    const bool precond = !(stop_rule & LSG_CG_NO_PRECOND);
+    const int rule = stop_rule & ~LSG_CG_NO_PRECOND;
+    if (rule != LSG_STOP_REF_ABS && rule != LSG_STOP_REL) return fail(LSG_ERR_ARG, "unknown stopping rule");
+    if (op->prec == 64)
+        rc = cgDevice<double>(op, (const double*)fD, xD, true, eps, rule, maxit, st, false, nullptr, nullptr, precond);
+    else
+        rc = cgDevice<float>(op, (const float*)fD, xD, false, eps, rule, maxit, st, false, nullptr, nullptr, precond);
     if (rc) return rc;
     (void)tot;
     if (layout == LSG_HOST_SLOT && (rc = download(op, op->x.p, (double*)x))) return rc;
@@ -1326,6 +1392,50 @@ int lsg_rhs_sines(lsg_op* op, int nterms, const double* eta, const int32_t* m, v
     else
         launch_rhs_sines<float>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, nterms, dEta.as<double>(),
                                 dM.as<int32_t>(), (float*)dst, g.n, op->batch, op->stream, pos, act);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(op->stream));
+    if (layout == LSG_DEVICE) return LSG_OK;
+    return download(op, op->ys.p, (double*)b);
+}
+
+int lsg_rhs_nodes(const lsg_op* op, int64_t* count, double* nodes) {
+    if (!op || !count) return fail(LSG_ERR_ARG, "null argument");
+    std::vector<double> x;
+    rhsNodes(op->G->maxLevel, x);
+    *count = (int64_t)x.size();
+    if (nodes) std::memcpy(nodes, x.data(), x.size() * sizeof(double));
+    return LSG_OK;
+}
+
+int lsg_rhs_separable(lsg_op* op, int nterms, const double* vals, const double* w, void* b, int layout) {
+    if (!op || nterms < 1 || !vals || !w || !b) return fail(LSG_ERR_ARG, "bad arguments");
+    const Grid& g = *op->G;
+    const int d = g.d;
+    const int L = g.maxLevel;
+    const int64_t nq = (int64_t(1) << L) * 4;
+    const int64_t tabLen = (int64_t(1) << L) - 1;
+    std::vector<double> tab((size_t)nterms * d * tabLen);
+    for (int64_t q = 0; q < (int64_t)nterms * d * nq; ++q)
+        if (!std::isfinite(vals[q])) return fail(LSG_ERR_NAN, "non-finite factor value");
+    for (int j = 0; j < nterms; ++j)
+        for (int i = 0; i < d; ++i)
+            factorMoments(L, vals + ((size_t)j * d + i) * nq, &tab[((size_t)j * d + i) * tabLen]);
+    DevMem& dW = op->rhsEta;
+    DevMem& dT = op->rhsM;
+    HIPCHK(hipStreamSynchronize(op->stream));   // an earlier call's kernels may still read them
+    if (dW.bytes < (size_t)nterms * op->batch * 8) HIPCHK(dW.alloc((size_t)nterms * op->batch * 8));
+    if (dT.bytes < tab.size() * 8) HIPCHK(dT.alloc(tab.size() * 8));
+    HIPCHK(hipMemcpy(dW.p, w, (size_t)nterms * op->batch * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(dT.p, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+    void* dst = layout == LSG_DEVICE ? b : op->ys.p;
+    const int32_t* pos = op->sparse ? op->sd.pos : nullptr;
+    const uint8_t* act = op->sparse ? op->activeDev.as<uint8_t>() : nullptr;
+    if (op->prec == 64)
+        launch_rhs_tab<double>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), d, nterms, dW.as<double>(),
+                               dT.as<double>(), tabLen, (double*)dst, g.n, op->batch, op->stream, pos, act);
+    else
+        launch_rhs_tab<float>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), d, nterms, dW.as<double>(),
+                              dT.as<double>(), tabLen, (float*)dst, g.n, op->batch, op->stream, pos, act);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(op->stream));
     if (layout == LSG_DEVICE) return LSG_OK;

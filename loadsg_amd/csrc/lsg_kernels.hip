@@ -1093,6 +1093,42 @@ __global__ void __launch_bounds__(256) rhs_sines_kernel(const SumItem* __restric
     }
 }
 
+// Load vector of a sum of separable functions: b_p = sum_j w_j prod_i m_ji(T_i, k_i) with
+// m_ji(t, k) = int g_ji(x) psi_{t,k}(x) dx the 1D moment of factor (j, i) against the
+// prewavelet of odd position k at level t, read from tab[(j d + i) tabLen + 2^(t-1) - 1 +
+// (k - 1) / 2] (moments computed on the host).
+template <typename V>
+__global__ void __launch_bounds__(256) rhs_tab_kernel(const SumItem* __restrict__ items, int d, int nterms,
+                                                      const double* __restrict__ w, const double* __restrict__ tab,
+                                                      int64_t tabLen, V* __restrict__ b, int64_t bStride,
+                                                      const int32_t* __restrict__ pos,
+                                                      const uint8_t* __restrict__ active) {
+    const int inst = blockIdx.y;
+    const LSG_CONST SumItem& it = cptr(items)[blockIdx.x];
+    int m[MAXD];
+    for (int i = 0; i < d; ++i) m[i] = 1 << (it.T[i] - 1);
+    const int cnt = (int)it.count;
+    for (int p = threadIdx.x; p < cnt; p += blockDim.x) {
+        int64_t off[MAXD];
+        int rem = p;
+        for (int i = 0; i < d; ++i) {
+            const int j = rem % m[i];
+            rem /= m[i];
+            const int t = it.T[i];
+            const int k = pos ? pos[(it.blk_off + p) * d + i] : 2 * j + 1;
+            off[i] = (int64_t)(1 << (t - 1)) - 1 + (k - 1) / 2;
+        }
+        double total = 0.0;
+        for (int term = 0; term < nterms; ++term) {
+            double prod = w[inst * nterms + term];
+            for (int i = 0; i < d; ++i) prod *= tab[(int64_t)(term * d + i) * tabLen + off[i]];
+            total += prod;
+        }
+        if (active && !active[it.blk_off + p]) total = 0.0;   // hanging nodes are no unknowns
+        b[inst * bStride + it.blk_off + p] = V(total);
+    }
+}
+
 // ------------------------------------------------------------- reductions
 __device__ inline double wave_sum(double v) {
 #pragma unroll
@@ -1418,6 +1454,14 @@ void launch_rhs_sines(const SumItem* items, int nitems, int d, int nterms, const
 }
 
 template <typename V>
+void launch_rhs_tab(const SumItem* items, int nitems, int d, int nterms, const double* w, const double* tab,
+                    int64_t tabLen, V* b, int64_t bStride, int batch, hipStream_t s, const int32_t* pos,
+                    const uint8_t* active) {
+    hipLaunchKernelGGL(rhs_tab_kernel<V>, dim3(nitems, batch), dim3(128), 0, s, items, d, nterms, w, tab, tabLen, b,
+                       bStride, pos, active);
+}
+
+template <typename V>
 void launch_cg_init(const V* f, V* r, V* h, V* dv, const V* D, int64_t n, int batch, double* partials, int nb,
                     hipStream_t s) {
     hipLaunchKernelGGL(cg_init_kernel<V>, dim3(nb, batch), dim3(256), 0, s, f, r, h, dv, D, n, n, partials);
@@ -1475,6 +1519,8 @@ int cg_partial_blocks(int64_t n) { return (int)gridFor(n); }
                                  hipStream_t);                                                                       \
     template void launch_rhs_sines<V>(const SumItem*, int, int, int, const double*, const int32_t*, V*, int64_t,     \
                                       int, hipStream_t, const int32_t*, const uint8_t*);                             \
+    template void launch_rhs_tab<V>(const SumItem*, int, int, int, const double*, const double*, int64_t, V*,        \
+                                    int64_t, int, hipStream_t, const int32_t*, const uint8_t*);                      \
     template void launch_cg_init<V>(const V*, V*, V*, V*, const V*, int64_t, int, double*, int, hipStream_t);         \
     template void launch_cg_iter_tail<V>(V*, void*, bool, V*, V*, const V*, const V*, int64_t, int, double*, int,     \
                                          CGState*, hipStream_t);

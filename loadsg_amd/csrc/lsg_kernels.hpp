@@ -128,6 +128,10 @@ template <typename V>
 void launch_diag(const SumItem* items, int nitems, int d, int nA, int nC, const V* coefTab, int64_t coefStride,
                  int64_t tabLen, V* z, int64_t zStride, int batch, hipStream_t s);
 template <typename V>
+void launch_rhs_tab(const SumItem* items, int nitems, int d, int nterms, const double* w, const double* tab,
+                    int64_t tabLen, V* b, int64_t bStride, int batch, hipStream_t s, const int32_t* pos,
+                    const uint8_t* active);
+template <typename V>
 void launch_rhs_sines(const SumItem* items, int nitems, int d, int nterms, const double* eta, const int32_t* m, V* b,
                       int64_t bStride, int batch, hipStream_t s, const int32_t* pos = nullptr,
                       const uint8_t* active = nullptr);

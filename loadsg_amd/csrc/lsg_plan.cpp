@@ -730,6 +730,8 @@ void regularIndexTable(int d, int level, std::vector<int32_t>& idx) {
     dfsSons(d, I, level, seen, idx);
 }
 
+int prewaveletStencil1D(int32_t k, int t, int32_t pos[5], double c[5]) { return prewavelet1D(k, t, pos, c); }
+
 int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid& g, std::string& err,
               bool forceSparse) {
     if (d < 1 || d > MAXD || n < 1 || !idx) { err = "bad arguments"; return LSG_ERR_ARG; }

@@ -220,6 +220,9 @@ int buildTransform(const Grid& g, TrTables& tr, std::string& err);
 // grid keeps the dense device order.
 int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid& g, std::string& err,
               bool forceSparse = false);
+// 1D prewavelet of odd position k at level t as (position, coefficient) pairs of level-t
+// nodal hats (IndexDimension::nextFive, indices/index.cc:180-327); returns the count.
+int prewaveletStencil1D(int32_t k, int t, int32_t pos[5], double c[5]);
 // Reference slot order of AddRecursiveSonsOfPoint(center, level).
 void regularIndexTable(int d, int level, std::vector<int32_t>& idx);
 

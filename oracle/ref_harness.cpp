@@ -18,6 +18,7 @@
 //   ref_harness adaptive <startlevel> <radius> <maxlevel> <coef> <outdir> [what...]
 //   ref_harness timemv   <level> <coef> <reps>            (timing sample, prints JSON)
 //   ref_harness timetr   <level>                          (calcPrewByNodal / calcNodalByPrew timing, JSON)
+//   ref_harness timesolve <level> <coef> [eps]           (the driver's whole PCG solve timed, JSON)
 // coef: const:<a0>:<c0> | cfg2 | cfg3 | expsin:<xi1>,<xi2>,... | gen
 //       (gen: a = 1 + x1 x2 + sin(pi (x1 + x2)), c = 1 + sin(pi x1 x2) / 2, not separable)
 // what: matvec precond cg transform   (default: all)
@@ -376,6 +377,55 @@ int main(int argc, char** argv) {
         std::sort(ts.begin(), ts.end());
         std::printf("{\"n\": %zu, \"setup_s\": %.6f, \"matvec_s_median\": %.6f, \"reps\": %d, \"check\": %.17g}\n",
                     n, std::chrono::duration<double>(t1 - t0).count(), ts[ts.size() / 2], reps, chk);
+        return 0;
+    }
+    if (mode == "timesolve" && argc >= 4) {
+        // CPU timing of the reference driver's whole solve on a regular grid
+        // (examples_doxygen/poisson_regular.cpp:39-120): grid and operator set-up, the
+        // right-hand side f_h = M prew(f_I) of f = d pi^2 prod sin(pi x_i), one matvec, and
+        // CG::solveHomogen to eps (its diagonal preconditioner built inside, cg_method.h:805).
+        int level = std::atoi(argv[2]);
+        CoefSpec cs = parseCoef(argv[3]);
+        double eps = argc >= 5 ? std::atof(argv[4]) : 1e-10;
+        const int d = DimensionSparseGrid;
+        auto now = [] { return std::chrono::steady_clock::now(); };
+        auto secs = [](std::chrono::steady_clock::time_point a, std::chrono::steady_clock::time_point b) {
+            return std::chrono::duration<double>(b - a).count();
+        };
+        auto t0 = now();
+        AdaptiveSparseGrid grid;
+        IndexDimension center;
+        grid.AddRecursiveSonsOfPoint(center, level);
+        MultiLevelAdaptiveSparseGrid mgrid(&grid);
+        MatrixVectorHomogen m(grid, mgrid, true, false);
+        VarCoeffStencil var(grid, cs);
+        HelmHoltz helm(grid);
+        size_t n = grid.getMaximalOccupiedSecondTable();
+        auto t1 = now();
+        VectorSparseG fvec(grid), prew(grid), f(grid), x(grid), u(grid);
+        fvec = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            if (!grid.getActiveTable()[i]) continue;
+            IndexDimension I = grid.getIndexOfTable(i);
+            fvec.setValue((unsigned long)i, double(d) * M_PI * M_PI * sinFunc(I));
+        }
+        prew = 0.0;
+        calcPrewByNodal(prew, fvec);
+        f = 0.0;
+        m.multiplication(prew, f, helm);
+        auto t2 = now();
+        VectorSparseG y(grid);
+        y = 0.0;
+        m.multiplication(prew, y, var);
+        auto t3 = now();
+        int it = 0;
+        double tp = 0.0;
+        x = 0.0;
+        CG::solveHomogen(eps, x, f, &it, u, m, var, &tp);
+        auto t4 = now();
+        std::printf("{\"n\": %zu, \"setup_s\": %.6f, \"rhs_s\": %.6f, \"matvec_s\": %.6f, \"cg_s\": %.6f, "
+                    "\"iterations\": %d, \"x0\": %.17g}\n",
+                    n, secs(t0, t1), secs(t1, t2), secs(t2, t3), secs(t3, t4), it, x.getValue((unsigned long)0));
         return 0;
     }
     std::fprintf(stderr, "bad arguments\n");

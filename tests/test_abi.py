@@ -107,3 +107,25 @@ def test_invalid_input_errors():
         L.AdaptiveSparseGrid(np.array([[2, 3]], dtype=np.int32))        # index 3 is outside (0,1)
     with pytest.raises(L.LoadsgError):
         L.AdaptiveSparseGrid(np.array([[2, 2], [2, 2]], dtype=np.int32))  # duplicate
+
+
+def test_capacity_and_structure_errors():
+    """Grids the path cannot take return a status instead of failing later: a 1D depth
+    above 15 (MaximumDepth, abbrevi.h:12), a depth list that is not downward closed."""
+    with pytest.raises(L.LoadsgError, match="error -2"):
+        L.AdaptiveSparseGrid(np.array([[2 ** 16, 2]], dtype=np.int32))      # 1D depth 16
+    # depth (2,1) without (1,1): point (4,2) alone
+    with pytest.raises(L.LoadsgError, match="error -2"):
+        L.AdaptiveSparseGrid(np.array([[4, 2]], dtype=np.int32))
+
+
+def test_cells_of_regular_grid():
+    """lsg_grid_cells (host only): sum over depths T of prod 2^T_i cells of depth T+1 on a
+    regular grid, centres inside (0,1), grouped by depth in depth-list order."""
+    for d, level in ((2, 4), (3, 3)):
+        g = L.AdaptiveSparseGrid.regular(d, level)
+        x, t = g.cells()
+        n = sum(2 ** k * math.comb(k + d - 1, d - 1) for k in range(level + 1))
+        assert x.shape == (2 ** d * n, d)
+        assert np.all((x > 0) & (x < 1))
+        assert np.all(np.diff(t) >= 0)
