@@ -142,6 +142,18 @@ int lsg_precond_diag(lsg_op* op, void* z, int layout);
  * a CG scalar turns NaN. */
 int lsg_cg_solve(lsg_op* op, const void* f, void* x, double eps, int stop_rule, int maxit,
                  lsg_solve_info* info, int layout);
+/* lsg_cg_solve (layout LSG_DEVICE) without any device allocation: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The CG work vectors, the
+ * diagonal, the partial sums and the per-instance state live in one fixed 64 MiB device
+ * scratch per GPU, held until the process ends: 5 * n * batch values of the operator's
+ * precision plus a few KB must fit, otherwise LSG_ERR_CAPACITY.  The operator itself is
+ * created beforehand (lsg_op_create allocates).  Every call on a GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (two host threads, or two operators' streams)
+ * race.  No CG chunk graphs, no compaction of finished instances; results equal
+ * lsg_cg_solve's within rounding.  Same statuses as lsg_cg_solve. */
+int lsg_cg_solve_noalloc(lsg_op* op, const void* f, void* x, double eps, int stop_rule, int maxit,
+                         lsg_solve_info* info);
 /* Mixed precision: FP32 operator inner PCG inside FP64 iterative refinement; the
  * outer residual f - A x, all dot products/norms and the stop test are FP64 on the
  * device.  op64 and op32 must share the grid and batch.  stop rule LSG_STOP_REL.

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -237,6 +238,7 @@ struct lsg_op {
     lsg_grid* grid = nullptr;
     const Grid* G = nullptr;   // the grid the kernels run on: grid->g, or its cell-wise copy
     bool cellKind = false;     // general coefficient given per cell (cellCoef filled on the host)
+    int device = 0;            // HIP device the operator's memory and stream belong to
     int prec = 64;
     size_t vs = 8;
     int batch = 1;
@@ -556,29 +558,34 @@ int ensureSolver(lsg_op* op) {
 // read at every solve)
 bool useGraphs() { return std::getenv("LSG_NO_GRAPH") == nullptr; }
 
-int computeDiag(lsg_op* op) {
-    if (op->diagValid) return LSG_OK;
+// precond3 diagonal of every instance into D (V values, device order)
+int launchDiag(lsg_op* op, void* Dp) {
     const Grid& g = *op->G;
     int64_t cs = (int64_t)(op->nA + op->nC) * g.d * op->tabLen;
     if (op->sparse) {
         if (int rc = ensureCellCoef(op)) return rc;
         if (op->prec == 64)
             launch_sp_diag<double>(op->sd, op->activeDev.as<uint8_t>(), (int)g.n, op->cellCoef.as<double>(),
-                                   2 * op->nCells, op->D.as<double>(), g.n, op->batch, op->stream);
+                                   2 * op->nCells, (double*)Dp, g.n, op->batch, op->stream);
         else
             launch_sp_diag<float>(op->sd, op->activeDev.as<uint8_t>(), (int)g.n, op->cellCoef.as<float>(),
-                                  2 * op->nCells, op->D.as<float>(), g.n, op->batch, op->stream);
+                                  2 * op->nCells, (float*)Dp, g.n, op->batch, op->stream);
         HIPCHK(hipGetLastError());
-        op->diagValid = true;
         return LSG_OK;
     }
     if (op->prec == 64)
         launch_diag<double>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, op->nA, op->nC,
-                            op->coefTab.as<double>(), cs, op->tabLen, op->D.as<double>(), g.n, op->batch, op->stream);
+                            op->coefTab.as<double>(), cs, op->tabLen, (double*)Dp, g.n, op->batch, op->stream);
     else
         launch_diag<float>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, op->nA, op->nC,
-                           op->coefTab.as<float>(), cs, op->tabLen, op->D.as<float>(), g.n, op->batch, op->stream);
+                           op->coefTab.as<float>(), cs, op->tabLen, (float*)Dp, g.n, op->batch, op->stream);
     HIPCHK(hipGetLastError());
+    return LSG_OK;
+}
+
+int computeDiag(lsg_op* op) {
+    if (op->diagValid) return LSG_OK;
+    if (int rc = launchDiag(op, op->D.p)) return rc;
     op->diagValid = true;
     return LSG_OK;
 }
@@ -587,13 +594,25 @@ int computeDiag(lsg_op* op) {
 // CG::solveHomogen (applications/cg_method.h:759-886).  `open`, if given, lists the
 // instances that may need iterations (the others have a zero right-hand side): the
 // matvecs launch only those from the first chunk on.
+// Work buffers of one CG solve: the op's own (compaction of finished instances and
+// chunk graphs allowed), or caller-provided scratch (neither: plain launches only).
+struct CgBufs {
+    void *r, *h, *dv, *g, *D;
+    double* part;
+    CGState* st;
+    bool own;
+};
+
+template <typename V>
+int cgRun(lsg_op* op, const CgBufs& bf, const V* f, void* x, bool x64, double eps, int rule, int maxit,
+          std::vector<CGState>& out, bool xZero, const double* epsInst, const std::vector<int>* open);
+
 template <typename V>
 int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, int maxit,
              std::vector<CGState>& out, bool xZero = false, const double* epsInst = nullptr,
              const std::vector<int>* open = nullptr, bool precond = true) {
     const int64_t n = op->n();
     const int B = op->batch;
-    hipStream_t s = op->stream;
     int rc = ensureSolver(op);
     if (rc) return rc;
     // the cell coefficients are refreshed eagerly here, outside any captured chunk: a chunk
@@ -612,14 +631,27 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
         }
     }
     if ((rc = ensureThreads(op))) return rc;
-    V* r = op->r.as<V>();
-    V* h = op->h.as<V>();
-    V* dv = op->dv.as<V>();
-    V* g = op->g.as<V>();
-    V* D = precond ? op->D.as<V>() : op->ones.as<V>();
+    CgBufs bf{op->r.p, op->h.p, op->dv.p, op->g.p, precond ? op->D.p : op->ones.p, op->partials.as<double>(),
+              op->st.as<CGState>(), true};
+    return cgRun<V>(op, bf, f, x, x64, eps, rule, maxit, out, xZero, epsInst, open);
+}
+
+// PCG loop (CG::solveHomogen, applications/cg_method.h:759-886) on the buffers bf.
+template <typename V>
+int cgRun(lsg_op* op, const CgBufs& bf, const V* f, void* x, bool x64, double eps, int rule, int maxit,
+          std::vector<CGState>& out, bool xZero, const double* epsInst, const std::vector<int>* open) {
+    const int64_t n = op->n();
+    const int B = op->batch;
+    hipStream_t s = op->stream;
+    int rc = LSG_OK;
+    V* r = (V*)bf.r;
+    V* h = (V*)bf.h;
+    V* dv = (V*)bf.dv;
+    V* g = (V*)bf.g;
+    V* D = (V*)bf.D;
     const int nb = cg_partial_blocks(n);
-    double* part = op->partials.as<double>();
-    CGState* st = op->st.as<CGState>();
+    double* part = bf.part;
+    CGState* st = bf.st;
     // r = A x (x in V for the matvec); a start vector known to be zero needs no matvec
     if (xZero) {
         launch_zero(r, n * B * (int64_t)sizeof(V), s);
@@ -639,14 +671,14 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
     op->nMap = 0;
     std::vector<int> act;     // host copy of the compacted list; the stream syncs before each rewrite
     act.reserve(B);
-    if (open && !open->empty() && (int)open->size() < B) {
+    if (bf.own && open && !open->empty() && (int)open->size() < B) {
         act = *open;
         HIPCHK(hipMemcpyAsync(op->actMap.p, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, s));
         op->nMap = (int)act.size();
     }
     for (int done = 0; done < maxit;) {
         int k = std::min(chunk, maxit - done);
-        if (op->nMap == 0 && useGraphs()) {
+        if (bf.own && op->nMap == 0 && useGraphs()) {
             // the chunk's launches depend only on the operator, x and k: capture once,
             // then replay (the kernels read the convergence state on the device)
             if (!op->chunkExec || op->chunkX != x || op->chunkK != k || op->chunkX64 != x64 || op->chunkD != D) {
@@ -690,7 +722,7 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
         if (act.empty()) break;
         // finished instances drop out of the matvec grids: their workgroups would only
         // read their flag and exit, and they delay the dispatch of the ones still running
-        if ((int)act.size() < B) {
+        if (bf.own && (int)act.size() < B) {
             HIPCHK(hipMemcpyAsync(op->actMap.p, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, s));
             op->nMap = (int)act.size();
         }
@@ -868,6 +900,71 @@ int transformAbi(lsg_op* op, int what, const void* in, void* out, int layout) {
     return download(op, op->tr->work2.p, (double*)out);
 }
 
+// ---- CG without device allocation (lsg_cg_solve_noalloc)
+// Work buffers of that entry point: the CG vectors r, h, d, g, the diagonal D, the
+// partial sums and the per-instance CG state.  One copy per device, held until the
+// process ends; host code reaches it only through the address hipGetSymbolAddress
+// returns for the current device (noallocScratch), never through the symbol itself.
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+std::mutex g_scratchMu;
+void* g_scratchDev[256] = {};
+
+// device address of `scratch` on the current device (looked up once per device)
+int noallocScratch(void** out) {
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 256) return fail(LSG_ERR_DEVICE, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchDev[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(LSG_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratchDev[dev] = p;
+    }
+    *out = g_scratchDev[dev];
+    return LSG_OK;
+}
+
+template <typename V>
+int cgNoalloc(lsg_op* op, const void* f, void* x, double eps, int stopRule, int maxit, std::vector<CGState>& st) {
+    const bool precond = !(stopRule & LSG_CG_NO_PRECOND);
+    const int rule = stopRule & ~LSG_CG_NO_PRECOND;
+    if (rule != LSG_STOP_REF_ABS && rule != LSG_STOP_REL) return fail(LSG_ERR_ARG, "unknown stopping rule");
+    const int64_t n = op->n();
+    const int B = op->batch;
+    const int nb = cg_partial_blocks(n);
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t vec = up((size_t)n * B * sizeof(V));
+    const size_t part = up((size_t)B * 3 * nb * sizeof(double));
+    const size_t stb = up((size_t)B * sizeof(CGState));
+    const size_t need = 5 * vec + part + stb;
+    if (need > NOALLOC_SCRATCH_BYTES)
+        return fail(LSG_ERR_CAPACITY, "lsg_cg_solve_noalloc: " + std::to_string(need) + " bytes of work buffers (" +
+                                          std::to_string(n) + " points x " + std::to_string(B) +
+                                          " instances) exceed its " + std::to_string(NOALLOC_SCRATCH_BYTES) +
+                                          "-byte scratch; use lsg_cg_solve");
+    void* base = nullptr;
+    if (int rc = noallocScratch(&base)) return rc;
+    unsigned char* p = (unsigned char*)base;
+    CgBufs bf{p, p + vec, p + 2 * vec, p + 3 * vec, p + 4 * vec, (double*)(p + 5 * vec),
+              (CGState*)(p + 5 * vec + part), false};
+    // every scratch buffer is written before it is read (the diagonal and D = 1 here, r by
+    // the start matvec, h, d and the partials by the init kernel, the state by the start
+    // kernel), so nothing carried over from an earlier call is used
+    int rc = ensureCellCoef(op);
+    if (rc) return rc;
+    if (precond) {
+        if ((rc = launchDiag(op, bf.D))) return rc;
+    } else {
+        launch_fill_one(bf.D, n * B, sizeof(V) == 8, op->stream);
+    }
+    if ((rc = ensureThreads(op))) return rc;
+    return cgRun<V>(op, bf, (const V*)f, x, sizeof(V) == 8, eps, rule, maxit, st, false, nullptr, nullptr);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1040,6 +1137,7 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     }
     int dev = 0;
     HIPCHK(hipGetDevice(&dev));
+    op->device = dev;
     int maxLds = 0;
     HIPCHK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     op->maxLds = maxLds;
@@ -1251,6 +1349,34 @@ int lsg_cg_solve(lsg_op* op, const void* f, void* x, double eps, int stop_rule, 
     if (rc) return rc;
     (void)tot;
     if (layout == LSG_HOST_SLOT && (rc = download(op, op->x.p, (double*)x))) return rc;
+    bool nan = false, noconv = false;
+    for (int b = 0; b < op->batch; ++b) {
+        if (info) {
+            info[b].iterations = st[b].it;
+            info[b].outer = 0;
+            info[b].final_delta = st[b].delta;
+            info[b].rel_residual = st[b].ff > 0 ? std::sqrt(st[b].rr / st[b].ff) : std::sqrt(st[b].rr);
+            info[b].converged = st[b].converged;
+        }
+        nan |= st[b].converged < 0;
+        noconv |= st[b].converged == 0;
+    }
+    if (nan) return fail(LSG_ERR_NAN, "NaN in CG");
+    if (noconv) return fail(LSG_ERR_NOCONV, "CG stopped at maxit before the stopping rule was met");
+    return LSG_OK;
+}
+
+int lsg_cg_solve_noalloc(lsg_op* op, const void* f, void* x, double eps, int stop_rule, int maxit,
+                         lsg_solve_info* info) {
+    if (!op || !f || !x || maxit < 0) return fail(LSG_ERR_ARG, "bad arguments");
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != op->device) HIPCHK(hipSetDevice(op->device));
+    std::vector<CGState> st;
+    int rc = op->prec == 64 ? cgNoalloc<double>(op, f, x, eps, stop_rule, maxit, st)
+                            : cgNoalloc<float>(op, f, x, eps, stop_rule, maxit, st);
+    if (cur != op->device) (void)hipSetDevice(cur);
+    if (rc) return rc;
     bool nan = false, noconv = false;
     for (int b = 0; b < op->batch; ++b) {
         if (info) {

@@ -1324,6 +1324,12 @@ __global__ void __launch_bounds__(256) axpy_correction_kernel(double* __restrict
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
         x[i] += double(e[i]);
 }
+template <typename V>
+__global__ void __launch_bounds__(256) fill_one_kernel(V* __restrict__ p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = V(1);
+}
+
 __global__ void __launch_bounds__(256) zero_kernel(void* __restrict__ p, int64_t bytes) {
     uint32_t* q = reinterpret_cast<uint32_t*>(p);
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < bytes / 4; i += (int64_t)gridDim.x * blockDim.x)
@@ -1496,6 +1502,10 @@ void launch_cast_f2d(const float* a, double* b, int64_t total, hipStream_t s) {
 }
 void launch_axpy_correction(double* x, const float* e, int64_t total, hipStream_t s) {
     hipLaunchKernelGGL(axpy_correction_kernel, dim3(gridFor(total)), dim3(256), 0, s, x, e, total);
+}
+void launch_fill_one(void* p, int64_t n, bool fp64, hipStream_t s) {
+    if (fp64) hipLaunchKernelGGL(fill_one_kernel<double>, dim3(gridFor(n)), dim3(256), 0, s, (double*)p, n);
+    else hipLaunchKernelGGL(fill_one_kernel<float>, dim3(gridFor(n)), dim3(256), 0, s, (float*)p, n);
 }
 void launch_zero(void* p, int64_t bytes, hipStream_t s) {
     hipLaunchKernelGGL(zero_kernel, dim3(gridFor(bytes / 4)), dim3(256), 0, s, p, bytes);

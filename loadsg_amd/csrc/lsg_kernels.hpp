@@ -151,6 +151,7 @@ void launch_cast_d2f(const double* a, float* b, int64_t total, hipStream_t s);
 void launch_cast_f2d(const float* a, double* b, int64_t total, hipStream_t s);
 void launch_axpy_correction(double* x, const float* e, int64_t total, hipStream_t s);
 void launch_zero(void* p, int64_t bytes, hipStream_t s);
+void launch_fill_one(void* p, int64_t n, bool fp64, hipStream_t s);   // p[i] = 1 (double or float)
 // rows rows[0..nrows) (device list) of a row-major (., n) FP64 array set to zero
 void launch_zero_rows(double* a, int64_t n, const int* rows, int nrows, hipStream_t s);
 int cg_partial_blocks(int64_t n);
