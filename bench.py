@@ -465,6 +465,7 @@ def main():
         s = L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[b])
         op64.set_coefficient(b, s)
         op32.set_coefficient(b, s)
+    stats = op32.stats()        # the plan the operators run on
     f = torch.zeros(B, n, dtype=torch.float64, device=dev)
     op64.rhs_sines(eta, m, out=f)
     x = torch.zeros(B, n, dtype=torch.float64, device=dev)

@@ -109,6 +109,10 @@ int lsg_grid_device_order(const lsg_grid* g, int64_t* dev_of_slot);
  * [11]=streaming-model values of the stages the apply stage performs (prolongation along
  * the last prolongation dimension, stencil, restrictions with the gM exchange, conversion). */
 int lsg_grid_stats(const lsg_grid* g, int64_t* out, int nout);
+/* The same statistics for the plan an operator runs on (dense grids: every prolongation
+ * dimension materialised and single-depth apply items by default; [10] and [11] then
+ * exclude the prolongation, which the prolongation stage performs). */
+int lsg_op_stats(const lsg_op* op, int64_t* out, int nout);
 /* Cells of depth T+1 of every depth T in the grid (CellData / CellStructure,
  * cells/CellStructure.h:23-61: level-T cells whose interior corners are all stored), in
  * the order LSG_COEF_CELL values are read: depth by depth in DepthList order

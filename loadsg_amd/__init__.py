@@ -76,6 +76,7 @@ def lib():
             "lsg_grid_slot_order": (i32, [vp, vp, vp]),
             "lsg_grid_device_order": (i32, [vp, vp]),
             "lsg_grid_stats": (i32, [vp, vp, i32]),
+            "lsg_op_stats": (i32, [vp, vp, i32]),
             "lsg_grid_cells": (i32, [vp, ctypes.POINTER(i64), vp, vp]),
             "lsg_grid_destroy": (i32, [vp]),
             "lsg_op_create": (i32, [vp, ctypes.POINTER(_Coef), i32, i32, ctypes.POINTER(vp)]),
@@ -491,6 +492,14 @@ class MatrixVectorHomogen:
             v = self._host(x, "x")
             _check(lib().lsg_vector_norms(self._h, _ptr(v), _ptr(out), LSG_HOST_SLOT))
         return out if self.batch > 1 else out[0]
+
+    def stats(self):
+        """Statistics of the plan this operator runs on (keys as AdaptiveSparseGrid.stats)."""
+        out = np.zeros(12, dtype=np.int64)
+        _check(lib().lsg_op_stats(self._h, _ptr(out), 12))
+        keys = ["depths", "multilevel_size", "apply_items", "wavefront_levels", "stream_values", "max_box",
+                "case_buffer", "w_buffer", "part_buffer", "moved_values", "apply_values", "stream_apply_values"]
+        return dict(zip(keys, (int(v) for v in out)))
 
     def last_timing(self):
         out = np.zeros(3)

@@ -65,6 +65,19 @@ struct DevMem {
 // level t.  Table of one (term, dim): level t occupies [2^t - 2, 2^(t+1) - 2).
 bool cellCoefKind(int kind) { return kind == LSG_COEF_CELL || kind == LSG_COEF_FUNC; }
 
+// Dense grids run on their single-item plan (every prolongation dimension materialised)
+// for batches up to singlePlanMaxBatch(): measured faster than the families along the last
+// prolongation dimension at every batch from 1 to 512 at d=5 level 7 (FP64: 0.21 vs
+// 0.52 ms at batch 1, 31.3 vs 32.8 ms at 512).  LSG_SINGLE_PLAN_BATCH=0 selects the family
+// plan.  Batches up to oneClassMaxBatch() run one apply launch per wavefront level
+// (LSG_ONE_CLASS_BATCH overrides).
+int envInt(const char* name, int dflt) {
+    const char* v = std::getenv(name);
+    return v ? std::atoi(v) : dflt;
+}
+int singlePlanMaxBatch() { return envInt("LSG_SINGLE_PLAN_BATCH", 1 << 30); }
+int oneClassMaxBatch() { return envInt("LSG_ONE_CLASS_BATCH", 4); }
+
 int coefTerms(const lsg_coef& c, int d, int& nA, int& nC) {
     switch (c.kind) {
         case LSG_COEF_CELL: nA = c.cell_a != nullptr; nC = c.cell_c != nullptr; break;
@@ -237,6 +250,7 @@ struct TrDev {
 struct lsg_op {
     lsg_grid* grid = nullptr;
     const Grid* G = nullptr;   // the grid the kernels run on: grid->g, or its cell-wise copy
+    const Plan* P = nullptr;   // its plan: G->plan, or the single-item plan of a dense grid
     bool cellKind = false;     // general coefficient given per cell (cellCoef filled on the host)
     int device = 0;            // HIP device the operator's memory and stream belong to
     int prec = 64;
@@ -323,7 +337,7 @@ int ensureCellCoef(lsg_op* op) {
 template <typename V>
 int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st, int nSub) {
     const Grid& g = *op->G;
-    const Plan& P = g.plan;
+    const Plan& P = *op->P;
     const int d = g.d;
     const int nm = nSub > 0 ? nSub : (st ? op->nMap : 0);
     const int* map = nm > 0 ? op->actMap.as<int>() : nullptr;
@@ -376,7 +390,7 @@ template <typename V>
 int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st, int nSub = 0) {
     if (op->sparse) return runMatvecSparse<V>(op, x, y, st, nSub);
     const Grid& g = *op->G;
-    const Plan& P = g.plan;
+    const Plan& P = *op->P;
     const int d = g.d;
     // inside a CG solve with a compacted instance list, or for a residual over a
     // subset (nSub > 0), launch only the slots listed in actMap
@@ -574,10 +588,10 @@ int launchDiag(lsg_op* op, void* Dp) {
         return LSG_OK;
     }
     if (op->prec == 64)
-        launch_diag<double>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, op->nA, op->nC,
+        launch_diag<double>(op->dSumItems.as<SumItem>(), (int)op->P->sum.size(), g.d, op->nA, op->nC,
                             op->coefTab.as<double>(), cs, op->tabLen, (double*)Dp, g.n, op->batch, op->stream);
     else
-        launch_diag<float>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, op->nA, op->nC,
+        launch_diag<float>(op->dSumItems.as<SumItem>(), (int)op->P->sum.size(), g.d, op->nA, op->nC,
                            op->coefTab.as<float>(), cs, op->tabLen, (float*)Dp, g.n, op->batch, op->stream);
     HIPCHK(hipGetLastError());
     return LSG_OK;
@@ -965,6 +979,13 @@ int cgNoalloc(lsg_op* op, const void* f, void* x, double eps, int stopRule, int 
     return cgRun<V>(op, bf, (const V*)f, x, sizeof(V) == 8, eps, rule, maxit, st, false, nullptr, nullptr);
 }
 
+void planStats(const Grid& g, const Plan& P, int64_t* out, int nout) {
+    int64_t v[12] = {(int64_t)g.depths.size(), g.nml, P.nApply, (int64_t)P.apply.size(),
+                     (int64_t)P.streamValues, P.maxBox, P.caseSize, P.wSize, P.partSize, (int64_t)P.movedValues,
+                     (int64_t)P.applyValues, (int64_t)P.streamApplyValues};
+    for (int i = 0; i < nout && i < 12; ++i) out[i] = v[i];
+}
+
 }  // namespace
 
 extern "C" {
@@ -1012,11 +1033,13 @@ int lsg_grid_device_order(const lsg_grid* g, int64_t* dev_of_slot) {
 
 int lsg_grid_stats(const lsg_grid* g, int64_t* out, int nout) {
     if (!g || !out) return fail(LSG_ERR_ARG, "null argument");
-    const Plan& P = g->g.plan;
-    int64_t v[12] = {(int64_t)g->g.depths.size(), g->g.nml, P.nApply, (int64_t)P.apply.size(),
-                     (int64_t)P.streamValues, P.maxBox, P.caseSize, P.wSize, P.partSize, (int64_t)P.movedValues,
-                     (int64_t)P.applyValues, (int64_t)P.streamApplyValues};
-    for (int i = 0; i < nout && i < 12; ++i) out[i] = v[i];
+    planStats(g->g, g->g.plan, out, nout);
+    return LSG_OK;
+}
+
+int lsg_op_stats(const lsg_op* op, int64_t* out, int nout) {
+    if (!op || !out) return fail(LSG_ERR_ARG, "null argument");
+    planStats(*op->G, *op->P, out, nout);
     return LSG_OK;
 }
 
@@ -1054,7 +1077,15 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         if (rc) return rc;
     }
     const Grid& g = *op->G;
-    const Plan& P = g.plan;
+    op->P = &g.plan;
+    if (!g.sparse && batch <= singlePlanMaxBatch()) {
+        // small batches: every prolongation dimension materialised, single-item apply
+        // launches (more workgroups per wavefront level, a shorter critical path)
+        std::string err;
+        if (int rc = buildSinglePlan(grid->g, err)) return fail(rc, err);
+        op->P = grid->g.planSingle.get();
+    }
+    const Plan& P = *op->P;
     if (coefTerms(*coef, g.d, op->nA, op->nC)) return fail(LSG_ERR_ARG, "bad coefficient");
     if (op->nA + op->nC == 0) return fail(LSG_ERR_ARG, "zero operator");
     op->general = !(op->nA == 1 && op->nC == 0);
@@ -1068,9 +1099,11 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     // take fewer, wider classes
     const int64_t fine[12] = {128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 4096, 8192, 65536};
     const int64_t coarse[5] = {128, 512, 2048, 8192, 65536};
+    const int64_t one[1] = {65536};
     const bool fineCls = batch >= 16;
-    const int64_t* bounds = fineCls ? fine : coarse;
-    const int ncls = fineCls ? 12 : 5;
+    const bool oneCls = batch <= oneClassMaxBatch();
+    const int64_t* bounds = fineCls ? fine : (oneCls ? one : coarse);
+    const int ncls = fineCls ? 12 : (oneCls ? 1 : 5);
     for (size_t L = 0; !op->sparse && L < P.apply.size(); ++L) {
         std::vector<std::vector<FamilyItem>> bucket(ncls);
         std::vector<int64_t> bmax(ncls, 0), wmax(ncls, 0);
@@ -1513,10 +1546,10 @@ int lsg_rhs_sines(lsg_op* op, int nterms, const double* eta, const int32_t* m, v
     const int32_t* pos = op->sparse ? op->sd.pos : nullptr;
     const uint8_t* act = op->sparse ? op->activeDev.as<uint8_t>() : nullptr;
     if (op->prec == 64)
-        launch_rhs_sines<double>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, nterms, dEta.as<double>(),
+        launch_rhs_sines<double>(op->dSumItems.as<SumItem>(), (int)op->P->sum.size(), g.d, nterms, dEta.as<double>(),
                                  dM.as<int32_t>(), (double*)dst, g.n, op->batch, op->stream, pos, act);
     else
-        launch_rhs_sines<float>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), g.d, nterms, dEta.as<double>(),
+        launch_rhs_sines<float>(op->dSumItems.as<SumItem>(), (int)op->P->sum.size(), g.d, nterms, dEta.as<double>(),
                                 dM.as<int32_t>(), (float*)dst, g.n, op->batch, op->stream, pos, act);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(op->stream));
@@ -1557,10 +1590,10 @@ int lsg_rhs_separable(lsg_op* op, int nterms, const double* vals, const double* 
     const int32_t* pos = op->sparse ? op->sd.pos : nullptr;
     const uint8_t* act = op->sparse ? op->activeDev.as<uint8_t>() : nullptr;
     if (op->prec == 64)
-        launch_rhs_tab<double>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), d, nterms, dW.as<double>(),
+        launch_rhs_tab<double>(op->dSumItems.as<SumItem>(), (int)op->P->sum.size(), d, nterms, dW.as<double>(),
                                dT.as<double>(), tabLen, (double*)dst, g.n, op->batch, op->stream, pos, act);
     else
-        launch_rhs_tab<float>(op->dSumItems.as<SumItem>(), (int)g.plan.sum.size(), d, nterms, dW.as<double>(),
+        launch_rhs_tab<float>(op->dSumItems.as<SumItem>(), (int)op->P->sum.size(), d, nterms, dW.as<double>(),
                               dT.as<double>(), tabLen, (float*)dst, g.n, op->batch, op->stream, pos, act);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(op->stream));

@@ -354,8 +354,12 @@ int buildSparse(Grid& g, const std::vector<int>& tOf, std::string& err) {
     return 0;
 }
 
-int buildPlan(Grid& g, std::string& err) {
-    Plan& P = g.plan;
+// allProl: materialise every prolongation dimension of a case in the case buffer, so that
+// every apply item is a single depth (sparse grids; dense grids for small batches, where
+// splitting the families along the last prolongation dimension shortens the critical path
+// of each wavefront level).  Otherwise the last prolongation dimension is fused into the
+// apply stage (families).
+int buildPlan(Grid& g, Plan& P, bool allProl, std::string& err) {
     const int d = g.d;
     const int nd = (int)g.depths.size();
     P = Plan();
@@ -443,7 +447,7 @@ int buildPlan(Grid& g, std::string& err) {
         if (sortedOf[c].empty()) continue;
         std::vector<int> Pd;
         for (int i = 0; i < d; ++i) if (!R[i]) Pd.push_back(i);
-        const int nStages = g.sparse ? (int)Pd.size() : (int)Pd.size() - 1;
+        const int nStages = allProl ? (int)Pd.size() : (int)Pd.size() - 1;
         unsigned Q = 0;
         for (int j = 0; j < nStages; ++j) {
             Q |= 1u << Pd[j];
@@ -504,9 +508,9 @@ int buildPlan(Grid& g, std::string& err) {
         // the last one (sparse grids: all of them)
         // (-1: the nodal box itself, whose offset is boxOff[t])
         std::unordered_map<int, int64_t> caseOff;
-        if (nP >= (g.sparse ? 1 : 2)) {
+        if (nP >= (allProl ? 1 : 2)) {
             unsigned Q = 0;
-            for (int i = 0; i < d; ++i) if (!R[i] && (g.sparse || i != lastP)) Q |= 1u << i;
+            for (int i = 0; i < d; ++i) if (!R[i] && (allProl || i != lastP)) Q |= 1u << i;
             for (int t : sorted) {
                 const Src& sr = prefixSrc.at(Q).at(t);
                 caseOff[t] = sr.nodal ? -1 : sr.off;
@@ -521,12 +525,12 @@ int buildPlan(Grid& g, std::string& err) {
                 for (size_t q = 1; q < ts.size(); ++q) {
                     const double v = double(g.boxSize[ts[q - 1]]) + 2.0 * double(g.boxSize[ts[q]]);
                     stream += v;
-                    if (dd == lastP) streamApply += v;
+                    if (dd == lastP && !allProl) streamApply += v;   // fused into the apply stage
                 }
         }
         // apply families along the last prolongation dim (single items if none)
         std::vector<std::vector<int>> applyFam;
-        if (lastP >= 0 && !g.sparse) {
+        if (lastP >= 0 && !allProl) {
             std::map<Depth, std::vector<int>> fam;
             if (int rc = families(lastP, fam)) return rc;
             for (auto& kv : fam) applyFam.push_back(kv.second);
@@ -537,7 +541,7 @@ int buildPlan(Grid& g, std::string& err) {
         for (size_t f = 0; f < applyFam.size(); ++f)
             for (int t : applyFam[f]) famOf[t] = (int)f;
         const int famBase = (int)famList.size();
-        for (auto& ts : applyFam) famList.push_back({c, lastP, ts});
+        for (auto& ts : applyFam) famList.push_back({c, allProl ? -1 : lastP, ts});
 
         // symbolic replay of the gM exchanges in the reference order
         std::unordered_map<int, Read> state;   // depth -> latest write
@@ -546,7 +550,7 @@ int buildPlan(Grid& g, std::string& err) {
             X.c = c; X.t = t; X.k = 0;
             for (int i = 0; i < d; ++i) if (R[i]) X.r[X.k++] = i;
             for (int j = 0; j <= MAXD; ++j) { X.reads[j] = Read(); X.wOff[j] = -1; }
-            X.caseOff = nP >= (g.sparse ? 1 : 2) ? caseOff[t] : -1;
+            X.caseOff = nP >= (allProl ? 1 : 2) ? caseOff[t] : -1;
             X.fam = famBase + famOf[t];
             int me = (int)items.size();
             auto st = state.find(t);
@@ -807,7 +811,7 @@ int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid&
             g.slotOfDev[dev] = s;
         }
         if (int rc = buildSparse(g, tOf, err)) return rc;
-        return buildPlan(g, err);
+        return buildPlan(g, g.plan, true, err);
     }
     for (int64_t s = 0; s < n; ++s) {
         const Depth& T = g.depths[tOf[s]];
@@ -826,7 +830,16 @@ int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid&
         g.sparse = true;
         if (int rc = buildSparse(g, tOf, err)) return rc;
     }
-    return buildPlan(g, err);
+    return buildPlan(g, g.plan, g.sparse, err);
+}
+
+int buildSinglePlan(Grid& g, std::string& err) {
+    if (g.planSingle) return 0;
+    if (g.sparse) { err = "sparse grids have single-item plans already"; return LSG_ERR_ARG; }
+    auto P = std::make_shared<Plan>();
+    if (int rc = buildPlan(g, *P, true, err)) return rc;
+    g.planSingle = P;
+    return 0;
 }
 
 

@@ -15,6 +15,7 @@
 #include <array>
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -171,7 +172,12 @@ struct Grid {
     Depth maxDepth{};
     int maxLevel = 0;              // largest 1D depth
     Plan plan;
+    // dense grids: the plan with every prolongation dimension materialised and single-item
+    // apply families (buildSinglePlan; for small batches)
+    std::shared_ptr<Plan> planSingle;
 };
+// Builds g.planSingle (dense grids only).
+int buildSinglePlan(Grid& g, std::string& err);
 
 // Basis transformations on the dense path (every box G_T complete): one item per depth.
 struct TrItem {

@@ -59,7 +59,7 @@ def main():
     c = torch.empty_like(a)
     torch.mul(a, 2.0, out=c)
     torch.cuda.synchronize()
-    st = grid.stats()
+    st = op.stats()
     print(json.dumps({"batch": B, "precision": args.precision, "reps": args.reps,
                       "apply_launches_per_matvec": launches,
                       "apply_ms_per_matvec_events": float(np.mean(apply_ms)),

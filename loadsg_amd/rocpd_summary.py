@@ -54,24 +54,30 @@ def sequence(db, start, tail):
     con = sqlite3.connect(db)
     rows = con.execute("select name, start, end from kernels order by start").fetchall()
     seqs, cur = [], []
-    for name, s, e in rows:
+    for k, (name, s, e) in enumerate(rows):
         if re.search(start, name) and cur:
             seqs.append(cur)
             cur = []
-        cur.append((_short(name), e - s))
+        gap = rows[k + 1][1] - e if k + 1 < len(rows) else 0
+        cur.append((_short(name), e - s, gap))
     seqs.append(cur)
     lengths = defaultdict(int)
     for q in seqs:
         lengths[len(q)] += 1
     L = max(lengths, key=lambda k: lengths[k])
     sel = [q for q in seqs if len(q) == L][-tail:] if tail else [q for q in seqs if len(q) == L]
-    tot = 0.0
-    print("%d sequences of %d dispatches (starting at %r); average per position:" % (len(sel), L, start))
+    tot = gaps = 0.0
+    print("%d sequences of %d dispatches (starting at %r); average per position "
+          "(duration, then the idle gap to the next dispatch):" % (len(sel), L, start))
     for i in range(L):
         v = sum(q[i][1] for q in sel) / len(sel) / 1e3
+        g = sum(q[i][2] for q in sel) / len(sel) / 1e3
         tot += v
-        print("%3d %-58s %10.2f us" % (i, sel[0][i][0][:58], v))
-    print("sum %.2f us per sequence" % tot)
+        if i + 1 < L:
+            gaps += g
+        print("%3d %-58s %10.2f us %10.2f us" % (i, sel[0][i][0][:58], v, g))
+    print("sum %.2f us of kernels per sequence, %.2f us of gaps between them (first to last dispatch: "
+          "%.2f us)" % (tot, gaps, tot + gaps))
 
 
 def pmc(db, kernel, tail, group, calib):
