@@ -23,7 +23,8 @@ its own metric; the default line above is the headline):
         1e-10 against the FP64 solve (loadsg_amd/measure_adaptive.py);
   cfg4  64 instances with d uniform in {3..6} and level in {5..8} (seeded), exp-sine
         coefficients (16 xi), 8-term sine right-hand sides, solved together to 1e-8
-        (loadsg_amd.solve_many: one stream per grid shape);
+        (lsg_ir_solve_many: one host thread advancing every grid shape's solve, the
+        shapes side by side on the GPU);
   transform  nodal values -> prewavelets (calcPrewByNodal), the inverse and the H1 norm on
         the d=5 level-7 grid for --batch instances, beside the reference's calcPrewByNodal.
 """

@@ -168,6 +168,15 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
 /* Load vector b_p = int f psi_p of f = sum_j eta_j prod_i sin(m_j pi x_i) (exact 1D
  * quadrature of each factor), for every instance: eta[inst*nterms + j], m[j]. */
 int lsg_rhs_sines(lsg_op* op, int nterms, const double* eta, const int32_t* m, void* b, int layout);
+/* lsg_ir_solve for several operator pairs at once (instances on different grids: an
+ * uneven batch), device layout: group q solves op64[q] / op32[q] (one grid and batch per
+ * group) with f[q], x[q] and fills info[q] (lsg_op_batch entries, may be NULL).  One host
+ * thread advances every group in lock-step; each phase (FP64 residuals, inner FP32 CG
+ * chunks, corrections) is issued for all groups on their operators' streams, so the GPU
+ * runs the groups side by side, and the host joins them once per phase.  No counterpart
+ * in the reference. */
+int lsg_ir_solve_many(int ngroups, lsg_op* const* op64, lsg_op* const* op32, const void* const* f, void* const* x,
+                      double rel_tol, double inner_rel_tol, int maxit, lsg_solve_info* const* info);
 /* Load vector of a sum of separable functions f = sum_j w_j prod_i g_ji(x_i):
  * b_p = int f psi_p = sum_j w[inst*nterms + j] prod_i int g_ji psi^(i)_p, the 1D moments
  * integrated with a 4-point Gauss rule on every cell of the grid's finest 1D level (exact

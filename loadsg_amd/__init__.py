@@ -86,6 +86,7 @@ def lib():
             "lsg_precond_diag": (i32, [vp, vp, i32]),
             "lsg_cg_solve": (i32, [vp, vp, vp, dbl, i32, i32, vp, i32]),
             "lsg_ir_solve": (i32, [vp, vp, vp, vp, dbl, dbl, i32, vp, i32]),
+            "lsg_ir_solve_many": (i32, [i32, vp, vp, vp, vp, dbl, dbl, i32, vp]),
             "lsg_rhs_sines": (i32, [vp, i32, vp, vp, vp, i32]),
             "lsg_rhs_nodes": (i32, [vp, ctypes.POINTER(i64), vp]),
             "lsg_rhs_separable": (i32, [vp, i32, vp, vp, vp, i32]),
@@ -607,22 +608,32 @@ class CG:
 
 
 def solve_many(problems, rel_tol, inner_rel_tol=1e-4, maxit=30):
-    """Mixed-precision solves of instances on different grids at once.
+    """Mixed-precision solves of instances on different grids at once (lsg_ir_solve_many).
 
     problems: list of (x, f, op64, op32), each an operator pair with its own grid and
-    batch and device tensors x, f of shape (batch, N) (see CG.solveMixed).  Every
-    operator pair runs on its own stream, driven by its own host thread, so the
-    groups share the GPU.  Returns the per-group lists of solve infos."""
-    from concurrent.futures import ThreadPoolExecutor
-    import torch
-    for x, f, _, _ in problems:
-        _sync_torch(x, f)
-    torch.cuda.synchronize()
-    with ThreadPoolExecutor(max(1, len(problems))) as ex:
-        futs = [ex.submit(CG.solveMixed, rel_tol, x, f, o64, o32, inner_rel_tol, maxit)
-                for (x, f, o64, o32) in problems]
-        res = [fu.result() for fu in futs]
-    return [r if isinstance(r, list) else [r] for r in res]
+    batch and device tensors x, f of shape (batch, N) (see CG.solveMixed).  One host
+    thread advances all groups in lock-step; the GPU runs them side by side on the
+    operators' streams.  Returns the per-group lists of solve infos."""
+    G = len(problems)
+    if G == 0:
+        return []
+    ops64 = (ctypes.c_void_p * G)()
+    ops32 = (ctypes.c_void_p * G)()
+    fs = (ctypes.c_void_p * G)()
+    xs = (ctypes.c_void_p * G)()
+    infos = [(SolveInfo * p[2].batch)() for p in problems]
+    iptr = (ctypes.c_void_p * G)()
+    for q, (x, f, o64, o32) in enumerate(problems):
+        o64._dev(x, "x")
+        o64._dev(f, "f")
+        ops64[q], ops32[q] = o64._h.value, o32._h.value
+        fs[q], xs[q] = f.data_ptr(), x.data_ptr()
+        iptr[q] = ctypes.cast(infos[q], ctypes.c_void_p).value
+    _sync_torch(*[p[0] for p in problems])
+    rc = lib().lsg_ir_solve_many(G, ops64, ops32, fs, xs, rel_tol, inner_rel_tol, maxit, iptr)
+    if rc != LSG_ERR_NOCONV:
+        _check(rc)
+    return [[inf[b].as_dict() for b in range(p[2].batch)] for inf, p in zip(infos, problems)]
 
 
 def device_available():

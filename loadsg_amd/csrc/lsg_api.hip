@@ -7,8 +7,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <chrono>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/loadsg_amd.h"
@@ -56,6 +58,23 @@ struct DevMem {
         if (p) { (void)hipFree(p); p = nullptr; }
         bytes = std::max<size_t>(b, 16);
         return hipMalloc(&p, bytes);
+    }
+    template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+// Pinned host memory (hipHostMalloc): targets of asynchronous copies that must not
+// serialise the host thread.
+struct PinnedBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() { if (p) (void)hipHostFree(p); }
+    hipError_t alloc(size_t b) {
+        if (p) { (void)hipHostFree(p); p = nullptr; }
+        bytes = std::max<size_t>(b, 16);
+        return hipHostMalloc(&p, bytes, hipHostMallocDefault);
     }
     template <typename T> T* as() const { return reinterpret_cast<T*>(p); }
 };
@@ -295,6 +314,7 @@ struct lsg_op {
     DevMem epsInst, rows;        // iterative refinement: per-instance inner tolerances, rows to clear
     DevMem norms2;               // lsg_vector_norms results (2 per instance)
     DevMem ones;                 // D = 1 of the unpreconditioned CG
+    PinnedBuf hostSt, hostMap, hostPart;   // host copies of the CG state, instance lists, residual partials
     DevMem rhsEta, rhsM;         // lsg_rhs_sines parameters (grown on demand)
     bool solverReady = false, diagValid = false;
     hipStream_t stream = nullptr;
@@ -564,6 +584,9 @@ int ensureSolver(lsg_op* op) {
     HIPCHK(op->actMap.alloc((size_t)op->batch * sizeof(int)));
     HIPCHK(op->epsInst.alloc((size_t)op->batch * sizeof(double)));
     HIPCHK(op->rows.alloc((size_t)op->batch * sizeof(int)));
+    HIPCHK(op->hostSt.alloc((size_t)op->batch * sizeof(CGState)));
+    HIPCHK(op->hostMap.alloc((size_t)op->batch * sizeof(int)));
+    HIPCHK(op->hostPart.alloc((size_t)op->batch * 3 * nbk * sizeof(double)));
     op->solverReady = true;
     return LSG_OK;
 }
@@ -650,135 +673,400 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
     return cgRun<V>(op, bf, f, x, x64, eps, rule, maxit, out, xZero, epsInst, open);
 }
 
-// PCG loop (CG::solveHomogen, applications/cg_method.h:759-886) on the buffers bf.
+// PCG (CG::solveHomogen, applications/cg_method.h:759-886) on the buffers bf, in phases
+// that only issue work on the operator's stream (cgStart, cgIssueChunk) or read back
+// what a completed chunk left (cgCollectChunk, after the stream has been synchronised),
+// so that one host thread can advance the solves of several operators at once.
 template <typename V>
-int cgRun(lsg_op* op, const CgBufs& bf, const V* f, void* x, bool x64, double eps, int rule, int maxit,
-          std::vector<CGState>& out, bool xZero, const double* epsInst, const std::vector<int>* open) {
+struct CgPhase {
+    lsg_op* op = nullptr;
+    CgBufs bf{};
+    const V* f = nullptr;
+    void* x = nullptr;
+    bool x64 = false;
+    int maxit = 0;
+    int done = 0;
+    int lastK = 0;
+    bool finished = false;
+    std::vector<CGState> out;
+};
+
+// host-visible copies of the CG state: pinned memory of the operator (asynchronous copies
+// from several streams do not serialise the host), or, without own buffers, out itself
+template <typename V>
+CGState* cgHostState(CgPhase<V>& c) {
+    return c.bf.own ? c.op->hostSt.template as<CGState>() : c.out.data();
+}
+
+template <typename V>
+int cgStart(CgPhase<V>& c, double eps, int rule, bool xZero, const double* epsInst, const std::vector<int>* open) {
+    lsg_op* op = c.op;
     const int64_t n = op->n();
     const int B = op->batch;
     hipStream_t s = op->stream;
-    int rc = LSG_OK;
-    V* r = (V*)bf.r;
-    V* h = (V*)bf.h;
-    V* dv = (V*)bf.dv;
-    V* g = (V*)bf.g;
-    V* D = (V*)bf.D;
     const int nb = cg_partial_blocks(n);
-    double* part = bf.part;
-    CGState* st = bf.st;
+    V* r = (V*)c.bf.r;
+    V* g = (V*)c.bf.g;
     // r = A x (x in V for the matvec); a start vector known to be zero needs no matvec
     if (xZero) {
         launch_zero(r, n * B * (int64_t)sizeof(V), s);
     } else {
         const V* xv;
-        if (x64 && sizeof(V) != 8) {
-            launch_cast_d2f((const double*)x, (float*)g, n * B, s);
+        if (c.x64 && sizeof(V) != 8) {
+            launch_cast_d2f((const double*)c.x, (float*)g, n * B, s);
             xv = g;
-        } else xv = (const V*)x;
-        if ((rc = runMatvec<V>(op, xv, r, nullptr))) return rc;
+        } else xv = (const V*)c.x;
+        if (int rc = runMatvec<V>(op, xv, r, nullptr)) return rc;
     }
-    launch_cg_init<V>(f, r, h, dv, D, n, B, part, nb, s);
-    launch_cg_start(st, part, nb, eps, epsInst, rule, maxit, B, s);
+    launch_cg_init<V>(c.f, r, (V*)c.bf.h, (V*)c.bf.dv, (V*)c.bf.D, n, B, c.bf.part, nb, s);
+    launch_cg_start(c.bf.st, c.bf.part, nb, eps, epsInst, rule, c.maxit, B, s);
     HIPCHK(hipGetLastError());
-    out.resize(B);
-    const int chunk = 8;
+    c.out.resize(B);
+    c.done = 0;
+    c.finished = c.maxit <= 0;
     op->nMap = 0;
-    std::vector<int> act;     // host copy of the compacted list; the stream syncs before each rewrite
-    act.reserve(B);
-    if (bf.own && open && !open->empty() && (int)open->size() < B) {
-        act = *open;
-        HIPCHK(hipMemcpyAsync(op->actMap.p, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, s));
-        op->nMap = (int)act.size();
+    if (c.bf.own && open && !open->empty() && (int)open->size() < B) {
+        int* hm = op->hostMap.as<int>();
+        std::copy(open->begin(), open->end(), hm);
+        HIPCHK(hipMemcpyAsync(op->actMap.p, hm, open->size() * sizeof(int), hipMemcpyHostToDevice, s));
+        op->nMap = (int)open->size();
     }
-    for (int done = 0; done < maxit;) {
-        int k = std::min(chunk, maxit - done);
-        if (bf.own && op->nMap == 0 && useGraphs()) {
-            // the chunk's launches depend only on the operator, x and k: capture once,
-            // then replay (the kernels read the convergence state on the device)
-            if (!op->chunkExec || op->chunkX != x || op->chunkK != k || op->chunkX64 != x64 || op->chunkD != D) {
-                if (op->chunkExec) HIPCHK(hipGraphExecDestroy(op->chunkExec));
-                op->chunkExec = nullptr;
-                hipGraph_t graph = nullptr;
-                HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                op->capturing = true;
-                for (int q = 0; q < k && !rc; ++q) {
-                    rc = runMatvec<V>(op, dv, g, st);
-                    if (!rc) launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s);
-                }
-                op->capturing = false;
-                const hipError_t ec = hipStreamEndCapture(s, &graph);
-                if (rc) {
-                    if (graph) (void)hipGraphDestroy(graph);
-                    return rc;
-                }
-                HIPCHK(ec);
-                const hipError_t ei = hipGraphInstantiate(&op->chunkExec, graph, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(graph);
-                HIPCHK(ei);
-                op->chunkX = x;
-                op->chunkK = k;
-                op->chunkX64 = x64;
-                op->chunkD = D;
-            }
-            HIPCHK(hipGraphLaunch(op->chunkExec, s));
-        } else {
-            for (int q = 0; q < k; ++q) {
-                if ((rc = runMatvec<V>(op, dv, g, st))) return rc;
-                launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s);
-            }
-        }
-        done += k;
-        HIPCHK(hipMemcpyAsync(out.data(), st, sizeof(CGState) * B, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        act.clear();
-        for (int b = 0; b < B; ++b)
-            if (out[b].active) act.push_back(b);
-        if (act.empty()) break;
-        // finished instances drop out of the matvec grids: their workgroups would only
-        // read their flag and exit, and they delay the dispatch of the ones still running
-        if (bf.own && (int)act.size() < B) {
-            HIPCHK(hipMemcpyAsync(op->actMap.p, act.data(), act.size() * sizeof(int), hipMemcpyHostToDevice, s));
-            op->nMap = (int)act.size();
-        }
-    }
-    op->nMap = 0;
-    HIPCHK(hipMemcpyAsync(out.data(), st, sizeof(CGState) * B, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
     return LSG_OK;
 }
 
-// true relative residual ||f - A x|| / ||f|| in FP64 (x, f double, device).  With a
+// the next chunk of up to 8 iterations, then the state copied towards the host
+template <typename V>
+int cgIssueChunk(CgPhase<V>& c) {
+    lsg_op* op = c.op;
+    const int64_t n = op->n();
+    const int B = op->batch;
+    hipStream_t s = op->stream;
+    const int nb = cg_partial_blocks(n);
+    V* r = (V*)c.bf.r;
+    V* h = (V*)c.bf.h;
+    V* dv = (V*)c.bf.dv;
+    V* g = (V*)c.bf.g;
+    V* D = (V*)c.bf.D;
+    double* part = c.bf.part;
+    CGState* st = c.bf.st;
+    void* x = c.x;
+    const bool x64 = c.x64;
+    int rc = LSG_OK;
+    const int k = std::min(8, c.maxit - c.done);
+    if (c.bf.own && op->nMap == 0 && useGraphs()) {
+        // the chunk's launches depend only on the operator, x and k: capture once, then
+        // replay (the kernels read the convergence state on the device)
+        if (!op->chunkExec || op->chunkX != x || op->chunkK != k || op->chunkX64 != x64 || op->chunkD != D) {
+            if (op->chunkExec) HIPCHK(hipGraphExecDestroy(op->chunkExec));
+            op->chunkExec = nullptr;
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            op->capturing = true;
+            for (int q = 0; q < k && !rc; ++q) {
+                rc = runMatvec<V>(op, dv, g, st);
+                if (!rc) launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s);
+            }
+            op->capturing = false;
+            const hipError_t ec = hipStreamEndCapture(s, &graph);
+            if (rc) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return rc;
+            }
+            HIPCHK(ec);
+            const hipError_t ei = hipGraphInstantiate(&op->chunkExec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            HIPCHK(ei);
+            op->chunkX = x;
+            op->chunkK = k;
+            op->chunkX64 = x64;
+            op->chunkD = D;
+        }
+        HIPCHK(hipGraphLaunch(op->chunkExec, s));
+    } else {
+        for (int q = 0; q < k; ++q) {
+            if ((rc = runMatvec<V>(op, dv, g, st))) return rc;
+            launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s);
+        }
+    }
+    c.lastK = k;
+    HIPCHK(hipMemcpyAsync(cgHostState(c), st, sizeof(CGState) * B, hipMemcpyDeviceToHost, s));
+    return LSG_OK;
+}
+
+// after the stream is synchronised: the state of the chunk just run; finished
+// instances drop out of the matvec grids (their workgroups would only read their flag
+// and exit, and they delay the dispatch of the ones still running)
+template <typename V>
+int cgCollectChunk(CgPhase<V>& c) {
+    lsg_op* op = c.op;
+    const int B = op->batch;
+    const CGState* hs = cgHostState(c);
+    if (hs != c.out.data()) std::copy(hs, hs + B, c.out.begin());
+    c.done += c.lastK;
+    int nact = 0;
+    int* hm = c.bf.own ? op->hostMap.as<int>() : nullptr;
+    for (int b = 0; b < B; ++b)
+        if (c.out[b].active) {
+            if (hm) hm[nact] = b;
+            ++nact;
+        }
+    if (nact == 0 || c.done >= c.maxit) {
+        c.finished = true;
+        op->nMap = 0;
+        return LSG_OK;
+    }
+    if (c.bf.own && nact < B) {
+        HIPCHK(hipMemcpyAsync(op->actMap.p, hm, nact * sizeof(int), hipMemcpyHostToDevice, op->stream));
+        op->nMap = nact;
+    }
+    return LSG_OK;
+}
+
+template <typename V>
+int cgRun(lsg_op* op, const CgBufs& bf, const V* f, void* x, bool x64, double eps, int rule, int maxit,
+          std::vector<CGState>& out, bool xZero, const double* epsInst, const std::vector<int>* open) {
+    CgPhase<V> c;
+    c.op = op;
+    c.bf = bf;
+    c.f = f;
+    c.x = x;
+    c.x64 = x64;
+    c.maxit = maxit;
+    int rc = cgStart<V>(c, eps, rule, xZero, epsInst, open);
+    if (rc) return rc;
+    if (c.finished) {      // no iteration allowed: the start state only
+        HIPCHK(hipMemcpyAsync(c.out.data(), bf.st, sizeof(CGState) * op->batch, hipMemcpyDeviceToHost, op->stream));
+    }
+    while (!c.finished) {
+        if ((rc = cgIssueChunk<V>(c))) return rc;
+        HIPCHK(hipStreamSynchronize(op->stream));
+        if ((rc = cgCollectChunk<V>(c))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(op->stream));
+    out = c.out;
+    return LSG_OK;
+}
+
+// True relative residual ||f - A x|| / ||f|| in FP64 (x, f double, device), issued on
+// op64's stream; residualCollect reads it after the stream is synchronised.  With a
 // non-null `sub` (a strict subset of the instances), only those instances' r and rel
 // are recomputed; the others keep theirs.
-int trueResidual(lsg_op* op64, const double* f, const double* x, double* tmpAx, double* r, std::vector<double>& rel,
-                 const std::vector<int>* sub = nullptr) {
+int residualIssue(lsg_op* op64, const double* f, const double* x, double* tmpAx, double* r,
+                  const std::vector<int>* sub) {
     const int64_t n = op64->n();
     const int B = op64->batch;
     hipStream_t s = op64->stream;
     const int nSub = (sub && (int)sub->size() < B) ? (int)sub->size() : 0;
-    if (nSub > 0) HIPCHK(hipMemcpyAsync(op64->actMap.p, sub->data(), nSub * sizeof(int), hipMemcpyHostToDevice, s));
+    if (nSub > 0) {
+        int* hm = op64->hostMap.as<int>();
+        std::copy(sub->begin(), sub->end(), hm);
+        HIPCHK(hipMemcpyAsync(op64->actMap.p, hm, nSub * sizeof(int), hipMemcpyHostToDevice, s));
+    }
     int rc = runMatvec<double>(op64, x, tmpAx, nullptr, nSub);
     if (rc) return rc;
     const int nb = cg_partial_blocks(n);
     launch_residual(f, tmpAx, r, n, nSub > 0 ? nSub : B, op64->partials.as<double>(), nb, s,
                     nSub > 0 ? op64->actMap.as<int>() : nullptr);
-    std::vector<double> hp((size_t)B * 2 * nb);
-    HIPCHK(hipMemcpyAsync(hp.data(), op64->partials.p, hp.size() * 8, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
+    HIPCHK(hipMemcpyAsync(op64->hostPart.p, op64->partials.p, (size_t)B * 2 * nb * 8, hipMemcpyDeviceToHost, s));
+    return LSG_OK;
+}
+
+void residualCollect(lsg_op* op64, std::vector<double>& rel, const std::vector<int>* sub) {
+    const int64_t n = op64->n();
+    const int B = op64->batch;
+    const int nb = cg_partial_blocks(n);
+    const double* hp = op64->hostPart.as<double>();
+    const int nSub = (sub && (int)sub->size() < B) ? (int)sub->size() : 0;
     rel.resize(B);
-    std::vector<int> all;
-    if (nSub == 0) {
-        all.resize(B);
-        for (int b = 0; b < B; ++b) all[b] = b;
-        sub = &all;
-    }
-    for (int b : *sub) {
+    auto one = [&](int b) {
         double rr = 0, ff = 0;
         for (int k = 0; k < nb; ++k) rr += hp[((size_t)b * 2 + 0) * nb + k];
         for (int k = 0; k < nb; ++k) ff += hp[((size_t)b * 2 + 1) * nb + k];
         rel[b] = ff > 0 ? std::sqrt(rr / ff) : std::sqrt(rr);
-    }
+    };
+    if (nSub > 0)
+        for (int b : *sub) one(b);
+    else
+        for (int b = 0; b < B; ++b) one(b);
+}
+
+int trueResidual(lsg_op* op64, const double* f, const double* x, double* tmpAx, double* r, std::vector<double>& rel,
+                 const std::vector<int>* sub = nullptr) {
+    if (int rc = residualIssue(op64, f, x, tmpAx, r, sub)) return rc;
+    HIPCHK(hipStreamSynchronize(op64->stream));
+    residualCollect(op64, rel, sub);
     return LSG_OK;
+}
+
+// ---- mixed-precision solves of several operator pairs at once (lsg_ir_solve_many)
+// One problem group: an FP64 / FP32 operator pair on one grid with its batch.  Both
+// operators of a group issue on one stream (the FP32 operator's) for the duration of the
+// solve, so a group's phases need no events between them.
+struct IrGroup {
+    lsg_op* op64 = nullptr;
+    lsg_op* op32 = nullptr;
+    double* f = nullptr;
+    double* x = nullptr;
+    int B = 0;
+    int64_t n = 0;
+    int sweep = 0;
+    enum Phase { RESIDUAL, RESTART, INNER, DONE } phase = RESIDUAL;
+    std::vector<int> iters, outer, conv, newly, open;
+    std::vector<double> rel, epsH;
+    CgPhase<float> cg;
+    hipStream_t saved64 = nullptr;
+};
+
+namespace {
+// the FP64 residual of the group's open instances (issued on the group's stream)
+int irIssueResidual(IrGroup& g) {
+    g.open.clear();
+    for (int b = 0; b < g.B; ++b)
+        if (!g.conv[b]) g.open.push_back(b);
+    return residualIssue(g.op64, g.f, g.x, g.op64->tmp.as<double>(), g.op64->r.as<double>(), &g.open);
+}
+
+// after the residual: converged instances, inner tolerances; then either the end of the
+// group's solve or its next inner FP32 PCG (right-hand side, start, first chunk)
+int irAfterResidual(IrGroup& g, double rel_tol, double inner_rel_tol, int maxit) {
+    residualCollect(g.op64, g.rel, g.phase == IrGroup::RESTART ? nullptr : &g.open);
+    if (g.sweep == 0 && g.phase == IrGroup::RESIDUAL) {
+        // a start vector worse than zero (||f - A x0|| > ||f||) is replaced by zero
+        bool redo = false;
+        for (int b = 0; b < g.B; ++b)
+            if (g.rel[b] > 1.0) {
+                HIPCHK(hipMemsetAsync(g.x + (size_t)b * g.n, 0, g.n * 8, g.op64->stream));
+                redo = true;
+            }
+        if (redo) {
+            g.phase = IrGroup::RESTART;
+            return residualIssue(g.op64, g.f, g.x, g.op64->tmp.as<double>(), g.op64->r.as<double>(), nullptr);
+        }
+    }
+    bool all = true;
+    g.newly.clear();
+    for (int b = 0; b < g.B; ++b) {
+        if (g.rel[b] != g.rel[b]) return fail(LSG_ERR_NAN, "NaN residual");
+        if (g.rel[b] <= rel_tol) {
+            if (!g.conv[b]) g.newly.push_back(b);
+            g.conv[b] = 1;
+        } else all = false;
+        // inner tolerance: inner_rel_tol, or less when the remaining reduction is smaller.
+        // The first sweep aims at inner_rel_tol * ||f|| rather than at inner_rel_tol times
+        // the start residual, so that a good start (rel < 1) shortens it; from a zero
+        // start (rel = 1) both are the same.
+        const double rr = std::max(g.rel[b], 1e-300);
+        const double target = g.sweep == 0 ? inner_rel_tol / std::min(rr, 1.0) : inner_rel_tol;
+        g.epsH[b] = std::min(0.5, std::max(target, 0.3 * rel_tol / rr));
+    }
+    if (all || g.sweep == maxit) {
+        g.phase = IrGroup::DONE;
+        return LSG_OK;
+    }
+    // inner right-hand side r32 = f - A x in FP32, rows of converged instances zero
+    hipStream_t s = g.op32->stream;
+    const int64_t tot = g.n * g.B;
+    double* r64 = g.op64->r.as<double>();
+    if (!g.newly.empty()) {
+        int* hm = g.op64->hostMap.as<int>();
+        std::copy(g.newly.begin(), g.newly.end(), hm);
+        HIPCHK(hipMemcpyAsync(g.op64->rows.p, hm, g.newly.size() * sizeof(int), hipMemcpyHostToDevice, s));
+        launch_zero_rows(r64, g.n, g.op64->rows.as<int>(), (int)g.newly.size(), s);
+    }
+    double* he = g.op64->hostPart.as<double>();      // (the residual partials are read already)
+    std::copy(g.epsH.begin(), g.epsH.end(), he);
+    HIPCHK(hipMemcpyAsync(g.op64->epsInst.p, he, (size_t)g.B * 8, hipMemcpyHostToDevice, s));
+    launch_cast_d2f(r64, g.op32->f.as<float>(), tot, s);
+    g.open.clear();
+    for (int b = 0; b < g.B; ++b)
+        if (!g.conv[b]) g.open.push_back(b);
+    // inner PCG from e = 0
+    CgPhase<float>& c = g.cg;
+    c = CgPhase<float>();
+    c.op = g.op32;
+    int rc = computeDiag(g.op32);
+    if (rc) return rc;
+    c.bf = CgBufs{g.op32->r.p, g.op32->h.p, g.op32->dv.p, g.op32->g.p, g.op32->D.p, g.op32->partials.as<double>(),
+                  g.op32->st.as<CGState>(), true};
+    c.f = g.op32->f.as<float>();
+    c.x = g.op32->x.p;
+    c.x64 = false;
+    c.maxit = 1000;
+    launch_zero(c.x, tot * 4, s);
+    if ((rc = cgStart<float>(c, inner_rel_tol, LSG_STOP_REL, true, g.op64->epsInst.as<double>(), &g.open))) return rc;
+    g.phase = IrGroup::INNER;
+    return cgIssueChunk<float>(c);
+}
+
+// after an inner chunk: the next chunk, or the correction x += e and the next residual
+int irAfterChunk(IrGroup& g) {
+    int rc = cgCollectChunk<float>(g.cg);
+    if (rc) return rc;
+    if (!g.cg.finished) return cgIssueChunk<float>(g.cg);
+    launch_axpy_correction(g.x, g.op32->x.as<float>(), g.n * g.B, g.op32->stream);
+    for (int b = 0; b < g.B; ++b)
+        if (!g.conv[b]) {
+            g.iters[b] += g.cg.out[b].it;
+            g.outer[b] += 1;
+        }
+    ++g.sweep;
+    g.phase = IrGroup::RESIDUAL;
+    return irIssueResidual(g);
+}
+}  // namespace
+
+// FP32 inner PCG under FP64 iterative refinement for every group.  One host thread drives
+// a state machine per group (residual -> inner PCG chunks -> correction -> residual ...):
+// it polls the groups' streams and advances each group as soon as its last phase has
+// completed, so small grids finish their sweeps while a large one is still iterating, and
+// the GPU runs the groups side by side.
+int irSolveGroups(std::vector<IrGroup>& G, double rel_tol, double inner_rel_tol, int maxit) {
+    int rc = LSG_OK;
+    for (IrGroup& g : G) {
+        if ((rc = ensureSolver(g.op64)) || (rc = ensureSolver(g.op32)) || (rc = ensureThreads(g.op64)) ||
+            (rc = ensureThreads(g.op32)) || (rc = ensureCellCoef(g.op32)) || (rc = ensureCellCoef(g.op64)))
+            return rc;
+        HIPCHK(hipStreamSynchronize(g.op64->stream));
+        g.B = g.op64->batch;
+        g.n = g.op64->n();
+        g.iters.assign(g.B, 0);
+        g.outer.assign(g.B, 0);
+        g.conv.assign(g.B, 0);
+        g.epsH.assign(g.B, 0.0);
+        g.sweep = 0;
+        g.phase = IrGroup::RESIDUAL;
+    }
+    for (IrGroup& g : G) {          // the group's FP64 work issues on its FP32 operator's stream
+        g.saved64 = g.op64->stream;
+        g.op64->stream = g.op32->stream;
+    }
+    for (IrGroup& g : G)
+        if ((rc = irIssueResidual(g))) break;
+    while (rc == LSG_OK) {
+        bool busy = false, advanced = false;
+        for (IrGroup& g : G) {
+            if (g.phase == IrGroup::DONE) continue;
+            busy = true;
+            const hipError_t q = hipStreamQuery(g.op32->stream);
+            if (q == hipErrorNotReady) continue;
+            if (q != hipSuccess) {
+                rc = fail(LSG_ERR_DEVICE, std::string("hipStreamQuery: ") + hipGetErrorString(q));
+                break;
+            }
+            advanced = true;
+            rc = g.phase == IrGroup::INNER ? irAfterChunk(g) : irAfterResidual(g, rel_tol, inner_rel_tol, maxit);
+            if (rc) break;
+        }
+        if (!busy || rc) break;
+        if (!advanced) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+    for (IrGroup& g : G) {
+        (void)hipStreamSynchronize(g.op32->stream);
+        g.op64->stream = g.saved64;
+        g.op32->nMap = 0;
+        g.op64->nMap = 0;
+    }
+    return rc;
 }
 
 // Tables and work vectors of the basis transformations (FP64 operators on the dense path).
@@ -1433,101 +1721,71 @@ int lsg_ir_solve(lsg_op* op64, lsg_op* op32, const void* f, void* x, double rel_
     if (op64->prec != 64 || op32->prec != 32 || op64->G != op32->G || op64->batch != op32->batch)
         return fail(LSG_ERR_ARG, "need an FP64 and an FP32 operator on the same grid and batch");
     int rc;
-    if ((rc = ensureSolver(op64)) || (rc = ensureSolver(op32)) || (rc = ensureThreads(op64)) ||
-        (rc = ensureThreads(op32)))
-        return rc;
-    const int64_t n = op64->n();
-    const int B = op64->batch;
-    const int64_t tot = n * B;
-    double* fD;
-    double* xD;
+    if ((rc = ensureSolver(op64))) return rc;
+    std::vector<IrGroup> G(1);
+    G[0].op64 = op64;
+    G[0].op32 = op32;
     if (layout == LSG_HOST_SLOT) {
         if ((rc = upload(op64, (const double*)f, op64->f.p)) || (rc = upload(op64, (const double*)x, op64->x.p)))
             return rc;
-        fD = op64->f.as<double>();
-        xD = op64->x.as<double>();
+        G[0].f = op64->f.as<double>();
+        G[0].x = op64->x.as<double>();
     } else {
-        fD = (double*)f;
-        xD = (double*)x;
+        G[0].f = (double*)f;
+        G[0].x = (double*)x;
     }
-    hipStream_t s = op64->stream;
-    double* Ax = op64->tmp.as<double>();
-    double* r64 = op64->r.as<double>();
-    float* r32 = op32->f.as<float>();       // inner right-hand side
-    float* e32 = op32->x.as<float>();       // inner solution
-    std::vector<int> iters(B, 0), outer(B, 0), conv(B, 0);
-    std::vector<double> rel, epsH(B);
-    std::vector<CGState> st;
-    DevMem& epsD = op64->epsInst;     // (allocated with the solver buffers, not per call)
-    DevMem& rowsD = op64->rows;
-    std::vector<int> newly;   // instances converged in this sweep: their r64 rows are zeroed
-    std::vector<int> open;    // instances not yet converged
-    for (int sweep = 0; sweep <= maxit; ++sweep) {
-        // a converged instance's correction is zero from then on (its inner right-hand
-        // side is zero), so its x, residual and rel stay as they are: recompute the
-        // others only
-        open.clear();
-        for (int b = 0; b < B; ++b)
-            if (!conv[b]) open.push_back(b);
-        if ((rc = trueResidual(op64, fD, xD, Ax, r64, rel, &open))) return rc;
-        if (sweep == 0) {
-            // a start vector worse than zero (||f - A x0|| > ||f||) is replaced by zero
-            bool any = false;
-            for (int b = 0; b < B; ++b)
-                if (rel[b] > 1.0) {
-                    HIPCHK(hipMemsetAsync(xD + (size_t)b * n, 0, n * 8, s));
-                    any = true;
-                }
-            if (any && (rc = trueResidual(op64, fD, xD, Ax, r64, rel))) return rc;
-        }
-        bool all = true;
-        newly.clear();
-        for (int b = 0; b < B; ++b) {
-            if (rel[b] != rel[b]) return fail(LSG_ERR_NAN, "NaN residual");
-            if (rel[b] <= rel_tol) {
-                if (!conv[b]) newly.push_back(b);
-                conv[b] = 1;
-            } else all = false;
-            // inner tolerance: inner_rel_tol, or less when the remaining reduction is smaller.
-            // The first sweep aims at inner_rel_tol * ||f|| rather than at inner_rel_tol times
-            // the start residual, so that a good start (rel < 1) shortens it; from a zero
-            // start (rel = 1) both are the same.
-            const double rr = std::max(rel[b], 1e-300);
-            const double target = sweep == 0 ? inner_rel_tol / std::min(rr, 1.0) : inner_rel_tol;
-            epsH[b] = std::min(0.5, std::max(target, 0.3 * rel_tol / rr));
-        }
-        if (all || sweep == maxit) break;
-        if (!newly.empty()) {
-            HIPCHK(hipMemcpyAsync(rowsD.p, newly.data(), newly.size() * sizeof(int), hipMemcpyHostToDevice, s));
-            launch_zero_rows(r64, n, rowsD.as<int>(), (int)newly.size(), s);
-        }
-        HIPCHK(hipMemcpyAsync(epsD.p, epsH.data(), (size_t)B * 8, hipMemcpyHostToDevice, s));
-        launch_cast_d2f(r64, r32, tot, s);
-        HIPCHK(hipMemsetAsync(e32, 0, tot * 4, s));
-        HIPCHK(hipStreamSynchronize(s));
-        open.clear();
-        for (int b = 0; b < B; ++b)
-            if (!conv[b]) open.push_back(b);
-        if ((rc = cgDevice<float>(op32, r32, e32, false, inner_rel_tol, LSG_STOP_REL, 1000, st, true,
-                                  epsD.as<double>(), &open)))
-            return rc;
-        launch_axpy_correction(xD, e32, tot, op32->stream);
-        HIPCHK(hipStreamSynchronize(op32->stream));
-        for (int b = 0; b < B; ++b)
-            if (!conv[b]) { iters[b] += st[b].it; outer[b] += 1; }
-    }
+    rc = irSolveGroups(G, rel_tol, inner_rel_tol, maxit);
+    if (rc) return rc;
     if (layout == LSG_HOST_SLOT && (rc = download(op64, op64->x.p, (double*)x))) return rc;
     bool noconv = false;
-    for (int b = 0; b < B; ++b) {
+    for (int b = 0; b < op64->batch; ++b) {
         if (info) {
-            info[b].iterations = iters[b];
-            info[b].outer = outer[b];
+            info[b].iterations = G[0].iters[b];
+            info[b].outer = G[0].outer[b];
             info[b].final_delta = 0.0;
-            info[b].rel_residual = rel[b];
-            info[b].converged = conv[b];
+            info[b].rel_residual = G[0].rel[b];
+            info[b].converged = G[0].conv[b];
         }
-        noconv |= !conv[b];
+        noconv |= !G[0].conv[b];
     }
+    if (noconv) return fail(LSG_ERR_NOCONV, "iterative refinement stopped at maxit sweeps before rel_tol was met");
+    return LSG_OK;
+}
+
+int lsg_ir_solve_many(int ngroups, lsg_op* const* op64, lsg_op* const* op32, const void* const* f, void* const* x,
+                      double rel_tol, double inner_rel_tol, int maxit, lsg_solve_info* const* info) {
+    if (ngroups < 1 || !op64 || !op32 || !f || !x) return fail(LSG_ERR_ARG, "bad arguments");
+    std::vector<IrGroup> G(ngroups);
+    for (int q = 0; q < ngroups; ++q) {
+        lsg_op* a = op64[q];
+        lsg_op* b = op32[q];
+        if (!a || !b || !f[q] || !x[q]) return fail(LSG_ERR_ARG, "null operator or vector in group " + std::to_string(q));
+        if (a->prec != 64 || b->prec != 32 || a->G != b->G || a->batch != b->batch)
+            return fail(LSG_ERR_ARG, "group " + std::to_string(q) +
+                                         ": need an FP64 and an FP32 operator on the same grid and batch");
+        for (int p = 0; p < q; ++p)
+            if (op64[p] == a || op32[p] == b || op64[p] == b || op32[p] == a)
+                return fail(LSG_ERR_ARG, "an operator appears in two groups");
+        G[q].op64 = a;
+        G[q].op32 = b;
+        G[q].f = (double*)f[q];
+        G[q].x = (double*)x[q];
+    }
+    int rc = irSolveGroups(G, rel_tol, inner_rel_tol, maxit);
+    if (rc) return rc;
+    bool noconv = false;
+    for (int q = 0; q < ngroups; ++q)
+        for (int b = 0; b < G[q].B; ++b) {
+            if (info && info[q]) {
+                lsg_solve_info& o = info[q][b];
+                o.iterations = G[q].iters[b];
+                o.outer = G[q].outer[b];
+                o.final_delta = 0.0;
+                o.rel_residual = G[q].rel[b];
+                o.converged = G[q].conv[b];
+            }
+            noconv |= !G[q].conv[b];
+        }
     if (noconv) return fail(LSG_ERR_NOCONV, "iterative refinement stopped at maxit sweeps before rel_tol was met");
     return LSG_OK;
 }
