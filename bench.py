@@ -500,7 +500,7 @@ def main():
     ws = None
     if args.warm_xi > 0:
         from loadsg_amd.warmstart import WarmStart
-        ws = WarmStart(grid, m, n_xi=NXI)
+        ws = WarmStart(grid, m, n_xi=NXI, progress=True)
         xi_train = np.random.default_rng(SEED + 1).normal(0.0, 0.5, (args.warm_xi, NXI))
         U = ws.basis_solutions(xi_train, batch=512)
         ws.fit(xi_train, U, steps=args.warm_steps)

@@ -289,9 +289,17 @@ struct lsg_op {
         int G;              // batch instances per workgroup
         std::unique_ptr<DevMem> fam;
     };
-    std::vector<ApplyLaunch> launches;
-    int threadsOf[16] = {};   // per size class; 0 = autotune on first use
-    int nClasses = 0;
+    // three bucketings of the apply families by box size (12 fine classes, 5 coarse, one):
+    // the autotuner times each with its best workgroup sizes and keeps the fastest
+    struct ClassSet {
+        std::vector<ApplyLaunch> launches;
+        int threadsOf[16] = {};   // per size class; 0 = autotune on first use
+        int nClasses = 0;
+    };
+    ClassSet sets[3];
+    int cs = 0;                 // active set
+    ClassSet& act() { return sets[cs]; }
+    const ClassSet& act() const { return sets[cs]; }
     int64_t maxLds = 0;
     DevMem dNodalItems, dSumItems, dSumList, dApplyItems, dShapes;
     std::vector<std::unique_ptr<DevMem>> dProl;   // per sweep dim
@@ -446,12 +454,12 @@ int runMatvec(lsg_op* op, const V* x, V* y, const CGState* st, int nSub = 0) {
     a.shapes = op->dShapes.as<BoxShape>();
     a.batch = B;
     a.map = map;
-    for (auto& L : op->launches) {
+    for (auto& L : op->act().launches) {
         a.fam = L.fam->as<FamilyItem>();
         a.maxBox = L.maxBox;
         a.maxW = L.maxW;
         a.G = L.G;
-        const int thr = op->threadsOf[L.cls] ? op->threadsOf[L.cls] : 256;
+        const int thr = op->act().threadsOf[L.cls] ? op->act().threadsOf[L.cls] : 256;
         const size_t lds = ((op->general ? 4 : 2) * L.maxBox + L.maxW) * sizeof(V);
         if (launch_apply<V>(a, L.count, B, thr, lds, s) != 0)
             return fail(LSG_ERR_UNSUPPORTED, "depth box too large for the chosen block size");
@@ -472,7 +480,7 @@ int matvecDev(lsg_op* op, const void* x, void* y, const CGState* st) {
 
 int64_t classMax(const lsg_op* op, int cls) {
     int64_t m = 0;
-    for (auto& L : op->launches)
+    for (auto& L : op->act().launches)
         if (L.cls == cls) m = std::max(m, L.maxBox);
     return m;
 }
@@ -485,11 +493,26 @@ int64_t applyLdsBytes(const lsg_op* op, int64_t maxBox, int64_t maxW, int thread
 }
 
 bool threadsFit(const lsg_op* op, int cls, int threads) {
-    for (auto& L : op->launches)
+    for (auto& L : op->act().launches)
         if (L.cls == cls && (applyLdsBytes(op, L.maxBox, L.maxW, threads) > op->maxLds ||
                              (L.maxBox + threads - 1) / threads > 16))
             return false;
     return true;
+}
+
+// apply-stage time (ms) of the active class set with its current workgroup sizes
+int timeApply(lsg_op* op, float& ms) {
+    float tot = 0.f;
+    for (int rep = 0; rep < 3; ++rep) {
+        int rc = matvecDev(op, op->xs.p, op->ys.p, nullptr);
+        if (rc) return rc;
+        HIPCHK(hipEventSynchronize(op->ev[3]));
+        float t = 0.f;
+        HIPCHK(hipEventElapsedTime(&t, op->ev[1], op->ev[2]));
+        if (rep > 0) tot += t;
+    }
+    ms = tot;
+    return LSG_OK;
 }
 
 int chooseThreads(lsg_op* op, int threads) {
@@ -499,43 +522,50 @@ int chooseThreads(lsg_op* op, int threads) {
         op->chunkExec = nullptr;
     }
     if (threads) {
-        for (int c = 0; c < op->nClasses; ++c) {
+        for (int c = 0; c < op->act().nClasses; ++c) {
             if (!threadsFit(op, c, threads)) return fail(LSG_ERR_UNSUPPORTED, "apply workgroup exceeds LDS");
-            op->threadsOf[c] = threads;
+            op->act().threadsOf[c] = threads;
         }
         return LSG_OK;
     }
-    // autotune each size class: time the apply stage for every admissible number of
-    // waves per workgroup (one wave per batch instance; the other classes keep their
-    // current choice).  Results do not depend on this choice.
-    for (int c = 0; c < op->nClasses; ++c) op->threadsOf[c] = 512;
-    for (int c = 0; c < op->nClasses; ++c) {
-        float best = 1e30f;
-        int bestT = 0;
-        for (int t : cands) {
-            if (!threadsFit(op, c, t)) continue;
-            if (t > 64 && classMax(op, c) <= t / 2) continue;   // half the block covers the largest box
-            op->threadsOf[c] = t;
-            float tot = 0.f;
-            for (int rep = 0; rep < 3; ++rep) {
-                int rc = matvecDev(op, op->xs.p, op->ys.p, nullptr);
-                if (rc) return rc;
-                HIPCHK(hipEventSynchronize(op->ev[3]));
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, op->ev[1], op->ev[2]));
-                if (rep > 0) tot += ms;
+    // autotune: for each class set, each size class's workgroup size by timing the apply
+    // stage over the admissible multiples of 64 threads (points per thread follow from
+    // it); then the set whose tuned apply stage is fastest.  Results do not depend on
+    // these choices.
+    const int forced = envInt("LSG_CLASS_SET", -1);
+    float bestSet = 1e30f;
+    int bestCs = op->cs;
+    for (int set = 0; set < 3; ++set) {
+        if (op->sets[set].nClasses == 0) continue;
+        if (forced >= 0 && forced != set) continue;
+        op->cs = set;
+        auto& A = op->act();
+        for (int c = 0; c < A.nClasses; ++c) A.threadsOf[c] = 512;
+        for (int c = 0; c < A.nClasses; ++c) {
+            float best = 1e30f;
+            int bestT = 0;
+            for (int t : cands) {
+                if (!threadsFit(op, c, t)) continue;
+                if (t > 64 && classMax(op, c) <= t / 2) continue;   // half the block covers the largest box
+                A.threadsOf[c] = t;
+                float tot = 0.f;
+                if (int rc = timeApply(op, tot)) return rc;
+                if (tot < best) { best = tot; bestT = t; }
             }
-            if (tot < best) { best = tot; bestT = t; }
+            if (!bestT) return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS");
+            A.threadsOf[c] = bestT;
         }
-        if (!bestT) return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS");
-        op->threadsOf[c] = bestT;
+        float tot = 0.f;
+        if (int rc = timeApply(op, tot)) return rc;
+        if (tot < bestSet) { bestSet = tot; bestCs = set; }
     }
+    op->cs = bestCs;
     return LSG_OK;
 }
 
 int ensureThreads(lsg_op* op) {
-    for (int c = 0; c < op->nClasses; ++c)
-        if (!op->threadsOf[c]) return chooseThreads(op, 0);
+    for (int c = 0; c < op->act().nClasses; ++c)
+        if (!op->act().threadsOf[c]) return chooseThreads(op, 0);
     return LSG_OK;
 }
 
@@ -1388,11 +1418,14 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     const int64_t fine[12] = {128, 192, 256, 384, 512, 768, 1024, 1536, 2048, 4096, 8192, 65536};
     const int64_t coarse[5] = {128, 512, 2048, 8192, 65536};
     const int64_t one[1] = {65536};
-    const bool fineCls = batch >= 16;
-    const bool oneCls = batch <= oneClassMaxBatch();
-    const int64_t* bounds = fineCls ? fine : (oneCls ? one : coarse);
-    const int ncls = fineCls ? 12 : (oneCls ? 1 : 5);
+    // before autotuning: fine classes for batches of 16 or more, one class up to
+    // oneClassMaxBatch(), else the coarse ones
+    op->cs = batch >= 16 ? 0 : (batch <= oneClassMaxBatch() ? 2 : 1);
+    for (int set = 0; set < 3; ++set)
     for (size_t L = 0; !op->sparse && L < P.apply.size(); ++L) {
+        const int64_t* bounds = set == 0 ? fine : (set == 1 ? coarse : one);
+        const int ncls = set == 0 ? 12 : (set == 1 ? 5 : 1);
+        auto& S = op->sets[set];
         std::vector<std::vector<FamilyItem>> bucket(ncls);
         std::vector<int64_t> bmax(ncls, 0), wmax(ncls, 0);
         for (const FamilyItem& f : P.apply[L]) {
@@ -1449,9 +1482,9 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
                 la.G = (int)std::max<int64_t>(G, 1);
             }
             la.fam.reset(new DevMem());
-            op->launches.push_back(std::move(la));
-            op->nClasses = std::max(op->nClasses, c + 1);
-            auto& dm = *op->launches.back().fam;
+            S.launches.push_back(std::move(la));
+            S.nClasses = std::max(S.nClasses, c + 1);
+            auto& dm = *S.launches.back().fam;
             HIPCHK(dm.alloc(bucket[c].size() * sizeof(FamilyItem)));
             HIPCHK(hipMemcpy(dm.p, bucket[c].data(), bucket[c].size() * sizeof(FamilyItem), hipMemcpyHostToDevice));
         }
@@ -1462,12 +1495,29 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     int maxLds = 0;
     HIPCHK(hipDeviceGetAttribute(&maxLds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     op->maxLds = maxLds;
-    for (auto& L : op->launches) {
-        const int64_t lds = applyLdsBytes(op.get(), L.maxBox, L.maxW, 512);
-        if (lds > maxLds)
-            return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS (" + std::to_string(lds) + " > " +
-                                                 std::to_string(maxLds) + " bytes)");
+    // a class set with a launch beyond the LDS of a workgroup is dropped (the one-class set
+    // combines the largest box and the largest weight table of a level)
+    int64_t ldsNeed = 0;
+    bool anySet = false;
+    for (auto& S : op->sets) {
+        bool fits = true;
+        for (auto& L : S.launches) {
+            const int64_t lds = applyLdsBytes(op.get(), L.maxBox, L.maxW, 512);
+            ldsNeed = std::max(ldsNeed, lds);
+            if (lds > maxLds) fits = false;
+        }
+        if (!fits) {
+            S.launches.clear();
+            S.nClasses = 0;
+        }
+        anySet |= S.nClasses > 0;
     }
+    if (!op->sparse && !anySet)
+        return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS (" + std::to_string(ldsNeed) + " > " +
+                                             std::to_string(maxLds) + " bytes)");
+    if (!op->sparse && op->act().nClasses == 0)
+        for (int set = 0; set < 3; ++set)
+            if (op->sets[set].nClasses > 0) { op->cs = set; break; }
     if (!op->sparse && (int64_t)(2 * P.maxBox * op->vs) > maxLds)
         return fail(LSG_ERR_UNSUPPORTED, "largest depth box does not fit in LDS");
     HIPCHK(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
@@ -1921,7 +1971,7 @@ int lsg_last_timing(const lsg_op* op, double* out, int nout) {
     float tot = 0.f, ap = 0.f;
     HIPCHK(hipEventElapsedTime(&tot, op->ev[0], op->ev[3]));
     HIPCHK(hipEventElapsedTime(&ap, op->ev[1], op->ev[2]));
-    double v[3] = {tot, ap, (double)op->launches.size()};
+    double v[3] = {tot, ap, (double)op->act().launches.size()};
     for (int i = 0; i < nout && i < 3; ++i) out[i] = v[i];
     return LSG_OK;
 }

@@ -20,6 +20,7 @@ The prediction is used as the start vector of CG.solveMixed, which replaces any 
 worse than zero (||f - A x0|| > ||f||) by zero, so a poor prediction costs at most one
 FP64 residual evaluation.
 """
+import sys
 import time
 
 import numpy as np
@@ -28,7 +29,8 @@ from . import (COEF_EXPSIN, CG, MLP, MatrixVectorHomogen, Poisson, VariableCoeff
 
 
 class WarmStart:
-    def __init__(self, grid, modes, n_xi=16, xi_scale=0.5, width=512, hidden=3, seed=0):
+    def __init__(self, grid, modes, n_xi=16, xi_scale=0.5, width=512, hidden=3, seed=0, progress=False):
+        self.progress = progress      # a line on stderr about every 20 s of data generation or training
         self.grid = grid
         self.modes = np.asarray(modes, dtype=np.int32)
         self.n_xi = n_xi
@@ -72,9 +74,16 @@ class WarmStart:
             CG.solveMixed(rel_tol, x, f, op64, op32, inner_rel_tol=inner_rel_tol)
             U[s0:s0 + cnt] = x[:cnt * J].view(cnt, J, n).float()
             solves += cnt * J
+            self._report(t0, "basis solves %d / %d" % (solves, S * J))
         torch.cuda.synchronize()
         self.stats.update({"basis_solves": solves, "basis_solve_s": time.perf_counter() - t0})
         return U
+
+    def _report(self, t0, msg):
+        now = time.perf_counter()
+        if self.progress and now - getattr(self, "_last_report", 0.0) > 20.0:
+            self._last_report = now
+            print("warm start: %s (%.0f s)" % (msg, now - t0), file=sys.stderr, flush=True)
 
     def features(self, xi, eta):
         import torch
@@ -104,6 +113,8 @@ class WarmStart:
                               torch.tensor(eta, dtype=torch.float32, device="cuda"), out=tgt)
             c = 0.5 * (1 + np.cos(np.pi * it / steps))
             losses.append(self.net.train_step(self.features(xis[idx], eta), tgt, lr_final + (lr - lr_final) * c))
+            if it % 1000 == 0:
+                self._report(t0, "training step %d / %d, loss %.3e" % (it, steps, losses[-1]))
         torch.cuda.synchronize()
         self.stats.update({"train_steps": steps, "train_batch": batch, "train_s": time.perf_counter() - t0,
                            "train_loss_first": losses[0], "train_loss_last": float(np.mean(losses[-50:]))})
