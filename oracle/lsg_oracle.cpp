@@ -18,6 +18,10 @@
 //                                          coefficients (exact by prewavelet semi-orthogonality)
 //   * precond3 diagonal                  : golden diagonal + brute force psi^T A_T psi
 //   * solveHomogen                       : golden iteration counts and solutions
+//   * basis transformations (Trafo)      : golden CalcHierarchicalBasis / calcPrewByNodal /
+//                                          calcNodalByPrew / H1 and L2 norms on regular and
+//                                          adaptive grids, brute-force basis evaluation on
+//                                          regular grids, round trips on adaptive grids
 //
 // Build: see oracle/Makefile (g++ -O2 -fopenmp -shared).
 #include <cstdint>
@@ -730,14 +734,6 @@ struct Trafo {
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sum(g.depths[a]) > sum(g.depths[b]); });
     }
     int sum(const Depth& T) const { int s = 0; for (int i = 0; i < g.d; ++i) s += T[i]; return s; }
-    bool complete() const {
-        for (size_t t = 0; t < g.depths.size(); ++t) {
-            int64_t c = 1;
-            for (int i = 0; i < g.d; ++i) c *= (int64_t(1) << g.depths[t][i]) - 1;
-            if ((int64_t)ml.slotsG[t].size() != c) return false;
-        }
-        return true;
-    }
     bool leq(const Depth& A, const Depth& B) const {
         for (int i = 0; i < g.d; ++i) if (A[i] > B[i]) return false;
         return true;
@@ -759,14 +755,16 @@ struct Trafo {
         for (int i = 0; i < g.d; ++i)
             for (int t : order) {
                 if (upTo && !leq(g.depths[t], *upTo)) continue;
-                for (int64_t s : g.slotsOfDepth[t]) u[s] = u[s] - 0.5 * (pole(u, s, i, true) + pole(u, s, i, false));
+                for (int64_t s : g.slotsOfDepth[t])
+                    if (g.active[s]) u[s] = u[s] - 0.5 * (pole(u, s, i, true) + pole(u, s, i, false));
             }
     }
     // CalcNodalBasis (BasisTransformations.cpp:174-198): coarsest subgrids first.
     void dehierarchize(std::vector<double>& u) const {
         for (int i = 0; i < g.d; ++i)
             for (auto it = order.rbegin(); it != order.rend(); ++it)
-                for (int64_t s : g.slotsOfDepth[*it]) u[s] = u[s] + 0.5 * (pole(u, s, i, true) + pole(u, s, i, false));
+                for (int64_t s : g.slotsOfDepth[*it])
+                    if (g.active[s]) u[s] = u[s] + 0.5 * (pole(u, s, i, true) + pole(u, s, i, false));
     }
     // PrewaveletMatrixHomogen::create (sgoperation/matrix.cpp:74-139) for a full line of
     // level t (n = 2^t - 1 positions, 1-based): column j is the basis function at position
@@ -814,50 +812,171 @@ struct Trafo {
             b[r] = s / M[(size_t)r * n + r];
         }
     }
-    // slot of the point at level-T box position k[] (1-based per dimension)
-    int64_t slotAt(const Depth& T, const int* k) const {
+    // Points by their level-T positions (1 .. 2^T_i - 1 interior; 0 and 2^T_i boundary).
+    static int32_t idxAtPos(int32_t k, int t) {
+        if (k <= 0) return 0;
+        if (k >= (1 << t)) return 1;
+        const int tz = __builtin_ctz((unsigned)k);
+        return (1 << (t - tz)) + (k >> (tz + 1));
+    }
+    static int32_t posOfIdx(int32_t v, int t) {
+        if (v == 0) return 0;
+        if (v == 1) return 1 << t;
+        const int sd = depth1(v);
+        return (2 * (v - (1 << sd)) + 1) << (t - sd);
+    }
+    // slot at level-T positions k[] (-1: not stored, or a boundary position)
+    int64_t slotAtPos(const Depth& T, const int32_t* k) const {
         Key key{};
         for (int i = 0; i < g.d; ++i) {
-            int tz = __builtin_ctz((unsigned)k[i]);
-            key.v[i] = (1 << (T[i] - tz)) + (k[i] >> (tz + 1));
+            if (k[i] <= 0 || k[i] >= (1 << T[i])) return -1;
+            key.v[i] = idxAtPos(k[i], T[i]);
         }
-        int64_t s = -1;
-        g.occupied(key, s);
-        return s;
+        int64_t q = -1;
+        return g.occupied(key, q) ? q : -1;
     }
-    // SolvePrewavelet / GetPrewaveletCoefficients (BasisTransformations.cpp:100-172,
-    // 468-484): along every dimension, every line of G_T is replaced by the solution of
-    // M x = line; the values left at the points of depth T are their prewavelet
-    // coefficients.
-    void solvePrewavelet(const Depth& T, const std::vector<double>& u, std::vector<double>& prew) const {
+    // stored and worked on (workonindex: active; hanging nodes are not worked on,
+    // sgrid/sparseGrid.h:189-193)
+    bool work(const Depth& T, const int32_t* k, int64_t* out = nullptr) const {
+        const int64_t q = slotAtPos(T, k);
+        if (out) *out = q;
+        return q >= 0 && g.active[q];
+    }
+    // ZusammenhangsKomponente::recursiveMarkArbeite (sgrid/komponente.cpp:43-73): depth-T
+    // points connected through the level-T 5^d stencil (IndexDimension::nextFive,
+    // indices/index.cc:124-176: a step along dimension i stops when its first
+    // level-T step reaches the boundary); the box [mn, mx] spans every stored, worked-on
+    // stencil point met on the way.
+    void markComponent(const Depth& T, int64_t s, std::vector<char>& marked, int32_t* mn, int32_t* mx) const {
         const int d = g.d;
-        int n[MAXD];
-        int64_t tot = 1;
-        for (int i = 0; i < d; ++i) { n[i] = (1 << T[i]) - 1; tot *= n[i]; }
-        std::vector<int64_t> slot(tot);
-        std::vector<double> box(tot);
-        for (int64_t p = 0; p < tot; ++p) {
-            int k[MAXD];
-            int64_t r = p;
-            for (int i = 0; i < d; ++i) { k[i] = int(r % n[i]) + 1; r /= n[i]; }
-            slot[p] = slotAt(T, k);
-            box[p] = u[slot[p]];
+        marked[s] = 1;
+        int32_t k0[MAXD];
+        for (int i = 0; i < d; ++i) k0[i] = posOfIdx(g.pts[s].v[i], T[i]);
+        int o[MAXD];
+        for (int i = 0; i < d; ++i) o[i] = -2;
+        while (true) {
+            bool ok = true;
+            int32_t k[MAXD];
+            for (int i = 0; i < d && ok; ++i) {
+                if (o[i] < 0 && k0[i] - 1 < 1) ok = false;
+                if (o[i] > 0 && k0[i] + 1 > (1 << T[i]) - 1) ok = false;
+                k[i] = k0[i] + o[i];
+            }
+            if (ok) {
+                int64_t q;
+                if (work(T, k, &q) && !marked[q]) {
+                    for (int i = 0; i < d; ++i) { mn[i] = std::min(mn[i], k[i]); mx[i] = std::max(mx[i], k[i]); }
+                    if (g.depthOf[q] == T) markComponent(T, q, marked, mn, mx);
+                }
+            }
+            int i = 0;
+            while (i < d && ++o[i] > 2) o[i++] = -2;
+            if (i == d) break;
         }
-        for (int i = 0; i < d; ++i) {
-            if (n[i] == 1) continue;
-            auto M = lineMatrix(T[i]);
-            int64_t stride = 1;
-            for (int j = 0; j < i; ++j) stride *= n[j];
-            std::vector<double> line(n[i]);
-            for (int64_t p = 0; p < tot; ++p) {
-                if ((p / stride) % n[i] != 0) continue;   // line starts
-                for (int q = 0; q < n[i]; ++q) line[q] = box[p + q * stride];
-                solveDense(M, line);
-                for (int q = 0; q < n[i]; ++q) box[p + q * stride] = line[q];
+        for (int i = 0; i < d; ++i) { mn[i] = std::min(mn[i], k0[i]); mx[i] = std::max(mx[i], k0[i]); }
+    }
+    // PrewaveletMatrixHomogen::create (sgoperation/matrix.cpp:74-139) for the line along dim
+    // that starts at level-T position k[]: column j is the basis function at line position
+    // j in level-T nodal values -- the prewavelet (0.1, -0.6, 1, -0.6, 0.1; 0.9 next to the
+    // boundary) at positions of depth T_dim, the coarser hat (0.5, 1, 0.5) elsewhere; an
+    // entry exists only where that neighbour is stored and worked on.  Entries that would
+    // fall outside the line are dropped.  Row-major size x size.
+    std::vector<double> lineMatrixAt(const Depth& T, const int32_t* kStart, int dim, int size) const {
+        std::vector<double> M((size_t)size * size, 0.0);
+        auto set = [&](int r, int c, double v) { if (r >= 0 && r < size) M[(size_t)r * size + c] = v; };
+        const int t = T[dim];
+        int32_t k[MAXD];
+        for (int i = 0; i < g.d; ++i) k[i] = kStart[i];
+        auto workAt = [&](int off) {
+            int32_t kk[MAXD];
+            for (int i = 0; i < g.d; ++i) kk[i] = k[i];
+            kk[dim] += off;
+            return work(T, kk);
+        };
+        for (int c = 0; c < size && work(T, k); ++c, ++k[dim]) {
+            const int dpos = t - __builtin_ctz((unsigned)k[dim]);       // depth of the position along dim
+            if (dpos == t) {
+                if (k[dim] - 1 == 0) {
+                    set(c, c, 0.9);
+                    if (t == 1) { set(c, c, 1.0); break; }
+                    if (workAt(1)) set(c + 1, c, -0.6);
+                    if (workAt(2)) set(c + 2, c, 0.1);
+                } else if (k[dim] + 1 == (1 << t)) {
+                    set(c, c, 0.9);
+                    if (workAt(-1)) set(c - 1, c, -0.6);
+                    if (workAt(-2)) set(c - 2, c, 0.1);
+                } else {
+                    set(c, c, 1.0);
+                    if (workAt(-1)) set(c - 1, c, -0.6);
+                    if (workAt(-2)) set(c - 2, c, 0.1);
+                    if (workAt(1)) set(c + 1, c, -0.6);
+                    if (workAt(2)) set(c + 2, c, 0.1);
+                }
+            } else {
+                set(c, c, 1.0);
+                if (workAt(-1)) set(c - 1, c, 0.5);
+                if (workAt(1)) set(c + 1, c, 0.5);
             }
         }
-        for (int64_t p = 0; p < tot; ++p)
-            if (g.depthOf[slot[p]] == T) prew[slot[p]] = box[p];
+        return M;
+    }
+    // SolvePrewavelet / GetPrewaveletCoefficients (BasisTransformations.cpp:100-172,
+    // 468-484): for each connected component of depth-T points (found in the order of
+    // their first point among the depth-T points, ZusammenhangsKomponente::findAnicht,
+    // komponente.cpp:9-41), along every dimension, every line of the component's box is
+    // replaced by the solution of M x = line (M from the line through the box's lower
+    // corner, of the length of its stored run); the values at worked-on points are
+    // written back, and those at depth-T points are their prewavelet coefficients.
+    void solvePrewavelet(const Depth& T, std::vector<double>& u, std::vector<double>& prew) const {
+        const int d = g.d;
+        const int t = g.depthIdx.at(T);
+        std::vector<char> marked(g.pts.size(), 0);
+        for (int64_t s0 : g.slotsOfDepth[t]) {
+            if (marked[s0] || !g.active[s0]) continue;
+            int32_t mn[MAXD], mx[MAXD];
+            for (int i = 0; i < d; ++i) mn[i] = mx[i] = posOfIdx(g.pts[s0].v[i], T[i]);
+            markComponent(T, s0, marked, mn, mx);
+            for (int dim = 0; dim < d; ++dim) {
+                int size = 1;
+                {
+                    int32_t k[MAXD];
+                    for (int i = 0; i < d; ++i) k[i] = mn[i];
+                    for (;;) {
+                        ++k[dim];
+                        if (k[dim] >= (1 << T[dim]) || !work(T, k)) break;
+                        ++size;
+                    }
+                }
+                const std::vector<double> M = lineMatrixAt(T, mn, dim, size);
+                // every line start: the box's positions with dim fixed at mn[dim]
+                int32_t k[MAXD];
+                for (int i = 0; i < d; ++i) k[i] = mn[i];
+                std::vector<double> line(size);
+                std::vector<int64_t> sl(size);
+                while (true) {
+                    int32_t kk[MAXD];
+                    for (int i = 0; i < d; ++i) kk[i] = k[i];
+                    for (int q = 0; q < size; ++q, ++kk[dim]) {
+                        sl[q] = slotAtPos(T, kk);
+                        line[q] = sl[q] >= 0 ? u[sl[q]] : 0.0;
+                    }
+                    solveDense(M, line);
+                    for (int q = 0; q < size; ++q) {
+                        const int64_t c = sl[q];
+                        if (c < 0 || !g.active[c]) continue;
+                        if (g.depthOf[c] == T) prew[c] = line[q];
+                        u[c] = line[q];
+                    }
+                    int i = 0;
+                    for (; i < d; ++i) {
+                        if (i == dim) continue;
+                        if (++k[i] <= mx[i]) break;
+                        k[i] = mn[i];
+                    }
+                    if (i == d) break;
+                }
+            }
+        }
     }
     // CalcHierbyPrew (BasisTransformations.cpp:321-353): the depth-T prewavelets of prew
     // as nodal values (the 5^d stencil of nextFive) hierarchized on the subgrids <= T.
@@ -874,7 +993,7 @@ struct Trafo {
                 double w = prew[s];
                 for (int i = 0; i < d; ++i) { k.v[i] = e[i][c[i]].idx; w *= e[i][c[i]].c; }
                 int64_t q;
-                if (g.occupied(k, q)) hier[q] += w;
+                if (g.occupied(k, q) && g.active[q]) hier[q] += w;
                 int i = 0;
                 while (i < d && ++c[i] == ne[i]) c[i++] = 0;
                 if (i == d) break;
@@ -892,7 +1011,7 @@ struct Trafo {
             solvePrewavelet(T, u, prew);
             hierByPrew(T, t, prew, hnew);
             for (size_t s = 0; s < hier.size(); ++s)
-                if (leq(g.depthOf[s], T)) hier[s] = hier[s] - hnew[s];
+                if (leq(g.depthOf[s], T) && g.active[s]) hier[s] = hier[s] - hnew[s];
             u = hier;
             dehierarchize(u);
         }
@@ -904,7 +1023,7 @@ struct Trafo {
             const Depth& T = g.depths[t];
             hierByPrew(T, t, prew, hnew);
             for (size_t s = 0; s < hier.size(); ++s)
-                if (leq(g.depthOf[s], T)) hier[s] = hier[s] + hnew[s];
+                if (leq(g.depthOf[s], T) && g.active[s]) hier[s] = hier[s] + hnew[s];
         }
         u = hier;
         dehierarchize(u);
@@ -1017,7 +1136,6 @@ int orc_cg(void* h, double eps, const double* f, double* x, int maxit, double* d
 int orc_transform(void* h, int what, const double* in, double* out) {
     auto* o = (orc::Oracle*)h;
     orc::Trafo tr(o->g, o->ml);
-    if (!tr.complete()) return -1;
     std::vector<double> a(in, in + o->N()), b;
     switch (what) {
         case 0: tr.hierarchize(a); b = a; break;

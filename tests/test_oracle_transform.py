@@ -27,6 +27,7 @@ from test_oracle import interp_to, kron_all, load, prewavelet_nodal_1d, relmax
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = json.load(open(os.path.join(GOLD, "cases.json")))
 REGULAR = sorted(c for c in CASES if CASES[c]["argv"][0] == "regular")
+ADAPTIVE = sorted(c for c in CASES if CASES[c]["argv"][0] == "adaptive")
 
 
 def coords(idx):
@@ -35,21 +36,39 @@ def coords(idx):
     return k / 2.0 ** depth
 
 
-@pytest.mark.parametrize("case", REGULAR)
+@pytest.mark.parametrize("case", REGULAR + ADAPTIVE)
 def test_golden_transforms(case):
+    """Regular and adaptive grids; on the adaptive ones the prewavelet solve works per
+    connected component of each depth's points, with line matrices cut at missing and
+    hanging points, and only active points are written."""
     gv = load(case)
     o = Oracle(gv["idx"], gv["active"])
     u = gv["tr_u"]
     assert relmax(o.transform(o.HIER, u), gv["tr_hier"]) < 1e-14
     assert relmax(o.transform(o.PREW_BY_NODAL, u), gv["tr_prew"]) < 1e-13
     assert relmax(o.transform(o.NODAL_BY_PREW, gv["x"]), gv["tr_nodal_of_x"]) < 1e-13
-    # the reference driver's f_prew = calcPrewByNodal(d pi^2 prod sin(pi x_i))
+    # the reference driver's f_prew = calcPrewByNodal(d pi^2 prod sin(pi x_i)) at the active points
     d = CASES[case]["d"]
-    f = d * math.pi ** 2 * np.prod(np.sin(math.pi * coords(gv["idx"])), axis=1)
+    f = d * math.pi ** 2 * np.prod(np.sin(math.pi * coords(gv["idx"])), axis=1) * gv["active"]
     assert relmax(o.transform(o.PREW_BY_NODAL, f), gv["f_prew"]) < 1e-13
 
 
-@pytest.mark.parametrize("case", REGULAR)
+@pytest.mark.parametrize("case", ADAPTIVE)
+def test_adaptive_round_trip(case):
+    """calcNodalByPrew inverts calcPrewByNodal at the active points, and neither writes a
+    hanging point."""
+    gv = load(case)
+    act = gv["active"].astype(bool)
+    assert (~act).any()
+    o = Oracle(gv["idx"], gv["active"])
+    p = o.transform(o.PREW_BY_NODAL, gv["tr_u"])
+    back = o.transform(o.NODAL_BY_PREW, p)
+    assert np.abs(back - gv["tr_u"])[act].max() < 1e-13
+    assert not p[~act].any() and not back[~act].any()
+    assert not o.transform(o.HIER, gv["tr_u"])[~act].any()
+
+
+@pytest.mark.parametrize("case", REGULAR + ADAPTIVE)
 def test_golden_norms(case):
     """H1::getValue / L2::getValue: sqrt(v^T A v) over active points, v = prew(u), with
     the Poisson (H1) or HelmHoltz (L2) operator (H1_Norm.cpp:8-78)."""
