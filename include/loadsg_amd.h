@@ -187,8 +187,11 @@ int lsg_ir_solve_many(int ngroups, lsg_op* const* op64, lsg_op* const* op32, con
 int lsg_rhs_nodes(const lsg_op* op, int64_t* count, double* nodes);
 int lsg_rhs_separable(lsg_op* op, int nterms, const double* vals, const double* w, void* b, int layout);
 /* ------------------------------------------------- basis transformations */
-/* FP64 operators on grids with complete depth boxes (regular grids); LSG_ERR_UNSUPPORTED
- * otherwise.  Vectors as in lsg_matvec; in and out must be different arrays.
+/* FP64 operators (LSG_ERR_UNSUPPORTED on FP32 ones), regular and adaptive grids.  On
+ * adaptive grids only active points are written (hanging points keep 0 in the outputs of
+ * lsg_prew_by_nodal and lsg_nodal_by_prew, and their input values in lsg_hierarchize /
+ * lsg_dehierarchize), as the reference's workonindex tests do.  Vectors as in
+ * lsg_matvec; in and out must be different arrays.
  * Hierarchical surpluses from nodal values: CalcHierarchicalBasis(u)
  * (BasisTransformations/BasisTransformations.cpp:390-409). */
 int lsg_hierarchize(lsg_op* op, const void* u, void* hier, int layout);

@@ -96,6 +96,9 @@ int envInt(const char* name, int dflt) {
 }
 int singlePlanMaxBatch() { return envInt("LSG_SINGLE_PLAN_BATCH", 1 << 30); }
 int oneClassMaxBatch() { return envInt("LSG_ONE_CLASS_BATCH", 4); }
+// LDS bytes the adaptive-grid transform may give one workgroup's boxes (default: the
+// device's limit); larger boxes are staged in global memory
+int64_t trLdsLimit(int64_t deviceMax) { return std::min<int64_t>(deviceMax, envInt("LSG_TR_LDS_BYTES", (int)deviceMax)); }
 
 int coefTerms(const lsg_coef& c, int d, int& nA, int& nC) {
     switch (c.kind) {
@@ -262,6 +265,7 @@ namespace {
 struct TrDev {
     TrTables host;
     DevMem items, gidx, list, rowDev, rowPtr, col, fac, poles, byLevel;
+    DevMem bstat, comps, compOff, afac;   // adaptive grids
     DevMem hier, work, work2, scratch, norms;
 };
 }  // namespace
@@ -1099,7 +1103,7 @@ int irSolveGroups(std::vector<IrGroup>& G, double rel_tol, double inner_rel_tol,
     return rc;
 }
 
-// Tables and work vectors of the basis transformations (FP64 operators on the dense path).
+// Tables and work vectors of the basis transformations (FP64 operators).
 int ensureTransform(lsg_op* op) {
     if (op->tr) return LSG_OK;
     if (op->prec != 64) return fail(LSG_ERR_UNSUPPORTED, "basis transformations run on an FP64 operator");
@@ -1108,7 +1112,8 @@ int ensureTransform(lsg_op* op) {
     std::string err;
     if (int rc = buildTransform(g, tr->host, err)) return fail(rc, err);
     const TrTables& h = tr->host;
-    if (h.maxBox * 8 > 65536) return fail(LSG_ERR_UNSUPPORTED, "depth box too large for the transform workgroup");
+    if (!h.adaptive && h.maxBox * 8 > 65536)
+        return fail(LSG_ERR_UNSUPPORTED, "depth box too large for the transform workgroup");
     auto up = [&](DevMem& m, const void* src, size_t bytes) -> hipError_t {
         hipError_t e = m.alloc(bytes);
         if (e != hipSuccess) return e;
@@ -1123,11 +1128,15 @@ int ensureTransform(lsg_op* op) {
     HIPCHK(up(tr->fac, h.fac.data(), h.fac.size() * 8));
     HIPCHK(up(tr->poles, h.poles.data(), h.poles.size() * 4));
     HIPCHK(up(tr->byLevel, h.byLevel.data(), h.byLevel.size() * 4));
+    HIPCHK(up(tr->bstat, h.bstat.data(), h.bstat.size()));
+    HIPCHK(up(tr->comps, h.comps.data(), h.comps.size() * sizeof(TrComp)));
+    HIPCHK(up(tr->compOff, h.compOff.data(), h.compOff.size() * 4));
+    HIPCHK(up(tr->afac, h.afac.data(), h.afac.size() * 8));
     const size_t nb = (size_t)g.n * op->batch * 8;
     HIPCHK(tr->hier.alloc(nb));
     HIPCHK(tr->work.alloc(nb));
     HIPCHK(tr->work2.alloc(nb));
-    HIPCHK(tr->scratch.alloc((size_t)g.nml * op->batch * 8));
+    HIPCHK(tr->scratch.alloc((size_t)h.boxTot * op->batch * 8));
     HIPCHK(tr->norms.alloc((size_t)op->batch * 8));
     op->tr = std::move(tr);
     return LSG_OK;
@@ -1139,7 +1148,7 @@ void trBox(lsg_op* op, int mode, const int32_t* list, int count, int64_t maxBox,
     const Grid& g = op->grid->g;
     TrDev& t = *op->tr;
     launch_tr_box(mode, t.items.as<TrItem>(), list, count, t.gidx.as<int32_t>(), t.fac.as<double>(), t.host.facTot,
-                  maxBox, in, out, t.scratch.as<double>(), g.n, g.nml, op->batch, op->stream);
+                  maxBox, in, out, t.scratch.as<double>(), g.n, t.host.boxTot, op->batch, op->stream);
 }
 
 void trGather(lsg_op* op, size_t group, double* v, int sub) {
@@ -1147,7 +1156,7 @@ void trGather(lsg_op* op, size_t group, double* v, int sub) {
     TrDev& t = *op->tr;
     const auto& gr = t.host.groups[group];
     launch_tr_gather(t.rowDev.as<int32_t>() + gr[0], t.rowPtr.as<int64_t>() + gr[0], t.col.as<int32_t>(), gr[1],
-                     t.scratch.as<double>(), v, g.n, g.nml, sub, op->batch, op->stream);
+                     t.scratch.as<double>(), v, g.n, t.host.boxTot, sub, op->batch, op->stream);
 }
 
 // CalcHierarchicalBasis as d point-parallel passes (one per dimension, in dimension
@@ -1179,6 +1188,19 @@ int trDehierPoles(lsg_op* op, double* v) {
     return LSG_OK;
 }
 
+// One adaptive-grid box launch (mode 0: depth `depth` of calcPrewByNodal, 1: every depth's
+// prewavelets in hierarchical form into the scratch).
+void trAdapt(lsg_op* op, int mode, int depth, const double* src, double* hier, double* out, int first) {
+    TrDev& t = *op->tr;
+    const TrTables& h = t.host;
+    const int nitems = mode == 0 ? 1 : (int)h.items.size();
+    const int64_t mb = mode == 0 ? h.items[depth].total : h.maxBox;
+    launch_tr_adapt(mode, t.items.as<TrItem>(), depth, nitems, mb, t.comps.as<TrComp>(), t.compOff.as<int32_t>(),
+                    t.gidx.as<int32_t>(), t.bstat.as<uint8_t>(), t.afac.as<double>(), h.afacTot, src, hier, out,
+                    t.scratch.as<double>(), op->n(), h.boxTot, first, trLdsLimit(op->maxLds), op->batch,
+                    op->stream);
+}
+
 // Device vectors (all instances, device order): what 0 hierarchize, 1 dehierarchize,
 // 2 prewavelets from nodal values, 3 nodal values from prewavelets.
 int transformDev(lsg_op* op, int what, const double* in, double* out) {
@@ -1187,6 +1209,26 @@ int transformDev(lsg_op* op, int what, const double* in, double* out) {
     TrDev& t = *op->tr;
     const int nd = (int)t.host.items.size();
     const int64_t mb = t.host.maxBox;
+    if (t.host.adaptive && what == 2) {   // calcPrewByNodal, depth by depth (BasisTransformations.cpp:486-545)
+        double* hier = t.hier.as<double>();
+        trHierPoles(op, in, hier);
+        HIPCHK(hipMemsetAsync(out, 0, (size_t)op->n() * op->batch * 8, op->stream));
+        for (size_t k = 0; k < t.host.seq.size(); ++k)
+            trAdapt(op, 0, t.host.seq[k], k == 0 ? in : hier, hier, out, k == 0);
+        HIPCHK(hipGetLastError());
+        return LSG_OK;
+    }
+    if (t.host.adaptive && what == 3) {   // calcNodalByPrew (BasisTransformations.cpp:670-707)
+        double* hier = t.hier.as<double>();
+        trAdapt(op, 1, -1, in, nullptr, nullptr, 0);
+        // the gather writes the active points only; hanging points start (and stay) at 0
+        HIPCHK(hipMemsetAsync(hier, 0, (size_t)op->n() * op->batch * 8, op->stream));
+        trGather(op, t.host.groups.size() - 1, hier, 0);
+        HIPCHK(hipMemcpyAsync(out, hier, (size_t)op->n() * op->batch * 8, hipMemcpyDeviceToDevice, op->stream));
+        if (int rc2 = trDehierPoles(op, out)) return rc2;
+        HIPCHK(hipGetLastError());
+        return LSG_OK;
+    }
     switch (what) {
         case 0: trHierPoles(op, in, out); break;
         case 1:

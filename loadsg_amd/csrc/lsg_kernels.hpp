@@ -162,6 +162,12 @@ void launch_tr_box(int mode, const TrItem* items, const int32_t* list, int nitem
                    int64_t n, int64_t nml, int batch, hipStream_t s);
 void launch_tr_gather(const int32_t* rowDev, const int64_t* rowPtr, const int32_t* col, int64_t nrows,
                       const double* scratch, double* v, int64_t n, int64_t nml, int sub, int batch, hipStream_t s);
+// adaptive grids: mode 0 one depth of calcPrewByNodal (nitems 1), 1 hierarchical
+// representation of the prewavelets of every depth (nitems = number of depths)
+void launch_tr_adapt(int mode, const TrItem* items, int depth, int nitems, int64_t maxBox, const TrComp* comps,
+                     const int32_t* compOff, const int32_t* gidx, const uint8_t* bstat, const double* afac,
+                     int64_t afacTot, const double* src, double* hier, double* out, double* scratch, int64_t n,
+                     int64_t boxTot, int first, int64_t ldsLimit, int batch, hipStream_t s);
 void launch_tr_pole(const int32_t* poles, const double* in, double* out, int64_t n, int batch, hipStream_t s);
 void launch_tr_dehier(const int32_t* list, int64_t cnt, const int32_t* poles, double* v, int64_t n, int batch,
                       hipStream_t s);

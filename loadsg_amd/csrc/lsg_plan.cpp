@@ -854,6 +854,37 @@ namespace {
 // to the boundary, 1 at level 1) at the odd positions and the level-(t-1) hat
 // (0.5, 1, 0.5) at the even ones.  The matrix is pentadiagonal and its pivots lie in
 // [0.9, 4/3], as in the reference's own unpivoted LR (matrix.cpp:141-182).
+// LU factors without pivoting of a pentadiagonal n x n matrix given by its band
+// B[r][c - r + 2] (1-based r, c): l1[k] = L[k][k-1], l2[k] = L[k][k-2], u0/u1/u2[k] =
+// U[k][k..k+2] (0-based k).  False on a zero pivot.
+bool bandFactor(int n, std::vector<std::array<double, 5>>& B, double* l1, double* l2, double* u0, double* u1,
+                double* u2) {
+    auto at = [&](int r, int c) -> double& { return B[r][c - r + 2]; };
+    for (int c = 1; c <= n; ++c) {
+        l1[c - 1] = 0.0;
+        l2[c - 1] = 0.0;
+    }
+    for (int c = 1; c <= n; ++c) {
+        const double p = at(c, c);
+        if (p == 0.0) return false;
+        for (int r = c + 1; r <= std::min(n, c + 2); ++r) {
+            const double f = at(r, c) / p;
+            (r == c + 1 ? l1 : l2)[r - 1] = f;
+            for (int k = c; k <= std::min(n, c + 2); ++k) at(r, k) -= f * at(c, k);
+        }
+        u0[c - 1] = p;
+        u1[c - 1] = c + 1 <= n ? at(c, c + 1) : 0.0;
+        u2[c - 1] = c + 2 <= n ? at(c, c + 2) : 0.0;
+    }
+    return true;
+}
+
+// LU factors without pivoting of the level-t line matrix of PrewaveletMatrixHomogen::create
+// (sgoperation/matrix.cpp:74-139): column j (1-based) is the basis function at line
+// position j in level-t nodal values, the prewavelet (0.1, -0.6, 1, -0.6, 0.1; 0.9 next
+// to the boundary, 1 at level 1) at the odd positions and the level-(t-1) hat
+// (0.5, 1, 0.5) at the even ones.  The matrix is pentadiagonal and its pivots lie in
+// [0.9, 4/3], as in the reference's own unpivoted LR (matrix.cpp:141-182).
 void lineFactors(int t, double* l1, double* l2, double* u0, double* u1, double* u2) {
     const int n = (1 << t) - 1;
     std::vector<std::array<double, 5>> B(n + 2);
@@ -869,31 +900,309 @@ void lineFactors(int t, double* l1, double* l2, double* u0, double* u1, double* 
             set(j, 1.0); set(j - 1, 0.5); set(j + 1, 0.5);
         }
     }
-    for (int c = 1; c <= n; ++c) {
-        l1[c - 1] = 0.0;
-        l2[c - 1] = 0.0;
-    }
-    for (int c = 1; c <= n; ++c) {
-        const double p = at(c, c);
-        for (int r = c + 1; r <= std::min(n, c + 2); ++r) {
-            const double f = at(r, c) / p;
-            (r == c + 1 ? l1 : l2)[r - 1] = f;
-            for (int k = c; k <= std::min(n, c + 2); ++k) at(r, k) -= f * at(c, k);
+    bandFactor(n, B, l1, l2, u0, u1, u2);
+}
+
+// ---- adaptive grids
+
+struct AdaptiveBox {
+    const TrItem& it;
+    const int32_t* gidx;     // this box's entries
+    const uint8_t* st;
+    int d;
+    // box entry of level-T positions k (-1 outside the interior)
+    int64_t entry(const int32_t* k) const {
+        int64_t p = 0;
+        for (int i = 0; i < d; ++i) {
+            if (k[i] < 1 || k[i] > it.n[i]) return -1;
+            p += int64_t(k[i] - 1) * it.S[i];
         }
-        u0[c - 1] = p;
-        u1[c - 1] = c + 1 <= n ? at(c, c + 1) : 0.0;
-        u2[c - 1] = c + 2 <= n ? at(c, c + 2) : 0.0;
+        return p;
     }
+    bool work(const int32_t* k) const {   // stored and worked on (active)
+        const int64_t p = entry(k);
+        return p >= 0 && st[p] == TR_ACTIVE;
+    }
+    bool ownDepth(const int32_t* k) const {
+        for (int i = 0; i < d; ++i) if (!(k[i] & 1)) return false;
+        return true;
+    }
+};
+
+// Components of the depth-T points (ZusammenhangsKomponente::findAnicht and
+// recursiveMarkArbeite, sgrid/komponente.cpp:9-73): in the order of their first point
+// among the depth-T points in slot order, each the closure of its points under the
+// level-T 5^d stencil of IndexDimension::nextFive (indices/index.cc:124-176: a step along
+// dimension i is taken only when the first level-T step stays inside); the box spans
+// every stored, worked-on stencil point met.  Line matrices per dimension follow
+// GetPrewaveletCoefficients and PrewaveletMatrixHomogen::create
+// (BasisTransformations.cpp:100-172, sgoperation/matrix.cpp:74-139).
+int adaptiveComponents(const AdaptiveBox& B, const std::vector<std::array<int32_t, MAXD>>& ownPts, TrTables& tr,
+                       std::string& err) {
+    const int d = B.d;
+    const TrItem& it = B.it;
+    std::vector<char> marked((size_t)it.total, 0);
+    for (const auto& k0 : ownPts) {
+        const int64_t p0 = B.entry(k0.data());
+        if (marked[p0] || B.st[p0] != TR_ACTIVE) continue;
+        TrComp c{};
+        for (int i = 0; i < d; ++i) c.mn[i] = c.mx[i] = k0[i];
+        std::vector<std::array<int32_t, MAXD>> stack{k0};
+        marked[p0] = 1;
+        while (!stack.empty()) {
+            const auto k = stack.back();
+            stack.pop_back();
+            int o[MAXD];
+            for (int i = 0; i < d; ++i) o[i] = -2;
+            while (true) {
+                bool ok = true;
+                int32_t q[MAXD];
+                for (int i = 0; i < d && ok; ++i) {
+                    if (o[i] < 0 && k[i] - 1 < 1) ok = false;
+                    if (o[i] > 0 && k[i] + 1 > it.n[i]) ok = false;
+                    q[i] = k[i] + o[i];
+                }
+                if (ok && B.work(q)) {
+                    const int64_t pq = B.entry(q);
+                    if (!marked[pq]) {
+                        for (int i = 0; i < d; ++i) { c.mn[i] = std::min(c.mn[i], q[i]); c.mx[i] = std::max(c.mx[i], q[i]); }
+                        if (B.ownDepth(q)) {
+                            marked[pq] = 1;
+                            std::array<int32_t, MAXD> nq{};
+                            for (int i = 0; i < d; ++i) nq[i] = q[i];
+                            stack.push_back(nq);
+                        }
+                    }
+                }
+                int i = 0;
+                while (i < d && ++o[i] > 2) o[i++] = -2;
+                if (i == d) break;
+            }
+        }
+        // line matrices through the lower corner, one per dimension
+        for (int dim = 0; dim < d; ++dim) {
+            int32_t k[MAXD];
+            for (int i = 0; i < d; ++i) k[i] = c.mn[i];
+            int size = 1;
+            for (;;) {
+                ++k[dim];
+                if (k[dim] > it.n[dim] || !B.work(k)) break;
+                ++size;
+            }
+            c.size[dim] = size;
+            std::vector<std::array<double, 5>> M(size + 2);
+            for (auto& r : M) r.fill(0.0);
+            auto set = [&](int r, int col, double v) {   // 0-based row/column, dropped outside the line
+                if (r >= 0 && r < size) M[r + 1][col - r + 2] = v;
+            };
+            const int t = it.T[dim];
+            for (int i = 0; i < d; ++i) k[i] = c.mn[i];
+            auto workAt = [&](int off) {
+                int32_t kk[MAXD];
+                for (int i = 0; i < d; ++i) kk[i] = k[i];
+                kk[dim] += off;
+                return B.work(kk);
+            };
+            for (int col = 0; col < size && B.work(k); ++col, ++k[dim]) {
+                if (k[dim] & 1) {   // depth T_dim: prewavelet column
+                    if (k[dim] == 1) {
+                        set(col, col, 0.9);
+                        if (t == 1) { set(col, col, 1.0); break; }
+                        if (workAt(1)) set(col + 1, col, -0.6);
+                        if (workAt(2)) set(col + 2, col, 0.1);
+                    } else if (k[dim] == it.n[dim]) {
+                        set(col, col, 0.9);
+                        if (workAt(-1)) set(col - 1, col, -0.6);
+                        if (workAt(-2)) set(col - 2, col, 0.1);
+                    } else {
+                        set(col, col, 1.0);
+                        if (workAt(-1)) set(col - 1, col, -0.6);
+                        if (workAt(-2)) set(col - 2, col, 0.1);
+                        if (workAt(1)) set(col + 1, col, -0.6);
+                        if (workAt(2)) set(col + 2, col, 0.1);
+                    }
+                } else {            // coarser: hat column
+                    set(col, col, 1.0);
+                    if (workAt(-1)) set(col - 1, col, 0.5);
+                    if (workAt(1)) set(col + 1, col, 0.5);
+                }
+            }
+            c.fac[dim] = tr.afacTot;
+            tr.afacTot += size;
+            const size_t o = tr.afac.size();
+            tr.afac.resize(o + 5 * (size_t)size);
+            double* f = tr.afac.data() + o;
+            if (!bandFactor(size, M, f, f + size, f + 2 * size, f + 3 * size, f + 4 * size)) {
+                err = "singular prewavelet line matrix (a component's lower box corner is not an active point)";
+                return LSG_ERR_UNSUPPORTED;
+            }
+        }
+        tr.comps.push_back(c);
+    }
+    return LSG_OK;
+}
+
+int buildTransformAdaptive(const Grid& g, TrTables& tr, std::string& err) {
+    const int d = g.d;
+    const int nd = (int)g.depths.size();
+    tr.adaptive = true;
+    tr.items.assign(nd, TrItem{});
+    std::unordered_map<Key2, int32_t, Key2Hash> devOf;
+    devOf.reserve((size_t)g.n * 2);
+    for (int64_t s = 0; s < g.n; ++s) devOf[packKey(&g.idx[s * d], d)] = (int32_t)g.devOfSlot[s];
+    auto find = [&](const int32_t* v) -> int32_t {
+        auto f = devOf.find(packKey(v, d));
+        return f == devOf.end() ? -1 : f->second;
+    };
+    std::vector<uint8_t> actDev(g.n);
+    for (int64_t s = 0; s < g.n; ++s) actDev[g.devOfSlot[s]] = g.active[s];
+    int64_t off = 0;
+    tr.maxBox = 0;
+    for (int t = 0; t < nd; ++t) {
+        TrItem& it = tr.items[t];
+        it.d = d;
+        int64_t S = 1;
+        for (int i = 0; i < d; ++i) {
+            it.T[i] = g.depths[t][i];
+            it.n[i] = (1 << it.T[i]) - 1;
+            it.S[i] = (int32_t)S;
+            S *= it.n[i];
+        }
+        if (S >= (int64_t)INT32_MAX) { err = "depth box too large"; return LSG_ERR_CAPACITY; }
+        it.total = (int32_t)S;
+        it.box_off = off;
+        off += S;
+        tr.maxBox = std::max<int64_t>(tr.maxBox, S);
+    }
+    tr.boxTot = off;
+    if (off >= (int64_t)INT32_MAX) { err = "depth boxes too large"; return LSG_ERR_CAPACITY; }
+    tr.gidx.assign((size_t)off, -1);
+    tr.bstat.assign((size_t)off, TR_MISSING);
+    for (int t = 0; t < nd; ++t) {
+        const TrItem& it = tr.items[t];
+        for (int32_t p = 0; p < it.total; ++p) {
+            int32_t v[MAXD];
+            int32_t r = p;
+            for (int i = 0; i < d; ++i) {
+                v[i] = idxAt(r % it.n[i] + 1, it.T[i]);
+                r /= it.n[i];
+            }
+            const int32_t q = find(v);
+            tr.gidx[it.box_off + p] = q;
+            if (q >= 0) tr.bstat[it.box_off + p] = actDev[q] ? TR_ACTIVE : TR_HANGING;
+        }
+    }
+    // poles along every dimension (missing or boundary: -1); a hanging point is never
+    // changed by a (de)hierarchization, so its poles are -1 as well
+    tr.poles.assign((size_t)d * g.n * 2, -1);
+    for (int64_t s = 0; s < g.n; ++s) {
+        if (!g.active[s]) continue;
+        const int64_t dev = g.devOfSlot[s];
+        int32_t v[MAXD];
+        for (int i = 0; i < d; ++i) v[i] = g.idx[s * d + i];
+        for (int i = 0; i < d; ++i) {
+            const int t = depth1(v[i]);
+            const int32_t k = posAt(v[i], t), keep = v[i];
+            if (k - 1 >= 1) { v[i] = idxAt(k - 1, t); tr.poles[((size_t)i * g.n + dev) * 2] = find(v); }
+            if (k + 1 <= (1 << t) - 1) { v[i] = idxAt(k + 1, t); tr.poles[((size_t)i * g.n + dev) * 2 + 1] = find(v); }
+            v[i] = keep;
+        }
+    }
+    // the reference's order: by |T|_1 descending, equal |T|_1 in reverse first-appearance
+    // order (ListOfDepthOrderedSubgrids from gotoEnd() through previous(),
+    // sgrid/ListOfDepthOrderedGrids.cc:77-137, 219-244)
+    tr.seq.resize(nd);
+    for (int t = 0; t < nd; ++t) tr.seq[t] = nd - 1 - t;
+    auto l1 = [&](int t) { int s = 0; for (int i = 0; i < d; ++i) s += g.depths[t][i]; return s; };
+    std::stable_sort(tr.seq.begin(), tr.seq.end(), [&](int a, int b) { return l1(a) > l1(b); });
+    // components per depth, from the depth's points in slot order
+    std::vector<std::vector<std::array<int32_t, MAXD>>> own(nd);
+    for (int64_t s = 0; s < g.n; ++s) {
+        std::array<int32_t, MAXD> k{};
+        Depth T{};
+        for (int i = 0; i < d; ++i) T[i] = depth1(g.idx[s * d + i]);
+        const int t = g.depthIdx.at(T);
+        for (int i = 0; i < d; ++i) k[i] = posAt(g.idx[s * d + i], T[i]);
+        own[t].push_back(k);
+    }
+    tr.compOff.assign(nd + 1, 0);
+    tr.comps.clear();
+    tr.afac.clear();
+    tr.afacTot = 0;
+    for (int t = 0; t < nd; ++t) {
+        tr.compOff[t] = (int32_t)tr.comps.size();
+        AdaptiveBox B{tr.items[t], tr.gidx.data() + tr.items[t].box_off, tr.bstat.data() + tr.items[t].box_off, d};
+        if (int rc = adaptiveComponents(B, own[t], tr, err)) return rc;
+    }
+    tr.compOff[nd] = (int32_t)tr.comps.size();
+    // the factor arrays were appended per line as (l1, l2, u0, u1, u2) blocks; regroup
+    // them into five arrays of afacTot entries
+    {
+        std::vector<double> f(5 * (size_t)tr.afacTot);
+        size_t o = 0;
+        for (const TrComp& c : tr.comps)
+            for (int i = 0; i < d; ++i) {
+                for (int a = 0; a < 5; ++a)
+                    for (int q = 0; q < c.size[i]; ++q) f[(size_t)a * tr.afacTot + c.fac[i] + q] = tr.afac[o + (size_t)a * c.size[i] + q];
+                o += 5 * (size_t)c.size[i];
+            }
+        tr.afac.swap(f);
+    }
+    // points by their depth along each dimension, for the dehierarchization
+    tr.byLevel.assign((size_t)d * g.n, 0);
+    tr.lvlOff.assign((size_t)d * (MAXLEV + 2), 0);
+    for (int i = 0; i < d; ++i) {
+        int64_t* lo = &tr.lvlOff[(size_t)i * (MAXLEV + 2)];
+        std::vector<int64_t> cnt(MAXLEV + 2, 0);
+        for (int t = 0; t < nd; ++t) cnt[g.depths[t][i]] += g.blkSize[t];
+        for (int t = 1; t <= MAXLEV + 1; ++t) lo[t] = lo[t - 1] + cnt[t - 1];
+        std::vector<int64_t> fill(lo, lo + MAXLEV + 2);
+        for (int t = 0; t < nd; ++t)
+            for (int64_t q = 0; q < g.blkSize[t]; ++q)
+                tr.byLevel[(size_t)i * g.n + fill[g.depths[t][i]]++] = (int32_t)(g.blkOff[t] + q);
+    }
+    // calcNodalByPrew: the box results summed per stored active point (one group over
+    // all depths)
+    {
+        std::vector<std::pair<int32_t, int32_t>> pairs;
+        for (int t = 0; t < nd; ++t) {
+            const TrItem& it = tr.items[t];
+            for (int32_t p = 0; p < it.total; ++p)
+                if (tr.bstat[it.box_off + p] == TR_ACTIVE) pairs.push_back({tr.gidx[it.box_off + p], (int32_t)(it.box_off + p)});
+        }
+        std::stable_sort(pairs.begin(), pairs.end(),
+                         [](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) { return a.first < b.first; });
+        tr.rowDev.clear();
+        tr.col.clear();
+        tr.rowPtr.assign(1, 0);
+        for (size_t k = 0; k < pairs.size(); ++k) {
+            if (k == 0 || pairs[k].first != pairs[k - 1].first) {
+                if (k) tr.rowPtr.push_back((int64_t)tr.col.size());
+                tr.rowDev.push_back(pairs[k].first);
+            }
+            tr.col.push_back(pairs[k].second);
+        }
+        if (!pairs.empty()) tr.rowPtr.push_back((int64_t)tr.col.size());
+        tr.groups.assign(1, {0, (int64_t)tr.rowDev.size()});
+    }
+    tr.levelList.clear();
+    tr.levels.clear();
+    tr.levelMaxBox.clear();
+    tr.fac.clear();
+    tr.facTot = 0;
+    return LSG_OK;
 }
 
 }  // namespace
 
 int buildTransform(const Grid& g, TrTables& tr, std::string& err) {
-    if (g.sparse) { err = "basis transformations need complete depth boxes (dense path)"; return LSG_ERR_UNSUPPORTED; }
+    tr = TrTables();
+    if (g.sparse) return buildTransformAdaptive(g, tr, err);
     const int d = g.d;
     const int nd = (int)g.depths.size();
     tr.items.assign(nd, TrItem{});
     tr.gidx.assign((size_t)g.nml, 0);
+    tr.boxTot = g.nml;
     tr.maxBox = 0;
     for (int t = 0; t < nd; ++t) {
         TrItem& it = tr.items[t];

@@ -179,7 +179,8 @@ struct Grid {
 // Builds g.planSingle (dense grids only).
 int buildSinglePlan(Grid& g, std::string& err);
 
-// Basis transformations on the dense path (every box G_T complete): one item per depth.
+// Basis transformations: one item per depth, its level-T box G_T (every position of the
+// level-T full grid; on adaptive grids some positions are not stored).
 struct TrItem {
     int32_t T[MAXD];
     int32_t n[MAXD];      // box extents 2^T_i - 1
@@ -189,9 +190,22 @@ struct TrItem {
     int64_t box_off;      // first entry of the box in the gather table and the scratch
 };
 
+// Adaptive grids: one connected component of the points of one depth T
+// (ZusammenhangsKomponente, sgrid/komponente.cpp) with its box and the factors of the
+// prewavelet line matrix along every dimension.
+struct TrComp {
+    int32_t mn[MAXD], mx[MAXD];   // box corners, level-T positions (1-based)
+    int32_t size[MAXD];           // line length along each dimension (from the lower corner)
+    int64_t fac[MAXD];            // first entry of that line's factors in TrTables::afac
+};
+
+// box entry status on adaptive grids
+enum : uint8_t { TR_MISSING = 0, TR_HANGING = 1, TR_ACTIVE = 2 };
+
 struct TrTables {
+    bool adaptive = false;                     // grid with hanging nodes or incomplete boxes
     std::vector<TrItem> items;                 // per depth, depth list order
-    std::vector<int32_t> gidx;                 // per box entry: device index of its point
+    std::vector<int32_t> gidx;                 // per box entry: device index of its point (-1: not stored)
     std::vector<int32_t> levelList;            // depth ids, |T|_1 descending
     std::vector<std::array<int32_t, 2>> levels;   // (first in levelList, count) per |T|_1, descending
     std::vector<int64_t> levelMaxBox;          // largest |G_T| per level
@@ -215,9 +229,18 @@ struct TrTables {
     std::vector<double> fac;
     int64_t facTot = 0;
     int64_t maxBox = 0;
+    int64_t boxTot = 0;                        // sum of the box sizes (scratch per instance)
+    // adaptive grids only
+    std::vector<uint8_t> bstat;                // per box entry: TR_MISSING / TR_HANGING / TR_ACTIVE
+    std::vector<int32_t> seq;                  // depth ids in the reference's transformation order
+    std::vector<int32_t> compOff;              // per depth t: components compOff[t] .. compOff[t + 1]
+    std::vector<TrComp> comps;
+    std::vector<double> afac;                  // l1, l2, u0, u1, u2 arrays of afacTot entries each
+    int64_t afacTot = 0;
 };
 inline int64_t facBase(int t) { return (int64_t(1) << t) - t - 1; }
-// Tables of the basis transformations for a dense-path grid.
+// Tables of the basis transformations (dense boxes on regular grids, component-wise line
+// solves on adaptive ones).
 int buildTransform(const Grid& g, TrTables& tr, std::string& err);
 
 // Build the grid from a reference slot-order index table.  Returns 0 or a

@@ -23,6 +23,19 @@
 //   * nodal values from prewavelets (calcNodalByPrew, :670-707): every depth's
 //     hierarchical representation in one launch, a gather-sum per point, then
 //     dehierarchization.
+// On adaptive grids (hanging nodes, incomplete boxes) the box G_T keeps every level-T
+// position with a status (not stored, hanging, active), and the reference's own order of
+// work is kept:
+//   * prewavelets from nodal values: one launch per depth, in the reference's order
+//     (|T|_1 descending, ties in reverse first appearance).  The workgroup dehierarchizes
+//     the running hierarchical representation on its box (only active entries change),
+//     solves the lines of every connected component of the depth's points with the
+//     factors of its cut line matrices (GetPrewaveletCoefficients,
+//     BasisTransformations.cpp:100-172; only active entries are written), writes the
+//     prewavelets, forms their hierarchical representation on the box (scatter onto
+//     stored active entries, hierarchization of active entries) and subtracts it from
+//     the running representation in place;
+//   * nodal values from prewavelets: as on regular grids, with the same masks.
 // Lines along one dimension are handled by one thread each (sequential recurrences).
 // The boxes are small (at most a few thousand values), so a workgroup takes the boxes
 // of one depth for several instances (up to 32 KB of LDS) to give its threads enough
@@ -244,6 +257,173 @@ __global__ void tr_gather_kernel(const int32_t* __restrict__ rowDev, const int64
     *y = sub ? *y - s : s;
 }
 
+// ---- adaptive grids
+// 1D (de)hierarchization of a line of a level-t box with entry status st: only active
+// entries change (hanging points are not worked on, sgrid/sparseGrid.h:189-193), and
+// entries that are not stored hold 0.
+__device__ inline void hierLineMasked(double* v, const uint8_t* st, int t, int s, bool inverse) {
+    const int n = (1 << t) - 1;
+    for (int kk = 0; kk < t - 1; ++kk) {
+        const int k = inverse ? kk + 2 : t - kk;
+        const int h = 1 << (t - k);
+        for (int q = h; q <= n; q += 2 * h) {
+            if (st[(q - 1) * s] != TR_ACTIVE) continue;
+            const double l = q - h >= 1 ? v[(q - h - 1) * s] : 0.0;
+            const double r = q + h <= n ? v[(q + h - 1) * s] : 0.0;
+            const double m = 0.5 * (l + r);
+            v[(q - 1) * s] = inverse ? v[(q - 1) * s] + m : v[(q - 1) * s] - m;
+        }
+    }
+}
+
+// Solve M x = b in place on a line of n entries at stride s, M given by its unpivoted LU
+// factors l1, l2, u0, u1, u2 (arrays facTot apart, f at this line's first l1 entry).
+__device__ inline void solveBand(double* v, int n, int s, const double* f, int64_t facTot) {
+    const double* l1 = f;
+    const double* l2 = f + facTot;
+    const double* u0 = f + 2 * facTot;
+    const double* u1 = f + 3 * facTot;
+    const double* u2 = f + 4 * facTot;
+    double y1 = 0.0, y2 = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const double y = v[k * s] - l1[k] * y1 - l2[k] * y2;
+        v[k * s] = y;
+        y2 = y1;
+        y1 = y;
+    }
+    double x1 = 0.0, x2 = 0.0;
+    for (int k = n - 1; k >= 0; --k) {
+        const double x = (v[k * s] - u1[k] * x1 - u2[k] * x2) / u0[k];
+        v[k * s] = x;
+        x2 = x1;
+        x1 = x;
+    }
+}
+
+enum { TA_SEQ = 0, TA_SYNTH = 1 };
+
+// Passes over whole lines of the g boxes (one thread per line): line op along dim i.
+enum MaskedOp { M_DEHIER, M_HIER, M_APPLY };
+__device__ void maskedPass(double* box0, int64_t boxStride, const uint8_t* st, const TrItem& it, int g, int i,
+                           MaskedOp op) {
+    const int n = it.n[i];
+    if (n > 1) {
+        const int S = it.S[i];
+        const int lines = it.total / n;
+        for (int l = threadIdx.x; l < g * lines; l += blockDim.x) {
+            const int j = l / lines;
+            const int ll = l - j * lines;
+            const int64_t base = (ll % S) + (int64_t)(ll / S) * S * n;
+            double* v = box0 + j * boxStride + base;
+            switch (op) {
+                case M_DEHIER: hierLineMasked(v, st + base, it.T[i], S, true); break;
+                case M_HIER: hierLineMasked(v, st + base, it.T[i], S, false); break;
+                case M_APPLY: applyLine(v, it.T[i], S); break;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// One workgroup per (depth, group of g instances).  TA_SEQ: one depth (`depth`) of
+// calcPrewByNodal; src holds the nodal input (first depth) or aliases hier, the running
+// hierarchical representation, which the kernel updates; out receives the prewavelets of
+// the depth's active points.  TA_SYNTH: every depth (blockIdx.x), src holds prewavelets,
+// the hierarchical representation of the depth's prewavelets goes to scratch.  The boxes
+// sit in LDS (useLds) or in the scratch (boxTot entries per instance).
+template <int MODE>
+__global__ void __launch_bounds__(256) tr_adapt_kernel(
+    const TrItem* __restrict__ items, int depth, const TrComp* __restrict__ comps, const int32_t* __restrict__ compOff,
+    const int32_t* __restrict__ gidx, const uint8_t* __restrict__ bstat, const double* __restrict__ afac,
+    int64_t afacTot, const double* src, double* hier, double* __restrict__ out, double* scratch, int64_t n,
+    int64_t boxTot, int G, int batch, int first, int useLds) {
+    extern __shared__ double lds[];
+    const int t = MODE == TA_SEQ ? depth : (int)blockIdx.x;
+    const TrItem it = items[t];
+    const int64_t b0 = (int64_t)blockIdx.y * G;
+    const int g = (int)min<int64_t>(G, batch - b0);
+    const int tot = it.total;
+    const int32_t* gi = gidx + it.box_off;
+    const uint8_t* st = bstat + it.box_off;
+    double* box0 = useLds ? lds : scratch + b0 * boxTot + it.box_off;
+    const int64_t bs = useLds ? tot : boxTot;
+    for (int p = threadIdx.x; p < g * tot; p += blockDim.x) {
+        const int j = p / tot;
+        const int q = p - j * tot;
+        const int32_t dv = gi[q];
+        double v = dv >= 0 ? src[(b0 + j) * n + dv] : 0.0;
+        // CalcHierbyPrew scatters the prewavelets of every stored point of the depth,
+        // hanging ones included (BasisTransformations.cpp:321-353)
+        if (MODE == TA_SYNTH && !ownDepth(it, q)) v = 0.0;
+        box0[j * bs + q] = v;
+    }
+    __syncthreads();
+    if (MODE == TA_SEQ) {
+        if (!first)
+            for (int i = 0; i < it.d; ++i) maskedPass(box0, bs, st, it, g, i, M_DEHIER);
+        // connected components, one after the other (their boxes may overlap)
+        for (int c = compOff[t]; c < compOff[t + 1]; ++c) {
+            const TrComp cc = comps[c];
+            for (int i = 0; i < it.d; ++i) {
+                int lines = 1;
+                for (int k = 0; k < it.d; ++k)
+                    if (k != i) lines *= cc.mx[k] - cc.mn[k] + 1;
+                const int S = it.S[i];
+                const int size = cc.size[i];
+                for (int l = threadIdx.x; l < g * lines; l += blockDim.x) {
+                    const int j = l / lines;
+                    int ll = l - j * lines;
+                    int64_t base = (int64_t)(cc.mn[i] - 1) * S;
+                    for (int k = 0; k < it.d; ++k) {
+                        if (k == i) continue;
+                        const int ext = cc.mx[k] - cc.mn[k] + 1;
+                        base += (int64_t)(cc.mn[k] + ll % ext - 1) * it.S[k];
+                        ll /= ext;
+                    }
+                    double* v = box0 + j * bs + base;
+                    solveBand(v, size, S, afac + cc.fac[i], afacTot);
+                    // only active entries take the solution
+                    for (int q = 0; q < size; ++q) {
+                        const uint8_t e = st[base + (int64_t)q * S];
+                        if (e == TR_ACTIVE) continue;
+                        v[q * S] = e == TR_MISSING ? 0.0 : src[(b0 + j) * n + gi[base + (int64_t)q * S]];
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // prewavelets of the depth's active points (those of its hanging points are 0: the
+        // output starts at 0 and only active points are written); everything else leaves
+        // the box
+        for (int p = threadIdx.x; p < g * tot; p += blockDim.x) {
+            const int j = p / tot;
+            const int q = p - j * tot;
+            double& v = box0[j * bs + q];
+            if (st[q] == TR_ACTIVE && ownDepth(it, q)) out[(b0 + j) * n + gi[q]] = v;
+            else v = 0.0;
+        }
+        __syncthreads();
+    }
+    // CalcHierbyPrew: the prewavelets as nodal values on stored active entries, then
+    // hierarchized over the active entries
+    for (int i = 0; i < it.d; ++i) maskedPass(box0, bs, st, it, g, i, M_APPLY);
+    for (int p = threadIdx.x; p < g * tot; p += blockDim.x) {
+        const int q = p % tot;
+        if (st[q] != TR_ACTIVE) box0[(p / tot) * bs + q] = 0.0;
+    }
+    __syncthreads();
+    for (int i = 0; i < it.d; ++i) maskedPass(box0, bs, st, it, g, i, M_HIER);
+    for (int p = threadIdx.x; p < g * tot; p += blockDim.x) {
+        const int j = p / tot;
+        const int q = p - j * tot;
+        if (MODE == TA_SEQ) {
+            if (st[q] == TR_ACTIVE) hier[(b0 + j) * n + gi[q]] -= box0[j * bs + q];
+        } else if (useLds) {
+            scratch[(b0 + j) * boxTot + it.box_off + q] = box0[j * bs + q];
+        }
+    }
+}
+
 // out[b] = sqrt(sum_i x_i y_i) for one instance per workgroup (fixed summation order)
 __global__ void __launch_bounds__(256) tr_energy_kernel(const double* __restrict__ x, const double* __restrict__ y,
                                                         int64_t n, double* __restrict__ out) {
@@ -314,6 +494,30 @@ void launch_tr_box(int mode, const TrItem* items, const int32_t* list, int nitem
                                                             batch);
             break;
     }
+}
+
+void launch_tr_adapt(int mode, const TrItem* items, int depth, int nitems, int64_t maxBox, const TrComp* comps,
+                     const int32_t* compOff, const int32_t* gidx, const uint8_t* bstat, const double* afac,
+                     int64_t afacTot, const double* src, double* hier, double* out, double* scratch, int64_t n,
+                     int64_t boxTot, int first, int64_t ldsLimit, int batch, hipStream_t s) {
+    if (nitems <= 0 || batch <= 0) return;
+    // instances per workgroup: up to 64 KB of LDS; one box per workgroup when it needs
+    // more, in the scratch when it needs more than ldsLimit
+    const int64_t boxBytes = std::max<int64_t>(maxBox, 1) * (int64_t)sizeof(double);
+    int G = (int)std::max<int64_t>(1, std::min<int64_t>({16, batch, 65536 / boxBytes}));
+    int useLds = 1;
+    if ((int64_t)G * boxBytes > ldsLimit) {
+        G = 1;
+        useLds = boxBytes <= ldsLimit ? 1 : 0;
+    }
+    const dim3 grid(nitems, (unsigned)((batch + G - 1) / G));
+    const size_t lds = useLds ? (size_t)G * boxBytes : 0;
+    if (mode == TA_SEQ)
+        tr_adapt_kernel<TA_SEQ><<<grid, 256, lds, s>>>(items, depth, comps, compOff, gidx, bstat, afac, afacTot, src,
+                                                       hier, out, scratch, n, boxTot, G, batch, first, useLds);
+    else
+        tr_adapt_kernel<TA_SYNTH><<<grid, 256, lds, s>>>(items, depth, comps, compOff, gidx, bstat, afac, afacTot, src,
+                                                         hier, out, scratch, n, boxTot, G, batch, first, useLds);
 }
 
 void launch_tr_pole(const int32_t* poles, const double* in, double* out, int64_t n, int batch, hipStream_t s) {

@@ -726,11 +726,14 @@ struct Oracle {
 struct Trafo {
     const Grid& g;
     const MultiLevel& ml;
-    std::vector<int> order;    // depths by |T|_1 descending: ListOfDepthOrderedSubgrids from
-                               // gotoEnd() through previous() (sgrid/ListOfDepthOrderedGrids.cc:77-110)
+    // depths in the order ListOfDepthOrderedSubgrids visits them from gotoEnd() through
+    // previous() (sgrid/ListOfDepthOrderedGrids.cc:77-137, 219-244): one list per |T|_1 in
+    // first-appearance order over the slots, walked from the largest |T|_1 down and each
+    // list from its back, so depths of equal |T|_1 come in reverse first-appearance order
+    std::vector<int> order;
     Trafo(const Grid& g_, const MultiLevel& ml_) : g(g_), ml(ml_) {
         order.resize(g.depths.size());
-        for (size_t t = 0; t < order.size(); ++t) order[t] = (int)t;
+        for (size_t t = 0; t < order.size(); ++t) order[t] = (int)(order.size() - 1 - t);
         std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return sum(g.depths[a]) > sum(g.depths[b]); });
     }
     int sum(const Depth& T) const { int s = 0; for (int i = 0; i < g.d; ++i) s += T[i]; return s; }

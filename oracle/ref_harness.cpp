@@ -16,6 +16,7 @@
 // Usage:
 //   ref_harness regular  <level> <coef> <outdir> [what...]
 //   ref_harness adaptive <startlevel> <radius> <maxlevel> <coef> <outdir> [what...]
+//   ref_harness adaptive2 <startlevel> <radius> <maxlevel> <coef> <outdir> [what...]
 //   ref_harness timemv   <level> <coef> <reps>            (timing sample, prints JSON)
 //   ref_harness timetr   <level>                          (calcPrewByNodal / calcNodalByPrew timing, JSON)
 //   ref_harness timesolve <level> <coef> [eps]           (the driver's whole PCG solve timed, JSON)
@@ -196,11 +197,14 @@ int main(int argc, char** argv) {
         runAll(grid, cs, out, what);
         return 0;
     }
-    if (mode == "adaptive" && argc >= 7) {
+    if ((mode == "adaptive" || mode == "adaptive2") && argc >= 7) {
         // Config-3 style locally adaptive grid: a regular start grid, then repeated
         // refinement (both sons in every dimension) of active points whose support
-        // box is closer than `radius` to the origin, while the son's sum of
-        // (depth_i - 1) <= maxlevel, then the reference's completeDirichletGrid().
+        // box is closer than `radius` to the origin (adaptive2: to the origin or to the
+        // opposite corner (1, ..., 1), two refined regions that do not touch), while the
+        // son's sum of (depth_i - 1) <= maxlevel, then the reference's
+        // completeDirichletGrid().
+        const bool twoCorners = mode == "adaptive2";
         int start = std::atoi(argv[2]);
         double radius = std::atof(argv[3]);
         int maxlevel = std::atoi(argv[4]);
@@ -218,11 +222,14 @@ int main(int argc, char** argv) {
                 IndexDimension I = grid.getIndexOfTable(i);
                 // distance from the origin to the point's support box
                 // [x_k - 2^-depth_k, x_k + 2^-depth_k]
-                double r2 = 0.0;
+                double r2 = 0.0, q2 = 0.0;
                 for (int k = 0; k < DimensionSparseGrid; ++k) {
                     double lo = I.coordinate(k) - 1.0 / double(1 << I.getDepth(k));
                     if (lo > 0.0) r2 += lo * lo;
+                    double hi = 1.0 - (I.coordinate(k) + 1.0 / double(1 << I.getDepth(k)));
+                    if (hi > 0.0) q2 += hi * hi;
                 }
+                if (twoCorners) r2 = std::min(r2, q2);
                 if (std::sqrt(r2) >= radius) continue;
                 for (int k = 0; k < DimensionSparseGrid; ++k) {
                     IndexDimension S[2] = {I.leftSon(k), I.rightSon(k)};

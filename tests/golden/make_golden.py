@@ -49,6 +49,10 @@ CASES = {
     "d3_l4_gen": (3, ["regular", "4", "gen"]),
     "d5_l3_gen": (5, ["regular", "3", "gen"]),
     "d3_adaptive_gen": (3, ["adaptive", "3", "0.15", "6", "gen"]),
+    # refined near the origin and near (1, ..., 1): the finest depths have two separate
+    # components (prewavelet line solves per component)
+    "d2_adaptive2_cfg2": (2, ["adaptive2", "3", "0.2", "7", "cfg2"]),
+    "d3_adaptive2_cfg3": (3, ["adaptive2", "3", "0.15", "6", "cfg3"]),
 }
 
 
@@ -91,7 +95,7 @@ def main():
         if args[0] == "regular":
             cmd = [exe, "regular", args[1], args[2], out]
         else:
-            cmd = [exe, "adaptive", args[1], args[2], args[3], args[4], out]
+            cmd = [exe, args[0], args[1], args[2], args[3], args[4], out]
         print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True, env=env)
     with open(os.path.join(HERE, "cases.json"), "w") as f:

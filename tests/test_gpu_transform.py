@@ -1,8 +1,10 @@
 """GPU parity of the basis transformations and energy norms (through the C ABI):
 lsg_hierarchize / lsg_dehierarchize / lsg_prew_by_nodal / lsg_nodal_by_prew /
 lsg_energy_norm against the compiled reference's golden vectors
-(CalcHierarchicalBasis, calcPrewByNodal, calcNodalByPrew, H1/L2::getValue), against
-the CPU oracle up to d=6, and through size-independent properties at d=5 level 7.
+(CalcHierarchicalBasis, calcPrewByNodal, calcNodalByPrew, H1/L2::getValue) on regular
+and adaptive grids (hanging nodes; one or two connected components per depth), against
+the CPU oracle up to d=6 and on the config-3 refined grids up to level 10, and through
+size-independent properties at d=5 level 7 and on the level-11 refined grid.
 
 Tolerances: 1e-12 relative (max-norm) against the reference and the oracle (the
 device sums the same terms in another order and solves the line systems by an
@@ -24,6 +26,8 @@ pytestmark = pytest.mark.gpu
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = json.load(open(os.path.join(GOLD, "cases.json")))
 REGULAR = sorted(c for c in CASES if CASES[c]["argv"][0] == "regular")
+ADAPTIVE = sorted(c for c in CASES if CASES[c]["argv"][0].startswith("adaptive"))
+GRIDS = os.path.join(GOLD, "grids")
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -49,7 +53,7 @@ def coords(idx):
     return (2 * (idx - 2 ** depth) + 1) / 2.0 ** depth
 
 
-@pytest.mark.parametrize("case", REGULAR)
+@pytest.mark.parametrize("case", REGULAR + ADAPTIVE)
 def test_transforms_match_reference_golden(case):
     gv = load(case)
     print("transform golden", case, flush=True)
@@ -60,7 +64,7 @@ def test_transforms_match_reference_golden(case):
     assert relmax(L.calcPrewByNodal(u, op), gv["tr_prew"]) < 1e-12
     assert relmax(L.calcNodalByPrew(gv["x"], op), gv["tr_nodal_of_x"]) < 1e-12
     d = CASES[case]["d"]
-    f = d * math.pi ** 2 * np.prod(np.sin(math.pi * coords(gv["idx"])), axis=1)
+    f = d * math.pi ** 2 * np.prod(np.sin(math.pi * coords(gv["idx"])), axis=1) * gv["active"]
     assert relmax(L.calcPrewByNodal(f, op), gv["f_prew"]) < 1e-12
     h1 = L.H1(grid).getValue(u)
     l2 = L.L2(grid).getValue(u)
@@ -82,6 +86,81 @@ def test_transforms_match_oracle(d, level):
     assert relmax(op.dehierarchize(u), o.transform(o.DEHIER, u)) < 1e-12
     assert relmax(op.prew_by_nodal(u), o.transform(o.PREW_BY_NODAL, u)) < 1e-12
     assert relmax(op.nodal_by_prew(p), o.transform(o.NODAL_BY_PREW, p)) < 1e-12
+
+
+def refined_grid(level):
+    z = np.load(os.path.join(GRIDS, "cfg3_start4_level%d.npz" % level))
+    return z["idx"], z["active"]
+
+
+@pytest.mark.parametrize("level", [8, 10])
+def test_adaptive_transforms_match_oracle(level):
+    """Config-3 refined grids (start level 4, refinement near the origin): every
+    transform against the oracle for random values at every point, hanging ones
+    included (they are read as poles and line values but never written)."""
+    print("adaptive transform oracle level %d" % level, flush=True)
+    idx, act = refined_grid(level)
+    o = Oracle(idx, act)
+    grid = L.AdaptiveSparseGrid(idx, act)
+    op = L.MatrixVectorHomogen(grid, L.Poisson(), 64)
+    rng = np.random.default_rng(level)
+    u = rng.standard_normal(idx.shape[0])
+    p = rng.standard_normal(idx.shape[0])
+    assert relmax(op.hierarchize(u), o.transform(o.HIER, u)) < 1e-12
+    assert relmax(op.dehierarchize(u), o.transform(o.DEHIER, u)) < 1e-12
+    assert relmax(op.prew_by_nodal(u), o.transform(o.PREW_BY_NODAL, u)) < 1e-12
+    assert relmax(op.nodal_by_prew(p), o.transform(o.NODAL_BY_PREW, p)) < 1e-12
+
+
+def test_adaptive_batch_and_global_boxes(monkeypatch):
+    """Two-component grid: a batch of three instances (boxes of several instances share a
+    workgroup) gives each single instance's result bit for bit, and so do boxes staged in
+    global memory instead of LDS (LSG_TR_LDS_BYTES=0)."""
+    import torch
+    print("adaptive transform batch / global boxes", flush=True)
+    gv = load("d3_adaptive2_cfg3")
+    grid = L.AdaptiveSparseGrid(gv["idx"], gv["active"])
+    n = grid.size
+    rng = np.random.default_rng(7)
+    U = rng.standard_normal((3, n))
+    op = L.MatrixVectorHomogen(grid, L.Poisson(), 64)
+    singles = [(op.prew_by_nodal(U[b]), op.nodal_by_prew(U[b])) for b in range(3)]
+    opb = L.MatrixVectorHomogen(grid, L.Poisson(), 64, batch=3)
+    dev = grid.device_order()
+    Ud = np.zeros_like(U)
+    Ud[:, dev] = U
+    tin = torch.tensor(Ud, dtype=torch.float64, device="cuda")
+    for lds in (None, "0"):
+        if lds is not None:
+            monkeypatch.setenv("LSG_TR_LDS_BYTES", lds)
+        tp = torch.zeros_like(tin)
+        tn = torch.zeros_like(tin)
+        opb.prew_by_nodal(tin, tp)
+        opb.nodal_by_prew(tin, tn)
+        P = tp.cpu().numpy()[:, dev]
+        N = tn.cpu().numpy()[:, dev]
+        for b in range(3):
+            assert np.array_equal(P[b], singles[b][0]), (lds, b)
+            assert np.array_equal(N[b], singles[b][1]), (lds, b)
+
+
+def test_adaptive_level11_round_trip():
+    """Refined grid of level 11 (N = 65,943; boxes up to 14,415 values, beyond 64 KB of
+    LDS): nodal -> prewavelets -> nodal is the identity on the active points, hanging
+    points are never written, and H1 = sqrt(p^T A p)."""
+    print("adaptive transform level 11 round trip", flush=True)
+    idx, act = refined_grid(11)
+    grid = L.AdaptiveSparseGrid(idx, act)
+    op = L.MatrixVectorHomogen(grid, L.Poisson(), 64)
+    a = act.astype(bool)
+    u = np.random.default_rng(11).standard_normal(grid.size) * a
+    p = op.prew_by_nodal(u)
+    assert not p[~a].any()
+    back = op.nodal_by_prew(p)
+    assert np.abs(back - u)[a].max() < 1e-12 * np.abs(u).max()
+    assert not back[~a].any()
+    Ap = op.multiplication(p)
+    assert abs(op.energy_norm(u) - math.sqrt(p @ Ap)) < 1e-12 * math.sqrt(p @ Ap)
 
 
 def test_full_size_round_trips_and_batch():
@@ -183,11 +262,6 @@ def test_vector_norms(where, prec):
 
 def test_unsupported_inputs_raise():
     print("transform errors", flush=True)
-    gv = load("d3_adaptive_cfg3")
-    grid = L.AdaptiveSparseGrid(gv["idx"], gv["active"])
-    op = L.MatrixVectorHomogen(grid, L.Poisson(), 64)
-    with pytest.raises(L.LoadsgError):
-        op.prew_by_nodal(np.ones(grid.size))
     reg = L.AdaptiveSparseGrid.regular(3, 4)
     op32 = L.MatrixVectorHomogen(reg, L.Poisson(), 32)
     with pytest.raises(L.LoadsgError):

@@ -27,7 +27,7 @@ from test_oracle import interp_to, kron_all, load, prewavelet_nodal_1d, relmax
 GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = json.load(open(os.path.join(GOLD, "cases.json")))
 REGULAR = sorted(c for c in CASES if CASES[c]["argv"][0] == "regular")
-ADAPTIVE = sorted(c for c in CASES if CASES[c]["argv"][0] == "adaptive")
+ADAPTIVE = sorted(c for c in CASES if CASES[c]["argv"][0].startswith("adaptive"))
 
 
 def coords(idx):
