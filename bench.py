@@ -19,8 +19,11 @@ its own metric; the default line above is the headline):
         FP64 PCG to 1e-10 (one instance, and matvecs at batch 64), beside the reference
         CPU matvec sampled on this host;
   cfg3  the locally adaptive d=3 grid (start level 4, refined towards the origin to sum
-        level 11, tests/golden/grids), a = 1 + |x|^2: FP32 PCG under FP64 refinement to
-        1e-10 against the FP64 solve (loadsg_amd/measure_adaptive.py);
+        level 11, tests/golden/grids), a = 1 + |x|^2: the manufactured solution
+        u = |x|^(2/3) prod x_i (1 - x_i) with the reference driver's right-hand side
+        f_h = M prew(f_I), solved in FP64 and mixed precision to 1e-10 with its
+        discretisation errors, and batched FP32 PCG under FP64 refinement to 1e-10 on
+        synthetic right-hand sides against the FP64 solve (loadsg_amd/measure_adaptive.py);
   cfg4  64 instances with d uniform in {3..6} and level in {5..8} (seeded), exp-sine
         coefficients (16 xi), 8-term sine right-hand sides, solved together to 1e-8
         (lsg_ir_solve_many: one host thread advancing every grid shape's solve, the
@@ -287,7 +290,8 @@ def workload_cfg3(args):
     r = run(os.path.join(ROOT, "tests", "golden", "grids", "cfg3_start4_level11.npz"), batch=64, reps=10, tol=1e-10)
     out = {"metric": "solves/s to 1e-10 rel. residual, locally adaptive d=3 grid (FP32 PCG under FP64 IR)",
            "value": r["mixed_batch_solves_per_s"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
-           "dtype": "f32 inner / f64 refinement", "data": "reference-generated grid, synthetic right-hand sides",
+           "dtype": "f32 inner / f64 refinement",
+           "data": "reference-generated grid; manufactured and synthetic right-hand sides",
            "config": {"workload": r["workload"], "points": r["points"], "hanging": r["hanging"], "batch": 64},
            "extra": r}
     print(json.dumps(out))

@@ -23,11 +23,18 @@ the grid points uses calcNodalByPrew (BasisTransformations.cpp:670-707), the qua
 reference's driver prints as L_infty (examples_doxygen/poisson_regular.cpp:130-150).
 The reference's own right-hand side convention f_h = M prew(f_I) (poisson_regular.cpp:
 57-113) is available as reference_rhs().
+
+Solutions that are not sums of products (config 3: u = |x|^(2/3) prod x_i (1 - x_i),
+a = 1 + |x|^2 on a grid refined towards the origin, where grad u is singular) are given
+pointwise (PointwiseProblem).  Their right-hand side is the reference driver's
+f_h = M prew(f_I), and their errors are the ones its drivers print: the largest nodal
+error over the active points and the L2 / H1 norms (L2::getValue, H1::getValue,
+applications/H1_Norm.cpp:8-78) of the sparse-grid interpolant of the nodal error.
 """
 import numpy as np
 
-from . import (CG, CG_NO_PRECOND, COEF_CFG2, STOP_REL, HelmHoltz, MatrixVectorHomogen, Poisson,
-               VariableCoefficient)
+from . import (CG, CG_NO_PRECOND, COEF_CFG2, COEF_CFG3, H1, L2, STOP_REL, HelmHoltz, MatrixVectorHomogen,
+               Poisson, VariableCoefficient)
 
 _GX, _GW = np.polynomial.legendre.leggauss(40)
 
@@ -182,3 +189,68 @@ def cfg2(d=4):
     alpha = [[const1(1.0)] * d, [cos1] + [cosj] * (d - 1)]
     gamma = [[const1(1.0)] * d, [xfn] + [const1(1.0)] * (d - 1)]
     return Problem("cfg2", d, [v1] + [vj] * (d - 1), alpha, gamma, VariableCoefficient(COEF_CFG2))
+
+
+class PointwiseProblem:
+    """-div(a grad u) + c u = f with u and f given pointwise on points x (n x d)."""
+
+    def __init__(self, name, d, u, f, stencil):
+        self.name, self.d, self.u, self.f, self.stencil = name, d, u, f, stencil
+
+    def reference_rhs(self, grid, op_fp64):
+        """f_h = M prew(f_I): f at the active points, calcPrewByNodal, the mass operator
+        (host slot order)."""
+        _, act = grid.slot_order()
+        fI = self.f(coordinates(grid)) * act
+        mass = MatrixVectorHomogen(grid, HelmHoltz())
+        return mass.multiplication(op_fp64.prew_by_nodal(fI))
+
+    def nodal_error(self, grid, x, op_fp64):
+        """u(x_p) - u_h(x_p) at the active points, 0 at the hanging ones (slot order)."""
+        _, act = grid.slot_order()
+        return (self.u(coordinates(grid)) - op_fp64.nodal_by_prew(x)) * act
+
+    def errors(self, grid, x, op_fp64):
+        e = self.nodal_error(grid, x, op_fp64)
+        return {"nodal_max": float(np.abs(e).max()), "l2": float(L2(grid).getValue(e)),
+                "h1": float(H1(grid).getValue(e))}
+
+    def solve(self, grid, tol=1e-10, maxit=5000):
+        """FP64 PCG on the reference right-hand side; returns (x, f, info, op)."""
+        op = MatrixVectorHomogen(grid, self.stencil)
+        f = self.reference_rhs(grid, op)
+        x = np.zeros(grid.size)
+        info = CG.solveHomogen(tol, x, f, op, stop_rule=STOP_REL, maxit=maxit)
+        return x, f, info, op
+
+
+def _cfg3_parts(x):
+    """r, g = r^(2/3), grad g, lap g, P = prod x_i (1 - x_i), grad P, lap P."""
+    x = np.atleast_2d(np.asarray(x, dtype=float))
+    d = x.shape[1]
+    r = np.sqrt(np.sum(x * x, axis=1))
+    g = r ** (2.0 / 3.0)
+    grad_g = (2.0 / 3.0) * r[:, None] ** (-4.0 / 3.0) * x
+    lap_g = (2.0 / 3.0) * (2.0 / 3.0 + d - 2.0) * r ** (-4.0 / 3.0)
+    q = x * (1.0 - x)
+    P = np.prod(q, axis=1)
+    others = np.stack([np.prod(np.delete(q, i, axis=1), axis=1) for i in range(d)], axis=1)
+    grad_P = (1.0 - 2.0 * x) * others
+    lap_P = -2.0 * np.sum(others, axis=1)
+    return x, r, g, grad_g, lap_g, P, grad_P, lap_P
+
+
+def cfg3(d=3):
+    """-div(a grad u) = f on (0,1)^d, a = 1 + |x|^2, u = |x|^(2/3) prod x_i (1 - x_i):
+    f = -(grad a . grad u + a lap u) with grad a = 2x, lap |x|^a = a (a + d - 2) |x|^(a-2)."""
+    def u(x):
+        _, _, g, _, _, P, _, _ = _cfg3_parts(x)
+        return g * P
+
+    def f(x):
+        x, r, g, grad_g, lap_g, P, grad_P, lap_P = _cfg3_parts(x)
+        grad_u = P[:, None] * grad_g + g[:, None] * grad_P
+        lap_u = P * lap_g + 2.0 * np.sum(grad_g * grad_P, axis=1) + g * lap_P
+        return -(np.sum(2.0 * x * grad_u, axis=1) + (1.0 + r * r) * lap_u)
+
+    return PointwiseProblem("cfg3", d, u, f, VariableCoefficient(COEF_CFG3))

@@ -9,6 +9,12 @@ f = sum_j eta_j prod_i sin(m_j pi x_i), eta ~ N(0, 1), m = (1, 2), one per insta
 
 Measured, with the inputs resident on the device:
   * FP64 and FP32 matvecs at batch 1 and at --batch (HIP events around the launches);
+  * the manufactured solution u = |x|^(2/3) prod x_i (1 - x_i) (loadsg_amd.manufactured.cfg3):
+    the reference driver's right-hand side f_h = M prew(f_I) (calcPrewByNodal on the grid's
+    hanging-node structure, then the mass operator), the plain FP64 PCG and the
+    mixed-precision solve of it to 1e-10, the error of the mixed solution against the FP64
+    one and the discretisation errors of both (largest nodal error, L2 and H1 norms of the
+    nodal error, applications/H1_Norm.cpp);
   * the plain FP64 PCG to 1e-10 relative residual at batch 1;
   * the mixed-precision solve (FP32 inner PCG, FP64 refinement) to 1e-10 at batch 1
     and at --batch, with its error against the FP64 solve of the same systems.
@@ -76,6 +82,9 @@ def run(grid_path, batch=64, reps=20, tol=1e-10, matvec_only=False):
         return (L.MatrixVectorHomogen(grid, coef, precision=64, batch=B),
                 L.MatrixVectorHomogen(grid, coef, precision=32, batch=B))
 
+    out["manufactured"] = manufactured(grid, tol)
+    print("manufactured solution done", flush=True, file=sys.stderr)
+
     # plain FP64 PCG, batch 1
     op64, op32 = ops(1)
     f1 = torch.zeros(1, N, dtype=torch.float64, device=dev)
@@ -125,6 +134,63 @@ def run(grid_path, batch=64, reps=20, tol=1e-10, matvec_only=False):
                mixed_batch_max_err_vs_fp64=float(err.max()))
     out["speedup_mixed_batch_vs_fp64_b1"] = out["mixed_batch_solves_per_s"] / out["fp64_cg_b1_solves_per_s"]
     return out
+
+
+def manufactured(grid, tol):
+    """Config 3's manufactured solution on `grid`: right-hand side, FP64 and mixed solves,
+    errors (see the module docstring)."""
+    import torch
+    import loadsg_amd as L
+    from loadsg_amd.manufactured import cfg3, coordinates
+    P = cfg3()
+    r = {"u": "|x|^(2/3) prod x_i (1 - x_i)", "a": "1 + |x|^2", "rhs": "f_h = M prew(f_I)"}
+    op64 = L.MatrixVectorHomogen(grid, P.stencil, precision=64)
+    op32 = L.MatrixVectorHomogen(grid, P.stencil, precision=32)
+    t0 = time.perf_counter()
+    f = P.reference_rhs(grid, op64)          # first call: transform tables and the mass operator
+    r["rhs_first_call_s"] = time.perf_counter() - t0
+    # device-resident right-hand side construction: prewavelets of f_I, then M
+    _, act = grid.slot_order()
+    dev = grid.device_order()
+    fI = np.zeros(grid.size)
+    fI[dev] = P.f(coordinates(grid)) * act
+    mass = L.MatrixVectorHomogen(grid, L.HelmHoltz(), precision=64)
+    tin = torch.tensor(fI[None, :], device="cuda")
+    tp, tf = torch.zeros_like(tin), torch.zeros_like(tin)
+    op64.prew_by_nodal(tin, tp)
+    mass.multiplication(tp, tf)
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    reps = 5
+    ev[0].record()
+    for _ in range(reps):
+        op64.prew_by_nodal(tin, tp)
+    ev[1].record()
+    for _ in range(reps):
+        mass.multiplication(tp, tf)
+    ev[2].record()
+    torch.cuda.synchronize()
+    r["prew_by_nodal_ms"] = ev[0].elapsed_time(ev[1]) / reps
+    r["mass_matvec_ms"] = ev[1].elapsed_time(ev[2]) / reps
+    assert np.allclose(tf.cpu().numpy()[0][dev], f, rtol=0, atol=1e-12 * np.abs(f).max())
+    # FP64 PCG and the mixed solve of f_h
+    x = np.zeros(grid.size)
+    L.CG.solveHomogen(tol, x.copy(), f, op64, stop_rule=L.STOP_REL, maxit=5000)   # warm-up
+    t0 = time.perf_counter()
+    info = L.CG.solveHomogen(tol, x, f, op64, stop_rule=L.STOP_REL, maxit=5000)
+    r["fp64_cg_s"] = time.perf_counter() - t0
+    r["fp64_cg_iterations"] = info["iterations"]
+    xm = np.zeros(grid.size)
+    L.CG.solveMixed(tol, xm.copy(), f, op64, op32)
+    t0 = time.perf_counter()
+    im = L.CG.solveMixed(tol, xm, f, op64, op32)
+    r["mixed_s"] = time.perf_counter() - t0
+    r["mixed_inner_iterations"] = im["iterations"]
+    r["mixed_rel_residual"] = im["rel_residual"]
+    r["mixed_err_vs_fp64"] = float(np.linalg.norm(xm - x) / np.linalg.norm(x))
+    r["errors_fp64"] = P.errors(grid, x, op64)
+    r["errors_mixed"] = P.errors(grid, xm, op64)
+    return r
 
 
 def main():

@@ -62,3 +62,25 @@ def test_separable_integrals_match_tensor_quadrature(name, mk, a, c):
     l2 = np.dot(wts, u * u)
     assert abs(p.energy_sq() - energy) < 1e-7 * energy
     assert abs(p.l2_sq() - l2) < 1e-10 * l2
+
+
+@pytest.mark.parametrize("d", [2, 3])
+def test_cfg3_rhs_matches_finite_differences(d):
+    """Config 3 (a = 1 + |x|^2, u = |x|^(2/3) prod x_i (1 - x_i)): f against central
+    differences of the flux a grad u at random interior points (O(h^2) = 1e-8); a dropped
+    grad a . grad u term, a wrong Laplacian of |x|^(2/3) (its d-dependence) or a sign
+    fails this."""
+    P = M.cfg3(d)
+    x = np.random.default_rng(d).uniform(0.05, 0.95, (30, d))
+    h = 1e-4
+
+    def a(y):
+        return 1.0 + np.sum(y * y, axis=1)
+
+    div = np.zeros(len(x))
+    for i in range(d):
+        e = np.zeros(d)
+        e[i] = h
+        div += (a(x + e / 2) * (P.u(x + e) - P.u(x)) - a(x - e / 2) * (P.u(x) - P.u(x - e))) / h ** 2
+    f = P.f(x)
+    assert np.max(np.abs(-div - f) / np.abs(f)) < 1e-6
