@@ -102,6 +102,34 @@ int lsg_grid_dim(const lsg_grid* g);
 int lsg_grid_slot_order(const lsg_grid* g, int32_t* idx_out, uint8_t* active_out);
 /* dev_of_slot[i] = position of slot i in the device (depth-blocked) order. */
 int lsg_grid_device_order(const lsg_grid* g, int64_t* dev_of_slot);
+/* ------------------------------------------------------- grid generation */
+/* A growing set of interior points with the reference's AdaptiveSparseGrid generation
+ * calls (host code, no GPU).  Points are 1D index tuples as above (no boundary points:
+ * LSG_ERR_ARG); slots are handed out in insertion order, as the reference's data table
+ * does, so the slot order of the result equals the reference's. */
+typedef struct lsg_gridgen lsg_gridgen;
+lsg_gridgen* lsg_gridgen_create(int d);
+void lsg_gridgen_destroy(lsg_gridgen* g);
+/* AdaptiveSparseGrid::AddPoint(I) (sgrid/sparseGrid.cc:482-517): added active, or made
+ * active when already stored. */
+int lsg_gridgen_add_point(lsg_gridgen* g, const int32_t* idx);
+/* AdaptiveSparseGrid::AddRecursiveSonsOfPoint(I, level) (sgrid/sparseGrid.cc:557-566). */
+int lsg_gridgen_add_recursive_sons(lsg_gridgen* g, const int32_t* idx, int level);
+/* Refinement towards the origin (corners = 1) or towards the origin and (1, ..., 1)
+ * (corners = 2): every active point whose support box [x - 2^-t, x + 2^-t] is closer than
+ * `radius` to the corner gets both sons in every dimension whose sum of (depth_i - 1) is
+ * at most max_level, repeated until nothing changes; the locally adaptive benchmark grids
+ * are this refinement of a regular grid followed by lsg_gridgen_complete_dirichlet.  No
+ * counterpart in the reference library (its drivers refine with their own loops). */
+int lsg_gridgen_refine_towards(lsg_gridgen* g, double radius, int max_level, int corners);
+/* AdaptiveSparseGrid::completeDirichletGrid (sgrid/sparseGrid.cc:771-787):
+ * completeGridWithoutBoundary, CompleteToLocalTensorProductGrid
+ * (sgrid/localTensorProduct.cc:189-213), completeGridWithoutBoundary, addHangingNodes
+ * (sparseGrid.cc:587-621), addPoints2 (sgrid/GridGeneration.cpp:333-378). */
+int lsg_gridgen_complete_dirichlet(lsg_gridgen* g);
+int64_t lsg_gridgen_size(const lsg_gridgen* g);
+/* Index table (size x d) and active flags in slot order, ready for lsg_grid_create. */
+int lsg_gridgen_points(const lsg_gridgen* g, int32_t* idx_out, uint8_t* active_out);
 /* Structural statistics: out[0]=#depths, [1]=multilevel size N_ML, [2]=#apply items,
  * [3]=#wavefront levels, [4]=algorithmic values moved per matvec (streaming model),
  * [5]=largest depth box, [6]=case-buffer size, [7]=W-buffer size, [8]=part-buffer size,

@@ -109,6 +109,14 @@ def lib():
             "lsg_mlp_get_grads": (i32, [vp, vp]),
             "lsg_mlp_mix_rows": (i32, [vp, vp, i64, i32, vp, vp, i32, vp]),
             "lsg_mlp_destroy": (i32, [vp]),
+            "lsg_gridgen_create": (vp, [i32]),
+            "lsg_gridgen_destroy": (None, [vp]),
+            "lsg_gridgen_add_point": (i32, [vp, vp]),
+            "lsg_gridgen_add_recursive_sons": (i32, [vp, vp, i32]),
+            "lsg_gridgen_refine_towards": (i32, [vp, dbl, i32, i32]),
+            "lsg_gridgen_complete_dirichlet": (i32, [vp]),
+            "lsg_gridgen_size": (i64, [vp]),
+            "lsg_gridgen_points": (i32, [vp, vp, vp]),
             "lsg_last_error": (ctypes.c_char_p, []),
             "lsg_device_available": (i32, []),
         }
@@ -139,6 +147,68 @@ def _sync_torch(*ts):
             import torch
             torch.cuda.synchronize(t.device)
             return
+
+
+class GridGenerator:
+    """The reference's grid generation calls on a growing point set (host code):
+    AddPoint, AddRecursiveSonsOfPoint, completeDirichletGrid (sgrid/sparseGrid.cc:482-787),
+    plus the refinement towards a corner of the locally adaptive benchmark grids.  Points
+    are tuples of 1D indices (2 = 0.5, sons 2j and 2j+1); slots follow insertion order as
+    in the reference's data table."""
+
+    def __init__(self, d):
+        self.d = d
+        self._h = lib().lsg_gridgen_create(d)
+        if not self._h:
+            raise LoadsgError("bad dimension %d" % d)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.lsg_gridgen_destroy(self._h)
+            self._h = None
+
+    def _gen(self, rc, what):
+        if rc != 0:
+            raise LoadsgError("loadsg_amd error %d: %s" % (rc, what))
+
+    def AddPoint(self, index):
+        v = np.ascontiguousarray(index, dtype=np.int32)
+        self._gen(lib().lsg_gridgen_add_point(self._h, _ptr(v)), "AddPoint: boundary point or depth above 15")
+
+    def AddRecursiveSonsOfPoint(self, index, level):
+        v = np.ascontiguousarray(index, dtype=np.int32)
+        self._gen(lib().lsg_gridgen_add_recursive_sons(self._h, _ptr(v), level),
+                  "AddRecursiveSonsOfPoint: boundary point or depth above 15")
+
+    def refine_towards(self, radius, max_level, corners=1):
+        self._gen(lib().lsg_gridgen_refine_towards(self._h, radius, max_level, corners), "refine_towards: bad argument")
+
+    def completeDirichletGrid(self):
+        self._gen(lib().lsg_gridgen_complete_dirichlet(self._h), "completeDirichletGrid")
+
+    @property
+    def size(self):
+        return lib().lsg_gridgen_size(self._h)
+
+    def points(self):
+        """(idx (size x d int32), active (size uint8)) in slot order."""
+        idx = np.zeros((self.size, self.d), dtype=np.int32)
+        act = np.zeros(self.size, dtype=np.uint8)
+        self._gen(lib().lsg_gridgen_points(self._h, _ptr(idx), _ptr(act)), "points")
+        return idx, act
+
+    def grid(self):
+        return AdaptiveSparseGrid(*self.points())
+
+    @classmethod
+    def refined(cls, d, start_level, radius, max_level, corners=1):
+        """A regular start grid refined towards the corner(s), then completeDirichletGrid:
+        the locally adaptive benchmark grids (start level 4, radius 0.15, d = 3)."""
+        g = cls(d)
+        g.AddRecursiveSonsOfPoint([2] * d, start_level)
+        g.refine_towards(radius, max_level, corners)
+        g.completeDirichletGrid()
+        return g
 
 
 class AdaptiveSparseGrid:

@@ -19,7 +19,7 @@ ARCH = "gfx950"
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-result"]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
-SOURCES = ["lsg_plan.cpp", "lsg_kernels.hip", "lsg_sparse.hip", "lsg_api.hip", "lsg_mlp.hip", "lsg_transform.hip"]
+SOURCES = ["lsg_plan.cpp", "lsg_gridgen.cpp", "lsg_kernels.hip", "lsg_sparse.hip", "lsg_api.hip", "lsg_mlp.hip", "lsg_transform.hip"]
 
 
 def _compile(src):

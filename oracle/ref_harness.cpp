@@ -17,6 +17,7 @@
 //   ref_harness regular  <level> <coef> <outdir> [what...]
 //   ref_harness adaptive <startlevel> <radius> <maxlevel> <coef> <outdir> [what...]
 //   ref_harness adaptive2 <startlevel> <radius> <maxlevel> <coef> <outdir> [what...]
+//   ref_harness points <pointfile> <outdir> [what...]
 //   ref_harness timemv   <level> <coef> <reps>            (timing sample, prints JSON)
 //   ref_harness timetr   <level>                          (calcPrewByNodal / calcNodalByPrew timing, JSON)
 //   ref_harness timesolve <level> <coef> [eps]           (the driver's whole PCG solve timed, JSON)
@@ -26,6 +27,7 @@
 #include "source/sparseEXPDE.h"
 #include "ref_plugin.hpp"
 
+#include <fstream>
 #include <string>
 #include <vector>
 #include <cstdio>
@@ -244,6 +246,29 @@ int main(int argc, char** argv) {
             }
         }
         grid.completeDirichletGrid();
+        runAll(grid, cs, out, what);
+        return 0;
+    }
+    if (mode == "points" && argc >= 4) {
+        // AddPoint for every point of a text file (d 1D indices per line, in order), then
+        // completeDirichletGrid(); writes the grid (and the numerics named after outdir).
+        std::ifstream in(argv[2]);
+        std::string out = argv[3];
+        std::vector<std::string> what(argv + 4, argv + argc);
+        AdaptiveSparseGrid grid;
+        for (;;) {
+            IndexDimension I;
+            bool ok = true;
+            for (int k = 0; k < DimensionSparseGrid && ok; ++k) {
+                long v;
+                if (!(in >> v)) ok = false;
+                else I.replace(k, (indexInteger)v);
+            }
+            if (!ok) break;
+            grid.AddPoint(I);
+        }
+        grid.completeDirichletGrid();
+        CoefSpec cs = parseCoef("const:1:0");
         runAll(grid, cs, out, what);
         return 0;
     }

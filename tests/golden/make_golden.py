@@ -21,7 +21,13 @@ origin, sum level L) followed by completeDirichletGrid, for the throughput measu
 and the larger sparse-path tests.  The generator's time grows about 3.5x per level
 (level 10: about 7 minutes on one core).
 
-Run:  python tests/golden/make_golden.py [--grids 7,8,9,10]
+It also writes tests/golden/gridgen_sets.npz (--gridgen): seeded random sets of interior
+points in d = 2, 3, each added with AddPoint and completed with completeDirichletGrid by
+the harness's `points` mode -- the inputs (pts<k>) and the reference's grids (idx<k>,
+active<k>).  Most of these sets make addPoints2 add hanging nodes, which the refined
+benchmark grids never do.
+
+Run:  python tests/golden/make_golden.py [--grids 7,8,9,10] [--gridgen]
 """
 import argparse
 import json
@@ -71,13 +77,44 @@ def make_grids(levels):
         np.savez_compressed(os.path.join(HERE, "grids", "cfg3_start4_level%d.npz" % lev), idx=idx, active=act)
 
 
+def make_gridgen(nsets=16, seed=20261204):
+    rng = np.random.default_rng(seed)
+    out = {}
+    for k in range(nsets):
+        d = int(rng.integers(2, 4))
+        pts = []
+        for _ in range(int(rng.integers(2, 12))):
+            q = []
+            for i in range(d):
+                t = int(rng.integers(1, 7))
+                q.append((1 << t) + int(rng.integers(0, 1 << (t - 1))))
+            pts.append(q)
+        exe = os.path.join(REF, "ref_harness_d%d" % d)
+        if not os.path.exists(exe):
+            sys.exit("missing %s: run `make -C oracle ref`" % exe)
+        with tempfile.TemporaryDirectory() as tmp:
+            f = os.path.join(tmp, "pts.txt")
+            with open(f, "w") as fh:
+                fh.write("\n".join(" ".join(map(str, q)) for q in pts) + "\n")
+            subprocess.run([exe, "points", f, tmp, "none"], check=True, env=dict(os.environ, OMP_NUM_THREADS="1"))
+            out["pts%d" % k] = np.array(pts, dtype=np.int32)
+            out["idx%d" % k] = np.load(os.path.join(tmp, "idx.npy")).astype(np.int32)
+            out["active%d" % k] = np.load(os.path.join(tmp, "active.npy")).astype(np.uint8)
+        print("set %d: d=%d, %d points -> %d" % (k, d, len(pts), len(out["idx%d" % k])), flush=True)
+    np.savez_compressed(os.path.join(HERE, "gridgen_sets.npz"), **out)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--grids", default="", help="comma-separated sum levels of the refined grids to write")
     ap.add_argument("--only", default="", help="comma-separated case names to (re)write; default all")
+    ap.add_argument("--gridgen", action="store_true", help="write gridgen_sets.npz only")
     args = ap.parse_args()
     if args.grids:
         make_grids([int(v) for v in args.grids.split(",")])
+        return
+    if args.gridgen:
+        make_gridgen()
         return
     only = set(filter(None, args.only.split(",")))
     meta = {}
