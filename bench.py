@@ -3,13 +3,18 @@
 (16 xi ~ N(0, 0.25)) and right-hand sides f = sum_j eta_j prod_i sin(m_j pi x_i)
 (8 eta ~ N(0,1), m_j in {1,2,3}); synthetic seeded inputs (PCG64 seed 20261203).
 
-One step = one batched solve of --batch instances: the learned warm start x0 = MLP(xi,
-eta) (inference inside the timed step; training on GPU-generated solves happens before,
-untimed, and is reported), then FP32 inner PCG under FP64 iterative refinement, every
-instance to ||f - A x||_2 <= 1e-8 ||f||_2 (checked in FP64).  The zero-start solve is
-reported beside it.  Inputs are resident in HBM before the timed region.  Also reported: the plain
-FP64 GPU CG (batch 1), the FP64 error of the mixed solutions, the apply-kernel
-roofline (FP32 batched matvec, the inner-solver kernel), and the reference CPU path timed on this host.
+One step = one batched solve of --batch (500) fresh instances out of 1,000 held-out
+instances (seeds disjoint from the training set; the default 2 steps solve each once):
+the instances' set-up (coefficient tables of the FP64 and FP32 operators, load vectors,
+and the preconditioner diagonal that the new coefficients invalidate), the learned warm
+start x0 = MLP(xi, eta) (training on 20,000 GPU-generated solves happens before, untimed,
+and is reported), then FP32 inner PCG under FP64 iterative refinement, every instance to
+||f - A x||_2 <= 1e-8 ||f||_2 (checked in FP64).  The zero-start solve of the same
+instances is timed the same way and reported beside it, with the set-up cost alone.
+The grid and operators are resident in HBM before the timed region.  Also reported: the
+plain FP64 GPU CG (batch 1) and the batched FP64 PCG, the FP64 error of the mixed
+solutions, the apply-kernel roofline (FP32 batched matvec, the inner-solver kernel), and
+the reference CPU path timed on this host.
 
 Prints one JSON line.
 
@@ -432,9 +437,11 @@ def workload_transform(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=2,
+                    help="timed steps; step k solves held-out batch k mod (eval / batch)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=512)
+    ap.add_argument("--batch", type=int, default=500)
+    ap.add_argument("--eval", type=int, default=1000, help="held-out instances (a multiple of --batch)")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -459,23 +466,42 @@ def main():
     dev = torch.device("cuda:0")
     rng = np.random.default_rng(SEED)
     B = args.batch
-    xi, eta, m = inputs(B, rng)
+    if args.eval % B:
+        sys.exit("bench.py: --eval must be a multiple of --batch")
+    # held-out evaluation instances (seeds disjoint from the training draw below), plus one
+    # batch for the untimed warm-up; the RHS modes m are shared by all instances
+    xi_all, eta_all, m = inputs(args.eval + B, rng)
+    nb = args.eval // B
+
+    def batch_of(k):       # k = -1: the warm-up batch
+        sl = slice(args.eval, args.eval + B) if k < 0 else slice(k * B, (k + 1) * B)
+        return xi_all[sl], eta_all[sl]
+
+    xi, eta = batch_of(0)
     grid = L.AdaptiveSparseGrid.regular(D, LEVEL)
     n = grid.size
-    stats = grid.stats()
     st0 = L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[0])
     op64 = L.MatrixVectorHomogen(grid, st0, precision=64, batch=B)
     op32 = L.MatrixVectorHomogen(grid, st0, precision=32, batch=B)
-    for b in range(B):
-        s = L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[b])
-        op64.set_coefficient(b, s)
-        op32.set_coefficient(b, s)
-    stats = op32.stats()        # the plan the operators run on
     f = torch.zeros(B, n, dtype=torch.float64, device=dev)
-    op64.rhs_sines(eta, m, out=f)
     x = torch.zeros(B, n, dtype=torch.float64, device=dev)
 
-    log("grid N=%d, %d instances set up" % (n, B))
+    def set_up(k):
+        """Per-batch set-up: coefficient tables of both operators for every instance
+        (lsg_op_set_coef) and the load vectors; the preconditioner diagonal follows in
+        the solve (it is invalidated by the new coefficients)."""
+        xk, ek = batch_of(k)
+        for b in range(B):
+            c = L.VariableCoefficient(L.COEF_EXPSIN, xi=xk[b])
+            op64.set_coefficient(b, c)
+            op32.set_coefficient(b, c)
+        op64.rhs_sines(ek, m, out=f)
+        return xk, ek
+
+    set_up(0)
+    stats = op32.stats()        # the plan the operators run on
+
+    log("grid N=%d, %d instances per batch, %d held-out instances" % (n, B, args.eval))
     # ---- plain FP64 GPU CG, one instance (the baseline the mixed/batched path is compared to)
     op1 = L.MatrixVectorHomogen(grid, st0, precision=64, batch=1)
     f1 = f[0:1].clone()
@@ -490,7 +516,7 @@ def main():
     fp64_solves = 1.0 / t_fp64
 
     log("plain FP64 CG: %d iterations, %.3f s" % (info1["iterations"], t_fp64))
-    # ---- reference FP64 solutions (tight) for the error statement, a few instances
+    # ---- reference FP64 solutions (tight) for the error statement: first instances of batch 0
     nchk = min(B, 4)
     opc = L.MatrixVectorHomogen(grid, st0, precision=64, batch=nchk)
     for b in range(nchk):
@@ -511,23 +537,36 @@ def main():
         del U
         log("warm start trained: %s" % json.dumps(ws.stats))
 
-    def solve_step(warm):
+    def solve_step(k, warm):
+        """One step: set-up of batch k, start vector, mixed-precision solve."""
+        xk, ek = set_up(k)
         if warm:
-            ws.predict(xi, eta, out=x)         # inference is part of the timed solve
+            ws.predict(xk, ek, out=x)         # inference is part of the timed solve
         else:
             x.zero_()
         return L.CG.solveMixed(1e-8, x, f, op64, op32, inner_rel_tol=args.inner_tol)
 
-    # ---- zero-start mixed solve (reported beside the timed value)
-    solve_step(False)
+    def timed_pass(warm, steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        infos = []
+        for s_ in range(steps):
+            r = solve_step(s_ % nb, warm)
+            infos += r if B > 1 else [r]
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) / steps, infos
+
+    # ---- per-batch set-up cost alone (inside every timed step below)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    infos0 = solve_step(False)
+    set_up(0)
     torch.cuda.synchronize()
-    dt0 = time.perf_counter() - t0
-    log("zero-start mixed-precision solve: %.3f s" % dt0)
+    setup_s = time.perf_counter() - t0
 
-    # ---- batched FP64 PCG from zero (batching alone, without mixed precision or warm start)
+    # ---- batched FP64 PCG from zero (batching alone, without mixed precision or warm start);
+    # one matvec first so that the FP64 operator's first-use autotuning is not timed
+    set_up(0)
+    op64.multiplication(x, torch.zeros_like(x))
     x.zero_()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -538,20 +577,19 @@ def main():
         infos64 = [infos64]
     log("batched FP64 PCG: %.3f s" % dt64)
 
-    # ---- timed: batched mixed-precision solves (warm-started when trained)
+    # ---- zero start over the held-out instances (same steps, reported beside the value)
+    solve_step(-1, False)
+    dt0, infos0 = timed_pass(False, args.steps)
+    log("zero-start mixed-precision solves: %.3f s per step" % dt0)
+
+    # ---- timed: batched mixed-precision solves of fresh instances (warm-started when trained)
     for _ in range(args.warmup):
-        solve_step(ws is not None)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    infos = None
-    for _ in range(args.steps):
-        infos = solve_step(ws is not None)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / args.steps
+        solve_step(-1, ws is not None)
+    dt, infos = timed_pass(ws is not None, args.steps)
     log("timed mixed-precision solves: %.3f s per step" % dt)
-    if B == 1:
-        infos, infos0 = [infos], [infos0]
     assert all(i["converged"] == 1 and i["rel_residual"] <= 1e-8 for i in infos), infos
+    # error of the mixed solutions against the tight FP64 solves (batch 0)
+    solve_step(0, ws is not None)
     err = float(torch.max(torch.linalg.norm(x[:nchk] - xref, dim=1) / torch.linalg.norm(xref, dim=1)))
 
     # ---- roofline of the dominant kernel: the apply stage of the FP32 batched matvec
@@ -621,10 +659,16 @@ def main():
             "fp64_plain_cg_iterations": info1["iterations"],
             "speedup_vs_fp64_plain": (B / dt) / fp64_solves,
             "fp64_batched_cg_solves_per_s": B / dt64,
+            "speedup_vs_fp64_batched": (B / dt) / (B / dt64),
+            "setup_s_per_batch": setup_s,
+            "setup_share_of_step": setup_s / dt,
+            "held_out_instances": args.eval,
+            "instances_timed": args.steps * B,
             "fp64_batched_cg_iterations_mean": float(np.mean([i["iterations"] for i in infos64])),
             "mixed_inner_iterations_mean": float(np.mean([i["iterations"] for i in infos])),
             "mixed_outer_sweeps_mean": float(np.mean([i["outer"] for i in infos])),
             "zero_start_solves_per_s": B / dt0,
+            "warm_start_gain": dt0 / dt,
             "zero_start_inner_iterations_mean": float(np.mean([i["iterations"] for i in infos0])),
             "zero_start_outer_sweeps_mean": float(np.mean([i["outer"] for i in infos0])),
             "warm_start": None if ws is None else dict(ws.stats, xi_samples=args.warm_xi,

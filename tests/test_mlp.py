@@ -81,16 +81,20 @@ def test_forward_and_gradients_vs_restatement(n_in, width, hidden, n_out, batch)
     assert _rel(y, y_ref) < 2e-3
     loss = m.gradients(xd, td)
     loss_ref, g_ref = R.loss_and_grads(layers, x, t)
-    assert abs(loss - loss_ref) < 1e-3 * loss_ref
+    assert abs(loss - loss_ref) < 1e-5 * loss_ref
     g = m.grads()
     o = 0
+    # Both sides round to BF16 at the same points; they differ by FP32 summation order,
+    # which now and then moves a value across a BF16 rounding boundary.  Measured: weight
+    # gradients within 2.8e-4 (relative 2-norm per layer), bias gradients within 4e-5, the
+    # loss within 6e-7; the bounds below leave a factor of about 3.
     for (i, n), (gW, gb) in zip(m.layer_shapes(), g_ref):
         GW = g[o:o + i * n].reshape(i, n)
         o += i * n
         Gb = g[o:o + n]
         o += n
-        assert np.linalg.norm(GW - gW) < 2e-2 * np.linalg.norm(gW)
-        assert np.linalg.norm(Gb - gb) < 2e-2 * np.linalg.norm(gb)
+        assert np.linalg.norm(GW - gW) < 1e-3 * np.linalg.norm(gW)
+        assert np.linalg.norm(Gb - gb) < 2e-4 * np.linalg.norm(gb)
     # deterministic: the same call gives the same bits
     m.gradients(xd, td)
     assert np.array_equal(m.grads(), g)
