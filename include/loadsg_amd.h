@@ -51,12 +51,26 @@ extern "C" {
  * with these coefficients run on the cell-wise (sparse) path on every grid. */
 #define LSG_COEF_CELL 4    /* cell_a[k], cell_c[k] for the cells in lsg_grid_cells() order */
 #define LSG_COEF_FUNC 5    /* host functions a_fn(x, ctx), c_fn(x, ctx) called at every cell centre */
+/* Stored element matrices: the counterpart of LocalStiffnessMatrices (a StencilTemplate's
+ * integration(cell, dirP, dirQ) evaluated once per cell and kept,
+ * localStiffnessMatrices/LocalStiffnessMatrices.h:76-321, LocalStiffnessMatrices.cpp:453-531)
+ * and of MatrixVectorHomogen::multiplication with such a plug-in (StoreLocalStiffnessMatrix,
+ * MatrixVectorHomogen.h:170-245,1619-1700).  Every cell of lsg_grid_cells() carries a
+ * symmetric 2^d x 2^d matrix between its corners, lsg_cell_entries(d) = 2^(d-1) (2^d + 1)
+ * values: corners p <= q (bit i of a corner = upper end of the cell in dimension i), entry
+ * e(p, q) = p 2^d - p (p - 1) / 2 + (q - p), the order of the reference's nested
+ * CellIndexIterator loops (cells/iterator_neu.cpp:34-49).  Entries of pairs with a corner on
+ * the boundary are never read (the reference does not store them).  Operators with stored
+ * matrices run on the cell-wise path and keep ncells x entries values per instance on the GPU. */
+#define LSG_COEF_MATRIX 6      /* cell_mat[k * lsg_cell_entries(d) + e] for cell k, entry e */
+#define LSG_COEF_INTEGRATOR 7  /* host integrator integ(d, level, pos, p, q, ctx) per cell and pair */
 
 /* CG stopping rules */
-#define LSG_STOP_REF_ABS 0   /* reference rule: r^T D^-1 r < eps^2 (cg_method.h:845-871) */
+#define LSG_STOP_REF_ABS 0   /* reference rule: iterate while delta = r^T D^-1 r > eps, stop early once
+                                * delta < eps^2 (cg_method.h:845-871) */
 #define LSG_STOP_REL 1       /* ||f - A x||_2 <= eps ||f||_2 */
-/* or-ed into stop_rule of lsg_cg_solve: plain CG, no diagonal preconditioner (D = 1; with
- * LSG_STOP_REF_ABS the rule reads r^T r < eps^2) */
+/* or-ed into stop_rule of lsg_cg_solve: plain CG, no diagonal preconditioner (D = 1 in
+ * LSG_STOP_REF_ABS's delta) */
 #define LSG_CG_NO_PRECOND 0x100
 
 typedef struct lsg_grid lsg_grid;
@@ -64,6 +78,12 @@ typedef struct lsg_op lsg_op;
 
 /* Host coefficient function: value at the point x (d coordinates in (0,1)). */
 typedef double (*lsg_coef_fn)(const double* x, void* ctx);
+/* Host element integrator: the entry between corners p <= q of the cell
+ * prod_i [pos_i 2^-level_i, (pos_i + 1) 2^-level_i] (a cell of depth level + 1), the value a
+ * plug-in's StencilTemplate::integration(cell, dirP, dirQ) returns (stencils/Stencil.h:740);
+ * called only for pairs whose corners are both interior. */
+typedef double (*lsg_cell_integrator_fn)(int d, const int32_t* level, const int32_t* pos, uint32_t p, uint32_t q,
+                                         void* ctx);
 
 typedef struct lsg_coef {
     int kind;
@@ -74,7 +94,9 @@ typedef struct lsg_coef {
     const double* cell_c; /* LSG_COEF_CELL: one value per cell (NULL: no mass term) */
     lsg_coef_fn a_fn;     /* LSG_COEF_FUNC (NULL: no stiffness term); only called during the */
     lsg_coef_fn c_fn;     /*   lsg_op_create / lsg_op_set_coef call that receives it */
-    void* ctx;
+    void* ctx;            /* context of a_fn, c_fn and integ */
+    const double* cell_mat;       /* LSG_COEF_MATRIX: ncells x lsg_cell_entries(d) values */
+    lsg_cell_integrator_fn integ; /* LSG_COEF_INTEGRATOR: only called during the call that receives it */
 } lsg_coef;
 
 typedef struct lsg_solve_info {
@@ -148,6 +170,12 @@ int lsg_op_stats(const lsg_op* op, int64_t* out, int nout);
  * may be NULL) the cell centres; depth_index (ncells, may be NULL) the position of the
  * cell's depth in the depth list. */
 int lsg_grid_cells(lsg_grid* g, int64_t* ncells, double* centres, int32_t* depth_index);
+/* The same cells as level-T boxes: level[k*d + i] = T_i of the cell's depth, pos[k*d + i] =
+ * lower position (the cell is prod_i [pos_i 2^-T_i, (pos_i + 1) 2^-T_i]).  Either output
+ * may be NULL. */
+int lsg_grid_cell_boxes(lsg_grid* g, int32_t* level, int32_t* pos);
+/* Values per cell of an LSG_COEF_MATRIX coefficient: 2^(d-1) (2^d + 1). */
+int64_t lsg_cell_entries(int d);
 int lsg_grid_destroy(lsg_grid* g);
 
 /* -------------------------------------------------------------- operators */

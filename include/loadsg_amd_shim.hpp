@@ -12,14 +12,19 @@
  *   template<class S> static bool CG::solveHomogen(double eps, VectorSparseG& x, VectorSparseG& f,
  *       int* iterations, VectorSparseG& usol, MatrixVectorHomogen&, S stencil, double* time_precon)
  *       (applications/cg_method.h:27-33,759-886): x holds the start vector, the stopping rule is the
- *       reference's r^T D^-1 r < eps^2 with the diagonal preconditioner precond3.
+ *       reference's (iterate while r^T D^-1 r > eps, stop early once it is below eps^2) with the
+ *       diagonal preconditioner precond3; the plug-in is
+ *       taken by const reference (the reference copies it, stored matrices included).
  * Vectors stay the reference's VectorSparseG (host doubles in slot order); the device
  * copies live inside the operator.  The grid must be final before the operator is
  * built, as for VectorSparseG (vector.cc:12-23).
  *
  * Stencil plug-ins: Poisson and HelmHoltz map to LSG_COEF_CONST; a variable
  * coefficient is passed as lsg_shim::Coefficient (an lsg_coef, or host functions a(x),
- * c(x) sampled at the cell centres).  Errors throw
+ * c(x) sampled at the cell centres); any StencilTemplate plug-in goes through
+ * lsg_shim::LocalStiffnessMatrices (its integration() evaluated once per cell and the
+ * element matrices kept on the GPU, as the reference's LocalStiffnessMatrices keeps them on
+ * its worker nodes).  Errors throw
  * std::runtime_error carrying lsg_last_error() (the reference prints and exits).
  */
 #ifndef LOADSG_AMD_SHIM_HPP
@@ -58,6 +63,14 @@ struct Coefficient {
         c.a0 = a0;
         c.c0 = c0;
         bind();
+    }
+    /* Stored element matrices (LSG_COEF_MATRIX): entries stay owned by the caller until the
+     * operator is built; tag names them in the operator cache. */
+    static Coefficient matrices(std::string tag, const double* entries) {
+        Coefficient k(LSG_COEF_MATRIX);
+        k.c.cell_mat = entries;
+        k.tag = std::move(tag);
+        return k;
     }
     /* General coefficient from host functions of the point x (DimensionSparseGrid
      * coordinates); pass an empty std::function for an absent term.  Operators are cached
@@ -127,6 +140,92 @@ class DeviceGrid {
     int64_t n_ = 0;
 };
 
+/* LocalStiffnessMatrices (localStiffnessMatrices/LocalStiffnessMatrices.h:76-321) for the
+ * device: the constructor evaluates the plug-in's integration(cell, dirP, dirQ)
+ * (stencils/Stencil.h:740) once for every cell of depth T+1 of every depth T and every pair
+ * of interior corners p <= q, after initialize(T) for the cell's depth, as
+ * LocalStiffnessMatrixFixedDepthSymmetric::setValues does (LocalStiffnessMatrices.cpp:453-500);
+ * addStencil adds a second plug-in's matrices (LocalStiffnessMatricesDynamicDistribution.h:
+ * 181-197).  Passed to MatrixVectorHomogen::multiplication or CG::solveHomogen like any
+ * plug-in, the matrices are uploaded once and applied on the GPU (the reference's
+ * StoreLocalStiffnessMatrix branch, MatrixVectorHomogen.h:170-245,1619-1700; the
+ * preconditioner is psi_p^T A psi_p from the same matrices, cg_method.h:544-641).
+ * number_processes is accepted for the signature's sake: one GPU holds every depth. */
+class LocalStiffnessMatrices {
+  public:
+    LocalStiffnessMatrices(AdaptiveSparseGrid& grid, StencilTemplate& stencil, int /*number_processes*/ = 1)
+        : grid_(grid) {
+        const int d = DimensionSparseGrid;
+        check(lsg_grid_cells(grid_.get(), &ncells_, nullptr, nullptr));
+        level_.resize((size_t)ncells_ * d);
+        pos_.resize((size_t)ncells_ * d);
+        check(lsg_grid_cell_boxes(grid_.get(), level_.data(), pos_.data()));
+        entries_.assign((size_t)(ncells_ * lsg_cell_entries(d)), 0.0);
+        add(stencil);
+    }
+    LocalStiffnessMatrices(const LocalStiffnessMatrices&) = delete;
+    LocalStiffnessMatrices& operator=(const LocalStiffnessMatrices&) = delete;
+    void addStencil(StencilTemplate& stencil) {
+        add(stencil);
+        ++version_;
+    }
+    TypeMatrixVectorMultiplication getTypeMatrixVectorMultiplication() { return StoreLocalStiffnessMatrix; }
+    Coefficient coefficient() const {
+        return Coefficient::matrices("lsm:" + std::to_string(reinterpret_cast<uintptr_t>(this)) + ":" +
+                                         std::to_string(version_),
+                                     entries_.data());
+    }
+    const std::vector<double>& entries() const { return entries_; }
+
+  private:
+    void add(StencilTemplate& stencil) {
+        const int d = DimensionSparseGrid;
+        const unsigned nc = 1u << d;
+        const int64_t ne = lsg_cell_entries(d);
+        Depth last;
+        bool have = false;
+        for (int64_t k = 0; k < ncells_; ++k) {
+            const int32_t* lev = &level_[(size_t)k * d];
+            const int32_t* pos = &pos_[(size_t)k * d];
+            Depth T;
+            for (int i = 0; i < d; ++i) T.set(lev[i], i);
+            if (!have || !(T == last)) {
+                stencil.initialize(T);
+                last = T;
+                have = true;
+            }
+            // the cell of depth T+1: 1D cell index 2^(T_i + 1) + pos_i (its centre
+            // (2 pos_i + 1) 2^-(T_i + 1)), cells/celldimension.h:117-141
+            CellDimension cell;
+            for (int i = 0; i < d; ++i) cell.replace(i, (indexInteger)((1u << (lev[i] + 1)) + (unsigned)pos[i]));
+            int e = 0;
+            for (unsigned p = 0; p < nc; ++p)
+                for (unsigned q = p; q < nc; ++q, ++e) {
+                    bool interior = true;
+                    for (int i = 0; i < d && interior; ++i) {
+                        const int32_t cp = pos[i] + (int32_t)((p >> i) & 1u), cq = pos[i] + (int32_t)((q >> i) & 1u);
+                        const int32_t top = int32_t(1) << lev[i];
+                        interior = cp > 0 && cp < top && cq > 0 && cq < top;
+                    }
+                    if (!interior) continue;
+                    CellIndexDirection dp, dq;
+                    for (int i = 0; i < d; ++i) {
+                        dp.setDirection(i, ((p >> i) & 1u) ? Right : Left);
+                        dq.setDirection(i, ((q >> i) & 1u) ? Right : Left);
+                    }
+                    entries_[(size_t)(k * ne + e)] += stencil.integration(cell, dp, dq);
+                }
+        }
+    }
+    DeviceGrid grid_;
+    int64_t ncells_ = 0;
+    std::vector<int32_t> level_, pos_;
+    std::vector<double> entries_;
+    int version_ = 0;
+};
+
+inline Coefficient coefficientOf(const LocalStiffnessMatrices& m) { return m.coefficient(); }
+
 /* MatrixVectorHomogen on the device.  One device operator per distinct coefficient
  * is created on first use and kept (the reference keeps its multilevel work arrays
  * in the MatrixVectorHomogen object in the same way). */
@@ -164,7 +263,7 @@ class MatrixVectorHomogen {
     };
     static bool same(const Coefficient& a, const Coefficient& b) {
         if (a.c.kind != b.c.kind) return false;
-        if (a.c.kind == LSG_COEF_FUNC) return a.tag == b.tag;
+        if (a.c.kind == LSG_COEF_FUNC || a.c.kind == LSG_COEF_MATRIX) return a.tag == b.tag;
         return a.c.a0 == b.c.a0 && a.c.c0 == b.c.c0 && a.xi == b.xi;
     }
     DeviceGrid grid_;
@@ -176,7 +275,7 @@ class MatrixVectorHomogen {
 struct CG {
     template <class S>
     static bool solveHomogen(double eps, VectorSparseG& x, VectorSparseG& f, int* iterations, VectorSparseG& /*usol*/,
-                             MatrixVectorHomogen& matrix, S stencil, double* time_precon,
+                             MatrixVectorHomogen& matrix, const S& stencil, double* time_precon,
                              int maxit = 500 /* CG::maxIteration, cg_method.h:35 */) {
         lsg_op* op = matrix.opFor(coefficientOf(stencil));
         if (time_precon) {

@@ -33,6 +33,7 @@ STOP_REF_ABS, STOP_REL = 0, 1
 CG_NO_PRECOND = 0x100     # or-ed into stop_rule: plain CG without the diagonal preconditioner
 COEF_CONST, COEF_CFG2, COEF_CFG3, COEF_EXPSIN = 0, 1, 2, 3
 COEF_CELL, COEF_FUNC = 4, 5
+COEF_MATRIX, COEF_INTEGRATOR = 6, 7
 
 _lib = None
 
@@ -42,13 +43,16 @@ class LoadsgError(RuntimeError):
 
 
 _COEF_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.POINTER(ctypes.c_double), ctypes.c_void_p)
+_INTEG_FN = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_int32),
+                             ctypes.POINTER(ctypes.c_int32), ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p)
 
 
 class _Coef(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("a0", ctypes.c_double), ("c0", ctypes.c_double),
                 ("nxi", ctypes.c_int), ("xi", ctypes.POINTER(ctypes.c_double)),
                 ("cell_a", ctypes.POINTER(ctypes.c_double)), ("cell_c", ctypes.POINTER(ctypes.c_double)),
-                ("a_fn", _COEF_FN), ("c_fn", _COEF_FN), ("ctx", ctypes.c_void_p)]
+                ("a_fn", _COEF_FN), ("c_fn", _COEF_FN), ("ctx", ctypes.c_void_p),
+                ("cell_mat", ctypes.POINTER(ctypes.c_double)), ("integ", _INTEG_FN)]
 
 
 class SolveInfo(ctypes.Structure):
@@ -78,6 +82,8 @@ def lib():
             "lsg_grid_stats": (i32, [vp, vp, i32]),
             "lsg_op_stats": (i32, [vp, vp, i32]),
             "lsg_grid_cells": (i32, [vp, ctypes.POINTER(i64), vp, vp]),
+            "lsg_grid_cell_boxes": (i32, [vp, vp, vp]),
+            "lsg_cell_entries": (i64, [i32]),
             "lsg_grid_destroy": (i32, [vp]),
             "lsg_op_create": (i32, [vp, ctypes.POINTER(_Coef), i32, i32, ctypes.POINTER(vp)]),
             "lsg_op_set_coef": (i32, [vp, i32, ctypes.POINTER(_Coef)]),
@@ -268,6 +274,15 @@ class AdaptiveSparseGrid:
         _check(lib().lsg_grid_cells(self._h, ctypes.byref(n), _ptr(x), _ptr(t)))
         return x, t
 
+    def cell_boxes(self):
+        """The same cells as level-T boxes: (level, pos), both ncells x d int32; cell k is
+        prod_i [pos_i 2^-level_i, (pos_i + 1) 2^-level_i]."""
+        n = self.cells()[0].shape[0]
+        lev = np.zeros((n, self.dim), dtype=np.int32)
+        pos = np.zeros((n, self.dim), dtype=np.int32)
+        _check(lib().lsg_grid_cell_boxes(self._h, _ptr(lev), _ptr(pos)))
+        return lev, pos
+
     def stats(self):
         out = np.zeros(12, dtype=np.int64)
         _check(lib().lsg_grid_stats(self._h, _ptr(out), 12))
@@ -364,6 +379,83 @@ class HostFunction(Stencil):
                 keep.append(cb)
                 setattr(cc, field, cb)
         cc._keep = keep
+        return cc
+
+
+def cell_entries(d):
+    """Values of one stored element matrix: 2^(d-1) (2^d + 1) (corners p <= q)."""
+    return (1 << (d - 1)) * ((1 << d) + 1)
+
+
+def cell_pairs(d):
+    """Corner pairs (p, q), p <= q, in stored-entry order (the reference's nested
+    CellIndexIterator loops, cells/iterator_neu.cpp:34-49); bit i of a corner = upper end of
+    the cell in dimension i."""
+    nc = 1 << d
+    return [(p, q) for p in range(nc) for q in range(p, nc)]
+
+
+class LocalStiffnessMatrices(Stencil):
+    """Stored element matrices (LocalStiffnessMatrices, localStiffnessMatrices/
+    LocalStiffnessMatrices.h:76-321): one symmetric 2^d x 2^d matrix per cell of
+    grid.cells(), computed once from a plug-in's integration and applied by the matvec in
+    place of the on-the-fly stencil (MatrixVectorHomogen.h:170-245,1619-1700).
+
+    integration: an (ncells x cell_entries(d)) array in cell_pairs(d) order, or a function
+    integration(level, pos, p, q) of arrays of cell boxes (ncells x d each, as
+    grid.cell_boxes()) and one corner pair, returning ncells values."""
+
+    def __init__(self, grid, integration):
+        self.kind = COEF_MATRIX
+        d = grid.dim
+        ncell = grid.cells()[0].shape[0]
+        if callable(integration):
+            lev, pos = grid.cell_boxes()
+            E = np.zeros((ncell, cell_entries(d)))
+            for e, (p, q) in enumerate(cell_pairs(d)):
+                E[:, e] = integration(lev, pos, p, q)
+        else:
+            E = np.array(integration, dtype=np.float64)
+        if E.shape != (ncell, cell_entries(d)):
+            raise LoadsgError("element matrices: expected shape (%d, %d), got %s" % (ncell, cell_entries(d), E.shape))
+        self.entries = np.ascontiguousarray(E)
+        self.ncell = ncell
+
+    def addStencil(self, other):
+        """Adds the element matrices of another LocalStiffnessMatrices on the same grid
+        (LocalStiffnessMatricesDynamicDistribution::addStencil, ADDLocalStiffnessMatrices)."""
+        if other.entries.shape != self.entries.shape:
+            raise LoadsgError("addStencil: element matrices of another grid")
+        self.entries = self.entries + other.entries
+        return self
+
+    def _c(self, grid):
+        if grid.cells()[0].shape[0] != self.ncell:
+            raise LoadsgError("element matrices were built for another grid")
+        cc = _Coef()
+        cc.kind = COEF_MATRIX
+        cc.cell_mat = self.entries.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        cc._keep = [self.entries]
+        return cc
+
+
+class CellIntegrator(Stencil):
+    """Stored element matrices from a scalar host integrator integration(level, pos, p, q)
+    of one cell (level, pos: length-d int arrays) and one corner pair, called by the library
+    once per cell and interior pair (LSG_COEF_INTEGRATOR: a plug-in's
+    StencilTemplate::integration(cell, dirP, dirQ), Stencil.h:740)."""
+
+    def __init__(self, integration):
+        self.kind, self.fn = COEF_INTEGRATOR, integration
+
+    def _c(self, grid):
+        fn = self.fn
+        cb = _INTEG_FN(lambda d, lev, pos, p, q, ctx: float(
+            fn(np.ctypeslib.as_array(lev, (d,)), np.ctypeslib.as_array(pos, (d,)), int(p), int(q))))
+        cc = _Coef()
+        cc.kind = COEF_INTEGRATOR
+        cc.integ = cb
+        cc._keep = [cb]
         return cc
 
 

@@ -82,7 +82,9 @@ struct PinnedBuf {
 // Separable representation of the coefficient families: a = sum_m prod_j g_mj(x_j),
 // c = sum_m prod_j h_mj(x_j), sampled at the 1D cell centres (2c+1)/2^(t+1) of every
 // level t.  Table of one (term, dim): level t occupies [2^t - 2, 2^(t+1) - 2).
-bool cellCoefKind(int kind) { return kind == LSG_COEF_CELL || kind == LSG_COEF_FUNC; }
+bool matCoefKind(int kind) { return kind == LSG_COEF_MATRIX || kind == LSG_COEF_INTEGRATOR; }
+bool cellCoefKind(int kind) { return kind == LSG_COEF_CELL || kind == LSG_COEF_FUNC || matCoefKind(kind); }
+int64_t cellEntries(int d) { return (int64_t(1) << (d - 1)) * ((int64_t(1) << d) + 1); }
 
 // Dense grids run on their single-item plan (every prolongation dimension materialised)
 // for batches up to singlePlanMaxBatch(): measured faster than the families along the last
@@ -104,6 +106,8 @@ int coefTerms(const lsg_coef& c, int d, int& nA, int& nC) {
     switch (c.kind) {
         case LSG_COEF_CELL: nA = c.cell_a != nullptr; nC = c.cell_c != nullptr; break;
         case LSG_COEF_FUNC: nA = c.a_fn != nullptr; nC = c.c_fn != nullptr; break;
+        case LSG_COEF_MATRIX: nA = c.cell_mat != nullptr; nC = 0; break;
+        case LSG_COEF_INTEGRATOR: nA = c.integ != nullptr; nC = 0; break;
         case LSG_COEF_CONST: nA = c.a0 != 0.0; nC = c.c0 != 0.0; break;
         case LSG_COEF_CFG2: nA = 2; nC = 2; break;
         case LSG_COEF_CFG3: nA = 1 + d; nC = 0; break;
@@ -211,6 +215,39 @@ int cellValues(const Grid& g, const lsg_coef& c, std::vector<double>& ac) {
     return LSG_OK;
 }
 
+// Stored element matrices of every cell for LSG_COEF_MATRIX / LSG_COEF_INTEGRATOR, laid out
+// entry-major (entry e of cell k at e * nCells + k: the cells a wave handles are adjacent in
+// memory for every entry).  Pairs with a boundary corner are left 0 and never read.
+int cellMatrices(const Grid& g, const lsg_coef& c, std::vector<double>& E) {
+    const int d = g.d;
+    const SparseTables& S = g.sp;
+    const int nd = (int)g.depths.size();
+    const int64_t nCells = S.cellOff[nd];
+    const unsigned nc = 1u << d;
+    const int64_t ne = cellEntries(d);
+    E.assign((size_t)(nCells * ne), 0.0);
+    int32_t level[MAXD], pos[MAXD];
+    for (int t = 0; t < nd; ++t) {
+        for (int i = 0; i < d; ++i) level[i] = g.depths[t][i];
+        for (int64_t k = S.cellOff[t]; k < S.cellOff[t + 1]; ++k) {
+            const int32_t* corner = &S.cellCorner[(size_t)(k * nc)];
+            for (int i = 0; i < d; ++i) pos[i] = S.cellLo[(size_t)(k * d + i)];
+            for (unsigned p = 0; p < nc; ++p) {
+                if (corner[p] < 0) continue;
+                for (unsigned q = p; q < nc; ++q) {
+                    if (corner[q] < 0) continue;
+                    const int e = cellEntry(p, q, nc);
+                    const double v = c.kind == LSG_COEF_MATRIX ? c.cell_mat[(size_t)(k * ne + e)]
+                                                               : c.integ(d, level, pos, p, q, c.ctx);
+                    if (!std::isfinite(v)) return fail(LSG_ERR_NAN, "non-finite element matrix entry");
+                    E[(size_t)(e * nCells + k)] = v;
+                }
+            }
+        }
+    }
+    return LSG_OK;
+}
+
 // ---- separable load vectors: 1D moments on the host
 // 4-point Gauss-Legendre rule on [-1, 1] (abscissae and weights of the Legendre
 // polynomial P_4), applied on every cell of the finest 1D level of the grid: every
@@ -275,6 +312,7 @@ struct lsg_op {
     const Grid* G = nullptr;   // the grid the kernels run on: grid->g, or its cell-wise copy
     const Plan* P = nullptr;   // its plan: G->plan, or the single-item plan of a dense grid
     bool cellKind = false;     // general coefficient given per cell (cellCoef filled on the host)
+    bool matKind = false;      // stored element matrices per cell (cellMat filled on the host)
     int device = 0;            // HIP device the operator's memory and stream belong to
     int prec = 64;
     size_t vs = 8;
@@ -316,6 +354,8 @@ struct lsg_op {
     SparseDev sd{};
     std::vector<std::pair<int, std::unique_ptr<DevMem>>> spLevels;   // (count, families)
     DevMem zbuf, activeDev, cellCoef;
+    DevMem cellMat;            // stored element matrices: [instance][entry][cell]
+    int64_t nEnt = 0;          // entries per cell
     // instances still iterating in a CG solve: the matvec grids cover only these slots
     DevMem actMap;
     int nMap = 0;             // > 0: launch nMap slots mapped through actMap
@@ -405,6 +445,9 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st, int nSub) {
     a.map = map;
     a.cellCoef = op->cellCoef.as<V>();
     a.ccStride = 2 * op->nCells;
+    a.cellMat = op->matKind ? op->cellMat.as<V>() : nullptr;
+    a.matStride = op->nCells * op->nEnt;
+    a.nCells = op->nCells;
     for (auto& L : op->spLevels) {
         a.fam = L.second->as<FamilyItem>();
         launch_sp_apply<V>(a, L.first, s);
@@ -635,12 +678,15 @@ int launchDiag(lsg_op* op, void* Dp) {
     int64_t cs = (int64_t)(op->nA + op->nC) * g.d * op->tabLen;
     if (op->sparse) {
         if (int rc = ensureCellCoef(op)) return rc;
+        const int64_t ms = op->nCells * op->nEnt;
         if (op->prec == 64)
             launch_sp_diag<double>(op->sd, op->activeDev.as<uint8_t>(), (int)g.n, op->cellCoef.as<double>(),
-                                   2 * op->nCells, (double*)Dp, g.n, op->batch, op->stream);
+                                   2 * op->nCells, op->matKind ? op->cellMat.as<double>() : nullptr, ms, op->nCells,
+                                   (double*)Dp, g.n, op->batch, op->stream);
         else
             launch_sp_diag<float>(op->sd, op->activeDev.as<uint8_t>(), (int)g.n, op->cellCoef.as<float>(),
-                                  2 * op->nCells, (float*)Dp, g.n, op->batch, op->stream);
+                                  2 * op->nCells, op->matKind ? op->cellMat.as<float>() : nullptr, ms, op->nCells,
+                                  (float*)Dp, g.n, op->batch, op->stream);
         HIPCHK(hipGetLastError());
         return LSG_OK;
     }
@@ -1416,6 +1462,24 @@ int lsg_grid_cells(lsg_grid* g, int64_t* ncells, double* centres, int32_t* depth
     return LSG_OK;
 }
 
+int lsg_grid_cell_boxes(lsg_grid* g, int32_t* level, int32_t* pos) {
+    if (!g) return fail(LSG_ERR_ARG, "null argument");
+    const Grid* cg = nullptr;
+    if (int rc = cellGrid(g, cg)) return rc;
+    const SparseTables& S = cg->sp;
+    const int d = cg->d;
+    const int nd = (int)cg->depths.size();
+    for (int t = 0; t < nd; ++t)
+        for (int64_t k = S.cellOff[t]; k < S.cellOff[t + 1]; ++k)
+            for (int i = 0; i < d; ++i) {
+                if (level) level[k * d + i] = cg->depths[t][i];
+                if (pos) pos[k * d + i] = S.cellLo[(size_t)(k * d + i)];
+            }
+    return LSG_OK;
+}
+
+int64_t lsg_cell_entries(int d) { return (d < 1 || d > MAXD) ? -1 : cellEntries(d); }
+
 int lsg_grid_destroy(lsg_grid* g) {
     delete g;
     return LSG_OK;
@@ -1431,6 +1495,7 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
     op->batch = batch;
     op->kind = coef->kind;
     op->cellKind = cellCoefKind(coef->kind);
+    op->matKind = matCoefKind(coef->kind);
     op->G = &grid->g;
     if (op->cellKind) {
         int rc = cellGrid(grid, op->G);
@@ -1650,6 +1715,16 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         HIPCHK(op->zbuf.alloc((size_t)P.zSize * batch * vs));
         op->nCells = T.cellOff.back();
         HIPCHK(op->cellCoef.alloc((size_t)std::max<int64_t>(1, 2 * op->nCells) * batch * vs));
+        if (op->matKind) {
+            op->nEnt = cellEntries(g.d);
+            const size_t bytes = (size_t)std::max<int64_t>(1, op->nCells * op->nEnt) * batch * vs;
+            size_t freeB = 0, totalB = 0;
+            HIPCHK(hipMemGetInfo(&freeB, &totalB));
+            if (bytes > freeB)
+                return fail(LSG_ERR_CAPACITY, "stored element matrices need " + std::to_string(bytes) +
+                                                  " bytes of device memory, " + std::to_string(freeB) + " are free");
+            HIPCHK(op->cellMat.alloc(bytes));
+        }
         std::vector<uint8_t> act(g.n);
         for (int64_t s0 = 0; s0 < g.n; ++s0) act[g.devOfSlot[s0]] = g.active[s0];
         HIPCHK(op->activeDev.alloc((size_t)g.n));
@@ -1667,8 +1742,25 @@ int lsg_op_set_coef(lsg_op* op, int instance, const lsg_coef* coef) {
     if (!op || !coef || instance < 0 || instance >= op->batch) return fail(LSG_ERR_ARG, "bad arguments");
     int nA, nC;
     const Grid& g = *op->G;
-    if (coefTerms(*coef, g.d, nA, nC) || nA != op->nA || nC != op->nC || cellCoefKind(coef->kind) != op->cellKind)
+    if (coefTerms(*coef, g.d, nA, nC) || nA != op->nA || nC != op->nC || cellCoefKind(coef->kind) != op->cellKind ||
+        matCoefKind(coef->kind) != op->matKind)
         return fail(LSG_ERR_ARG, "coefficient family differs from the operator's");
+    if (op->matKind) {
+        // stored element matrices, straight into cellMat
+        std::vector<double> E;
+        if (int rc = cellMatrices(g, *coef, E)) return rc;
+        const size_t per = (size_t)(op->nCells * op->nEnt);
+        if (E.size() != per) return fail(LSG_ERR_ARG, "cell count mismatch");
+        if (op->prec == 64) {
+            HIPCHK(hipMemcpy(op->cellMat.as<double>() + per * instance, E.data(), per * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> ef(E.begin(), E.end());
+            HIPCHK(hipMemcpy(op->cellMat.as<float>() + per * instance, ef.data(), per * 4, hipMemcpyHostToDevice));
+        }
+        op->diagValid = false;
+        op->cellCoefValid = true;
+        return LSG_OK;
+    }
     if (op->cellKind) {
         // general coefficient: (a, c) per cell sampled on the host, straight into cellCoef
         std::vector<double> ac;

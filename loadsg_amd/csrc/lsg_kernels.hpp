@@ -95,7 +95,18 @@ struct SparseApplyArgs {
     const int* map;           // slot -> instance (null: slot = instance)
     const V* cellCoef;        // per instance: (a, c) at the centre of every cell of depth T+1
     int64_t ccStride;
+    // stored element matrices (null: element matrices from cellCoef): entry e of cell c of
+    // instance i at cellMat[i * matStride + e * nCells + c], e = cellEntry(p, q)
+    const V* cellMat;
+    int64_t matStride, nCells;
 };
+
+// Position of the pair of cell corners p <= q (bit i = upper end in dimension i) among the
+// 2^(d-1) (2^d + 1) entries of a symmetric element matrix, in the order of the reference's
+// nested CellIndexIterator loops (cells/iterator_neu.cpp:34-49, LocalStiffnessMatrices.cpp:463-500).
+__host__ __device__ inline int cellEntry(unsigned p, unsigned q, unsigned nc) {
+    return (int)(p * nc - (p * (p - 1)) / 2 + (q - p));
+}
 
 template <typename V>
 void launch_sp_cellcoef(const SparseDev& s, int64_t nCells, int nA, int nC, const V* coefTab, int64_t coefStride,
@@ -109,8 +120,9 @@ void launch_sp_prol(const ProlItem* items, int nitems, const SparseDev& s, const
 template <typename V>
 void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str);
 template <typename V>
-void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, const V* cellCoef, int64_t ccStride, V* z,
-                    int64_t zStride, int batch, hipStream_t str);
+void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, const V* cellCoef, int64_t ccStride,
+                    const V* cellMat, int64_t matStride, int64_t nCells, V* z, int64_t zStride, int batch,
+                    hipStream_t str);
 
 template <typename V>
 void launch_nodal(const NodalItem* items, int nitems, int d, const V* x, int64_t xStride, V* nodal, int64_t nStride,

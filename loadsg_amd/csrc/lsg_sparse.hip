@@ -157,7 +157,11 @@ __global__ void __launch_bounds__(256) sp_prol_kernel(const ProlItem* __restrict
 // the 2^D corner values of a cell once, forms the local product with the element
 // matrix held in registers and stores it; each point then sums its entries of the
 // cells around it.  D = 0: any d, each point gathers the corners of its cells.
-template <typename V, int D>
+// STORED: the element matrices are the caller's, one symmetric matrix per cell read from
+// HBM (the stored local stiffness matrices of LocalStiffnessMatrices::applyLocalMatrix,
+// localStiffnessMatrices/LocalStiffnessMatrices.cpp:506-531); otherwise a(centre) times
+// the Q1 stiffness plus c(centre) times the Q1 mass matrix.
+template <typename V, int D, bool STORED>
 __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
     const int inst = a.map ? a.map[blockIdx.x % a.batch] : (int)(blockIdx.x % a.batch);
     if (a.st && !a.st[inst].active) return;
@@ -173,7 +177,69 @@ __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
     const V* cc = a.cellCoef + inst * a.ccStride;
     const unsigned nc = 1u << d;
     const bool mass = a.nC > 0;
-    if constexpr (D > 0) {
+    if constexpr (STORED) {
+        const int64_t c0 = s.cellOff[t];
+        const int64_t ncell = s.cellOff[t + 1] - c0;
+        const V* E = a.cellMat + inst * a.matStride + c0;   // entry e of cell c: E[e * nCells + c]
+        const int64_t es = a.nCells;
+        const int64_t r0 = s.boxOff[t];
+        const V* g = it.rd_off[0] >= 0 ? wb + it.rd_off[0] : nullptr;
+        if constexpr (D > 0) {
+            // one thread per cell: y = E u over the cell's corners (each entry read once,
+            // consecutive cells of a wave read consecutive addresses), then each point sums
+            // its entries of the cells around it
+            constexpr int NC = 1 << D;
+            V* Y = A + 2 * size;
+            for (int64_t c = threadIdx.x; c < ncell; c += blockDim.x) {
+                const int32_t* corner = s.cellCorner + (c0 + c) * NC;
+                V uq[NC], y[NC];
+#pragma unroll
+                for (int q = 0; q < NC; ++q) {
+                    const int32_t j = corner[q];
+                    uq[q] = j >= 0 ? u[j] : V(0);
+                    y[q] = V(0);
+                }
+                const V* Ec = E + c;
+#pragma unroll
+                for (int p = 0; p < NC; ++p)
+#pragma unroll
+                    for (int q = p; q < NC; ++q) {
+                        const V m = Ec[(int64_t)cellEntry((unsigned)p, (unsigned)q, NC) * es];
+                        y[p] = fma(m, uq[q], y[p]);
+                        if (q != p) y[q] = fma(m, uq[p], y[q]);
+                    }
+#pragma unroll
+                for (int p = 0; p < NC; ++p) Y[c * NC + p] = y[p];
+            }
+            pass_barrier();
+            for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
+                V z = V(0);
+                const int64_t k1 = s.rcPtr[r0 + i + 1];
+                for (int64_t k = s.rcPtr[r0 + i]; k < k1; ++k) z += Y[s.rcY[k]];
+                A[i] = g ? z + g[i] : z;
+            }
+            pass_barrier();
+        } else {
+            // any d: each point forms its rows of the matrices of the cells around it
+            for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
+                V z = V(0);
+                const int64_t k1 = s.rcPtr[r0 + i + 1];
+                for (int64_t k = s.rcPtr[r0 + i]; k < k1; ++k) {
+                    const int64_t cl = s.rcCell[k];
+                    const unsigned cj = (unsigned)s.rcCorner[k];
+                    const int32_t* corner = s.cellCorner + (c0 + cl) * nc;
+                    for (unsigned q = 0; q < nc; ++q) {
+                        const int32_t j = corner[q];
+                        if (j < 0) continue;
+                        const int e = q >= cj ? cellEntry(cj, q, nc) : cellEntry(q, cj, nc);
+                        z = fma(E[(int64_t)e * es + cl], u[j], z);
+                    }
+                }
+                A[i] = g ? z + g[i] : z;
+            }
+            pass_barrier();
+        }
+    } else if constexpr (D > 0) {
         // element matrix of a level-T cell, stiffness and mass parts: the entry between
         // corners p and q depends only on the mask p ^ q
         constexpr int NC = 1 << D;
@@ -303,12 +369,16 @@ __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
 // with applyStencilLocal over the cells of depth T+1 in the grid): for every stored
 // point J of p's prewavelet stencil, the cells around J, their corners q and the
 // stencil coefficient psi_p(q) from the position offset.  Hanging nodes get 1.
+// With stored element matrices (cellMat) the entries come from them, as in
+// Preconditioning::precondLocalStiffnessMatrix (cg_method.h:544-700).
 template <typename V>
 __global__ void __launch_bounds__(256) sp_diag_kernel(SparseDev s, const uint8_t* __restrict__ activeDev, int n,
                                                       const V* __restrict__ cellCoef, int64_t ccStride,
+                                                      const V* __restrict__ cellMat, int64_t matStride, int64_t nCells,
                                                       V* __restrict__ z, int64_t zStride) {
     const int inst = blockIdx.y;
     const V* cc = cellCoef + inst * ccStride;
+    const V* E = cellMat ? cellMat + inst * matStride : nullptr;
     const int d = s.d;
     const unsigned nc = 1u << d;
     for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
@@ -328,7 +398,7 @@ __global__ void __launch_bounds__(256) sp_diag_kernel(SparseDev s, const uint8_t
                 const int64_t cell = c0 + s.rcCell[e];
                 const unsigned cj = (unsigned)s.rcCorner[e];
                 const int32_t* lo = s.cellLo + cell * d;
-                const V aC = cc[2 * cell], cC = cc[2 * cell + 1];
+                const V aC = E ? V(0) : cc[2 * cell], cC = E ? V(0) : cc[2 * cell + 1];
                 const int32_t* corner = s.cellCorner + cell * nc;
                 for (unsigned q = 0; q < nc; ++q) {
                     if (corner[q] < 0) continue;
@@ -345,7 +415,8 @@ __global__ void __launch_bounds__(256) sp_diag_kernel(SparseDev s, const uint8_t
                         psi *= c1;
                     }
                     if (psi == V(0)) continue;
-                    row += element<V>(d, T, cj, q, aC, cC) * psi;
+                    if (E) row += E[(int64_t)(q >= cj ? cellEntry(cj, q, nc) : cellEntry(q, cj, nc)) * nCells + cell] * psi;
+                    else row += element<V>(d, T, cj, q, aC, cC) * psi;
                 }
             }
             total += V(s.convVal[k]) * row;
@@ -380,19 +451,30 @@ void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str) {
     if (nitems == 0) return;
     const dim3 grid((unsigned)nitems * (unsigned)a.batch);
     static_assert(SP_CELL_MAXD == 4, "one instantiation per cell-wise dimension");
+    if (a.cellMat) {
+        switch (a.sp.d) {
+            case 1: hipLaunchKernelGGL((sp_apply_kernel<V, 1, true>), grid, dim3(256), 0, str, a); break;
+            case 2: hipLaunchKernelGGL((sp_apply_kernel<V, 2, true>), grid, dim3(256), 0, str, a); break;
+            case 3: hipLaunchKernelGGL((sp_apply_kernel<V, 3, true>), grid, dim3(256), 0, str, a); break;
+            case 4: hipLaunchKernelGGL((sp_apply_kernel<V, 4, true>), grid, dim3(256), 0, str, a); break;
+            default: hipLaunchKernelGGL((sp_apply_kernel<V, 0, true>), grid, dim3(256), 0, str, a); break;
+        }
+        return;
+    }
     switch (a.sp.d) {
-        case 1: hipLaunchKernelGGL((sp_apply_kernel<V, 1>), grid, dim3(256), 0, str, a); break;
-        case 2: hipLaunchKernelGGL((sp_apply_kernel<V, 2>), grid, dim3(256), 0, str, a); break;
-        case 3: hipLaunchKernelGGL((sp_apply_kernel<V, 3>), grid, dim3(256), 0, str, a); break;
-        case 4: hipLaunchKernelGGL((sp_apply_kernel<V, 4>), grid, dim3(256), 0, str, a); break;
-        default: hipLaunchKernelGGL((sp_apply_kernel<V, 0>), grid, dim3(256), 0, str, a); break;
+        case 1: hipLaunchKernelGGL((sp_apply_kernel<V, 1, false>), grid, dim3(256), 0, str, a); break;
+        case 2: hipLaunchKernelGGL((sp_apply_kernel<V, 2, false>), grid, dim3(256), 0, str, a); break;
+        case 3: hipLaunchKernelGGL((sp_apply_kernel<V, 3, false>), grid, dim3(256), 0, str, a); break;
+        case 4: hipLaunchKernelGGL((sp_apply_kernel<V, 4, false>), grid, dim3(256), 0, str, a); break;
+        default: hipLaunchKernelGGL((sp_apply_kernel<V, 0, false>), grid, dim3(256), 0, str, a); break;
     }
 }
 template <typename V>
-void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, const V* cellCoef, int64_t ccStride, V* z,
-                    int64_t zStride, int batch, hipStream_t str) {
+void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, const V* cellCoef, int64_t ccStride,
+                    const V* cellMat, int64_t matStride, int64_t nCells, V* z, int64_t zStride, int batch,
+                    hipStream_t str) {
     hipLaunchKernelGGL(sp_diag_kernel<V>, dim3(gridOf(n, 256), batch), dim3(256), 0, str, s, activeDev, n, cellCoef,
-                       ccStride, z, zStride);
+                       ccStride, cellMat, matStride, nCells, z, zStride);
 }
 
 #define LSG_SP_INST(V)                                                                                              \
@@ -403,8 +485,8 @@ void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, const V
     template void launch_sp_prol<V>(const ProlItem*, int, const SparseDev&, const V*, int64_t, V*, int64_t, int,    \
                                     const CGState*, hipStream_t, const int*);                                       \
     template void launch_sp_apply<V>(const SparseApplyArgs<V>&, int, hipStream_t);                                  \
-    template void launch_sp_diag<V>(const SparseDev&, const uint8_t*, int, const V*, int64_t, V*, int64_t, int,      \
-                                    hipStream_t);
+    template void launch_sp_diag<V>(const SparseDev&, const uint8_t*, int, const V*, int64_t, const V*, int64_t,     \
+                                    int64_t, V*, int64_t, int, hipStream_t);
 LSG_SP_INST(double)
 LSG_SP_INST(float)
 #undef LSG_SP_INST

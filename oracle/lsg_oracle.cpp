@@ -345,6 +345,16 @@ struct Oracle {
     std::vector<std::vector<std::array<int32_t, MAXD>>> cellLo;
     std::vector<std::vector<std::array<double, MAXD>>> cellCtr;
     std::vector<std::vector<std::vector<int32_t>>> levelNodes;   // per depth, per dim, position -> index
+    // Stored element matrices (LocalStiffnessMatrices, LocalStiffnessMatrices.cpp:453-531),
+    // when set: per depth, per cell of cellLo, the 2^(d-1) (2^d + 1) entries of corners p <= q
+    // in the order of the nested CellIndexIterator loops (cells/iterator_neu.cpp:34-49)
+    std::vector<std::vector<double>> mats;
+    std::vector<std::unordered_map<Key, int64_t, KeyHash>> cellOf;   // per depth: lower position -> cell
+    int64_t nEnt() const { return (int64_t(1) << (g.d - 1)) * ((int64_t(1) << g.d) + 1); }
+    static int entryOf(unsigned p, unsigned q, unsigned nc) {
+        if (p > q) std::swap(p, q);
+        return (int)(p * nc - (p * (p - 1)) / 2 + (q - p));
+    }
 
     void init() {
         g.build();
@@ -353,6 +363,12 @@ struct Oracle {
         cellLo.assign(nd, {}); cellCtr.assign(nd, {}); levelNodes.assign(nd, {});
         for (size_t t = 0; t < nd; ++t) {
             cellsOfDepth(g, g.depths[t], cellLo[t], cellCtr[t]);
+            cellOf.emplace_back();
+            for (size_t c = 0; c < cellLo[t].size(); ++c) {
+                Key K{};
+                for (int i = 0; i < g.d; ++i) K.v[i] = cellLo[t][c][i];
+                cellOf[t][K] = (int64_t)c;
+            }
             levelNodes[t].resize(g.d);
             for (int i = 0; i < g.d; ++i) {
                 int L = g.depths[t][i];
@@ -522,7 +538,8 @@ struct Oracle {
                 if (!occ[p]) continue;
                 for (unsigned q = p; q < nc; ++q) {
                     if (!occ[q]) continue;
-                    double val = op.element(d, h, cellCtr[ti][c].data(), p, q);
+                    double val = mats.empty() ? op.element(d, h, cellCtr[ti][c].data(), p, q)
+                                              : mats[ti][c * nEnt() + entryOf(p, q, nc)];
                     if (p == q) z[ks[p]] += val * u[ks[p]];
                     else { z[ks[q]] += val * u[ks[p]]; z[ks[p]] += val * u[ks[q]]; }
                 }
@@ -648,11 +665,18 @@ struct Oracle {
             if (!ok) continue;
             double ctr[MAXD];
             for (int i = 0; i < d; ++i) ctr[i] = (double(lo[i]) + 0.5) * h[i];
+            int64_t cell = -1;
+            if (!mats.empty()) {
+                Key L{};
+                for (int i = 0; i < d; ++i) L.v[i] = lo[i];
+                cell = cellOf[ti].at(L);
+            }
             for (unsigned q = 0; q < nc; ++q) {
                 if (!interior[q]) continue;
                 auto it = v.find(Ks[q]);
                 if (it == v.end()) continue;
-                val += op.element(d, h, ctr, pj, q) * it->second;
+                const double e = cell < 0 ? op.element(d, h, ctr, pj, q) : mats[ti][cell * nEnt() + entryOf(pj, q, nc)];
+                val += e * it->second;
             }
         }
         return val;
@@ -1085,6 +1109,38 @@ int orc_set_coef(void* h, int kind, double a0, double c0, const double* xi, int 
 }
 
 int orc_num_depths(void* h) { return (int)((orc::Oracle*)h)->g.depths.size(); }
+
+// Cells of depth T+1 of every depth (depth-list order, cellsOfDepth order within a depth):
+// level-T boxes prod_i [pos_i 2^-T_i, (pos_i + 1) 2^-T_i].  Returns the count.
+int64_t orc_cell_boxes(void* h, int32_t* level, int32_t* pos) {
+    auto* o = (orc::Oracle*)h;
+    int64_t k = 0;
+    for (size_t t = 0; t < o->g.depths.size(); ++t)
+        for (const auto& lo : o->cellLo[t]) {
+            for (int i = 0; i < o->g.d; ++i) {
+                if (level) level[k * o->g.d + i] = o->g.depths[t][i];
+                if (pos) pos[k * o->g.d + i] = lo[i];
+            }
+            ++k;
+        }
+    return k;
+}
+
+// Stored element matrices for the cells of orc_cell_boxes (ncells x 2^(d-1) (2^d + 1));
+// NULL returns to the coefficient's element matrices.
+int orc_set_cell_matrices(void* h, const double* entries) {
+    auto* o = (orc::Oracle*)h;
+    o->mats.clear();
+    if (!entries) return 0;
+    const int64_t ne = o->nEnt();
+    int64_t k = 0;
+    for (size_t t = 0; t < o->g.depths.size(); ++t) {
+        const int64_t nc = (int64_t)o->cellLo[t].size();
+        o->mats.emplace_back(entries + k * ne, entries + (k + nc) * ne);
+        k += nc;
+    }
+    return 0;
+}
 
 void orc_depths(void* h, int32_t* out) {
     auto* o = (orc::Oracle*)h;

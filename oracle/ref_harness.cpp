@@ -23,7 +23,7 @@
 //   ref_harness timesolve <level> <coef> [eps]           (the driver's whole PCG solve timed, JSON)
 // coef: const:<a0>:<c0> | cfg2 | cfg3 | expsin:<xi1>,<xi2>,... | gen
 //       (gen: a = 1 + x1 x2 + sin(pi (x1 + x2)), c = 1 + sin(pi x1 x2) / 2, not separable)
-// what: matvec precond cg transform   (default: all)
+// what: matvec precond cg transform stored   (default: all)
 #include "source/sparseEXPDE.h"
 #include "ref_plugin.hpp"
 
@@ -121,6 +121,47 @@ void runAll(AdaptiveSparseGrid& grid, const CoefSpec& cs, const std::string& out
         { Preconditioning<VarCoeffStencil> P2(z2, var); }
         writeVec(out, "diag_poisson", z1, n);
         writeVec(out, "diag_var", z2, n);
+    }
+    if (want(what, "stored")) {
+        // Stored element matrices (LocalStiffnessMatrices, built by the plug-in's
+        // integration() once per cell) applied by MatrixVectorHomogen::multiplication's
+        // StoreLocalStiffnessMatrix branch (MatrixVectorHomogen.h:753-756,170-245), and the
+        // same plug-in applied on the fly (applyStencilOnCell, Stencil.h:187-246).
+        QuadStencil qs(grid, cs);
+        LocalStiffnessMatricesMemoryDistribution lsm(grid, qs, 1);
+        y = 0.0; m.multiplication(x, y, lsm); writeVec(out, "Ax_stored", y, n);
+        y = 0.0; m.multiplication(x, y, qs);  writeVec(out, "Ax_stored_otf", y, n);
+        // Preconditioning::precondLocalStiffnessMatrix (cg_method.h:544-641) for this
+        // process: D_p = psi_p^T A_T psi_p, A_T applied by the stored matrices of depth
+        // T = depth(p) on the prewavelet stencil of p.  (The reference's copy compares an
+        // uninitialised world_rank with the owning node when MPI is off; here the single
+        // process owns every depth.  The work vectors are cleared at hanging nodes too, so
+        // every D_p is the definition, without values left over from earlier points.)
+        VectorSparseG z(grid), v(grid), vout(grid);
+        z = 0.0;
+        grid.WorkOnHangingNodes = true;
+        v = 0.0; vout = 0.0;
+        for (size_t i = 0; i < n; ++i) {
+            if (!act[i]) continue;
+            IndexDimension I = grid.getIndexOfTable(i);
+            Depth T(I);
+            std::vector<IndexDimension> Js;
+            std::vector<double> cs5;
+            for (MultiDimFiveCompass mc; mc.goon(); ++mc) {
+                double bc = 1.0;
+                bool next = true;
+                IndexDimension J = I.nextFive(&mc, T, &bc, &next);
+                if (next) { v.setValue(J, bc); Js.push_back(J); cs5.push_back(bc); }
+            }
+            lsm.applyLocalStiffnessMatricesOnIndices_onNode(v, vout, T, Js);
+            double coeff = 0.0;
+            for (size_t j = 0; j < Js.size(); ++j) coeff += vout.getValue(Js[j]) * cs5[j];
+            v = 0.0;
+            vout = 0.0;
+            z.setValue((unsigned long)i, coeff);
+        }
+        grid.WorkOnHangingNodes = false;
+        writeVec(out, "diag_stored", z, n);
     }
     if (want(what, "cg")) {
         // RHS in the reference driver's convention f_h = M * prew(f_I)

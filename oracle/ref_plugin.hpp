@@ -102,4 +102,50 @@ public:
     CoefSpec spec;
 };
 
+// Element matrices by quadrature, for the stored-matrix path: integration(cell, dirP, dirQ)
+// = int_cell a(x) grad phi_p . grad phi_q + c(x) phi_p phi_q with the tensor 2-point
+// Gauss-Legendre rule (nodes m_i +- h_i / (2 sqrt 3), weights h_i / 2).  a and c vary inside
+// the cell, so the matrices are not a(centre) Poisson + c(centre) HelmHoltz.
+class QuadStencil : public StencilTemplate {
+public:
+    QuadStencil(AdaptiveSparseGrid& g, const CoefSpec& cs) : StencilTemplate(g), spec(cs) {}
+    TypeMatrixVectorMultiplication getTypeMatrixVectorMultiplication() { return StencilOnTheFly; }
+    inline double integration(CellDimension& cell, CellIndexDirection& dirP, CellIndexDirection& dirQ) {
+        const int d = DimensionSparseGrid;
+        double lo[DimensionSparseGrid], h[DimensionSparseGrid];
+        bool rp[DimensionSparseGrid], rq[DimensionSparseGrid];
+        for (int i = 0; i < d; ++i) {
+            lo[i] = cell.getLeftBoundary(i);
+            h[i] = cell.getRightBoundary(i) - lo[i];
+            rp[i] = dirP.getDirection(i) == Right;
+            rq[i] = dirQ.getDirection(i) == Right;
+        }
+        const double g = 0.5 / std::sqrt(3.0);
+        double sum = 0.0;
+        for (int m = 0; m < (1 << d); ++m) {
+            double x[DimensionSparseGrid], fp[DimensionSparseGrid], fq[DimensionSparseGrid];
+            double w = 1.0;
+            for (int i = 0; i < d; ++i) {
+                const double t = ((m >> i) & 1) ? 0.5 + g : 0.5 - g;   // local coordinate in [0, 1]
+                x[i] = lo[i] + t * h[i];
+                fp[i] = rp[i] ? t : 1.0 - t;
+                fq[i] = rq[i] ? t : 1.0 - t;
+                w *= 0.5 * h[i];
+            }
+            double mass = 1.0;
+            for (int i = 0; i < d; ++i) mass *= fp[i] * fq[i];
+            double stiff = 0.0;
+            for (int i = 0; i < d; ++i) {
+                double prod = (rp[i] == rq[i] ? 1.0 : -1.0) / (h[i] * h[i]);
+                for (int j = 0; j < d; ++j)
+                    if (j != i) prod *= fp[j] * fq[j];
+                stiff += prod;
+            }
+            sum += w * (spec.a(x) * stiff + spec.c(x) * mass);
+        }
+        return sum;
+    }
+    CoefSpec spec;
+};
+
 #endif

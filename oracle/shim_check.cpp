@@ -11,7 +11,11 @@
 // and H1 / L2 ::getValue (applications/H1_Norm.cpp:8-78), and for a general coefficient:
 // the reference with the plug-in of ref_plugin.hpp (a(centre) Poisson + c(centre)
 // HelmHoltz per cell, non-separable a and c), the drop-in with
-// lsg_shim::Coefficient::functions of the same a and c.
+// lsg_shim::Coefficient::functions of the same a and c.  Stored element matrices:
+// LocalStiffnessMatricesMemoryDistribution of the quadrature plug-in (QuadStencil) on the
+// reference side, lsg_shim::LocalStiffnessMatrices of the same plug-in on the drop-in side
+// (matvec; addStencil(Poisson) against the reference's two products added; CG checked by
+// the reference's stored-matrix residual of the drop-in's solution).
 //
 // Usage: shim_check <level>      prints one JSON line; exit status 0 when
 //   matvecs agree to 1e-12 (relative, max norm), CG iteration counts are equal and
@@ -108,13 +112,47 @@ int main(int argc, char** argv) {
     lsg_shim::CG::solveHomogen(1e-8, solDevG, f, &itDevG, usol, dev, fun, &tpDev);
     const double cgRelG = relDiff(solRefG, solDevG, n);
 
+    // stored element matrices of a plug-in whose coefficient varies inside each cell
+    QuadStencil quad(grid, gen);
+    LocalStiffnessMatricesMemoryDistribution lsmRef(grid, quad, 1);
+    lsg_shim::LocalStiffnessMatrices lsmDev(grid, quad, 1);
+    ref.multiplication(x, yRef, lsmRef);
+    dev.multiplication(x, yDev, lsmDev);
+    const double mvStored = relDiff(yRef, yDev, n);
+    VectorSparseG yP(grid);
+    ref.multiplication(x, yP, poisson);
+    for (size_t i = 0; i < n; ++i) yRef.setValue((unsigned long)i, yRef.getValue((unsigned long)i) + yP.getValue((unsigned long)i));
+    lsmDev.addStencil(poisson);
+    dev.multiplication(x, yDev, lsmDev);
+    const double mvStoredAdd = relDiff(yRef, yDev, n);
+    // CG with eps = 1e-24 ||f||^2: the reference's loop runs while r^T D^-1 r > eps
+    // (cg_method.h:852), so ||r|| / ||f|| ends near 1e-12 sqrt(max D), and the residual of the
+    // solution under the reference's stored-matrix operator stays of that order unless the
+    // two operators differ
+    VectorSparseG solS(grid), rS(grid);
+    solS = 0.0;
+    int itS = 0;
+    double fn = 0.0;
+    for (size_t i = 0; i < n; ++i) fn += f.getValue((unsigned long)i) * f.getValue((unsigned long)i);
+    lsg_shim::LocalStiffnessMatrices lsmSolve(grid, quad, 1);
+    lsg_shim::CG::solveHomogen(1e-24 * fn, solS, f, &itS, usol, dev, lsmSolve, &tpDev);
+    ref.multiplication(solS, rS, lsmRef);
+    double rn = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const double r = f.getValue((unsigned long)i) - rS.getValue((unsigned long)i);
+        rn += r * r;
+    }
+    const double cgStoredRes = std::sqrt(rn / fn);
+
     const bool ok = mvPoisson < 1e-12 && mvHelm < 1e-12 && itRef == itDev && cgRel < 1e-10 && trPrew < 1e-12 &&
-                    trNodal < 1e-12 && normRel < 1e-12 && mvGen < 1e-12 && itRefG == itDevG && cgRelG < 1e-10;
+                    trNodal < 1e-12 && normRel < 1e-12 && mvGen < 1e-12 && itRefG == itDevG && cgRelG < 1e-10 &&
+                    mvStored < 1e-12 && mvStoredAdd < 1e-12 && cgStoredRes < 1e-9;
     std::printf("{\"d\": %d, \"level\": %d, \"n\": %zu, \"matvec_poisson_rel\": %.3e, \"matvec_helm_rel\": %.3e, "
                 "\"cg_iters_ref\": %d, \"cg_iters_dev\": %d, \"cg_x_rel\": %.3e, \"prew_by_nodal_rel\": %.3e, "
                 "\"nodal_by_prew_rel\": %.3e, \"norms_rel\": %.3e, \"matvec_gen_rel\": %.3e, "
-                "\"cg_gen_iters_ref\": %d, \"cg_gen_iters_dev\": %d, \"cg_gen_x_rel\": %.3e, \"ok\": %s}\n",
+                "\"cg_gen_iters_ref\": %d, \"cg_gen_iters_dev\": %d, \"cg_gen_x_rel\": %.3e, \"matvec_stored_rel\": %.3e, "
+                "\"matvec_stored_add_rel\": %.3e, \"cg_stored_iters\": %d, \"cg_stored_ref_residual\": %.3e, \"ok\": %s}\n",
                 DimensionSparseGrid, level, n, mvPoisson, mvHelm, itRef, itDev, cgRel, trPrew, trNodal, normRel, mvGen,
-                itRefG, itDevG, cgRelG, ok ? "true" : "false");
+                itRefG, itDevG, cgRelG, mvStored, mvStoredAdd, itS, cgStoredRes, ok ? "true" : "false");
     return ok ? 0 : 1;
 }

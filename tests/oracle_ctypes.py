@@ -22,6 +22,46 @@ def gen_coef(x):
     x1, x2 = x[:, 0], x[:, 1 % x.shape[1]]
     return 1.0 + x1 * x2 + np.sin(np.pi * (x1 + x2)), 1.0 + 0.5 * np.sin(np.pi * x1 * x2)
 
+
+def quad_element_matrices(level, pos, coef=gen_coef):
+    """Element matrices of the harness's quadrature plug-in (oracle/ref_plugin.hpp,
+    QuadStencil): entry (p, q) of cell k = int_cell a grad phi_p . grad phi_q + c phi_p phi_q
+    by the tensor 2-point Gauss-Legendre rule, for the cells prod_i [pos_i 2^-level_i,
+    (pos_i + 1) 2^-level_i] (arrays ncells x d).  Returns ncells x 2^(d-1) (2^d + 1) entries
+    in the order of the nested CellIndexIterator loops (corners p <= q, bit i = upper end)."""
+    level = np.asarray(level, dtype=np.int64)
+    pos = np.asarray(pos, dtype=np.float64)
+    K, d = level.shape
+    h = np.ldexp(1.0, -level)
+    lo = pos * h
+    g = 0.5 / np.sqrt(3.0)
+    nc = 1 << d
+    pairs = [(p, q) for p in range(nc) for q in range(p, nc)]
+    E = np.zeros((K, len(pairs)))
+    for m in range(nc):
+        t = np.array([0.5 + g if (m >> i) & 1 else 0.5 - g for i in range(d)])
+        x = lo + t * h
+        a, c = coef(x)
+        w = np.ones(K)
+        for i in range(d):
+            w = w * (0.5 * h[:, i])
+        for e, (p, q) in enumerate(pairs):
+            fp = [t[i] if (p >> i) & 1 else 1.0 - t[i] for i in range(d)]
+            fq = [t[i] if (q >> i) & 1 else 1.0 - t[i] for i in range(d)]
+            mass = 1.0
+            for i in range(d):
+                mass = mass * (fp[i] * fq[i])
+            stiff = np.zeros(K)
+            for i in range(d):
+                prod = (1.0 if ((p >> i) & 1) == ((q >> i) & 1) else -1.0) / (h[:, i] * h[:, i])
+                for j in range(d):
+                    if j != i:
+                        prod = prod * (fp[j] * fq[j])
+                stiff = stiff + prod
+            E[:, e] += w * (a * stiff + c * mass)
+    return E
+
+
 _lib = None
 
 
@@ -48,6 +88,9 @@ def lib():
         L.orc_cg.argtypes = [ctypes.c_void_p, ctypes.c_double, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
                              ctypes.c_void_p, ctypes.c_int]
         L.orc_transform.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_cell_boxes.restype = ctypes.c_int64
+        L.orc_cell_boxes.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_set_cell_matrices.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -85,6 +128,25 @@ class Oracle:
     def set_coef(self, kind, a0=1.0, c0=0.0, xi=None):
         xi = np.zeros(0) if xi is None else np.ascontiguousarray(xi, dtype=np.float64)
         lib().orc_set_coef(self.h, kind, a0, c0, _p(xi) if xi.size else None, int(xi.size))
+        return self
+
+    def cell_boxes(self):
+        """(level, pos) of the oracle's cells of depth T+1 (ncells x d each)."""
+        n = lib().orc_cell_boxes(self.h, None, None)
+        lev = np.zeros((n, self.d), dtype=np.int32)
+        pos = np.zeros((n, self.d), dtype=np.int32)
+        lib().orc_cell_boxes(self.h, _p(lev), _p(pos))
+        return lev, pos
+
+    def set_cell_matrices(self, E):
+        """Stored element matrices for the cells of cell_boxes() (None: back to the
+        coefficient's element matrices)."""
+        if E is None:
+            self._mats = None
+            lib().orc_set_cell_matrices(self.h, None)
+        else:
+            self._mats = np.ascontiguousarray(E, dtype=np.float64)
+            lib().orc_set_cell_matrices(self.h, _p(self._mats))
         return self
 
     def depths(self):

@@ -37,6 +37,7 @@ def test_shim_links_cpu():
 @pytest.mark.parametrize("d,level", [(3, 6), (5, 4)])
 def test_shim_vs_reference(d, level):
     rc, res = _run(d, level)
+    print(json.dumps(res), flush=True)
     assert rc == 0, res
     assert res["matvec_poisson_rel"] < 1e-12 and res["matvec_helm_rel"] < 1e-12
     assert res["cg_iters_ref"] == res["cg_iters_dev"]
