@@ -450,6 +450,8 @@ def main():
                          "GPU-generated solves); 0 = zero start")
     ap.add_argument("--warm-steps", type=int, default=96000)
     ap.add_argument("--workload", choices=("d5l7", "cfg1", "cfg2", "cfg3", "cfg4", "transform"), default="d5l7")
+    ap.add_argument("--precond", choices=("block", "diag"), default="block",
+                    help="preconditioner of the inner FP32 solves: depth-block Jacobi or the reference's diagonal")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (batched solves shard by instances across GPUs)",
@@ -483,6 +485,8 @@ def main():
     st0 = L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[0])
     op64 = L.MatrixVectorHomogen(grid, st0, precision=64, batch=B)
     op32 = L.MatrixVectorHomogen(grid, st0, precision=32, batch=B)
+    inner_pre = L.PRECOND_DEPTH_BLOCK if args.precond == "block" else L.PRECOND_DIAG
+    op32.set_precond(inner_pre)
     f = torch.zeros(B, n, dtype=torch.float64, device=dev)
     x = torch.zeros(B, n, dtype=torch.float64, device=dev)
 
@@ -577,10 +581,18 @@ def main():
         infos64 = [infos64]
     log("batched FP64 PCG: %.3f s" % dt64)
 
-    # ---- zero start over the held-out instances (same steps, reported beside the value)
+    # ---- zero start over the held-out instances (same steps, reported beside the value), and
+    # one zero-start step with the reference's diagonal in the inner solves
     solve_step(-1, False)
     dt0, infos0 = timed_pass(False, args.steps)
     log("zero-start mixed-precision solves: %.3f s per step" % dt0)
+    dtd, infosd = None, []
+    if inner_pre != L.PRECOND_DIAG:
+        op32.set_precond(L.PRECOND_DIAG)
+        solve_step(-1, False)
+        dtd, infosd = timed_pass(False, 1)
+        op32.set_precond(inner_pre)
+        log("zero-start mixed-precision solves, diagonal inner preconditioner: %.3f s per step" % dtd)
 
     # ---- timed: batched mixed-precision solves of fresh instances (warm-started when trained)
     for _ in range(args.warmup):
@@ -641,8 +653,9 @@ def main():
         "dtype": "f32 inner / f64 refinement",
         "data": "synthetic (seeded PCG64 %d)" % SEED,
         "config": {"workload": "d=5 regular sparse grid level 7 (N=%d), var. coefficient exp-sine (16 xi), "
-                               "8-term sine RHS, batch %d, FP32 PCG under FP64 IR to 1e-8%s" %
-                               (n, B, ", MLP warm start" if ws is not None else ""),
+                               "8-term sine RHS, batch %d, FP32 PCG (%s preconditioner) under FP64 IR to 1e-8%s" %
+                               (n, B, "depth-block" if inner_pre != L.PRECOND_DIAG else "diagonal",
+                                ", MLP warm start" if ws is not None else ""),
                    "batch": B, "N": n, "inner_rel_tol": args.inner_tol},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK,
@@ -671,6 +684,11 @@ def main():
             "warm_start_gain": dt0 / dt,
             "zero_start_inner_iterations_mean": float(np.mean([i["iterations"] for i in infos0])),
             "zero_start_outer_sweeps_mean": float(np.mean([i["outer"] for i in infos0])),
+            "inner_preconditioner": "depth-block Jacobi (constant-coefficient Kronecker blocks scaled by the "
+                                    "diagonal)" if inner_pre != L.PRECOND_DIAG else "diagonal (precond3)",
+            "zero_start_diag_inner_solves_per_s": None if dtd is None else B / dtd,
+            "zero_start_diag_inner_iterations_mean": (float(np.mean([i["iterations"] for i in infosd]))
+                                                      if infosd else None),
             "warm_start": None if ws is None else dict(ws.stats, xi_samples=args.warm_xi,
                                                       mlp="24-512-512-512-%d GELU, BF16 MFMA" % n),
             "max_rel_err_vs_fp64_solve": err,

@@ -187,6 +187,19 @@ int lsg_grid_destroy(lsg_grid* g);
 int lsg_op_create(lsg_grid* g, const lsg_coef* coef, int precision, int batch, lsg_op** out);
 int lsg_op_set_coef(lsg_op* op, int instance, const lsg_coef* coef);
 int lsg_op_batch(const lsg_op* op);
+/* Preconditioner of lsg_cg_solve and of the inner FP32 solves of lsg_ir_solve on this
+ * operator.  LSG_PRECOND_DIAG (default): the reference's diagonal precond3
+ * (applications/cg_method.h:322-451), so that lsg_cg_solve reproduces CG::solveHomogen.
+ * LSG_PRECOND_DEPTH_BLOCK: no counterpart in the reference -- block Jacobi over the depth
+ * blocks, each block the Galerkin block of the constant-coefficient Laplacian on the
+ * prewavelets of that depth (a sum of Kronecker products of 1D prewavelet stiffness and mass
+ * matrices, inverted once per grid), scaled symmetrically by sqrt(D0 / D) with D this
+ * operator's diagonal and D0 that of the constant-coefficient blocks; blocks above 128
+ * points on FP32 operators (64 on FP64 ones) keep the diagonal.  Dense grids only (LSG_ERR_UNSUPPORTED on the cell-wise path).
+ * Solutions meet the same stopping rules; iteration counts differ from the reference's. */
+#define LSG_PRECOND_DIAG 0
+#define LSG_PRECOND_DEPTH_BLOCK 1
+int lsg_op_set_precond(lsg_op* op, int kind);
 /* y = A x: MatrixVectorHomogen::multiplication(prew, Ax, stencil)
  * (MatrixVectorHomogen.h:660-759), for every batch instance (x and y hold batch x n
  * values in either layout). */

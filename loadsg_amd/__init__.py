@@ -34,6 +34,7 @@ CG_NO_PRECOND = 0x100     # or-ed into stop_rule: plain CG without the diagonal 
 COEF_CONST, COEF_CFG2, COEF_CFG3, COEF_EXPSIN = 0, 1, 2, 3
 COEF_CELL, COEF_FUNC = 4, 5
 COEF_MATRIX, COEF_INTEGRATOR = 6, 7
+PRECOND_DIAG, PRECOND_DEPTH_BLOCK = 0, 1
 
 _lib = None
 
@@ -88,6 +89,7 @@ def lib():
             "lsg_op_create": (i32, [vp, ctypes.POINTER(_Coef), i32, i32, ctypes.POINTER(vp)]),
             "lsg_op_set_coef": (i32, [vp, i32, ctypes.POINTER(_Coef)]),
             "lsg_op_batch": (i32, [vp]),
+            "lsg_op_set_precond": (i32, [vp, i32]),
             "lsg_matvec": (i32, [vp, vp, vp, i32]),
             "lsg_precond_diag": (i32, [vp, vp, i32]),
             "lsg_cg_solve": (i32, [vp, vp, vp, dbl, i32, i32, vp, i32]),
@@ -496,6 +498,14 @@ class MatrixVectorHomogen:
 
     def set_block(self, threads):
         _check(lib().lsg_op_set_block(self._h, 0, threads))
+
+    def set_precond(self, kind):
+        """Preconditioner of CG.solveHomogen and of the inner solves of CG.solveMixed on this
+        operator: PRECOND_DIAG (the reference's precond3, default) or PRECOND_DEPTH_BLOCK
+        (block Jacobi over the depth blocks with constant-coefficient Galerkin blocks scaled
+        by the diagonal; dense grids only)."""
+        _check(lib().lsg_op_set_precond(self._h, kind))
+        return self
 
     # ---- argument checks: the C ABI takes raw pointers to batch x N values
     def _host(self, v, name="vector", out=False):

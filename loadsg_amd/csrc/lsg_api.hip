@@ -379,6 +379,12 @@ struct lsg_op {
     const void* chunkD = nullptr;   // diagonal the chunk divides by (D, or ones when unpreconditioned)
     int chunkK = 0;
     bool chunkX64 = false;
+    bool chunkPre = false;      // chunk captured with the depth-block preconditioner
+    // preconditioner of lsg_cg_solve and of the inner solves of lsg_ir_solve
+    // (LSG_PRECOND_DIAG: the reference's precond3; LSG_PRECOND_DEPTH_BLOCK: BlockPre)
+    int precKind = LSG_PRECOND_DIAG;
+    DevMem preItems, preInv, preD0, prePart;
+    int preNblk = 0, preMaxN = 0;
     bool capturing = false;   // matvec launches record no timing events while captured
     std::unique_ptr<TrDev> tr;   // basis transformations (built on first use)
     int64_t n() const { return G->n; }
@@ -718,7 +724,49 @@ struct CgBufs {
     double* part;
     CGState* st;
     bool own;
+    bool pre = false;     // depth-block preconditioner (own buffers only)
 };
+
+// Device tables of the depth-block preconditioner of a dense-grid operator (built once)
+int ensureBlockPre(lsg_op* op) {
+    if (op->preItems.p) return LSG_OK;
+    BlockPre bp;
+    std::string err;
+    // dense block inverses staged in LDS: up to 128 points in FP32 (64 KB), 64 in FP64 (32 KB)
+    if (int rc = buildBlockPre(*op->G, op->prec == 32 ? 128 : 64, bp, err)) return fail(rc, err);
+    op->preNblk = (int)bp.blocks.size();
+    op->preMaxN = bp.maxN;
+    HIPCHK(op->preItems.alloc(bp.blocks.size() * sizeof(PreBlock)));
+    HIPCHK(hipMemcpy(op->preItems.p, bp.blocks.data(), bp.blocks.size() * sizeof(PreBlock), hipMemcpyHostToDevice));
+    auto upload = [&](DevMem& m, const std::vector<double>& v) -> int {
+        HIPCHK(m.alloc(std::max<size_t>(1, v.size()) * op->vs));
+        if (v.empty()) return LSG_OK;
+        if (op->prec == 64) {
+            HIPCHK(hipMemcpy(m.p, v.data(), v.size() * 8, hipMemcpyHostToDevice));
+        } else {
+            std::vector<float> f(v.begin(), v.end());
+            HIPCHK(hipMemcpy(m.p, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+        }
+        return LSG_OK;
+    };
+    if (int rc = upload(op->preInv, bp.inv)) return rc;
+    if (int rc = upload(op->preD0, bp.d0)) return rc;
+    HIPCHK(op->prePart.alloc((size_t)op->batch * op->preNblk * sizeof(double)));
+    return LSG_OK;
+}
+
+template <typename V>
+BlockPreArgs<V> blockPreArgs(lsg_op* op, bool on) {
+    BlockPreArgs<V> a;
+    if (!on) return a;
+    a.items = op->preItems.as<PreBlock>();
+    a.nblk = op->preNblk;
+    a.maxN = op->preMaxN;
+    a.inv = op->preInv.as<V>();
+    a.d0 = op->preD0.as<V>();
+    a.part = op->prePart.as<double>();
+    return a;
+}
 
 template <typename V>
 int cgRun(lsg_op* op, const CgBufs& bf, const V* f, void* x, bool x64, double eps, int rule, int maxit,
@@ -750,6 +798,10 @@ int cgDevice(lsg_op* op, const V* f, void* x, bool x64, double eps, int rule, in
     if ((rc = ensureThreads(op))) return rc;
     CgBufs bf{op->r.p, op->h.p, op->dv.p, op->g.p, precond ? op->D.p : op->ones.p, op->partials.as<double>(),
               op->st.as<CGState>(), true};
+    if (precond && op->precKind == LSG_PRECOND_DEPTH_BLOCK) {
+        if ((rc = ensureBlockPre(op))) return rc;
+        bf.pre = true;
+    }
     return cgRun<V>(op, bf, f, x, x64, eps, rule, maxit, out, xZero, epsInst, open);
 }
 
@@ -798,8 +850,9 @@ int cgStart(CgPhase<V>& c, double eps, int rule, bool xZero, const double* epsIn
         } else xv = (const V*)c.x;
         if (int rc = runMatvec<V>(op, xv, r, nullptr)) return rc;
     }
-    launch_cg_init<V>(c.f, r, (V*)c.bf.h, (V*)c.bf.dv, (V*)c.bf.D, n, B, c.bf.part, nb, s);
-    launch_cg_start(c.bf.st, c.bf.part, nb, eps, epsInst, rule, c.maxit, B, s);
+    const BlockPreArgs<V> pre = blockPreArgs<V>(op, c.bf.pre);
+    launch_cg_init<V>(c.f, r, (V*)c.bf.h, (V*)c.bf.dv, (V*)c.bf.D, n, B, c.bf.part, nb, s, pre);
+    launch_cg_start(c.bf.st, c.bf.part, nb, eps, epsInst, rule, c.maxit, B, s, pre.part, pre.nblk);
     HIPCHK(hipGetLastError());
     c.out.resize(B);
     c.done = 0;
@@ -832,11 +885,13 @@ int cgIssueChunk(CgPhase<V>& c) {
     void* x = c.x;
     const bool x64 = c.x64;
     int rc = LSG_OK;
+    const BlockPreArgs<V> pre = blockPreArgs<V>(op, c.bf.pre);
     const int k = std::min(8, c.maxit - c.done);
     if (c.bf.own && op->nMap == 0 && useGraphs()) {
         // the chunk's launches depend only on the operator, x and k: capture once, then
         // replay (the kernels read the convergence state on the device)
-        if (!op->chunkExec || op->chunkX != x || op->chunkK != k || op->chunkX64 != x64 || op->chunkD != D) {
+        if (!op->chunkExec || op->chunkX != x || op->chunkK != k || op->chunkX64 != x64 || op->chunkD != D ||
+            op->chunkPre != c.bf.pre) {
             if (op->chunkExec) HIPCHK(hipGraphExecDestroy(op->chunkExec));
             op->chunkExec = nullptr;
             hipGraph_t graph = nullptr;
@@ -844,7 +899,7 @@ int cgIssueChunk(CgPhase<V>& c) {
             op->capturing = true;
             for (int q = 0; q < k && !rc; ++q) {
                 rc = runMatvec<V>(op, dv, g, st);
-                if (!rc) launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s);
+                if (!rc) launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s, pre);
             }
             op->capturing = false;
             const hipError_t ec = hipStreamEndCapture(s, &graph);
@@ -860,12 +915,13 @@ int cgIssueChunk(CgPhase<V>& c) {
             op->chunkK = k;
             op->chunkX64 = x64;
             op->chunkD = D;
+            op->chunkPre = c.bf.pre;
         }
         HIPCHK(hipGraphLaunch(op->chunkExec, s));
     } else {
         for (int q = 0; q < k; ++q) {
             if ((rc = runMatvec<V>(op, dv, g, st))) return rc;
-            launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s);
+            launch_cg_iter_tail<V>(r, x, x64, h, dv, g, D, n, B, part, nb, st, s, pre);
         }
     }
     c.lastK = k;
@@ -1068,6 +1124,10 @@ int irAfterResidual(IrGroup& g, double rel_tol, double inner_rel_tol, int maxit)
     if (rc) return rc;
     c.bf = CgBufs{g.op32->r.p, g.op32->h.p, g.op32->dv.p, g.op32->g.p, g.op32->D.p, g.op32->partials.as<double>(),
                   g.op32->st.as<CGState>(), true};
+    if (g.op32->precKind == LSG_PRECOND_DEPTH_BLOCK) {
+        if ((rc = ensureBlockPre(g.op32))) return rc;
+        c.bf.pre = true;
+    }
     c.f = g.op32->f.as<float>();
     c.x = g.op32->x.p;
     c.x64 = false;
@@ -1792,6 +1852,17 @@ int lsg_op_set_coef(lsg_op* op, int instance, const lsg_coef* coef) {
 }
 
 int lsg_op_batch(const lsg_op* op) { return op ? op->batch : -1; }
+
+int lsg_op_set_precond(lsg_op* op, int kind) {
+    if (!op) return fail(LSG_ERR_ARG, "null operator");
+    if (kind != LSG_PRECOND_DIAG && kind != LSG_PRECOND_DEPTH_BLOCK) return fail(LSG_ERR_ARG, "unknown preconditioner");
+    if (kind == LSG_PRECOND_DEPTH_BLOCK) {
+        if (op->sparse) return fail(LSG_ERR_UNSUPPORTED, "the depth-block preconditioner needs complete depth blocks");
+        if (int rc = ensureBlockPre(op)) return rc;
+    }
+    op->precKind = kind;
+    return LSG_OK;
+}
 
 int lsg_op_set_block(lsg_op* op, int stage, int threads) {
     if (!op || stage != 0) return fail(LSG_ERR_ARG, "only stage 0 (apply) is tunable");

@@ -1181,21 +1181,24 @@ __device__ inline double sum_partials(const double* p, int nb) {
     return s;
 }
 
-// r = f - r ; h = r / D ; d = -h ; partials: r.h, r.r, f.f
+// r = f - r ; h = r / D ; d = -h ; partials: r.h, r.r, f.f.  pre: h, d and r.h are left to the
+// depth-block preconditioner (blockpre_kernel)
 template <typename V>
 __global__ void __launch_bounds__(256) cg_init_kernel(const V* __restrict__ f, V* __restrict__ r, V* __restrict__ h,
                                                       V* __restrict__ dv, const V* __restrict__ D, int64_t n,
-                                                      int64_t stride, double* __restrict__ partials) {
+                                                      int64_t stride, double* __restrict__ partials, int pre) {
     const int inst = blockIdx.y;
     double v[3] = {0.0, 0.0, 0.0};
     const int64_t o = inst * stride;
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         V ri = f[o + i] - r[o + i];
-        V hi = ri / D[o + i];
         r[o + i] = ri;
-        h[o + i] = hi;
-        dv[o + i] = V(-1) * hi;
-        v[0] += double(ri) * double(hi);
+        if (!pre) {
+            V hi = ri / D[o + i];
+            h[o + i] = hi;
+            dv[o + i] = V(-1) * hi;
+            v[0] += double(ri) * double(hi);
+        }
         v[1] += double(ri) * double(ri);
         v[2] += double(f[o + i]) * double(f[o + i]);
     }
@@ -1206,11 +1209,13 @@ __global__ void __launch_bounds__(256) cg_init_kernel(const V* __restrict__ f, V
 }
 
 __global__ void cg_start_scalar(CGState* st, const double* __restrict__ partials, int nb, double eps,
-                                const double* __restrict__ epsInst, int rule, int maxit) {
+                                const double* __restrict__ epsInst, int rule, int maxit,
+                                const double* __restrict__ bpart, int nbb) {
     const int inst = blockIdx.x;
     if (threadIdx.x != 0) return;
     CGState& s = st[inst];
-    s.delta = sum_partials(partials + ((int64_t)inst * 3 + 0) * nb, nb);
+    s.delta = bpart ? sum_partials(bpart + (int64_t)inst * nbb, nbb)
+                    : sum_partials(partials + ((int64_t)inst * 3 + 0) * nb, nb);
     s.rr = sum_partials(partials + ((int64_t)inst * 3 + 1) * nb, nb);
     s.ff = sum_partials(partials + ((int64_t)inst * 3 + 2) * nb, nb);
     if (epsInst) eps = epsInst[inst];
@@ -1237,12 +1242,13 @@ __global__ void cg_tau_scalar(CGState* st, const double* __restrict__ partials, 
     s.it += 1;
 }
 
-// r += tau g ; x -= tau d ; h = r / D ; partials r.h, r.r
+// r += tau g ; x -= tau d ; h = r / D ; partials r.h, r.r (pre: h and r.h by blockpre_kernel)
 template <typename V, typename X>
 __global__ void __launch_bounds__(256) cg_update_kernel(V* __restrict__ r, X* __restrict__ x, V* __restrict__ h,
                                                         const V* __restrict__ dv, const V* __restrict__ g,
                                                         const V* __restrict__ D, int64_t n, int64_t stride,
-                                                        double* __restrict__ partials, const CGState* __restrict__ st) {
+                                                        double* __restrict__ partials, const CGState* __restrict__ st,
+                                                        int pre) {
     const int inst = blockIdx.y;
     if (!st[inst].active) return;
     const V tau = V(st[inst].tau);
@@ -1251,10 +1257,12 @@ __global__ void __launch_bounds__(256) cg_update_kernel(V* __restrict__ r, X* __
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         V ri = r[o + i] + tau * g[o + i];
         x[o + i] = x[o + i] - X(tau) * X(dv[o + i]);
-        V hi = V(1) * ri / D[o + i];
         r[o + i] = ri;
-        h[o + i] = hi;
-        v[0] += double(ri) * double(hi);
+        if (!pre) {
+            V hi = V(1) * ri / D[o + i];
+            h[o + i] = hi;
+            v[0] += double(ri) * double(hi);
+        }
         v[1] += double(ri) * double(ri);
     }
     double out[2];
@@ -1263,12 +1271,14 @@ __global__ void __launch_bounds__(256) cg_update_kernel(V* __restrict__ r, X* __
         for (int q = 0; q < 2; ++q) partials[((int64_t)inst * 2 + q) * gridDim.x + blockIdx.x] = out[q];
 }
 
-__global__ void cg_beta_scalar(CGState* st, const double* __restrict__ partials, int nb) {
+__global__ void cg_beta_scalar(CGState* st, const double* __restrict__ partials, int nb,
+                               const double* __restrict__ bpart, int nbb) {
     const int inst = blockIdx.x;
     if (threadIdx.x != 0) return;
     CGState& s = st[inst];
     if (!s.active) return;
-    double dp = sum_partials(partials + ((int64_t)inst * 2 + 0) * nb, nb);
+    double dp = bpart ? sum_partials(bpart + (int64_t)inst * nbb, nbb)
+                      : sum_partials(partials + ((int64_t)inst * 2 + 0) * nb, nb);
     s.rr = sum_partials(partials + ((int64_t)inst * 2 + 1) * nb, nb);
     if (dp != dp) { s.active = 0; s.converged = -1; s.delta = dp; return; }
     bool stop = (s.rule == 0) ? (dp < s.eps * s.eps) : (s.rr <= s.eps * s.eps * s.ff);
@@ -1280,6 +1290,93 @@ __global__ void cg_beta_scalar(CGState* st, const double* __restrict__ partials,
     if (!cont || s.it >= s.maxit) {
         s.active = 0;
         s.converged = cont ? 0 : 1;
+    }
+}
+
+// Depth-block preconditioner (the inner solves of the mixed-precision method; no counterpart
+// in the reference, whose CG divides by the diagonal): workgroup (block T, group of PRE_IPW
+// instances); the block's inverse is staged in LDS once and applied by each wave to one
+// instance at a time: lane k forms h_k = s_k sum_j B^-1_kj s_j r_j for the points k = lane and
+// k = lane + 64 of a block of up to 128 points (x = s .* r broadcast from LDS, the inverse read
+// column-wise), s = sqrt(d0 / D).  Blocks without an inverse: h = r / D.  The r.h partial of
+// (instance, block) is summed over the wave in a fixed order.  init: also d = -h (the start of
+// a solve, every instance).
+constexpr int PRE_IPW = 16;
+constexpr int PRE_MAXN = 128;
+template <typename V, bool INIT>
+__global__ void __launch_bounds__(256) blockpre_kernel(const PreBlock* __restrict__ items, int nblk,
+                                                       const V* __restrict__ inv, const V* __restrict__ d0,
+                                                       const V* __restrict__ r, const V* __restrict__ D,
+                                                       V* __restrict__ h, V* __restrict__ dv, int64_t n, int batch,
+                                                       double* __restrict__ part, const CGState* __restrict__ st) {
+    extern __shared__ char smem_raw[];
+    const PreBlock it = items[blockIdx.x];
+    const int nl = it.n;
+    V* Bs = reinterpret_cast<V*>(smem_raw);
+    const bool dense = it.inv >= 0;
+    V* xs = Bs + (dense ? nl * nl : 0);
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (dense) {
+        for (int e = threadIdx.x; e < nl * nl; e += blockDim.x) Bs[e] = inv[(int64_t)it.inv + e];
+    }
+    __syncthreads();
+    V* x = xs + wave * PRE_MAXN;
+    const int i0 = blockIdx.y * PRE_IPW;
+    for (int q = 0; q < PRE_IPW; q += 4) {
+        const int inst = i0 + q + wave;
+        const bool run = inst < batch && (INIT || st[inst].active);
+        const int64_t o = (int64_t)inst * n + it.off;
+        double acc = 0.0;
+        if (dense) {
+            // nl <= 128: points lane and lane + 64
+            V rk[2] = {V(0), V(0)}, sk[2] = {V(0), V(0)};
+            if (run) {
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int k = lane + 64 * u;
+                    if (k < nl) {
+                        rk[u] = r[o + k];
+                        sk[u] = sqrt(d0[it.off + k] / D[o + k]);
+                        x[k] = sk[u] * rk[u];
+                    }
+                }
+            }
+            __syncthreads();
+            if (run) {
+                V a0 = V(0), a1 = V(0);
+                const bool two = lane + 64 < nl;
+                if (lane < nl) {
+                    for (int j = 0; j < nl; ++j) {
+                        const V xj = x[j];
+                        a0 = fma(Bs[j * nl + lane], xj, a0);
+                        if (two) a1 = fma(Bs[j * nl + lane + 64], xj, a1);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 2; ++u) {
+                    const int k = lane + 64 * u;
+                    if (k < nl) {
+                        const V hk = sk[u] * (u ? a1 : a0);
+                        h[o + k] = hk;
+                        if (INIT) dv[o + k] = V(-1) * hk;
+                        acc += double(rk[u]) * double(hk);
+                    }
+                }
+            }
+            __syncthreads();
+        } else if (run) {
+            for (int k = lane; k < nl; k += 64) {
+                const V rk = r[o + k];
+                const V hk = rk / D[o + k];
+                h[o + k] = hk;
+                if (INIT) dv[o + k] = V(-1) * hk;
+                acc += double(rk) * double(hk);
+            }
+        }
+        if (run) {
+            acc = wave_sum(acc);
+            if (lane == 0) part[(int64_t)inst * nblk + blockIdx.x] = acc;
+        }
     }
 }
 
@@ -1468,26 +1565,44 @@ void launch_rhs_tab(const SumItem* items, int nitems, int d, int nterms, const d
 }
 
 template <typename V>
+static void launch_blockpre(const BlockPreArgs<V>& pre, bool init, const V* r, const V* D, V* h, V* dv, int64_t n,
+                            int batch, const CGState* st, hipStream_t s) {
+    const dim3 grid((unsigned)pre.nblk, (unsigned)((batch + PRE_IPW - 1) / PRE_IPW));
+    const size_t lds = ((size_t)pre.maxN * pre.maxN + 4 * PRE_MAXN) * sizeof(V);
+    if (init)
+        hipLaunchKernelGGL((blockpre_kernel<V, true>), grid, dim3(256), lds, s, pre.items, pre.nblk, pre.inv, pre.d0, r,
+                           D, h, dv, n, batch, pre.part, st);
+    else
+        hipLaunchKernelGGL((blockpre_kernel<V, false>), grid, dim3(256), lds, s, pre.items, pre.nblk, pre.inv, pre.d0,
+                           r, D, h, dv, n, batch, pre.part, st);
+}
+template <typename V>
 void launch_cg_init(const V* f, V* r, V* h, V* dv, const V* D, int64_t n, int batch, double* partials, int nb,
-                    hipStream_t s) {
-    hipLaunchKernelGGL(cg_init_kernel<V>, dim3(nb, batch), dim3(256), 0, s, f, r, h, dv, D, n, n, partials);
+                    hipStream_t s, const BlockPreArgs<V>& pre) {
+    hipLaunchKernelGGL(cg_init_kernel<V>, dim3(nb, batch), dim3(256), 0, s, f, r, h, dv, D, n, n, partials,
+                       pre.items ? 1 : 0);
+    if (pre.items) launch_blockpre<V>(pre, true, r, D, h, dv, n, batch, nullptr, s);
 }
 void launch_cg_start(CGState* st, const double* partials, int nb, double eps, const double* epsInst, int rule,
-                     int maxit, int batch, hipStream_t s) {
-    hipLaunchKernelGGL(cg_start_scalar, dim3(batch), dim3(64), 0, s, st, partials, nb, eps, epsInst, rule, maxit);
+                     int maxit, int batch, hipStream_t s, const double* bpart, int nbb) {
+    hipLaunchKernelGGL(cg_start_scalar, dim3(batch), dim3(64), 0, s, st, partials, nb, eps, epsInst, rule, maxit,
+                       bpart, nbb);
 }
 template <typename V>
 void launch_cg_iter_tail(V* r, void* x, bool x64, V* h, V* dv, const V* g, const V* D, int64_t n, int batch,
-                         double* partials, int nb, CGState* st, hipStream_t s) {
+                         double* partials, int nb, CGState* st, hipStream_t s, const BlockPreArgs<V>& pre) {
+    const int pf = pre.items ? 1 : 0;
     hipLaunchKernelGGL((dot_kernel<V, V>), dim3(nb, batch), dim3(256), 0, s, dv, g, n, n, partials, st);
     hipLaunchKernelGGL(cg_tau_scalar, dim3(batch), dim3(64), 0, s, st, partials, nb);
     if (x64)
         hipLaunchKernelGGL((cg_update_kernel<V, double>), dim3(nb, batch), dim3(256), 0, s, r, (double*)x, h, dv, g, D,
-                           n, n, partials, st);
+                           n, n, partials, st, pf);
     else
         hipLaunchKernelGGL((cg_update_kernel<V, V>), dim3(nb, batch), dim3(256), 0, s, r, (V*)x, h, dv, g, D, n, n,
-                           partials, st);
-    hipLaunchKernelGGL(cg_beta_scalar, dim3(batch), dim3(64), 0, s, st, partials, nb);
+                           partials, st, pf);
+    if (pf) launch_blockpre<V>(pre, false, r, D, h, dv, n, batch, st, s);
+    hipLaunchKernelGGL(cg_beta_scalar, dim3(batch), dim3(64), 0, s, st, partials, nb, pf ? pre.part : nullptr,
+                       pre.nblk);
     hipLaunchKernelGGL(cg_dir_kernel<V>, dim3(nb, batch), dim3(256), 0, s, dv, h, n, n, st);
 }
 void launch_residual(const double* f, const double* Ax, double* r, int64_t n, int batch, double* partials, int nb,
@@ -1531,9 +1646,10 @@ int cg_partial_blocks(int64_t n) { return (int)gridFor(n); }
                                       int, hipStream_t, const int32_t*, const uint8_t*);                             \
     template void launch_rhs_tab<V>(const SumItem*, int, int, int, const double*, const double*, int64_t, V*,        \
                                     int64_t, int, hipStream_t, const int32_t*, const uint8_t*);                      \
-    template void launch_cg_init<V>(const V*, V*, V*, V*, const V*, int64_t, int, double*, int, hipStream_t);         \
+    template void launch_cg_init<V>(const V*, V*, V*, V*, const V*, int64_t, int, double*, int, hipStream_t,          \
+                                    const BlockPreArgs<V>&);                                                        \
     template void launch_cg_iter_tail<V>(V*, void*, bool, V*, V*, const V*, const V*, int64_t, int, double*, int,     \
-                                         CGState*, hipStream_t);
+                                         CGState*, hipStream_t, const BlockPreArgs<V>&);
 LSG_INST(double)
 LSG_INST(float)
 #undef LSG_INST

@@ -147,15 +147,30 @@ template <typename V>
 void launch_rhs_sines(const SumItem* items, int nitems, int d, int nterms, const double* eta, const int32_t* m, V* b,
                       int64_t bStride, int batch, hipStream_t s, const int32_t* pos = nullptr,
                       const uint8_t* active = nullptr);
+// Depth-block preconditioner of a CG solve (BlockPre, lsg_plan.hpp) on the device: h = s .*
+// (B_T^-1 (s .* r)) on every depth block T with an inverse, s = sqrt(d0 / D), h = r / D on the
+// others; part receives the r.h partial of every (instance, block).  items == null: the
+// diagonal preconditioner.
+template <typename V>
+struct BlockPreArgs {
+    const PreBlock* items = nullptr;
+    int nblk = 0;
+    int maxN = 0;           // largest block with an inverse (LDS of the kernel)
+    const V* inv = nullptr;
+    const V* d0 = nullptr;
+    double* part = nullptr;
+};
 template <typename V>
 void launch_cg_init(const V* f, V* r, V* h, V* dv, const V* D, int64_t n, int batch, double* partials, int nb,
-                    hipStream_t s);
-// epsInst: per-instance tolerances (device array) or null for eps
+                    hipStream_t s, const BlockPreArgs<V>& pre = BlockPreArgs<V>());
+// epsInst: per-instance tolerances (device array) or null for eps; bpart / nbb: r.h partials
+// of the depth-block preconditioner (null: slot 0 of partials)
 void launch_cg_start(CGState* st, const double* partials, int nb, double eps, const double* epsInst, int rule,
-                     int maxit, int batch, hipStream_t s);
+                     int maxit, int batch, hipStream_t s, const double* bpart = nullptr, int nbb = 0);
 template <typename V>
 void launch_cg_iter_tail(V* r, void* x, bool x64, V* h, V* dv, const V* g, const V* D, int64_t n, int batch,
-                         double* partials, int nb, CGState* st, hipStream_t s);
+                         double* partials, int nb, CGState* st, hipStream_t s,
+                         const BlockPreArgs<V>& pre = BlockPreArgs<V>());
 // r = f - Ax and the r.r, f.f partials for `batch` instances (slots map[0..batch) if map)
 void launch_residual(const double* f, const double* Ax, double* r, int64_t n, int batch, double* partials, int nb,
                      hipStream_t s, const int* map = nullptr);

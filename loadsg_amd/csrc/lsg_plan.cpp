@@ -19,6 +19,7 @@
 #include "../../include/loadsg_amd.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <unordered_map>
 #include <set>
@@ -831,6 +832,121 @@ int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid&
         if (int rc = buildSparse(g, tOf, err)) return rc;
     }
     return buildPlan(g, g.plan, g.sparse, err);
+}
+
+// 1D Galerkin matrices of the level-t prewavelets (odd positions k = 2j + 1 of the level-t
+// full grid, zero Dirichlet ends): K = Psi^T K_t Psi, M = Psi^T M_t Psi with the nodal
+// K_t = 2^t tridiag(-1, 2, -1) and M_t = 2^-t / 6 tridiag(1, 4, 1); m = 2^(t-1), row-major.
+static void prewavelet1DMatrices(int t, std::vector<double>& K, std::vector<double>& M) {
+    const int n = (1 << t) - 1, m = 1 << (t - 1);
+    std::vector<double> P((size_t)n * m, 0.0);
+    for (int j = 0; j < m; ++j) {
+        int32_t pos[5];
+        double c[5];
+        const int cnt = prewavelet1D(2 * j + 1, t, pos, c);
+        for (int e = 0; e < cnt; ++e) P[(size_t)(pos[e] - 1) * m + j] = c[e];
+    }
+    const double h = std::ldexp(1.0, -t);
+    K.assign((size_t)m * m, 0.0);
+    M.assign((size_t)m * m, 0.0);
+    for (int a = 0; a < m; ++a)
+        for (int b = 0; b < m; ++b) {
+            double ks = 0.0, ms = 0.0;
+            for (int q = 0; q < n; ++q) {
+                const double pa = P[(size_t)q * m + a];
+                if (pa == 0.0) continue;
+                // (K_t P)_(q, b) and (M_t P)_(q, b)
+                const double pq = P[(size_t)q * m + b];
+                const double pl = q > 0 ? P[(size_t)(q - 1) * m + b] : 0.0;
+                const double pr = q + 1 < n ? P[(size_t)(q + 1) * m + b] : 0.0;
+                ks += pa * (2.0 * pq - pl - pr) / h;
+                ms += pa * (4.0 * pq + pl + pr) * h / 6.0;
+            }
+            K[(size_t)a * m + b] = ks;
+            M[(size_t)a * m + b] = ms;
+        }
+}
+
+int buildBlockPre(const Grid& g, int maxDense, BlockPre& out, std::string& err) {
+    if (g.sparse) { err = "the depth-block preconditioner needs complete depth blocks"; return LSG_ERR_UNSUPPORTED; }
+    const int d = g.d;
+    const int nd = (int)g.depths.size();
+    out = BlockPre();
+    out.d0.assign((size_t)g.n, 0.0);
+    std::map<int, std::pair<std::vector<double>, std::vector<double>>> mats;   // level -> (K, M)
+    auto mat = [&](int t) -> const std::pair<std::vector<double>, std::vector<double>>& {
+        auto it = mats.find(t);
+        if (it == mats.end()) {
+            std::pair<std::vector<double>, std::vector<double>> km;
+            prewavelet1DMatrices(t, km.first, km.second);
+            it = mats.emplace(t, std::move(km)).first;
+        }
+        return it->second;
+    };
+    for (int t = 0; t < nd; ++t) {
+        const Depth& T = g.depths[t];
+        const int64_t nb = g.blkSize[t];
+        PreBlock pb{g.blkOff[t], (int32_t)nb, -1};
+        int m[MAXD];
+        for (int i = 0; i < d; ++i) m[i] = 1 << (T[i] - 1);
+        // local 1D indices of point p of the block (dimension 0 fastest)
+        auto local = [&](int64_t p, int i) {
+            for (int q = 0; q < i; ++q) p /= m[q];
+            return (int)(p % m[i]);
+        };
+        auto entry = [&](int64_t p, int64_t q) {
+            double s = 0.0;
+            for (int i = 0; i < d; ++i) {
+                const auto& Ki = mat(T[i]).first;
+                double prod = Ki[(size_t)local(p, i) * m[i] + local(q, i)];
+                for (int j = 0; j < d && prod != 0.0; ++j) {
+                    if (j == i) continue;
+                    const auto& Mj = mat(T[j]).second;
+                    prod *= Mj[(size_t)local(p, j) * m[j] + local(q, j)];
+                }
+                s += prod;
+            }
+            return s;
+        };
+        for (int64_t p = 0; p < nb; ++p) out.d0[(size_t)(pb.off + p)] = entry(p, p);
+        if (nb <= maxDense) {
+            // Cholesky factor of the block, then its inverse column by column
+            const int n = (int)nb;
+            std::vector<double> A((size_t)n * n), L((size_t)n * n, 0.0);
+            for (int a = 0; a < n; ++a)
+                for (int b = 0; b < n; ++b) A[(size_t)a * n + b] = entry(a, b);
+            for (int j = 0; j < n; ++j) {
+                double s = A[(size_t)j * n + j];
+                for (int q = 0; q < j; ++q) s -= L[(size_t)j * n + q] * L[(size_t)j * n + q];
+                if (!(s > 0.0)) { err = "depth block not positive definite"; return LSG_ERR_ARG; }
+                L[(size_t)j * n + j] = std::sqrt(s);
+                for (int i = j + 1; i < n; ++i) {
+                    double v = A[(size_t)i * n + j];
+                    for (int q = 0; q < j; ++q) v -= L[(size_t)i * n + q] * L[(size_t)j * n + q];
+                    L[(size_t)i * n + j] = v / L[(size_t)j * n + j];
+                }
+            }
+            pb.inv = (int32_t)out.inv.size();
+            out.inv.resize(out.inv.size() + (size_t)n * n);
+            double* inv = &out.inv[(size_t)pb.inv];
+            std::vector<double> y(n);
+            for (int k = 0; k < n; ++k) {   // column k of the inverse: L L^T x = e_k
+                for (int i = 0; i < n; ++i) {
+                    double v = (i == k) ? 1.0 : 0.0;
+                    for (int q = 0; q < i; ++q) v -= L[(size_t)i * n + q] * y[q];
+                    y[i] = v / L[(size_t)i * n + i];
+                }
+                for (int i = n - 1; i >= 0; --i) {
+                    double v = y[i];
+                    for (int q = i + 1; q < n; ++q) v -= L[(size_t)q * n + i] * inv[(size_t)k * n + q];
+                    inv[(size_t)k * n + i] = v / L[(size_t)i * n + i];
+                }
+            }
+            out.maxN = std::max(out.maxN, n);
+        }
+        out.blocks.push_back(pb);
+    }
+    return LSG_OK;
 }
 
 int buildSinglePlan(Grid& g, std::string& err) {

@@ -255,6 +255,25 @@ int prewaveletStencil1D(int32_t k, int t, int32_t pos[5], double c[5]);
 // Reference slot order of AddRecursiveSonsOfPoint(center, level).
 void regularIndexTable(int d, int level, std::vector<int32_t>& idx);
 
+// Depth-block preconditioner of the inner (mixed-precision) solves: for every depth block
+// T of a dense grid the Galerkin block of the constant-coefficient Laplacian on the
+// prewavelets of depth T, sum_i K_(T_i) (x) prod_(j != i) M_(T_j) with the 1D prewavelet
+// stiffness / mass matrices of level t (no counterpart in the reference, whose CG uses the
+// diagonal).  Blocks of at most maxDense points keep their inverse; larger ones fall back
+// to the diagonal.
+struct PreBlock {
+    int64_t off;      // first point of the block (device order)
+    int32_t n;        // points of the block
+    int32_t inv;      // first entry of its inverse in BlockPre::inv (n x n, column k contiguous); -1: diagonal
+};
+struct BlockPre {
+    std::vector<PreBlock> blocks;   // depth-list order
+    std::vector<double> inv;        // block inverses
+    std::vector<double> d0;         // diagonal of the constant-coefficient blocks, device order
+    int maxN = 0;                   // largest block with an inverse
+};
+int buildBlockPre(const Grid& g, int maxDense, BlockPre& out, std::string& err);
+
 inline int64_t boxCount(const int32_t* T, int d) {
     int64_t s = 1;
     for (int i = 0; i < d; ++i) s *= (int64_t(1) << T[i]) - 1;

@@ -25,7 +25,7 @@ import time
 
 import numpy as np
 
-from . import (COEF_EXPSIN, CG, MLP, MatrixVectorHomogen, Poisson, VariableCoefficient)
+from . import (COEF_EXPSIN, CG, MLP, PRECOND_DEPTH_BLOCK, MatrixVectorHomogen, Poisson, VariableCoefficient)
 
 
 class WarmStart:
@@ -46,9 +46,9 @@ class WarmStart:
         self.D = d[0].float()
 
     # -------------------------------------------------------------- data
-    def basis_solutions(self, xis, batch=64, rel_tol=1e-5, inner_rel_tol=1e-4):
+    def basis_solutions(self, xis, batch=64, rel_tol=1e-5, inner_rel_tol=1e-4, precond=PRECOND_DEPTH_BLOCK):
         """U[s, j] (float32, device order) for every xi sample s and mode j, from batched
-        mixed-precision solves to rel_tol."""
+        mixed-precision solves to rel_tol (inner solves with the preconditioner `precond`)."""
         import torch
         xis = np.asarray(xis, dtype=np.float64)
         S, J, n = len(xis), self.n_eta, self.grid.size
@@ -56,7 +56,7 @@ class WarmStart:
         B = per * J
         st0 = VariableCoefficient(COEF_EXPSIN, xi=xis[0])
         op64 = MatrixVectorHomogen(self.grid, st0, precision=64, batch=B)
-        op32 = MatrixVectorHomogen(self.grid, st0, precision=32, batch=B)
+        op32 = MatrixVectorHomogen(self.grid, st0, precision=32, batch=B).set_precond(precond)
         eta = np.tile(np.eye(J), (per, 1))   # instance b solves mode b % J
         f = torch.zeros(B, n, dtype=torch.float64, device="cuda")
         op64.rhs_sines(eta, self.modes, out=f)

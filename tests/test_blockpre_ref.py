@@ -1,0 +1,36 @@
+"""The CPU restatement of the depth-block preconditioner (tests/blockpre_ref.py) pinned on
+the CPU: its constant-coefficient Kronecker blocks are the diagonal depth blocks of the
+oracle's Poisson operator (a = 1), column by column; the restated PCG with it converges in
+fewer iterations than with the diagonal on the same system."""
+import numpy as np
+import pytest
+
+from blockpre_ref import constant_blocks, make_preconditioner, pcg
+from oracle_ctypes import COEF_CONST, COEF_EXPSIN, Oracle, regular_grid
+
+
+@pytest.mark.parametrize("d,level", [(2, 5), (3, 4), (4, 3)])
+def test_kronecker_blocks_are_the_poisson_depth_blocks(d, level):
+    idx, act = regular_grid(d, level)
+    o = Oracle(idx, act).set_coef(COEF_CONST, a0=1.0, c0=0.0)
+    n = len(idx)
+    rng = np.random.default_rng(d)
+    for T, (ids, B) in constant_blocks(idx).items():
+        for k in rng.choice(len(ids), min(3, len(ids)), replace=False):
+            e = np.zeros(n)
+            e[ids[k]] = 1.0
+            col = o.matvec(e)
+            assert np.abs(col[ids] - B[:, k]).max() <= 1e-13 * np.abs(col).max()
+
+
+def test_block_pcg_beats_diagonal_on_a_variable_coefficient():
+    idx, act = regular_grid(3, 4)
+    o = Oracle(idx, act).set_coef(COEF_EXPSIN, xi=np.random.default_rng(0).normal(0, 0.5, 16))
+    n = len(idx)
+    A = np.array([o.matvec(np.eye(n)[j]) for j in range(n)]).T
+    D = np.diag(A).copy()
+    b = np.random.default_rng(1).standard_normal(n)
+    x1, it_diag = pcg(lambda v: A @ v, b, lambda r: r / D, 1e-10)
+    x2, it_block = pcg(lambda v: A @ v, b, make_preconditioner(idx, D), 1e-10)
+    assert it_block < 0.7 * it_diag
+    assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x1)
