@@ -495,10 +495,9 @@ def main():
         (lsg_op_set_coef) and the load vectors; the preconditioner diagonal follows in
         the solve (it is invalidated by the new coefficients)."""
         xk, ek = batch_of(k)
-        for b in range(B):
-            c = L.VariableCoefficient(L.COEF_EXPSIN, xi=xk[b])
-            op64.set_coefficient(b, c)
-            op32.set_coefficient(b, c)
+        cs = [L.VariableCoefficient(L.COEF_EXPSIN, xi=xk[b]) for b in range(B)]
+        op64.set_coefficients(cs)
+        op32.set_coefficients(cs)
         op64.rhs_sines(ek, m, out=f)
         return xk, ek
 

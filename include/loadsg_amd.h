@@ -186,6 +186,10 @@ int lsg_grid_destroy(lsg_grid* g);
  * batch: number of instances sharing the grid (coefficients may differ per instance). */
 int lsg_op_create(lsg_grid* g, const lsg_coef* coef, int precision, int batch, lsg_op** out);
 int lsg_op_set_coef(lsg_op* op, int instance, const lsg_coef* coef);
+/* lsg_op_set_coef for instances first .. first + count - 1 (coefs[0 .. count)) in one call: the
+ * tables of the separable families are built on several host threads and uploaded with one
+ * copy.  No counterpart in the reference (one operator per coefficient there). */
+int lsg_op_set_coefs(lsg_op* op, int first, int count, const lsg_coef* coefs);
 int lsg_op_batch(const lsg_op* op);
 /* Preconditioner of lsg_cg_solve and of the inner FP32 solves of lsg_ir_solve on this
  * operator.  LSG_PRECOND_DIAG (default): the reference's diagonal precond3

@@ -88,6 +88,7 @@ def lib():
             "lsg_grid_destroy": (i32, [vp]),
             "lsg_op_create": (i32, [vp, ctypes.POINTER(_Coef), i32, i32, ctypes.POINTER(vp)]),
             "lsg_op_set_coef": (i32, [vp, i32, ctypes.POINTER(_Coef)]),
+            "lsg_op_set_coefs": (i32, [vp, i32, i32, vp]),
             "lsg_op_batch": (i32, [vp]),
             "lsg_op_set_precond": (i32, [vp, i32]),
             "lsg_matvec": (i32, [vp, vp, vp, i32]),
@@ -495,6 +496,12 @@ class MatrixVectorHomogen:
     def set_coefficient(self, instance, stencil):
         c = stencil._c(self.grid)
         _check(lib().lsg_op_set_coef(self._h, instance, ctypes.byref(c)))
+
+    def set_coefficients(self, stencils, first=0):
+        """set_coefficient for instances first, first + 1, ... in one library call."""
+        cs = [st._c(self.grid) for st in stencils]
+        arr = (_Coef * len(cs))(*cs)
+        _check(lib().lsg_op_set_coefs(self._h, first, len(cs), ctypes.cast(arr, ctypes.c_void_p)))
 
     def set_block(self, threads):
         _check(lib().lsg_op_set_block(self._h, 0, threads))

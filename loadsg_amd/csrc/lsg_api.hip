@@ -1851,6 +1851,48 @@ int lsg_op_set_coef(lsg_op* op, int instance, const lsg_coef* coef) {
     return LSG_OK;
 }
 
+// Coefficients of instances first .. first + count - 1 in one call: the coefficient tables are
+// computed on up to 16 host threads and uploaded with one copy (general coefficients, which
+// are sampled per cell, go through lsg_op_set_coef one by one).
+int lsg_op_set_coefs(lsg_op* op, int first, int count, const lsg_coef* coefs) {
+    if (!op || !coefs || first < 0 || count < 0 || first + count > op->batch) return fail(LSG_ERR_ARG, "bad arguments");
+    if (count == 0) return LSG_OK;
+    const Grid& g = *op->G;
+    if (op->cellKind) {
+        for (int b = 0; b < count; ++b)
+            if (int rc = lsg_op_set_coef(op, first + b, &coefs[b])) return rc;
+        return LSG_OK;
+    }
+    for (int b = 0; b < count; ++b) {
+        int nA, nC;
+        if (coefTerms(coefs[b], g.d, nA, nC) || nA != op->nA || nC != op->nC || cellCoefKind(coefs[b].kind))
+            return fail(LSG_ERR_ARG, "coefficient family differs from the operator's");
+    }
+    const size_t per = (size_t)(op->nA + op->nC) * g.d * op->tabLen;
+    std::vector<double> tab(per * count);
+    const int nth = std::max(1, std::min<int>({16, count, (int)std::thread::hardware_concurrency()}));
+    auto work = [&](int w) {
+        std::vector<double> one;
+        for (int b = w; b < count; b += nth) {
+            coefTables(coefs[b], g.d, g.maxLevel, op->tabLen, one);
+            std::copy(one.begin(), one.end(), tab.begin() + per * b);
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 1; w < nth; ++w) th.emplace_back(work, w);
+    work(0);
+    for (auto& t : th) t.join();
+    if (op->prec == 64) {
+        HIPCHK(hipMemcpy(op->coefTab.as<double>() + per * first, tab.data(), tab.size() * 8, hipMemcpyHostToDevice));
+    } else {
+        std::vector<float> tf(tab.begin(), tab.end());
+        HIPCHK(hipMemcpy(op->coefTab.as<float>() + per * first, tf.data(), tf.size() * 4, hipMemcpyHostToDevice));
+    }
+    op->diagValid = false;
+    op->cellCoefValid = false;
+    return LSG_OK;
+}
+
 int lsg_op_batch(const lsg_op* op) { return op ? op->batch : -1; }
 
 int lsg_op_set_precond(lsg_op* op, int kind) {

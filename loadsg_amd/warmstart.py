@@ -66,10 +66,9 @@ class WarmStart:
         t0 = time.perf_counter()
         for s0 in range(0, S, per):
             cnt = min(per, S - s0)
-            for b in range(B):
-                st = VariableCoefficient(COEF_EXPSIN, xi=xis[min(s0 + b // J, S - 1)])
-                op64.set_coefficient(b, st)
-                op32.set_coefficient(b, st)
+            cs = [VariableCoefficient(COEF_EXPSIN, xi=xis[min(s0 + b // J, S - 1)]) for b in range(B)]
+            op64.set_coefficients(cs)
+            op32.set_coefficients(cs)
             x.zero_()
             CG.solveMixed(rel_tol, x, f, op64, op32, inner_rel_tol=inner_rel_tol)
             U[s0:s0 + cnt] = x[:cnt * J].view(cnt, J, n).float()

@@ -96,3 +96,25 @@ def test_refinement_subsets_match_single():
         assert i1["rel_residual"] == info[b]["rel_residual"], b
         assert info[b]["converged"] == 1 and info[b]["rel_residual"] <= 1e-9
         assert torch.equal(xs[0], xm[b]), b
+
+
+def test_set_coefficients_equals_one_by_one():
+    """lsg_op_set_coefs (tables built on host threads, one upload) gives the operator that
+    lsg_op_set_coef instance by instance gives, bit for bit, in FP64 and FP32."""
+    import torch
+    import loadsg_amd as L
+    grid = L.AdaptiveSparseGrid.regular(4, 5)
+    B = 7
+    rng = np.random.default_rng(11)
+    cs = [L.VariableCoefficient(L.COEF_EXPSIN, xi=rng.normal(0.0, 0.5, 16)) for _ in range(B)]
+    for prec, dt in ((64, torch.float64), (32, torch.float32)):
+        a = L.MatrixVectorHomogen(grid, cs[0], precision=prec, batch=B)
+        b = L.MatrixVectorHomogen(grid, cs[0], precision=prec, batch=B)
+        a.set_coefficients(cs[1:], first=1)
+        for i in range(1, B):
+            b.set_coefficient(i, cs[i])
+        x = torch.randn(B, grid.size, dtype=dt, device="cuda")
+        ya, yb = torch.zeros_like(x), torch.zeros_like(x)
+        a.multiplication(x, ya)
+        b.multiplication(x, yb)
+        assert torch.equal(ya, yb)
