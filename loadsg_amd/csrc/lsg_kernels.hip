@@ -1294,21 +1294,21 @@ __global__ void cg_beta_scalar(CGState* st, const double* __restrict__ partials,
 }
 
 // Depth-block preconditioner (the inner solves of the mixed-precision method; no counterpart
-// in the reference, whose CG divides by the diagonal): workgroup (block T, group of PRE_IPW
-// instances); the block's inverse is staged in LDS once and applied by each wave to one
+// in the reference, whose CG divides by the diagonal): workgroup (block T, group of ipw
+// instances, a multiple of 4); the block's inverse is staged in LDS once and applied by each wave to one
 // instance at a time: lane k forms h_k = s_k sum_j B^-1_kj s_j r_j for the points k = lane and
 // k = lane + 64 of a block of up to 128 points (x = s .* r broadcast from LDS, the inverse read
 // column-wise), s = sqrt(d0 / D).  Blocks without an inverse: h = r / D.  The r.h partial of
 // (instance, block) is summed over the wave in a fixed order.  init: also d = -h (the start of
 // a solve, every instance).
-constexpr int PRE_IPW = 16;
 constexpr int PRE_MAXN = 128;
 template <typename V, bool INIT>
 __global__ void __launch_bounds__(256) blockpre_kernel(const PreBlock* __restrict__ items, int nblk,
                                                        const V* __restrict__ inv, const V* __restrict__ d0,
                                                        const V* __restrict__ r, const V* __restrict__ D,
                                                        V* __restrict__ h, V* __restrict__ dv, int64_t n, int batch,
-                                                       double* __restrict__ part, const CGState* __restrict__ st) {
+                                                       int ipw, double* __restrict__ part,
+                                                       const CGState* __restrict__ st) {
     extern __shared__ char smem_raw[];
     const PreBlock it = items[blockIdx.x];
     const int nl = it.n;
@@ -1321,8 +1321,8 @@ __global__ void __launch_bounds__(256) blockpre_kernel(const PreBlock* __restric
     }
     __syncthreads();
     V* x = xs + wave * PRE_MAXN;
-    const int i0 = blockIdx.y * PRE_IPW;
-    for (int q = 0; q < PRE_IPW; q += 4) {
+    const int i0 = blockIdx.y * ipw;
+    for (int q = 0; q < ipw; q += 4) {
         const int inst = i0 + q + wave;
         const bool run = inst < batch && (INIT || st[inst].active);
         const int64_t o = (int64_t)inst * n + it.off;
@@ -1567,14 +1567,16 @@ void launch_rhs_tab(const SumItem* items, int nitems, int d, int nterms, const d
 template <typename V>
 static void launch_blockpre(const BlockPreArgs<V>& pre, bool init, const V* r, const V* D, V* h, V* dv, int64_t n,
                             int batch, const CGState* st, hipStream_t s) {
-    const dim3 grid((unsigned)pre.nblk, (unsigned)((batch + PRE_IPW - 1) / PRE_IPW));
+    // instances per workgroup: the staged inverse (up to 64 KB) serves up to 64 of them
+    const int ipw = std::min(64, (batch + 3) / 4 * 4);
+    const dim3 grid((unsigned)pre.nblk, (unsigned)((batch + ipw - 1) / ipw));
     const size_t lds = ((size_t)pre.maxN * pre.maxN + 4 * PRE_MAXN) * sizeof(V);
     if (init)
         hipLaunchKernelGGL((blockpre_kernel<V, true>), grid, dim3(256), lds, s, pre.items, pre.nblk, pre.inv, pre.d0, r,
-                           D, h, dv, n, batch, pre.part, st);
+                           D, h, dv, n, batch, ipw, pre.part, st);
     else
         hipLaunchKernelGGL((blockpre_kernel<V, false>), grid, dim3(256), lds, s, pre.items, pre.nblk, pre.inv, pre.d0,
-                           r, D, h, dv, n, batch, pre.part, st);
+                           r, D, h, dv, n, batch, ipw, pre.part, st);
 }
 template <typename V>
 void launch_cg_init(const V* f, V* r, V* h, V* dv, const V* D, int64_t n, int batch, double* partials, int nb,

@@ -1,10 +1,12 @@
 """Summaries of rocprofv3 result databases (rocpd SQLite: tables `kernels` and
 `counters_collection`).
 
-    python3 -m loadsg_amd.rocpd_summary trace DB [--kernel RE --tail N --group K]
+    python3 -m loadsg_amd.rocpd_summary trace DB [--kernel RE --tail N --group K] [--after RE]
         per-kernel table (calls, total ms, average / max us, share); with --kernel the
         last N dispatches whose name matches RE are also summed and divided into
-        groups of K dispatches (K = apply launches of one matvec -> time per matvec).
+        groups of K dispatches (K = apply launches of one matvec -> time per matvec);
+        with --after only the dispatches after the last one matching RE are counted (and
+        the span from the first to the last of them is printed).
     python3 -m loadsg_amd.rocpd_summary sequence DB --kernel RE --tail N
         average duration per position of the dispatch sequences that start at each
         dispatch matching RE (e.g. one MLP training step), over the last N sequences.
@@ -23,9 +25,16 @@ def _short(name):
     return re.sub(r"\(.*", "", name)
 
 
-def trace(db, kernel=None, tail=0, group=1, top=20):
+def trace(db, kernel=None, tail=0, group=1, top=20, after=None):
     con = sqlite3.connect(db)
     rows = con.execute("select name, start, end from kernels order by start").fetchall()
+    if after:
+        marks = [k for k, (name, _, _) in enumerate(rows) if re.search(after, name)]
+        if marks:
+            rows = rows[marks[-1] + 1:]
+        if rows:
+            print("window after the last %r dispatch: %d dispatches, %.3f ms from the first start to the last end"
+                  % (after, len(rows), (rows[-1][2] - rows[0][1]) / 1e6))
     agg = defaultdict(lambda: [0, 0, 0])
     for name, s, e in rows:
         a = agg[_short(name)]
@@ -114,9 +123,10 @@ def main():
     ap.add_argument("--group", type=int, default=1)
     ap.add_argument("--calib", default="elementwise")
     ap.add_argument("--top", type=int, default=20)
+    ap.add_argument("--after")
     a = ap.parse_args()
     if a.mode == "trace":
-        trace(a.db, a.kernel, a.tail, a.group, a.top)
+        trace(a.db, a.kernel, a.tail, a.group, a.top, a.after)
     elif a.mode == "sequence":
         sequence(a.db, a.kernel, a.tail)
     else:
