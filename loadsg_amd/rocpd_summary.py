@@ -58,8 +58,8 @@ def trace(db, kernel=None, tail=0, group=1, top=20, after=None):
 
 def sequence(db, start, tail):
     """Split the dispatch list at every dispatch whose name matches `start` and average the
-    duration of each position over the last `tail` sequences that have the most common
-    length (one training step = one sequence)."""
+    duration of each position over the sequences among the last `tail` that have the most
+    common length there (one training step = one sequence)."""
     con = sqlite3.connect(db)
     rows = con.execute("select name, start, end from kernels order by start").fetchall()
     seqs, cur = [], []
@@ -71,7 +71,7 @@ def sequence(db, start, tail):
         cur.append((_short(name), e - s, gap))
     seqs.append(cur)
     lengths = defaultdict(int)
-    for q in seqs:
+    for q in (seqs[-tail:] if tail else seqs):     # the most common length among the last `tail`
         lengths[len(q)] += 1
     L = max(lengths, key=lambda k: lengths[k])
     sel = [q for q in seqs if len(q) == L][-tail:] if tail else [q for q in seqs if len(q) == L]
