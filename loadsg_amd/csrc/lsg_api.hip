@@ -383,8 +383,9 @@ struct lsg_op {
     // preconditioner of lsg_cg_solve and of the inner solves of lsg_ir_solve
     // (LSG_PRECOND_DIAG: the reference's precond3; LSG_PRECOND_DEPTH_BLOCK: BlockPre)
     int precKind = LSG_PRECOND_DIAG;
-    DevMem preItems, preInv, preD0, prePart;
-    int preNblk = 0, preMaxN = 0;
+    DevMem preItems, preInv, preD0, prePart, preFdList, preFdV, preFdLam;
+    int preNblk = 0, preMaxN = 0, preNfd = 0, preMaxFd = 0;
+    int32_t preVOff[FD_MAXLEV + 1] = {}, preLOff[FD_MAXLEV + 1] = {};
     bool capturing = false;   // matvec launches record no timing events while captured
     std::unique_ptr<TrDev> tr;   // basis transformations (built on first use)
     int64_t n() const { return G->n; }
@@ -751,6 +752,17 @@ int ensureBlockPre(lsg_op* op) {
     };
     if (int rc = upload(op->preInv, bp.inv)) return rc;
     if (int rc = upload(op->preD0, bp.d0)) return rc;
+    if (int rc = upload(op->preFdV, bp.fdV)) return rc;
+    if (int rc = upload(op->preFdLam, bp.fdLam)) return rc;
+    op->preNfd = (int)bp.fdList.size();
+    for (int32_t b : bp.fdList) op->preMaxFd = std::max(op->preMaxFd, bp.blocks[(size_t)b].n);
+    HIPCHK(op->preFdList.alloc(std::max<size_t>(1, bp.fdList.size()) * sizeof(int32_t)));
+    if (!bp.fdList.empty())
+        HIPCHK(hipMemcpy(op->preFdList.p, bp.fdList.data(), bp.fdList.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    for (int t = 0; t <= FD_MAXLEV; ++t) {
+        op->preVOff[t] = (int32_t)bp.fdVoff[t];
+        op->preLOff[t] = (int32_t)bp.fdLoff[t];
+    }
     HIPCHK(op->prePart.alloc((size_t)op->batch * op->preNblk * sizeof(double)));
     return LSG_OK;
 }
@@ -765,6 +777,16 @@ BlockPreArgs<V> blockPreArgs(lsg_op* op, bool on) {
     a.inv = op->preInv.as<V>();
     a.d0 = op->preD0.as<V>();
     a.part = op->prePart.as<double>();
+    a.d = op->G->d;
+    a.nfd = op->preNfd;
+    a.maxFd = op->preMaxFd;
+    a.fdList = op->preFdList.as<int32_t>();
+    a.fdV = op->preFdV.as<V>();
+    a.fdLam = op->preFdLam.as<V>();
+    for (int t = 0; t <= FD_MAXLEV; ++t) {
+        a.vOff[t] = op->preVOff[t];
+        a.lOff[t] = op->preLOff[t];
+    }
     return a;
 }
 

@@ -1311,6 +1311,7 @@ __global__ void __launch_bounds__(256) blockpre_kernel(const PreBlock* __restric
                                                        const CGState* __restrict__ st) {
     extern __shared__ char smem_raw[];
     const PreBlock it = items[blockIdx.x];
+    if (it.inv == -2) return;              // fast-diagonalisation block: blockfd_kernel
     const int nl = it.n;
     V* Bs = reinterpret_cast<V*>(smem_raw);
     const bool dense = it.inv >= 0;
@@ -1377,6 +1378,104 @@ __global__ void __launch_bounds__(256) blockpre_kernel(const PreBlock* __restric
             acc = wave_sum(acc);
             if (lane == 0) part[(int64_t)inst * nblk + blockIdx.x] = acc;
         }
+    }
+}
+
+// The larger depth blocks by fast diagonalisation of the Kronecker sum:
+// h = s .* (V Lam^-1 V^T (s .* r)) with V = (x)_i V_(T_i), applied mode by mode in LDS (block
+// points dimension 0 fastest, 2^(T_i - 1) along dimension i, so coordinate i is a bit field).
+// Workgroup (block, group of ipw instances); each wave transforms one instance at a time in
+// its own two LDS buffers; every wave of a workgroup runs the same passes, so the passes are
+// separated by workgroup barriers.
+template <typename V, bool INIT>
+__global__ void __launch_bounds__(256) blockfd_kernel(BlockPreArgs<V> a, const V* __restrict__ r,
+                                                      const V* __restrict__ D, V* __restrict__ h, V* __restrict__ dv,
+                                                      int64_t n, int batch, int ipw, const CGState* __restrict__ st) {
+    extern __shared__ char smem_raw[];
+    const int bi = a.fdList[blockIdx.x];
+    const PreBlock it = a.items[bi];
+    const int nl = it.n, d = a.d;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    V* A = reinterpret_cast<V*>(smem_raw) + (int64_t)wave * 2 * a.maxFd;
+    V* B = A + a.maxFd;
+    int sh[MAXD], lev[MAXD];
+    int o = 0, ones = 0;
+    for (int i = 0; i < d; ++i) {
+        lev[i] = it.lev[i];
+        sh[i] = o;
+        o += lev[i] - 1;
+        ones += lev[i] == 1;
+    }
+    // level-1 factors: V = v1 (1 x 1) forward and backward, lam_1 in every sum
+    const V v1 = a.fdV[a.vOff[1]];
+    V f = V(1);
+    for (int c = 0; c < ones; ++c) f *= v1 * v1;
+    const int i0 = blockIdx.y * ipw;
+    for (int q = 0; q < ipw; q += 4) {
+        const int inst = i0 + q + wave;
+        const bool run = inst < batch && (INIT || st[inst].active);
+        const int64_t ob = (int64_t)inst * n + it.off;
+        V* X = A;
+        V* Y = B;
+        if (run)
+            for (int p = lane; p < nl; p += 64) X[p] = sqrt(a.d0[it.off + p] / D[ob + p]) * r[ob + p];
+        __syncthreads();
+        // forward: along every dimension with more than one point, X <- V_t^T X
+        for (int i = 0; i < d; ++i) {
+            if (lev[i] == 1) continue;
+            const int m = 1 << (lev[i] - 1);
+            const V* Vt = a.fdV + a.vOff[lev[i]];
+            if (run)
+                for (int p = lane; p < nl; p += 64) {
+                    const int k = (p >> sh[i]) & (m - 1);
+                    const int base = p - (k << sh[i]);
+                    V acc = V(0);
+                    for (int j = 0; j < m; ++j) acc = fma(Vt[j * m + k], X[base + (j << sh[i])], acc);
+                    Y[p] = acc;
+                }
+            __syncthreads();
+            V* t = X; X = Y; Y = t;
+        }
+        // 1 / (sum of the 1D eigenvalues), times the level-1 factors
+        if (run)
+            for (int p = lane; p < nl; p += 64) {
+                V ls = V(0);
+                for (int i = 0; i < d; ++i) {
+                    const int m = 1 << (lev[i] - 1);
+                    ls += a.fdLam[a.lOff[lev[i]] + ((p >> sh[i]) & (m - 1))];
+                }
+                X[p] = f * X[p] / ls;
+            }
+        __syncthreads();
+        // backward: X <- V_t X along the same dimensions
+        for (int i = 0; i < d; ++i) {
+            if (lev[i] == 1) continue;
+            const int m = 1 << (lev[i] - 1);
+            const V* Vt = a.fdV + a.vOff[lev[i]];
+            if (run)
+                for (int p = lane; p < nl; p += 64) {
+                    const int k = (p >> sh[i]) & (m - 1);
+                    const int base = p - (k << sh[i]);
+                    V acc = V(0);
+                    for (int j = 0; j < m; ++j) acc = fma(Vt[k * m + j], X[base + (j << sh[i])], acc);
+                    Y[p] = acc;
+                }
+            __syncthreads();
+            V* t = X; X = Y; Y = t;
+        }
+        double acc = 0.0;
+        if (run) {
+            for (int p = lane; p < nl; p += 64) {
+                const V rk = r[ob + p];
+                const V hk = sqrt(a.d0[it.off + p] / D[ob + p]) * X[p];
+                h[ob + p] = hk;
+                if (INIT) dv[ob + p] = V(-1) * hk;
+                acc += double(rk) * double(hk);
+            }
+            acc = wave_sum(acc);
+            if (lane == 0) a.part[(int64_t)inst * a.nblk + bi] = acc;
+        }
+        __syncthreads();
     }
 }
 
@@ -1577,6 +1676,16 @@ static void launch_blockpre(const BlockPreArgs<V>& pre, bool init, const V* r, c
     else
         hipLaunchKernelGGL((blockpre_kernel<V, false>), grid, dim3(256), lds, s, pre.items, pre.nblk, pre.inv, pre.d0,
                            r, D, h, dv, n, batch, ipw, pre.part, st);
+    if (pre.nfd > 0) {
+        // 16 instances per workgroup: the blocks are few and large
+        const int ipf = std::min(16, (batch + 3) / 4 * 4);
+        const dim3 gfd((unsigned)pre.nfd, (unsigned)((batch + ipf - 1) / ipf));
+        const size_t ldf = (size_t)8 * pre.maxFd * sizeof(V);
+        if (init)
+            hipLaunchKernelGGL((blockfd_kernel<V, true>), gfd, dim3(256), ldf, s, pre, r, D, h, dv, n, batch, ipf, st);
+        else
+            hipLaunchKernelGGL((blockfd_kernel<V, false>), gfd, dim3(256), ldf, s, pre, r, D, h, dv, n, batch, ipf, st);
+    }
 }
 template <typename V>
 void launch_cg_init(const V* f, V* r, V* h, V* dv, const V* D, int64_t n, int batch, double* partials, int nb,

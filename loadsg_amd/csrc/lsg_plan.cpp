@@ -867,6 +867,89 @@ static void prewavelet1DMatrices(int t, std::vector<double>& K, std::vector<doub
         }
 }
 
+// Generalised symmetric eigenproblem K v = lam M v (M positive definite, m x m row-major):
+// M = L L^T, C = L^-1 K L^-T, C = W diag(lam) W^T by cyclic Jacobi rotations, V = L^-T W.
+static int generalisedEigen(int m, const std::vector<double>& K, const std::vector<double>& M, std::vector<double>& V,
+                            std::vector<double>& lam) {
+    std::vector<double> L((size_t)m * m, 0.0);
+    for (int j = 0; j < m; ++j) {
+        double s = M[(size_t)j * m + j];
+        for (int q = 0; q < j; ++q) s -= L[(size_t)j * m + q] * L[(size_t)j * m + q];
+        if (!(s > 0.0)) return -1;
+        L[(size_t)j * m + j] = std::sqrt(s);
+        for (int i = j + 1; i < m; ++i) {
+            double v = M[(size_t)i * m + j];
+            for (int q = 0; q < j; ++q) v -= L[(size_t)i * m + q] * L[(size_t)j * m + q];
+            L[(size_t)i * m + j] = v / L[(size_t)j * m + j];
+        }
+    }
+    // Y = L^-1 K (forward substitution per column), C = Y L^-T = (L^-1 Y^T)^T
+    std::vector<double> Y((size_t)m * m), C((size_t)m * m);
+    for (int c = 0; c < m; ++c)
+        for (int i = 0; i < m; ++i) {
+            double v = K[(size_t)i * m + c];
+            for (int q = 0; q < i; ++q) v -= L[(size_t)i * m + q] * Y[(size_t)q * m + c];
+            Y[(size_t)i * m + c] = v / L[(size_t)i * m + i];
+        }
+    for (int r = 0; r < m; ++r)            // row r of C = L^-1 applied to row r of Y, as a column
+        for (int i = 0; i < m; ++i) {
+            double v = Y[(size_t)r * m + i];
+            for (int q = 0; q < i; ++q) v -= L[(size_t)i * m + q] * C[(size_t)r * m + q];
+            C[(size_t)r * m + i] = v / L[(size_t)i * m + i];
+        }
+    for (int i = 0; i < m; ++i)
+        for (int j = i + 1; j < m; ++j) {
+            const double a = 0.5 * (C[(size_t)i * m + j] + C[(size_t)j * m + i]);
+            C[(size_t)i * m + j] = C[(size_t)j * m + i] = a;
+        }
+    std::vector<double> W((size_t)m * m, 0.0);
+    for (int i = 0; i < m; ++i) W[(size_t)i * m + i] = 1.0;
+    for (int sweep = 0; sweep < 100; ++sweep) {
+        double off = 0.0, tot = 0.0;
+        for (int i = 0; i < m; ++i)
+            for (int j = 0; j < m; ++j) {
+                const double c2 = C[(size_t)i * m + j] * C[(size_t)i * m + j];
+                tot += c2;
+                if (i != j) off += c2;
+            }
+        if (off <= 1e-30 * tot) break;
+        for (int p = 0; p < m - 1; ++p)
+            for (int q = p + 1; q < m; ++q) {
+                const double apq = C[(size_t)p * m + q];
+                if (std::fabs(apq) < 1e-300) continue;
+                const double app = C[(size_t)p * m + p], aqq = C[(size_t)q * m + q];
+                const double th = 0.5 * (aqq - app) / apq;
+                const double t = (th >= 0 ? 1.0 : -1.0) / (std::fabs(th) + std::sqrt(th * th + 1.0));
+                const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < m; ++k) {       // columns p, q of C
+                    const double akp = C[(size_t)k * m + p], akq = C[(size_t)k * m + q];
+                    C[(size_t)k * m + p] = c * akp - s * akq;
+                    C[(size_t)k * m + q] = s * akp + c * akq;
+                }
+                for (int k = 0; k < m; ++k) {       // rows p, q of C
+                    const double apk = C[(size_t)p * m + k], aqk = C[(size_t)q * m + k];
+                    C[(size_t)p * m + k] = c * apk - s * aqk;
+                    C[(size_t)q * m + k] = s * apk + c * aqk;
+                }
+                for (int k = 0; k < m; ++k) {       // W <- W J
+                    const double wkp = W[(size_t)k * m + p], wkq = W[(size_t)k * m + q];
+                    W[(size_t)k * m + p] = c * wkp - s * wkq;
+                    W[(size_t)k * m + q] = s * wkp + c * wkq;
+                }
+            }
+    }
+    lam.resize(m);
+    for (int i = 0; i < m; ++i) lam[i] = C[(size_t)i * m + i];
+    V.assign((size_t)m * m, 0.0);                 // V = L^-T W (back substitution per column)
+    for (int c = 0; c < m; ++c)
+        for (int i = m - 1; i >= 0; --i) {
+            double v = W[(size_t)i * m + c];
+            for (int q = i + 1; q < m; ++q) v -= L[(size_t)q * m + i] * V[(size_t)q * m + c];
+            V[(size_t)i * m + c] = v / L[(size_t)i * m + i];
+        }
+    return 0;
+}
+
 int buildBlockPre(const Grid& g, int maxDense, BlockPre& out, std::string& err) {
     if (g.sparse) { err = "the depth-block preconditioner needs complete depth blocks"; return LSG_ERR_UNSUPPORTED; }
     const int d = g.d;
@@ -883,10 +966,18 @@ int buildBlockPre(const Grid& g, int maxDense, BlockPre& out, std::string& err) 
         }
         return it->second;
     };
+    // level 1 of the fast diagonalisation: m = 1, K = 2^1 * ..., stored like the others
+    {
+        const auto& km = mat(1);
+        out.fdVoff[1] = (int64_t)out.fdV.size();
+        out.fdLoff[1] = (int64_t)out.fdLam.size();
+        out.fdV.push_back(1.0 / std::sqrt(km.second[0]));
+        out.fdLam.push_back(km.first[0] / km.second[0]);
+    }
     for (int t = 0; t < nd; ++t) {
         const Depth& T = g.depths[t];
         const int64_t nb = g.blkSize[t];
-        PreBlock pb{g.blkOff[t], (int32_t)nb, -1};
+        PreBlock pb{g.blkOff[t], (int32_t)nb, -1, {}};
         int m[MAXD];
         for (int i = 0; i < d; ++i) m[i] = 1 << (T[i] - 1);
         // local 1D indices of point p of the block (dimension 0 fastest)
@@ -909,7 +1000,28 @@ int buildBlockPre(const Grid& g, int maxDense, BlockPre& out, std::string& err) 
             return s;
         };
         for (int64_t p = 0; p < nb; ++p) out.d0[(size_t)(pb.off + p)] = entry(p, p);
-        if (nb <= maxDense) {
+        for (int i = 0; i < MAXD; ++i) pb.lev[i] = i < d ? T[i] : 0;
+        bool fd = nb > maxDense && nb <= FD_MAXN;
+        for (int i = 0; i < d; ++i) fd = fd && T[i] <= FD_MAXLEV;
+        if (fd) {
+            for (int i = 0; i < d; ++i) {
+                const int t = T[i];
+                if (out.fdLoff[t] || t == 1) continue;     // level 1: V = 1 / sqrt(M), one value
+                const auto& km = mat(t);
+                const int mt = 1 << (t - 1);
+                std::vector<double> V, lam;
+                if (generalisedEigen(mt, km.first, km.second, V, lam)) {
+                    err = "1D prewavelet mass matrix not positive definite";
+                    return LSG_ERR_ARG;
+                }
+                out.fdVoff[t] = (int64_t)out.fdV.size();
+                out.fdLoff[t] = (int64_t)out.fdLam.size();
+                out.fdV.insert(out.fdV.end(), V.begin(), V.end());
+                out.fdLam.insert(out.fdLam.end(), lam.begin(), lam.end());
+            }
+            pb.inv = -2;
+            out.fdList.push_back((int32_t)out.blocks.size());
+        } else if (nb <= maxDense) {
             // Cholesky factor of the block, then its inverse column by column
             const int n = (int)nb;
             std::vector<double> A((size_t)n * n), L((size_t)n * n, 0.0);

@@ -257,20 +257,31 @@ void regularIndexTable(int d, int level, std::vector<int32_t>& idx);
 
 // Depth-block preconditioner of the inner (mixed-precision) solves: for every depth block
 // T of a dense grid the Galerkin block of the constant-coefficient Laplacian on the
-// prewavelets of depth T, sum_i K_(T_i) (x) prod_(j != i) M_(T_j) with the 1D prewavelet
+// prewavelets of depth T, B_T = sum_i K_(T_i) (x) prod_(j != i) M_(T_j) with the 1D prewavelet
 // stiffness / mass matrices of level t (no counterpart in the reference, whose CG uses the
-// diagonal).  Blocks of at most maxDense points keep their inverse; larger ones fall back
-// to the diagonal.
+// diagonal).  Blocks of at most maxDense points keep their inverse.  Larger ones use the fast
+// diagonalisation of the Kronecker sum: with V_t^T M_t V_t = I and V_t^T K_t V_t = diag(lam_t)
+// (generalised eigenvectors of level t), B_T^-1 = (x)_i V_(T_i) diag(1 / sum_i lam_(T_i)) (x)_i
+// V_(T_i)^T, for levels up to FD_MAXLEV; blocks with a deeper level keep the diagonal.
+constexpr int FD_MAXLEV = 9;      // 1D eigenproblems up to 256 x 256
+constexpr int FD_MAXN = 1024;     // largest block on the fast-diagonalisation path
 struct PreBlock {
     int64_t off;      // first point of the block (device order)
     int32_t n;        // points of the block
-    int32_t inv;      // first entry of its inverse in BlockPre::inv (n x n, column k contiguous); -1: diagonal
+    int32_t inv;      // >= 0: first entry of its inverse in BlockPre::inv (n x n, column k
+                      // contiguous); -1: diagonal; -2: fast diagonalisation
+    int32_t lev[MAXD];    // the depth T (fast diagonalisation: the levels of the 1D factors)
 };
 struct BlockPre {
     std::vector<PreBlock> blocks;   // depth-list order
     std::vector<double> inv;        // block inverses
     std::vector<double> d0;         // diagonal of the constant-coefficient blocks, device order
     int maxN = 0;                   // largest block with an inverse
+    // fast diagonalisation: V_t row-major (row j = basis function j, column k = eigenvector k)
+    // at fdV[fdVoff[t]], lam_t at fdLam[fdLoff[t]] (levels 1 .. FD_MAXLEV; 2^(t-1) each)
+    std::vector<double> fdV, fdLam;
+    int64_t fdVoff[FD_MAXLEV + 1] = {}, fdLoff[FD_MAXLEV + 1] = {};
+    std::vector<int32_t> fdList;    // blocks on the fast-diagonalisation path
 };
 int buildBlockPre(const Grid& g, int maxDense, BlockPre& out, std::string& err);
 

@@ -7,10 +7,12 @@ constant-coefficient Laplacian on the prewavelets of depth T,
     B_T = sum_i K_(T_i) (x) prod_(j != i) M_(T_j),
 with the 1D prewavelet stiffness and mass matrices of level t (prewavelets of the odd level-t
 positions: 1 / -0.6 / 0.1, 0.9 at the centre next to the boundary, as in
-indices/index.cc:180-327), and applies h = s .* (B_T^-1 (s .* r)) with s = sqrt(D0 / D); blocks
-above max_dense points (the library: 128 on FP32 operators, 64 on FP64 ones) fall back to
-h = r / D.  tests/test_blockpre_ref.py checks the Kronecker blocks against the oracle's
-constant-coefficient matvec.
+indices/index.cc:180-327), and applies h = s .* (B_T^-1 (s .* r)) with s = sqrt(D0 / D).  The
+library keeps dense inverses up to max_dense points (128 on FP32 operators, 64 on FP64 ones) and
+applies larger blocks of up to max_fd points whose levels are at most fd_maxlev by fast
+diagonalisation, which is the same B_T^-1 up to rounding; other blocks fall back to h = r / D.
+tests/test_blockpre_ref.py checks the Kronecker blocks against the oracle's constant-coefficient
+matvec and the fast diagonalisation against the dense inverse.
 """
 import numpy as np
 
@@ -70,7 +72,29 @@ def constant_blocks(idx):
     return out
 
 
-def make_preconditioner(idx, D, max_dense=128):
+def fast_diagonal_inverse(T, idx_block):
+    """B_T^-1 by fast diagonalisation: (x)_i V_(T_i) diag(1 / sum_i lam) (x)_i V_(T_i)^T with
+    the generalised eigenvectors K V = M V diag(lam), V^T M V = I of every 1D factor."""
+    import scipy.linalg as sl
+    groups, loc = depth_blocks(idx_block)
+    L = loc
+    d = len(T)
+    fac = []
+    for t in T:
+        K, M = prewavelet_1d(t)
+        lam, V = sl.eigh(K, M)
+        fac.append((lam, V))
+    nb = len(idx_block)
+    Vfull = np.ones((nb, nb))
+    lamsum = np.zeros(nb)
+    for i in range(d):
+        lam, V = fac[i]
+        Vfull = Vfull * V[np.ix_(L[:, i], L[:, i])]
+        lamsum += lam[L[:, i]]
+    return Vfull @ np.diag(1.0 / lamsum) @ Vfull.T
+
+
+def make_preconditioner(idx, D, max_dense=128, max_fd=1024, fd_maxlev=9):
     """The preconditioner as a function r -> h for the diagonal D of the operator."""
     blocks = constant_blocks(idx)
     n = len(idx)
@@ -78,7 +102,8 @@ def make_preconditioner(idx, D, max_dense=128):
     parts = []
     for T, (ids, B) in blocks.items():
         D0[ids] = np.diag(B)
-        parts.append((ids, np.linalg.inv(B) if len(ids) <= max_dense else None))
+        exact = len(ids) <= max_dense or (len(ids) <= max_fd and max(T) <= fd_maxlev)
+        parts.append((ids, np.linalg.inv(B) if exact else None))
     s = np.sqrt(D0 / D)
 
     def apply(r):

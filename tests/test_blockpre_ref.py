@@ -5,7 +5,7 @@ fewer iterations than with the diagonal on the same system."""
 import numpy as np
 import pytest
 
-from blockpre_ref import constant_blocks, make_preconditioner, pcg
+from blockpre_ref import constant_blocks, fast_diagonal_inverse, make_preconditioner, pcg
 from oracle_ctypes import COEF_CONST, COEF_EXPSIN, Oracle, regular_grid
 
 
@@ -34,3 +34,16 @@ def test_block_pcg_beats_diagonal_on_a_variable_coefficient():
     x2, it_block = pcg(lambda v: A @ v, b, make_preconditioner(idx, D), 1e-10)
     assert it_block < 0.7 * it_diag
     assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x1)
+
+
+@pytest.mark.parametrize("d,level", [(2, 6), (3, 5)])
+def test_fast_diagonalisation_is_the_block_inverse(d, level):
+    """(x)V diag(1 / sum lam) (x)V^T with the 1D generalised eigenvectors equals B_T^-1."""
+    idx, _ = regular_grid(d, level)
+    idx = np.asarray(idx)
+    for T, (ids, B) in constant_blocks(idx).items():
+        if len(ids) < 8:
+            continue
+        F = fast_diagonal_inverse(T, idx[ids])
+        Bi = np.linalg.inv(B)
+        assert np.abs(F - Bi).max() <= 1e-9 * np.abs(Bi).max()

@@ -1,13 +1,14 @@
 """The depth-block preconditioner on the GPU (lsg_op_set_precond, LSG_PRECOND_DEPTH_BLOCK):
 no counterpart in the reference, checked against its numpy restatement
 (tests/blockpre_ref.py, pinned on the CPU by tests/test_blockpre_ref.py) driven by the same
-operator and diagonal.  PCG to a relative residual of 1e-10: iteration counts within one of
-the restatement's, solutions within 1e-9; dense blocks (up to 64 points on FP64 operators) and
-the diagonal fallback for larger blocks (d = 2, level 9) are both exercised; the FP32 operator
-of the mixed solve keeps blocks of up to 128 points (d = 5, level 7).  The mixed-precision solve
-with the preconditioner on its FP32 operator reaches 1e-10 in fewer inner iterations than
-with the diagonal; batches match single instances bit for bit; graph replays match direct
-launches bit for bit.
+operator and diagonal.  PCG to a relative residual of 1e-10: iteration counts within one of the
+restatement's, solutions within 1e-9.  Both ways of applying a block are exercised: dense
+inverses (up to 64 points on FP64 operators) and the fast diagonalisation of larger blocks
+(d = 2 level 8 and 9, d = 3 level 8: blocks of 128 and 256 points); the FP32 operator of the
+mixed solve keeps dense blocks of up to 128 points (d = 5, level 7).  The mixed-precision solve
+with the preconditioner on its FP32 operator reaches 1e-10 in fewer inner iterations than with
+the diagonal; batches match single instances bit for bit; graph replays match direct launches
+bit for bit.
 """
 import os
 
@@ -34,7 +35,7 @@ def expsin(seed):
     return L.VariableCoefficient(L.COEF_EXPSIN, xi=np.random.default_rng(seed).normal(0.0, 0.5, 16))
 
 
-@pytest.mark.parametrize("d,level", [(2, 8), (2, 9), (3, 5), (5, 4)])
+@pytest.mark.parametrize("d,level", [(2, 8), (2, 9), (3, 5), (3, 8), (5, 4)])
 def test_block_pcg_matches_restatement(d, level):
     print("d=%d level=%d" % (d, level), flush=True)
     grid = L.AdaptiveSparseGrid.regular(d, level)
