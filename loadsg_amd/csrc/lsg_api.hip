@@ -733,8 +733,10 @@ int ensureBlockPre(lsg_op* op) {
     if (op->preItems.p) return LSG_OK;
     BlockPre bp;
     std::string err;
-    // dense block inverses staged in LDS: up to 128 points in FP32 (64 KB), 64 in FP64 (32 KB)
-    if (int rc = buildBlockPre(*op->G, op->prec == 32 ? 128 : 64, bp, err)) return fail(rc, err);
+    // dense block inverses staged in LDS: up to 128 points in FP32 (64 KB), 64 in FP64 (32 KB);
+    // LSG_PRE_DENSE_MAX lowers the limit (larger blocks then take the fast diagonalisation)
+    const int dmax = std::min(op->prec == 32 ? 128 : 64, envInt("LSG_PRE_DENSE_MAX", 128));
+    if (int rc = buildBlockPre(*op->G, dmax, bp, err)) return fail(rc, err);
     op->preNblk = (int)bp.blocks.size();
     op->preMaxN = bp.maxN;
     HIPCHK(op->preItems.alloc(bp.blocks.size() * sizeof(PreBlock)));
