@@ -198,9 +198,13 @@ int lsg_op_batch(const lsg_op* op);
  * blocks, each block the Galerkin block of the constant-coefficient Laplacian on the
  * prewavelets of that depth (a sum of Kronecker products of 1D prewavelet stiffness and mass
  * matrices, inverted once per grid), scaled symmetrically by sqrt(D0 / D) with D this
- * operator's diagonal and D0 that of the constant-coefficient blocks; blocks above 128
- * points on FP32 operators (64 on FP64 ones) keep the diagonal.  Dense grids only (LSG_ERR_UNSUPPORTED on the cell-wise path).
- * Solutions meet the same stopping rules; iteration counts differ from the reference's. */
+ * operator's diagonal and D0 that of the constant-coefficient blocks.  Blocks of up to 128
+ * points on FP32 operators (64 on FP64 ones) keep their dense inverse; larger blocks whose
+ * points form a product set of 1D positions (up to 256 per dimension, 2048 points) are
+ * applied by fast diagonalisation of the Kronecker sum; others keep the diagonal.  On
+ * adaptive grids a block is taken over the active points of its depth (hanging points: the
+ * identity).  Solutions meet the same stopping rules; iteration counts differ from the
+ * reference's. */
 #define LSG_PRECOND_DIAG 0
 #define LSG_PRECOND_DEPTH_BLOCK 1
 int lsg_op_set_precond(lsg_op* op, int kind);

@@ -510,7 +510,7 @@ class MatrixVectorHomogen:
         """Preconditioner of CG.solveHomogen and of the inner solves of CG.solveMixed on this
         operator: PRECOND_DIAG (the reference's precond3, default) or PRECOND_DEPTH_BLOCK
         (block Jacobi over the depth blocks with constant-coefficient Galerkin blocks scaled
-        by the diagonal; dense grids only)."""
+        by the diagonal; on adaptive grids over the active points of each depth)."""
         _check(lib().lsg_op_set_precond(self._h, kind))
         return self
 

@@ -383,9 +383,8 @@ struct lsg_op {
     // preconditioner of lsg_cg_solve and of the inner solves of lsg_ir_solve
     // (LSG_PRECOND_DIAG: the reference's precond3; LSG_PRECOND_DEPTH_BLOCK: BlockPre)
     int precKind = LSG_PRECOND_DIAG;
-    DevMem preItems, preInv, preD0, prePart, preFdList, preFdV, preFdLam;
+    DevMem preItems, preInv, preD0, prePart, preFdList, preFdPerm, preFdV, preFdLam;
     int preNblk = 0, preMaxN = 0, preNfd = 0, preMaxFd = 0;
-    int32_t preVOff[FD_MAXLEV + 1] = {}, preLOff[FD_MAXLEV + 1] = {};
     bool capturing = false;   // matvec launches record no timing events while captured
     std::unique_ptr<TrDev> tr;   // basis transformations (built on first use)
     int64_t n() const { return G->n; }
@@ -728,7 +727,7 @@ struct CgBufs {
     bool pre = false;     // depth-block preconditioner (own buffers only)
 };
 
-// Device tables of the depth-block preconditioner of a dense-grid operator (built once)
+// Device tables of the depth-block preconditioner of an operator (built once)
 int ensureBlockPre(lsg_op* op) {
     if (op->preItems.p) return LSG_OK;
     BlockPre bp;
@@ -757,14 +756,14 @@ int ensureBlockPre(lsg_op* op) {
     if (int rc = upload(op->preFdV, bp.fdV)) return rc;
     if (int rc = upload(op->preFdLam, bp.fdLam)) return rc;
     op->preNfd = (int)bp.fdList.size();
-    for (int32_t b : bp.fdList) op->preMaxFd = std::max(op->preMaxFd, bp.blocks[(size_t)b].n);
-    HIPCHK(op->preFdList.alloc(std::max<size_t>(1, bp.fdList.size()) * sizeof(int32_t)));
-    if (!bp.fdList.empty())
-        HIPCHK(hipMemcpy(op->preFdList.p, bp.fdList.data(), bp.fdList.size() * sizeof(int32_t), hipMemcpyHostToDevice));
-    for (int t = 0; t <= FD_MAXLEV; ++t) {
-        op->preVOff[t] = (int32_t)bp.fdVoff[t];
-        op->preLOff[t] = (int32_t)bp.fdLoff[t];
-    }
+    op->preMaxFd = bp.maxFd;
+    auto upload32 = [&](DevMem& m, const std::vector<int32_t>& v) -> int {
+        HIPCHK(m.alloc(std::max<size_t>(1, v.size()) * sizeof(int32_t)));
+        if (!v.empty()) HIPCHK(hipMemcpy(m.p, v.data(), v.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        return LSG_OK;
+    };
+    if (int rc = upload32(op->preFdList, bp.fdList)) return rc;
+    if (int rc = upload32(op->preFdPerm, bp.fdPerm)) return rc;
     HIPCHK(op->prePart.alloc((size_t)op->batch * op->preNblk * sizeof(double)));
     return LSG_OK;
 }
@@ -783,12 +782,9 @@ BlockPreArgs<V> blockPreArgs(lsg_op* op, bool on) {
     a.nfd = op->preNfd;
     a.maxFd = op->preMaxFd;
     a.fdList = op->preFdList.as<int32_t>();
+    a.fdPerm = op->preFdPerm.as<int32_t>();
     a.fdV = op->preFdV.as<V>();
     a.fdLam = op->preFdLam.as<V>();
-    for (int t = 0; t <= FD_MAXLEV; ++t) {
-        a.vOff[t] = op->preVOff[t];
-        a.lOff[t] = op->preLOff[t];
-    }
     return a;
 }
 
@@ -1923,7 +1919,6 @@ int lsg_op_set_precond(lsg_op* op, int kind) {
     if (!op) return fail(LSG_ERR_ARG, "null operator");
     if (kind != LSG_PRECOND_DIAG && kind != LSG_PRECOND_DEPTH_BLOCK) return fail(LSG_ERR_ARG, "unknown preconditioner");
     if (kind == LSG_PRECOND_DEPTH_BLOCK) {
-        if (op->sparse) return fail(LSG_ERR_UNSUPPORTED, "the depth-block preconditioner needs complete depth blocks");
         if (int rc = ensureBlockPre(op)) return rc;
     }
     op->precKind = kind;

@@ -1311,10 +1311,10 @@ __global__ void __launch_bounds__(256) blockpre_kernel(const PreBlock* __restric
                                                        const CGState* __restrict__ st) {
     extern __shared__ char smem_raw[];
     const PreBlock it = items[blockIdx.x];
-    if (it.inv == -2) return;              // fast-diagonalisation block: blockfd_kernel
+    if (it.mode == PRE_FD) return;         // fast-diagonalisation block: blockfd_kernel
     const int nl = it.n;
     V* Bs = reinterpret_cast<V*>(smem_raw);
-    const bool dense = it.inv >= 0;
+    const bool dense = it.mode == PRE_DENSE;
     V* xs = Bs + (dense ? nl * nl : 0);
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     if (dense) {
@@ -1382,11 +1382,13 @@ __global__ void __launch_bounds__(256) blockpre_kernel(const PreBlock* __restric
 }
 
 // The larger depth blocks by fast diagonalisation of the Kronecker sum:
-// h = s .* (V Lam^-1 V^T (s .* r)) with V = (x)_i V_(T_i), applied mode by mode in LDS (block
-// points dimension 0 fastest, 2^(T_i - 1) along dimension i, so coordinate i is a bit field).
-// Workgroup (block, group of ipw instances); each wave transforms one instance at a time in
-// its own two LDS buffers; every wave of a workgroup runs the same passes, so the passes are
-// separated by workgroup barriers.
+// h = s .* (V Lam^-1 V^T (s .* r)) with V = (x)_i V_i over the block's product set of 1D
+// positions S_1 x ... x S_d, applied mode by mode in LDS (the product set ordered dimension 0
+// fastest, |S_i| along dimension i; a point order table maps it to the block's device order
+// when they differ, and lists the hanging points after it, which take h = r / D).  Workgroup
+// (block, group of ipw instances); each wave transforms one instance at a time in its own two
+// LDS buffers; every wave of a workgroup runs the same passes, so the passes are separated by
+// workgroup barriers.
 template <typename V, bool INIT>
 __global__ void __launch_bounds__(256) blockfd_kernel(BlockPreArgs<V> a, const V* __restrict__ r,
                                                       const V* __restrict__ D, V* __restrict__ h, V* __restrict__ dv,
@@ -1394,22 +1396,26 @@ __global__ void __launch_bounds__(256) blockfd_kernel(BlockPreArgs<V> a, const V
     extern __shared__ char smem_raw[];
     const int bi = a.fdList[blockIdx.x];
     const PreBlock it = a.items[bi];
-    const int nl = it.n, d = a.d;
+    const int nl = it.n, np = it.np, d = a.d;
+    const int32_t* perm = it.inv >= 0 ? a.fdPerm + it.inv : nullptr;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     V* A = reinterpret_cast<V*>(smem_raw) + (int64_t)wave * 2 * a.maxFd;
     V* B = A + a.maxFd;
-    int sh[MAXD], lev[MAXD];
-    int o = 0, ones = 0;
-    for (int i = 0; i < d; ++i) {
-        lev[i] = it.lev[i];
-        sh[i] = o;
-        o += lev[i] - 1;
-        ones += lev[i] == 1;
+    int str[MAXD], sh[MAXD];
+    bool p2 = true;                        // every extent a power of two: coordinates are bit fields
+    V f = V(1);                            // extents of 1: V_i = v (1 x 1) forward and backward
+    for (int i = 0, o = 1; i < d; ++i) {
+        const int m = it.m[i];
+        str[i] = o;
+        sh[i] = __builtin_ctz((unsigned)o);
+        p2 = p2 && (m & (m - 1)) == 0;
+        o *= m;
+        if (m == 1) {
+            const V v = a.fdV[it.fv[i]];
+            f *= v * v;
+        }
     }
-    // level-1 factors: V = v1 (1 x 1) forward and backward, lam_1 in every sum
-    const V v1 = a.fdV[a.vOff[1]];
-    V f = V(1);
-    for (int c = 0; c < ones; ++c) f *= v1 * v1;
+    auto coord = [&](int p, int i) { return p2 ? (p >> sh[i]) & (it.m[i] - 1) : (p / str[i]) % it.m[i]; };
     const int i0 = blockIdx.y * ipw;
     for (int q = 0; q < ipw; q += 4) {
         const int inst = i0 + q + wave;
@@ -1418,46 +1424,46 @@ __global__ void __launch_bounds__(256) blockfd_kernel(BlockPreArgs<V> a, const V
         V* X = A;
         V* Y = B;
         if (run)
-            for (int p = lane; p < nl; p += 64) X[p] = sqrt(a.d0[it.off + p] / D[ob + p]) * r[ob + p];
+            for (int p = lane; p < np; p += 64) {
+                const int64_t k = perm ? perm[p] : p;
+                X[p] = sqrt(a.d0[it.off + k] / D[ob + k]) * r[ob + k];
+            }
         __syncthreads();
-        // forward: along every dimension with more than one point, X <- V_t^T X
+        // forward: along every dimension with more than one position, X <- V_i^T X
         for (int i = 0; i < d; ++i) {
-            if (lev[i] == 1) continue;
-            const int m = 1 << (lev[i] - 1);
-            const V* Vt = a.fdV + a.vOff[lev[i]];
+            const int m = it.m[i];
+            if (m == 1) continue;
+            const V* Vt = a.fdV + it.fv[i];
             if (run)
-                for (int p = lane; p < nl; p += 64) {
-                    const int k = (p >> sh[i]) & (m - 1);
-                    const int base = p - (k << sh[i]);
+                for (int p = lane; p < np; p += 64) {
+                    const int k = coord(p, i);
+                    const int base = p - k * str[i];
                     V acc = V(0);
-                    for (int j = 0; j < m; ++j) acc = fma(Vt[j * m + k], X[base + (j << sh[i])], acc);
+                    for (int j = 0; j < m; ++j) acc = fma(Vt[j * m + k], X[base + j * str[i]], acc);
                     Y[p] = acc;
                 }
             __syncthreads();
             V* t = X; X = Y; Y = t;
         }
-        // 1 / (sum of the 1D eigenvalues), times the level-1 factors
+        // 1 / (sum of the 1D eigenvalues), times the factors of the extents of 1
         if (run)
-            for (int p = lane; p < nl; p += 64) {
+            for (int p = lane; p < np; p += 64) {
                 V ls = V(0);
-                for (int i = 0; i < d; ++i) {
-                    const int m = 1 << (lev[i] - 1);
-                    ls += a.fdLam[a.lOff[lev[i]] + ((p >> sh[i]) & (m - 1))];
-                }
+                for (int i = 0; i < d; ++i) ls += a.fdLam[it.fl[i] + coord(p, i)];
                 X[p] = f * X[p] / ls;
             }
         __syncthreads();
-        // backward: X <- V_t X along the same dimensions
+        // backward: X <- V_i X along the same dimensions
         for (int i = 0; i < d; ++i) {
-            if (lev[i] == 1) continue;
-            const int m = 1 << (lev[i] - 1);
-            const V* Vt = a.fdV + a.vOff[lev[i]];
+            const int m = it.m[i];
+            if (m == 1) continue;
+            const V* Vt = a.fdV + it.fv[i];
             if (run)
-                for (int p = lane; p < nl; p += 64) {
-                    const int k = (p >> sh[i]) & (m - 1);
-                    const int base = p - (k << sh[i]);
+                for (int p = lane; p < np; p += 64) {
+                    const int k = coord(p, i);
+                    const int base = p - k * str[i];
                     V acc = V(0);
-                    for (int j = 0; j < m; ++j) acc = fma(Vt[k * m + j], X[base + (j << sh[i])], acc);
+                    for (int j = 0; j < m; ++j) acc = fma(Vt[k * m + j], X[base + j * str[i]], acc);
                     Y[p] = acc;
                 }
             __syncthreads();
@@ -1466,10 +1472,11 @@ __global__ void __launch_bounds__(256) blockfd_kernel(BlockPreArgs<V> a, const V
         double acc = 0.0;
         if (run) {
             for (int p = lane; p < nl; p += 64) {
-                const V rk = r[ob + p];
-                const V hk = sqrt(a.d0[it.off + p] / D[ob + p]) * X[p];
-                h[ob + p] = hk;
-                if (INIT) dv[ob + p] = V(-1) * hk;
+                const int64_t k = perm ? perm[p] : p;
+                const V rk = r[ob + k];
+                const V hk = p < np ? sqrt(a.d0[it.off + k] / D[ob + k]) * X[p] : rk / D[ob + k];
+                h[ob + k] = hk;
+                if (INIT) dv[ob + k] = V(-1) * hk;
                 acc += double(rk) * double(hk);
             }
             acc = wave_sum(acc);

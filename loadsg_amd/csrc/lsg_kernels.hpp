@@ -159,13 +159,13 @@ struct BlockPreArgs {
     const V* inv = nullptr;
     const V* d0 = nullptr;
     double* part = nullptr;
-    // fast diagonalisation of the larger blocks (BlockPre::fdV / fdLam / fdList)
+    // fast diagonalisation of the larger blocks (BlockPre::fdV / fdLam / fdList / fdPerm)
     int d = 0;
-    int nfd = 0, maxFd = 0; // blocks on that path, the largest of them
+    int nfd = 0, maxFd = 0; // blocks on that path, the largest product set of them
     const int32_t* fdList = nullptr;
+    const int32_t* fdPerm = nullptr;
     const V* fdV = nullptr;
     const V* fdLam = nullptr;
-    int32_t vOff[FD_MAXLEV + 1] = {}, lOff[FD_MAXLEV + 1] = {};
 };
 template <typename V>
 void launch_cg_init(const V* f, V* r, V* h, V* dv, const V* D, int64_t n, int batch, double* partials, int nb,

@@ -834,37 +834,35 @@ int buildGrid(int d, int64_t n, const int32_t* idx, const uint8_t* active, Grid&
     return buildPlan(g, g.plan, g.sparse, err);
 }
 
-// 1D Galerkin matrices of the level-t prewavelets (odd positions k = 2j + 1 of the level-t
-// full grid, zero Dirichlet ends): K = Psi^T K_t Psi, M = Psi^T M_t Psi with the nodal
-// K_t = 2^t tridiag(-1, 2, -1) and M_t = 2^-t / 6 tridiag(1, 4, 1); m = 2^(t-1), row-major.
-static void prewavelet1DMatrices(int t, std::vector<double>& K, std::vector<double>& M) {
-    const int n = (1 << t) - 1, m = 1 << (t - 1);
-    std::vector<double> P((size_t)n * m, 0.0);
-    for (int j = 0; j < m; ++j) {
-        int32_t pos[5];
-        double c[5];
-        const int cnt = prewavelet1D(2 * j + 1, t, pos, c);
-        for (int e = 0; e < cnt; ++e) P[(size_t)(pos[e] - 1) * m + j] = c[e];
-    }
+// 1D Galerkin entries of the level-t prewavelets j and l (odd positions 2j + 1 and 2l + 1 of
+// the level-t full grid, zero Dirichlet ends): K = Psi^T K_t Psi, M = Psi^T M_t Psi with the
+// nodal K_t = 2^t tridiag(-1, 2, -1) and M_t = 2^-t / 6 tridiag(1, 4, 1).  A prewavelet spans
+// five nodal positions, so entries with |j - l| > 2 vanish.
+static void prewavelet1DPair(int t, int j, int l, double& k, double& m) {
+    k = m = 0.0;
+    if (std::abs(j - l) > 2) return;
+    int32_t pa[5], pb[5];
+    double ca[5], cb[5];
+    const int na = prewavelet1D(2 * j + 1, t, pa, ca), nb = prewavelet1D(2 * l + 1, t, pb, cb);
     const double h = std::ldexp(1.0, -t);
+    for (int e = 0; e < na; ++e)
+        for (int f = 0; f < nb; ++f) {
+            const int dq = std::abs(pa[e] - pb[f]);
+            const double c = ca[e] * cb[f];
+            if (dq == 0) { k += 2.0 * c / h; m += 4.0 * c * h / 6.0; }
+            else if (dq == 1) { k -= c / h; m += c * h / 6.0; }
+        }
+}
+
+// The restrictions K[S, S], M[S, S] of the level-t matrices to the positions S (row-major).
+static void prewavelet1DMatrices(int t, const std::vector<int32_t>& S, std::vector<double>& K,
+                                 std::vector<double>& M) {
+    const int m = (int)S.size();
     K.assign((size_t)m * m, 0.0);
     M.assign((size_t)m * m, 0.0);
     for (int a = 0; a < m; ++a)
-        for (int b = 0; b < m; ++b) {
-            double ks = 0.0, ms = 0.0;
-            for (int q = 0; q < n; ++q) {
-                const double pa = P[(size_t)q * m + a];
-                if (pa == 0.0) continue;
-                // (K_t P)_(q, b) and (M_t P)_(q, b)
-                const double pq = P[(size_t)q * m + b];
-                const double pl = q > 0 ? P[(size_t)(q - 1) * m + b] : 0.0;
-                const double pr = q + 1 < n ? P[(size_t)(q + 1) * m + b] : 0.0;
-                ks += pa * (2.0 * pq - pl - pr) / h;
-                ms += pa * (4.0 * pq + pl + pr) * h / 6.0;
-            }
-            K[(size_t)a * m + b] = ks;
-            M[(size_t)a * m + b] = ms;
-        }
+        for (int b = 0; b < m; ++b)
+            prewavelet1DPair(t, S[a], S[b], K[(size_t)a * m + b], M[(size_t)a * m + b]);
 }
 
 // Generalised symmetric eigenproblem K v = lam M v (M positive definite, m x m row-major):
@@ -951,93 +949,143 @@ static int generalisedEigen(int m, const std::vector<double>& K, const std::vect
 }
 
 int buildBlockPre(const Grid& g, int maxDense, BlockPre& out, std::string& err) {
-    if (g.sparse) { err = "the depth-block preconditioner needs complete depth blocks"; return LSG_ERR_UNSUPPORTED; }
     const int d = g.d;
     const int nd = (int)g.depths.size();
     out = BlockPre();
-    out.d0.assign((size_t)g.n, 0.0);
-    std::map<int, std::pair<std::vector<double>, std::vector<double>>> mats;   // level -> (K, M)
-    auto mat = [&](int t) -> const std::pair<std::vector<double>, std::vector<double>>& {
-        auto it = mats.find(t);
-        if (it == mats.end()) {
-            std::pair<std::vector<double>, std::vector<double>> km;
-            prewavelet1DMatrices(t, km.first, km.second);
-            it = mats.emplace(t, std::move(km)).first;
-        }
-        return it->second;
-    };
-    // level 1 of the fast diagonalisation: m = 1, K = 2^1 * ..., stored like the others
-    {
-        const auto& km = mat(1);
-        out.fdVoff[1] = (int64_t)out.fdV.size();
-        out.fdLoff[1] = (int64_t)out.fdLam.size();
-        out.fdV.push_back(1.0 / std::sqrt(km.second[0]));
-        out.fdLam.push_back(km.first[0] / km.second[0]);
-    }
+    out.d0.assign((size_t)g.n, 1.0);
+    // 1D factors of the fast diagonalisation, shared by the blocks with the same (t, S)
+    std::map<std::pair<int, std::vector<int32_t>>, std::pair<int32_t, int32_t>> factors;
+    std::vector<int32_t> loc;       // local 1D positions of the block's points, nb x d
+    std::vector<int32_t> act;       // block offsets of its active points
     for (int t = 0; t < nd; ++t) {
         const Depth& T = g.depths[t];
         const int64_t nb = g.blkSize[t];
-        PreBlock pb{g.blkOff[t], (int32_t)nb, -1, {}};
-        int m[MAXD];
-        for (int i = 0; i < d; ++i) m[i] = 1 << (T[i] - 1);
-        // local 1D indices of point p of the block (dimension 0 fastest)
-        auto local = [&](int64_t p, int i) {
-            for (int q = 0; q < i; ++q) p /= m[q];
-            return (int)(p % m[i]);
-        };
-        auto entry = [&](int64_t p, int64_t q) {
-            double s = 0.0;
+        PreBlock pb{};
+        pb.off = g.blkOff[t];
+        pb.n = (int32_t)nb;
+        pb.mode = PRE_DIAG;
+        pb.inv = -1;
+        loc.assign((size_t)nb * d, 0);
+        act.clear();
+        for (int64_t p = 0; p < nb; ++p) {
+            const int64_t s = g.slotOfDev[(size_t)(pb.off + p)];
+            for (int i = 0; i < d; ++i) loc[(size_t)p * d + i] = g.idx[(size_t)s * d + i] - (1 << T[i]);
+            if (g.active[(size_t)s]) act.push_back((int32_t)p);
+        }
+        // diagonal of B_T at the active points
+        for (int32_t p : act) {
+            double kd[MAXD], md[MAXD], sum = 0.0;
+            for (int j = 0; j < d; ++j)
+                prewavelet1DPair(T[j], loc[(size_t)p * d + j], loc[(size_t)p * d + j], kd[j], md[j]);
             for (int i = 0; i < d; ++i) {
-                const auto& Ki = mat(T[i]).first;
-                double prod = Ki[(size_t)local(p, i) * m[i] + local(q, i)];
-                for (int j = 0; j < d && prod != 0.0; ++j) {
-                    if (j == i) continue;
-                    const auto& Mj = mat(T[j]).second;
-                    prod *= Mj[(size_t)local(p, j) * m[j] + local(q, j)];
-                }
-                s += prod;
+                double prod = kd[i];
+                for (int j = 0; j < d; ++j)
+                    if (j != i) prod *= md[j];
+                sum += prod;
             }
-            return s;
-        };
-        for (int64_t p = 0; p < nb; ++p) out.d0[(size_t)(pb.off + p)] = entry(p, p);
-        for (int i = 0; i < MAXD; ++i) pb.lev[i] = i < d ? T[i] : 0;
-        bool fd = nb > maxDense && nb <= FD_MAXN;
-        for (int i = 0; i < d; ++i) fd = fd && T[i] <= FD_MAXLEV;
+            out.d0[(size_t)(pb.off + p)] = sum;
+        }
+        // the 1D position sets of the active points and whether they form their product
+        std::vector<std::vector<int32_t>> S(d);
+        int64_t prod = act.empty() ? 0 : 1;
+        for (int i = 0; i < d; ++i) {
+            for (int32_t p : act) S[i].push_back(loc[(size_t)p * d + i]);
+            std::sort(S[i].begin(), S[i].end());
+            S[i].erase(std::unique(S[i].begin(), S[i].end()), S[i].end());
+            prod *= (int64_t)S[i].size();
+        }
+        bool fd = nb > maxDense && !act.empty() && prod == (int64_t)act.size() && prod <= FD_MAXN;
+        for (int i = 0; i < d && fd; ++i) fd = (int)S[i].size() <= FD_MAXM;
         if (fd) {
             for (int i = 0; i < d; ++i) {
-                const int t = T[i];
-                if (out.fdLoff[t] || t == 1) continue;     // level 1: V = 1 / sqrt(M), one value
-                const auto& km = mat(t);
-                const int mt = 1 << (t - 1);
-                std::vector<double> V, lam;
-                if (generalisedEigen(mt, km.first, km.second, V, lam)) {
-                    err = "1D prewavelet mass matrix not positive definite";
-                    return LSG_ERR_ARG;
+                auto key = std::make_pair((int)T[i], S[i]);
+                auto it = factors.find(key);
+                if (it == factors.end()) {
+                    std::vector<double> K, M, V, lam;
+                    prewavelet1DMatrices(T[i], S[i], K, M);
+                    if (generalisedEigen((int)S[i].size(), K, M, V, lam)) {
+                        err = "1D prewavelet mass matrix not positive definite";
+                        return LSG_ERR_ARG;
+                    }
+                    it = factors.emplace(key, std::make_pair((int32_t)out.fdV.size(), (int32_t)out.fdLam.size())).first;
+                    out.fdV.insert(out.fdV.end(), V.begin(), V.end());
+                    out.fdLam.insert(out.fdLam.end(), lam.begin(), lam.end());
                 }
-                out.fdVoff[t] = (int64_t)out.fdV.size();
-                out.fdLoff[t] = (int64_t)out.fdLam.size();
-                out.fdV.insert(out.fdV.end(), V.begin(), V.end());
-                out.fdLam.insert(out.fdLam.end(), lam.begin(), lam.end());
+                pb.m[i] = (int32_t)S[i].size();
+                pb.fv[i] = it->second.first;
+                pb.fl[i] = it->second.second;
             }
-            pb.inv = -2;
+            // point order: the product set dimension 0 fastest, then the hanging points
+            std::vector<int32_t> order((size_t)nb, -1);
+            std::vector<char> used((size_t)nb, 0);
+            for (int32_t p : act) {
+                int64_t r = 0, mul = 1;
+                for (int i = 0; i < d; ++i) {
+                    const int32_t k = (int32_t)(std::lower_bound(S[i].begin(), S[i].end(), loc[(size_t)p * d + i]) -
+                                                S[i].begin());
+                    r += k * mul;
+                    mul *= S[i].size();
+                }
+                order[(size_t)r] = p;
+                used[(size_t)p] = 1;
+            }
+            int64_t fill = prod;
+            for (int64_t p = 0; p < nb; ++p)
+                if (!used[(size_t)p]) order[(size_t)fill++] = (int32_t)p;
+            bool identity = true;
+            for (int64_t r = 0; r < nb && identity; ++r) identity = order[(size_t)r] == r;
+            if (!identity) {
+                pb.inv = (int32_t)out.fdPerm.size();
+                out.fdPerm.insert(out.fdPerm.end(), order.begin(), order.end());
+            }
+            pb.np = (int32_t)prod;
+            pb.mode = PRE_FD;
+            out.maxFd = std::max(out.maxFd, (int)prod);
             out.fdList.push_back((int32_t)out.blocks.size());
         } else if (nb <= maxDense) {
-            // Cholesky factor of the block, then its inverse column by column
+            // Cholesky factor of the block (the identity at hanging points), then its inverse
+            // column by column
             const int n = (int)nb;
-            std::vector<double> A((size_t)n * n), L((size_t)n * n, 0.0);
-            for (int a = 0; a < n; ++a)
-                for (int b = 0; b < n; ++b) A[(size_t)a * n + b] = entry(a, b);
+            // 1D matrices over the positions the block's points take along every dimension
+            std::vector<std::vector<int32_t>> P(d);
+            std::vector<std::vector<double>> K1(d), M1(d);
+            std::vector<int32_t> rk((size_t)n * d);
+            for (int i = 0; i < d; ++i) {
+                for (int p = 0; p < n; ++p) P[i].push_back(loc[(size_t)p * d + i]);
+                std::sort(P[i].begin(), P[i].end());
+                P[i].erase(std::unique(P[i].begin(), P[i].end()), P[i].end());
+                for (int p = 0; p < n; ++p)
+                    rk[(size_t)p * d + i] = (int32_t)(std::lower_bound(P[i].begin(), P[i].end(),
+                                                                       loc[(size_t)p * d + i]) - P[i].begin());
+                prewavelet1DMatrices(T[i], P[i], K1[i], M1[i]);
+            }
+            std::vector<double> A((size_t)n * n, 0.0), L((size_t)n * n, 0.0);
+            for (int a = 0; a < n; ++a) A[(size_t)a * n + a] = 1.0;
+            for (int32_t a : act)
+                for (int32_t b : act) {
+                    double sum = 0.0;
+                    for (int i = 0; i < d; ++i) {
+                        double prod = 1.0;
+                        for (int j = 0; j < d && prod != 0.0; ++j) {
+                            const size_t e = (size_t)rk[(size_t)a * d + j] * P[j].size() + rk[(size_t)b * d + j];
+                            prod *= j == i ? K1[j][e] : M1[j][e];
+                        }
+                        sum += prod;
+                    }
+                    A[(size_t)a * n + b] = sum;
+                }
             for (int j = 0; j < n; ++j) {
-                double s = A[(size_t)j * n + j];
-                for (int q = 0; q < j; ++q) s -= L[(size_t)j * n + q] * L[(size_t)j * n + q];
-                if (!(s > 0.0)) { err = "depth block not positive definite"; return LSG_ERR_ARG; }
-                L[(size_t)j * n + j] = std::sqrt(s);
+                double v0 = A[(size_t)j * n + j];
+                for (int q = 0; q < j; ++q) v0 -= L[(size_t)j * n + q] * L[(size_t)j * n + q];
+                if (!(v0 > 0.0)) { err = "depth block not positive definite"; return LSG_ERR_ARG; }
+                L[(size_t)j * n + j] = std::sqrt(v0);
                 for (int i = j + 1; i < n; ++i) {
                     double v = A[(size_t)i * n + j];
                     for (int q = 0; q < j; ++q) v -= L[(size_t)i * n + q] * L[(size_t)j * n + q];
                     L[(size_t)i * n + j] = v / L[(size_t)j * n + j];
                 }
             }
+            pb.mode = PRE_DENSE;
             pb.inv = (int32_t)out.inv.size();
             out.inv.resize(out.inv.size() + (size_t)n * n);
             double* inv = &out.inv[(size_t)pb.inv];
@@ -1057,6 +1105,11 @@ int buildBlockPre(const Grid& g, int maxDense, BlockPre& out, std::string& err) 
             out.maxN = std::max(out.maxN, n);
         }
         out.blocks.push_back(pb);
+    }
+    if ((int64_t)out.inv.size() > INT32_MAX || (int64_t)out.fdV.size() > INT32_MAX ||
+        (int64_t)out.fdPerm.size() > INT32_MAX) {
+        err = "depth-block preconditioner tables too large";
+        return LSG_ERR_UNSUPPORTED;
     }
     return LSG_OK;
 }

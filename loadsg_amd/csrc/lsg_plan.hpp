@@ -256,32 +256,44 @@ int prewaveletStencil1D(int32_t k, int t, int32_t pos[5], double c[5]);
 void regularIndexTable(int d, int level, std::vector<int32_t>& idx);
 
 // Depth-block preconditioner of the inner (mixed-precision) solves: for every depth block
-// T of a dense grid the Galerkin block of the constant-coefficient Laplacian on the
-// prewavelets of depth T, B_T = sum_i K_(T_i) (x) prod_(j != i) M_(T_j) with the 1D prewavelet
-// stiffness / mass matrices of level t (no counterpart in the reference, whose CG uses the
-// diagonal).  Blocks of at most maxDense points keep their inverse.  Larger ones use the fast
-// diagonalisation of the Kronecker sum: with V_t^T M_t V_t = I and V_t^T K_t V_t = diag(lam_t)
-// (generalised eigenvectors of level t), B_T^-1 = (x)_i V_(T_i) diag(1 / sum_i lam_(T_i)) (x)_i
-// V_(T_i)^T, for levels up to FD_MAXLEV; blocks with a deeper level keep the diagonal.
-constexpr int FD_MAXLEV = 9;      // 1D eigenproblems up to 256 x 256
-constexpr int FD_MAXN = 1024;     // largest block on the fast-diagonalisation path
+// T the Galerkin block of the constant-coefficient Laplacian on the prewavelets of depth T,
+// B_T = sum_i K_(T_i) (x) prod_(j != i) M_(T_j) with the 1D prewavelet stiffness / mass
+// matrices of level t (no counterpart in the reference, whose CG uses the diagonal).  On
+// adaptive grids the block is taken over the active points of depth T (hanging points: the
+// identity, as the operator's diagonal is 1 there).  Blocks of at most maxDense points keep
+// their inverse.  Larger ones whose active points form a product set S_1 x ... x S_d of 1D
+// positions (every depth block of a dense grid; the blocks of grids refined towards a corner)
+// use the fast diagonalisation of the Kronecker sum: with V_i^T M_i V_i = I and
+// V_i^T K_i V_i = diag(lam_i) for the restrictions K_i = K_(T_i)[S_i, S_i],
+// M_i = M_(T_i)[S_i, S_i], B^-1 = (x)_i V_i diag(1 / sum_i lam_i) (x)_i V_i^T; up to
+// FD_MAXM positions per dimension and FD_MAXN points.  Other blocks keep the diagonal.
+constexpr int FD_MAXM = 256;      // 1D eigenproblems up to 256 x 256
+constexpr int FD_MAXN = 2048;     // largest product set on the fast-diagonalisation path
+enum PreMode : int32_t { PRE_DIAG = 0, PRE_DENSE = 1, PRE_FD = 2 };
 struct PreBlock {
     int64_t off;      // first point of the block (device order)
     int32_t n;        // points of the block
-    int32_t inv;      // >= 0: first entry of its inverse in BlockPre::inv (n x n, column k
-                      // contiguous); -1: diagonal; -2: fast diagonalisation
-    int32_t lev[MAXD];    // the depth T (fast diagonalisation: the levels of the 1D factors)
+    int32_t mode;     // PreMode
+    int32_t inv;      // PRE_DENSE: first entry of the inverse in BlockPre::inv (n x n, column k
+                      // contiguous); PRE_FD: first entry of the block's point order in
+                      // BlockPre::fdPerm, or -1 when it is the identity
+    int32_t np;       // PRE_FD: points of the product set (the first np of the point order;
+                      // the other n - np are hanging points)
+    int32_t m[MAXD];  // PRE_FD: |S_i| (the product set is ordered dimension 0 fastest)
+    int32_t fv[MAXD]; // PRE_FD: V_i at fdV[fv[i]] (row j = position j of S_i, column k =
+                      // eigenvector k)
+    int32_t fl[MAXD]; // PRE_FD: lam_i at fdLam[fl[i]]
 };
 struct BlockPre {
     std::vector<PreBlock> blocks;   // depth-list order
     std::vector<double> inv;        // block inverses
     std::vector<double> d0;         // diagonal of the constant-coefficient blocks, device order
+                                    // (1 at hanging points)
     int maxN = 0;                   // largest block with an inverse
-    // fast diagonalisation: V_t row-major (row j = basis function j, column k = eigenvector k)
-    // at fdV[fdVoff[t]], lam_t at fdLam[fdLoff[t]] (levels 1 .. FD_MAXLEV; 2^(t-1) each)
-    std::vector<double> fdV, fdLam;
-    int64_t fdVoff[FD_MAXLEV + 1] = {}, fdLoff[FD_MAXLEV + 1] = {};
+    std::vector<double> fdV, fdLam; // 1D factors of the fast diagonalisation
     std::vector<int32_t> fdList;    // blocks on the fast-diagonalisation path
+    std::vector<int32_t> fdPerm;    // point orders (offsets inside the block)
+    int maxFd = 0;                  // largest product set
 };
 int buildBlockPre(const Grid& g, int maxDense, BlockPre& out, std::string& err);
 

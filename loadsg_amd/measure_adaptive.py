@@ -46,8 +46,11 @@ def _matvec_ms(op, x, y, reps):
     return ev0.elapsed_time(ev1) / reps
 
 
-def run(grid_path, batch=64, reps=20, tol=1e-10, matvec_only=False):
-    """Measure the sparse path on the grid in `grid_path`; returns a dict."""
+def run(grid_path, batch=64, reps=20, tol=1e-10, matvec_only=False, precond="block"):
+    """Measure the sparse path on the grid in `grid_path`; returns a dict.  precond: the
+    preconditioner of the FP32 inner solves of the mixed-precision method ("block": depth
+    blocks over the active points, "diag": the diagonal); the batched mixed solve is also timed
+    with the diagonal for comparison."""
     import torch
     import loadsg_amd as L
 
@@ -78,11 +81,17 @@ def run(grid_path, batch=64, reps=20, tol=1e-10, matvec_only=False):
     if matvec_only:
         return out
 
-    def ops(B):
-        return (L.MatrixVectorHomogen(grid, coef, precision=64, batch=B),
-                L.MatrixVectorHomogen(grid, coef, precision=32, batch=B))
+    pk = L.PRECOND_DEPTH_BLOCK if precond == "block" else L.PRECOND_DIAG
+    out["inner_preconditioner"] = "depth-block" if precond == "block" else "diagonal"
 
-    out["manufactured"] = manufactured(grid, tol)
+    def ops(B):
+        op32 = L.MatrixVectorHomogen(grid, coef, precision=32, batch=B)
+        t0 = time.perf_counter()
+        op32.set_precond(pk)
+        out["precond_setup_s"] = time.perf_counter() - t0
+        return L.MatrixVectorHomogen(grid, coef, precision=64, batch=B), op32
+
+    out["manufactured"] = manufactured(grid, tol, pk)
     print("manufactured solution done", flush=True, file=sys.stderr)
 
     # plain FP64 PCG, batch 1
@@ -133,19 +142,31 @@ def run(grid_path, batch=64, reps=20, tol=1e-10, matvec_only=False):
                mixed_batch_all_converged=bool(all(i["converged"] == 1 for i in ib)),
                mixed_batch_max_err_vs_fp64=float(err.max()))
     out["speedup_mixed_batch_vs_fp64_b1"] = out["mixed_batch_solves_per_s"] / out["fp64_cg_b1_solves_per_s"]
+    if pk != L.PRECOND_DIAG:
+        # the same batch with the diagonal in the inner solves
+        op32.set_precond(L.PRECOND_DIAG)
+        xd = torch.zeros_like(f)
+        L.CG.solveMixed(tol, xd.clone(), f, op64, op32)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        idg = L.CG.solveMixed(tol, xd, f, op64, op32)
+        torch.cuda.synchronize()
+        td = time.perf_counter() - t0
+        out.update(mixed_batch_diag_solves_per_s=B / td,
+                   mixed_batch_diag_mean_inner_iterations=float(np.mean([i["iterations"] for i in idg])))
     return out
 
 
-def manufactured(grid, tol):
-    """Config 3's manufactured solution on `grid`: right-hand side, FP64 and mixed solves,
-    errors (see the module docstring)."""
+def manufactured(grid, tol, precond):
+    """Config 3's manufactured solution on `grid`: right-hand side, FP64 and mixed solves
+    (inner preconditioner `precond`), errors (see the module docstring)."""
     import torch
     import loadsg_amd as L
     from loadsg_amd.manufactured import cfg3, coordinates
     P = cfg3()
     r = {"u": "|x|^(2/3) prod x_i (1 - x_i)", "a": "1 + |x|^2", "rhs": "f_h = M prew(f_I)"}
     op64 = L.MatrixVectorHomogen(grid, P.stencil, precision=64)
-    op32 = L.MatrixVectorHomogen(grid, P.stencil, precision=32)
+    op32 = L.MatrixVectorHomogen(grid, P.stencil, precision=32).set_precond(precond)
     t0 = time.perf_counter()
     f = P.reference_rhs(grid, op64)          # first call: transform tables and the mass operator
     r["rhs_first_call_s"] = time.perf_counter() - t0
@@ -200,8 +221,10 @@ def main():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--matvec-only", action="store_true", help="time the matvecs only (for profiling)")
+    ap.add_argument("--precond", choices=("block", "diag"), default="block",
+                    help="preconditioner of the FP32 inner solves")
     args = ap.parse_args()
-    print(json.dumps(run(args.grid, args.batch, args.reps, args.tol, args.matvec_only)), flush=True)
+    print(json.dumps(run(args.grid, args.batch, args.reps, args.tol, args.matvec_only, args.precond)), flush=True)
 
 
 if __name__ == "__main__":

@@ -7,10 +7,12 @@ constant-coefficient Laplacian on the prewavelets of depth T,
     B_T = sum_i K_(T_i) (x) prod_(j != i) M_(T_j),
 with the 1D prewavelet stiffness and mass matrices of level t (prewavelets of the odd level-t
 positions: 1 / -0.6 / 0.1, 0.9 at the centre next to the boundary, as in
-indices/index.cc:180-327), and applies h = s .* (B_T^-1 (s .* r)) with s = sqrt(D0 / D).  The
-library keeps dense inverses up to max_dense points (128 on FP32 operators, 64 on FP64 ones) and
-applies larger blocks of up to max_fd points whose levels are at most fd_maxlev by fast
-diagonalisation, which is the same B_T^-1 up to rounding; other blocks fall back to h = r / D.
+indices/index.cc:180-327), and applies h = s .* (B_T^-1 (s .* r)) with s = sqrt(D0 / D).  On adaptive grids the blocks
+are taken over the active points (hanging points keep h = r / D).  The library keeps dense
+inverses up to max_dense points (128 on FP32 operators, 64 on FP64 ones) and applies larger
+blocks whose active points form a product set of 1D positions (up to fd_maxm per dimension and
+max_fd points) by fast diagonalisation, which is the same B_T^-1 up to rounding; other blocks
+fall back to h = r / D.
 tests/test_blockpre_ref.py checks the Kronecker blocks against the oracle's constant-coefficient
 matvec and the fast diagonalisation against the dense inverse.
 """
@@ -73,45 +75,57 @@ def constant_blocks(idx):
 
 
 def fast_diagonal_inverse(T, idx_block):
-    """B_T^-1 by fast diagonalisation: (x)_i V_(T_i) diag(1 / sum_i lam) (x)_i V_(T_i)^T with
-    the generalised eigenvectors K V = M V diag(lam), V^T M V = I of every 1D factor."""
+    """B_T^-1 by fast diagonalisation, for a block whose points form a product set
+    S_1 x ... x S_d of 1D positions: (x)_i V_i diag(1 / sum_i lam_i) (x)_i V_i^T with the
+    generalised eigenvectors K_i V_i = M_i V_i diag(lam_i), V_i^T M_i V_i = I of the restrictions
+    K_i = K_(T_i)[S_i, S_i], M_i = M_(T_i)[S_i, S_i]."""
     import scipy.linalg as sl
-    groups, loc = depth_blocks(idx_block)
-    L = loc
+    _, loc = depth_blocks(idx_block)
     d = len(T)
-    fac = []
-    for t in T:
-        K, M = prewavelet_1d(t)
-        lam, V = sl.eigh(K, M)
-        fac.append((lam, V))
     nb = len(idx_block)
     Vfull = np.ones((nb, nb))
     lamsum = np.zeros(nb)
     for i in range(d):
-        lam, V = fac[i]
-        Vfull = Vfull * V[np.ix_(L[:, i], L[:, i])]
-        lamsum += lam[L[:, i]]
+        S, rank = np.unique(loc[:, i], return_inverse=True)
+        K, M = prewavelet_1d(T[i])
+        lam, V = sl.eigh(K[np.ix_(S, S)], M[np.ix_(S, S)])
+        Vfull = Vfull * V[np.ix_(rank, rank)]
+        lamsum += lam[rank]
     return Vfull @ np.diag(1.0 / lamsum) @ Vfull.T
 
 
-def make_preconditioner(idx, D, max_dense=128, max_fd=1024, fd_maxlev=9):
-    """The preconditioner as a function r -> h for the diagonal D of the operator."""
+def make_preconditioner(idx, D, max_dense=128, max_fd=2048, fd_maxm=256, active=None):
+    """The preconditioner as a function r -> h for the diagonal D of the operator.
+
+    Blocks are taken over the active points of a depth (hanging points: h = r / D, which is r
+    as the operator's diagonal is 1 there).  A block keeps its exact inverse when it has at most
+    max_dense points (hanging ones included), or when its active points form a product set
+    S_1 x ... x S_d of 1D positions with |S_i| <= fd_maxm and at most max_fd points (the library
+    applies those by fast diagonalisation); other blocks take h = r / D."""
     blocks = constant_blocks(idx)
+    _, loc = depth_blocks(idx)
     n = len(idx)
-    D0 = np.zeros(n)
+    act = np.ones(n, bool) if active is None else np.asarray(active) == 1
+    D0 = np.ones(n)
     parts = []
     for T, (ids, B) in blocks.items():
-        D0[ids] = np.diag(B)
-        exact = len(ids) <= max_dense or (len(ids) <= max_fd and max(T) <= fd_maxlev)
-        parts.append((ids, np.linalg.inv(B) if exact else None))
+        keep = act[ids]
+        ida = ids[keep]
+        if len(ida) == 0:
+            continue
+        Bk = B[np.ix_(keep, keep)]
+        D0[ida] = np.diag(Bk)
+        sets = [np.unique(loc[ida, i]) for i in range(len(T))]
+        product = (int(np.prod([len(q) for q in sets])) == len(ida) and len(ida) <= max_fd
+                   and max(len(q) for q in sets) <= fd_maxm)
+        exact = len(ids) <= max_dense or product
+        parts.append((ida, np.linalg.inv(Bk) if exact else None))
     s = np.sqrt(D0 / D)
 
     def apply(r):
-        h = np.empty_like(r)
+        h = r / D
         for ids, Binv in parts:
-            if Binv is None:
-                h[ids] = r[ids] / D[ids]
-            else:
+            if Binv is not None:
                 h[ids] = s[ids] * (Binv @ (s[ids] * r[ids]))
         return h
     return apply

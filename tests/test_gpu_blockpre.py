@@ -8,7 +8,8 @@ inverses (up to 64 points on FP64 operators) and the fast diagonalisation of lar
 mixed solve keeps dense blocks of up to 128 points (d = 5, level 7).  The mixed-precision solve
 with the preconditioner on its FP32 operator reaches 1e-10 in fewer inner iterations than with
 the diagonal; batches match single instances bit for bit; graph replays match direct launches
-bit for bit.
+bit for bit.  On adaptive grids (hanging points, partly filled depth blocks) the blocks are
+taken over the active points, and the fast diagonalisation over product sets of 1D positions.
 """
 import os
 
@@ -86,14 +87,85 @@ def test_mixed_solve_with_block_preconditioner(level):
     assert float(torch.max(torch.linalg.norm(xd - xb, dim=1) / torch.linalg.norm(xd, dim=1))) < 1e-8
 
 
-def test_block_batch_matches_single_and_graph_replay():
+def _adaptive_grid(name):
+    if name == "golden":
+        from test_gpu_parity import load
+        gv = load("d3_adaptive_cfg3")
+        return np.asarray(gv["idx"]), np.asarray(gv["active"])
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "grids", name + ".npz"))
+    return z["idx"], z["active"]
+
+
+@pytest.mark.parametrize("name", ["golden", "cfg3_start4_level8", "cfg3_start4_level10"])
+def test_block_pcg_on_adaptive_grids(name):
+    """Grids refined towards a corner (hanging points, partly filled depth blocks): dense
+    inverses of the small blocks (hanging points as the identity), fast diagonalisation of
+    the product sets of the larger ones in a point order that differs from the device order;
+    the cfg3 coefficient 1 + |x|^2."""
+    print("adaptive", name, flush=True)
+    idx, act = _adaptive_grid(name)
+    grid = L.AdaptiveSparseGrid(idx, act)
+    op = L.MatrixVectorHomogen(grid, L.VariableCoefficient(L.COEF_CFG3))
+    D = op.diagonal()
+    assert np.all(D[act == 0] == 1.0)
+    b = np.random.default_rng(len(idx)).standard_normal(len(idx))
+    b[act == 0] = 0.0
+    xr, itr = pcg(op.multiplication, b, make_preconditioner(idx, D, max_dense=64, active=act), 1e-10)
+    op.set_precond(L.PRECOND_DEPTH_BLOCK)
+    x = np.zeros(len(idx))
+    info = L.CG.solveHomogen(1e-10, x, b, op, stop_rule=L.STOP_REL, maxit=3000)
+    assert info["converged"] == 1
+    assert abs(info["iterations"] - itr) <= 1, (info["iterations"], itr)
+    assert np.linalg.norm(x - xr) <= 1e-9 * np.linalg.norm(xr)
+    assert np.all(x[act == 0] == 0.0)
+    op.set_precond(L.PRECOND_DIAG)
+    xd = np.zeros(len(idx))
+    infod = L.CG.solveHomogen(1e-10, xd, b, op, stop_rule=L.STOP_REL, maxit=3000)
+    print("iterations diag %d, depth-block %d" % (infod["iterations"], info["iterations"]), flush=True)
+    assert info["iterations"] < 0.7 * infod["iterations"]
+    assert np.linalg.norm(x - xd) <= 1e-8 * np.linalg.norm(xd)
+
+
+def test_mixed_solve_with_block_preconditioner_on_an_adaptive_grid():
     import torch
-    grid = L.AdaptiveSparseGrid.regular(3, 6)
+    idx, act = _adaptive_grid("cfg3_start4_level8")
+    grid = L.AdaptiveSparseGrid(idx, act)
+    B = 4
+    ops = {p: L.MatrixVectorHomogen(grid, L.VariableCoefficient(L.COEF_CFG3), precision=p, batch=B)
+           for p in (64, 32)}
+    # device tensors hold the device order: hanging points at grid.device_order()[slot]
+    hang = np.zeros(len(idx), bool)
+    hang[grid.device_order()] = act == 0
+    f = torch.tensor(np.random.default_rng(9).standard_normal((B, len(idx))), device="cuda")
+    f[:, torch.tensor(hang, device="cuda")] = 0.0
+    res = {}
+    for kind in (L.PRECOND_DIAG, L.PRECOND_DEPTH_BLOCK):
+        ops[32].set_precond(kind)
+        x = torch.zeros_like(f)
+        info = L.CG.solveMixed(1e-10, x, f, ops[64], ops[32])
+        assert all(i["converged"] == 1 and i["rel_residual"] <= 1e-10 for i in info)
+        res[kind] = (x, np.mean([i["iterations"] for i in info]))
+    print("inner iterations diag %.1f, depth-block %.1f" % (res[0][1], res[1][1]), flush=True)
+    assert res[L.PRECOND_DEPTH_BLOCK][1] < 0.6 * res[L.PRECOND_DIAG][1]
+    xd, xb = res[0][0], res[1][0]
+    assert float(torch.max(torch.linalg.norm(xd - xb, dim=1) / torch.linalg.norm(xd, dim=1))) < 1e-8
+
+
+@pytest.mark.parametrize("kind", ["regular", "adaptive"])
+def test_block_batch_matches_single_and_graph_replay(kind):
+    import torch
+    if kind == "regular":
+        grid = L.AdaptiveSparseGrid.regular(3, 6)
+        act = np.ones(grid.size, np.uint8)
+    else:
+        idx, act = _adaptive_grid("golden")
+        grid = L.AdaptiveSparseGrid(idx, act)
     B = 3
     op = L.MatrixVectorHomogen(grid, expsin(0), batch=B).set_precond(L.PRECOND_DEPTH_BLOCK)
     for b in range(B):
         op.set_coefficient(b, expsin(10 + b))
     F = np.random.default_rng(5).standard_normal((B, grid.size))
+    F[:, act == 0] = 0.0
     X = np.zeros((B, grid.size))
     infos = L.CG.solveHomogen(1e-10, X, F, op, stop_rule=L.STOP_REL, maxit=2000)
     for b in range(B):
@@ -114,11 +186,3 @@ def test_block_batch_matches_single_and_graph_replay():
     X3 = np.zeros((B, grid.size))
     L.CG.solveHomogen(1e-10, X3, F, op, stop_rule=L.STOP_REL, maxit=2000)
     assert np.array_equal(X, X3)
-
-
-def test_block_preconditioner_rejected_on_adaptive_grids():
-    from test_gpu_parity import load
-    gv = load("d3_adaptive_cfg3")
-    op = L.MatrixVectorHomogen(L.AdaptiveSparseGrid(gv["idx"], gv["active"]), L.Poisson())
-    with pytest.raises(L.LoadsgError):
-        op.set_precond(L.PRECOND_DEPTH_BLOCK)
