@@ -454,6 +454,7 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st, int nSub) {
     a.cellMat = op->matKind ? op->cellMat.as<V>() : nullptr;
     a.matStride = op->nCells * op->nEnt;
     a.nCells = op->nCells;
+    a.group = envInt("LSG_SP_GROUP", 4);
     for (auto& L : op->spLevels) {
         a.fam = L.second->as<FamilyItem>();
         launch_sp_apply<V>(a, L.first, s);

@@ -99,6 +99,8 @@ struct SparseApplyArgs {
     // instance i at cellMat[i * matStride + e * nCells + c], e = cellEntry(p, q)
     const V* cellMat;
     int64_t matStride, nCells;
+    // instances per workgroup of the cell-wise kernels (1, 2 or 4; batches above 1)
+    int group;
 };
 
 // Position of the pair of cell corners p <= q (bit i = upper end in dimension i) among the

@@ -364,6 +364,164 @@ __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
     }
 }
 
+// The cell-wise apply (D = d <= SP_CELL_MAXD, element matrices from the cell coefficients)
+// for NI instances per workgroup: every index-table entry (cell corners, row lists,
+// restriction triples, conversion columns and weights) is loaded once and serves the NI
+// instances, whose gathers are independent and in flight together.  Same passes, same
+// order of operations per instance as sp_apply_kernel<V, D, false>, so the results are the
+// same bits.  Slots past the batch (the last group) repeat the group's first instance and
+// store nothing.
+template <typename V, int D, int NI>
+__global__ void __launch_bounds__(256) sp_apply_grp_kernel(SparseApplyArgs<V> a) {
+    const int ng = (a.batch + NI - 1) / NI;
+    const int grp = (int)(blockIdx.x % ng);
+    const ApplyItem& it = a.items[a.fam[blockIdx.x / ng].first];
+    const SparseDev& s = a.sp;
+    int inst[NI];
+    bool on[NI];
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < NI; ++k) {
+        const int j = grp * NI + k;
+        on[k] = j < a.batch;
+        inst[k] = on[k] ? (a.map ? a.map[j] : j) : (a.map ? a.map[grp * NI] : grp * NI);
+        if (on[k] && a.st && !a.st[inst[k]].active) on[k] = false;
+        any = any || on[k];
+    }
+    if (!any) return;
+    const int t = it.tidx;
+    const int64_t size = s.boxSize[t];
+    constexpr int NC = 1 << D;
+    const V* u[NI];
+    V* A[NI];
+    V* B[NI];
+    V* Y[NI];
+    const V* cc[NI];
+#pragma unroll
+    for (int k = 0; k < NI; ++k) {
+        u[k] = (it.in_nodal ? a.nodal + inst[k] * a.nStride : a.cbuf + inst[k] * a.cStride) + it.in_off;
+        A[k] = a.zbuf + inst[k] * a.zStride + it.z_off;
+        B[k] = A[k] + size;
+        Y[k] = A[k] + 2 * size;
+        cc[k] = a.cellCoef + inst[k] * a.ccStride;
+    }
+    const bool mass = a.nC > 0;
+    V eS[NC], eM[NC];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+        eS[m] = element<V>(D, it.T, 0u, (unsigned)m, V(1), V(0));
+        eM[m] = element<V>(D, it.T, 0u, (unsigned)m, V(0), V(1));
+    }
+    const int64_t c0 = s.cellOff[t];
+    const int64_t ncell = s.cellOff[t + 1] - c0;
+    for (int64_t c = threadIdx.x; c < ncell; c += blockDim.x) {
+        const int32_t* corner = s.cellCorner + (c0 + c) * NC;
+        int32_t jq[NC];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) jq[q] = corner[q];
+        V uq[NI][NC];
+        V aC[NI], cC[NI];
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+#pragma unroll
+            for (int q = 0; q < NC; ++q) uq[k][q] = jq[q] >= 0 ? u[k][jq[q]] : V(0);
+            aC[k] = cc[k][2 * (c0 + c)];
+            cC[k] = mass ? cc[k][2 * (c0 + c) + 1] : V(0);
+        }
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            V y[NC];
+#pragma unroll
+            for (int p = 0; p < NC; ++p) {
+                V acc = V(0);
+#pragma unroll
+                for (int q = 0; q < NC; ++q) acc = fma(eS[p ^ q], uq[k][q], acc);
+                y[p] = aC[k] * acc;
+            }
+            if (mass) {
+#pragma unroll
+                for (int p = 0; p < NC; ++p) {
+                    V acc = V(0);
+#pragma unroll
+                    for (int q = 0; q < NC; ++q) acc = fma(eM[p ^ q], uq[k][q], acc);
+                    y[p] = fma(cC[k], acc, y[p]);
+                }
+            }
+            if (on[k]) {
+#pragma unroll
+                for (int p = 0; p < NC; ++p) Y[k][c * NC + p] = y[p];
+            }
+        }
+    }
+    pass_barrier();
+    // z_i = sum of the row's entries of the cells around point i (+ gM)
+    {
+        const int64_t r0 = s.boxOff[t];
+        const bool rd = it.rd_off[0] >= 0;
+        for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
+            V z[NI];
+#pragma unroll
+            for (int k = 0; k < NI; ++k) z[k] = V(0);
+            const int64_t k1 = s.rcPtr[r0 + i + 1];
+            for (int64_t e = s.rcPtr[r0 + i]; e < k1; ++e) {
+                const int32_t y = s.rcY[e];
+#pragma unroll
+                for (int k = 0; k < NI; ++k) z[k] += Y[k][y];
+            }
+#pragma unroll
+            for (int k = 0; k < NI; ++k) {
+                const V* g = a.wbuf + inst[k] * a.wStride + it.rd_off[0];
+                if (on[k]) A[k][i] = rd ? z[k] + g[i] : z[k];
+            }
+        }
+    }
+    pass_barrier();
+    // restrictions along the restriction dims (ascending) with the gM exchange
+    for (int j = 1; j <= it.k; ++j) {
+        const int dd = it.r[j - 1];
+        const int tf = it.ctidx[j - 1], tc = it.ctidx[j];
+        const int64_t nco = s.boxSize[tc];
+        const int32_t* slr = s.restSLR + 3 * s.restOff[(int64_t)tf * s.d + dd];
+        const bool w = it.w_off[j] >= 0, rd = it.rd_off[j] >= 0;
+        for (int64_t i = threadIdx.x; i < nco; i += blockDim.x) {
+            const int32_t sf = slr[3 * i], l = slr[3 * i + 1], r = slr[3 * i + 2];
+#pragma unroll
+            for (int k = 0; k < NI; ++k) {
+                const V v = A[k][sf] + V(0.5) * ((l >= 0 ? A[k][l] : V(0)) + (r >= 0 ? A[k][r] : V(0)));
+                V* wb = a.wbuf + inst[k] * a.wStride;
+                if (on[k]) {
+                    if (w) wb[it.w_off[j] + i] = v;
+                    B[k][i] = rd ? v + wb[it.rd_off[j] + i] : v;
+                }
+            }
+        }
+        pass_barrier();
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            V* tmp = A[k]; A[k] = B[k]; B[k] = tmp;
+        }
+    }
+    // prewavelet conversion of the points of the target depth
+    {
+        const int tc = it.ctidx[it.k];
+        const int64_t b0 = s.blkOff[tc], nb = s.blkSize[tc];
+        for (int64_t i = threadIdx.x; i < nb; i += blockDim.x) {
+            V v[NI];
+#pragma unroll
+            for (int k = 0; k < NI; ++k) v[k] = V(0);
+            for (int64_t e = s.convPtr[b0 + i]; e < s.convPtr[b0 + i + 1]; ++e) {
+                const V wgt = V(s.convVal[e]);
+                const int32_t col = s.convCol[e];
+#pragma unroll
+                for (int k = 0; k < NI; ++k) v[k] += wgt * A[k][col];
+            }
+#pragma unroll
+            for (int k = 0; k < NI; ++k)
+                if (on[k]) (a.part + inst[k] * a.pStride + it.out_off)[i] = v[k];
+        }
+    }
+}
+
 // ------------------------------------------------------------------- diagonal
 // z_p = psi_p^T A_T psi_p (Preconditioning::precond3, applications/cg_method.h:322-451,
 // with applyStencilLocal over the cells of depth T+1 in the grid): for every stored
@@ -459,6 +617,23 @@ void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str) {
             case 4: hipLaunchKernelGGL((sp_apply_kernel<V, 4, true>), grid, dim3(256), 0, str, a); break;
             default: hipLaunchKernelGGL((sp_apply_kernel<V, 0, true>), grid, dim3(256), 0, str, a); break;
         }
+        return;
+    }
+    // the cell-wise kernels for groups of instances (one index load per group)
+    const int ni = a.group;
+    if (a.sp.d <= SP_CELL_MAXD && a.batch > 1 && ni > 1) {
+        const int g = ni >= 4 ? 4 : 2;
+        const dim3 gg((unsigned)nitems * (unsigned)((a.batch + g - 1) / g));
+#define LSG_SP_GRP(DD)                                                                                          \
+    if (g == 4) hipLaunchKernelGGL((sp_apply_grp_kernel<V, DD, 4>), gg, dim3(256), 0, str, a);                 \
+    else hipLaunchKernelGGL((sp_apply_grp_kernel<V, DD, 2>), gg, dim3(256), 0, str, a);
+        switch (a.sp.d) {
+            case 1: LSG_SP_GRP(1) break;
+            case 2: LSG_SP_GRP(2) break;
+            case 3: LSG_SP_GRP(3) break;
+            default: LSG_SP_GRP(4) break;
+        }
+#undef LSG_SP_GRP
         return;
     }
     switch (a.sp.d) {

@@ -154,3 +154,53 @@ def test_refined_cfg3_grid_vs_oracle():
     xr = torch.zeros_like(f)
     L.CG.solveHomogen(1e-13, xr, f, o64, stop_rule=L.STOP_REL, maxit=5000)
     assert float(torch.max(torch.linalg.norm(xm - xr, dim=1) / torch.linalg.norm(xr, dim=1))) < 1e-8
+
+
+@pytest.mark.parametrize("case,kind", [("d2_adaptive_cfg2", "mass"), ("d3_adaptive_cfg3", "expsin")])
+def test_sparse_instance_groups_match_single(case, kind):
+    """Batches above 1 take the cell-wise apply over groups of instances (one index-table
+    load per group; LSG_SP_GROUP = 4, 2 or 1): batch 9 (two full groups and one of one) in
+    FP64 and FP32 must equal 9 single-instance operators bit for bit, in the matvec and in the
+    mixed-precision solve, whose instances converge after different counts (the grids are then
+    compacted to the instances still iterating, so groups mix instances)."""
+    import torch
+    print("groups", case, kind, flush=True)
+    gv = load(case)
+    grid = L.AdaptiveSparseGrid(gv["idx"], gv["active"])
+    B = 9
+    rng = np.random.default_rng(31)
+    xis = rng.normal(0.0, 0.5, (B, 8))
+
+    def coef(b):
+        if kind == "mass":      # constant a and c per instance: stiffness and mass terms
+            return L.Stencil(L.COEF_CONST, a0=0.7 + 0.1 * b, c0=0.3 + 0.2 * b)
+        return L.VariableCoefficient(L.COEF_EXPSIN, xi=xis[b])
+
+    dev = grid.device_order()
+    x = torch.tensor(rng.standard_normal((B, grid.size)), device="cuda")
+    f = torch.zeros(B, grid.size, dtype=torch.float64, device="cuda")
+    ops = {}
+    for prec, dt in ((64, torch.float64), (32, torch.float32)):
+        op = L.MatrixVectorHomogen(grid, coef(0), precision=prec, batch=B)
+        for b in range(B):
+            op.set_coefficient(b, coef(b))
+        ops[prec] = op
+        xb = x.to(dt)
+        y = torch.zeros_like(xb)
+        op.multiplication(xb, y)
+        for b in range(B):
+            s = L.MatrixVectorHomogen(grid, coef(b), precision=prec)
+            ys = torch.zeros_like(xb[b:b + 1])
+            s.multiplication(xb[b:b + 1].contiguous(), ys)
+            assert torch.equal(ys[0], y[b]), (prec, b)
+    ops[64].rhs_sines(rng.standard_normal((B, 2)) * np.logspace(-2, 2, B)[:, None],
+                      np.array([1, 2], dtype=np.int32), out=f)
+    xm = torch.zeros_like(f)
+    info = L.CG.solveMixed(1e-9, xm, f, ops[64], ops[32])
+    for b in range(B):
+        s64 = L.MatrixVectorHomogen(grid, coef(b), precision=64)
+        s32 = L.MatrixVectorHomogen(grid, coef(b), precision=32)
+        xs = torch.zeros_like(f[b:b + 1])
+        i1 = L.CG.solveMixed(1e-9, xs, f[b:b + 1].contiguous(), s64, s32)
+        assert i1["iterations"] == info[b]["iterations"], b
+        assert torch.equal(xs[0], xm[b]), b
