@@ -352,7 +352,15 @@ struct lsg_op {
     bool sparse = false;
     std::vector<std::unique_ptr<DevMem>> spTables;
     SparseDev sd{};
-    std::vector<std::pair<int, std::unique_ptr<DevMem>>> spLevels;   // (count, families)
+    // sparse apply launches: per wavefront level and scratch class (SpClass) the families,
+    // their count and the LDS values per instance
+    struct SpLaunch {
+        int count;
+        int cls;
+        int64_t ldsStride;
+        std::unique_ptr<DevMem> fam;
+    };
+    std::vector<SpLaunch> spLevels;
     DevMem zbuf, activeDev, cellCoef;
     DevMem cellMat;            // stored element matrices: [instance][entry][cell]
     int64_t nEnt = 0;          // entries per cell
@@ -454,10 +462,10 @@ int runMatvecSparse(lsg_op* op, const V* x, V* y, const CGState* st, int nSub) {
     a.cellMat = op->matKind ? op->cellMat.as<V>() : nullptr;
     a.matStride = op->nCells * op->nEnt;
     a.nCells = op->nCells;
-    a.group = envInt("LSG_SP_GROUP", 4);
     for (auto& L : op->spLevels) {
-        a.fam = L.second->as<FamilyItem>();
-        launch_sp_apply<V>(a, L.first, s);
+        a.fam = L.fam->as<FamilyItem>();
+        a.ldsStride = L.ldsStride;
+        launch_sp_apply<V>(a, L.count, s, L.cls);
     }
     if (!op->capturing) HIPCHK(hipEventRecord(op->ev[2], s));
     launch_sum<V>(op->dSumItems.as<SumItem>(), (int)P.sum.size(), op->dSumList.as<int64_t>(), d, op->part.as<V>(),
@@ -1779,6 +1787,7 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         LSG_UP(nodPtr, T.nodPtr) LSG_UP(nodCol, T.nodCol) LSG_UP(nodVal, T.nodVal)
         LSG_UP(prolOff, T.prolOff) LSG_UP(prolAB, T.prolAB)
         LSG_UP(cellOff, T.cellOff) LSG_UP(cellCorner, T.cellCorner) LSG_UP(cellLo, T.cellLo)
+        LSG_UP(cellColOff, T.cellColOff) LSG_UP(cellCol, T.cellCol)
         LSG_UP(rcPtr, T.rcPtr) LSG_UP(rcCell, T.rcCell) LSG_UP(rcCorner, T.rcCorner) LSG_UP(rcY, T.rcY)
         LSG_UP(restOff, T.restOff) LSG_UP(restSLR, T.restSLR)
         LSG_UP(convPtr, T.convPtr) LSG_UP(convCol, T.convCol) LSG_UP(convVal, T.convVal)
@@ -1787,11 +1796,30 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
 #undef LSG_UP
         op->sd.d = g.d;
         op->sd.nd = (int)g.depths.size();
+        // the cell-wise apply keeps an item's point values in LDS: the box and, after the
+        // first restriction, the coarser box; items are launched by the size of that scratch
+        // (SP_SMALL up to 8 KB per instance, four instances per workgroup; SP_MEDIUM up to
+        // 32 KB; SP_LARGE up to the device's LDS; larger ones in zbuf)
+        const int64_t ldsMax = std::min<int64_t>(op->maxLds, envInt("LSG_SP_LDS_BYTES", (int)op->maxLds));
         for (const auto& lev : P.apply) {
-            std::unique_ptr<DevMem> m(new DevMem());
-            HIPCHK(m->alloc(lev.size() * sizeof(FamilyItem)));
-            HIPCHK(hipMemcpy(m->p, lev.data(), lev.size() * sizeof(FamilyItem), hipMemcpyHostToDevice));
-            op->spLevels.emplace_back((int)lev.size(), std::move(m));
+            std::vector<FamilyItem> byCls[SP_NCLASS];
+            int64_t stride[SP_NCLASS] = {};
+            for (const FamilyItem& f : lev) {
+                const ApplyItem& it = P.applyItems[(size_t)f.first];
+                const int64_t need = g.boxSize[it.tidx] + (it.k >= 1 ? g.boxSize[it.ctidx[1]] : 0);
+                const int64_t bytes = need * (int64_t)vs;
+                const int c = bytes <= 8192 ? SP_SMALL : bytes <= 32768 ? SP_MEDIUM : bytes <= ldsMax ? SP_LARGE
+                                                                                                      : SP_GLOBAL;
+                byCls[c].push_back(f);
+                stride[c] = std::max(stride[c], need);
+            }
+            for (int c = 0; c < SP_NCLASS; ++c) {
+                if (byCls[c].empty()) continue;
+                std::unique_ptr<DevMem> m(new DevMem());
+                HIPCHK(m->alloc(byCls[c].size() * sizeof(FamilyItem)));
+                HIPCHK(hipMemcpy(m->p, byCls[c].data(), byCls[c].size() * sizeof(FamilyItem), hipMemcpyHostToDevice));
+                op->spLevels.push_back({(int)byCls[c].size(), c, c == SP_GLOBAL ? 0 : stride[c], std::move(m)});
+            }
         }
         HIPCHK(op->zbuf.alloc((size_t)P.zSize * batch * vs));
         op->nCells = T.cellOff.back();

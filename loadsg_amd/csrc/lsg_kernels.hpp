@@ -54,6 +54,8 @@ struct SparseDev {
     const int64_t* cellOff;
     const int32_t* cellCorner;
     const int32_t* cellLo;
+    const int64_t* cellColOff;
+    const int32_t* cellCol;
     const int64_t* rcPtr;
     const int32_t* rcCell;
     const int32_t* rcCorner;
@@ -99,8 +101,9 @@ struct SparseApplyArgs {
     // instance i at cellMat[i * matStride + e * nCells + c], e = cellEntry(p, q)
     const V* cellMat;
     int64_t matStride, nCells;
-    // instances per workgroup of the cell-wise kernels (1, 2 or 4; batches above 1)
-    int group;
+    // cell-wise kernels (element matrices from the cell coefficients, d <= SP_CELL_MAXD):
+    // values per instance of the LDS scratch of this launch (0: scratch in zbuf)
+    int64_t ldsStride;
 };
 
 // Position of the pair of cell corners p <= q (bit i = upper end in dimension i) among the
@@ -120,7 +123,10 @@ template <typename V>
 void launch_sp_prol(const ProlItem* items, int nitems, const SparseDev& s, const V* nodal, int64_t nStride, V* cbuf,
                     int64_t cStride, int batch, const CGState* st, hipStream_t str, const int* map = nullptr);
 template <typename V>
-void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str);
+void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str, int cls);
+// scratch classes of the cell-wise sparse apply (one launch per class and wavefront level)
+enum SpClass : int { SP_SMALL = 0, SP_MEDIUM = 1, SP_LARGE = 2, SP_GLOBAL = 3 };
+constexpr int SP_NCLASS = 4;
 template <typename V>
 void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, const V* cellCoef, int64_t ccStride,
                     const V* cellMat, int64_t matStride, int64_t nCells, V* z, int64_t zStride, int batch,

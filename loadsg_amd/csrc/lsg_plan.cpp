@@ -282,6 +282,24 @@ int buildSparse(Grid& g, const std::vector<int>& tOf, std::string& err) {
                 err = "too many cells in one depth";
                 return LSG_ERR_CAPACITY;
             }
+        // cells of each depth grouped by the parity of their position (2^d colours): two
+        // cells of one colour share no corner
+        S.cellColOff.assign((size_t)nd * (nc + 1), 0);
+        S.cellCol.reserve((size_t)S.cellOff[nd]);
+        for (int t = 0; t < nd; ++t) {
+            const int64_t c0 = S.cellOff[t], n0 = S.cellOff[t + 1] - c0;
+            std::vector<std::vector<int32_t>> byCol(nc);
+            for (int64_t c = 0; c < n0; ++c) {
+                unsigned col = 0;
+                for (int i = 0; i < d; ++i) col |= (unsigned)(S.cellLo[(size_t)(c0 + c) * d + i] & 1) << i;
+                byCol[col].push_back((int32_t)c);
+            }
+            for (unsigned col = 0; col < nc; ++col) {
+                S.cellColOff[(size_t)t * (nc + 1) + col] = (int64_t)S.cellCol.size();
+                S.cellCol.insert(S.cellCol.end(), byCol[col].begin(), byCol[col].end());
+            }
+            S.cellColOff[(size_t)t * (nc + 1) + nc] = (int64_t)S.cellCol.size();
+        }
         for (int64_t r = 0; r < g.nml; ++r) S.rcPtr[r + 1] = S.rcPtr[r] + (int64_t)rc[r].size();
         for (auto& row : rc)
             for (auto& e : row) {

@@ -135,6 +135,10 @@ struct SparseTables {
     std::vector<int64_t> cellOff;          // [t], first cell of depth t
     std::vector<int32_t> cellCorner;
     std::vector<int32_t> cellLo;
+    // cells of each depth by colour (parity of the lower position, bit i = dimension i):
+    // cellCol[cellColOff[t * (2^d + 1) + col] ..] are the local cell indices of colour col
+    std::vector<int64_t> cellColOff;
+    std::vector<int32_t> cellCol;
     // cells around each multilevel row: CSR (cell id, corner id of the row's point)
     std::vector<int64_t> rcPtr;
     std::vector<int32_t> rcCell;

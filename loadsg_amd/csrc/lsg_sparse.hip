@@ -153,14 +153,14 @@ __global__ void __launch_bounds__(256) sp_prol_kernel(const ProlItem* __restrict
 // ---------------------------------------------------------------------- apply
 // One workgroup per (case, depth) item and instance: stencil on G_T (+ gM), the
 // restrictions with their gM exchanges, and the conversion into the part buffer.
-// D > 0 (d = D <= SP_CELL_MAXD): the stencil runs cell by cell -- one thread loads
-// the 2^D corner values of a cell once, forms the local product with the element
-// matrix held in registers and stores it; each point then sums its entries of the
-// cells around it.  D = 0: any d, each point gathers the corners of its cells.
 // STORED: the element matrices are the caller's, one symmetric matrix per cell read from
 // HBM (the stored local stiffness matrices of LocalStiffnessMatrices::applyLocalMatrix,
-// localStiffnessMatrices/LocalStiffnessMatrices.cpp:506-531); otherwise a(centre) times
-// the Q1 stiffness plus c(centre) times the Q1 mass matrix.
+// localStiffnessMatrices/LocalStiffnessMatrices.cpp:506-531); D > 0 (d = D <=
+// SP_CELL_MAXD): one thread loads the 2^D corner values of a cell once, forms the local
+// product and stores it, then each point sums its entries of the cells around it; D = 0:
+// each point gathers the corners of its cells.  Not STORED (D = 0 only; smaller d takes
+// sp_apply_col_kernel): a(centre) times the Q1 stiffness plus c(centre) times the Q1 mass
+// matrix, each point gathering the corners of its cells.
 template <typename V, int D, bool STORED>
 __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
     const int inst = a.map ? a.map[blockIdx.x % a.batch] : (int)(blockIdx.x % a.batch);
@@ -239,60 +239,6 @@ __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
             }
             pass_barrier();
         }
-    } else if constexpr (D > 0) {
-        // element matrix of a level-T cell, stiffness and mass parts: the entry between
-        // corners p and q depends only on the mask p ^ q
-        constexpr int NC = 1 << D;
-        V eS[NC], eM[NC];
-#pragma unroll
-        for (int m = 0; m < NC; ++m) {
-            eS[m] = element<V>(D, it.T, 0u, (unsigned)m, V(1), V(0));
-            eM[m] = element<V>(D, it.T, 0u, (unsigned)m, V(0), V(1));
-        }
-        const int64_t c0 = s.cellOff[t];
-        const int64_t ncell = s.cellOff[t + 1] - c0;
-        V* Y = A + 2 * size;
-        for (int64_t c = threadIdx.x; c < ncell; c += blockDim.x) {
-            const int32_t* corner = s.cellCorner + (c0 + c) * NC;
-            V uq[NC];
-#pragma unroll
-            for (int q = 0; q < NC; ++q) {
-                const int32_t j = corner[q];
-                uq[q] = j >= 0 ? u[j] : V(0);
-            }
-            const V aC = cc[2 * (c0 + c)];
-            V y[NC];
-#pragma unroll
-            for (int p = 0; p < NC; ++p) {
-                V acc = V(0);
-#pragma unroll
-                for (int q = 0; q < NC; ++q) acc = fma(eS[p ^ q], uq[q], acc);
-                y[p] = aC * acc;
-            }
-            if (mass) {
-                const V cC = cc[2 * (c0 + c) + 1];
-#pragma unroll
-                for (int p = 0; p < NC; ++p) {
-                    V acc = V(0);
-#pragma unroll
-                    for (int q = 0; q < NC; ++q) acc = fma(eM[p ^ q], uq[q], acc);
-                    y[p] = fma(cC, acc, y[p]);
-                }
-            }
-#pragma unroll
-            for (int p = 0; p < NC; ++p) Y[c * NC + p] = y[p];
-        }
-        pass_barrier();
-        // z_i = sum of the row's entries of the cells around point i (+ gM)
-        const int64_t r0 = s.boxOff[t];
-        const V* g = it.rd_off[0] >= 0 ? wb + it.rd_off[0] : nullptr;
-        for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
-            V z = V(0);
-            const int64_t k1 = s.rcPtr[r0 + i + 1];
-            for (int64_t k = s.rcPtr[r0 + i]; k < k1; ++k) z += Y[s.rcY[k]];
-            A[i] = g ? z + g[i] : z;
-        }
-        pass_barrier();
     } else {
     // element matrix of a level-T cell, split into its stiffness and mass parts: the
     // entry between corners p and q depends only on the mask p ^ q
@@ -365,14 +311,18 @@ __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
 }
 
 // The cell-wise apply (D = d <= SP_CELL_MAXD, element matrices from the cell coefficients)
-// for NI instances per workgroup: every index-table entry (cell corners, row lists,
-// restriction triples, conversion columns and weights) is loaded once and serves the NI
-// instances, whose gathers are independent and in flight together.  Same passes, same
-// order of operations per instance as sp_apply_kernel<V, D, false>, so the results are the
-// same bits.  Slots past the batch (the last group) repeat the group's first instance and
-// store nothing.
-template <typename V, int D, int NI>
-__global__ void __launch_bounds__(256) sp_apply_grp_kernel(SparseApplyArgs<V> a) {
+// with the stencil accumulated point by point: the cells of depth T+1 are taken colour by
+// colour (parity of their position; cells of one colour share no corner), each thread forms
+// a cell's local product and adds it to the cell's corners, so every point receives its
+// cells' entries in a fixed order (gM value first, then colour 0, 1, ...) without an
+// intermediate per-cell array.  LDSM: the point values (the box and, after the first
+// restriction, the coarser box) live in LDS, ldsStride values per instance; otherwise in
+// zbuf.  NI instances per workgroup share every index-table load (cell corners, restriction
+// triples, conversion columns and weights); slots past the batch (the last group) repeat the
+// group's first instance and store nothing.
+template <typename V, int D, int NI, bool LDSM, int NT>
+__global__ void __launch_bounds__(NT) sp_apply_col_kernel(SparseApplyArgs<V> a) {
+    extern __shared__ char smem_raw[];
     const int ng = (a.batch + NI - 1) / NI;
     const int grp = (int)(blockIdx.x % ng);
     const ApplyItem& it = a.items[a.fam[blockIdx.x / ng].first];
@@ -395,87 +345,73 @@ __global__ void __launch_bounds__(256) sp_apply_grp_kernel(SparseApplyArgs<V> a)
     const V* u[NI];
     V* A[NI];
     V* B[NI];
-    V* Y[NI];
     const V* cc[NI];
+    V* wb[NI];
 #pragma unroll
     for (int k = 0; k < NI; ++k) {
         u[k] = (it.in_nodal ? a.nodal + inst[k] * a.nStride : a.cbuf + inst[k] * a.cStride) + it.in_off;
-        A[k] = a.zbuf + inst[k] * a.zStride + it.z_off;
+        if constexpr (LDSM) A[k] = reinterpret_cast<V*>(smem_raw) + k * a.ldsStride;
+        else A[k] = a.zbuf + inst[k] * a.zStride + it.z_off;
         B[k] = A[k] + size;
-        Y[k] = A[k] + 2 * size;
         cc[k] = a.cellCoef + inst[k] * a.ccStride;
+        wb[k] = a.wbuf + inst[k] * a.wStride;
     }
     const bool mass = a.nC > 0;
+    // the gM value (or 0) starts every point's sum
+    {
+        const bool rd = it.rd_off[0] >= 0;
+        for (int64_t i = threadIdx.x; i < size; i += NT)
+#pragma unroll
+            for (int k = 0; k < NI; ++k)
+                if (on[k]) A[k][i] = rd ? wb[k][it.rd_off[0] + i] : V(0);
+    }
     V eS[NC], eM[NC];
 #pragma unroll
     for (int m = 0; m < NC; ++m) {
         eS[m] = element<V>(D, it.T, 0u, (unsigned)m, V(1), V(0));
         eM[m] = element<V>(D, it.T, 0u, (unsigned)m, V(0), V(1));
     }
+    __syncthreads();
     const int64_t c0 = s.cellOff[t];
-    const int64_t ncell = s.cellOff[t + 1] - c0;
-    for (int64_t c = threadIdx.x; c < ncell; c += blockDim.x) {
-        const int32_t* corner = s.cellCorner + (c0 + c) * NC;
-        int32_t jq[NC];
+    const int64_t* colOff = s.cellColOff + (int64_t)t * (NC + 1);
+    for (int col = 0; col < NC; ++col) {
+        const int64_t e1 = colOff[col + 1];
+        for (int64_t e = colOff[col] + threadIdx.x; e < e1; e += NT) {
+            const int64_t c = c0 + s.cellCol[e];
+            const int32_t* corner = s.cellCorner + c * NC;
+            int32_t jq[NC];
 #pragma unroll
-        for (int q = 0; q < NC; ++q) jq[q] = corner[q];
-        V uq[NI][NC];
-        V aC[NI], cC[NI];
+            for (int q = 0; q < NC; ++q) jq[q] = corner[q];
+            V uq[NI][NC];
+            V aC[NI], cC[NI];
 #pragma unroll
-        for (int k = 0; k < NI; ++k) {
+            for (int k = 0; k < NI; ++k) {
 #pragma unroll
-            for (int q = 0; q < NC; ++q) uq[k][q] = jq[q] >= 0 ? u[k][jq[q]] : V(0);
-            aC[k] = cc[k][2 * (c0 + c)];
-            cC[k] = mass ? cc[k][2 * (c0 + c) + 1] : V(0);
-        }
-#pragma unroll
-        for (int k = 0; k < NI; ++k) {
-            V y[NC];
-#pragma unroll
-            for (int p = 0; p < NC; ++p) {
-                V acc = V(0);
-#pragma unroll
-                for (int q = 0; q < NC; ++q) acc = fma(eS[p ^ q], uq[k][q], acc);
-                y[p] = aC[k] * acc;
+                for (int q = 0; q < NC; ++q) uq[k][q] = jq[q] >= 0 ? u[k][jq[q]] : V(0);
+                aC[k] = cc[k][2 * c];
+                cC[k] = mass ? cc[k][2 * c + 1] : V(0);
             }
-            if (mass) {
+#pragma unroll
+            for (int k = 0; k < NI; ++k) {
+                if (!on[k]) continue;
 #pragma unroll
                 for (int p = 0; p < NC; ++p) {
                     V acc = V(0);
 #pragma unroll
-                    for (int q = 0; q < NC; ++q) acc = fma(eM[p ^ q], uq[k][q], acc);
-                    y[p] = fma(cC[k], acc, y[p]);
+                    for (int q = 0; q < NC; ++q) acc = fma(eS[p ^ q], uq[k][q], acc);
+                    V y = aC[k] * acc;
+                    if (mass) {
+                        V accm = V(0);
+#pragma unroll
+                        for (int q = 0; q < NC; ++q) accm = fma(eM[p ^ q], uq[k][q], accm);
+                        y = fma(cC[k], accm, y);
+                    }
+                    if (jq[p] >= 0) A[k][jq[p]] += y;
                 }
             }
-            if (on[k]) {
-#pragma unroll
-                for (int p = 0; p < NC; ++p) Y[k][c * NC + p] = y[p];
-            }
         }
+        __syncthreads();
     }
-    pass_barrier();
-    // z_i = sum of the row's entries of the cells around point i (+ gM)
-    {
-        const int64_t r0 = s.boxOff[t];
-        const bool rd = it.rd_off[0] >= 0;
-        for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
-            V z[NI];
-#pragma unroll
-            for (int k = 0; k < NI; ++k) z[k] = V(0);
-            const int64_t k1 = s.rcPtr[r0 + i + 1];
-            for (int64_t e = s.rcPtr[r0 + i]; e < k1; ++e) {
-                const int32_t y = s.rcY[e];
-#pragma unroll
-                for (int k = 0; k < NI; ++k) z[k] += Y[k][y];
-            }
-#pragma unroll
-            for (int k = 0; k < NI; ++k) {
-                const V* g = a.wbuf + inst[k] * a.wStride + it.rd_off[0];
-                if (on[k]) A[k][i] = rd ? z[k] + g[i] : z[k];
-            }
-        }
-    }
-    pass_barrier();
     // restrictions along the restriction dims (ascending) with the gM exchange
     for (int j = 1; j <= it.k; ++j) {
         const int dd = it.r[j - 1];
@@ -483,19 +419,17 @@ __global__ void __launch_bounds__(256) sp_apply_grp_kernel(SparseApplyArgs<V> a)
         const int64_t nco = s.boxSize[tc];
         const int32_t* slr = s.restSLR + 3 * s.restOff[(int64_t)tf * s.d + dd];
         const bool w = it.w_off[j] >= 0, rd = it.rd_off[j] >= 0;
-        for (int64_t i = threadIdx.x; i < nco; i += blockDim.x) {
+        for (int64_t i = threadIdx.x; i < nco; i += NT) {
             const int32_t sf = slr[3 * i], l = slr[3 * i + 1], r = slr[3 * i + 2];
 #pragma unroll
             for (int k = 0; k < NI; ++k) {
+                if (!on[k]) continue;
                 const V v = A[k][sf] + V(0.5) * ((l >= 0 ? A[k][l] : V(0)) + (r >= 0 ? A[k][r] : V(0)));
-                V* wb = a.wbuf + inst[k] * a.wStride;
-                if (on[k]) {
-                    if (w) wb[it.w_off[j] + i] = v;
-                    B[k][i] = rd ? v + wb[it.rd_off[j] + i] : v;
-                }
+                if (w) wb[k][it.w_off[j] + i] = v;
+                B[k][i] = rd ? v + wb[k][it.rd_off[j] + i] : v;
             }
         }
-        pass_barrier();
+        __syncthreads();
 #pragma unroll
         for (int k = 0; k < NI; ++k) {
             V* tmp = A[k]; A[k] = B[k]; B[k] = tmp;
@@ -505,11 +439,12 @@ __global__ void __launch_bounds__(256) sp_apply_grp_kernel(SparseApplyArgs<V> a)
     {
         const int tc = it.ctidx[it.k];
         const int64_t b0 = s.blkOff[tc], nb = s.blkSize[tc];
-        for (int64_t i = threadIdx.x; i < nb; i += blockDim.x) {
+        for (int64_t i = threadIdx.x; i < nb; i += NT) {
             V v[NI];
 #pragma unroll
             for (int k = 0; k < NI; ++k) v[k] = V(0);
-            for (int64_t e = s.convPtr[b0 + i]; e < s.convPtr[b0 + i + 1]; ++e) {
+            const int64_t e1 = s.convPtr[b0 + i + 1];
+            for (int64_t e = s.convPtr[b0 + i]; e < e1; ++e) {
                 const V wgt = V(s.convVal[e]);
                 const int32_t col = s.convCol[e];
 #pragma unroll
@@ -605,7 +540,7 @@ void launch_sp_prol(const ProlItem* items, int nitems, const SparseDev& s, const
                        nStride, cbuf, cStride, batch, st, map);
 }
 template <typename V>
-void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str) {
+void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str, int cls) {
     if (nitems == 0) return;
     const dim3 grid((unsigned)nitems * (unsigned)a.batch);
     static_assert(SP_CELL_MAXD == 4, "one instantiation per cell-wise dimension");
@@ -619,30 +554,40 @@ void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str) {
         }
         return;
     }
-    // the cell-wise kernels for groups of instances (one index load per group)
-    const int ni = a.group;
-    if (a.sp.d <= SP_CELL_MAXD && a.batch > 1 && ni > 1) {
-        const int g = ni >= 4 ? 4 : 2;
-        const dim3 gg((unsigned)nitems * (unsigned)((a.batch + g - 1) / g));
-#define LSG_SP_GRP(DD)                                                                                          \
-    if (g == 4) hipLaunchKernelGGL((sp_apply_grp_kernel<V, DD, 4>), gg, dim3(256), 0, str, a);                 \
-    else hipLaunchKernelGGL((sp_apply_grp_kernel<V, DD, 2>), gg, dim3(256), 0, str, a);
-        switch (a.sp.d) {
-            case 1: LSG_SP_GRP(1) break;
-            case 2: LSG_SP_GRP(2) break;
-            case 3: LSG_SP_GRP(3) break;
-            default: LSG_SP_GRP(4) break;
-        }
-#undef LSG_SP_GRP
+    if (a.sp.d > SP_CELL_MAXD) {
+        hipLaunchKernelGGL((sp_apply_kernel<V, 0, false>), grid, dim3(256), 0, str, a);
         return;
     }
-    switch (a.sp.d) {
-        case 1: hipLaunchKernelGGL((sp_apply_kernel<V, 1, false>), grid, dim3(256), 0, str, a); break;
-        case 2: hipLaunchKernelGGL((sp_apply_kernel<V, 2, false>), grid, dim3(256), 0, str, a); break;
-        case 3: hipLaunchKernelGGL((sp_apply_kernel<V, 3, false>), grid, dim3(256), 0, str, a); break;
-        case 4: hipLaunchKernelGGL((sp_apply_kernel<V, 4, false>), grid, dim3(256), 0, str, a); break;
-        default: hipLaunchKernelGGL((sp_apply_kernel<V, 0, false>), grid, dim3(256), 0, str, a); break;
+    // cell-wise kernels by scratch class: SP_SMALL (LDS, 4 instances per workgroup from
+    // batch 2 on), SP_MEDIUM (LDS, one instance, 256 threads), SP_LARGE (LDS, one instance,
+    // 512 threads), SP_GLOBAL (zbuf, 4 instances per workgroup)
+    const bool grp = a.batch > 1 && (cls == SP_SMALL || cls == SP_GLOBAL);
+    const dim3 gg((unsigned)nitems * (unsigned)(grp ? (a.batch + 3) / 4 : a.batch));
+    const size_t lds = (size_t)a.ldsStride * sizeof(V) * (grp ? 4 : 1);
+#define LSG_SP_COL(DD)                                                                                          \
+    switch (cls) {                                                                                              \
+        case SP_SMALL:                                                                                          \
+            if (grp) hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 4, true, 256>), gg, dim3(256), lds, str, a); \
+            else hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 256>), gg, dim3(256), lds, str, a);     \
+            break;                                                                                              \
+        case SP_MEDIUM:                                                                                         \
+            hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 256>), gg, dim3(256), lds, str, a);          \
+            break;                                                                                              \
+        case SP_LARGE:                                                                                          \
+            hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 512>), gg, dim3(512), lds, str, a);          \
+            break;                                                                                              \
+        default:                                                                                                \
+            if (grp) hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 4, false, 256>), gg, dim3(256), 0, str, a);  \
+            else hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, false, 256>), gg, dim3(256), 0, str, a);      \
+            break;                                                                                              \
     }
+    switch (a.sp.d) {
+        case 1: LSG_SP_COL(1) break;
+        case 2: LSG_SP_COL(2) break;
+        case 3: LSG_SP_COL(3) break;
+        default: LSG_SP_COL(4) break;
+    }
+#undef LSG_SP_COL
 }
 template <typename V>
 void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, const V* cellCoef, int64_t ccStride,
@@ -659,7 +604,7 @@ void launch_sp_diag(const SparseDev& s, const uint8_t* activeDev, int n, const V
                                      hipStream_t, const int*);                                                      \
     template void launch_sp_prol<V>(const ProlItem*, int, const SparseDev&, const V*, int64_t, V*, int64_t, int,    \
                                     const CGState*, hipStream_t, const int*);                                       \
-    template void launch_sp_apply<V>(const SparseApplyArgs<V>&, int, hipStream_t);                                  \
+    template void launch_sp_apply<V>(const SparseApplyArgs<V>&, int, hipStream_t, int);                                  \
     template void launch_sp_diag<V>(const SparseDev&, const uint8_t*, int, const V*, int64_t, const V*, int64_t,     \
                                     int64_t, V*, int64_t, int, hipStream_t);
 LSG_SP_INST(double)
