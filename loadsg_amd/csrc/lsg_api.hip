@@ -1801,6 +1801,7 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         // (SP_SMALL up to 8 KB per instance, four instances per workgroup; SP_MEDIUM up to
         // 32 KB; SP_LARGE up to the device's LDS; larger ones in zbuf)
         const int64_t ldsMax = std::min<int64_t>(op->maxLds, envInt("LSG_SP_LDS_BYTES", (int)op->maxLds));
+        const int64_t smallMax = envInt("LSG_SP_SMALL_BYTES", 8192), mediumMax = envInt("LSG_SP_MEDIUM_BYTES", 32768);
         for (const auto& lev : P.apply) {
             std::vector<FamilyItem> byCls[SP_NCLASS];
             int64_t stride[SP_NCLASS] = {};
@@ -1808,8 +1809,10 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
                 const ApplyItem& it = P.applyItems[(size_t)f.first];
                 const int64_t need = g.boxSize[it.tidx] + (it.k >= 1 ? g.boxSize[it.ctidx[1]] : 0);
                 const int64_t bytes = need * (int64_t)vs;
-                const int c = bytes <= 8192 ? SP_SMALL : bytes <= 32768 ? SP_MEDIUM : bytes <= ldsMax ? SP_LARGE
-                                                                                                      : SP_GLOBAL;
+                const int c = bytes <= std::min(smallMax, ldsMax / 4) ? SP_SMALL
+                              : bytes <= std::min(mediumMax, ldsMax)     ? SP_MEDIUM
+                              : bytes <= ldsMax                          ? SP_LARGE
+                                                                         : SP_GLOBAL;
                 byCls[c].push_back(f);
                 stride[c] = std::max(stride[c], need);
             }

@@ -96,54 +96,94 @@ __global__ void __launch_bounds__(256) sp_cellcoef_kernel(SparseDev s, int64_t n
     }
 }
 
+// Instance slots of a workgroup that takes NI instances: slot j = first + k maps to
+// map[j] (or j); slots past the batch, and instances that have stopped iterating, are off
+// (they repeat an instance of the group for their loads and store nothing).  Returns
+// whether any slot is on.
+template <int NI>
+__device__ inline bool group_slots(int first, int batch, const int* __restrict__ map,
+                                   const CGState* __restrict__ st, int (&inst)[NI], bool (&on)[NI]) {
+    bool any = false;
+#pragma unroll
+    for (int k = 0; k < NI; ++k) {
+        const int j = first + k;
+        on[k] = j < batch;
+        inst[k] = on[k] ? (map ? map[j] : j) : (map ? map[first] : first);
+        if (on[k] && st && !st[inst[k]].active) on[k] = false;
+        any = any || on[k];
+    }
+    return any;
+}
+
 // ------------------------------------------------------------------ calcNodal
-template <typename V>
+// NI instances per thread: every (column, coefficient) entry is read once for all of them.
+template <typename V, int NI>
 __global__ void __launch_bounds__(256) sp_nodal_kernel(SparseDev s, const V* __restrict__ x, int64_t xStride,
-                                                       V* __restrict__ nodal, int64_t nStride, int64_t nml,
+                                                       V* __restrict__ nodal, int64_t nStride, int64_t nml, int batch,
                                                        const CGState* __restrict__ st, const int* __restrict__ map) {
-    const int inst = map ? map[blockIdx.y] : (int)blockIdx.y;
-    if (st && !st[inst].active) return;
-    const V* xi = x + inst * xStride;
-    V* out = nodal + inst * nStride;
+    int inst[NI];
+    bool on[NI];
+    if (!group_slots<NI>((int)blockIdx.y * NI, batch, map, st, inst, on)) return;
+    const V* xi[NI];
+#pragma unroll
+    for (int k = 0; k < NI; ++k) xi[k] = x + inst[k] * xStride;
     for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nml; r += (int64_t)gridDim.x * blockDim.x) {
-        V v = V(0);
+        V v[NI];
+#pragma unroll
+        for (int k = 0; k < NI; ++k) v[k] = V(0);
         const int64_t k1 = s.nodPtr[r + 1];
-        for (int64_t k = s.nodPtr[r]; k < k1; ++k) v = fma(V(s.nodVal[k]), xi[s.nodCol[k]], v);
-        out[r] = v;
+        for (int64_t e = s.nodPtr[r]; e < k1; ++e) {
+            const V w = V(s.nodVal[e]);
+            const int32_t c = s.nodCol[e];
+#pragma unroll
+            for (int k = 0; k < NI; ++k) v[k] = fma(w, xi[k][c], v[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < NI; ++k)
+            if (on[k]) nodal[inst[k] * nStride + r] = v[k];
     }
 }
 
 // ---------------------------------------------------------------- prolongation
-// One workgroup per (chain along dd, instance): box q = input(q) + I_dd(box q-1).
-template <typename V>
+// One workgroup per (chain along dd, group of NI instances): box q = input(q) + I_dd(box q-1).
+template <typename V, int NI>
 __global__ void __launch_bounds__(256) sp_prol_kernel(const ProlItem* __restrict__ items, SparseDev s,
                                                       const V* __restrict__ nodal, int64_t nStride,
                                                       V* __restrict__ cbuf, int64_t cStride, int batch,
                                                       const CGState* __restrict__ st, const int* __restrict__ map) {
-    const int inst = map ? map[blockIdx.x % batch] : (int)(blockIdx.x % batch);
-    if (st && !st[inst].active) return;
-    const ProlItem& it = items[blockIdx.x / batch];
+    const int ng = (batch + NI - 1) / NI;
+    int inst[NI];
+    bool on[NI];
+    if (!group_slots<NI>((int)(blockIdx.x % ng) * NI, batch, map, st, inst, on)) return;
+    const ProlItem& it = items[blockIdx.x / ng];
     const int dd = it.dd;
     // box q reads the nodal buffer if bit q of from_nodal is set, else the case buffer;
     // box 0 keeps its values where they are (nothing coarser along dd)
-    const V* nodalI = nodal + inst * nStride;
-    const V* cbufI = cbuf + inst * cStride;
-    V* outBase = cbuf + inst * cStride;
     for (int q = 1; q < it.nt; ++q) {
         const int t = it.tidx[q];
         const int64_t size = s.boxSize[t];
-        const V* in = (((it.from_nodal >> q) & 1) ? nodalI : cbufI) + it.in_off[q];
-        V* out = outBase + it.out_off[q];
-        {
-            const V* prev = q == 1 ? (((it.from_nodal & 1) ? nodalI : cbufI) + it.in_off[0])
-                                   : outBase + it.out_off[q - 1];
-            const int32_t* ab = s.prolAB + 2 * s.prolOff[(int64_t)t * s.d + dd];
-            for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
-                const int32_t a = ab[2 * i], b = ab[2 * i + 1];
+        const int32_t* ab = s.prolAB + 2 * s.prolOff[(int64_t)t * s.d + dd];
+        const bool inNodal = (it.from_nodal >> q) & 1;
+        const V* in[NI];
+        const V* prev[NI];
+        V* out[NI];
+#pragma unroll
+        for (int k = 0; k < NI; ++k) {
+            const V* nodalI = nodal + inst[k] * nStride;
+            const V* cbufI = cbuf + inst[k] * cStride;
+            in[k] = (inNodal ? nodalI : cbufI) + it.in_off[q];
+            out[k] = cbuf + inst[k] * cStride + it.out_off[q];
+            prev[k] = q == 1 ? (((it.from_nodal & 1) ? nodalI : cbufI) + it.in_off[0]) : cbufI + it.out_off[q - 1];
+        }
+        for (int64_t i = threadIdx.x; i < size; i += blockDim.x) {
+            const int32_t pa = ab[2 * i], pb = ab[2 * i + 1];
+#pragma unroll
+            for (int k = 0; k < NI; ++k) {
                 V add;
-                if (b == -2) add = prev[a];
-                else add = V(0.5) * ((a >= 0 ? prev[a] : V(0)) + (b >= 0 ? prev[b] : V(0)));
-                out[i] = in[i] + add;
+                if (pb == -2) add = prev[k][pa];
+                else add = V(0.5) * ((pa >= 0 ? prev[k][pa] : V(0)) + (pb >= 0 ? prev[k][pb] : V(0)));
+                const V v = in[k][i] + add;
+                if (on[k]) out[k][i] = v;
             }
         }
         pass_barrier();
@@ -329,16 +369,7 @@ __global__ void __launch_bounds__(NT) sp_apply_col_kernel(SparseApplyArgs<V> a) 
     const SparseDev& s = a.sp;
     int inst[NI];
     bool on[NI];
-    bool any = false;
-#pragma unroll
-    for (int k = 0; k < NI; ++k) {
-        const int j = grp * NI + k;
-        on[k] = j < a.batch;
-        inst[k] = on[k] ? (a.map ? a.map[j] : j) : (a.map ? a.map[grp * NI] : grp * NI);
-        if (on[k] && a.st && !a.st[inst[k]].active) on[k] = false;
-        any = any || on[k];
-    }
-    if (!any) return;
+    if (!group_slots<NI>(grp * NI, a.batch, a.map, a.st, inst, on)) return;
     const int t = it.tidx;
     const int64_t size = s.boxSize[t];
     constexpr int NC = 1 << D;
@@ -529,15 +560,24 @@ void launch_sp_cellcoef(const SparseDev& s, int64_t nCells, int nA, int nC, cons
 template <typename V>
 void launch_sp_nodal(const SparseDev& s, const V* x, int64_t xStride, V* nodal, int64_t nStride, int64_t nml,
                      int batch, const CGState* st, hipStream_t str, const int* map) {
-    hipLaunchKernelGGL(sp_nodal_kernel<V>, dim3(gridOf(nml), batch), dim3(256), 0, str, s, x, xStride, nodal, nStride,
-                       nml, st, map);
+    // four instances per thread from batch 2 on
+    if (batch > 1)
+        hipLaunchKernelGGL((sp_nodal_kernel<V, 4>), dim3(gridOf(nml), (batch + 3) / 4), dim3(256), 0, str, s, x,
+                           xStride, nodal, nStride, nml, batch, st, map);
+    else
+        hipLaunchKernelGGL((sp_nodal_kernel<V, 1>), dim3(gridOf(nml), batch), dim3(256), 0, str, s, x, xStride,
+                           nodal, nStride, nml, batch, st, map);
 }
 template <typename V>
 void launch_sp_prol(const ProlItem* items, int nitems, const SparseDev& s, const V* nodal, int64_t nStride, V* cbuf,
                     int64_t cStride, int batch, const CGState* st, hipStream_t str, const int* map) {
     if (nitems == 0) return;
-    hipLaunchKernelGGL(sp_prol_kernel<V>, dim3((unsigned)nitems * (unsigned)batch), dim3(256), 0, str, items, s, nodal,
-                       nStride, cbuf, cStride, batch, st, map);
+    if (batch > 1)
+        hipLaunchKernelGGL((sp_prol_kernel<V, 4>), dim3((unsigned)nitems * (unsigned)((batch + 3) / 4)), dim3(256), 0,
+                           str, items, s, nodal, nStride, cbuf, cStride, batch, st, map);
+    else
+        hipLaunchKernelGGL((sp_prol_kernel<V, 1>), dim3((unsigned)nitems * (unsigned)batch), dim3(256), 0, str, items,
+                           s, nodal, nStride, cbuf, cStride, batch, st, map);
 }
 template <typename V>
 void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str, int cls) {
