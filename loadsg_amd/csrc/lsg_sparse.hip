@@ -145,7 +145,8 @@ __global__ void __launch_bounds__(256) sp_nodal_kernel(SparseDev s, const V* __r
 }
 
 // ---------------------------------------------------------------- prolongation
-// One workgroup per (chain along dd, group of NI instances): box q = input(q) + I_dd(box q-1).
+// One workgroup per (chain along dd, group of NI instances): box q = input(q) + I_dd(box q-1)
+// (prolongation1D_inplace + addMultiLevelValues, MatrixVectorHomogen.h:444-474,997-1019).
 template <typename V, int NI>
 __global__ void __launch_bounds__(256) sp_prol_kernel(const ProlItem* __restrict__ items, SparseDev s,
                                                       const V* __restrict__ nodal, int64_t nStride,
@@ -572,12 +573,11 @@ template <typename V>
 void launch_sp_prol(const ProlItem* items, int nitems, const SparseDev& s, const V* nodal, int64_t nStride, V* cbuf,
                     int64_t cStride, int batch, const CGState* st, hipStream_t str, const int* map) {
     if (nitems == 0) return;
-    if (batch > 1)
-        hipLaunchKernelGGL((sp_prol_kernel<V, 4>), dim3((unsigned)nitems * (unsigned)((batch + 3) / 4)), dim3(256), 0,
-                           str, items, s, nodal, nStride, cbuf, cStride, batch, st, map);
-    else
-        hipLaunchKernelGGL((sp_prol_kernel<V, 1>), dim3((unsigned)nitems * (unsigned)batch), dim3(256), 0, str, items,
-                           s, nodal, nStride, cbuf, cStride, batch, st, map);
+    // one instance per workgroup: the chains are serial box by box, and grouping instances
+    // cut the workgroups in flight more than it saved in table loads (level-11 adaptive grid,
+    // batch 64: 1.40 ms with four instances per workgroup, 0.83 ms with one)
+    hipLaunchKernelGGL((sp_prol_kernel<V, 1>), dim3((unsigned)nitems * (unsigned)batch), dim3(256), 0, str, items, s,
+                       nodal, nStride, cbuf, cStride, batch, st, map);
 }
 template <typename V>
 void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str, int cls) {
