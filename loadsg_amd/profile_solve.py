@@ -24,6 +24,7 @@ def main():
     ap.add_argument("--batch", type=int, default=500)
     ap.add_argument("--solves", type=int, default=1)
     ap.add_argument("--precond", choices=("block", "diag"), default="block")
+    ap.add_argument("--inner-tol", type=float, default=1e-4, help="inner_rel_tol of the refinement")
     args = ap.parse_args()
 
     import torch
@@ -46,7 +47,7 @@ def main():
         op32.set_coefficients(cs)
         op64.rhs_sines(rng.standard_normal((B, 8)), m, out=f)
         x.zero_()
-        return L.CG.solveMixed(1e-8, x, f, op64, op32)
+        return L.CG.solveMixed(1e-8, x, f, op64, op32, inner_rel_tol=args.inner_tol)
 
     solve()
     torch.cuda.synchronize()
@@ -59,7 +60,8 @@ def main():
         infos += solve()
     torch.cuda.synchronize()
     dt = (time.perf_counter() - t0) / args.solves
-    print(json.dumps({"batch": B, "precond": args.precond, "solves": args.solves, "s_per_solve_batch": dt,
+    print(json.dumps({"batch": B, "precond": args.precond, "inner_tol": args.inner_tol, "solves": args.solves,
+                      "s_per_solve_batch": dt,
                       "solves_per_s": B / dt,
                       "inner_iterations_mean": float(np.mean([i["iterations"] for i in infos])),
                       "outer_sweeps_mean": float(np.mean([i["outer"] for i in infos]))}))
