@@ -191,6 +191,25 @@ __device__ inline void prolong_dim(V* __restrict__ cur, const V* __restrict__ pr
     }
 }
 
+// I_fd(prev) at fine point p (prolongation1D_inplace, MatrixVectorHomogen.h:444-474): the
+// coarse value below an even fine position, the mean of the two coarse neighbours of an odd
+// one (0 outside the coarse box of nc = (nf - 1) / 2 points along the dimension of stride
+// S).  Branch-free (neighbouring lanes alternate between the two cases): both reads come from
+// in-box addresses and are selected afterwards; the same arithmetic as prolong_dim.
+template <typename V>
+__device__ inline V prol_add(const V* __restrict__ prev, int p, int S, float rS, int nf, float rnf) {
+    const int nc = (nf - 1) >> 1;
+    const Split sp = split(p, S, rS, nf, rnf);
+    const int j = sp.k >> 1;                    // coarse position below (k - 1) / 2, k = sp.k + 1
+    const bool odd = (sp.k & 1) == 0;
+    const int ob = sp.lo + mul24(sp.hi, S * nc) + mul24(j, S);
+    const bool hasA = odd && j >= 1, hasB = !odd || j < nc;
+    const V ra = prev[hasA ? ob - S : 0];
+    const V rb = prev[hasB ? ob : 0];
+    const V va = hasA ? ra : V(0), vb = hasB ? rb : V(0);
+    return odd ? V(0.5) * (va + vb) : vb;
+}
+
 // Weighted 1D Q1 operators at level t on one line (positions 1..n, cell c between
 // positions c and c+1 carries weight w[c]):
 //   (K v)_k = 2^t  [ w_{k-1}(v_k - v_{k-1}) + w_k (v_k - v_{k+1}) ]
@@ -412,26 +431,12 @@ __global__ void __launch_bounds__(512) prol_chain_kernel(const ProlItem* __restr
             }
         } else {
             V* out = outBase + it.out_off[q];
-            const int nc = (nq - 1) >> 1;
             const float rnf = rcpf(nq);
 #pragma unroll
             for (int s = 0; s < SC; ++s) {
                 const int p = threadIdx.x + s * B - off;
-                if (p >= 0 && p < tq) {
-                    Split sp = split(p, S, rS, nq, rnf);
-                    const V* c = prev + sp.lo + mul24(sp.hi, S * nc);
-                    const int k = sp.k + 1;
-                    const V* cb = c + mul24((k - 1) >> 1, S);
-                    V add;
-                    if (k & 1) {
-                        const int cl = (k - 1) >> 1, cr = (k + 1) >> 1;
-                        V va = (cl >= 1) ? cb[-S] : V(0);
-                        V vb = (cr <= nc) ? cb[0] : V(0);
-                        add = V(0.5) * (va + vb);
-                    } else {
-                        add = cb[0];
-                    }
-                    const V v = nin[s] + add;
+                if (p >= 0 && p < tq) {         // the slots of this box only
+                    const V v = nin[s] + prol_add<V>(prev, p, S, rS, nq, rnf);
                     out[p] = v;
                     if (keep) cur[p] = v;
                 }
@@ -482,7 +487,7 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
 #pragma unroll
         for (int s = 0; s < PPT; ++s) {
             const int p = threadIdx.x + s * blockDim.x;
-            if (p < total) nin[s] = in[p];
+            nin[s] = p < total ? in[p] : V(0);
         }
     }
     const float rS = rcpf(S);
@@ -493,27 +498,13 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
 #pragma unroll
             for (int s = 0; s < PPT; ++s) v[s] = nin[s];
         } else {
-            const int nc = (nf - 1) >> 1;
             const float rnf = rcpf(nf);
 #pragma unroll
             for (int s = 0; s < PPT; ++s) {
                 const int p = threadIdx.x + s * blockDim.x;
-                if (p < total) {
-                    Split sp = split(p, S, rS, nf, rnf);
-                    const V* c = prev + sp.lo + mul24(sp.hi, S * nc);
-                    const int k = sp.k + 1;
-                    V add;
-                    const V* cb = c + mul24((k - 1) >> 1, S);
-                    if (k & 1) {
-                        const int cl = (k - 1) >> 1, cr = (k + 1) >> 1;
-                        V va = (cl >= 1) ? cb[-S] : V(0);
-                        V vb = (cr <= nc) ? cb[0] : V(0);
-                        add = V(0.5) * (va + vb);
-                    } else {
-                        add = cb[0];
-                    }
-                    v[s] = nin[s] + add;
-                }
+                V vs = V(0);
+                if (p < total) vs = nin[s] + prol_add<V>(prev, p, S, rS, nf, rnf);
+                v[s] = vs;
             }
         }
         const int totalN = keep ? total / nf * (2 * nf + 1) : 0;
@@ -522,7 +513,7 @@ __global__ void __launch_bounds__(512) prol_pf_kernel(const ProlItem* __restrict
 #pragma unroll
             for (int s = 0; s < PPT; ++s) {
                 const int p = threadIdx.x + s * blockDim.x;
-                if (p < totalN) nin[s] = in[p];
+                nin[s] = p < totalN ? in[p] : V(0);
             }
         }
         V* out = outBase + it.out_off[q];
@@ -788,7 +779,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
 #pragma unroll
         for (int s = 0; s < PPT; ++s) {
             const int p = threadIdx.x + s * blockDim.x;
-            if (p < total) nin[s] = in[p];
+            nin[s] = p < total ? in[p] : V(0);
         }
     }
 
@@ -805,40 +796,28 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
         // addMultiLevelValues, MatrixVectorHomogen.h:444-474,997-1019)
         V pk[PPT];      // P of this box, parked for the next box at the end
         const V* in = (it.in_nodal ? nodal : cbuf) + it.in_off;
+        // every element of pk is assigned on every path (lanes past the box hold 0), so the
+        // register array is not copied around the bounds test
         if (q == 0) {
 #pragma unroll
             for (int s = 0; s < PPT; ++s) {
                 const int p = threadIdx.x + s * blockDim.x;
-                if (p < total) {
-                    const V v = kPf ? nin[kPf ? s : 0] : in[p];
-                    Y[p] = v;
-                    pk[s] = v;
-                }
+                const bool inb = p < total;
+                const V v = kPf ? nin[kPf ? s : 0] : (inb ? in[p] : V(0));
+                if (inb) Y[p] = v;
+                pk[s] = v;
             }
         } else {
-            const int nc = (nfd - 1) >> 1;
             const float rS = rcpf(Sfd), rnf = rcpf(nfd);
 #pragma unroll
             for (int s = 0; s < PPT; ++s) {
                 const int p = threadIdx.x + s * blockDim.x;
+                V v = V(0);
                 if (p < total) {
-                    Split sp = split(p, Sfd, rS, nfd, rnf);
-                    const V* c = U + sp.lo + mul24(sp.hi, Sfd * nc);
-                    const int k = sp.k + 1;
-                    V add;
-                    const V* cb = c + mul24((k - 1) >> 1, Sfd);
-                    if (k & 1) {
-                        const int cl = (k - 1) >> 1, cr = (k + 1) >> 1;
-                        V va = (cl >= 1) ? cb[-Sfd] : V(0);
-                        V vb = (cr <= nc) ? cb[0] : V(0);
-                        add = V(0.5) * (va + vb);
-                    } else {
-                        add = cb[0];
-                    }
-                    const V v = (kPf ? nin[kPf ? s : 0] : in[p]) + add;
+                    v = (kPf ? nin[kPf ? s : 0] : in[p]) + prol_add<V>(U, p, Sfd, rS, nfd, rnf);
                     Y[p] = v;
-                    pk[s] = v;
                 }
+                pk[s] = v;
             }
         }
         Z = U;
@@ -853,7 +832,7 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
 #pragma unroll
             for (int s = 0; s < PPT; ++s) {
                 const int p = threadIdx.x + s * blockDim.x;
-                if (p < totalN) nin[kPf ? s : 0] = inN[p];
+                nin[kPf ? s : 0] = p < totalN ? inN[p] : V(0);
             }
         }
         // shapes of the box and after each restriction (host-computed, scalar loads)
