@@ -391,8 +391,8 @@ struct lsg_op {
     // preconditioner of lsg_cg_solve and of the inner solves of lsg_ir_solve
     // (LSG_PRECOND_DIAG: the reference's precond3; LSG_PRECOND_DEPTH_BLOCK: BlockPre)
     int precKind = LSG_PRECOND_DIAG;
-    DevMem preItems, preInv, preD0, prePart, preFdList, preFdPerm, preFdV, preFdLam;
-    int preNblk = 0, preMaxN = 0, preNfd = 0, preMaxFd = 0;
+    DevMem preItems, preInv, preD0, prePart, preFdList, preFdPerm, preFdV, preFdLam, preMmList;
+    int preNblk = 0, preMaxN = 0, preNfd = 0, preMaxFd = 0, preNmm = 0, preMmMax = 0;
     bool capturing = false;   // matvec launches record no timing events while captured
     std::unique_ptr<TrDev> tr;   // basis transformations (built on first use)
     int64_t n() const { return G->n; }
@@ -746,7 +746,22 @@ int ensureBlockPre(lsg_op* op) {
     const int dmax = std::min(op->prec == 32 ? 128 : 64, envInt("LSG_PRE_DENSE_MAX", 128));
     if (int rc = buildBlockPre(*op->G, dmax, bp, err)) return fail(rc, err);
     op->preNblk = (int)bp.blocks.size();
-    op->preMaxN = bp.maxN;
+    op->preMaxN = 0;
+    // FP32 operators apply the dense blocks of at least PRE_MM_MIN points on the matrix
+    // cores; the others (and every dense block of an FP64 operator) in blockpre_kernel
+    std::vector<int32_t> mm;
+    op->preMmMax = 0;
+    for (size_t b = 0; b < bp.blocks.size(); ++b) {
+        const PreBlock& pb = bp.blocks[b];
+        if (pb.mode != PRE_DENSE) continue;
+        if (op->prec == 32 && pb.n >= PRE_MM_MIN && !envInt("LSG_NO_PRE_MM", 0)) {
+            mm.push_back((int32_t)b);
+            op->preMmMax = std::max(op->preMmMax, (int)pb.n);
+        } else {
+            op->preMaxN = std::max(op->preMaxN, (int)pb.n);
+        }
+    }
+    op->preNmm = (int)mm.size();
     HIPCHK(op->preItems.alloc(bp.blocks.size() * sizeof(PreBlock)));
     HIPCHK(hipMemcpy(op->preItems.p, bp.blocks.data(), bp.blocks.size() * sizeof(PreBlock), hipMemcpyHostToDevice));
     auto upload = [&](DevMem& m, const std::vector<double>& v) -> int {
@@ -773,6 +788,7 @@ int ensureBlockPre(lsg_op* op) {
     };
     if (int rc = upload32(op->preFdList, bp.fdList)) return rc;
     if (int rc = upload32(op->preFdPerm, bp.fdPerm)) return rc;
+    if (int rc = upload32(op->preMmList, mm)) return rc;
     HIPCHK(op->prePart.alloc((size_t)op->batch * op->preNblk * sizeof(double)));
     return LSG_OK;
 }
@@ -792,6 +808,9 @@ BlockPreArgs<V> blockPreArgs(lsg_op* op, bool on) {
     a.maxFd = op->preMaxFd;
     a.fdList = op->preFdList.as<int32_t>();
     a.fdPerm = op->preFdPerm.as<int32_t>();
+    a.nmm = op->preNmm;
+    a.mmMax = op->preMmMax;
+    a.mmList = op->preMmList.as<int32_t>();
     a.fdV = op->preFdV.as<V>();
     a.fdLam = op->preFdLam.as<V>();
     return a;

@@ -9,6 +9,7 @@
 // (fixed-order wave shuffles + fixed-order partial sums), so results are
 // bit-reproducible for a given launch shape.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "lsg_kernels.hpp"
 
@@ -1287,10 +1288,11 @@ __global__ void __launch_bounds__(256) blockpre_kernel(const PreBlock* __restric
                                                        const V* __restrict__ r, const V* __restrict__ D,
                                                        V* __restrict__ h, V* __restrict__ dv, int64_t n, int batch,
                                                        int ipw, double* __restrict__ part,
-                                                       const CGState* __restrict__ st) {
+                                                       const CGState* __restrict__ st, int mmMin) {
     extern __shared__ char smem_raw[];
     const PreBlock it = items[blockIdx.x];
     if (it.mode == PRE_FD) return;         // fast-diagonalisation block: blockfd_kernel
+    if (it.mode == PRE_DENSE && it.n >= mmMin) return;     // blockmm_kernel
     const int nl = it.n;
     V* Bs = reinterpret_cast<V*>(smem_raw);
     const bool dense = it.mode == PRE_DENSE;
@@ -1357,6 +1359,135 @@ __global__ void __launch_bounds__(256) blockpre_kernel(const PreBlock* __restric
             acc = wave_sum(acc);
             if (lane == 0) part[(int64_t)inst * nblk + blockIdx.x] = acc;
         }
+    }
+}
+
+// FP32 dense blocks of at least PRE_MM_MIN points on the matrix cores: Y = B^-1 X for a group of
+// 64 instances is a (n x n) x (n x 64) product, formed with v_mfma_f32_32x32x2_f32 (FP32
+// operands and accumulation) over 32 x 32 output tiles, the block padded to a multiple of
+// 32 with zeros.  Workgroup (block, group of 64 instances): the inverse (row stride np + 1)
+// and X = s .* r (instance-major, stride np + 1, so the 32 rows or instances a wave reads
+// fall in distinct banks) are staged in LDS; each wave takes output tiles (row strip, 32
+// instances) and accumulates over k in a fixed order, so an instance's result does not
+// depend on the others in its group.  The tiles are parked in LDS and written out
+// instance by instance (lanes along the block's points), with h = s .* y and the r.h partial
+// summed over the wave in a fixed order as in blockpre_kernel.
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+template <bool INIT>
+__global__ void __launch_bounds__(256) blockmm_kernel(BlockPreArgs<float> a, const float* __restrict__ r,
+                                                      const float* __restrict__ D, float* __restrict__ h,
+                                                      float* __restrict__ dv, int64_t n, int batch, int groups,
+                                                      const CGState* __restrict__ st) {
+    extern __shared__ char smem_raw[];
+    const int bi = a.mmList[blockIdx.x];
+    const PreBlock it = a.items[bi];
+    const int nl = it.n, np = (nl + 31) & ~31, ld = np + 1;
+    float* Bs = reinterpret_cast<float*>(smem_raw);   // np x ld, row-major
+    float* X = Bs + np * ld;                            // 64 x ld, instance-major (then Y)
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const float* inv = a.inv + it.inv;                   // column k contiguous: (i, k) at k * nl + i
+    // the inverse, four columns per wave and step (eight loads in flight per lane)
+    for (int k0 = wave; k0 < np; k0 += 16) {
+        float v[4][2];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                const int k = k0 + 4 * u, i = lane + 64 * w;
+                v[u][w] = (k < nl && i < nl) ? inv[k * nl + i] : 0.0f;
+            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int w = 0; w < 2; ++w) {
+                const int k = k0 + 4 * u, i = lane + 64 * w;
+                if (k < np && i < np) Bs[i * ld + k] = v[u][w];
+            }
+    }
+    // output tiles: wave w takes (row strip R0 = w / 2, instances C = w % 2) and, for blocks
+    // of 128 points, also row strip R0 + 2 with the same instances (one B read, two MFMAs)
+    const int nR = np >> 5;
+    const int C = wave & 1, R0 = wave >> 1, R1 = R0 + 2;
+    const bool t0 = R0 < nR, t1 = R1 < nR;
+    for (int grp = 0; grp < groups; ++grp) {
+        const int i0 = (blockIdx.y * groups + grp) * 64;
+        if (i0 >= batch) break;
+        // X = s .* r, four instances per wave and step
+        for (int c0 = wave; c0 < 64; c0 += 16) {
+            float v[4][2];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int inst = i0 + c0 + 4 * u;
+                const bool run = inst < batch && (INIT || st[inst].active);
+#pragma unroll
+                for (int w = 0; w < 2; ++w) {
+                    const int k = lane + 64 * w;
+                    float x = 0.0f;
+                    if (run && k < nl) {
+                        const int64_t o = (int64_t)inst * n + it.off + k;
+                        x = sqrtf(a.d0[it.off + k] / D[o]) * r[o];
+                    }
+                    v[u][w] = x;
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+#pragma unroll
+                for (int w = 0; w < 2; ++w) {
+                    const int k = lane + 64 * w;
+                    if (k < np) X[(c0 + 4 * u) * ld + k] = v[u][w];
+                }
+        }
+        __syncthreads();
+        // lane holds instance C * 32 + (lane & 31) and rows R * 32 + 8 (q / 4) + 4 (lane / 32)
+        // + q % 4 of its 16 accumulators; k in a fixed order
+        floatx16_t c0v = {}, c1v = {};
+        {
+            const float* ar0 = Bs + (R0 * 32 + (lane & 31)) * ld + (lane >> 5);
+            const float* ar1 = Bs + (R1 * 32 + (lane & 31)) * ld + (lane >> 5);
+            const float* br = X + (C * 32 + (lane & 31)) * ld + (lane >> 5);
+            if (t1) {
+#pragma unroll 4
+                for (int k0 = 0; k0 < np; k0 += 2) {
+                    const float b = br[k0];
+                    c0v = __builtin_amdgcn_mfma_f32_32x32x2f32(ar0[k0], b, c0v, 0, 0, 0);
+                    c1v = __builtin_amdgcn_mfma_f32_32x32x2f32(ar1[k0], b, c1v, 0, 0, 0);
+                }
+            } else if (t0) {
+#pragma unroll 4
+                for (int k0 = 0; k0 < np; k0 += 2)
+                    c0v = __builtin_amdgcn_mfma_f32_32x32x2f32(ar0[k0], br[k0], c0v, 0, 0, 0);
+            }
+        }
+        __syncthreads();
+        {
+            float* y = X + (C * 32 + (lane & 31)) * ld + 4 * (lane >> 5);
+            if (t0) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) y[R0 * 32 + 8 * (q >> 2) + (q & 3)] = c0v[q];
+            }
+            if (t1) {
+#pragma unroll
+                for (int q = 0; q < 16; ++q) y[R1 * 32 + 8 * (q >> 2) + (q & 3)] = c1v[q];
+            }
+        }
+        __syncthreads();
+        for (int c = wave; c < 64; c += 4) {
+            const int inst = i0 + c;
+            if (!(inst < batch && (INIT || st[inst].active))) continue;
+            const int64_t o = (int64_t)inst * n + it.off;
+            double accd = 0.0;
+            for (int k = lane; k < nl; k += 64) {
+                const float rk = r[o + k];
+                const float hk = sqrtf(a.d0[it.off + k] / D[o + k]) * X[c * ld + k];
+                h[o + k] = hk;
+                if (INIT) dv[o + k] = -1.0f * hk;
+                accd += double(rk) * double(hk);
+            }
+            accd = wave_sum(accd);
+            if (lane == 0) a.part[(int64_t)inst * a.nblk + bi] = accd;
+        }
+        __syncthreads();
     }
 }
 
@@ -1656,12 +1787,28 @@ static void launch_blockpre(const BlockPreArgs<V>& pre, bool init, const V* r, c
     const int ipw = std::min(64, (batch + 3) / 4 * 4);
     const dim3 grid((unsigned)pre.nblk, (unsigned)((batch + ipw - 1) / ipw));
     const size_t lds = ((size_t)pre.maxN * pre.maxN + 4 * PRE_MAXN) * sizeof(V);
+    const int mmMin = pre.nmm > 0 ? PRE_MM_MIN : 1 << 30;
     if (init)
         hipLaunchKernelGGL((blockpre_kernel<V, true>), grid, dim3(256), lds, s, pre.items, pre.nblk, pre.inv, pre.d0, r,
-                           D, h, dv, n, batch, ipw, pre.part, st);
+                           D, h, dv, n, batch, ipw, pre.part, st, mmMin);
     else
         hipLaunchKernelGGL((blockpre_kernel<V, false>), grid, dim3(256), lds, s, pre.items, pre.nblk, pre.inv, pre.d0,
-                           r, D, h, dv, n, batch, ipw, pre.part, st);
+                           r, D, h, dv, n, batch, ipw, pre.part, st, mmMin);
+    if constexpr (std::is_same<V, float>::value) {
+        if (pre.nmm > 0) {
+            const int np = (pre.mmMax + 31) & ~31;
+            const size_t ldm = (size_t)(np * (np + 1) + 64 * (np + 1)) * sizeof(float);
+            // groups of 64 instances per workgroup: the staged inverse serves up to 2 of them
+            const int ng = (batch + 63) / 64, groups = std::min(2, ng);
+            const dim3 gmm((unsigned)pre.nmm, (unsigned)((ng + groups - 1) / groups));
+            if (init)
+                hipLaunchKernelGGL((blockmm_kernel<true>), gmm, dim3(256), ldm, s, pre, r, D, h, dv, n, batch, groups,
+                                   st);
+            else
+                hipLaunchKernelGGL((blockmm_kernel<false>), gmm, dim3(256), ldm, s, pre, r, D, h, dv, n, batch,
+                                   groups, st);
+        }
+    }
     if (pre.nfd > 0) {
         // 16 instances per workgroup: the blocks are few and large
         const int ipf = std::min(16, (batch + 3) / 4 * 4);

@@ -159,6 +159,7 @@ void launch_rhs_sines(const SumItem* items, int nitems, int d, int nterms, const
 // (B_T^-1 (s .* r)) on every depth block T with an inverse, s = sqrt(d0 / D), h = r / D on the
 // others; part receives the r.h partial of every (instance, block).  items == null: the
 // diagonal preconditioner.
+constexpr int PRE_MM_MIN = 32;
 template <typename V>
 struct BlockPreArgs {
     const PreBlock* items = nullptr;
@@ -172,6 +173,9 @@ struct BlockPreArgs {
     int nfd = 0, maxFd = 0; // blocks on that path, the largest product set of them
     const int32_t* fdList = nullptr;
     const int32_t* fdPerm = nullptr;
+    // FP32: dense blocks of at least PRE_MM_MIN points on the matrix cores (blockmm_kernel)
+    int nmm = 0, mmMax = 0;
+    const int32_t* mmList = nullptr;
     const V* fdV = nullptr;
     const V* fdLam = nullptr;
 };

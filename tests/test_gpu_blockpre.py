@@ -186,3 +186,24 @@ def test_block_batch_matches_single_and_graph_replay(kind):
     X3 = np.zeros((B, grid.size))
     L.CG.solveHomogen(1e-10, X3, F, op, stop_rule=L.STOP_REL, maxit=2000)
     assert np.array_equal(X, X3)
+
+
+@pytest.mark.parametrize("d,level", [(3, 8), (5, 6), (2, 7)])
+def test_fp32_block_application_matches_restatement(d, level):
+    """FP32 operators apply their dense blocks of 32 points and more on the matrix cores
+    (blockmm_kernel), the smaller ones in blockpre_kernel.  One PCG step from zero gives
+    x1 = alpha M^-1 f; its direction must be the restatement's M^-1 f (FP64 inverses) to FP32
+    accuracy.  d=3 L8: blocks of 32, 64 and 128 points; d=5 L6: up to 32; d=2 L7: up to 32."""
+    print("fp32 block application d=%d level=%d" % (d, level), flush=True)
+    grid = L.AdaptiveSparseGrid.regular(d, level)
+    idx, _ = grid.slot_order()
+    op = L.MatrixVectorHomogen(grid, expsin(7), precision=32).set_precond(L.PRECOND_DEPTH_BLOCK)
+    D = np.asarray(op.diagonal(), dtype=np.float64)
+    b = np.random.default_rng(d * 10 + level).standard_normal(grid.size)
+    x = np.zeros(grid.size)
+    L.CG.solveHomogen(1e-30, x, b, op, stop_rule=L.STOP_REL, maxit=1)
+    href = make_preconditioner(idx, D, max_dense=128)(b)
+    u, v = x / np.linalg.norm(x), href / np.linalg.norm(href)
+    if u @ v < 0:
+        u = -u
+    assert np.linalg.norm(u - v) < 2e-5, np.linalg.norm(u - v)
