@@ -666,7 +666,11 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
 // next box; the last sweep does not store Y, folds in the dimensions of extent 1
 // (Z = cM Z + cK Y, see apply_sep_kernel) and adds the gM value g0 read for this box
 // (held in registers).
-template <typename V, int PPT>
+// DEC: bit 0 -- dimension i is the fastest of the box (S = 1, no division for the lower
+// part); bit 1 -- no dimension above i has more than one point (S n = total, no division for
+// the upper part).  The callers pick it per sweep, so the coordinate decode costs one
+// division instead of two on such sweeps.
+template <typename V, int PPT, int DEC>
 __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const LSG_CONST BoxShape& sh, int i,
                                  const V* __restrict__ w, int t, bool first, bool needY, V* __restrict__ keep,
                                  bool last, bool fold, V cM, V cK, const V* __restrict__ g0) {
@@ -680,8 +684,8 @@ __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const LSG
     for (int s = 0; s < PPT; ++s) {
         const int p = threadIdx.x + s * B;
         if (p < total) {
-            const int q = fdiv(p, rS);
-            const int kq = q - mul24(fdiv(q, rn), n);
+            const int q = (DEC & 1) ? p : fdiv(p, rS);
+            const int kq = (DEC & 2) ? q : q - mul24(fdiv(q, rn), n);
             const bool hasL = kq > 0, hasR = kq < n - 1;
             // neighbours loaded from an in-box address, then selected
             const V yc = Y[p];
@@ -879,8 +883,15 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
         } else {
             for (int i = hi; i >= lo; --i) {
                 if (it.T[i] == 1) continue;
-                sep_sweep<V, PPT>(Y, Z, sh, i, W + woff[i] + (i == fd ? wq : 0), it.T[i], i == hi, true,
-                                  nullptr, i == lo, fold, cM, cK, i == lo ? g0 : nullptr);
+                const int dec = (sh.S[i] == 1 ? 1 : 0) | (sh.S[i] * sh.n[i] == sh.total ? 2 : 0);
+                const V* wi = W + woff[i] + (i == fd ? wq : 0);
+                const V* gi = i == lo ? g0 : nullptr;
+                switch (dec) {
+                    case 0: sep_sweep<V, PPT, 0>(Y, Z, sh, i, wi, it.T[i], i == hi, true, nullptr, i == lo, fold, cM, cK, gi); break;
+                    case 1: sep_sweep<V, PPT, 1>(Y, Z, sh, i, wi, it.T[i], i == hi, true, nullptr, i == lo, fold, cM, cK, gi); break;
+                    case 2: sep_sweep<V, PPT, 2>(Y, Z, sh, i, wi, it.T[i], i == hi, true, nullptr, i == lo, fold, cM, cK, gi); break;
+                    default: sep_sweep<V, PPT, 3>(Y, Z, sh, i, wi, it.T[i], i == hi, true, nullptr, i == lo, fold, cM, cK, gi); break;
+                }
             }
         }
         // restrictions along the restriction dims (ascending) with the gM exchange
