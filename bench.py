@@ -3,7 +3,7 @@
 (16 xi ~ N(0, 0.25)) and right-hand sides f = sum_j eta_j prod_i sin(m_j pi x_i)
 (8 eta ~ N(0,1), m_j in {1,2,3}); synthetic seeded inputs (PCG64 seed 20261203).
 
-One step = one batched solve of --batch (500) fresh instances out of 1,000 held-out
+One step = one batched solve of --batch (1,000) fresh instances out of 2,000 held-out
 instances (seeds disjoint from the training set; the default 2 steps solve each once):
 the instances' set-up (coefficient tables of the FP64 and FP32 operators, load vectors,
 and the preconditioner diagonal that the new coefficients invalidate), the learned warm
@@ -443,8 +443,8 @@ def main():
     ap.add_argument("--steps", type=int, default=2,
                     help="timed steps; step k solves held-out batch k mod (eval / batch)")
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=500)
-    ap.add_argument("--eval", type=int, default=1000, help="held-out instances (a multiple of --batch)")
+    ap.add_argument("--batch", type=int, default=1000)
+    ap.add_argument("--eval", type=int, default=2000, help="held-out instances (a multiple of --batch)")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
