@@ -811,6 +811,7 @@ BlockPreArgs<V> blockPreArgs(lsg_op* op, bool on) {
     a.nmm = op->preNmm;
     a.mmMax = op->preMmMax;
     a.mmList = op->preMmList.as<int32_t>();
+    a.mmGroups = envInt("LSG_PRE_MM_GROUPS", 1);
     a.fdV = op->preFdV.as<V>();
     a.fdLam = op->preFdLam.as<V>();
     return a;

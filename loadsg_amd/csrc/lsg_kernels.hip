@@ -1376,13 +1376,14 @@ __global__ void __launch_bounds__(256) blockpre_kernel(const PreBlock* __restric
 // FP32 dense blocks of at least PRE_MM_MIN points on the matrix cores: Y = B^-1 X for a group of
 // 64 instances is a (n x n) x (n x 64) product, formed with v_mfma_f32_32x32x2_f32 (FP32
 // operands and accumulation) over 32 x 32 output tiles, the block padded to a multiple of
-// 32 with zeros.  Workgroup (block, group of 64 instances): the inverse (row stride np + 1)
-// and X = s .* r (instance-major, stride np + 1, so the 32 rows or instances a wave reads
-// fall in distinct banks) are staged in LDS; each wave takes output tiles (row strip, 32
-// instances) and accumulates over k in a fixed order, so an instance's result does not
-// depend on the others in its group.  The tiles are parked in LDS and written out
-// instance by instance (lanes along the block's points), with h = s .* y and the r.h partial
-// summed over the wave in a fixed order as in blockpre_kernel.
+// 32 with zeros.  Workgroup (block, `groups` groups of 64 instances): X = s .* r is staged in
+// LDS (instance-major, stride n + 1, so the 32 instances a wave reads fall in distinct
+// banks); the A operand comes straight from the inverse in global memory, which every group
+// reads and so stays in L2, leaving the LDS to X alone (33 KB: four workgroups per CU).  Each
+// wave takes output tiles (row strip, 32 instances) and accumulates over k in a fixed order,
+// so an instance's result does not depend on the others in its group.  The tiles are parked
+// in LDS and written out instance by instance (lanes along the block's points), with
+// h = s .* y and the r.h partial summed over the wave in a fixed order as in blockpre_kernel.
 typedef float floatx16_t __attribute__((ext_vector_type(16)));
 template <bool INIT>
 __global__ void __launch_bounds__(256) blockmm_kernel(BlockPreArgs<float> a, const float* __restrict__ r,
@@ -1393,28 +1394,9 @@ __global__ void __launch_bounds__(256) blockmm_kernel(BlockPreArgs<float> a, con
     const int bi = a.mmList[blockIdx.x];
     const PreBlock it = a.items[bi];
     const int nl = it.n, np = (nl + 31) & ~31, ld = np + 1;
-    float* Bs = reinterpret_cast<float*>(smem_raw);   // np x ld, row-major
-    float* X = Bs + np * ld;                            // 64 x ld, instance-major (then Y)
+    float* X = reinterpret_cast<float*>(smem_raw);     // 64 x ld, instance-major (then Y)
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const float* inv = a.inv + it.inv;                   // column k contiguous: (i, k) at k * nl + i
-    // the inverse, four columns per wave and step (eight loads in flight per lane)
-    for (int k0 = wave; k0 < np; k0 += 16) {
-        float v[4][2];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                const int k = k0 + 4 * u, i = lane + 64 * w;
-                v[u][w] = (k < nl && i < nl) ? inv[k * nl + i] : 0.0f;
-            }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int w = 0; w < 2; ++w) {
-                const int k = k0 + 4 * u, i = lane + 64 * w;
-                if (k < np && i < np) Bs[i * ld + k] = v[u][w];
-            }
-    }
     // output tiles: wave w takes (row strip R0 = w / 2, instances C = w % 2) and, for blocks
     // of 128 points, also row strip R0 + 2 with the same instances (one B read, two MFMAs)
     const int nR = np >> 5;
@@ -1454,20 +1436,32 @@ __global__ void __launch_bounds__(256) blockmm_kernel(BlockPreArgs<float> a, con
         // + q % 4 of its 16 accumulators; k in a fixed order
         floatx16_t c0v = {}, c1v = {};
         {
-            const float* ar0 = Bs + (R0 * 32 + (lane & 31)) * ld + (lane >> 5);
-            const float* ar1 = Bs + (R1 * 32 + (lane & 31)) * ld + (lane >> 5);
+            // A operand straight from the inverse in global memory (L2-resident: every group
+            // of instances reads the same block): lanes 0-31 read rows of column k0, lanes
+            // 32-63 of column k0 + 1, two 128-byte segments; rows and columns past the block
+            // (padding to 32) read an in-block address and take 0
+            const int row0 = R0 * 32 + (lane & 31), row1 = R1 * 32 + (lane & 31);
+            const bool in0 = row0 < nl, in1 = row1 < nl;
+            const float* ar0 = inv + (in0 ? row0 : 0) + (int64_t)(lane >> 5) * nl;
+            const float* ar1 = inv + (in1 ? row1 : 0) + (int64_t)(lane >> 5) * nl;
             const float* br = X + (C * 32 + (lane & 31)) * ld + (lane >> 5);
+            const int kEnd = nl - (lane >> 5);     // this lane's columns k0 + lane / 32 < nl
             if (t1) {
 #pragma unroll 4
                 for (int k0 = 0; k0 < np; k0 += 2) {
+                    const bool kin = k0 < kEnd;
+                    const float x0 = ar0[kin ? (int64_t)k0 * nl : 0], x1 = ar1[kin ? (int64_t)k0 * nl : 0];
                     const float b = br[k0];
-                    c0v = __builtin_amdgcn_mfma_f32_32x32x2f32(ar0[k0], b, c0v, 0, 0, 0);
-                    c1v = __builtin_amdgcn_mfma_f32_32x32x2f32(ar1[k0], b, c1v, 0, 0, 0);
+                    c0v = __builtin_amdgcn_mfma_f32_32x32x2f32((kin && in0) ? x0 : 0.0f, b, c0v, 0, 0, 0);
+                    c1v = __builtin_amdgcn_mfma_f32_32x32x2f32((kin && in1) ? x1 : 0.0f, b, c1v, 0, 0, 0);
                 }
             } else if (t0) {
 #pragma unroll 4
-                for (int k0 = 0; k0 < np; k0 += 2)
-                    c0v = __builtin_amdgcn_mfma_f32_32x32x2f32(ar0[k0], br[k0], c0v, 0, 0, 0);
+                for (int k0 = 0; k0 < np; k0 += 2) {
+                    const bool kin = k0 < kEnd;
+                    const float x0 = ar0[kin ? (int64_t)k0 * nl : 0];
+                    c0v = __builtin_amdgcn_mfma_f32_32x32x2f32((kin && in0) ? x0 : 0.0f, br[k0], c0v, 0, 0, 0);
+                }
             }
         }
         __syncthreads();
@@ -1808,9 +1802,9 @@ static void launch_blockpre(const BlockPreArgs<V>& pre, bool init, const V* r, c
     if constexpr (std::is_same<V, float>::value) {
         if (pre.nmm > 0) {
             const int np = (pre.mmMax + 31) & ~31;
-            const size_t ldm = (size_t)(np * (np + 1) + 64 * (np + 1)) * sizeof(float);
-            // groups of 64 instances per workgroup: the staged inverse serves up to 2 of them
-            const int ng = (batch + 63) / 64, groups = std::min(2, ng);
+            const size_t ldm = (size_t)64 * (np + 1) * sizeof(float);
+            // groups of 64 instances per workgroup (LSG_PRE_MM_GROUPS, default 1)
+            const int ng = (batch + 63) / 64, groups = std::max(1, std::min(pre.mmGroups, ng));
             const dim3 gmm((unsigned)pre.nmm, (unsigned)((ng + groups - 1) / groups));
             if (init)
                 hipLaunchKernelGGL((blockmm_kernel<true>), gmm, dim3(256), ldm, s, pre, r, D, h, dv, n, batch, groups,

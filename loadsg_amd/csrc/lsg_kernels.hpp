@@ -174,7 +174,7 @@ struct BlockPreArgs {
     const int32_t* fdList = nullptr;
     const int32_t* fdPerm = nullptr;
     // FP32: dense blocks of at least PRE_MM_MIN points on the matrix cores (blockmm_kernel)
-    int nmm = 0, mmMax = 0;
+    int nmm = 0, mmMax = 0, mmGroups = 1;
     const int32_t* mmList = nullptr;
     const V* fdV = nullptr;
     const V* fdLam = nullptr;
