@@ -292,13 +292,13 @@ def workload_cfg2(args):
 
 def workload_cfg3(args):
     from loadsg_amd.measure_adaptive import run
-    r = run(os.path.join(ROOT, "tests", "golden", "grids", "cfg3_start4_level11.npz"), batch=64, reps=10, tol=1e-10,
+    r = run(os.path.join(ROOT, "tests", "golden", "grids", "cfg3_start4_level11.npz"), batch=128, reps=10, tol=1e-10,
             precond=args.precond)
     out = {"metric": "solves/s to 1e-10 rel. residual, locally adaptive d=3 grid (FP32 PCG under FP64 IR)",
            "value": r["mixed_batch_solves_per_s"], "unit": "solves/s", "n_gpus": 1, "higher_is_better": True,
            "dtype": "f32 inner / f64 refinement",
            "data": "reference-generated grid; manufactured and synthetic right-hand sides",
-           "config": {"workload": r["workload"], "points": r["points"], "hanging": r["hanging"], "batch": 64,
+           "config": {"workload": r["workload"], "points": r["points"], "hanging": r["hanging"], "batch": 128,
                       "inner_preconditioner": r["inner_preconditioner"]},
            "extra": r}
     print(json.dumps(out))

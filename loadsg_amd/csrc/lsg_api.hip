@@ -1822,6 +1822,7 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         // 32 KB; SP_LARGE up to the device's LDS; larger ones in zbuf)
         const int64_t ldsMax = std::min<int64_t>(op->maxLds, envInt("LSG_SP_LDS_BYTES", (int)op->maxLds));
         const int64_t smallMax = envInt("LSG_SP_SMALL_BYTES", 8192), mediumMax = envInt("LSG_SP_MEDIUM_BYTES", 49152);
+        bool anyGlobal = false;
         for (const auto& lev : P.apply) {
             std::vector<FamilyItem> byCls[SP_NCLASS];
             int64_t stride[SP_NCLASS] = {};
@@ -1842,9 +1843,13 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
                 HIPCHK(m->alloc(byCls[c].size() * sizeof(FamilyItem)));
                 HIPCHK(hipMemcpy(m->p, byCls[c].data(), byCls[c].size() * sizeof(FamilyItem), hipMemcpyHostToDevice));
                 op->spLevels.push_back({(int)byCls[c].size(), c, c == SP_GLOBAL ? 0 : stride[c], std::move(m)});
+                anyGlobal = anyGlobal || c == SP_GLOBAL;
             }
         }
-        HIPCHK(op->zbuf.alloc((size_t)P.zSize * batch * vs));
+        // the per-item scratch in HBM serves the stored element matrices, d > SP_CELL_MAXD and
+        // items whose point values exceed the LDS; otherwise the apply keeps it all in LDS
+        const bool needZ = op->matKind || g.d > SP_CELL_MAXD || anyGlobal;
+        HIPCHK(op->zbuf.alloc(needZ ? (size_t)P.zSize * batch * vs : vs));
         op->nCells = T.cellOff.back();
         HIPCHK(op->cellCoef.alloc((size_t)std::max<int64_t>(1, 2 * op->nCells) * batch * vs));
         if (op->matKind) {
