@@ -250,3 +250,34 @@ def test_cfg3_full_level11_grid():
         its[kind] = np.mean([i["iterations"] for i in info])
     print("inner iterations diag %.1f, depth-block %.1f" % (its[0], its[1]), flush=True)
     assert its[L.PRECOND_DEPTH_BLOCK] < 0.7 * its[L.PRECOND_DIAG]
+
+
+@pytest.mark.parametrize("knob", ["LSG_SP_LDS_BYTES=0", "LSG_SP_SMALL_BYTES=0", "LSG_SP_MEDIUM_BYTES=0",
+                                  "LSG_SP_LDS_BYTES=20000"])
+def test_sparse_scratch_classes_give_the_same_bits(knob):
+    """The cell-wise apply launches its items by the size of their point scratch: in LDS with
+    four instances per workgroup, in LDS with one (256 or 512 threads), or in HBM.  Every class
+    sums a point's entries in the same order, so operators whose items are forced into other
+    classes (the knob, read when the operator is made) must give the default operator's bits,
+    at batch 1 and batch 6, FP64 and FP32."""
+    import torch
+    print("scratch classes", knob, flush=True)
+    z = np.load(os.path.join(GOLD, "grids", "cfg3_start4_level8.npz"))
+    grid = L.AdaptiveSparseGrid(z["idx"], z["active"])
+    st = L.VariableCoefficient(L.COEF_CFG3)
+    name, val = knob.split("=")
+    rng = np.random.default_rng(3)
+    for prec, dt in ((64, torch.float64), (32, torch.float32)):
+        for B in (1, 6):
+            x = torch.tensor(rng.standard_normal((B, grid.size)), device="cuda").to(dt)
+            ref = L.MatrixVectorHomogen(grid, st, precision=prec, batch=B)
+            yr = torch.zeros_like(x)
+            ref.multiplication(x, yr)
+            os.environ[name] = val
+            try:
+                op = L.MatrixVectorHomogen(grid, st, precision=prec, batch=B)
+            finally:
+                del os.environ[name]
+            y = torch.zeros_like(x)
+            op.multiplication(x, y)
+            assert torch.equal(y, yr), (knob, prec, B)
