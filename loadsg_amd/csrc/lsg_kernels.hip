@@ -661,9 +661,9 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
 }
 
 // ------------------------------------------------ apply, one separable term
-// Horner sweep of the separable apply kernel.  The first sweep reads P from Y, starts
-// Z from zero and, in its write phase, keeps P in `keep` for the prolongation of the
-// next box; the last sweep does not store Y, folds in the dimensions of extent 1
+// Horner sweep of the separable apply kernel.  The first sweep reads P from Y and starts
+// Z from zero (P of the box stays in registers for the next box's prolongation, parked by
+// the kernel); the last sweep does not store Y, folds in the dimensions of extent 1
 // (Z = cM Z + cK Y, see apply_sep_kernel) and adds the gM value g0 read for this box
 // (held in registers).
 // DEC: bit 0 -- dimension i is the fastest of the box (S = 1, no division for the lower
@@ -672,13 +672,13 @@ __global__ void __launch_bounds__(512) apply_kernel(ApplyArgs<V> a) {
 // division instead of two on such sweeps.
 template <typename V, int PPT, int DEC>
 __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const LSG_CONST BoxShape& sh, int i,
-                                 const V* __restrict__ w, int t, bool first, bool needY, V* __restrict__ keep,
-                                 bool last, bool fold, V cM, V cK, const V* __restrict__ g0) {
+                                 const V* __restrict__ w, int t, bool first, bool needY, bool last, bool fold,
+                                 V cM, V cK, const V* __restrict__ g0) {
     const int S = sh.S[i], n = sh.n[i], total = sh.total;
     const float rS = sh.rS[i], rn = sh.rn[i];
     const V ih = V(1 << t);
     const V h6 = inv6pow2<V>(t);
-    V ny[PPT], nz[PPT], yk[PPT];
+    V ny[PPT], nz[PPT];
     const int B = blockDim.x;
 #pragma unroll
     for (int s = 0; s < PPT; ++s) {
@@ -703,7 +703,6 @@ __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const LSG
                 out += h6 * (wl * (zl + V(2) * zc) + wr * (V(2) * zc + zr));
             }
             nz[s] = out;
-            yk[s] = yc;
         }
     }
     __syncthreads();
@@ -718,7 +717,6 @@ __device__ inline void sep_sweep(V* __restrict__ Y, V* __restrict__ Z, const LSG
             }
             if (needY && !last) Y[p] = ny[s];
             Z[p] = (last && g0) ? z + g0[p] : z;
-            if (keep) keep[p] = yk[s];
         }
     }
     __syncthreads();
@@ -887,10 +885,10 @@ __global__ void __launch_bounds__(512) apply_sep_kernel(ApplyArgs<V> a) {
                 const V* wi = W + woff[i] + (i == fd ? wq : 0);
                 const V* gi = i == lo ? g0 : nullptr;
                 switch (dec) {
-                    case 0: sep_sweep<V, PPT, 0>(Y, Z, sh, i, wi, it.T[i], i == hi, true, nullptr, i == lo, fold, cM, cK, gi); break;
-                    case 1: sep_sweep<V, PPT, 1>(Y, Z, sh, i, wi, it.T[i], i == hi, true, nullptr, i == lo, fold, cM, cK, gi); break;
-                    case 2: sep_sweep<V, PPT, 2>(Y, Z, sh, i, wi, it.T[i], i == hi, true, nullptr, i == lo, fold, cM, cK, gi); break;
-                    default: sep_sweep<V, PPT, 3>(Y, Z, sh, i, wi, it.T[i], i == hi, true, nullptr, i == lo, fold, cM, cK, gi); break;
+                    case 0: sep_sweep<V, PPT, 0>(Y, Z, sh, i, wi, it.T[i], i == hi, true, i == lo, fold, cM, cK, gi); break;
+                    case 1: sep_sweep<V, PPT, 1>(Y, Z, sh, i, wi, it.T[i], i == hi, true, i == lo, fold, cM, cK, gi); break;
+                    case 2: sep_sweep<V, PPT, 2>(Y, Z, sh, i, wi, it.T[i], i == hi, true, i == lo, fold, cM, cK, gi); break;
+                    default: sep_sweep<V, PPT, 3>(Y, Z, sh, i, wi, it.T[i], i == hi, true, i == lo, fold, cM, cK, gi); break;
                 }
             }
         }

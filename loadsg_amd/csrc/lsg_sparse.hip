@@ -600,7 +600,7 @@ void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str, i
     }
     // cell-wise kernels by scratch class: SP_SMALL (LDS, 4 instances per workgroup from
     // batch 2 on), SP_MEDIUM (LDS, one instance, 256 threads), SP_LARGE (LDS, one instance,
-    // 512 threads), SP_GLOBAL (zbuf, 4 instances per workgroup)
+    // 1024 threads), SP_GLOBAL (zbuf, 4 instances per workgroup)
     const bool grp = a.batch > 1 && (cls == SP_SMALL || cls == SP_GLOBAL);
     const dim3 gg((unsigned)nitems * (unsigned)(grp ? (a.batch + 3) / 4 : a.batch));
     const size_t lds = (size_t)a.ldsStride * sizeof(V) * (grp ? 4 : 1);
@@ -614,7 +614,7 @@ void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str, i
             hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 256>), gg, dim3(256), lds, str, a);          \
             break;                                                                                              \
         case SP_LARGE:                                                                                          \
-            hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 512>), gg, dim3(512), lds, str, a);          \
+            hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 1024>), gg, dim3(1024), lds, str, a);          \
             break;                                                                                              \
         default:                                                                                                \
             if (grp) hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 4, false, 256>), gg, dim3(256), 0, str, a);  \
