@@ -1819,7 +1819,7 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         // the cell-wise apply keeps an item's point values in LDS: the box and, after the
         // first restriction, the coarser box; items are launched by the size of that scratch
         // (SP_SMALL up to 8 KB per instance, four instances per workgroup; SP_MEDIUM up to
-        // 32 KB; SP_LARGE up to the device's LDS; larger ones in zbuf)
+        // 48 KB; SP_LARGE up to the device's LDS; larger ones in zbuf)
         const int64_t ldsMax = std::min<int64_t>(op->maxLds, envInt("LSG_SP_LDS_BYTES", (int)op->maxLds));
         const int64_t smallMax = envInt("LSG_SP_SMALL_BYTES", 8192), mediumMax = envInt("LSG_SP_MEDIUM_BYTES", 49152);
         bool anyGlobal = false;
