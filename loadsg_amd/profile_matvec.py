@@ -1,7 +1,8 @@
 """Profiling driver for the apply stage of the batched matvec (FP32 by default).
 
 Operator: d=5 regular sparse grid of level 7 (N = 61,183), exp-sine coefficient with
-16 xi drawn as in bench.py (PCG64 seed 20261203), --batch instances.  The first
+16 xi drawn as in bench.py (PCG64 seed 20261203), --batch instances; or, with --grid, an
+adaptive grid from an .npz (sparse path) with a = 1 + |x|^2.  The first
 matvec autotunes the apply block sizes; then --reps matvecs follow, each timed with
 HIP events around the apply launches (lsg_last_timing).  Finally one elementwise
 kernel y = 2 x over --copy-mib MiB of float64 runs, whose HBM traffic is known
@@ -28,6 +29,9 @@ def main():
     ap.add_argument("--reps", type=int, default=10)
     ap.add_argument("--precision", type=int, default=32, choices=(32, 64))
     ap.add_argument("--copy-mib", type=int, default=1024)
+    ap.add_argument("--grid", default=None,
+                    help="an .npz of idx / active (an adaptive grid, sparse path) instead of d=5 level 7; "
+                         "coefficient 1 + |x|^2 there")
     args = ap.parse_args()
 
     import torch
@@ -36,11 +40,16 @@ def main():
     B = args.batch
     rng = np.random.default_rng(20261203)
     xi = rng.normal(0.0, 0.5, (B, 16))
-    grid = L.AdaptiveSparseGrid.regular(5, 7)
-    op = L.MatrixVectorHomogen(grid, L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[0]),
-                               precision=args.precision, batch=B)
-    for b in range(B):
-        op.set_coefficient(b, L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[b]))
+    if args.grid:
+        z = np.load(args.grid)
+        grid = L.AdaptiveSparseGrid(z["idx"], z["active"])
+        op = L.MatrixVectorHomogen(grid, L.VariableCoefficient(L.COEF_CFG3), precision=args.precision, batch=B)
+    else:
+        grid = L.AdaptiveSparseGrid.regular(5, 7)
+        op = L.MatrixVectorHomogen(grid, L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[0]),
+                                   precision=args.precision, batch=B)
+        for b in range(B):
+            op.set_coefficient(b, L.VariableCoefficient(L.COEF_EXPSIN, xi=xi[b]))
     dt = torch.float64 if args.precision == 64 else torch.float32
     dev = torch.device("cuda:0")
     x = torch.randn(B, grid.size, dtype=dt, device=dev)
