@@ -1811,6 +1811,10 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
         LSG_UP(rcPtr, T.rcPtr) LSG_UP(rcCell, T.rcCell) LSG_UP(rcCorner, T.rcCorner) LSG_UP(rcY, T.rcY)
         LSG_UP(restOff, T.restOff) LSG_UP(restSLR, T.restSLR)
         LSG_UP(convPtr, T.convPtr) LSG_UP(convCol, T.convCol) LSG_UP(convVal, T.convVal)
+        {
+            const std::vector<float> cf(T.convVal.begin(), T.convVal.end()), nf(T.nodVal.begin(), T.nodVal.end());
+            LSG_UP(convValF, cf) LSG_UP(nodValF, nf)
+        }
         LSG_UP(pos, T.pos) LSG_UP(tOfDev, T.tOfDev) LSG_UP(depthT, T.depthT)
         LSG_UP(boxOff, g.boxOff) LSG_UP(boxSize, g.boxSize) LSG_UP(blkOff, g.blkOff) LSG_UP(blkSize, g.blkSize)
 #undef LSG_UP

@@ -65,6 +65,10 @@ struct SparseDev {
     const int64_t* convPtr;
     const int32_t* convCol;
     const double* convVal;
+    // convVal / nodVal rounded to FP32 for the FP32 kernels (half the table traffic; float(w)
+    // is what those kernels formed from the FP64 table anyway, so results do not change)
+    const float* convValF;
+    const float* nodValF;
     const int32_t* pos;
     const int32_t* tOfDev;
     const int32_t* depthT;

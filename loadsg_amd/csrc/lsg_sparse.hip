@@ -14,6 +14,7 @@
 // path's (buildPlan), with one workgroup per (apply item, instance).  Passes of a
 // workgroup communicate through global scratch, separated by a workgroup barrier.  Every sum has a fixed order.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 
 #include "lsg_kernels.hpp"
 
@@ -133,7 +134,9 @@ __global__ void __launch_bounds__(256) sp_nodal_kernel(SparseDev s, const V* __r
         for (int k = 0; k < NI; ++k) v[k] = V(0);
         const int64_t k1 = s.nodPtr[r + 1];
         for (int64_t e = s.nodPtr[r]; e < k1; ++e) {
-            const V w = V(s.nodVal[e]);
+            V w;
+            if constexpr (std::is_same<V, float>::value) w = s.nodValF[e];
+            else w = V(s.nodVal[e]);
             const int32_t c = s.nodCol[e];
 #pragma unroll
             for (int k = 0; k < NI; ++k) v[k] = fma(w, xi[k][c], v[k]);
@@ -477,7 +480,9 @@ __global__ void __launch_bounds__(NT) sp_apply_col_kernel(SparseApplyArgs<V> a) 
             for (int k = 0; k < NI; ++k) v[k] = V(0);
             const int64_t e1 = s.convPtr[b0 + i + 1];
             for (int64_t e = s.convPtr[b0 + i]; e < e1; ++e) {
-                const V wgt = V(s.convVal[e]);
+                V wgt;
+                if constexpr (std::is_same<V, float>::value) wgt = s.convValF[e];
+                else wgt = V(s.convVal[e]);
                 const int32_t col = s.convCol[e];
 #pragma unroll
                 for (int k = 0; k < NI; ++k) v[k] += wgt * A[k][col];
