@@ -1815,6 +1815,19 @@ int lsg_op_create(lsg_grid* grid, const lsg_coef* coef, int precision, int batch
             const std::vector<float> cf(T.convVal.begin(), T.convVal.end()), nf(T.nodVal.begin(), T.nodVal.end());
             LSG_UP(convValF, cf) LSG_UP(nodValF, nf)
         }
+        {
+            // 16-bit index tables when every box and every depth's cell list fits them
+            int64_t big = 0;
+            for (size_t t = 0; t < g.depths.size(); ++t)
+                big = std::max({big, g.boxSize[t], T.cellOff[t + 1] - T.cellOff[t]});
+            op->sd.cellCorner16 = op->sd.cellCol16 = op->sd.restSLR16 = op->sd.convCol16 = nullptr;
+            if (big < 32768 && !envInt("LSG_SP_IX32", 0)) {
+                auto narrow = [](const std::vector<int32_t>& v) { return std::vector<int16_t>(v.begin(), v.end()); };
+                const std::vector<int16_t> cc16 = narrow(T.cellCorner), cl16 = narrow(T.cellCol),
+                                           rs16 = narrow(T.restSLR), cv16 = narrow(T.convCol);
+                LSG_UP(cellCorner16, cc16) LSG_UP(cellCol16, cl16) LSG_UP(restSLR16, rs16) LSG_UP(convCol16, cv16)
+            }
+        }
         LSG_UP(pos, T.pos) LSG_UP(tOfDev, T.tOfDev) LSG_UP(depthT, T.depthT)
         LSG_UP(boxOff, g.boxOff) LSG_UP(boxSize, g.boxSize) LSG_UP(blkOff, g.blkOff) LSG_UP(blkSize, g.blkSize)
 #undef LSG_UP

@@ -69,6 +69,12 @@ struct SparseDev {
     // is what those kernels formed from the FP64 table anyway, so results do not change)
     const float* convValF;
     const float* nodValF;
+    // 16-bit copies of the cell-wise apply's index tables (null unless every box and every
+    // depth's cell list has fewer than 32768 entries)
+    const int16_t* cellCorner16;
+    const int16_t* cellCol16;
+    const int16_t* restSLR16;
+    const int16_t* convCol16;
     const int32_t* pos;
     const int32_t* tOfDev;
     const int32_t* depthT;

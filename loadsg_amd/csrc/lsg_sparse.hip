@@ -364,13 +364,23 @@ __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
 // zbuf.  NI instances per workgroup share every index-table load (cell corners, restriction
 // triples, conversion columns and weights); slots past the batch (the last group) repeat the
 // group's first instance and store nothing.
-template <typename V, int D, int NI, bool LDSM, int NT>
+template <typename V, int D, int NI, bool LDSM, int NT, typename IX>
 __global__ void __launch_bounds__(NT) sp_apply_col_kernel(SparseApplyArgs<V> a) {
     extern __shared__ char smem_raw[];
     const int ng = (a.batch + NI - 1) / NI;
     const int grp = (int)(blockIdx.x % ng);
     const ApplyItem& it = a.items[a.fam[blockIdx.x / ng].first];
     const SparseDev& s = a.sp;
+    // index tables: 16-bit copies when every box and cell list of the grid fits them
+    const IX* cellCornerT;
+    const IX* cellColT;
+    const IX* restSLRT;
+    const IX* convColT;
+    if constexpr (sizeof(IX) == 2) {
+        cellCornerT = s.cellCorner16; cellColT = s.cellCol16; restSLRT = s.restSLR16; convColT = s.convCol16;
+    } else {
+        cellCornerT = s.cellCorner; cellColT = s.cellCol; restSLRT = s.restSLR; convColT = s.convCol;
+    }
     int inst[NI];
     bool on[NI];
     if (!group_slots<NI>(grp * NI, a.batch, a.map, a.st, inst, on)) return;
@@ -412,8 +422,8 @@ __global__ void __launch_bounds__(NT) sp_apply_col_kernel(SparseApplyArgs<V> a) 
     for (int col = 0; col < NC; ++col) {
         const int64_t e1 = colOff[col + 1];
         for (int64_t e = colOff[col] + threadIdx.x; e < e1; e += NT) {
-            const int64_t c = c0 + s.cellCol[e];
-            const int32_t* corner = s.cellCorner + c * NC;
+            const int64_t c = c0 + cellColT[e];
+            const IX* corner = cellCornerT + c * NC;
             int32_t jq[NC];
 #pragma unroll
             for (int q = 0; q < NC; ++q) jq[q] = corner[q];
@@ -452,7 +462,7 @@ __global__ void __launch_bounds__(NT) sp_apply_col_kernel(SparseApplyArgs<V> a) 
         const int dd = it.r[j - 1];
         const int tf = it.ctidx[j - 1], tc = it.ctidx[j];
         const int64_t nco = s.boxSize[tc];
-        const int32_t* slr = s.restSLR + 3 * s.restOff[(int64_t)tf * s.d + dd];
+        const IX* slr = restSLRT + 3 * s.restOff[(int64_t)tf * s.d + dd];
         const bool w = it.w_off[j] >= 0, rd = it.rd_off[j] >= 0;
         for (int64_t i = threadIdx.x; i < nco; i += NT) {
             const int32_t sf = slr[3 * i], l = slr[3 * i + 1], r = slr[3 * i + 2];
@@ -483,7 +493,7 @@ __global__ void __launch_bounds__(NT) sp_apply_col_kernel(SparseApplyArgs<V> a) 
                 V wgt;
                 if constexpr (std::is_same<V, float>::value) wgt = s.convValF[e];
                 else wgt = V(s.convVal[e]);
-                const int32_t col = s.convCol[e];
+                const int32_t col = convColT[e];
 #pragma unroll
                 for (int k = 0; k < NI; ++k) v[k] += wgt * A[k][col];
             }
@@ -609,28 +619,29 @@ void launch_sp_apply(const SparseApplyArgs<V>& a, int nitems, hipStream_t str, i
     const bool grp = a.batch > 1 && (cls == SP_SMALL || cls == SP_GLOBAL);
     const dim3 gg((unsigned)nitems * (unsigned)(grp ? (a.batch + 3) / 4 : a.batch));
     const size_t lds = (size_t)a.ldsStride * sizeof(V) * (grp ? 4 : 1);
-#define LSG_SP_COL(DD)                                                                                          \
-    switch (cls) {                                                                                              \
-        case SP_SMALL:                                                                                          \
-            if (grp) hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 4, true, 256>), gg, dim3(256), lds, str, a); \
-            else hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 256>), gg, dim3(256), lds, str, a);     \
-            break;                                                                                              \
-        case SP_MEDIUM:                                                                                         \
-            hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 256>), gg, dim3(256), lds, str, a);          \
-            break;                                                                                              \
-        case SP_LARGE:                                                                                          \
-            hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 1024>), gg, dim3(1024), lds, str, a);          \
-            break;                                                                                              \
-        default:                                                                                                \
-            if (grp) hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 4, false, 256>), gg, dim3(256), 0, str, a);  \
-            else hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, false, 256>), gg, dim3(256), 0, str, a);      \
-            break;                                                                                              \
+#define LSG_SP_COL(DD, IX)                                                                                        \
+    switch (cls) {                                                                                                \
+        case SP_SMALL:                                                                                            \
+            if (grp) hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 4, true, 256, IX>), gg, dim3(256), lds, str, a); \
+            else hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 256, IX>), gg, dim3(256), lds, str, a);     \
+            break;                                                                                                \
+        case SP_MEDIUM:                                                                                           \
+            hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 256, IX>), gg, dim3(256), lds, str, a);          \
+            break;                                                                                                \
+        case SP_LARGE:                                                                                            \
+            hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, true, 1024, IX>), gg, dim3(1024), lds, str, a);        \
+            break;                                                                                                \
+        default:                                                                                                  \
+            if (grp) hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 4, false, 256, IX>), gg, dim3(256), 0, str, a);  \
+            else hipLaunchKernelGGL((sp_apply_col_kernel<V, DD, 1, false, 256, IX>), gg, dim3(256), 0, str, a);      \
+            break;                                                                                                \
     }
+    const bool ix16 = a.sp.cellCorner16 != nullptr;
     switch (a.sp.d) {
-        case 1: LSG_SP_COL(1) break;
-        case 2: LSG_SP_COL(2) break;
-        case 3: LSG_SP_COL(3) break;
-        default: LSG_SP_COL(4) break;
+        case 1: if (ix16) { LSG_SP_COL(1, int16_t) } else { LSG_SP_COL(1, int32_t) } break;
+        case 2: if (ix16) { LSG_SP_COL(2, int16_t) } else { LSG_SP_COL(2, int32_t) } break;
+        case 3: if (ix16) { LSG_SP_COL(3, int16_t) } else { LSG_SP_COL(3, int32_t) } break;
+        default: if (ix16) { LSG_SP_COL(4, int16_t) } else { LSG_SP_COL(4, int32_t) } break;
     }
 #undef LSG_SP_COL
 }

@@ -253,13 +253,14 @@ def test_cfg3_full_level11_grid():
 
 
 @pytest.mark.parametrize("knob", ["LSG_SP_LDS_BYTES=0", "LSG_SP_SMALL_BYTES=0", "LSG_SP_MEDIUM_BYTES=0",
-                                  "LSG_SP_LDS_BYTES=20000"])
+                                  "LSG_SP_LDS_BYTES=20000", "LSG_SP_IX32=1"])
 def test_sparse_scratch_classes_give_the_same_bits(knob):
     """The cell-wise apply launches its items by the size of their point scratch: in LDS with
-    four instances per workgroup, in LDS with one (256 or 512 threads), or in HBM.  Every class
-    sums a point's entries in the same order, so operators whose items are forced into other
-    classes (the knob, read when the operator is made) must give the default operator's bits,
-    at batch 1 and batch 6, FP64 and FP32."""
+    four instances per workgroup, in LDS with one (256 or 1024 threads), or in HBM; and it reads
+    16-bit copies of its index tables where they fit (LSG_SP_IX32=1: the 32-bit tables).  Every
+    variant sums a point's entries in the same order, so operators made with the knob (read when
+    the operator is made) must give the default operator's bits, at batch 1 and batch 6, FP64
+    and FP32."""
     import torch
     print("scratch classes", knob, flush=True)
     z = np.load(os.path.join(GOLD, "grids", "cfg3_start4_level8.npz"))
