@@ -400,7 +400,7 @@ struct lsg_op {
 
 namespace {
 
-// Sparse path: (a, c) at the centre of every cell, per instance, from the coefficient
+// Sparse path: a, then c, at the centre of every cell, per instance, from the coefficient
 // tables; the stencil and the diagonal read these.  Operators with a general coefficient
 // receive them from the host (lsg_op_set_coef) and never run this.
 int ensureCellCoef(lsg_op* op) {
@@ -1921,6 +1921,14 @@ int lsg_op_set_coef(lsg_op* op, int instance, const lsg_coef* coef) {
         if (int rc = cellValues(g, *coef, ac)) return rc;
         const size_t per = (size_t)2 * op->nCells;
         if (ac.size() != per) return fail(LSG_ERR_ARG, "cell count mismatch");
+        {   // device layout: the a values of all cells, then the c values
+            std::vector<double> planar(per);
+            for (int64_t c = 0; c < op->nCells; ++c) {
+                planar[(size_t)c] = ac[(size_t)2 * c];
+                planar[(size_t)(op->nCells + c)] = ac[(size_t)2 * c + 1];
+            }
+            ac.swap(planar);
+        }
         if (op->prec == 64) {
             HIPCHK(hipMemcpy(op->cellCoef.as<double>() + per * instance, ac.data(), per * 8, hipMemcpyHostToDevice));
         } else {

@@ -105,7 +105,7 @@ struct SparseApplyArgs {
     int batch;
     const CGState* st;
     const int* map;           // slot -> instance (null: slot = instance)
-    const V* cellCoef;        // per instance: (a, c) at the centre of every cell of depth T+1
+    const V* cellCoef;        // per instance: a at the centre of every cell of depth T+1, then c
     int64_t ccStride;
     // stored element matrices (null: element matrices from cellCoef): entry e of cell c of
     // instance i at cellMat[i * matStride + e * nCells + c], e = cellEntry(p, q)

@@ -75,7 +75,8 @@ __device__ inline V element(int d, const int32_t* __restrict__ T, unsigned p, un
 }  // namespace
 
 // ---------------------------------------------------------- cell coefficients
-// (a, c) at the centre of every cell of depth T+1 of every depth, per instance; the
+// a, then c, at the centre of every cell of depth T+1 of every depth, per instance (a for
+// all cells first, so an operator without a mass term reads one contiguous half); the
 // coefficient does not change between matvecs, so the apply stage only loads these.
 template <typename V>
 __global__ void __launch_bounds__(256) sp_cellcoef_kernel(SparseDev s, int64_t nCells, int nA, int nC,
@@ -92,8 +93,8 @@ __global__ void __launch_bounds__(256) sp_cellcoef_kernel(SparseDev s, int64_t n
         }
         V aC, cC;
         cell_coef<V>(coef, tabLen, nA, nC, s.d, s.depthT + (int64_t)lo * s.d, s.cellLo + c * s.d, aC, cC);
-        cc[inst * ccStride + 2 * c] = aC;
-        cc[inst * ccStride + 2 * c + 1] = cC;
+        cc[inst * ccStride + c] = aC;
+        cc[inst * ccStride + nCells + c] = cC;
     }
 }
 
@@ -312,8 +313,8 @@ __global__ void __launch_bounds__(256) sp_apply_kernel(SparseApplyArgs<V> a) {
                     zs = fma(eS[cj ^ q], uj, zs);
                     if (mass) zm = fma(eM[cj ^ q], uj, zm);
                 }
-                z = fma(cc[2 * cell], zs, z);
-                if (mass) z = fma(cc[2 * cell + 1], zm, z);
+                z = fma(cc[cell], zs, z);
+                if (mass) z = fma(cc[a.nCells + cell], zm, z);
             }
             A[i] = g ? z + g[i] : z;
         }
@@ -433,8 +434,8 @@ __global__ void __launch_bounds__(NT) sp_apply_col_kernel(SparseApplyArgs<V> a) 
             for (int k = 0; k < NI; ++k) {
 #pragma unroll
                 for (int q = 0; q < NC; ++q) uq[k][q] = jq[q] >= 0 ? u[k][jq[q]] : V(0);
-                aC[k] = cc[k][2 * c];
-                cC[k] = mass ? cc[k][2 * c + 1] : V(0);
+                aC[k] = cc[k][c];
+                cC[k] = mass ? cc[k][a.nCells + c] : V(0);
             }
 #pragma unroll
             for (int k = 0; k < NI; ++k) {
@@ -538,7 +539,7 @@ __global__ void __launch_bounds__(256) sp_diag_kernel(SparseDev s, const uint8_t
                 const int64_t cell = c0 + s.rcCell[e];
                 const unsigned cj = (unsigned)s.rcCorner[e];
                 const int32_t* lo = s.cellLo + cell * d;
-                const V aC = E ? V(0) : cc[2 * cell], cC = E ? V(0) : cc[2 * cell + 1];
+                const V aC = E ? V(0) : cc[cell], cC = E ? V(0) : cc[nCells + cell];
                 const int32_t* corner = s.cellCorner + cell * nc;
                 for (unsigned q = 0; q < nc; ++q) {
                     if (corner[q] < 0) continue;
